@@ -1,0 +1,182 @@
+"""Benchmark: sum-factorised hyperelastic Jv on one MI355X.
+
+Workload (the plain matrix-free configuration): compressible neo-Hookean (lam = 2, mu = 1) on the unit cube,
+64^3 Q2 hexahedra (6,343,293 free dofs), FP64, 27 Gauss points per cell.  State: the smooth random displacement
+(8 Fourier modes per component, amplitude 1e-2, SplitMix64 seed 2); v ~ N(0, 1) (seed 3).  A step is one
+y = J(x) v over all free dofs with the prepared tangents, inputs resident in HBM.
+
+Prints one JSON line.  ``value`` is Jv GDoF/s (free dofs / Jv time).  ``roofline`` prices the Jv against the
+HBM roofline with the algorithmic bytes of one Jv (the reference's stored-T byte model):
+    B_Jv = Ncell*Nq*81*8 + Ncell*10*8 + Ncell*ndl*4 + 2*Nfree*8
+``cpu_baseline`` is the oracle (the C++/OpenMP restatement of the reference CPU algorithm) timed on the host
+cores on a 32^3 sample of the same recipe.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "hyperfemkernels.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "Q2 hyperelastic Jv GDoF/s + solve time to 1e-8: learned/FP32-MG vs FP64 p-MG"
+HBM_PEAK_GBS = 8000.0
+
+
+def jv_bytes(ncells, nq, ndl, nfree, s_t=8, s_v=8, nent=81):
+    return ncells * nq * nent * s_t + ncells * 10 * 8 + ncells * ndl * 4 + 2 * nfree * s_v
+
+
+def host_cores():
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(n=32, reps=3):
+    import numpy as np
+
+    from hfk.mesh import hex_space
+    from hfk.synthetic import normal, smooth_field
+    from oracle import oracle as orc
+
+    sp = hex_space((n, n, n), 2)
+    u = smooth_field(2)
+    op = orc.OracleOp(sp, "neohookean", 2.0, 1.0, dvals=sp.dirichlet_values(u))
+    prep = op.prepare(sp.free_values(u))
+    v = normal(3, sp.nfree)
+    cores = host_cores()
+    op.jv(prep, v, nthreads=cores)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        op.jv(prep, v, nthreads=cores)
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    return {"value": sp.nfree / t / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "port",
+            "sample": f"oracle stored-T sum-factorised Jv (OpenMP, colour chunks), Q2 neo-Hookean {n}^3 "
+                      f"({sp.nfree} free dofs), min of {reps} after 1 warm-up, {t * 1e3:.1f} ms per Jv"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--storage", default="full64", choices=["full64", "sym64", "full32", "sym32"])
+    ap.add_argument("--variants", default="sym64,full32,sym32", help="extra storages timed beside the headline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the Jv shards by cell partition; see DESIGN.md); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from hfk import HyperelasticForm, KernelOperator, MatrixFreeJacobian
+    from hfk.mesh import hex_space
+    from hfk.synthetic import normal, smooth_field
+
+    t0 = time.perf_counter()
+    n = args.n
+    sp = hex_space((n, n, n), 2)
+    u = smooth_field(2)
+    x, dvals = sp.free_values(u), sp.dirichlet_values(u)
+    v = normal(3, sp.nfree)
+    print(f"bench: mesh {n}^3 Q2, {sp.nfree} free dofs ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    op = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=4), sp, jacobian="matrixfree",
+                        dirichlet_values=dvals)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+    xd = torch.from_numpy(x).to(dev)
+    vd = torch.from_numpy(v).to(dev)
+    yd = torch.empty_like(vd)
+
+    def time_storage(storage, steps, warmup):
+        mf = MatrixFreeJacobian(op, storage)
+        torch.cuda.synchronize()
+        # prepare (once per Newton step in a solve), timed with events too
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        mf.prepare_dev(xd.data_ptr(), sptr)
+        e0.record(stream)
+        mf.prepare_dev(xd.data_ptr(), sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t_prep = e0.elapsed_time(e1) * 1e-3
+        for _ in range(warmup):
+            mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t_start = time.perf_counter()
+        for a, b in evs:
+            a.record(stream)
+            mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        per = np.array([a.elapsed_time(b) * 1e-3 for a, b in evs])
+        return mf, t_prep, per, wall
+
+    mf, t_prep, per, wall = time_storage(args.storage, args.steps, args.warmup)
+    y = yd.cpu().numpy()
+    t_jv = float(per.mean())
+    ncells, nq, ndl = sp.ncells, 27, sp.nd
+    B = jv_bytes(ncells, nq, ndl, sp.nfree)
+    info = op.info()
+    achieved = B / t_jv / 1e9
+    res = {
+        "metric": METRIC,
+        "value": sp.nfree / t_jv / 1e9,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_jv * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 smooth state seed 2, v ~ N(0,1) seed 3)",
+        "config": {"workload": f"Jv, compressible neo-Hookean lam=2 mu=1, unit cube {n}^3 Q2 hex, 27 points/cell, "
+                               f"FP64, tangent storage {args.storage}",
+                   "free_dofs": sp.nfree, "cells": ncells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one Jv = 1 fill + 8 colour launches of k_apply<3,3>",
+                     "algorithmic_bytes": B, "stored_tangent_bytes": info["tangent_bytes"]},
+        "prepare_ms": t_prep * 1e3,
+        "jv_ms_min": float(per.min()) * 1e3,
+        "jv_ms_median": float(np.median(per)) * 1e3,
+        "wall_ms_per_step": wall / args.steps * 1e3,
+        "checks": {"y_norm": float(np.linalg.norm(y)), "y_finite": bool(np.isfinite(y).all())},
+    }
+    variants = {}
+    for st in [s for s in args.variants.split(",") if s and s != args.storage]:
+        mfv, tpv, perv, _ = time_storage(st, max(10, args.steps // 2), 2)
+        yv = yd.cpu().numpy()
+        tv = float(perv.mean())
+        nent = 45 if st.startswith("sym") else 81
+        s_t = 4 if st.endswith("32") else 8
+        own = jv_bytes(ncells, nq, ndl, sp.nfree, s_t=s_t, nent=nent) - ncells * 10 * 8
+        variants[st] = {"gdofs": sp.nfree / tv / 1e9, "ms": tv * 1e3, "prepare_ms": tpv * 1e3,
+                        "own_bytes": own, "own_GBs": own / tv / 1e9, "own_frac": own / tv / 1e9 / HBM_PEAK_GBS,
+                        "rel_err_vs_full64": float(np.linalg.norm(yv - y) / np.linalg.norm(y))}
+        del mfv
+    res["variants"] = variants
+    if not args.no_cpu:
+        print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,616 @@
+// Host side of libhfk: the C ABI of include/hfk.h.  Handles own device memory; host arrays are borrowed.
+//
+// Set-up work that the reference does once per operator on the host stays on the host here too: the greedy
+// cell colouring (src/coloring.jl:4-28) and the sparsity pattern with its positions (src/assembly.jl:773-899),
+// both integer algorithms restated exactly, so that colours, colptr/rowval and nzpos are bit-identical to the
+// reference's on the same cell dof ids.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/hfk.h"
+#include "hfk_internal.hpp"
+
+namespace hfk {
+bool supported_qp(int Q, int P);
+}
+
+using namespace hfk;
+
+struct hfk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string t_err;
+
+int set_err(hfk_ctx* ctx, int code, const std::string& msg) {
+  t_err = msg;
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                          \
+  do {                                                                                             \
+    hipError_t e__ = (expr);                                                                       \
+    if (e__ != hipSuccess)                                                                         \
+      return set_err((ctx), HFK_EHIP, std::string("HyperFEMKernels: HIP error in ") + #expr + ": " + \
+                                           hipGetErrorString(e__));                                \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) {
+    if (p && bytes >= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// 1D Lagrange basis on Q equispaced nodes of [0,1] (midpoint if Q = 1) at xi (lagrange1d, src/tensorproduct.jl:24-47)
+void lagrange1d(int Q, double xi, double* N, double* dN) {
+  std::vector<double> xs(Q);
+  for (int i = 0; i < Q; ++i) xs[i] = Q == 1 ? 0.5 : double(i) / (Q - 1);
+  for (int i = 0; i < Q; ++i) {
+    double v = 1;
+    for (int j = 0; j < Q; ++j)
+      if (j != i) v *= (xi - xs[j]) / (xs[i] - xs[j]);
+    N[i] = v;
+    double d = 0;
+    for (int m = 0; m < Q; ++m) {
+      if (m == i) continue;
+      double t = 1.0 / (xs[i] - xs[m]);
+      for (int j = 0; j < Q; ++j)
+        if (j != i && j != m) t *= (xi - xs[j]) / (xs[i] - xs[j]);
+      d += t;
+    }
+    dN[i] = d;
+  }
+}
+
+// Gauss-Legendre points (ascending) and weights on [0, 1]: Newton iteration on the Legendre three-term recurrence.
+void gauss01(int P, double* x, double* w) {
+  for (int i = 0; i < P; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (P + 0.5)), dp = 1;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1, p1 = z;
+      for (int j = 2; j <= P; ++j) {
+        const double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = P * (z * p1 - p0) / (z * z - 1);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = 0.5 * (1 - z);
+    w[i] = 1.0 / ((1 - z * z) * dp * dp);
+  }
+}
+
+}  // namespace
+
+struct hfk_op {
+  hfk_ctx* ctx = nullptr;
+  int Q = 0, P = 0, nn = 0, nd = 0, np = 0, C = 0;
+  int64_t ncells = 0, nfree = 0, ndir = 0, nbatches = 0;
+  int law = 0;
+  double lam = 0, mu = 0;
+  Tab tab{};
+  std::vector<int32_t> cell_dofs;           // Gridap order, ncells*nd
+  std::vector<int32_t> ldof, nodeperm;      // as passed
+  std::vector<int32_t> color;               // 1-based colour of each cell
+  std::vector<BatchRange> color_batches;    // batches of each colour
+  std::vector<int64_t> batch_cell;          // nbatches*C, -1 padding
+  std::vector<int64_t> slot_of_cell;        // cell -> batch*C + position
+  std::vector<int32_t> loc_of;              // tensor index c*nn + t -> table-local dof
+  DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64;
+  bool has_mu_field = false;
+  int storage = -1;  // storage of the prepared tangents, -1: not prepared
+  // pattern (built on first use, as build_pattern!)
+  bool pattern_built = false;
+  std::vector<int64_t> colptr, rowval;
+  std::vector<int32_t> nzpos;  // Gridap cell order [cell][ii + nd*jj]
+
+  OpDev dev() const {
+    OpDev d;
+    d.Q = Q;
+    d.P = P;
+    d.C = C;
+    d.nn = nn;
+    d.np = np;
+    d.nd = nd;
+    d.law = law;
+    d.lam = lam;
+    d.mu = mu;
+    d.tab = tab;
+    d.ids = ids.as<int32_t>();
+    d.coords = coords.as<double>();
+    d.mu_pts = has_mu_field ? mu_pts.as<double>() : nullptr;
+    d.dvals = dvals.as<double>();
+    d.loc_of = loc_of_d.as<int32_t>();
+    d.nzpos = nzpos_d.as<int32_t>();
+    return d;
+  }
+  hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
+};
+
+namespace {
+
+// src/coloring.jl:4-28: greedy first fit over the free (test) dofs with 128-bit masks; colour = trailing_ones
+int color_cells(hfk_op* op) {
+  std::vector<unsigned __int128> masks((size_t)op->nfree, 0);
+  op->color.assign((size_t)op->ncells, 0);
+  for (int64_t cell = 0; cell < op->ncells; ++cell) {
+    unsigned __int128 used = 0;
+    const int32_t* ids = &op->cell_dofs[(size_t)cell * op->nd];
+    for (int j = 0; j < op->nd; ++j)
+      if (ids[j] > 0) used |= masks[ids[j] - 1];
+    int c = 0;
+    while (c < 128 && ((used >> c) & 1)) ++c;
+    if (c >= 128) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: more than 128 colours needed");
+    op->color[cell] = c + 1;
+    const unsigned __int128 bit = (unsigned __int128)1 << c;
+    for (int j = 0; j < op->nd; ++j)
+      if (ids[j] > 0) masks[ids[j] - 1] |= bit;
+  }
+  return HFK_OK;
+}
+
+int sync(hfk_op* op) {
+  HIPCHK(op->ctx, hipStreamSynchronize(op->ctx->stream));
+  return HFK_OK;
+}
+
+int bind(hfk_op* op) {
+  if (!op || !op->ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
+  return HFK_OK;
+}
+
+int run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) {
+  if (storage < 0 || storage > 3) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
+  const size_t bytes = (size_t)op->nbatches * store_entries(storage) * op->C * op->np * store_bytes(storage);
+  HIPCHK(op->ctx, op->tangents.alloc(bytes));
+  HIPCHK(op->ctx, launch_prepare(op->dev(), storage, op->tangents.p, x_dev, BatchRange{0, (int)op->nbatches}, s));
+  op->storage = storage;
+  return HFK_OK;
+}
+
+int run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s) {
+  if (op->storage < 0)
+    return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: mul! before prepare! (no tangents stored)");
+  HIPCHK(op->ctx, launch_fill(y_dev, op->nfree, 0.0, s));
+  const OpDev d = op->dev();
+  for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_apply(d, op->storage, op->tangents.p, dx_dev, y_dev, r, s));
+  return HFK_OK;
+}
+
+int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) {
+  HIPCHK(op->ctx, launch_fill(b_dev, op->nfree, 0.0, s));
+  const OpDev d = op->dev();
+  for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_residual(d, x_dev, b_dev, r, s));
+  return HFK_OK;
+}
+
+int stage_in(hfk_op* op, DevBuf& buf, const double* h, int64_t n) {
+  HIPCHK(op->ctx, buf.alloc((size_t)std::max<int64_t>(n, 1) * sizeof(double)));
+  if (n > 0) HIPCHK(op->ctx, hipMemcpyAsync(buf.p, h, (size_t)n * sizeof(double), hipMemcpyHostToDevice, op->ctx->stream));
+  return HFK_OK;
+}
+
+// sparsity (src/assembly.jl:773-818) with full_keys! (:835-842, :883-893) and cell_positions! (:864-880)
+int build_pattern(hfk_op* op) {
+  if (op->pattern_built) return HFK_OK;
+  const int nd = op->nd;
+  std::vector<uint64_t> keys;
+  keys.reserve((size_t)op->ncells * nd * nd);
+  for (int64_t cell = 0; cell < op->ncells; ++cell) {
+    const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
+    for (int jj = 0; jj < nd; ++jj) {
+      const int32_t c = ids[jj];
+      if (c <= 0) continue;
+      for (int ii = 0; ii < nd; ++ii) {
+        const int32_t r = ids[ii];
+        if (r > 0) keys.push_back(((uint64_t)c << 32) | (uint64_t)r);
+      }
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const int64_t ncols = op->nfree;
+  op->rowval.resize(keys.size());
+  std::vector<int64_t> cnt((size_t)ncols, 0);
+  for (size_t k = 0; k < keys.size(); ++k) {
+    cnt[(keys[k] >> 32) - 1]++;
+    op->rowval[k] = (int64_t)(keys[k] & 0xffffffffu);
+  }
+  op->colptr.resize((size_t)ncols + 1);
+  op->colptr[0] = 1;
+  for (int64_t c = 0; c < ncols; ++c) op->colptr[c + 1] = op->colptr[c] + cnt[c];
+  op->nzpos.assign((size_t)op->ncells * nd * nd, 0);
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nth; ++t)
+    pool.emplace_back([op, nd, t, nth]() {
+      for (int64_t cell = t; cell < op->ncells; cell += nth) {
+        const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
+        int32_t* pos = &op->nzpos[(size_t)cell * nd * nd];
+        for (int jj = 0; jj < nd; ++jj) {
+          const int32_t c = ids[jj];
+          if (c <= 0) continue;
+          const int64_t lo = op->colptr[c - 1] - 1, hi = op->colptr[c] - 1;
+          for (int ii = 0; ii < nd; ++ii) {
+            const int32_t r = ids[ii];
+            if (r <= 0) continue;
+            const int64_t* it = std::lower_bound(op->rowval.data() + lo, op->rowval.data() + hi, (int64_t)r);
+            pos[ii + nd * jj] = (int32_t)(it - op->rowval.data() + 1);
+          }
+        }
+      }
+    });
+  for (auto& th : pool) th.join();
+  op->pattern_built = true;
+  return HFK_OK;
+}
+
+int upload_nzpos(hfk_op* op) {
+  if (op->nzpos_d.p) return HFK_OK;
+  const size_t per = (size_t)op->nd * op->nd;
+  std::vector<int32_t> h((size_t)op->nbatches * op->C * per, 0);
+  for (int64_t cell = 0; cell < op->ncells; ++cell)
+    std::memcpy(&h[(size_t)op->slot_of_cell[cell] * per], &op->nzpos[(size_t)cell * per], per * sizeof(int32_t));
+  HIPCHK(op->ctx, op->nzpos_d.alloc(h.size() * sizeof(int32_t)));
+  HIPCHK(op->ctx, hipMemcpy(op->nzpos_d.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  return HFK_OK;
+}
+
+int run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s) {
+  int rc = build_pattern(op);
+  if (rc) return rc;
+  if ((rc = upload_nzpos(op))) return rc;
+  // tangents in FP64 full storage into a buffer of their own (the matrix-free tangents are left untouched)
+  const size_t bytes = (size_t)op->nbatches * 81 * op->C * op->np * sizeof(double);
+  HIPCHK(op->ctx, op->D64.alloc(bytes));
+  const OpDev d = op->dev();
+  HIPCHK(op->ctx, launch_prepare(d, 0, op->D64.p, x_dev, BatchRange{0, (int)op->nbatches}, s));
+  HIPCHK(op->ctx, launch_fill(nz_dev, (int64_t)op->rowval.size(), 0.0, s));
+  for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_assemble(d, op->D64.as<double>(), nz_dev, r, s));
+  return HFK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfk_version(void) { return 1; }
+
+const char* hfk_last_error(const hfk_ctx* ctx) { return ctx ? ctx->err.c_str() : t_err.c_str(); }
+
+int hfk_open(int device, hfk_ctx** out) {
+  if (!out) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null output pointer");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return set_err(nullptr, HFK_EHIP, "HyperFEMKernels: no HIP device available");
+  if (device < 0 || device >= n) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: device index out of range");
+  auto* ctx = new hfk_ctx();
+  ctx->device = device;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return set_err(nullptr, HFK_EHIP, std::string("HyperFEMKernels: ") + hipGetErrorString(e));
+  }
+  *out = ctx;
+  return HFK_OK;
+}
+
+int hfk_close(hfk_ctx* ctx) {
+  if (!ctx) return HFK_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HFK_OK;
+}
+
+int hfk_synchronize(hfk_ctx* ctx) {
+  if (!ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null context");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return HFK_OK;
+}
+
+int hfk_op_create(hfk_ctx* ctx, int Q, int P, int64_t ncells, const int32_t* cell_dofs, const int32_t* ldof,
+                  const int32_t* nodeperm, const double* coords, int64_t nfree, int64_t ndir, int law, double lam,
+                  double mu, hfk_op** out) {
+  if (!ctx || !out) return set_err(ctx, HFK_EARG, "HyperFEMKernels: null context or output pointer");
+  *out = nullptr;
+  if (!supported_qp(Q, P))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: unsupported (nodes, points) per direction: Q in 2..4, P in Q..4");
+  if (ncells <= 0 || !cell_dofs || !ldof || !nodeperm || !coords)
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: empty mesh or null array");
+  if (nfree <= 0 || nfree >= (int64_t)1 << 31 || ndir < 0 || ndir >= (int64_t)1 << 31)
+    return set_err(ctx, HFK_EDIM, "HyperFEMKernels: free/Dirichlet dof counts out of range for Int32 ids");
+  if (law < 0 || law > 2) return set_err(ctx, HFK_EARG, "HyperFEMKernels: unknown law");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<hfk_op> op(new hfk_op());
+  op->ctx = ctx;
+  op->Q = Q;
+  op->P = P;
+  op->nn = Q * Q * Q;
+  op->nd = 3 * op->nn;
+  op->np = P * P * P;
+  op->C = cells_per_batch(P);
+  op->ncells = ncells;
+  op->nfree = nfree;
+  op->ndir = ndir;
+  op->law = law;
+  op->lam = lam;
+  op->mu = mu;
+  const int nn = op->nn, nd = op->nd;
+  op->cell_dofs.assign(cell_dofs, cell_dofs + (size_t)ncells * nd);
+  op->ldof.assign(ldof, ldof + nd);
+  op->nodeperm.assign(nodeperm, nodeperm + nn);
+  // validate the tables and ids
+  std::vector<int> seen(nd, 0), seen_n(nn, 0);
+  for (int j = 0; j < nd; ++j) {
+    if (ldof[j] < 0 || ldof[j] >= nd || seen[ldof[j]]++) return set_err(ctx, HFK_EARG, "HyperFEMKernels: ldof is not a permutation");
+  }
+  for (int t = 0; t < nn; ++t) {
+    if (nodeperm[t] < 0 || nodeperm[t] >= nn || seen_n[nodeperm[t]]++)
+      return set_err(ctx, HFK_EARG, "HyperFEMKernels: nodeperm is not a permutation");
+  }
+  for (size_t i = 0; i < op->cell_dofs.size(); ++i) {
+    const int32_t id = op->cell_dofs[i];
+    if (id == 0 || id > nfree || -(int64_t)id > ndir)
+      return set_err(ctx, HFK_EDIM, "HyperFEMKernels: cell dof id out of range");
+  }
+  // 1D tables
+  double xq[4], wq[4];
+  gauss01(P, xq, wq);
+  for (int a = 0; a < P; ++a) {
+    lagrange1d(Q, xq[a], &op->tab.N[a * Q], &op->tab.dN[a * Q]);
+    op->tab.xq[a] = xq[a];
+    op->tab.wq[a] = wq[a];
+  }
+  int rc = color_cells(op.get());
+  if (rc) return rc;
+  // batches, colour by colour (cells of a colour in increasing order, as the reference's groups)
+  const int ncol = ncells ? *std::max_element(op->color.begin(), op->color.end()) : 0;
+  std::vector<std::vector<int64_t>> groups(ncol);
+  for (int64_t cell = 0; cell < ncells; ++cell) groups[op->color[cell] - 1].push_back(cell);
+  const int C = op->C;
+  op->slot_of_cell.assign((size_t)ncells, -1);
+  for (int c = 0; c < ncol; ++c) {
+    const int64_t nb = ((int64_t)groups[c].size() + C - 1) / C;
+    op->color_batches.push_back(BatchRange{(int)op->nbatches, (int)nb});
+    for (int64_t b = 0; b < nb; ++b)
+      for (int i = 0; i < C; ++i) {
+        const int64_t k = b * C + i;
+        const int64_t cell = k < (int64_t)groups[c].size() ? groups[c][k] : -1;
+        if (cell >= 0) op->slot_of_cell[cell] = (int64_t)op->batch_cell.size();
+        op->batch_cell.push_back(cell);
+      }
+    op->nbatches += nb;
+  }
+  if (op->nbatches >= (int64_t)1 << 30) return set_err(ctx, HFK_EDIM, "HyperFEMKernels: mesh too large");
+  // device arrays in batch order, tensor node order
+  const size_t nslot = (size_t)op->nbatches * C;
+  std::vector<int32_t> hids(nslot * nd, 0);
+  std::vector<double> hx(nslot * 24);
+  static const double unit[24] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 1, 1, 0, 0, 0, 1, 1, 0, 1, 0, 1, 1, 1, 1, 1};
+  for (size_t s = 0; s < nslot; ++s) {
+    const int64_t cell = op->batch_cell[s];
+    if (cell < 0) {
+      std::memcpy(&hx[s * 24], unit, sizeof(unit));
+      continue;
+    }
+    const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
+    for (int c = 0; c < 3; ++c)
+      for (int t = 0; t < nn; ++t) hids[s * nd + c * nn + t] = ids[ldof[c * nn + nodeperm[t]]];
+    std::memcpy(&hx[s * 24], &coords[(size_t)cell * 24], 24 * sizeof(double));
+  }
+  op->loc_of.resize(nd);
+  for (int c = 0; c < 3; ++c)
+    for (int t = 0; t < nn; ++t) op->loc_of[c * nn + t] = ldof[c * nn + nodeperm[t]];
+  HIPCHK(ctx, op->ids.alloc(hids.size() * sizeof(int32_t)));
+  HIPCHK(ctx, hipMemcpy(op->ids.p, hids.data(), hids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(ctx, op->coords.alloc(hx.size() * sizeof(double)));
+  HIPCHK(ctx, hipMemcpy(op->coords.p, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(ctx, op->loc_of_d.alloc(nd * sizeof(int32_t)));
+  HIPCHK(ctx, hipMemcpy(op->loc_of_d.p, op->loc_of.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(ctx, op->dvals.alloc((size_t)std::max<int64_t>(ndir, 1) * sizeof(double)));
+  HIPCHK(ctx, hipMemset(op->dvals.p, 0, (size_t)std::max<int64_t>(ndir, 1) * sizeof(double)));
+  *out = op.release();
+  return HFK_OK;
+}
+
+int hfk_op_destroy(hfk_op* op) {
+  if (!op) return HFK_OK;
+  (void)hipSetDevice(op->ctx->device);
+  delete op;
+  return HFK_OK;
+}
+
+int hfk_op_set_dirichlet(hfk_op* op, const double* dvals) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (op->ndir > 0 && !dvals) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null Dirichlet values");
+  if (op->ndir > 0)
+    HIPCHK(op->ctx, hipMemcpy(op->dvals.p, dvals, (size_t)op->ndir * sizeof(double), hipMemcpyHostToDevice));
+  return HFK_OK;
+}
+
+int hfk_op_set_mu_field(hfk_op* op, const double* mu_points) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!mu_points) {
+    op->has_mu_field = false;
+    return HFK_OK;
+  }
+  const size_t nslot = (size_t)op->nbatches * op->C;
+  std::vector<double> h(nslot * op->np, op->mu);
+  for (size_t s = 0; s < nslot; ++s) {
+    const int64_t cell = op->batch_cell[s];
+    if (cell >= 0) std::memcpy(&h[s * op->np], &mu_points[(size_t)cell * op->np], op->np * sizeof(double));
+  }
+  HIPCHK(op->ctx, op->mu_pts.alloc(h.size() * sizeof(double)));
+  HIPCHK(op->ctx, hipMemcpy(op->mu_pts.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  op->has_mu_field = true;
+  return HFK_OK;
+}
+
+int hfk_op_info(hfk_op* op, int64_t* info) {
+  if (!op || !info) return set_err(op ? op->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  info[0] = op->nfree;
+  info[1] = op->ndir;
+  info[2] = op->ncells;
+  info[3] = op->nd;
+  info[4] = op->np;
+  info[5] = (int64_t)op->color_batches.size();
+  info[6] = op->storage < 0 ? 0 : (int64_t)op->nbatches * store_entries(op->storage) * op->C * op->np * store_bytes(op->storage);
+  info[7] = op->pattern_built ? (int64_t)op->rowval.size() : -1;
+  return HFK_OK;
+}
+
+int hfk_op_colors(hfk_op* op, int32_t* color_of_cell) {
+  if (!op || !color_of_cell) return set_err(op ? op->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  std::memcpy(color_of_cell, op->color.data(), op->color.size() * sizeof(int32_t));
+  return HFK_OK;
+}
+
+int hfk_residual(hfk_op* op, const double* x, double* b) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x || !b) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
+  if ((rc = stage_in(op, op->x_stage, x, op->nfree))) return rc;
+  HIPCHK(op->ctx, op->y_stage.alloc((size_t)op->nfree * sizeof(double)));
+  if ((rc = run_residual(op, op->x_stage.as<double>(), op->y_stage.as<double>(), op->ctx->stream))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(b, op->y_stage.p, (size_t)op->nfree * sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+  return sync(op);
+}
+
+int hfk_residual_dev(hfk_op* op, const double* x_dev, double* b_dev, void* stream) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x_dev || !b_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
+  return run_residual(op, x_dev, b_dev, op->stream(stream));
+}
+
+int hfk_mf_prepare(hfk_op* op, const double* x, int storage) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null state");
+  if ((rc = stage_in(op, op->x_stage, x, op->nfree))) return rc;
+  if ((rc = run_prepare(op, op->x_stage.as<double>(), storage, op->ctx->stream))) return rc;
+  return sync(op);
+}
+
+int hfk_mf_prepare_dev(hfk_op* op, const double* x_dev, int storage, void* stream) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null state");
+  return run_prepare(op, x_dev, storage, op->stream(stream));
+}
+
+int hfk_mf_apply(hfk_op* op, const double* dx, double* y) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!dx || !y) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
+  if ((rc = stage_in(op, op->x_stage, dx, op->nfree))) return rc;
+  HIPCHK(op->ctx, op->y_stage.alloc((size_t)op->nfree * sizeof(double)));
+  if ((rc = run_apply(op, op->x_stage.as<double>(), op->y_stage.as<double>(), op->ctx->stream))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(y, op->y_stage.p, (size_t)op->nfree * sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+  return sync(op);
+}
+
+int hfk_mf_apply_dev(hfk_op* op, const double* dx_dev, double* y_dev, void* stream) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!dx_dev || !y_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
+  return run_apply(op, dx_dev, y_dev, op->stream(stream));
+}
+
+int hfk_mf_tangents(hfk_op* op, double* D) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!D) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null output");
+  if (op->storage < 0) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: no tangents prepared");
+  const size_t nslot = (size_t)op->nbatches * op->C;
+  DevBuf tmp;
+  HIPCHK(op->ctx, tmp.alloc(nslot * op->np * 81 * sizeof(double)));
+  HIPCHK(op->ctx, launch_expand_tangents(op->dev(), op->storage, op->tangents.p, tmp.as<double>(), op->nbatches, op->ctx->stream));
+  std::vector<double> h(nslot * op->np * 81);
+  HIPCHK(op->ctx, hipMemcpyAsync(h.data(), tmp.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+  if ((rc = sync(op))) return rc;
+  const size_t per = (size_t)op->np * 81;
+  for (int64_t cell = 0; cell < op->ncells; ++cell)
+    std::memcpy(&D[(size_t)cell * per], &h[(size_t)op->slot_of_cell[cell] * per], per * sizeof(double));
+  return HFK_OK;
+}
+
+int hfk_pattern_nnz(hfk_op* op, int64_t* nnz) {
+  if (!op || !nnz) return set_err(op ? op->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  int rc = build_pattern(op);
+  if (rc) return rc;
+  *nnz = (int64_t)op->rowval.size();
+  return HFK_OK;
+}
+
+int hfk_pattern(hfk_op* op, int64_t* colptr, int64_t* rowval, int32_t* nzpos) {
+  if (!op) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  int rc = build_pattern(op);
+  if (rc) return rc;
+  if (colptr) std::memcpy(colptr, op->colptr.data(), op->colptr.size() * sizeof(int64_t));
+  if (rowval) std::memcpy(rowval, op->rowval.data(), op->rowval.size() * sizeof(int64_t));
+  if (nzpos) std::memcpy(nzpos, op->nzpos.data(), op->nzpos.size() * sizeof(int32_t));
+  return HFK_OK;
+}
+
+int hfk_jacobian(hfk_op* op, const double* x, double* nzval) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x || !nzval) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  if ((rc = build_pattern(op))) return rc;
+  if ((rc = stage_in(op, op->x_stage, x, op->nfree))) return rc;
+  const size_t nnz = op->rowval.size();
+  DevBuf nz;
+  HIPCHK(op->ctx, nz.alloc(std::max<size_t>(nnz, 1) * sizeof(double)));
+  if ((rc = run_jacobian(op, op->x_stage.as<double>(), nz.as<double>(), op->ctx->stream))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(nzval, nz.p, nnz * sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+  return sync(op);
+}
+
+int hfk_jacobian_dev(hfk_op* op, const double* x_dev, double* nzval_dev, void* stream) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x_dev || !nzval_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  return run_jacobian(op, x_dev, nzval_dev, op->stream(stream));
+}
+
+}  // extern "C"

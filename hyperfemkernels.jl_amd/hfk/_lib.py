@@ -1,0 +1,87 @@
+"""ctypes binding of libhfk.so (the C ABI of include/hfk.h).
+
+The library is built in-tree by ``hfk.build``; loading fails loudly when it is missing -- there is no CPU
+fallback for any operator of this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhfk.so")
+
+HFK_OK = 0
+STATUS = {-1: "ArgumentError", -2: "DimensionMismatch", -3: "ErrorException", -4: "HIP error", -5: "allocation failure"}
+
+STORE = {"full64": 0, "sym64": 1, "full32": 2, "sym32": 3}
+LAWS = {"svk": 0, "neohookean": 1, "linear": 2}
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); every symbol that include/hfk.h declares
+SIGNATURES = {
+    "hfk_version": (C.c_int, []),
+    "hfk_open": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "hfk_close": (C.c_int, [_vp]),
+    "hfk_last_error": (C.c_char_p, [_vp]),
+    "hfk_synchronize": (C.c_int, [_vp]),
+    "hfk_op_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int64, _i32p, _i32p, _i32p, _f64p, C.c_int64, C.c_int64,
+                                C.c_int, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "hfk_op_destroy": (C.c_int, [_vp]),
+    "hfk_op_set_dirichlet": (C.c_int, [_vp, _f64p]),
+    "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
+    "hfk_op_info": (C.c_int, [_vp, _i64p]),
+    "hfk_op_colors": (C.c_int, [_vp, _i32p]),
+    "hfk_residual": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_residual_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_mf_prepare": (C.c_int, [_vp, _f64p, C.c_int]),
+    "hfk_mf_prepare_dev": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "hfk_mf_apply": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_mf_apply_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_mf_tangents": (C.c_int, [_vp, _f64p]),
+    "hfk_pattern_nnz": (C.c_int, [_vp, _i64p]),
+    "hfk_pattern": (C.c_int, [_vp, _i64p, _i64p, _i32p]),
+    "hfk_jacobian": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_jacobian_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class HFKError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, 'error')} ({status}): {msg}")
+        self.status = status
+
+
+def load(path: str = LIB_PATH):
+    """Load libhfk.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"libhfk.so not found at {path}: build it with hfk.build.build() "
+                          "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, ctx=None):
+    if status != HFK_OK:
+        lib = load()
+        msg = lib.hfk_last_error(ctx)
+        raise HFKError(status, msg.decode() if msg else "")
+
+
+def ptr(a, ctype):
+    """Pointer to the data of a contiguous numpy array."""
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,264 @@
+"""Host mirror of the reference's operator surface, over the C ABI of libhfk.
+
+Mirrors (src/assembly.jl:630-666, 904-933; src/matrixfree.jl:33-260):
+
+* ``KernelOperator(form, U, V; jacobian=:assembled|:matrixfree, backend=:mi355x)`` -> ``KernelOperator(form,
+  space, jacobian="assembled"|"matrixfree", backend="mi355x")``, with ``residual``/``jacobian``/
+  ``residual_and_jacobian``/``allocate_residual``/``allocate_jacobian``/``zero_initial_guess``/
+  ``sparsity_pattern``/``nonzero_positions``;
+* ``MatrixFreeJacobian(op)`` with ``prepare``, ``mul`` (``mul!``), ``@`` (``*``), ``shape`` (``size``).
+
+Julia's ``!`` methods overwrite their first argument and return it; the methods here do the same.  Errors map
+to the reference's exceptions: ``ValueError`` for ``ArgumentError``/``DimensionMismatch``, ``RuntimeError``
+for ``ErrorException``.  Everything runs in the HIP library: there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import LAWS, STORE, check, ptr
+
+
+@dataclass(frozen=True)
+class HyperelasticForm:
+    """The weak form ``res(u, v) = ∫(∇(v)' ⊙ (P ∘ (F ∘ ∇(u)')))dΩ`` with ``jac(u, du, v) = ∫(∇(v)' ⊙ (dP ∘
+    (F ∘ ∇(u)', ∇(du)')))dΩ`` (README.md:53-67) for a built-in law, ``F = I + ∇u'``, on a measure of
+    ``degree`` (``Measure(Ω, degree)``: ``degree // 2 + 1`` Gauss points per direction)."""
+
+    law: str = "svk"
+    lam: float = 2.0
+    mu: float = 1.0
+    degree: int = 4
+
+    @property
+    def points1d(self) -> int:
+        return self.degree // 2 + 1
+
+
+class Context:
+    """A HIP device and stream (``hfk_open``)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.hfk_open(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        check(self.lib.hfk_synchronize(self.handle), self.handle)
+
+    def close(self):
+        if self.handle:
+            self.lib.hfk_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class KernelOperator:
+    """Residual and Jacobian of ``form`` on ``space`` (a ``mesh.HexSpace``), assembled on the GPU."""
+
+    def __init__(self, form: HyperelasticForm, space, jacobian: str = "assembled", backend: str = "mi355x",
+                 dirichlet_values=None, ctx: Context | None = None):
+        if backend != "mi355x":
+            raise ValueError(f"HyperFEMKernels: unknown backend {backend!r} (this build is mi355x only)")
+        if jacobian not in ("assembled", "matrixfree"):
+            raise ValueError("HyperFEMKernels: jacobian must be 'assembled' or 'matrixfree'")
+        if form.law not in LAWS:
+            raise ValueError(f"HyperFEMKernels: unknown law {form.law!r}")
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.form, self.space, self.jacobian_kind = form, space, jacobian
+        self.nfree, self.ndir = int(space.nfree), int(space.ndir)
+        h = C.c_void_p()
+        cd = np.ascontiguousarray(space.cell_dofs, dtype=np.int32)
+        ld = np.ascontiguousarray(space.ldof, dtype=np.int32)
+        npm = np.ascontiguousarray(space.nodeperm, dtype=np.int32)
+        xy = np.ascontiguousarray(space.coords, dtype=np.float64)
+        check(self.lib.hfk_op_create(self.ctx.handle, space.Q, form.points1d, space.ncells, ptr(cd, C.c_int32),
+                                     ptr(ld, C.c_int32), ptr(npm, C.c_int32), ptr(xy, C.c_double), self.nfree,
+                                     self.ndir, LAWS[form.law], float(form.lam), float(form.mu), C.byref(h)),
+              self.ctx.handle)
+        self.handle = h
+        self._pattern = None
+        if dirichlet_values is not None:
+            self.set_dirichlet(dirichlet_values)
+
+    # --- data --------------------------------------------------------------------------------------------
+    def set_dirichlet(self, dvals):
+        d = np.ascontiguousarray(dvals, dtype=np.float64)
+        if d.size != self.ndir:
+            raise ValueError("HyperFEMKernels: Dirichlet values of the wrong length")
+        check(self.lib.hfk_op_set_dirichlet(self.handle, ptr(d, C.c_double)), self.ctx.handle)
+
+    def set_mu_field(self, mu_points):
+        if mu_points is None:
+            check(self.lib.hfk_op_set_mu_field(self.handle, None), self.ctx.handle)
+            return
+        m = np.ascontiguousarray(mu_points, dtype=np.float64)
+        if m.size != self.space.ncells * self.form.points1d ** 3:
+            raise ValueError("HyperFEMKernels: mu field must have np values per cell")
+        check(self.lib.hfk_op_set_mu_field(self.handle, ptr(m, C.c_double)), self.ctx.handle)
+
+    def info(self) -> dict:
+        a = np.zeros(8, dtype=np.int64)
+        check(self.lib.hfk_op_info(self.handle, ptr(a, C.c_int64)), self.ctx.handle)
+        keys = ["nfree", "ndir", "ncells", "nd", "np", "ncolors", "tangent_bytes", "nnz"]
+        return dict(zip(keys, (int(v) for v in a)))
+
+    def colors(self) -> np.ndarray:
+        c = np.zeros(self.space.ncells, dtype=np.int32)
+        check(self.lib.hfk_op_colors(self.handle, ptr(c, C.c_int32)), self.ctx.handle)
+        return c
+
+    # --- Gridap Algebra interface ------------------------------------------------------------------------
+    def _vec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.nfree:
+            raise ValueError(f"HyperFEMKernels: DimensionMismatch: vector of length {x.size}, expected {self.nfree}")
+        return x
+
+    def allocate_residual(self, x=None):
+        return np.zeros(self.nfree)
+
+    def zero_initial_guess(self):
+        return np.zeros(self.nfree)
+
+    def residual(self, b, x):
+        """``residual!(b, op, x)``: b overwritten with the residual at state x (free rows)."""
+        x = self._vec(x)
+        if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != self.nfree:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: residual vector")
+        check(self.lib.hfk_residual(self.handle, ptr(x, C.c_double), ptr(b, C.c_double)), self.ctx.handle)
+        return b
+
+    def pattern(self):
+        if self._pattern is None:
+            nnz = C.c_int64()
+            check(self.lib.hfk_pattern_nnz(self.handle, C.byref(nnz)), self.ctx.handle)
+            colptr = np.zeros(self.nfree + 1, dtype=np.int64)
+            rowval = np.zeros(nnz.value, dtype=np.int64)
+            nzpos = np.zeros((self.space.ncells, self.space.nd * self.space.nd), dtype=np.int32)
+            check(self.lib.hfk_pattern(self.handle, ptr(colptr, C.c_int64), ptr(rowval, C.c_int64),
+                                       ptr(nzpos, C.c_int32)), self.ctx.handle)
+            self._pattern = (colptr, rowval, nzpos)
+        return self._pattern
+
+    def sparsity_pattern(self):
+        """CSC pattern (1-based colptr/rowval as Julia's SparseMatrixCSC) as a scipy matrix of zeros."""
+        import scipy.sparse as sp
+
+        colptr, rowval, _ = self.pattern()
+        return sp.csc_matrix((np.zeros(rowval.size), rowval - 1, colptr - 1), shape=(self.nfree, self.nfree))
+
+    def nonzero_positions(self):
+        return self.pattern()[2]
+
+    def allocate_jacobian(self, x=None):
+        return self.sparsity_pattern()
+
+    def jacobian(self, A, x):
+        """``jacobian!(A, op, x)``: the nonzeros of ``A`` (scipy CSC with this operator's pattern) overwritten."""
+        x = self._vec(x)
+        colptr, rowval, _ = self.pattern()
+        if A.shape != (self.nfree, self.nfree) or A.nnz != rowval.size or not np.array_equal(A.indptr, colptr - 1):
+            raise RuntimeError("HyperFEMKernels: jacobian! on a matrix with another sparsity pattern")
+        if A.data.dtype != np.float64 or not A.data.flags.c_contiguous:
+            A.data = np.ascontiguousarray(A.data, dtype=np.float64)
+        check(self.lib.hfk_jacobian(self.handle, ptr(x, C.c_double), ptr(A.data, C.c_double)), self.ctx.handle)
+        return A
+
+    def jacobian_values(self, x):
+        x = self._vec(x)
+        colptr, rowval, _ = self.pattern()
+        nz = np.zeros(rowval.size)
+        check(self.lib.hfk_jacobian(self.handle, ptr(x, C.c_double), ptr(nz, C.c_double)), self.ctx.handle)
+        return nz
+
+    def residual_and_jacobian(self, b, A, x):
+        self.residual(b, x)
+        self.jacobian(A, x)
+        return b, A
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.hfk_op_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MatrixFreeJacobian:
+    """``y = J(x) dx`` without assembling (src/matrixfree.jl:33-260).  ``storage`` selects how the point
+    tangents are kept: "full64" (the reference's 81 FP64 per point), "sym64" (45, for symmetric tangents),
+    "full32"/"sym32" (mixed precision)."""
+
+    def __init__(self, op: KernelOperator, storage: str = "full64"):
+        if storage not in STORE:
+            raise ValueError(f"HyperFEMKernels: unknown tangent storage {storage!r}")
+        self.op, self.storage = op, storage
+        self.x = None
+        self.shape = (op.nfree, op.nfree)
+
+    @property
+    def size(self):
+        return self.shape
+
+    def prepare(self, x):
+        """``prepare!(mf, x)``: tangents at the state x (a copy of x is kept, as the reference does)."""
+        x = self.op._vec(x)
+        self.x = x.copy()
+        check(self.op.lib.hfk_mf_prepare(self.op.handle, ptr(x, C.c_double), STORE[self.storage]),
+              self.op.ctx.handle)
+        return self
+
+    def mul(self, y, dx):
+        """``mul!(y, mf, dx)``."""
+        dx = self.op._vec(dx)
+        if y.dtype != np.float64 or not y.flags.c_contiguous or y.size != self.op.nfree:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: output vector")
+        check(self.op.lib.hfk_mf_apply(self.op.handle, ptr(dx, C.c_double), ptr(y, C.c_double)), self.op.ctx.handle)
+        return y
+
+    def __matmul__(self, dx):
+        return self.mul(np.zeros(self.op.nfree), dx)
+
+    def tangents(self) -> np.ndarray:
+        """Stored tangents in reference coordinates, (ncells, np, 81) with entry i + 9 j."""
+        sp = self.op.space
+        npts = self.op.form.points1d ** 3
+        D = np.zeros((sp.ncells, npts, 81))
+        check(self.op.lib.hfk_mf_tangents(self.op.handle, ptr(D, C.c_double)), self.op.ctx.handle)
+        return D
+
+    # device-pointer variants (torch tensors or raw addresses), asynchronous on `stream`
+    def prepare_dev(self, x_ptr: int, stream: int = 0):
+        check(self.op.lib.hfk_mf_prepare_dev(self.op.handle, C.c_void_p(x_ptr), STORE[self.storage],
+                                             C.c_void_p(stream)), self.op.ctx.handle)
+
+    def mul_dev(self, y_ptr: int, dx_ptr: int, stream: int = 0):
+        check(self.op.lib.hfk_mf_apply_dev(self.op.handle, C.c_void_p(dx_ptr), C.c_void_p(y_ptr),
+                                           C.c_void_p(stream)), self.op.ctx.handle)

@@ -1,0 +1,109 @@
+/* libhfk — MI355X (gfx950) kernels for the matrix-free and assembly hot path of HyperFEMKernels.jl.
+ *
+ * C ABI, bound by `ccall` from the Julia host (hyperfemkernels.jl_amd/julia/) and by ctypes from the Python
+ * mirror (hyperfemkernels.jl_amd/hfk/).  The reference is pure Julia and has no FFI: each entry point below
+ * replaces a Julia method of the reference, cited as file:line of HyperFEMKernels.jl v0.2.6.
+ *
+ * Conventions
+ *   - Every function returns HFK_OK (0) or a negative status; hfk_last_error() gives the message.  Misuse
+ *     maps to the reference's exceptions (ArgumentError -> HFK_EARG, DimensionMismatch -> HFK_EDIM,
+ *     ErrorException -> HFK_ESTATE).  Numerical failure is never an error status (solver calls report it
+ *     through their status outputs, as the reference's `converged`/`indefinite` flags).
+ *   - Host pointers are borrowed for the duration of the call.  Device memory is owned by the handles and
+ *     freed by *_destroy / hfk_close.  `*_dev` variants take device pointers and a hipStream_t (NULL: the
+ *     context's stream) and return without synchronising.
+ *   - Ids follow Gridap's cell dof ids: cell_dofs[cell*nd + j] > 0 is a free dof (1-based), < 0 a Dirichlet
+ *     value (-1-based).  Table indices (ldof, nodeperm) are 0-based.  CSC arrays (colptr, rowval) and the
+ *     positions nzpos are 1-based as in Julia's SparseMatrixCSC, nzpos = 0 meaning "not stored".
+ *   - A context is bound to one device and one stream; neither a context nor an operator is reentrant
+ *     (src/assembly.jl:363-367).
+ */
+#ifndef HFK_H
+#define HFK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFK_OK 0
+#define HFK_EARG (-1)   /* ArgumentError */
+#define HFK_EDIM (-2)   /* DimensionMismatch */
+#define HFK_ESTATE (-3) /* ErrorException: call out of order, unsupported configuration */
+#define HFK_EHIP (-4)   /* HIP runtime failure */
+#define HFK_EALLOC (-5) /* device or host allocation failure */
+
+/* Pointwise laws of the built-in forms  ∫ ∇(v)' ⊙ P(F(∇u')) dΩ  with F = I + ∇u' (README.md:53-67,
+ * test/common.jl:2-8): tangent T[(J,c),(L,k)] = ∂P_cJ/∂F_kL (src/factorforms.jl:317-322). */
+#define HFK_LAW_SVK 0         /* St Venant-Kirchhoff, test/common.jl:2-8 */
+#define HFK_LAW_NEOHOOKE 1    /* W = mu/2 (I1-3) - mu lnJ + lam/2 (lnJ)^2 */
+#define HFK_LAW_LINEAR 2      /* sigma = lam tr(eps) I + 2 mu eps on G = F - I */
+
+/* Storage of the prepared point tangents (src/matrixfree.jl:14-23 stores 81 FP64 per point). */
+#define HFK_STORE_FULL64 0    /* 81 FP64 per point (the reference's layout, in reference coordinates) */
+#define HFK_STORE_SYM64 1     /* 45 FP64 per point: the symmetric part of a tangent that is symmetric */
+#define HFK_STORE_FULL32 2    /* 81 FP32 per point (mixed precision) */
+#define HFK_STORE_SYM32 3     /* 45 FP32 per point */
+
+typedef struct hfk_ctx hfk_ctx;
+typedef struct hfk_op hfk_op;
+
+/* ---- context ----------------------------------------------------------------------------------- */
+int hfk_version(void);
+/* Bind a context to HIP device `device` with its own stream (default priority). */
+int hfk_open(int device, hfk_ctx** ctx);
+int hfk_close(hfk_ctx* ctx);
+/* Message of the last failing call on `ctx` (or of this thread when ctx is NULL). */
+const char* hfk_last_error(const hfk_ctx* ctx);
+int hfk_synchronize(hfk_ctx* ctx);
+
+/* ---- operator: KernelOperator(res, jac, U, V; backend = :mi355x)  (src/assembly.jl:630-666) ----- */
+/* One volume group of a vector Q_{Q-1} Lagrangian field on hexahedra with P^3 tensor Gauss points.
+ *   cell_dofs  nd*ncells, nd = 3*Q^3, Gridap order (get_cell_dof_ids, src/tables.jl:329-342)
+ *   ldof       3*Q^3: ldof[c*nn + n] = local dof of table node n, component c (src/tables.jl:171-237)
+ *   nodeperm   Q^3:   tensor node i + Q j + Q^2 k -> table node (src/tensorproduct.jl:13-91)
+ *   coords     ncells*8*3 vertices of the trilinear cell map, v = i + 2j + 4k (src/assembly.jl:1073-1111)
+ * Colours the cells with the reference's greedy first fit (src/coloring.jl:4-28). */
+int hfk_op_create(hfk_ctx* ctx, int Q, int P, int64_t ncells, const int32_t* cell_dofs, const int32_t* ldof,
+                  const int32_t* nodeperm, const double* coords, int64_t nfree, int64_t ndir, int law,
+                  double lam, double mu, hfk_op** op);
+int hfk_op_destroy(hfk_op* op);
+/* Dirichlet values, re-read by every assembly (refresh_dirichlet_values!, src/assembly.jl:393-410). */
+int hfk_op_set_dirichlet(hfk_op* op, const double* dvals);
+/* Pointwise shear modulus mu(x) at the quadrature points, np*ncells in tensor point order, [cell][q]
+ * (dense auxiliary point data; NULL restores the constant mu). */
+int hfk_op_set_mu_field(hfk_op* op, const double* mu_points);
+/* info[0..7] = nfree, ndir, ncells, nd, np, ncolors, tangent bytes of the current storage, nnz (or -1). */
+int hfk_op_info(hfk_op* op, int64_t* info);
+/* Colour of each cell, 1-based (the reference's `color[cell]`, src/coloring.jl:4-28). */
+int hfk_op_colors(hfk_op* op, int32_t* color_of_cell);
+
+/* ---- residual: Algebra.residual!(b, op, x)  (src/assembly.jl:909-916, 1219-1241) ----------------- */
+int hfk_residual(hfk_op* op, const double* x, double* b);
+int hfk_residual_dev(hfk_op* op, const double* x_dev, double* b_dev, void* stream);
+
+/* ---- matrix-free Jacobian (src/matrixfree.jl) ---------------------------------------------------- */
+/* prepare!(mf, x)  (src/matrixfree.jl:140-159, 199-228): point tangents at state x, stored per `storage`. */
+int hfk_mf_prepare(hfk_op* op, const double* x, int storage);
+int hfk_mf_prepare_dev(hfk_op* op, const double* x_dev, int storage, void* stream);
+/* mul!(y, mf, dx)  (src/matrixfree.jl:246-260, 341-424, 496-520): y = J(x) dx on free dofs. */
+int hfk_mf_apply(hfk_op* op, const double* dx, double* y);
+int hfk_mf_apply_dev(hfk_op* op, const double* dx_dev, double* y_dev, void* stream);
+/* The prepared tangent in reference coordinates, D = dV iJt' T iJt per point, [cell][q][i + 9 j] (81 per
+ * point, expanded from the stored form).  Inspection / tests. */
+int hfk_mf_tangents(hfk_op* op, double* D);
+
+/* ---- assembled Jacobian (src/assembly.jl:750-899, 909-927, 1316-1408) ---------------------------- */
+/* sparsity_pattern / nonzero_positions: nnz first, then colptr (nfree+1), rowval (nnz), nzpos
+ * (nd*nd*ncells, [cell][ii + nd*jj]).  Columns are trial dofs, rows test dofs. */
+int hfk_pattern_nnz(hfk_op* op, int64_t* nnz);
+int hfk_pattern(hfk_op* op, int64_t* colptr, int64_t* rowval, int32_t* nzpos);
+/* jacobian!(A, op, x): nonzeros of the CSC matrix, summed colour by colour through nzpos. */
+int hfk_jacobian(hfk_op* op, const double* x, double* nzval);
+int hfk_jacobian_dev(hfk_op* op, const double* x_dev, double* nzval_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

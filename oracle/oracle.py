@@ -1,0 +1,210 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes binding of the oracle (liboracle.so, built by oracle/Makefile).
+
+The oracle is the CPU restatement of the reference's FP64 algorithms (hfk_oracle.cpp, each function citing the
+HyperFEMKernels.jl file:line it follows).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+may import this module, and only as the checker: the product path (hfk, libhfk.so) never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    sig = {
+        "orc_gauss01": (None, [C.c_int, _f64p, _f64p]),
+        "orc_lagrange1d": (None, [C.c_int, C.c_double, _f64p, _f64p]),
+        "orc_law_P": (None, [C.c_int, C.c_double, C.c_double, _f64p, _f64p]),
+        "orc_law_T": (None, [C.c_int, C.c_double, C.c_double, _f64p, _f64p]),
+        "orc_law_W": (C.c_double, [C.c_int, C.c_double, C.c_double, _f64p]),
+        "orc_op_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _i32p, _f64p, C.c_int64, C.c_int64,
+                                       C.c_int, C.c_double, C.c_double, _f64p]),
+        "orc_op_destroy": (None, [C.c_void_p]),
+        "orc_op_set_mu_field": (None, [C.c_void_p, _f64p]),
+        "orc_op_set_dirichlet": (None, [C.c_void_p, _f64p]),
+        "orc_op_ncolors": (C.c_int, [C.c_void_p]),
+        "orc_op_colors": (C.c_int, [C.c_void_p, _i32p]),
+        "orc_last_error": (C.c_char_p, []),
+        "orc_prepare": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _f64p]),
+        "orc_jv": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int]),
+        "orc_energy": (C.c_double, [C.c_void_p, _f64p]),
+        "orc_residual": (C.c_int, [C.c_void_p, _f64p, _f64p]),
+        "orc_element_jacobians": (C.c_int, [C.c_void_p, _f64p, _f64p]),
+        "orc_pattern_nnz": (C.c_int64, [C.c_void_p]),
+        "orc_pattern": (C.c_int, [C.c_void_p, _i64p, _i64p, _i32p]),
+        "orc_jacobian": (C.c_int, [C.c_void_p, _f64p, _i32p, _f64p, C.c_int64]),
+        "orc_node_blocks": (C.c_int, [C.c_void_p, _i32p, _i32p, C.c_int64, _i64p]),
+        "orc_blocks_from_tangents": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _i32p, _i32p, C.c_int64, _f64p]),
+        "orc_apply_blocks": (None, [_f64p, _i32p, C.c_int64, _f64p, _f64p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+LAWS = {"svk": 0, "neohookean": 1, "linear": 2}
+
+
+def gauss01(P):
+    x, w = np.zeros(P), np.zeros(P)
+    load().orc_gauss01(P, _p(x, C.c_double), _p(w, C.c_double))
+    return x, w
+
+
+def lagrange1d(Q, xi):
+    N, dN = np.zeros(Q), np.zeros(Q)
+    load().orc_lagrange1d(Q, float(xi), _p(N, C.c_double), _p(dN, C.c_double))
+    return N, dN
+
+
+def law_P(law, lam, mu, g):
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    R = np.zeros(9)
+    load().orc_law_P(LAWS[law], lam, mu, _p(g, C.c_double), _p(R, C.c_double))
+    return R
+
+
+def law_T(law, lam, mu, g):
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    T = np.zeros(81)
+    load().orc_law_T(LAWS[law], lam, mu, _p(g, C.c_double), _p(T, C.c_double))
+    return T
+
+
+def law_W(law, lam, mu, g):
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    return load().orc_law_W(LAWS[law], lam, mu, _p(g, C.c_double))
+
+
+class OracleOp:
+    """The oracle's operator on the same arrays the C ABI takes (mesh.HexSpace conventions)."""
+
+    def __init__(self, space, law="svk", lam=2.0, mu=1.0, points1d=None, dvals=None, body=None):
+        L = load()
+        self.space = space
+        self.P = points1d or (space.order + 1)
+        self.np = self.P ** 3
+        self.nd = space.nd
+        self.ncells = space.ncells
+        self.nfree = space.nfree
+        self._keep = [np.ascontiguousarray(space.cell_dofs, dtype=np.int32),
+                      np.ascontiguousarray(space.nodeperm, dtype=np.int32),
+                      np.ascontiguousarray(space.ldof.reshape(-1), dtype=np.int32),
+                      np.ascontiguousarray(space.coords, dtype=np.float64)]
+        b = None if body is None else np.ascontiguousarray(body, dtype=np.float64)
+        self.h = L.orc_op_create(space.Q, self.P, space.ncells, _p(self._keep[0], C.c_int32),
+                                 _p(self._keep[1], C.c_int32), _p(self._keep[2], C.c_int32),
+                                 _p(self._keep[3], C.c_double), space.nfree, space.ndir, LAWS[law], lam, mu,
+                                 None if b is None else _p(b, C.c_double))
+        if not self.h:
+            raise RuntimeError(L.orc_last_error().decode())
+        if dvals is not None:
+            self.set_dirichlet(dvals)
+
+    def __del__(self):
+        try:
+            load().orc_op_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_dirichlet(self, dvals):
+        self._dv = np.ascontiguousarray(dvals, dtype=np.float64)
+        load().orc_op_set_dirichlet(self.h, _p(self._dv, C.c_double))
+
+    def set_mu_field(self, mu):
+        self._mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        load().orc_op_set_mu_field(self.h, None if mu is None else _p(self._mu, C.c_double))
+
+    def colors(self):
+        c = np.zeros(self.ncells, dtype=np.int32)
+        load().orc_op_colors(self.h, _p(c, C.c_int32))
+        return c
+
+    def prepare(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        T = np.zeros((self.ncells, self.np, 81))
+        iJt = np.zeros((self.ncells, self.np, 9))
+        dV = np.zeros((self.ncells, self.np))
+        load().orc_prepare(self.h, _p(x, C.c_double), _p(T, C.c_double), _p(iJt, C.c_double), _p(dV, C.c_double))
+        return T, iJt, dV
+
+    def jv(self, prep, dx, nthreads=0):
+        T, iJt, dV = prep
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        y = np.zeros(self.nfree)
+        load().orc_jv(self.h, _p(T, C.c_double), _p(iJt, C.c_double), _p(dV, C.c_double), _p(dx, C.c_double),
+                      _p(y, C.c_double), int(nthreads))
+        return y
+
+    def energy(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return load().orc_energy(self.h, _p(x, C.c_double))
+
+    def residual(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        b = np.zeros(self.nfree)
+        load().orc_residual(self.h, _p(x, C.c_double), _p(b, C.c_double))
+        return b
+
+    def element_jacobians(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        K = np.zeros((self.ncells, self.nd, self.nd))  # K[cell, jj, ii] = Ke[ii, jj] (column-major ii + nd*jj)
+        load().orc_element_jacobians(self.h, _p(x, C.c_double), _p(K, C.c_double))
+        return np.ascontiguousarray(K.transpose(0, 2, 1))  # Ke[cell, ii, jj]
+
+    def pattern(self):
+        L = load()
+        nnz = L.orc_pattern_nnz(self.h)
+        colptr = np.zeros(self.nfree + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzpos = np.zeros((self.ncells, self.nd * self.nd), dtype=np.int32)
+        L.orc_pattern(self.h, _p(colptr, C.c_int64), _p(rowval, C.c_int64), _p(nzpos, C.c_int32))
+        return colptr, rowval, nzpos
+
+    def jacobian(self, x, nzpos, nnz):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nzpos = np.ascontiguousarray(nzpos, dtype=np.int32)
+        nz = np.zeros(nnz)
+        load().orc_jacobian(self.h, _p(x, C.c_double), _p(nzpos, C.c_int32), _p(nz, C.c_double), nnz)
+        return nz
+
+
+def reference_coordinate_tangents(T, iJt, dV):
+    """D[cell, q, (K,c) + 9 (M,k)] = dV sum_J sum_L iJt[J,K] T[(J,c),(L,k)] iJt[L,M] from the reference-layout
+    T, iJt, dV (the storage libhfk keeps; see hfk_kernels.hip)."""
+    nc, npts = dV.shape
+    T4 = T.reshape(nc, npts, 3, 3, 3, 3, order="C")  # index order reversed: [k, L, c, J] for T[J + 3c + 9(L + 3k)]
+    Tm = np.transpose(T4, (0, 1, 5, 4, 3, 2))  # [J, c, L, k]
+    B = iJt.reshape(nc, npts, 3, 3)  # iJt[J + 3K] -> reversed [K, J]
+    Bm = np.transpose(B, (0, 1, 3, 2))  # [J, K]
+    D = np.einsum("xqJK,xqJcLk,xqLM->xqKcMk", Bm, Tm, Bm) * dV[:, :, None, None, None, None]
+    # flatten to i + 9 j with i = K + 3c, j = M + 3k: C-order reversed indices [k, M, c, K]
+    return np.ascontiguousarray(np.transpose(D, (0, 1, 5, 4, 3, 2)).reshape(nc, npts, 81))

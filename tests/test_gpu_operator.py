@@ -1,0 +1,139 @@
+"""GPU parity of libhfk against the oracle (the CPU restatement of the reference's FP64 algorithms).
+
+Bars: integer outputs (colours, CSC pattern, nzpos) bit-exact; FP64 outputs (tangents, Jv, residual, CSR
+values) within 1e-12 relative in the 2-norm (the reference's TOL, test/runtests.jl:9); FP32 tangent storage
+within 1e-5 relative (reported, mixed precision).
+"""
+import numpy as np
+import pytest
+
+import hfk
+from hfk import HyperelasticForm, KernelOperator, MatrixFreeJacobian
+from hfk.mesh import hex_space
+from hfk.synthetic import normal, smooth_field
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def problem(order, P, law, n, perturb=0.0, seed=0, amp=0.02):
+    sp = hex_space(n, order, perturb=perturb, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    dv = amp * rng.standard_normal(sp.ndir)
+    x = amp * rng.standard_normal(sp.nfree)
+    return sp, dv, x
+
+
+CASES = [
+    (1, 2, "svk", (4, 3, 3), 0.0),
+    (1, 3, "neohookean", (3, 3, 2), 0.1),
+    (2, 3, "svk", (3, 2, 2), 0.0),
+    (2, 3, "neohookean", (4, 3, 3), 0.15),
+    (2, 3, "linear", (3, 3, 3), 0.1),
+    (2, 4, "neohookean", (2, 2, 2), 0.1),
+    (3, 4, "neohookean", (2, 2, 2), 0.1),
+    (3, 4, "svk", (3, 2, 1), 0.0),
+]
+
+
+@pytest.mark.parametrize("order,P,law,n,perturb", CASES)
+def test_tangents_jv_residual(oracle, hip_ctx, order, P, law, n, perturb):
+    print(f"case order={order} P={P} law={law} n={n}", flush=True)
+    sp, dv, x = problem(order, P, law, n, perturb)
+    form = HyperelasticForm(law, 2.0, 1.0, degree=2 * (P - 1))
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, points1d=P, dvals=dv)
+    assert np.array_equal(op.colors(), orc.colors())
+    # prepare: stored reference-coordinate tangents
+    mf = MatrixFreeJacobian(op, "full64").prepare(x)
+    prep = orc.prepare(x)
+    Dref = oracle.reference_coordinate_tangents(*prep)
+    assert rel(mf.tangents(), Dref) < TOL
+    # Jv
+    dx = np.random.default_rng(7).standard_normal(sp.nfree)
+    yref = orc.jv(prep, dx)
+    assert rel(mf @ dx, yref) < TOL
+    # residual
+    assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < TOL
+
+
+@pytest.mark.parametrize("storage,tol", [("sym64", TOL), ("full32", 1e-5), ("sym32", 1e-5)])
+def test_storage_variants(oracle, hip_ctx, storage, tol):
+    sp, dv, x = problem(2, 3, "neohookean", (4, 3, 3), 0.1)
+    op = KernelOperator(HyperelasticForm("neohookean"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, "neohookean", 2.0, 1.0, dvals=dv)
+    dx = np.random.default_rng(8).standard_normal(sp.nfree)
+    yref = orc.jv(orc.prepare(x), dx)
+    mf = MatrixFreeJacobian(op, storage).prepare(x)
+    e = rel(mf @ dx, yref)
+    print(f"storage {storage}: rel error {e:.3e}", flush=True)
+    assert e < tol
+    info = op.info()
+    nent = 45 if storage.startswith("sym") else 81
+    assert info["tangent_bytes"] >= sp.ncells * 27 * nent * (4 if storage.endswith("32") else 8)
+
+
+def test_mu_field(oracle, hip_ctx):
+    sp, dv, x = problem(2, 3, "neohookean", (3, 3, 2), 0.05)
+    mu = np.exp(0.5 * np.random.default_rng(3).standard_normal(sp.ncells * 27))
+    op = KernelOperator(HyperelasticForm("neohookean"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    op.set_mu_field(mu)
+    orc = oracle.OracleOp(sp, "neohookean", 2.0, 1.0, dvals=dv)
+    orc.set_mu_field(mu)
+    dx = np.random.default_rng(9).standard_normal(sp.nfree)
+    prep = orc.prepare(x)
+    mf = MatrixFreeJacobian(op).prepare(x)
+    assert rel(mf @ dx, orc.jv(prep, dx)) < TOL
+    assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < TOL
+
+
+@pytest.mark.parametrize("order,law,n", [(1, "neohookean", (4, 3, 3)), (2, "svk", (3, 2, 2)), (2, "neohookean", (4, 4, 3))])
+def test_pattern_and_assembly(oracle, hip_ctx, order, law, n):
+    print(f"assembly order={order} law={law} n={n}", flush=True)
+    sp, dv, x = problem(order, order + 1, law, n, 0.1)
+    op = KernelOperator(HyperelasticForm(law, degree=2 * order), sp, dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, dvals=dv)
+    colptr, rowval, nzpos = op.pattern()
+    c2, r2, p2 = orc.pattern()
+    assert np.array_equal(colptr, c2) and np.array_equal(rowval, r2) and np.array_equal(nzpos, p2)
+    A = op.jacobian(op.allocate_jacobian(), x)
+    ref = orc.jacobian(x, p2, r2.size)
+    assert rel(A.data, ref) < TOL
+    # matrix-free = assembled . dx (test/test_matrixfree.jl:74-96)
+    mf = MatrixFreeJacobian(op).prepare(x)
+    dx = np.random.default_rng(4).standard_normal(sp.nfree)
+    assert rel(mf @ dx, A @ dx) < 1e-11
+
+
+def test_errors(hip_ctx):
+    sp, dv, x = problem(2, 3, "svk", (2, 2, 2))
+    op = KernelOperator(HyperelasticForm("svk"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    mf = MatrixFreeJacobian(op)
+    with pytest.raises(hfk._lib.HFKError, match="ErrorException"):
+        mf @ np.zeros(sp.nfree)  # mul! before prepare!
+    with pytest.raises(ValueError):
+        mf.prepare(np.zeros(sp.nfree + 1))
+    with pytest.raises(ValueError):
+        MatrixFreeJacobian(op, "bf16")
+
+
+def test_benchmark_size_properties(oracle, hip_ctx):
+    """64^3 Q2 neo-Hookean at the smooth random state: J is symmetric (w.Jv = v.Jw), linear, and equals
+    the oracle's product on the same state to 1e-12 (oracle on the host cores)."""
+    sp = hex_space((64, 64, 64), 2)
+    u = smooth_field(2)
+    x, dvals = sp.free_values(u), sp.dirichlet_values(u)
+    op = KernelOperator(HyperelasticForm("neohookean"), sp, jacobian="matrixfree", dirichlet_values=dvals, ctx=hip_ctx)
+    mf = MatrixFreeJacobian(op).prepare(x)
+    v = normal(3, sp.nfree)
+    w = normal(5, sp.nfree)
+    Jv, Jw = mf @ v, mf @ w
+    assert abs(w @ Jv - v @ Jw) / abs(w @ Jv) < 1e-11
+    assert rel(mf @ (2.0 * v - 3.0 * w), 2.0 * Jv - 3.0 * Jw) < 1e-13
+    orc = oracle.OracleOp(sp, "neohookean", 2.0, 1.0, dvals=dvals)
+    assert rel(Jv, orc.jv(orc.prepare(x), v)) < TOL

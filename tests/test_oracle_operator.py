@@ -1,0 +1,161 @@
+"""Pins of the oracle's operator-level functions on tiny meshes (brute force and invariants).
+
+* residual = gradient of the total energy (central differences), with Dirichlet values;
+* element Jacobians assembled densely = derivative of the residual (central differences);
+* sum-factorised stored-T product (src/matrixfree.jl:341-424) = dense assembled matrix times dx
+  (test/test_matrixfree.jl:74-96), serial = threaded (coloured determinism, test/test_assembly.jl:40-47);
+* pattern (src/assembly.jl:773-899) = Python set of (row, col) pairs; positions consistent;
+* assembled values through nzpos = dense matrix entries;
+* colouring (src/coloring.jl:4-28): no two cells of a colour share a free dof, and each cell has the smallest
+  colour not used by an earlier cell it shares a free dof with (greedy first fit);
+* geometry: sum of dV = volume; affine patch test (linear law, affine field: interior residual 0);
+* DOF counts of the benchmark meshes (open Dirichlet faces x = 0 and x = 1).
+"""
+import numpy as np
+import pytest
+
+from hfk.mesh import dirichlet_closed_x0, dirichlet_none, hex_space
+
+
+def dense_matrix(space, Ke):
+    n = space.nfree
+    K = np.zeros((n, n))
+    for cell in range(space.ncells):
+        ids = space.cell_dofs[cell]
+        f = ids > 0
+        r = ids[f] - 1
+        K[np.ix_(r, r)] += Ke[cell][np.ix_(f, f)]
+    return K
+
+
+def setup(order, n=(2, 2, 2), law="svk", perturb=0.0, dirichlet=None, seed=0):
+    kw = {} if dirichlet is None else {"dirichlet": dirichlet}
+    sp = hex_space(n, order, perturb=perturb, seed=seed, **kw)
+    rng = np.random.default_rng(seed + 10)
+    dv = 0.02 * rng.standard_normal(sp.ndir)
+    x = 0.02 * rng.standard_normal(sp.nfree)
+    return sp, dv, x
+
+
+@pytest.mark.parametrize("order,law", [(1, "svk"), (2, "neohookean"), (2, "svk"), (3, "linear")])
+def test_residual_is_energy_gradient(oracle, order, law):
+    n = (2, 1, 1) if order == 3 else (2, 2, 1)
+    sp, dv, x = setup(order, n, law, perturb=0.15)
+    op = oracle.OracleOp(sp, law, 2.0, 1.0, dvals=dv)
+    b = op.residual(x)
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    idx = rng.choice(sp.nfree, size=min(40, sp.nfree), replace=False)
+    for i in idx:
+        e = np.zeros(sp.nfree)
+        e[i] = h
+        fd = (op.energy(x + e) - op.energy(x - e)) / (2 * h)
+        assert fd == pytest.approx(b[i], abs=1e-9, rel=1e-6)
+
+
+@pytest.mark.parametrize("order,law", [(1, "neohookean"), (2, "svk"), (2, "linear")])
+def test_jacobian_is_residual_derivative(oracle, order, law):
+    sp, dv, x = setup(order, (2, 2, 1), law, perturb=0.1)
+    op = oracle.OracleOp(sp, law, 2.0, 1.0, dvals=dv)
+    K = dense_matrix(sp, op.element_jacobians(x))
+    h = 1e-6
+    rng = np.random.default_rng(1)
+    for j in rng.choice(sp.nfree, size=15, replace=False):
+        e = np.zeros(sp.nfree)
+        e[j] = h
+        fd = (op.residual(x + e) - op.residual(x - e)) / (2 * h)
+        assert np.allclose(K[:, j], fd, atol=1e-8, rtol=1e-6)
+
+
+@pytest.mark.parametrize("order,P,law", [(1, 2, "svk"), (2, 3, "neohookean"), (2, 4, "svk"), (3, 4, "neohookean")])
+def test_sum_factorised_equals_assembled(oracle, order, P, law):
+    n = (2, 2, 1) if order == 3 else (3, 2, 2)
+    sp, dv, x = setup(order, n, law, perturb=0.12)
+    op = oracle.OracleOp(sp, law, 2.0, 1.0, points1d=P, dvals=dv)
+    K = dense_matrix(sp, op.element_jacobians(x))
+    prep = op.prepare(x)
+    dx = np.random.default_rng(2).standard_normal(sp.nfree)
+    y1 = op.jv(prep, dx, nthreads=1)
+    y4 = op.jv(prep, dx, nthreads=4)
+    assert np.array_equal(y1, y4)  # colour order: bitwise identical for any thread count
+    ref = K @ dx
+    assert np.linalg.norm(y1 - ref) / np.linalg.norm(ref) < 1e-12
+
+
+def test_pattern_brute_force(oracle):
+    sp, dv, x = setup(2, (3, 2, 2))
+    op = oracle.OracleOp(sp, "svk", 2.0, 1.0, dvals=dv)
+    colptr, rowval, nzpos = op.pattern()
+    pairs = set()
+    for cell in range(sp.ncells):
+        ids = sp.cell_dofs[cell]
+        f = ids[ids > 0]
+        for c in f:
+            for r in f:
+                pairs.add((int(c), int(r)))
+    assert rowval.size == len(pairs)
+    got = set()
+    for c in range(sp.nfree):
+        rows = rowval[colptr[c] - 1:colptr[c + 1] - 1]
+        assert np.all(np.diff(rows) > 0)  # sorted, unique
+        got.update((c + 1, int(r)) for r in rows)
+    assert got == pairs
+    nd = sp.nd
+    for cell in range(sp.ncells):
+        ids = sp.cell_dofs[cell]
+        for jj in range(nd):
+            for ii in range(nd):
+                p = nzpos[cell, ii + nd * jj]
+                if ids[ii] > 0 and ids[jj] > 0:
+                    c = ids[jj]
+                    assert colptr[c - 1] <= p < colptr[c] and rowval[p - 1] == ids[ii]
+                else:
+                    assert p == 0
+    # assembled values = dense matrix entries
+    K = dense_matrix(sp, op.element_jacobians(x))
+    nz = op.jacobian(x, nzpos, rowval.size)
+    for c in range(sp.nfree):
+        for p in range(colptr[c] - 1, colptr[c + 1] - 1):
+            assert nz[p] == pytest.approx(K[rowval[p] - 1, c], abs=1e-13)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_colouring_greedy_first_fit(oracle, order):
+    sp, dv, x = setup(order, (4, 3, 3))
+    col = oracle.OracleOp(sp, "svk", 2.0, 1.0).colors()
+    cells_of = {}
+    for cell in range(sp.ncells):
+        for d in sp.cell_dofs[cell]:
+            if d > 0:
+                cells_of.setdefault(int(d), []).append(cell)
+    for cell in range(sp.ncells):
+        earlier = {col[o] for d in sp.cell_dofs[cell] if d > 0 for o in cells_of[int(d)] if o < cell}
+        expect = 1
+        while expect in earlier:
+            expect += 1
+        assert col[cell] == expect
+    for d, cells in cells_of.items():
+        assert len({col[c] for c in cells}) == len(cells)
+    assert col.max() == 8  # a structured hex mesh needs 8 colours
+
+
+def test_volume_and_patch(oracle):
+    # sum dV over all points is the volume; a linear field under the linear law has zero interior residual
+    sp = hex_space((3, 2, 2), 2, perturb=0.15, seed=3, dirichlet=lambda I, J, K, L: (I == 0) | (I == L[0] - 1) |
+                   (J == 0) | (J == L[1] - 1) | (K == 0) | (K == L[2] - 1))
+    A = np.array([[0.01, 0.02, -0.01], [0.0, 0.015, 0.005], [0.01, -0.02, 0.0]])
+    f = lambda xyz: xyz @ A.T
+    op = oracle.OracleOp(sp, "linear", 2.0, 1.0, dvals=sp.dirichlet_values(f))
+    T, iJt, dV = op.prepare(sp.free_values(f))
+    assert dV.sum() == pytest.approx(1.0, abs=1e-13)
+    b = op.residual(sp.free_values(f))
+    assert np.abs(b).max() < 1e-14
+
+
+def test_benchmark_dof_counts():
+    sp = hex_space((20, 10, 10), 2)
+    assert (sp.nfree, sp.nfree + sp.ndir) == (52077, 54243)
+    sp = hex_space((8, 8, 8), 1, dirichlet=dirichlet_closed_x0)
+    assert sp.ndir == 3 * 81
+    sp = hex_space((2, 2, 2), 3, dirichlet=dirichlet_none)
+    assert sp.nfree == 3 * 7 ** 3
