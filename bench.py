@@ -95,8 +95,13 @@ def main():
     op = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=4), sp, jacobian="matrixfree",
                         dirichlet_values=dvals)
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    # a stream of our own (default priority): the library launches on the stream it is given, and the events
+    # below are recorded on the same one (torch's default stream has the null handle, which the library reads
+    # as "the context's stream")
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
+    assert sptr != 0
     xd = torch.from_numpy(x).to(dev)
     vd = torch.from_numpy(v).to(dev)
     yd = torch.empty_like(vd)

@@ -1,9 +1,9 @@
 // Host side of libhfk: the C ABI of include/hfk.h.  Handles own device memory; host arrays are borrowed.
 //
 // Set-up work that the reference does once per operator on the host stays on the host here too: the greedy
-// cell colouring (src/coloring.jl:4-28) and the sparsity pattern with its positions (src/assembly.jl:773-899),
-// both integer algorithms restated exactly, so that colours, colptr/rowval and nzpos are bit-identical to the
-// reference's on the same cell dof ids.
+// cell colouring (src/coloring.jl:4-28), the sparsity pattern with its positions (src/assembly.jl:773-899) and
+// the nodal blocks (src/preconditioners.jl:151-171), all integer algorithms whose results (colours,
+// colptr/rowval, nzpos, block numbering) are bit-identical to the reference's on the same cell dof ids.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,8 +15,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/hfk.h"
-#include "hfk_internal.hpp"
+#include "hfk_op.hpp"
 
 namespace hfk {
 bool supported_qp(int Q, int P);
@@ -24,13 +23,7 @@ bool supported_qp(int Q, int P);
 
 using namespace hfk;
 
-struct hfk_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-};
-
-namespace {
+namespace hfk {
 
 thread_local std::string t_err;
 
@@ -40,35 +33,176 @@ int set_err(hfk_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
-#define HIPCHK(ctx, expr)                                                                          \
-  do {                                                                                             \
-    hipError_t e__ = (expr);                                                                       \
-    if (e__ != hipSuccess)                                                                         \
-      return set_err((ctx), HFK_EHIP, std::string("HyperFEMKernels: HIP error in ") + #expr + ": " + \
-                                           hipGetErrorString(e__));                                \
-  } while (0)
+int op_bind(hfk_op* op) {
+  if (!op || !op->ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
+  return HFK_OK;
+}
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
+// prepare!: tangents at the state x_dev; a device copy of the state is kept (the reference's mf.x,
+// src/matrixfree.jl:140-143), read by the coarse levels of p-multigrid
+int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) {
+  if (storage < 0 || storage > 3) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
+  const size_t bytes = (size_t)op->nbatches * store_entries(storage) * op->C * op->np * store_bytes(storage);
+  HIPCHK(op->ctx, op->tangents.alloc(bytes));
+  HIPCHK(op->ctx, op->state.alloc((size_t)op->nfree * sizeof(double)));
+  if (x_dev != op->state.as<double>())
+    HIPCHK(op->ctx, hipMemcpyAsync(op->state.p, x_dev, (size_t)op->nfree * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(op->ctx, launch_prepare(op->dev(), storage, op->tangents.p, x_dev, BatchRange{0, (int)op->nbatches}, s));
+  op->storage = storage;
+  return HFK_OK;
+}
+
+// y = J dx (accumulate = false) or y += alpha J dx (accumulate = true)
+int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s, double alpha, bool accumulate) {
+  if (op->storage < 0)
+    return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: mul! before prepare! (no tangents stored)");
+  if (!accumulate) HIPCHK(op->ctx, launch_fill(y_dev, op->nfree, 0.0, s));
+  const OpDev d = op->dev();
+  for (const BatchRange& r : op->color_batches)
+    HIPCHK(op->ctx, launch_apply(d, op->storage, op->tangents.p, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
+  return HFK_OK;
+}
+
+// sparsity (src/assembly.jl:773-818, full_keys!/cell_keys! :835-893) and cell_positions! (:864-880).
+// The reference sorts the keys (col << 32) | row of every free pair of every cell and removes duplicates; the
+// result is, per column, the sorted set of rows coupled to it.  Here each column's set is formed directly from
+// the cells that contain the column's dof (same CSC, bit for bit), in parallel over columns.
+int op_build_pattern(hfk_op* op) {
+  if (op->pattern_built) return HFK_OK;
+  const int nd = op->nd;
+  const int64_t ncols = op->nfree, ncells = op->ncells;
+  // cells of every free dof
+  std::vector<int64_t> cstart((size_t)ncols + 1, 0);
+  for (int64_t cell = 0; cell < ncells; ++cell)
+    for (int j = 0; j < nd; ++j) {
+      const int32_t id = op->cell_dofs[(size_t)cell * nd + j];
+      if (id > 0) cstart[id]++;
+    }
+  for (int64_t c = 0; c < ncols; ++c) cstart[c + 1] += cstart[c];
+  std::vector<int64_t> fillp(cstart.begin(), cstart.end() - 1);
+  std::vector<int32_t> cells_of((size_t)cstart[ncols]);
+  for (int64_t cell = 0; cell < ncells; ++cell)
+    for (int j = 0; j < nd; ++j) {
+      const int32_t id = op->cell_dofs[(size_t)cell * nd + j];
+      if (id > 0) cells_of[fillp[id - 1]++] = (int32_t)cell;
+    }
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto rows_of = [&](int64_t c, std::vector<int32_t>& buf) {
+    buf.clear();
+    for (int64_t k = cstart[c]; k < cstart[c + 1]; ++k) {
+      const int32_t* ids = &op->cell_dofs[(size_t)cells_of[k] * nd];
+      for (int ii = 0; ii < nd; ++ii)
+        if (ids[ii] > 0) buf.push_back(ids[ii]);
+    }
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
+  std::vector<int64_t> cnt((size_t)ncols, 0);
+  {
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; ++t)
+      pool.emplace_back([&, t]() {
+        std::vector<int32_t> buf;
+        for (int64_t c = t; c < ncols; c += nth) {
+          rows_of(c, buf);
+          cnt[c] = (int64_t)buf.size();
+        }
+      });
+    for (auto& th : pool) th.join();
   }
-  hipError_t alloc(size_t n) {
-    if (p && bytes >= n) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    if (n == 0) return hipSuccess;
-    hipError_t e = hipMalloc(&p, n);
-    if (e == hipSuccess) bytes = n;
-    return e;
+  op->colptr.assign((size_t)ncols + 1, 1);
+  for (int64_t c = 0; c < ncols; ++c) op->colptr[c + 1] = op->colptr[c] + cnt[c];
+  op->rowval.assign((size_t)(op->colptr[ncols] - 1), 0);
+  {
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; ++t)
+      pool.emplace_back([&, t]() {
+        std::vector<int32_t> buf;
+        for (int64_t c = t; c < ncols; c += nth) {
+          rows_of(c, buf);
+          int64_t* out = &op->rowval[(size_t)(op->colptr[c] - 1)];
+          for (size_t k = 0; k < buf.size(); ++k) out[k] = buf[k];
+        }
+      });
+    for (auto& th : pool) th.join();
   }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(p);
+  op->nzpos.assign((size_t)ncells * nd * nd, 0);
+  {
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; ++t)
+      pool.emplace_back([op, nd, t, nth]() {
+        for (int64_t cell = t; cell < op->ncells; cell += nth) {
+          const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
+          int32_t* pos = &op->nzpos[(size_t)cell * nd * nd];
+          for (int jj = 0; jj < nd; ++jj) {
+            const int32_t c = ids[jj];
+            if (c <= 0) continue;
+            const int64_t lo = op->colptr[c - 1] - 1, hi = op->colptr[c] - 1;
+            for (int ii = 0; ii < nd; ++ii) {
+              const int32_t r = ids[ii];
+              if (r <= 0) continue;
+              const int64_t* it = std::lower_bound(op->rowval.data() + lo, op->rowval.data() + hi, (int64_t)r);
+              pos[ii + nd * jj] = (int32_t)(it - op->rowval.data() + 1);
+            }
+          }
+        }
+      });
+    for (auto& th : pool) th.join();
   }
-};
+  op->pattern_built = true;
+  return HFK_OK;
+}
+
+// field_blocks (src/preconditioners.jl:151-171): a block per node with a free component, first-seen order over
+// (cell, table node); block_of in the batch layout, tensor node order
+int op_build_blocks(hfk_op* op) {
+  NodeBlocks& nb = op->blocks;
+  if (nb.built) return HFK_OK;
+  const int nn = op->nn, nd = op->nd;
+  std::vector<int32_t> block_of_dof((size_t)op->nfree, -1);
+  std::vector<int32_t> blk_cell((size_t)op->ncells * nn, -1);
+  nb.dofs_h.clear();
+  int64_t count = 0;
+  for (int64_t cell = 0; cell < op->ncells; ++cell)
+    for (int n = 0; n < nn; ++n) {
+      int32_t key[3];
+      int32_t first = 0;
+      for (int c = 0; c < 3; ++c) {
+        key[c] = op->cell_dofs[(size_t)cell * nd + op->ldof[c * nn + n]];
+        if (key[c] > 0 && first == 0) first = key[c];
+      }
+      if (first == 0) continue;  // every component fixed: no block
+      int32_t blk = block_of_dof[first - 1];
+      if (blk < 0) {
+        blk = (int32_t)count++;
+        for (int c = 0; c < 3; ++c) {
+          nb.dofs_h.push_back(key[c] > 0 ? key[c] : 0);
+          if (key[c] > 0) block_of_dof[key[c] - 1] = blk;
+        }
+      }
+      blk_cell[(size_t)cell * nn + n] = blk;
+    }
+  nb.nblocks = count;
+  std::vector<int32_t> h((size_t)op->nslots() * nn, -1);
+  for (int64_t s = 0; s < op->nslots(); ++s) {
+    const int64_t cell = op->batch_cell[s];
+    if (cell < 0) continue;
+    for (int t = 0; t < nn; ++t) h[(size_t)s * nn + t] = blk_cell[(size_t)cell * nn + op->nodeperm[t]];
+  }
+  HIPCHK(op->ctx, nb.block_of.alloc(h.size() * sizeof(int32_t)));
+  HIPCHK(op->ctx, hipMemcpy(nb.block_of.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, nb.dofs.alloc(std::max<size_t>(nb.dofs_h.size(), 1) * sizeof(int32_t)));
+  if (!nb.dofs_h.empty())
+    HIPCHK(op->ctx, hipMemcpy(nb.dofs.p, nb.dofs_h.data(), nb.dofs_h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, nb.B.alloc(std::max<int64_t>(count, 1) * 9 * sizeof(double)));
+  HIPCHK(op->ctx, nb.Binv.alloc(std::max<int64_t>(count, 1) * 9 * sizeof(double)));
+  HIPCHK(op->ctx, nb.bad.alloc(sizeof(int)));
+  nb.built = true;
+  return HFK_OK;
+}
+
+namespace {
 
 // 1D Lagrange basis on Q equispaced nodes of [0,1] (midpoint if Q = 1) at xi (lagrange1d, src/tensorproduct.jl:24-47)
 void lagrange1d(int Q, double xi, double* N, double* dN) {
@@ -112,55 +246,6 @@ void gauss01(int P, double* x, double* w) {
   }
 }
 
-}  // namespace
-
-struct hfk_op {
-  hfk_ctx* ctx = nullptr;
-  int Q = 0, P = 0, nn = 0, nd = 0, np = 0, C = 0;
-  int64_t ncells = 0, nfree = 0, ndir = 0, nbatches = 0;
-  int law = 0;
-  double lam = 0, mu = 0;
-  Tab tab{};
-  std::vector<int32_t> cell_dofs;           // Gridap order, ncells*nd
-  std::vector<int32_t> ldof, nodeperm;      // as passed
-  std::vector<int32_t> color;               // 1-based colour of each cell
-  std::vector<BatchRange> color_batches;    // batches of each colour
-  std::vector<int64_t> batch_cell;          // nbatches*C, -1 padding
-  std::vector<int64_t> slot_of_cell;        // cell -> batch*C + position
-  std::vector<int32_t> loc_of;              // tensor index c*nn + t -> table-local dof
-  DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64;
-  bool has_mu_field = false;
-  int storage = -1;  // storage of the prepared tangents, -1: not prepared
-  // pattern (built on first use, as build_pattern!)
-  bool pattern_built = false;
-  std::vector<int64_t> colptr, rowval;
-  std::vector<int32_t> nzpos;  // Gridap cell order [cell][ii + nd*jj]
-
-  OpDev dev() const {
-    OpDev d;
-    d.Q = Q;
-    d.P = P;
-    d.C = C;
-    d.nn = nn;
-    d.np = np;
-    d.nd = nd;
-    d.law = law;
-    d.lam = lam;
-    d.mu = mu;
-    d.tab = tab;
-    d.ids = ids.as<int32_t>();
-    d.coords = coords.as<double>();
-    d.mu_pts = has_mu_field ? mu_pts.as<double>() : nullptr;
-    d.dvals = dvals.as<double>();
-    d.loc_of = loc_of_d.as<int32_t>();
-    d.nzpos = nzpos_d.as<int32_t>();
-    return d;
-  }
-  hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
-};
-
-namespace {
-
 // src/coloring.jl:4-28: greedy first fit over the free (test) dofs with 128-bit masks; colour = trailing_ones
 int color_cells(hfk_op* op) {
   std::vector<unsigned __int128> masks((size_t)op->nfree, 0);
@@ -186,28 +271,14 @@ int sync(hfk_op* op) {
   return HFK_OK;
 }
 
-int bind(hfk_op* op) {
-  if (!op || !op->ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
-  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
-  return HFK_OK;
-}
+int bind(hfk_op* op) { return op_bind(op); }
 
 int run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) {
-  if (storage < 0 || storage > 3) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
-  const size_t bytes = (size_t)op->nbatches * store_entries(storage) * op->C * op->np * store_bytes(storage);
-  HIPCHK(op->ctx, op->tangents.alloc(bytes));
-  HIPCHK(op->ctx, launch_prepare(op->dev(), storage, op->tangents.p, x_dev, BatchRange{0, (int)op->nbatches}, s));
-  op->storage = storage;
-  return HFK_OK;
+  return op_run_prepare(op, x_dev, storage, s);
 }
 
 int run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s) {
-  if (op->storage < 0)
-    return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: mul! before prepare! (no tangents stored)");
-  HIPCHK(op->ctx, launch_fill(y_dev, op->nfree, 0.0, s));
-  const OpDev d = op->dev();
-  for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_apply(d, op->storage, op->tangents.p, dx_dev, y_dev, r, s));
-  return HFK_OK;
+  return op_run_apply(op, dx_dev, y_dev, s, 1.0, false);
 }
 
 int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) {
@@ -223,60 +294,7 @@ int stage_in(hfk_op* op, DevBuf& buf, const double* h, int64_t n) {
   return HFK_OK;
 }
 
-// sparsity (src/assembly.jl:773-818) with full_keys! (:835-842, :883-893) and cell_positions! (:864-880)
-int build_pattern(hfk_op* op) {
-  if (op->pattern_built) return HFK_OK;
-  const int nd = op->nd;
-  std::vector<uint64_t> keys;
-  keys.reserve((size_t)op->ncells * nd * nd);
-  for (int64_t cell = 0; cell < op->ncells; ++cell) {
-    const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
-    for (int jj = 0; jj < nd; ++jj) {
-      const int32_t c = ids[jj];
-      if (c <= 0) continue;
-      for (int ii = 0; ii < nd; ++ii) {
-        const int32_t r = ids[ii];
-        if (r > 0) keys.push_back(((uint64_t)c << 32) | (uint64_t)r);
-      }
-    }
-  }
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  const int64_t ncols = op->nfree;
-  op->rowval.resize(keys.size());
-  std::vector<int64_t> cnt((size_t)ncols, 0);
-  for (size_t k = 0; k < keys.size(); ++k) {
-    cnt[(keys[k] >> 32) - 1]++;
-    op->rowval[k] = (int64_t)(keys[k] & 0xffffffffu);
-  }
-  op->colptr.resize((size_t)ncols + 1);
-  op->colptr[0] = 1;
-  for (int64_t c = 0; c < ncols; ++c) op->colptr[c + 1] = op->colptr[c] + cnt[c];
-  op->nzpos.assign((size_t)op->ncells * nd * nd, 0);
-  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<std::thread> pool;
-  for (unsigned t = 0; t < nth; ++t)
-    pool.emplace_back([op, nd, t, nth]() {
-      for (int64_t cell = t; cell < op->ncells; cell += nth) {
-        const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
-        int32_t* pos = &op->nzpos[(size_t)cell * nd * nd];
-        for (int jj = 0; jj < nd; ++jj) {
-          const int32_t c = ids[jj];
-          if (c <= 0) continue;
-          const int64_t lo = op->colptr[c - 1] - 1, hi = op->colptr[c] - 1;
-          for (int ii = 0; ii < nd; ++ii) {
-            const int32_t r = ids[ii];
-            if (r <= 0) continue;
-            const int64_t* it = std::lower_bound(op->rowval.data() + lo, op->rowval.data() + hi, (int64_t)r);
-            pos[ii + nd * jj] = (int32_t)(it - op->rowval.data() + 1);
-          }
-        }
-      }
-    });
-  for (auto& th : pool) th.join();
-  op->pattern_built = true;
-  return HFK_OK;
-}
+int build_pattern(hfk_op* op) { return op_build_pattern(op); }
 
 int upload_nzpos(hfk_op* op) {
   if (op->nzpos_d.p) return HFK_OK;
@@ -289,11 +307,14 @@ int upload_nzpos(hfk_op* op) {
   return HFK_OK;
 }
 
-int run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s) {
+}  // namespace
+
+// jacobian!: tangents in FP64 full storage into a buffer of their own (the matrix-free tangents are left
+// untouched), zeroed nonzeros, element matrices summed colour by colour
+int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s) {
   int rc = build_pattern(op);
   if (rc) return rc;
   if ((rc = upload_nzpos(op))) return rc;
-  // tangents in FP64 full storage into a buffer of their own (the matrix-free tangents are left untouched)
   const size_t bytes = (size_t)op->nbatches * 81 * op->C * op->np * sizeof(double);
   HIPCHK(op->ctx, op->D64.alloc(bytes));
   const OpDev d = op->dev();
@@ -303,8 +324,13 @@ int run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s)
   return HFK_OK;
 }
 
-}  // namespace
+}  // namespace hfk
 
+namespace {
+int run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s) {
+  return op_run_jacobian(op, x_dev, nz_dev, s);
+}
+}  // namespace
 extern "C" {
 
 int hfk_version(void) { return 1; }

@@ -43,8 +43,12 @@ struct OpDev {
 
 // Launchers (hfk_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_prepare(const OpDev& op, int storage, void* tangents, const double* x, BatchRange r, hipStream_t s);
+// y[id] += alpha * (J dx)[id] over the batches r (the caller zeroes y first for a plain product)
 hipError_t launch_apply(const OpDev& op, int storage, const void* tangents, const double* dx, double* y, BatchRange r,
-                        hipStream_t s);
+                        hipStream_t s, double alpha = 1.0);
+// nodal 3x3 blocks of the tangent, accumulated into B[blk*9 + ci + 3cj] over the batches r of one colour
+hipError_t launch_blocks(const OpDev& op, int storage, const void* tangents, const int32_t* block_of, double* B,
+                         BatchRange r, hipStream_t s);
 hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRange r, hipStream_t s);
 hipError_t launch_assemble(const OpDev& op, const double* tangents64, double* nzval, BatchRange r, hipStream_t s);
 hipError_t launch_fill(double* y, int64_t n, double v, hipStream_t s);

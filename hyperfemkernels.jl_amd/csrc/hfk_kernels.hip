@@ -430,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_prepare(OpDev op, void* __restrict
 template <int Q, int P, class TS, bool SYM>
 __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __restrict__ tangents,
                                                     const double* __restrict__ dx, double* __restrict__ y,
-                                                    int batch0) {
+                                                    int batch0, double alpha) {
   using Dm = Dims<Q, P>;
   constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
   __shared__ double lds[Dm::LDS];
@@ -476,8 +476,66 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
   }
   contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t, double r) {
     const int id = ids[cell * 3 * Dm::NN + c * Dm::NN + t];
-    if (id > 0) y[id - 1] += r;  // race-free: the cells of one colour share no free dof
+    if (id > 0) y[id - 1] += alpha * r;  // race-free: the cells of one colour share no free dof
   });
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// BlockJacobi from the stored tangents (_blocks_from_tangents!, src/preconditioners.jl:234-263), the cells of
+// one colour: B_n[ci,cj] += sum_q sum_K sum_M dN_n/dxi_K D_q[(K,ci),(M,cj)] dN_n/dxi_M.  One thread per
+// (cell, node); the tangents of the current point of the batch's cells are staged in LDS.
+template <int Q, int P, class TS, bool SYM>
+__global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __restrict__ tangents,
+                                                     const int32_t* __restrict__ block_of, double* __restrict__ Bk,
+                                                     int batch0) {
+  using Dm = Dims<Q, P>;
+  constexpr int C = Dm::C, NN = Dm::NN, NP = Dm::NP, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
+  __shared__ double Dl[C * 81];
+  const int tid = threadIdx.x;
+  const int batch = batch0 + blockIdx.x;
+  const bool active = tid < C * NN;
+  const int cell = tid / NN, t = tid - cell * NN;
+  const int i = t % Q, j = (t / Q) % Q, k = t / (Q * Q);
+  const TS* Dg = static_cast<const TS*>(tangents) + (size_t)batch * NE * NSLOT;
+  double acc[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = 0.0;
+  for (int q = 0; q < NP; ++q) {
+    __syncthreads();
+    for (int it = tid; it < C * 81; it += kThreads) {
+      const int c2 = it / 81, e = it - c2 * 81, ii = e % 9, jj = e / 9;
+      const int se = SYM ? (ii <= jj ? sym_index(ii, jj) : sym_index(jj, ii)) : e;
+      Dl[it] = static_cast<double>(Dg[(size_t)se * NSLOT + c2 * NP + q]);
+    }
+    __syncthreads();
+    if (active) {
+      const int a = q % P, b = (q / P) % P, cc = q / (P * P);
+      const double na = op.tab.N[a * Q + i], nb = op.tab.N[b * Q + j], nc = op.tab.N[cc * Q + k];
+      const double G[3] = {op.tab.dN[a * Q + i] * nb * nc, na * op.tab.dN[b * Q + j] * nc, na * nb * op.tab.dN[cc * Q + k]};
+      const double* D = Dl + cell * 81;
+#pragma unroll
+      for (int cj = 0; cj < 3; ++cj)
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) {
+          double s = 0;
+#pragma unroll
+          for (int M = 0; M < 3; ++M) {
+            double u = 0;
+#pragma unroll
+            for (int K = 0; K < 3; ++K) u += G[K] * D[(K + 3 * ci) + 9 * (M + 3 * cj)];
+            s += u * G[M];
+          }
+          acc[ci + 3 * cj] += s;
+        }
+    }
+  }
+  if (active) {
+    const int blk = block_of[((size_t)batch * C + cell) * NN + t];
+    if (blk >= 0) {
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Bk[(size_t)blk * 9 + e] += acc[e];  // race-free within a colour
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -655,13 +713,26 @@ hipError_t prepare_law(const OpDev& op, int storage, void* T, const double* x, B
 
 template <int Q, int P>
 hipError_t apply_qp(const OpDev& op, int storage, const void* T, const double* dx, double* y, BatchRange r,
-                    hipStream_t s) {
+                    hipStream_t s, double alpha) {
   const dim3 g(r.count), b(kThreads);
   switch (storage) {
-    case 0: hipLaunchKernelGGL((k_apply<Q, P, double, false>), g, b, 0, s, op, T, dx, y, r.first); break;
-    case 1: hipLaunchKernelGGL((k_apply<Q, P, double, true>), g, b, 0, s, op, T, dx, y, r.first); break;
-    case 2: hipLaunchKernelGGL((k_apply<Q, P, float, false>), g, b, 0, s, op, T, dx, y, r.first); break;
-    default: hipLaunchKernelGGL((k_apply<Q, P, float, true>), g, b, 0, s, op, T, dx, y, r.first); break;
+    case 0: hipLaunchKernelGGL((k_apply<Q, P, double, false>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
+    case 1: hipLaunchKernelGGL((k_apply<Q, P, double, true>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
+    case 2: hipLaunchKernelGGL((k_apply<Q, P, float, false>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
+    default: hipLaunchKernelGGL((k_apply<Q, P, float, true>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
+  }
+  return hipGetLastError();
+}
+
+template <int Q, int P>
+hipError_t blocks_qp(const OpDev& op, int storage, const void* T, const int32_t* block_of, double* B, BatchRange r,
+                     hipStream_t s) {
+  const dim3 g(r.count), b(kThreads);
+  switch (storage) {
+    case 0: hipLaunchKernelGGL((k_blocks<Q, P, double, false>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 1: hipLaunchKernelGGL((k_blocks<Q, P, double, true>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 2: hipLaunchKernelGGL((k_blocks<Q, P, float, false>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    default: hipLaunchKernelGGL((k_blocks<Q, P, float, true>), g, b, 0, s, op, T, block_of, B, r.first); break;
   }
   return hipGetLastError();
 }
@@ -713,9 +784,17 @@ hipError_t launch_prepare(const OpDev& op, int storage, void* T, const double* x
 }
 
 hipError_t launch_apply(const OpDev& op, int storage, const void* T, const double* dx, double* y, BatchRange r,
-                        hipStream_t s) {
+                        hipStream_t s, double alpha) {
   if (r.count <= 0) return hipSuccess;
-#define CALL(q, p) apply_qp<q, p>(op, storage, T, dx, y, r, s)
+#define CALL(q, p) apply_qp<q, p>(op, storage, T, dx, y, r, s, alpha)
+  HFK_DISPATCH_QP(op.Q, op.P, CALL);
+#undef CALL
+}
+
+hipError_t launch_blocks(const OpDev& op, int storage, const void* T, const int32_t* block_of, double* B, BatchRange r,
+                         hipStream_t s) {
+  if (r.count <= 0) return hipSuccess;
+#define CALL(q, p) blocks_qp<q, p>(op, storage, T, block_of, B, r, s)
   HFK_DISPATCH_QP(op.Q, op.P, CALL);
 #undef CALL
 }

@@ -1,0 +1,123 @@
+// Host-side handles of libhfk (context, operator) shared by the C-ABI sources.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hfk.h"
+#include "hfk_internal.hpp"
+
+struct hfk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace hfk {
+
+int set_err(hfk_ctx* ctx, int code, const std::string& msg);
+
+#define HIPCHK(ctx, expr)                                                                          \
+  do {                                                                                             \
+    hipError_t e__ = (expr);                                                                       \
+    if (e__ != hipSuccess)                                                                         \
+      return ::hfk::set_err((ctx), HFK_EHIP, std::string("HyperFEMKernels: HIP error in ") + #expr + ": " + \
+                                                 hipGetErrorString(e__));                          \
+  } while (0)
+
+// A device allocation owned by a handle.  alloc() keeps the buffer when it is already large enough.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) {
+    if (p && bytes >= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Nodal blocks of one operator (FieldBlocks, src/preconditioners.jl:120-171): one block per node with a free
+// component, numbered in first-seen order over (cell, table node).
+struct NodeBlocks {
+  bool built = false;
+  int64_t nblocks = 0;
+  std::vector<int32_t> dofs_h;  // 3*nblocks, free ids (1-based), 0 = fixed component
+  DevBuf block_of;              // [slot][t] tensor order, -1: every component fixed
+  DevBuf dofs, B, Binv;
+  DevBuf bad;                   // int counter of non-finite inverted blocks
+};
+
+}  // namespace hfk
+
+struct hfk_op {
+  hfk_ctx* ctx = nullptr;
+  int Q = 0, P = 0, nn = 0, nd = 0, np = 0, C = 0;
+  int64_t ncells = 0, nfree = 0, ndir = 0, nbatches = 0;
+  int law = 0;
+  double lam = 0, mu = 0;
+  hfk::Tab tab{};
+  std::vector<int32_t> cell_dofs;               // Gridap order, ncells*nd
+  std::vector<int32_t> ldof, nodeperm;          // as passed
+  std::vector<int32_t> color;                   // 1-based colour of each cell
+  std::vector<hfk::BatchRange> color_batches;   // batches of each colour
+  std::vector<int64_t> batch_cell;              // nbatches*C, -1 padding
+  std::vector<int64_t> slot_of_cell;            // cell -> batch*C + position
+  std::vector<int32_t> loc_of;                  // tensor index c*nn + t -> table-local dof
+  hfk::DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64, state;
+  bool has_mu_field = false;
+  int storage = -1;  // storage of the prepared tangents, -1: not prepared
+  bool pattern_built = false;
+  std::vector<int64_t> colptr, rowval;
+  std::vector<int32_t> nzpos;  // Gridap cell order [cell][ii + nd*jj]
+  hfk::NodeBlocks blocks;
+
+  hfk::OpDev dev() const {
+    hfk::OpDev d;
+    d.Q = Q;
+    d.P = P;
+    d.C = C;
+    d.nn = nn;
+    d.np = np;
+    d.nd = nd;
+    d.law = law;
+    d.lam = lam;
+    d.mu = mu;
+    d.tab = tab;
+    d.ids = ids.as<int32_t>();
+    d.coords = coords.as<double>();
+    d.mu_pts = has_mu_field ? mu_pts.as<double>() : nullptr;
+    d.dvals = dvals.as<double>();
+    d.loc_of = loc_of_d.as<int32_t>();
+    d.nzpos = nzpos_d.as<int32_t>();
+    return d;
+  }
+  hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
+  int64_t nslots() const { return nbatches * C; }
+};
+
+namespace hfk {
+// shared helpers of the C-ABI sources (hfk_capi.hip)
+int op_bind(hfk_op* op);
+int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s);
+int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s, double alpha, bool accumulate);
+int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s);
+int op_build_pattern(hfk_op* op);
+int op_build_blocks(hfk_op* op);
+}  // namespace hfk

@@ -1,0 +1,1002 @@
+// Host orchestration of the solver layer (C ABI): BlockJacobi, the Chebyshev smoother with its Arnoldi
+// interval, the p-multigrid transfers, the V-cycle with an assembled coarse level, and CG / flexible CG.
+// Every vector lives on the device; the host only reads back the scalars a decision needs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "hfk_op.hpp"
+#include "hfk_solver.hpp"
+
+using namespace hfk;
+
+namespace {
+
+int vec_alloc(hfk_ctx* ctx, DevBuf& b, int64_t n) {
+  HIPCHK(ctx, b.alloc((size_t)std::max<int64_t>(n, 1) * sizeof(double)));
+  return HFK_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Eigenvalues (real parts) of a small upper Hessenberg matrix H (n x n, column-major h[i + n j]): shifted QR
+// iterations with Givens rotations on the active window, Wilkinson shift, deflation of 1x1 and 2x2 blocks
+// (a 2x2 block contributes the real part of its eigenvalue pair).  n <= 64.
+void hessenberg_eig_real(std::vector<double> h, int n, std::vector<double>& out) {
+  out.clear();
+  auto H = [&](int i, int j) -> double& { return h[i + n * j]; };
+  int hi = n - 1, its = 0;
+  while (hi >= 0) {
+    if (hi == 0) {
+      out.push_back(H(0, 0));
+      hi = -1;
+      break;
+    }
+    int l = hi;
+    while (l > 0) {
+      const double s = std::fabs(H(l - 1, l - 1)) + std::fabs(H(l, l));
+      if (std::fabs(H(l, l - 1)) <= 1e-15 * (s == 0 ? 1.0 : s)) {
+        H(l, l - 1) = 0.0;
+        break;
+      }
+      --l;
+    }
+    if (l == hi) {
+      out.push_back(H(hi, hi));
+      --hi;
+      its = 0;
+      continue;
+    }
+    if (l == hi - 1 || its > 200) {
+      const double a = H(hi - 1, hi - 1), b = H(hi - 1, hi), c = H(hi, hi - 1), d = H(hi, hi);
+      const double tr = 0.5 * (a + d), disc = 0.25 * (a - d) * (a - d) + b * c;
+      if (disc >= 0) {
+        out.push_back(tr + std::sqrt(disc));
+        out.push_back(tr - std::sqrt(disc));
+      } else {
+        out.push_back(tr);
+        out.push_back(tr);
+      }
+      hi -= 2;
+      its = 0;
+      continue;
+    }
+    // Wilkinson shift from the trailing 2x2 of the window (its real part when the pair is complex)
+    const double a = H(hi - 1, hi - 1), b = H(hi - 1, hi), c = H(hi, hi - 1), d = H(hi, hi);
+    const double tr = 0.5 * (a + d), disc = 0.25 * (a - d) * (a - d) + b * c;
+    double mu = tr;
+    if (disc >= 0) {
+      const double e1 = tr + std::sqrt(disc), e2 = tr - std::sqrt(disc);
+      mu = std::fabs(e1 - d) < std::fabs(e2 - d) ? e1 : e2;
+    }
+    if (its == 20 || its == 60) mu += 0.75 * std::fabs(H(hi, hi - 1));  // exceptional shift
+    // QR step on the window [l, hi]: H - mu I = QR, H <- RQ + mu I (rotations applied to the window only)
+    for (int i = l; i <= hi; ++i) H(i, i) -= mu;
+    std::vector<double> cs(hi - l), sn(hi - l);
+    for (int k = l; k < hi; ++k) {
+      const double x = H(k, k), y = H(k + 1, k);
+      const double r = std::hypot(x, y);
+      const double cc = r == 0 ? 1.0 : x / r, ss = r == 0 ? 0.0 : y / r;
+      cs[k - l] = cc;
+      sn[k - l] = ss;
+      for (int j = k; j <= hi; ++j) {
+        const double t1 = H(k, j), t2 = H(k + 1, j);
+        H(k, j) = cc * t1 + ss * t2;
+        H(k + 1, j) = -ss * t1 + cc * t2;
+      }
+    }
+    for (int k = l; k < hi; ++k) {
+      const double cc = cs[k - l], ss = sn[k - l];
+      for (int i = l; i <= std::min(k + 2, hi); ++i) {
+        const double t1 = H(i, k), t2 = H(i, k + 1);
+        H(i, k) = cc * t1 + ss * t2;
+        H(i, k + 1) = -ss * t1 + cc * t2;
+      }
+    }
+    for (int i = l; i <= hi; ++i) H(i, i) += mu;
+    ++its;
+  }
+}
+
+// chebyshev_start (src/smoothers.jl:68-71): v_i = sin(1 + 0.7 i) + 0.3 cos(2.3 i), i = 1..n, normalised
+std::vector<double> chebyshev_start(int64_t n) {
+  std::vector<double> v((size_t)n);
+  double s = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double k = (double)(i + 1);
+    v[i] = std::sin(1.0 + 0.7 * k) + 0.3 * std::cos(2.3 * k);
+    s += v[i] * v[i];
+  }
+  s = std::sqrt(s);
+  for (auto& x : v) x /= s;
+  return v;
+}
+
+struct Dots {
+  DevBuf part, out;
+  int init(hfk_ctx* ctx) {
+    HIPCHK(ctx, part.alloc(dot_partial_doubles() * sizeof(double)));
+    HIPCHK(ctx, out.alloc(4 * sizeof(double)));
+    return HFK_OK;
+  }
+  // blocking: computes up to 3 dots and reads them back
+  int run(hfk_ctx* ctx, hipStream_t s, int64_t n, const double* a0, const double* b0, double* res,
+          const double* a1 = nullptr, const double* b1 = nullptr, const double* a2 = nullptr, const double* b2 = nullptr) {
+    HIPCHK(ctx, vec_dots(n, a0, b0, a1, b1, a2, b2, part.as<double>(), out.as<double>(), s));
+    const int nd = a2 ? 3 : (a1 ? 2 : 1);
+    HIPCHK(ctx, hipMemcpyAsync(res, out.p, nd * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return HFK_OK;
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// BlockJacobi on the stored tangents of op (block_diagonal!, src/preconditioners.jl:186-218)
+int blocks_update(hfk_op* op, hipStream_t s, bool* ok) {
+  int rc = op_build_blocks(op);
+  if (rc) return rc;
+  if (op->storage < 0) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: BlockJacobi before prepare!");
+  NodeBlocks& nb = op->blocks;
+  HIPCHK(op->ctx, launch_fill(nb.B.as<double>(), nb.nblocks * 9, 0.0, s));
+  const OpDev d = op->dev();
+  for (const BatchRange& r : op->color_batches)
+    HIPCHK(op->ctx, launch_blocks(d, op->storage, op->tangents.p, nb.block_of.as<int32_t>(), nb.B.as<double>(), r, s));
+  HIPCHK(op->ctx, hipMemsetAsync(nb.bad.p, 0, sizeof(int), s));
+  HIPCHK(op->ctx, blocks_invert(nb.nblocks, nb.dofs.as<int32_t>(), nb.B.as<double>(), nb.Binv.as<double>(), nb.bad.as<int>(), s));
+  int bad = 0;
+  HIPCHK(op->ctx, hipMemcpyAsync(&bad, nb.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  *ok = bad == 0;
+  return HFK_OK;
+}
+
+int blocks_apply(hfk_op* op, const double* r, double* y, hipStream_t s) {
+  NodeBlocks& nb = op->blocks;
+  if (!nb.built) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: BlockJacobi applied before its update");
+  HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, y, nullptr, 0.0, 1.0, s));
+  return HFK_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+struct hfk_cheb {
+  hfk_op* op = nullptr;
+  int degree = 2, eig_iters = 10;
+  double range = 20, inflate = 1.1;
+  double lmin = 0, lmax = 0, a = 0, b = 0;
+  bool ok = false, blocks_ok = false;
+  int nbuild = 0;
+  DevBuf r, d, tmp, start, V;
+  Dots dots;
+};
+
+namespace {
+
+// eigenvalue_bounds (src/preconditioners.jl:374-389): k Arnoldi steps (modified Gram-Schmidt) on J M^-1 from
+// `start`; extreme real parts of the Ritz values of the (truncated at a near breakdown) Hessenberg matrix
+int arnoldi_bounds(hfk_op* op, hfk_cheb* ch, int k, hipStream_t s, double* lmin, double* lmax) {
+  const int64_t n = op->nfree;
+  k = (int)std::min<int64_t>(k, n);
+  HIPCHK(op->ctx, ch->V.alloc((size_t)(k + 1) * n * sizeof(double)));
+  double* V = ch->V.as<double>();
+  HIPCHK(op->ctx, vec_copy(n, ch->start.as<double>(), V, s));
+  std::vector<double> Hm((size_t)(k + 1) * k, 0.0);  // (k+1) x k column-major
+  int rc;
+  int steps = k;
+  for (int j = 0; j < k; ++j) {
+    double* vj = V + (size_t)j * n;
+    double* w = V + (size_t)(j + 1) * n;
+    if ((rc = blocks_apply(op, vj, ch->tmp.as<double>(), s))) return rc;
+    if ((rc = op_run_apply(op, ch->tmp.as<double>(), w, s, 1.0, false))) return rc;
+    for (int i = 0; i <= j; ++i) {
+      double hij;
+      if ((rc = ch->dots.run(op->ctx, s, n, V + (size_t)i * n, w, &hij))) return rc;
+      Hm[i + (k + 1) * j] = hij;
+      HIPCHK(op->ctx, vec_axpby(n, -hij, V + (size_t)i * n, 1.0, w, s));
+    }
+    double ww;
+    if ((rc = ch->dots.run(op->ctx, s, n, w, w, &ww))) return rc;
+    const double hn = std::sqrt(ww);
+    Hm[j + 1 + (k + 1) * j] = hn;
+    if (!(hn > 0) || !std::isfinite(hn)) {
+      steps = j + 1;
+      break;
+    }
+    HIPCHK(op->ctx, vec_axpby(n, 0.0, w, 1.0 / hn, w, s));
+  }
+  // truncate at the first (near) breakdown, as the reference
+  int jj = steps;
+  for (int i = 1; i < steps; ++i) {
+    double mx = 1.0;
+    for (int c = 0; c < i; ++c)
+      for (int r = 0; r < i; ++r) mx = std::max(mx, std::fabs(Hm[r + (k + 1) * c]));
+    if (std::fabs(Hm[i + (k + 1) * (i - 1)]) <= 1e-8 * mx) {
+      jj = i;
+      break;
+    }
+  }
+  std::vector<double> h((size_t)jj * jj);
+  bool fin = true;
+  for (int c = 0; c < jj; ++c)
+    for (int r = 0; r < jj; ++r) {
+      h[r + jj * c] = Hm[r + (k + 1) * c];
+      fin = fin && std::isfinite(h[r + jj * c]);
+    }
+  if (!fin) {
+    *lmin = *lmax = NAN;
+    return HFK_OK;
+  }
+  std::vector<double> ev;
+  hessenberg_eig_real(h, jj, ev);
+  *lmin = *std::min_element(ev.begin(), ev.end());
+  *lmax = *std::max_element(ev.begin(), ev.end());
+  return HFK_OK;
+}
+
+int cheb_update(hfk_cheb* ch, hipStream_t s) {
+  hfk_op* op = ch->op;
+  bool bok = false;
+  int rc = blocks_update(op, s, &bok);
+  if (rc) return rc;
+  ch->blocks_ok = bok;
+  if (!bok) {
+    ch->ok = false;
+    return HFK_OK;
+  }
+  double lmin, lmax;
+  if ((rc = arnoldi_bounds(op, ch, ch->eig_iters, s, &lmin, &lmax))) return rc;
+  ch->lmin = lmin;
+  ch->lmax = lmax;
+  // smoother_interval! (src/smoothers.jl:92-107): usable only for a finite positive spectrum estimate
+  ch->ok = std::isfinite(lmax) && lmax > 0 && lmin > 0;
+  if (!ch->ok) return HFK_OK;
+  ch->b = ch->inflate * lmax;
+  ch->a = ch->b / ch->range;
+  ch->nbuild++;
+  return HFK_OK;
+}
+
+// smooth! (src/smoothers.jl:116-146), the block application fused with the direction update and x += d
+int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, double* residual, hipStream_t s) {
+  hfk_op* op = ch->op;
+  if (!ch->ok)
+    return set_err(op->ctx, HFK_ESTATE,
+                   "HyperFEMKernels: Chebyshev: the smoother has no usable interval (non-finite or indefinite tangent)");
+  const int64_t n = op->nfree;
+  double* r = ch->r.as<double>();
+  double* d = ch->d.as<double>();
+  const NodeBlocks& nb = op->blocks;
+  const double th = 0.5 * (ch->b + ch->a), de = 0.5 * (ch->b - ch->a), sg = th / de;
+  int rc;
+  HIPCHK(op->ctx, vec_copy(n, rhs, r, s));
+  if (zero_guess) {
+    HIPCHK(op->ctx, launch_fill(x, n, 0.0, s));
+  } else {
+    if ((rc = op_run_apply(op, x, r, s, -1.0, true))) return rc;  // r = rhs - A x
+  }
+  // z = M^-1 r; d = z / theta; x += d
+  HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, 0.0, 1.0 / th, s));
+  double rho = 1.0 / sg;
+  const int k = ch->degree;
+  for (int i = 1; i <= k; ++i) {
+    if (i == k && !residual) break;
+    if ((rc = op_run_apply(op, d, r, s, -1.0, true))) return rc;  // r -= A d
+    if (i == k) break;
+    const double rn = 1.0 / (2.0 * sg - rho);
+    HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, rn * rho, 2.0 * rn / de, s));
+    rho = rn;
+  }
+  if (residual) HIPCHK(op->ctx, vec_copy(n, r, residual, s));
+  return HFK_OK;
+}
+
+int cheb_init(hfk_op* op, int degree, double range, double inflate, int eig_iters, hfk_cheb* ch) {
+  ch->op = op;
+  ch->degree = degree;
+  ch->range = range;
+  ch->inflate = inflate;
+  ch->eig_iters = eig_iters;
+  const int64_t n = op->nfree;
+  int rc;
+  if ((rc = vec_alloc(op->ctx, ch->r, n)) || (rc = vec_alloc(op->ctx, ch->d, n)) || (rc = vec_alloc(op->ctx, ch->tmp, n)) ||
+      (rc = vec_alloc(op->ctx, ch->start, n)) || (rc = ch->dots.init(op->ctx)))
+    return rc;
+  std::vector<double> v = chebyshev_start(n);
+  HIPCHK(op->ctx, hipMemcpy(ch->start.p, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, launch_fill(ch->d.as<double>(), n, 0.0, op->ctx->stream));
+  return HFK_OK;
+}
+
+int check_cheb_args(hfk_ctx* ctx, int degree, double range, double inflate, int eig_iters) {
+  if (degree < 1) return set_err(ctx, HFK_EARG, "HyperFEMKernels: Chebyshev: degree must be >= 1");
+  if (!(range > 1)) return set_err(ctx, HFK_EARG, "HyperFEMKernels: Chebyshev: range must be > 1");
+  if (!(inflate >= 1)) return set_err(ctx, HFK_EARG, "HyperFEMKernels: Chebyshev: inflate must be >= 1");
+  if (eig_iters < 2) return set_err(ctx, HFK_EARG, "HyperFEMKernels: Chebyshev: eig_iters must be >= 2");
+  return HFK_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+// Prolongation between nested Q_k spaces on one mesh (src/transfers.jl:16-154)
+struct hfk_transfer {
+  hfk_op* coarse = nullptr;
+  hfk_op* fine = nullptr;
+  int NF = 0, NC = 0;
+  DevBuf E, Einj, cid, invmult;
+};
+
+namespace {
+
+int transfer_build(hfk_op* coarse, hfk_op* fine, hfk_transfer* T) {
+  hfk_ctx* ctx = fine->ctx;
+  if (coarse->ncells != fine->ncells) return set_err(ctx, HFK_EDIM, "HyperFEMKernels: the two levels live on different meshes");
+  if (coarse->Q >= fine->Q) return set_err(ctx, HFK_EARG, "HyperFEMKernels: the coarse space has more nodes per cell than the fine one");
+  const int Qf = fine->Q, Qc = coarse->Q, NF = fine->nn, NC = coarse->nn;
+  T->coarse = coarse;
+  T->fine = fine;
+  T->NF = NF;
+  T->NC = NC;
+  // E[tf, tc] = coarse shape function tc at fine node tf; Einj[tc, tf] = fine shape function tf at coarse node tc
+  auto lag = [](int Q, double xi, int i) {
+    const double h = 1.0 / (Q - 1);
+    double v = 1;
+    for (int j = 0; j < Q; ++j)
+      if (j != i) v *= (xi - j * h) / (i * h - j * h);
+    return v;
+  };
+  auto snap = [](double e) { return std::fabs(e) < 1e-7 ? 0.0 : (std::fabs(e - 1) < 1e-7 ? 1.0 : e); };
+  std::vector<double> E((size_t)NF * NC), Ei((size_t)NC * NF);
+  for (int tf = 0; tf < NF; ++tf)
+    for (int tc = 0; tc < NC; ++tc) {
+      const int fi[3] = {tf % Qf, (tf / Qf) % Qf, tf / (Qf * Qf)};
+      const int ci[3] = {tc % Qc, (tc / Qc) % Qc, tc / (Qc * Qc)};
+      double e = 1, ei = 1;
+      for (int d = 0; d < 3; ++d) {
+        e *= lag(Qc, fi[d] / double(Qf - 1), ci[d]);
+        ei *= lag(Qf, ci[d] / double(Qc - 1), fi[d]);
+      }
+      E[(size_t)tf * NC + tc] = snap(e);
+      Ei[(size_t)tc * NF + tf] = snap(ei);
+    }
+  // coarse ids in the fine operator's batch layout, Dirichlet consistency (src/transfers.jl:112-127)
+  const int64_t nslot = fine->nslots();
+  std::vector<int32_t> cid((size_t)nslot * 3 * NC, 0);
+  for (int64_t sl = 0; sl < nslot; ++sl) {
+    const int64_t cell = fine->batch_cell[sl];
+    if (cell < 0) continue;
+    const int32_t* cd = &coarse->cell_dofs[(size_t)cell * coarse->nd];
+    const int32_t* fd = &fine->cell_dofs[(size_t)cell * fine->nd];
+    for (int c = 0; c < 3; ++c)
+      for (int tc = 0; tc < NC; ++tc) cid[((size_t)sl * 3 + c) * NC + tc] = cd[coarse->ldof[c * NC + coarse->nodeperm[tc]]];
+    for (int c = 0; c < 3; ++c) {
+      for (int tf = 0; tf < NF; ++tf) {
+        if (fd[fine->ldof[c * NF + fine->nodeperm[tf]]] > 0) continue;
+        for (int tc = 0; tc < NC; ++tc)
+          if (E[(size_t)tf * NC + tc] != 0 && cid[((size_t)sl * 3 + c) * NC + tc] > 0)
+            return set_err(ctx, HFK_ESTATE,
+                           "HyperFEMKernels: the two levels have different Dirichlet boundaries (a fine Dirichlet dof is "
+                           "interpolated from a free coarse dof; use the same closed Dirichlet set on every level)");
+      }
+      for (int tc = 0; tc < NC; ++tc) {
+        if (cid[((size_t)sl * 3 + c) * NC + tc] > 0) continue;
+        for (int tf = 0; tf < NF; ++tf)
+          if (Ei[(size_t)tc * NF + tf] != 0 && fd[fine->ldof[c * NF + fine->nodeperm[tf]]] > 0)
+            return set_err(ctx, HFK_ESTATE,
+                           "HyperFEMKernels: the two levels have different Dirichlet boundaries (a coarse Dirichlet dof "
+                           "is interpolated from a free fine dof)");
+      }
+    }
+  }
+  std::vector<double> inv((size_t)fine->nfree, 0.0);
+  for (int64_t cell = 0; cell < fine->ncells; ++cell)
+    for (int j = 0; j < fine->nd; ++j) {
+      const int32_t id = fine->cell_dofs[(size_t)cell * fine->nd + j];
+      if (id > 0) inv[id - 1] += 1.0;
+    }
+  for (auto& v : inv) v = v > 0 ? 1.0 / v : 0.0;
+  HIPCHK(ctx, T->E.alloc(E.size() * sizeof(double)));
+  HIPCHK(ctx, hipMemcpy(T->E.p, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(ctx, T->Einj.alloc(Ei.size() * sizeof(double)));
+  HIPCHK(ctx, hipMemcpy(T->Einj.p, Ei.data(), Ei.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(ctx, T->cid.alloc(cid.size() * sizeof(int32_t)));
+  HIPCHK(ctx, hipMemcpy(T->cid.p, cid.data(), cid.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(ctx, T->invmult.alloc(inv.size() * sizeof(double)));
+  HIPCHK(ctx, hipMemcpy(T->invmult.p, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+  return HFK_OK;
+}
+
+// prolongate! (src/transfers.jl:238-245): yf = P xc, Dirichlet dofs read as zero and never written
+int do_prolongate(hfk_transfer* T, const double* xc, double* yf, hipStream_t s) {
+  hfk_op* f = T->fine;
+  HIPCHK(f->ctx, launch_fill(yf, f->nfree, 0.0, s));
+  HIPCHK(f->ctx, transfer_prolongate(f->nslots(), T->NF, T->NC, f->ids.as<int32_t>(), T->cid.as<int32_t>(),
+                                     T->E.as<double>(), xc, yf, s));
+  return HFK_OK;
+}
+
+// restrict! (src/transfers.jl:252-259): yc = P' yf, colour by colour of the fine cells
+int do_restrict(hfk_transfer* T, const double* yf, double* yc, hipStream_t s) {
+  hfk_op* f = T->fine;
+  HIPCHK(f->ctx, launch_fill(yc, T->coarse->nfree, 0.0, s));
+  for (const BatchRange& r : f->color_batches)
+    HIPCHK(f->ctx, transfer_restrict((int64_t)r.first * f->C, (int64_t)r.count * f->C, T->NF, T->NC, f->ids.as<int32_t>(),
+                                     T->cid.as<int32_t>(), T->E.as<double>(), T->invmult.as<double>(), yf, yc, s));
+  return HFK_OK;
+}
+
+// inject! (src/transfers.jl:285-313)
+int do_inject(hfk_transfer* T, const double* xf, const double* df, double* xc, double* dc, hipStream_t s) {
+  hfk_op* f = T->fine;
+  HIPCHK(f->ctx, transfer_inject(f->nslots(), T->NF, T->NC, f->ids.as<int32_t>(), T->cid.as<int32_t>(),
+                                 T->Einj.as<double>(), xf, df, xc, dc, s));
+  return HFK_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+// p-multigrid (src/pmultigrid.jl): levels fine first; the last is assembled
+struct hfk_pmg {
+  hfk_ctx* ctx = nullptr;
+  std::vector<hfk_op*> ops;
+  std::vector<std::unique_ptr<hfk_transfer>> P;  // P[l]: level l+1 -> level l
+  std::vector<std::unique_ptr<hfk_cheb>> cheb;   // smoothed levels 0..L-2
+  std::vector<std::unique_ptr<DevBuf>> r, e, bc, xc;
+  int coarse_kind = 0, coarse_maxiter = 500;
+  double coarse_rtol = 1e-2;
+  // coarse CSR (CSC of A, 0-based int32)
+  DevBuf colptr, rowval, nzval;
+  int64_t nnz = 0;
+  DevBuf cr, cz, cp, cq, cs;  // inner CG work vectors and device scalars
+  Dots dots;
+  bool ok = false, indefinite = false, built = false, coarse_ok = false;
+  int nupdate = 0, ncycle = 0, last_inner = 0, total_inner = 0;
+  int storage = 0;
+};
+
+namespace {
+
+int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
+  hfk_op* c = g->ops.back();
+  int rc;
+  if (!g->colptr.p) {
+    if ((rc = op_build_pattern(c))) return rc;
+    if ((int64_t)c->rowval.size() >= ((int64_t)1 << 31)) return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: coarse matrix too large");
+    std::vector<int32_t> cp(c->colptr.size()), rv(c->rowval.size());
+    for (size_t i = 0; i < cp.size(); ++i) cp[i] = (int32_t)(c->colptr[i] - 1);
+    for (size_t i = 0; i < rv.size(); ++i) rv[i] = (int32_t)(c->rowval[i] - 1);
+    g->nnz = (int64_t)rv.size();
+    HIPCHK(g->ctx, g->colptr.alloc(cp.size() * sizeof(int32_t)));
+    HIPCHK(g->ctx, hipMemcpy(g->colptr.p, cp.data(), cp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(g->ctx, g->rowval.alloc(std::max<size_t>(rv.size(), 1) * sizeof(int32_t)));
+    HIPCHK(g->ctx, hipMemcpy(g->rowval.p, rv.data(), rv.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(g->ctx, g->nzval.alloc(std::max<size_t>(rv.size(), 1) * sizeof(double)));
+    if ((rc = op_build_blocks(c))) return rc;
+  }
+  if ((rc = op_run_jacobian(c, c->state.as<double>(), g->nzval.as<double>(), s))) return rc;
+  NodeBlocks& nb = c->blocks;
+  HIPCHK(g->ctx, csr_blocks(nb.nblocks, nb.dofs.as<int32_t>(), g->colptr.as<int32_t>(), g->rowval.as<int32_t>(),
+                            g->nzval.as<double>(), nb.B.as<double>(), s));
+  HIPCHK(g->ctx, hipMemsetAsync(nb.bad.p, 0, sizeof(int), s));
+  HIPCHK(g->ctx, count_nonfinite(g->nnz, g->nzval.as<double>(), nb.bad.as<int>(), s));
+  HIPCHK(g->ctx, blocks_invert(nb.nblocks, nb.dofs.as<int32_t>(), nb.B.as<double>(), nb.Binv.as<double>(), nb.bad.as<int>(), s));
+  int bad = 0;
+  HIPCHK(g->ctx, hipMemcpyAsync(&bad, nb.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
+  *ok = bad == 0;
+  return HFK_OK;
+}
+
+// coarse solve: block-Jacobi PCG on the assembled coarse matrix from zero to coarse_rtol (relative)
+int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s) {
+  hfk_op* c = g->ops.back();
+  const int64_t n = c->nfree;
+  NodeBlocks& nb = c->blocks;
+  double* r = g->cr.as<double>();
+  double* z = g->cz.as<double>();
+  double* p = g->cp.as<double>();
+  double* q = g->cq.as<double>();
+  double* sc = g->cs.as<double>();
+  int rc;
+  HIPCHK(g->ctx, launch_fill(x, n, 0.0, s));
+  HIPCHK(g->ctx, vec_copy(n, b, r, s));
+  HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, z, nullptr, 0.0, 1.0, s));
+  HIPCHK(g->ctx, vec_copy(n, z, p, s));
+  double h[3];
+  if ((rc = g->dots.run(g->ctx, s, n, r, z, h, b, b))) return rc;
+  const double bn = std::sqrt(h[1]);
+  HIPCHK(g->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZ, s));
+  g->last_inner = 0;
+  if (bn == 0) return HFK_OK;
+  for (int it = 1; it <= g->coarse_maxiter; ++it) {
+    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), p, q, 1.0, 0.0, s));
+    HIPCHK(g->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kPQ, s));
+    HIPCHK(g->ctx, cg_update(n, sc, p, q, x, r, s));
+    HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, z, nullptr, 0.0, 1.0, s));
+    HIPCHK(g->ctx, vec_dots(n, r, z, r, r, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZN, s));
+    g->last_inner = it;
+    if (it % 5 == 0 || it == g->coarse_maxiter) {
+      double rr[2];
+      HIPCHK(g->ctx, hipMemcpyAsync(rr, sc + kRZN, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(g->ctx, hipStreamSynchronize(s));
+      if (!(std::sqrt(rr[1]) > g->coarse_rtol * bn)) break;  // converged (or NaN: leave it to the outer check)
+    }
+    HIPCHK(g->ctx, cg_direction(n, sc, z, p, 0, s));
+  }
+  g->total_inner += g->last_inner;
+  return HFK_OK;
+}
+
+// vcycle! (src/pmultigrid.jl:316-329)
+int vcycle(hfk_pmg* g, size_t l, double* x, const double* b, hipStream_t s) {
+  hfk_cheb* S = g->cheb[l].get();
+  int rc;
+  if ((rc = cheb_smooth(S, x, b, true, g->r[l]->as<double>(), s))) return rc;
+  if ((rc = do_restrict(g->P[l].get(), g->r[l]->as<double>(), g->bc[l]->as<double>(), s))) return rc;
+  if (l + 2 == g->ops.size()) {
+    if ((rc = coarse_solve(g, g->bc[l]->as<double>(), g->xc[l]->as<double>(), s))) return rc;
+  } else {
+    if ((rc = vcycle(g, l + 1, g->xc[l]->as<double>(), g->bc[l]->as<double>(), s))) return rc;
+  }
+  if ((rc = do_prolongate(g->P[l].get(), g->xc[l]->as<double>(), g->e[l]->as<double>(), s))) return rc;
+  HIPCHK(g->ctx, vec_axpby(g->ops[l]->nfree, 1.0, g->e[l]->as<double>(), 1.0, x, s));
+  return cheb_smooth(S, x, b, false, nullptr, s);
+}
+
+// update_levels! + smoother_update! (src/pmultigrid.jl:168-221, 299-314)
+int pmg_update(hfk_pmg* g, hipStream_t s) {
+  hfk_op* f = g->ops[0];
+  if (f->storage < 0) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid update before prepare! of the working operator");
+  g->nupdate++;
+  int rc;
+  bool ok = true;
+  for (size_t l = 1; l < g->ops.size(); ++l) {
+    hfk_op* pf = g->ops[l - 1];
+    hfk_op* pc = g->ops[l];
+    if ((rc = do_inject(g->P[l - 1].get(), pf->state.as<double>(), pf->dvals.as<double>(), pc->state.as<double>(),
+                        pc->dvals.as<double>(), s)))
+      return rc;
+    if (l + 1 < g->ops.size()) {
+      if ((rc = op_run_prepare(pc, pc->state.as<double>(), g->storage, s))) return rc;
+    } else {
+      bool cok = false;
+      if ((rc = coarse_setup(g, s, &cok))) return rc;
+      g->coarse_ok = cok;
+      ok = ok && cok;
+    }
+  }
+  bool indef = false;
+  for (auto& ch : g->cheb) {
+    if ((rc = cheb_update(ch.get(), s))) return rc;
+    ok = ok && ch->ok;
+    // smoother_indefinite (src/smoothers.jl:48): finite blocks and Ritz values, a non-positive one
+    if (!ch->ok && ch->blocks_ok && std::isfinite(ch->lmax) && std::isfinite(ch->lmin) && (ch->lmin <= 0 || ch->lmax <= 0))
+      indef = true;
+  }
+  g->ok = ok;
+  g->indefinite = !ok && indef;
+  g->built = true;
+  return HFK_OK;
+}
+
+int pmg_apply(hfk_pmg* g, const double* r, double* z, hipStream_t s) {
+  if (!g->ok) {  // ldiv! returns NaN: the Krylov solve reports a failure (src/pmultigrid.jl:331-335)
+    HIPCHK(g->ctx, launch_fill(z, g->ops[0]->nfree, NAN, s));
+    return HFK_OK;
+  }
+  g->ncycle++;
+  return vcycle(g, 0, z, r, s);
+}
+
+// MatrixFreeSolver solve! (src/solvers.jl:111-158) as CG (kind 0) or flexible CG (kind 1) from x = 0 on device,
+// preconditioned by nothing (0), the working operator's nodal blocks (1) or one p-multigrid V-cycle (2).
+// Stops when ||r||_2 <= atol + rtol ||b||_2 (recurrence residual), or on a non-positive curvature p'Ap
+// (indefinite), or at maxiter; the final decision uses the true residual b - A x.
+struct KrylovWork {
+  DevBuf r, z, p, q, sc;
+  Dots dots;
+};
+
+int krylov(hfk_op* op, hfk_pmg* g, int kind, int precond, const double* b, double* x, double rtol, double atol,
+           int maxiter, int* status, double* rnorm, hipStream_t s) {
+  const int64_t n = op->nfree;
+  KrylovWork w;
+  int rc;
+  if ((rc = vec_alloc(op->ctx, w.r, n)) || (rc = vec_alloc(op->ctx, w.z, n)) || (rc = vec_alloc(op->ctx, w.p, n)) ||
+      (rc = vec_alloc(op->ctx, w.q, n)) || (rc = w.dots.init(op->ctx)))
+    return rc;
+  HIPCHK(op->ctx, w.sc.alloc(kNS * sizeof(double)));
+  double *r = w.r.as<double>(), *z = w.z.as<double>(), *p = w.p.as<double>(), *q = w.q.as<double>(), *sc = w.sc.as<double>();
+  status[0] = 0;
+  status[1] = 0;
+  status[2] = 0;
+  if (precond == 2 && (!g || g->ops[0] != op)) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the p-multigrid is not built on this operator");
+  if (precond == 2 && !g->built) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid used before its update");
+  if (precond == 2 && !g->ok) {  // src/solvers.jl:114-121: a failed preconditioner returns at once, not converged
+    status[2] = g->indefinite ? 1 : 0;
+    rnorm[0] = rnorm[1] = NAN;
+    return HFK_OK;
+  }
+  auto M = [&](const double* rin, double* zout) -> int {
+    if (precond == 0) {
+      HIPCHK(op->ctx, vec_copy(n, rin, zout, s));
+      return HFK_OK;
+    }
+    if (precond == 1) return blocks_apply(op, rin, zout, s);
+    return pmg_apply(g, rin, zout, s);
+  };
+  HIPCHK(op->ctx, launch_fill(x, n, 0.0, s));
+  HIPCHK(op->ctx, vec_copy(n, b, r, s));
+  double h[3];
+  if ((rc = w.dots.run(op->ctx, s, n, b, b, h))) return rc;
+  const double bn = std::sqrt(h[0]);
+  rnorm[1] = bn;
+  if (bn == 0) {
+    status[0] = 1;
+    rnorm[0] = 0;
+    return HFK_OK;
+  }
+  const double tol = atol + rtol * bn;
+  if ((rc = M(r, z))) return rc;
+  HIPCHK(op->ctx, vec_copy(n, z, p, s));
+  HIPCHK(op->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, w.dots.part.as<double>(), sc + kRZ, s));
+  int it = 0;
+  bool conv = false, indef = false;
+  while (it < maxiter) {
+    if ((rc = op_run_apply(op, p, q, s, 1.0, false))) return rc;
+    double pq;
+    HIPCHK(op->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, w.dots.part.as<double>(), sc + kPQ, s));
+    HIPCHK(op->ctx, hipMemcpyAsync(&pq, sc + kPQ, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(op->ctx, hipStreamSynchronize(s));
+    if (!(pq > 0)) {  // negative curvature (or NaN): stop, as cg with linesearch
+      indef = std::isfinite(pq);
+      break;
+    }
+    HIPCHK(op->ctx, cg_update(n, sc, p, q, x, r, s));
+    ++it;
+    double rr;
+    if ((rc = w.dots.run(op->ctx, s, n, r, r, &rr))) return rc;
+    if (!std::isfinite(rr)) break;
+    if (std::sqrt(rr) <= tol) {
+      conv = true;
+      break;
+    }
+    if ((rc = M(r, z))) return rc;
+    HIPCHK(op->ctx, vec_dots(n, r, z, z, q, nullptr, nullptr, w.dots.part.as<double>(), sc + kRZN, s));
+    // kRZN and kZQ: vec_dots writes consecutive outputs; kZQ is read from sc[kRZN + 1]
+    HIPCHK(op->ctx, hipMemcpyAsync(sc + kZQ, sc + kRZN + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(op->ctx, cg_direction(n, sc, z, p, kind == 1 ? 1 : 0, s));
+  }
+  // true residual
+  if ((rc = op_run_apply(op, x, r, s, 1.0, false))) return rc;
+  HIPCHK(op->ctx, vec_axpby(n, 1.0, b, -1.0, r, s));
+  double rt;
+  if ((rc = w.dots.run(op->ctx, s, n, r, r, &rt))) return rc;
+  rnorm[0] = std::sqrt(rt);
+  status[0] = (conv || rnorm[0] <= tol) && std::isfinite(rnorm[0]) ? 1 : 0;
+  status[1] = it;
+  status[2] = indef ? 1 : 0;
+  return HFK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- BlockJacobi ----------------------------------------------------------------------------------------
+int hfk_block_jacobi_update(hfk_op* op, int* ok) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  bool b = false;
+  if ((rc = blocks_update(op, op->ctx->stream, &b))) return rc;
+  if (ok) *ok = b ? 1 : 0;
+  return HFK_OK;
+}
+
+int hfk_block_jacobi_apply_dev(hfk_op* op, const double* r_dev, double* y_dev, void* stream) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  return blocks_apply(op, r_dev, y_dev, op->stream(stream));
+}
+
+int hfk_block_jacobi_apply(hfk_op* op, const double* r, double* y) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  if (!r || !y) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
+  const int64_t n = op->nfree;
+  DevBuf rd, yd;
+  if ((rc = vec_alloc(op->ctx, rd, n)) || (rc = vec_alloc(op->ctx, yd, n))) return rc;
+  hipStream_t s = op->ctx->stream;
+  HIPCHK(op->ctx, hipMemcpyAsync(rd.p, r, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, launch_fill(yd.as<double>(), n, 0.0, s));
+  if ((rc = blocks_apply(op, rd.as<double>(), yd.as<double>(), s))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(y, yd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+int hfk_block_jacobi_blocks(hfk_op* op, int64_t* nblocks, int32_t* dofs, double* Binv) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  if ((rc = op_build_blocks(op))) return rc;
+  if (nblocks) *nblocks = op->blocks.nblocks;
+  if (dofs) std::memcpy(dofs, op->blocks.dofs_h.data(), op->blocks.dofs_h.size() * sizeof(int32_t));
+  if (Binv) HIPCHK(op->ctx, hipMemcpy(Binv, op->blocks.Binv.p, op->blocks.nblocks * 9 * sizeof(double), hipMemcpyDeviceToHost));
+  return HFK_OK;
+}
+
+// ---- Chebyshev ------------------------------------------------------------------------------------------
+int hfk_cheb_create(hfk_op* op, int degree, double range, double inflate, int eig_iters, hfk_cheb** out) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  if (!out) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null output pointer");
+  if ((rc = check_cheb_args(op->ctx, degree, range, inflate, eig_iters))) return rc;
+  std::unique_ptr<hfk_cheb> ch(new hfk_cheb());
+  if ((rc = cheb_init(op, degree, range, inflate, eig_iters, ch.get()))) return rc;
+  *out = ch.release();
+  return HFK_OK;
+}
+
+int hfk_cheb_destroy(hfk_cheb* ch) {
+  delete ch;
+  return HFK_OK;
+}
+
+int hfk_cheb_update(hfk_cheb* ch, int* status, double* interval) {
+  if (!ch) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null smoother");
+  int rc = op_bind(ch->op);
+  if (rc) return rc;
+  if ((rc = cheb_update(ch, ch->op->ctx->stream))) return rc;
+  if (status) {
+    status[0] = ch->ok ? 1 : 0;
+    status[1] = (!ch->ok && ch->blocks_ok && std::isfinite(ch->lmax) && std::isfinite(ch->lmin) &&
+                 (ch->lmin <= 0 || ch->lmax <= 0)) ? 1 : 0;
+  }
+  if (interval) {
+    interval[0] = ch->lmin;
+    interval[1] = ch->lmax;
+    interval[2] = ch->a;
+    interval[3] = ch->b;
+  }
+  return HFK_OK;
+}
+
+int hfk_cheb_smooth_dev(hfk_cheb* ch, double* x, const double* rhs, int zero_guess, double* residual, void* stream) {
+  if (!ch) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null smoother");
+  int rc = op_bind(ch->op);
+  if (rc) return rc;
+  return cheb_smooth(ch, x, rhs, zero_guess != 0, residual, ch->op->stream(stream));
+}
+
+int hfk_cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, int zero_guess, double* residual) {
+  if (!ch || !x || !rhs) return set_err(ch ? ch->op->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  hfk_op* op = ch->op;
+  int rc = op_bind(op);
+  if (rc) return rc;
+  const int64_t n = op->nfree;
+  hipStream_t s = op->ctx->stream;
+  DevBuf xd, bd, rd;
+  if ((rc = vec_alloc(op->ctx, xd, n)) || (rc = vec_alloc(op->ctx, bd, n)) || (rc = vec_alloc(op->ctx, rd, n))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(xd.p, x, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(bd.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if ((rc = cheb_smooth(ch, xd.as<double>(), bd.as<double>(), zero_guess != 0, residual ? rd.as<double>() : nullptr, s))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(x, xd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (residual) HIPCHK(op->ctx, hipMemcpyAsync(residual, rd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+int hfk_eigenvalue_bounds(hfk_op* op, int k, double* lmin, double* lmax) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  if (k < 1) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: eigenvalue_bounds: k must be >= 1");
+  hfk_cheb ch;
+  if ((rc = cheb_init(op, 2, 20.0, 1.1, std::max(k, 2), &ch))) return rc;
+  if (!op->blocks.built) {
+    bool ok;
+    if ((rc = blocks_update(op, op->ctx->stream, &ok))) return rc;
+  }
+  return arnoldi_bounds(op, &ch, k, op->ctx->stream, lmin, lmax);
+}
+
+// ---- transfers ------------------------------------------------------------------------------------------
+int hfk_transfer_create(hfk_op* coarse, hfk_op* fine, hfk_transfer** out) {
+  if (!coarse || !fine || !out) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  int rc = op_bind(fine);
+  if (rc) return rc;
+  std::unique_ptr<hfk_transfer> T(new hfk_transfer());
+  if ((rc = transfer_build(coarse, fine, T.get()))) return rc;
+  *out = T.release();
+  return HFK_OK;
+}
+
+int hfk_transfer_destroy(hfk_transfer* T) {
+  delete T;
+  return HFK_OK;
+}
+
+namespace {
+int transfer_host(hfk_transfer* T, int which, const double* a, const double* b, double* c, double* d) {
+  if (!T) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null transfer");
+  hfk_op* f = T->fine;
+  hfk_op* co = T->coarse;
+  int rc = op_bind(f);
+  if (rc) return rc;
+  hipStream_t s = f->ctx->stream;
+  DevBuf A, B, Cb, Db;
+  const int64_t nf = f->nfree, nc = co->nfree;
+  if (which == 0) {  // prolongate: a = xc -> c = yf
+    if ((rc = vec_alloc(f->ctx, A, nc)) || (rc = vec_alloc(f->ctx, Cb, nf))) return rc;
+    HIPCHK(f->ctx, hipMemcpyAsync(A.p, a, nc * sizeof(double), hipMemcpyHostToDevice, s));
+    if ((rc = do_prolongate(T, A.as<double>(), Cb.as<double>(), s))) return rc;
+    HIPCHK(f->ctx, hipMemcpyAsync(c, Cb.p, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+  } else if (which == 1) {  // restrict: a = yf -> c = yc
+    if ((rc = vec_alloc(f->ctx, A, nf)) || (rc = vec_alloc(f->ctx, Cb, nc))) return rc;
+    HIPCHK(f->ctx, hipMemcpyAsync(A.p, a, nf * sizeof(double), hipMemcpyHostToDevice, s));
+    if ((rc = do_restrict(T, A.as<double>(), Cb.as<double>(), s))) return rc;
+    HIPCHK(f->ctx, hipMemcpyAsync(c, Cb.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+  } else {  // inject: (a = xf, b = df) -> (c = xc, d = dc)
+    if ((rc = vec_alloc(f->ctx, A, nf)) || (rc = vec_alloc(f->ctx, B, f->ndir)) || (rc = vec_alloc(f->ctx, Cb, nc)) ||
+        (rc = vec_alloc(f->ctx, Db, co->ndir)))
+      return rc;
+    HIPCHK(f->ctx, hipMemcpyAsync(A.p, a, nf * sizeof(double), hipMemcpyHostToDevice, s));
+    if (f->ndir) HIPCHK(f->ctx, hipMemcpyAsync(B.p, b, f->ndir * sizeof(double), hipMemcpyHostToDevice, s));
+    if ((rc = do_inject(T, A.as<double>(), B.as<double>(), Cb.as<double>(), Db.as<double>(), s))) return rc;
+    HIPCHK(f->ctx, hipMemcpyAsync(c, Cb.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (co->ndir) HIPCHK(f->ctx, hipMemcpyAsync(d, Db.p, co->ndir * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(f->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+}  // namespace
+
+int hfk_prolongate(hfk_transfer* T, const double* xc, double* yf) { return transfer_host(T, 0, xc, nullptr, yf, nullptr); }
+int hfk_restrict(hfk_transfer* T, const double* yf, double* yc) { return transfer_host(T, 1, yf, nullptr, yc, nullptr); }
+int hfk_inject(hfk_transfer* T, const double* xf, const double* df, double* xc, double* dc) {
+  return transfer_host(T, 2, xf, df, xc, dc);
+}
+
+// ---- p-multigrid ----------------------------------------------------------------------------------------
+int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, double inflate, int eig_iters,
+                   int coarse_kind, double coarse_rtol, int coarse_maxiter, hfk_pmg** out) {
+  if (!levels || !out || nlevels < 2)
+    return set_err(nullptr, HFK_EARG, "HyperFEMKernels: PMultigrid needs at least two levels");
+  hfk_ctx* ctx = levels[0]->ctx;
+  int rc = op_bind(levels[0]);
+  if (rc) return rc;
+  if ((rc = check_cheb_args(ctx, degree, range, inflate, eig_iters))) return rc;
+  if (coarse_kind != 0 || !(coarse_rtol > 0) || coarse_maxiter < 1)
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unsupported coarse solver settings");
+  std::unique_ptr<hfk_pmg> g(new hfk_pmg());
+  g->ctx = ctx;
+  g->coarse_kind = coarse_kind;
+  g->coarse_rtol = coarse_rtol;
+  g->coarse_maxiter = coarse_maxiter;
+  for (int l = 0; l < nlevels; ++l) {
+    if (!levels[l] || levels[l]->ctx != ctx) return set_err(ctx, HFK_EARG, "HyperFEMKernels: levels on different contexts");
+    g->ops.push_back(levels[l]);
+  }
+  for (int l = 0; l + 1 < nlevels; ++l) {
+    std::unique_ptr<hfk_transfer> T(new hfk_transfer());
+    if ((rc = transfer_build(g->ops[l + 1], g->ops[l], T.get()))) return rc;
+    g->P.push_back(std::move(T));
+    std::unique_ptr<hfk_cheb> ch(new hfk_cheb());
+    if ((rc = cheb_init(g->ops[l], degree, range, inflate, eig_iters, ch.get()))) return rc;
+    g->cheb.push_back(std::move(ch));
+    for (auto* v : {&g->r, &g->e, &g->bc, &g->xc}) v->push_back(std::unique_ptr<DevBuf>(new DevBuf()));
+    if ((rc = vec_alloc(ctx, *g->r.back(), g->ops[l]->nfree)) || (rc = vec_alloc(ctx, *g->e.back(), g->ops[l]->nfree)) ||
+        (rc = vec_alloc(ctx, *g->bc.back(), g->ops[l + 1]->nfree)) || (rc = vec_alloc(ctx, *g->xc.back(), g->ops[l + 1]->nfree)))
+      return rc;
+  }
+  for (int l = 1; l < nlevels; ++l) HIPCHK(ctx, g->ops[l]->state.alloc((size_t)g->ops[l]->nfree * sizeof(double)));
+  const int64_t nc = g->ops.back()->nfree;
+  if ((rc = vec_alloc(ctx, g->cr, nc)) || (rc = vec_alloc(ctx, g->cz, nc)) || (rc = vec_alloc(ctx, g->cp, nc)) ||
+      (rc = vec_alloc(ctx, g->cq, nc)) || (rc = g->dots.init(ctx)))
+    return rc;
+  HIPCHK(ctx, g->cs.alloc(kNS * sizeof(double)));
+  *out = g.release();
+  return HFK_OK;
+}
+
+int hfk_pmg_destroy(hfk_pmg* g) {
+  delete g;
+  return HFK_OK;
+}
+
+int hfk_pmg_update(hfk_pmg* g, int* status) {
+  if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  g->storage = g->ops[0]->storage < 0 ? 0 : g->ops[0]->storage;
+  if ((rc = pmg_update(g, g->ctx->stream))) return rc;
+  if (status) {
+    status[0] = g->ok ? 1 : 0;
+    status[1] = g->indefinite ? 1 : 0;
+  }
+  return HFK_OK;
+}
+
+int hfk_pmg_vcycle_dev(hfk_pmg* g, const double* r_dev, double* z_dev, void* stream) {
+  if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  if (!g->built) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid used before its update");
+  return pmg_apply(g, r_dev, z_dev, g->ops[0]->stream(stream));
+}
+
+int hfk_pmg_vcycle(hfk_pmg* g, const double* r, double* z) {
+  if (!g || !r || !z) return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  if (!g->built) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid used before its update");
+  const int64_t n = g->ops[0]->nfree;
+  hipStream_t s = g->ctx->stream;
+  DevBuf rd, zd;
+  if ((rc = vec_alloc(g->ctx, rd, n)) || (rc = vec_alloc(g->ctx, zd, n))) return rc;
+  HIPCHK(g->ctx, hipMemcpyAsync(rd.p, r, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if ((rc = pmg_apply(g, rd.as<double>(), zd.as<double>(), s))) return rc;
+  HIPCHK(g->ctx, hipMemcpyAsync(z, zd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+int hfk_pmg_info(hfk_pmg* g, double* info) {
+  if (!g || !info) return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  info[0] = (double)g->ops.size();
+  info[1] = g->ok ? 1.0 : 0.0;
+  info[2] = (double)g->ncycle;
+  info[3] = (double)g->last_inner;
+  info[4] = (double)g->total_inner;
+  info[5] = g->coarse_ok ? 1.0 : 0.0;
+  for (size_t l = 0; l < g->cheb.size(); ++l) {
+    double* o = info + 6 + 6 * l;
+    o[0] = g->cheb[l]->lmin;
+    o[1] = g->cheb[l]->lmax;
+    o[2] = g->cheb[l]->a;
+    o[3] = g->cheb[l]->b;
+    o[4] = g->cheb[l]->ok ? 1.0 : 0.0;
+    o[5] = g->cheb[l]->blocks_ok ? 1.0 : 0.0;
+  }
+  return HFK_OK;
+}
+
+// ---- Krylov ---------------------------------------------------------------------------------------------
+int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b_dev, double* x_dev, double rtol,
+                         double atol, int maxiter, int* status, double* rnorm, void* stream) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  if (!b_dev || !x_dev || !status || !rnorm) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  if (kind < 0 || kind > 1 || precond < 0 || precond > 2 || maxiter < 0 || rtol < 0 || atol < 0)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: MatrixFreeSolver: invalid settings");
+  if (op->storage < 0) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: solve! before prepare!");
+  if (precond == 1 && !op->blocks.built) {
+    bool ok;
+    if ((rc = blocks_update(op, op->stream(stream), &ok))) return rc;
+  }
+  return krylov(op, pmg, kind, precond, b_dev, x_dev, rtol, atol, maxiter, status, rnorm, op->stream(stream));
+}
+
+int hfk_krylov_solve(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b, double* x, double rtol,
+                     double atol, int maxiter, int* status, double* rnorm) {
+  int rc = op_bind(op);
+  if (rc) return rc;
+  if (!b || !x) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
+  const int64_t n = op->nfree;
+  hipStream_t s = op->ctx->stream;
+  DevBuf bd, xd;
+  if ((rc = vec_alloc(op->ctx, bd, n)) || (rc = vec_alloc(op->ctx, xd, n))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(bd.p, b, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if ((rc = hfk_krylov_solve_dev(op, pmg, kind, precond, bd.as<double>(), xd.as<double>(), rtol, atol, maxiter, status,
+                                 rnorm, nullptr)))
+    return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(x, xd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+}  // extern "C"

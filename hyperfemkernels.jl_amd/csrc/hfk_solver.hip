@@ -1,0 +1,342 @@
+// Device kernels of the solver layer: vector updates with deterministic reductions, the nodal block Jacobi
+// (invert/apply), the fused Chebyshev step, the p-multigrid transfers and the CSR product of the assembled coarse
+// level.  Host orchestration is in hfk_pmg.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "hfk_solver.hpp"
+
+namespace hfk {
+namespace {
+
+constexpr int kRedBlocks = 512;  // fixed grid of the reductions: the summation order never depends on the device
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// partial[b] = sum over the grid-stride chunk of block b of a[i]*b[i] (up to 3 dot products at once)
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial(int64_t n, const double* __restrict__ a0,
+                                                             const double* __restrict__ b0, const double* __restrict__ a1,
+                                                             const double* __restrict__ b1, const double* __restrict__ a2,
+                                                             const double* __restrict__ b2, double* __restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads) {
+    s0 += a0[i] * b0[i];
+    if (a1) s1 += a1[i] * b1[i];
+    if (a2) s2 += a2[i] * b2[i];
+  }
+  s0 = block_sum(s0, sh);
+  if (a1) s1 = block_sum(s1, sh);
+  if (a2) s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s0;
+    part[kRedBlocks + blockIdx.x] = s1;
+    part[2 * kRedBlocks + blockIdx.x] = s2;
+  }
+}
+
+// out[k] = sum_b part[k*kRedBlocks + b], in a fixed tree order
+__global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restrict__ part, int ndots,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  for (int k = 0; k < ndots; ++k) {
+    double v = part[k * kRedBlocks + threadIdx.x] + part[k * kRedBlocks + threadIdx.x + kRedThreads];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) out[k] = v;
+  }
+}
+
+__global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+// CG step on device scalars s[]: alpha = s[RZ]/s[PQ];  x += alpha p;  r -= alpha q
+__global__ void k_cg_update(int64_t n, const double* __restrict__ s, const double* __restrict__ p,
+                            const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r) {
+  const double alpha = s[kRZ] / s[kPQ];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * q[i];
+  }
+}
+
+// p = z + beta p: beta = s[RZN]/s[RZ] (CG) or -s[ZQ]/s[PQ] (flexible CG); then s[RZ] = s[RZN] (one thread)
+__global__ void k_cg_direction(int64_t n, double* __restrict__ s, const double* __restrict__ z, double* __restrict__ p,
+                               int flexible) {
+  const double beta = flexible ? -s[kZQ] / s[kPQ] : s[kRZN] / s[kRZ];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+__global__ void k_scalar_copy(double* s, int dst, int src) { s[dst] = s[src]; }
+
+// --- nodal blocks ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double det3d(const double* a) {
+  return a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2]) + a[6] * (a[1] * a[5] - a[4] * a[2]);
+}
+
+// _invert_blocks! (src/preconditioners.jl:292-307): fixed rows/columns -> identity; a non-finite block is kept
+// as it is and counted in bad[0]
+__global__ void k_invert_blocks(int64_t nb, const int32_t* __restrict__ dofs, const double* __restrict__ B,
+                                double* __restrict__ Binv, int* __restrict__ bad) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
+    double M[9];
+    bool fin = true;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) {
+        const bool fixed = dofs[3 * k + i] == 0 || dofs[3 * k + j] == 0;
+        M[i + 3 * j] = fixed ? (i == j ? 1.0 : 0.0) : B[9 * k + i + 3 * j];
+        fin = fin && isfinite(M[i + 3 * j]);
+      }
+    double R[9];
+    if (fin) {
+      const double id = 1.0 / det3d(M);
+      R[0] = (M[4] * M[8] - M[7] * M[5]) * id;
+      R[3] = -(M[3] * M[8] - M[6] * M[5]) * id;
+      R[6] = (M[3] * M[7] - M[6] * M[4]) * id;
+      R[1] = -(M[1] * M[8] - M[7] * M[2]) * id;
+      R[4] = (M[0] * M[8] - M[6] * M[2]) * id;
+      R[7] = -(M[0] * M[7] - M[6] * M[1]) * id;
+      R[2] = (M[1] * M[5] - M[4] * M[2]) * id;
+      R[5] = -(M[0] * M[5] - M[3] * M[2]) * id;
+      R[8] = (M[0] * M[4] - M[3] * M[1]) * id;
+    } else {
+      for (int e = 0; e < 9; ++e) R[e] = M[e];
+    }
+    bool ok = true;
+    for (int e = 0; e < 9; ++e) {
+      Binv[9 * k + e] = R[e];
+      ok = ok && isfinite(R[e]);
+    }
+    if (!ok) atomicAdd(bad, 1);
+  }
+}
+
+// _apply_blocks! (src/preconditioners.jl:327-342) fused with the Chebyshev direction update
+// (src/smoothers.jl:130-141):  z = Binv r on the block;  d = cd d + cz z;  x += d (if x)
+// With cd = 0, cz = 1 and x = null this is ldiv!(y = d, blocks, r).
+__global__ void k_block_step(int64_t nb, const int32_t* __restrict__ dofs, const double* __restrict__ Binv,
+                             const double* __restrict__ r, double* __restrict__ d, double* __restrict__ x, double cd,
+                             double cz) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
+    int id[3];
+    double rv[3];
+    for (int j = 0; j < 3; ++j) {
+      id[j] = dofs[3 * k + j];
+      rv[j] = id[j] > 0 ? r[id[j] - 1] : 0.0;
+    }
+    for (int i = 0; i < 3; ++i) {
+      if (id[i] <= 0) continue;
+      double z = 0;
+      for (int j = 0; j < 3; ++j)
+        if (id[j] > 0) z += Binv[9 * k + i + 3 * j] * rv[j];
+      const double dn = (cd == 0.0 ? 0.0 : cd * d[id[i] - 1]) + cz * z;
+      d[id[i] - 1] = dn;
+      if (x) x[id[i] - 1] += dn;
+    }
+  }
+}
+
+// nodal blocks of an assembled matrix (CSC of A, 0-based): B[i + 3j] = A[dof_i, dof_j]
+__global__ void k_csr_blocks(int64_t nb, const int32_t* __restrict__ dofs, const int32_t* __restrict__ colptr,
+                             const int32_t* __restrict__ rowval, const double* __restrict__ val, double* __restrict__ B) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) {
+        const int ci = dofs[3 * k + i], cj = dofs[3 * k + j];
+        double v = 0;
+        if (ci > 0 && cj > 0) {
+          int lo = colptr[cj - 1], hi = colptr[cj];
+          while (lo < hi) {  // first row >= ci
+            const int mid = (lo + hi) >> 1;
+            if (rowval[mid] < ci - 1) lo = mid + 1; else hi = mid;
+          }
+          if (lo < colptr[cj] && rowval[lo] == ci - 1) v = val[lo];
+        }
+        B[9 * k + i + 3 * j] = v;
+      }
+  }
+}
+
+// y = alpha * A' x + beta * y over the columns of the CSC of A (= rows of A' ; A symmetric at the coarse level),
+// 16 lanes per column
+__global__ __launch_bounds__(256) void k_csc_t_spmv(int64_t ncols, const int32_t* __restrict__ colptr,
+                                                    const int32_t* __restrict__ rowval, const double* __restrict__ val,
+                                                    const double* __restrict__ x, double* __restrict__ y, double alpha,
+                                                    double beta) {
+  const int lane = threadIdx.x & 15;
+  for (int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; c < ncols;
+       c += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+    double s = 0;
+    for (int p = colptr[c] + lane; p < colptr[c + 1]; p += 16) s += val[p] * x[rowval[p]];
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (lane == 0) y[c] = alpha * s + (beta == 0.0 ? 0.0 : beta * y[c]);
+  }
+}
+
+// --- transfers (src/transfers.jl:188-221, 295-313) ---------------------------------------------------------
+// fine ids: [slot][c][tf] (the fine operator's batch layout); coarse ids of the same cells: [slot][c][tc]
+__global__ void k_prolongate(int64_t nslot, int NF, int NC, const int32_t* __restrict__ fid,
+                             const int32_t* __restrict__ cid, const double* __restrict__ E, const double* __restrict__ xc,
+                             double* __restrict__ yf) {
+  const int64_t n = nslot * 3 * NF;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+    const int id = fid[g];
+    if (id <= 0) continue;
+    const int64_t sc = g / NF;  // slot*3 + c
+    const int tf = (int)(g - sc * NF);
+    const int32_t* ci = cid + sc * NC;
+    double v = 0;
+    for (int tc = 0; tc < NC; ++tc) {
+      const double e = E[tf * NC + tc];
+      const int j = ci[tc];
+      if (e != 0.0 && j > 0) v += e * xc[j - 1];
+    }
+    yf[id - 1] = v;  // insertion: the cells sharing a fine dof write the same value
+  }
+}
+
+// restriction over the batches of one colour of the fine operator (cells of a colour share no fine free dof,
+// hence no coarse free dof): yc[j] += sum_tf E[tf, tc] yf[id] / mult[id]
+__global__ void k_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* __restrict__ fid,
+                           const int32_t* __restrict__ cid, const double* __restrict__ E,
+                           const double* __restrict__ invmult, const double* __restrict__ yf, double* __restrict__ yc) {
+  const int64_t n = nslot * 3 * NC;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t sc = slot0 * 3 + g / NC;
+    const int tc = (int)(g % NC);
+    const int j = cid[sc * NC + tc];
+    if (j <= 0) continue;
+    const int32_t* fi = fid + sc * NF;
+    double v = 0;
+    for (int tf = 0; tf < NF; ++tf) {
+      const double e = E[tf * NC + tc];
+      const int id = fi[tf];
+      if (e != 0.0 && id > 0) v += e * yf[id - 1] * invmult[id - 1];
+    }
+    yc[j - 1] += v;
+  }
+}
+
+// nodal interpolation of the fine state (free xf, Dirichlet df) at the coarse nodes
+__global__ void k_inject(int64_t nslot, int NF, int NC, const int32_t* __restrict__ fid, const int32_t* __restrict__ cid,
+                         const double* __restrict__ Einj, const double* __restrict__ xf, const double* __restrict__ df,
+                         double* __restrict__ xc, double* __restrict__ dc) {
+  const int64_t n = nslot * 3 * NC;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+    const int j = cid[g];
+    if (j == 0) continue;
+    const int64_t sc = g / NC;
+    const int tc = (int)(g - sc * NC);
+    const int32_t* fi = fid + sc * NF;
+    double v = 0;
+    for (int tf = 0; tf < NF; ++tf) {
+      const double w = Einj[tc * NF + tf];
+      if (w == 0.0) continue;
+      const int id = fi[tf];
+      v += w * (id > 0 ? xf[id - 1] : (id < 0 ? df[-id - 1] : 0.0));
+    }
+    if (j > 0) xc[j - 1] = v; else dc[-j - 1] = v;
+  }
+}
+
+__global__ void k_count_nonfinite(int64_t n, const double* __restrict__ x, int* __restrict__ bad) {
+  int local = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    local += isfinite(x[i]) ? 0 : 1;
+  if (local) atomicAdd(bad, local);
+}
+
+inline dim3 grid_for(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > 4096) b = 4096;
+  return dim3((unsigned)b);
+}
+
+}  // namespace
+
+hipError_t vec_dots(int64_t n, const double* a0, const double* b0, const double* a1, const double* b1, const double* a2,
+                    const double* b2, double* part, double* out, hipStream_t s) {
+  const int nd = a2 ? 3 : (a1 ? 2 : 1);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, a0, b0, a1, b1, a2, b2, part);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, s, part, nd, out);
+  return hipGetLastError();
+}
+size_t dot_partial_doubles() { return 3 * kRedBlocks; }
+
+hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(256), 0, s, n, a, x, b, y);
+  return hipGetLastError();
+}
+hipError_t vec_copy(int64_t n, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, grid_for(n), dim3(256), 0, s, n, x, y);
+  return hipGetLastError();
+}
+hipError_t cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_update, grid_for(n), dim3(256), 0, s, n, sc, p, q, x, r);
+  return hipGetLastError();
+}
+hipError_t cg_direction(int64_t n, double* sc, const double* z, double* p, int flexible, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_direction, grid_for(n), dim3(256), 0, s, n, sc, z, p, flexible);
+  hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, s, sc, kRZ, kRZN);
+  return hipGetLastError();
+}
+hipError_t blocks_invert(int64_t nb, const int32_t* dofs, const double* B, double* Binv, int* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_invert_blocks, grid_for(nb), dim3(256), 0, s, nb, dofs, B, Binv, bad);
+  return hipGetLastError();
+}
+hipError_t blocks_step(int64_t nb, const int32_t* dofs, const double* Binv, const double* r, double* d, double* x,
+                       double cd, double cz, hipStream_t s) {
+  hipLaunchKernelGGL(k_block_step, grid_for(nb), dim3(256), 0, s, nb, dofs, Binv, r, d, x, cd, cz);
+  return hipGetLastError();
+}
+hipError_t csr_blocks(int64_t nb, const int32_t* dofs, const int32_t* colptr, const int32_t* rowval, const double* val,
+                      double* B, hipStream_t s) {
+  hipLaunchKernelGGL(k_csr_blocks, grid_for(nb), dim3(256), 0, s, nb, dofs, colptr, rowval, val, B);
+  return hipGetLastError();
+}
+hipError_t csc_t_spmv(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, const double* x,
+                      double* y, double alpha, double beta, hipStream_t s) {
+  hipLaunchKernelGGL(k_csc_t_spmv, grid_for(ncols * 16), dim3(256), 0, s, ncols, colptr, rowval, val, x, y, alpha, beta);
+  return hipGetLastError();
+}
+hipError_t transfer_prolongate(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* E,
+                               const double* xc, double* yf, hipStream_t s) {
+  hipLaunchKernelGGL(k_prolongate, grid_for(nslot * 3 * NF), dim3(256), 0, s, nslot, NF, NC, fid, cid, E, xc, yf);
+  return hipGetLastError();
+}
+hipError_t transfer_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid,
+                             const double* E, const double* invmult, const double* yf, double* yc, hipStream_t s) {
+  if (nslot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_restrict, grid_for(nslot * 3 * NC), dim3(256), 0, s, slot0, nslot, NF, NC, fid, cid, E, invmult, yf, yc);
+  return hipGetLastError();
+}
+hipError_t transfer_inject(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* Einj,
+                           const double* xf, const double* df, double* xc, double* dc, hipStream_t s) {
+  hipLaunchKernelGGL(k_inject, grid_for(nslot * 3 * NC), dim3(256), 0, s, nslot, NF, NC, fid, cid, Einj, xf, df, xc, dc);
+  return hipGetLastError();
+}
+hipError_t count_nonfinite(int64_t n, const double* x, int* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_nonfinite, grid_for(n), dim3(256), 0, s, n, x, bad);
+  return hipGetLastError();
+}
+
+}  // namespace hfk

@@ -1,0 +1,37 @@
+// Internal declarations of the solver-layer kernels (hfk_solver.hip).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace hfk {
+
+// device scalars of a CG/FCG iteration
+enum : int { kRZ = 0, kPQ = 1, kZQ = 2, kRZN = 3, kRR = 4, kBB = 5, kNS = 8 };
+
+// out[0..] = a0.b0 (, a1.b1 (, a2.b2)): deterministic two-pass reductions on a fixed grid; `part` holds
+// dot_partial_doubles() doubles
+hipError_t vec_dots(int64_t n, const double* a0, const double* b0, const double* a1, const double* b1, const double* a2,
+                    const double* b2, double* part, double* out, hipStream_t s);
+size_t dot_partial_doubles();
+hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);  // y = a x + b y
+hipError_t vec_copy(int64_t n, const double* x, double* y, hipStream_t s);
+hipError_t cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, hipStream_t s);
+hipError_t cg_direction(int64_t n, double* sc, const double* z, double* p, int flexible, hipStream_t s);
+hipError_t blocks_invert(int64_t nb, const int32_t* dofs, const double* B, double* Binv, int* bad, hipStream_t s);
+hipError_t blocks_step(int64_t nb, const int32_t* dofs, const double* Binv, const double* r, double* d, double* x,
+                       double cd, double cz, hipStream_t s);
+hipError_t csr_blocks(int64_t nb, const int32_t* dofs, const int32_t* colptr, const int32_t* rowval, const double* val,
+                      double* B, hipStream_t s);
+hipError_t csc_t_spmv(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, const double* x,
+                      double* y, double alpha, double beta, hipStream_t s);
+hipError_t transfer_prolongate(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* E,
+                               const double* xc, double* yf, hipStream_t s);
+hipError_t transfer_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid,
+                             const double* E, const double* invmult, const double* yf, double* yc, hipStream_t s);
+hipError_t transfer_inject(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* Einj,
+                           const double* xf, const double* df, double* xc, double* dc, hipStream_t s);
+hipError_t count_nonfinite(int64_t n, const double* x, int* bad, hipStream_t s);
+
+}  // namespace hfk

@@ -47,6 +47,32 @@ SIGNATURES = {
     "hfk_pattern": (C.c_int, [_vp, _i64p, _i64p, _i32p]),
     "hfk_jacobian": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_jacobian_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_block_jacobi_update": (C.c_int, [_vp, _i32p]),
+    "hfk_block_jacobi_apply": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_block_jacobi_apply_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_block_jacobi_blocks": (C.c_int, [_vp, _i64p, _i32p, _f64p]),
+    "hfk_cheb_create": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)]),
+    "hfk_cheb_destroy": (C.c_int, [_vp]),
+    "hfk_cheb_update": (C.c_int, [_vp, _i32p, _f64p]),
+    "hfk_cheb_smooth": (C.c_int, [_vp, _f64p, _f64p, C.c_int, _f64p]),
+    "hfk_cheb_smooth_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp]),
+    "hfk_eigenvalue_bounds": (C.c_int, [_vp, C.c_int, _f64p, _f64p]),
+    "hfk_transfer_create": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "hfk_transfer_destroy": (C.c_int, [_vp]),
+    "hfk_prolongate": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_restrict": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_inject": (C.c_int, [_vp, _f64p, _f64p, _f64p, _f64p]),
+    "hfk_pmg_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_int, C.POINTER(_vp)]),
+    "hfk_pmg_destroy": (C.c_int, [_vp]),
+    "hfk_pmg_update": (C.c_int, [_vp, _i32p]),
+    "hfk_pmg_vcycle": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_pmg_vcycle_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_pmg_info": (C.c_int, [_vp, _f64p]),
+    "hfk_krylov_solve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _f64p, _f64p, C.c_double, C.c_double, C.c_int, _i32p,
+                                   _f64p]),
+    "hfk_krylov_solve_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_int, _i32p,
+                                       _f64p, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,242 @@
+"""Host mirror of the reference's solver layer over libhfk (all work on the GPU).
+
+Mirrors src/preconditioners.jl (``BlockJacobi``, ``eigenvalue_bounds``), src/smoothers.jl (``Chebyshev``,
+``smooth!``), src/transfers.jl (``Prolongation``, ``prolongate!``, ``restrict!``, ``inject!``),
+src/pmultigrid.jl (``PMultigrid``) and src/solvers.jl (``MatrixFreeSolver``).  Numerical failure is reported
+through status fields, never raised (src/solvers.jl:114-121); misuse raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import check, ptr
+from .operator import KernelOperator, MatrixFreeJacobian
+
+
+def _vec(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != n:
+        raise ValueError(f"HyperFEMKernels: DimensionMismatch: vector of length {a.size}, expected {n}")
+    return a
+
+
+class BlockJacobi:
+    """Inverse of the nodal 3x3 block diagonal of the prepared tangent (src/preconditioners.jl:100-342)."""
+
+    def __init__(self, mf: MatrixFreeJacobian):
+        self.op = mf.op
+        self.ok = False
+        self.update(mf)
+
+    def update(self, mf: MatrixFreeJacobian):
+        ok = C.c_int32()
+        check(self.op.lib.hfk_block_jacobi_update(self.op.handle, C.byref(ok)), self.op.ctx.handle)
+        self.ok = bool(ok.value)
+        return self
+
+    def ldiv(self, y, r):
+        r = _vec(r, self.op.nfree)
+        check(self.op.lib.hfk_block_jacobi_apply(self.op.handle, ptr(r, C.c_double), ptr(y, C.c_double)),
+              self.op.ctx.handle)
+        return y
+
+    def blocks(self):
+        nb = C.c_int64()
+        lib = self.op.lib
+        check(lib.hfk_block_jacobi_blocks(self.op.handle, C.byref(nb), None, None), self.op.ctx.handle)
+        dofs = np.zeros((nb.value, 3), dtype=np.int32)
+        Binv = np.zeros((nb.value, 9))
+        check(lib.hfk_block_jacobi_blocks(self.op.handle, C.byref(nb), ptr(dofs, C.c_int32), ptr(Binv, C.c_double)),
+              self.op.ctx.handle)
+        return dofs, Binv
+
+
+def eigenvalue_bounds(mf: MatrixFreeJacobian, k: int = 10):
+    """Extreme real parts of the Ritz values of J M^-1 after k Arnoldi steps from chebyshev_start (the nodal
+    blocks of the operator must be up to date)."""
+    lo, hi = C.c_double(), C.c_double()
+    check(mf.op.lib.hfk_eigenvalue_bounds(mf.op.handle, int(k), C.byref(lo), C.byref(hi)), mf.op.ctx.handle)
+    return lo.value, hi.value
+
+
+@dataclass(frozen=True)
+class Chebyshev:
+    """Smoother configuration (src/smoothers.jl:21-33)."""
+
+    degree: int = 2
+    range: float = 20.0
+    inflate: float = 1.1
+    eig_iters: int = 10
+
+    def __post_init__(self):
+        if self.degree < 1:
+            raise ValueError("HyperFEMKernels: Chebyshev: degree must be >= 1")
+        if not self.range > 1:
+            raise ValueError("HyperFEMKernels: Chebyshev: range must be > 1")
+        if not self.inflate >= 1:
+            raise ValueError("HyperFEMKernels: Chebyshev: inflate must be >= 1")
+        if self.eig_iters < 2:
+            raise ValueError("HyperFEMKernels: Chebyshev: eig_iters must be >= 2")
+
+    def damping(self) -> float:
+        """max |p_k| on the interval: 1/T_k(theta/delta) (chebyshev_damping, src/smoothers.jl:36-40)."""
+        b, a = 1.0, 1.0 / self.range
+        th, de = (b + a) / 2, (b - a) / 2
+        return 1.0 / np.cosh(self.degree * np.arccosh(th / de))
+
+
+class ChebyshevState:
+    """smoother_setup / smoother_update! / smooth! on one operator."""
+
+    def __init__(self, cfg: Chebyshev, mf: MatrixFreeJacobian):
+        self.cfg, self.op = cfg, mf.op
+        h = C.c_void_p()
+        check(self.op.lib.hfk_cheb_create(self.op.handle, cfg.degree, cfg.range, cfg.inflate, cfg.eig_iters,
+                                          C.byref(h)), self.op.ctx.handle)
+        self.handle = h
+        self.update(mf)
+
+    def update(self, mf: MatrixFreeJacobian):
+        st = np.zeros(2, dtype=np.int32)
+        iv = np.zeros(4)
+        check(self.op.lib.hfk_cheb_update(self.handle, ptr(st, C.c_int32), ptr(iv, C.c_double)), self.op.ctx.handle)
+        self.ok, self.indefinite = bool(st[0]), bool(st[1])
+        self.lmin, self.lmax, self.a, self.b = (float(v) for v in iv)
+        return self
+
+    def smooth(self, x, rhs, zero_guess=False, residual=None):
+        rhs = _vec(rhs, self.op.nfree)
+        if x.dtype != np.float64 or not x.flags.c_contiguous or x.size != self.op.nfree:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: smoother iterate")
+        rp = None if residual is None else ptr(residual, C.c_double)
+        check(self.op.lib.hfk_cheb_smooth(self.handle, ptr(x, C.c_double), ptr(rhs, C.c_double), int(bool(zero_guess)),
+                                          rp), self.op.ctx.handle)
+        return x
+
+    def __del__(self):
+        try:
+            self.op.lib.hfk_cheb_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Prolongation:
+    """Exact prolongation from the coarse operator's space to the fine one's on the same mesh."""
+
+    def __init__(self, coarse: KernelOperator, fine: KernelOperator):
+        h = C.c_void_p()
+        check(fine.lib.hfk_transfer_create(coarse.handle, fine.handle, C.byref(h)), fine.ctx.handle)
+        self.handle, self.coarse, self.fine, self.lib = h, coarse, fine, fine.lib
+        self.shape = (fine.nfree, coarse.nfree)
+
+    def prolongate(self, yf, xc):
+        xc = _vec(xc, self.coarse.nfree)
+        check(self.lib.hfk_prolongate(self.handle, ptr(xc, C.c_double), ptr(yf, C.c_double)), self.fine.ctx.handle)
+        return yf
+
+    def restrict(self, yc, yf):
+        yf = _vec(yf, self.fine.nfree)
+        check(self.lib.hfk_restrict(self.handle, ptr(yf, C.c_double), ptr(yc, C.c_double)), self.fine.ctx.handle)
+        return yc
+
+    def inject(self, xf, df):
+        xf = _vec(xf, self.fine.nfree)
+        df = _vec(df, self.fine.ndir) if self.fine.ndir else np.zeros(1)
+        xc = np.zeros(self.coarse.nfree)
+        dc = np.zeros(max(self.coarse.ndir, 1))
+        check(self.lib.hfk_inject(self.handle, ptr(xf, C.c_double), ptr(df, C.c_double), ptr(xc, C.c_double),
+                                  ptr(dc, C.c_double)), self.fine.ctx.handle)
+        return xc, dc[: self.coarse.ndir]
+
+    def __matmul__(self, xc):
+        return self.prolongate(np.zeros(self.fine.nfree), xc)
+
+    def __del__(self):
+        try:
+            self.lib.hfk_transfer_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class PMultigrid:
+    """One V-cycle of p-multigrid as a preconditioner (src/pmultigrid.jl:223-343).  ``levels``: the operators,
+    finest first (the first is the Newton's, matrix-free; the last assembled)."""
+
+    def __init__(self, levels, smoother: Chebyshev = Chebyshev(), coarse_rtol: float = 1e-2,
+                 coarse_maxiter: int = 500):
+        if len(levels) < 2:
+            raise ValueError("HyperFEMKernels: PMultigrid needs at least two levels")
+        self.levels = list(levels)
+        arr = (C.c_void_p * len(levels))(*[op.handle.value for op in levels])
+        h = C.c_void_p()
+        lib = levels[0].lib
+        check(lib.hfk_pmg_create(arr, len(levels), smoother.degree, smoother.range, smoother.inflate,
+                                 smoother.eig_iters, 0, float(coarse_rtol), int(coarse_maxiter), C.byref(h)),
+              levels[0].ctx.handle)
+        self.handle, self.lib, self.smoother = h, lib, smoother
+        self.ok = self.indefinite = False
+
+    def update(self):
+        """preconditioner_update!: follow the state of the working operator's last prepare."""
+        st = np.zeros(2, dtype=np.int32)
+        check(self.lib.hfk_pmg_update(self.handle, ptr(st, C.c_int32)), self.levels[0].ctx.handle)
+        self.ok, self.indefinite = bool(st[0]), bool(st[1])
+        return self
+
+    def ldiv(self, z, r):
+        r = _vec(r, self.levels[0].nfree)
+        check(self.lib.hfk_pmg_vcycle(self.handle, ptr(r, C.c_double), ptr(z, C.c_double)), self.levels[0].ctx.handle)
+        return z
+
+    def info(self) -> dict:
+        a = np.zeros(6 + 6 * len(self.levels))
+        check(self.lib.hfk_pmg_info(self.handle, ptr(a, C.c_double)), self.levels[0].ctx.handle)
+        lv = [dict(lambda_min=a[6 + 6 * l], lambda_max=a[7 + 6 * l], a=a[8 + 6 * l], b=a[9 + 6 * l],
+                   ok=bool(a[10 + 6 * l]), blocks_ok=bool(a[11 + 6 * l])) for l in range(len(self.levels) - 1)]
+        return dict(ok=bool(a[1]), cycles=int(a[2]), last_coarse_iters=int(a[3]), coarse_iters=int(a[4]),
+                    coarse_ok=bool(a[5]), levels=lv)
+
+    def __del__(self):
+        try:
+            self.lib.hfk_pmg_destroy(self.handle)
+        except Exception:
+            pass
+
+
+@dataclass
+class SolveResult:
+    converged: bool
+    iterations: int
+    indefinite: bool
+    residual: float
+    rhs_norm: float
+
+
+class MatrixFreeSolver:
+    """Krylov solve on a prepared MatrixFreeJacobian (src/solvers.jl:33-174): ``krylov`` "cg" or "fcg" (flexible
+    CG); ``preconditioner`` None, "blockjacobi" or a PMultigrid."""
+
+    def __init__(self, krylov: str = "cg", rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 1000,
+                 preconditioner=None):
+        if krylov not in ("cg", "fcg"):
+            raise ValueError("HyperFEMKernels: MatrixFreeSolver: krylov must be 'cg' or 'fcg'")
+        if rtol < 0 or atol < 0 or maxiter < 0:
+            raise ValueError("HyperFEMKernels: MatrixFreeSolver: tolerances and maxiter must be >= 0")
+        self.krylov, self.rtol, self.atol, self.maxiter, self.preconditioner = krylov, rtol, atol, maxiter, preconditioner
+        self.last = None
+
+    def solve(self, x, mf: MatrixFreeJacobian, b) -> SolveResult:
+        op = mf.op
+        b = _vec(b, op.nfree)
+        pc = self.preconditioner
+        prec, g = (0, None) if pc is None else ((1, None) if pc == "blockjacobi" else (2, pc.handle))
+        st = np.zeros(3, dtype=np.int32)
+        rn = np.zeros(2)
+        check(op.lib.hfk_krylov_solve(op.handle, g, 0 if self.krylov == "cg" else 1, prec, ptr(b, C.c_double),
+                                      ptr(x, C.c_double), self.rtol, self.atol, self.maxiter, ptr(st, C.c_int32),
+                                      ptr(rn, C.c_double)), op.ctx.handle)
+        self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
+        return self.last

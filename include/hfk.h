@@ -103,6 +103,70 @@ int hfk_pattern(hfk_op* op, int64_t* colptr, int64_t* rowval, int32_t* nzpos);
 int hfk_jacobian(hfk_op* op, const double* x, double* nzval);
 int hfk_jacobian_dev(hfk_op* op, const double* x_dev, double* nzval_dev, void* stream);
 
+/* ---- BlockJacobi (src/preconditioners.jl:100-342) ---------------------------------------------------- */
+/* preconditioner_setup/update!: nodal 3x3 blocks of the prepared tangent (block_diagonal! route :tangents),
+ * inverted (fixed components -> identity).  ok = 0 when an inverted block is not finite (non-strict mode). */
+int hfk_block_jacobi_update(hfk_op* op, int* ok);
+/* ldiv!(y, blocks, r) (free rows of every block written) */
+int hfk_block_jacobi_apply(hfk_op* op, const double* r, double* y);
+int hfk_block_jacobi_apply_dev(hfk_op* op, const double* r_dev, double* y_dev, void* stream);
+/* the block numbering (first-seen order over cells and table nodes): nblocks, then 3*nblocks free ids (0 = fixed
+ * component) and the 9*nblocks inverted blocks [i + 3j] of the last update (either output may be NULL) */
+int hfk_block_jacobi_blocks(hfk_op* op, int64_t* nblocks, int32_t* dofs, double* Binv);
+
+/* ---- Chebyshev smoother (src/smoothers.jl) ----------------------------------------------------------- */
+typedef struct hfk_cheb hfk_cheb;
+/* Chebyshev(degree, range, inflate, eig_iters) on the block-preconditioned tangent of op */
+int hfk_cheb_create(hfk_op* op, int degree, double range, double inflate, int eig_iters, hfk_cheb** out);
+int hfk_cheb_destroy(hfk_cheb* ch);
+/* smoother_update!: blocks and interval at op's current tangents.  status[0] = ok, status[1] = indefinite;
+ * interval = (lambda_min, lambda_max, a, b) */
+int hfk_cheb_update(hfk_cheb* ch, int* status, double* interval);
+/* smooth!(x, st, rhs; zero_guess, residual) (src/smoothers.jl:116-146); residual may be NULL */
+int hfk_cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, int zero_guess, double* residual);
+int hfk_cheb_smooth_dev(hfk_cheb* ch, double* x_dev, const double* rhs_dev, int zero_guess, double* residual_dev,
+                        void* stream);
+/* eigenvalue_bounds(mf, blocks; k, b = chebyshev_start) (src/preconditioners.jl:374-389) */
+int hfk_eigenvalue_bounds(hfk_op* op, int k, double* lambda_min, double* lambda_max);
+
+/* ---- transfers between nested Q_k spaces on one mesh (src/transfers.jl) ------------------------------- */
+typedef struct hfk_transfer hfk_transfer;
+/* Prolongation(Uc, Uf): both operators on the same cells; errors when the Dirichlet sets differ */
+int hfk_transfer_create(hfk_op* coarse, hfk_op* fine, hfk_transfer** out);
+int hfk_transfer_destroy(hfk_transfer* T);
+int hfk_prolongate(hfk_transfer* T, const double* xc, double* yf);  /* prolongate! */
+int hfk_restrict(hfk_transfer* T, const double* yf, double* yc);    /* restrict! */
+int hfk_inject(hfk_transfer* T, const double* xf, const double* df, double* xc, double* dc);  /* inject! */
+
+/* ---- p-multigrid (src/pmultigrid.jl) ------------------------------------------------------------------ */
+typedef struct hfk_pmg hfk_pmg;
+/* PMultigrid(spaces; smoother = Chebyshev(...), coarse): levels[0] is the Newton's operator (prepared with
+ * hfk_mf_prepare*), levels[1..n-2] matrix-free, levels[n-1] assembled.  coarse_kind 0: block-Jacobi PCG on the
+ * assembled coarse matrix to coarse_rtol (at most coarse_maxiter iterations; a variable cycle, so the outer
+ * Krylov method should be the flexible CG).  The coarse matrix must be symmetric. */
+int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, double inflate, int eig_iters,
+                   int coarse_kind, double coarse_rtol, int coarse_maxiter, hfk_pmg** out);
+int hfk_pmg_destroy(hfk_pmg* g);
+/* update_levels! + smoother_update!: inject the state of levels[0] (its last prepare), re-prepare / re-assemble
+ * the levels, blocks and intervals.  status[0] = ok (preconditioner_ok), status[1] = indefinite */
+int hfk_pmg_update(hfk_pmg* g, int* status);
+/* ldiv!: one V-cycle z = M^-1 r (NaN-filled when not ok) */
+int hfk_pmg_vcycle(hfk_pmg* g, const double* r, double* z);
+int hfk_pmg_vcycle_dev(hfk_pmg* g, const double* r_dev, double* z_dev, void* stream);
+/* info[0..5] = nlevels, ok, cycles, inner coarse iterations of the last cycle, their total, coarse level ok;
+ * then per smoothed level (lambda_min, lambda_max, a, b, smoother ok, blocks ok): 6 + 6 (nlevels - 1) values */
+int hfk_pmg_info(hfk_pmg* g, double* info);
+
+/* ---- MatrixFreeSolver solve! (src/solvers.jl:111-158) ------------------------------------------------- */
+/* kind 0 = CG, 1 = flexible CG; precond 0 = none, 1 = op's BlockJacobi (updated by hfk_block_jacobi_update),
+ * 2 = one V-cycle of pmg.  From x = 0 to ||b - A x||_2 <= atol + rtol ||b||_2.  status = (converged,
+ * iterations, indefinite); rnorm = (true residual norm, ||b||).  A failed preconditioner (pmg not ok) returns at
+ * once with converged = 0. */
+int hfk_krylov_solve(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b, double* x, double rtol,
+                     double atol, int maxiter, int* status, double* rnorm);
+int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b_dev, double* x_dev,
+                         double rtol, double atol, int maxiter, int* status, double* rnorm, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

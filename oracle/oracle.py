@@ -196,6 +196,35 @@ class OracleOp:
         load().orc_jacobian(self.h, _p(x, C.c_double), _p(nzpos, C.c_int32), _p(nz, C.c_double), nnz)
         return nz
 
+    def csc(self, x):
+        """The assembled Jacobian at x as a scipy CSC matrix (pattern and values of the oracle)."""
+        import scipy.sparse as sp
+
+        colptr, rowval, nzpos = self.pattern()
+        nz = self.jacobian(x, nzpos, rowval.size)
+        return sp.csc_matrix((nz, rowval - 1, colptr - 1), shape=(self.nfree, self.nfree))
+
+    def node_blocks(self):
+        """field_blocks: (block_of[cell, table node], dofs[block, 3]) in first-seen order."""
+        nn = self.space.nn
+        block_of = np.zeros((self.ncells, nn), dtype=np.int32)
+        cap = self.ncells * nn
+        dofs = np.zeros((cap, 3), dtype=np.int32)
+        nb = C.c_int64()
+        rc = load().orc_node_blocks(self.h, _p(block_of, C.c_int32), _p(dofs, C.c_int32), cap, C.byref(nb))
+        assert rc == 0
+        return block_of, dofs[: nb.value].copy()
+
+    def blocks_from_tangents(self, prep):
+        """Inverted nodal blocks Binv[block, i + 3j] from the prepared tangents."""
+        T, iJt, dV = prep
+        block_of, dofs = self.node_blocks()
+        Binv = np.zeros((dofs.shape[0], 9))
+        load().orc_blocks_from_tangents(self.h, _p(T, C.c_double), _p(iJt, C.c_double), _p(dV, C.c_double),
+                                        _p(block_of, C.c_int32), _p(dofs, C.c_int32), dofs.shape[0],
+                                        _p(Binv, C.c_double))
+        return dofs, Binv
+
 
 def reference_coordinate_tangents(T, iJt, dV):
     """D[cell, q, (K,c) + 9 (M,k)] = dV sum_J sum_L iJt[J,K] T[(J,c),(L,k)] iJt[L,M] from the reference-layout
@@ -208,3 +237,129 @@ def reference_coordinate_tangents(T, iJt, dV):
     D = np.einsum("xqJK,xqJcLk,xqLM->xqKcMk", Bm, Tm, Bm) * dV[:, :, None, None, None, None]
     # flatten to i + 9 j with i = K + 3c, j = M + 3k: C-order reversed indices [k, M, c, K]
     return np.ascontiguousarray(np.transpose(D, (0, 1, 5, 4, 3, 2)).reshape(nc, npts, 81))
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Solver-layer restatements on assembled (dense or scipy) matrices, small problems only.
+
+def chebyshev_start(n):
+    """src/smoothers.jl:68-71"""
+    i = np.arange(1, n + 1, dtype=np.float64)
+    v = np.sin(1.0 + 0.7 * i) + 0.3 * np.cos(2.3 * i)
+    return v / np.linalg.norm(v)
+
+
+def block_inverse_matrix(dofs, Binv, n):
+    """The sparse matrix of ldiv!(y, blocks, r) (src/preconditioners.jl:327-342)."""
+    import scipy.sparse as sp
+
+    rows, cols, vals = [], [], []
+    for k in range(dofs.shape[0]):
+        for i in range(3):
+            if dofs[k, i] <= 0:
+                continue
+            for j in range(3):
+                if dofs[k, j] <= 0:
+                    continue
+                rows.append(dofs[k, i] - 1)
+                cols.append(dofs[k, j] - 1)
+                vals.append(Binv[k, i + 3 * j])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def arnoldi_bounds(Aop, start, k):
+    """eigenvalue_bounds (src/preconditioners.jl:374-389): k steps of Arnoldi (modified Gram-Schmidt) on the linear
+    map Aop from start; truncation at a near breakdown; extreme real parts of the Ritz values."""
+    n = start.size
+    k = min(k, n)
+    V = np.zeros((n, k + 1))
+    H = np.zeros((k + 1, k))
+    V[:, 0] = start / np.linalg.norm(start)
+    steps = k
+    for j in range(k):
+        w = Aop(V[:, j])
+        for i in range(j + 1):
+            H[i, j] = V[:, i] @ w
+            w = w - H[i, j] * V[:, i]
+        H[j + 1, j] = np.linalg.norm(w)
+        if not H[j + 1, j] > 0:
+            steps = j + 1
+            break
+        V[:, j + 1] = w / H[j + 1, j]
+    jj = steps
+    for i in range(1, steps):
+        if abs(H[i, i - 1]) <= 1e-8 * max(1.0, np.abs(H[:i, :i]).max()):
+            jj = i
+            break
+    ev = np.linalg.eigvals(H[:jj, :jj]).real
+    return ev.min(), ev.max()
+
+
+def chebyshev_smooth(A, Minv, x, rhs, a, b, degree, zero_guess=False, want_residual=False):
+    """smooth! (src/smoothers.jl:116-146) with matrices A and M^-1 (anything supporting @)."""
+    th, de = (b + a) / 2, (b - a) / 2
+    sg = th / de
+    x = np.zeros_like(rhs) if zero_guess else x.copy()
+    r = rhs.copy() if zero_guess else rhs - A @ x
+    z = Minv @ r
+    rho = 1 / sg
+    d = z / th
+    for i in range(1, degree + 1):
+        x = x + d
+        if i == degree and not want_residual:
+            break
+        r = r - A @ d
+        if i == degree:
+            break
+        z = Minv @ r
+        rn = 1 / (2 * sg - rho)
+        d = rn * rho * d + (2 * rn / de) * z
+        rho = rn
+    return (x, r) if want_residual else x
+
+
+def lagrange_embedding(Qc, Qf):
+    """E[tf, tc]: coarse tensor shape function tc at fine tensor node tf (equispaced nodes on [0,1]^3), from the
+    oracle's 1D Lagrange basis."""
+    NF, NC = Qf ** 3, Qc ** 3
+    E = np.zeros((NF, NC))
+    for tf in range(NF):
+        fi = (tf % Qf, (tf // Qf) % Qf, tf // (Qf * Qf))
+        vals = [lagrange1d(Qc, f / (Qf - 1))[0] for f in fi]
+        for tc in range(NC):
+            ci = (tc % Qc, (tc // Qc) % Qc, tc // (Qc * Qc))
+            E[tf, tc] = vals[0][ci[0]] * vals[1][ci[1]] * vals[2][ci[2]]
+    E[np.abs(E) < 1e-7] = 0.0
+    E[np.abs(E - 1) < 1e-7] = 1.0
+    return E
+
+
+def prolongation_matrix(coarse_space, fine_space):
+    """The matrix of prolongate! (src/transfers.jl:188-245): fine free dofs x coarse free dofs, insertion at shared
+    fine nodes (every cell writes the same row)."""
+    import scipy.sparse as sp
+
+    Qc, Qf = coarse_space.Q, fine_space.Q
+    E = lagrange_embedding(Qc, Qf)
+    rows = {}
+    NF, NC = Qf ** 3, Qc ** 3
+    for cell in range(fine_space.ncells):
+        fd, cd = fine_space.cell_dofs[cell], coarse_space.cell_dofs[cell]
+        for c in range(3):
+            for tf in range(NF):
+                idf = fd[fine_space.ldof[c, fine_space.nodeperm[tf]]]
+                if idf <= 0:
+                    continue
+                row = {}
+                for tc in range(NC):
+                    idc = cd[coarse_space.ldof[c, coarse_space.nodeperm[tc]]]
+                    if E[tf, tc] != 0 and idc > 0:
+                        row[idc - 1] = E[tf, tc]
+                rows[idf - 1] = row
+    r, cidx, v = [], [], []
+    for i, row in rows.items():
+        for j, e in row.items():
+            r.append(i)
+            cidx.append(j)
+            v.append(e)
+    return sp.csr_matrix((v, (r, cidx)), shape=(fine_space.nfree, coarse_space.nfree))

@@ -1,0 +1,173 @@
+"""GPU parity of the solver layer against the oracle and the reference's own solver tests.
+
+* BlockJacobi inverted blocks = the oracle's (same first-seen block numbering), 1e-12;
+* Arnoldi interval = the oracle's Arnoldi on the assembled J M^-1 from the same start vector;
+* Chebyshev smooth! = the oracle's recurrence with the same interval, 1e-12;
+* transfers: prolongation = the oracle's embedding matrix, restriction = its transpose, injection of a
+  prolongated field is the identity (test/solvers/test_transfers.jl);
+* p-multigrid FCG: same solution as a direct solve (<= 1e-7, test_pmultigrid.jl:37-51), at most 30
+  iterations, nearly independent of h; a NaN state makes the solve return not converged at once
+  (test_pmultigrid.jl:117-133).
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from hfk import HyperelasticForm, KernelOperator, MatrixFreeJacobian
+from hfk.mesh import dirichlet_closed_x0, hex_space
+from hfk.solvers import (BlockJacobi, Chebyshev, ChebyshevState, MatrixFreeSolver, PMultigrid, Prolongation,
+                         eigenvalue_bounds)
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def setup(hip_ctx, oracle, n=(3, 2, 2), order=2, law="neohookean", seed=0, amp=0.01):
+    sp = hex_space(n, order, dirichlet=dirichlet_closed_x0, perturb=0.1, seed=seed)
+    rng = np.random.default_rng(seed)
+    x = amp / order * rng.standard_normal(sp.nfree)
+    dv = amp / order * rng.standard_normal(sp.ndir)
+    form = HyperelasticForm(law, degree=2 * order)
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    mf = MatrixFreeJacobian(op).prepare(x)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, dvals=dv)
+    return sp, op, mf, orc, x
+
+
+def test_block_jacobi(hip_ctx, oracle):
+    sp, op, mf, orc, x = setup(hip_ctx, oracle)
+    bj = BlockJacobi(mf)
+    assert bj.ok
+    dofs, Binv = bj.blocks()
+    d2, B2 = orc.blocks_from_tangents(orc.prepare(x))
+    assert np.array_equal(dofs, d2)
+    assert rel(Binv, B2) < 1e-12
+    r = np.random.default_rng(1).standard_normal(sp.nfree)
+    y = bj.ldiv(np.zeros(sp.nfree), r)
+    assert rel(y, oracle.block_inverse_matrix(d2, B2, sp.nfree) @ r) < 1e-12
+
+
+def test_eigenvalue_bounds_and_smoother(hip_ctx, oracle):
+    sp, op, mf, orc, x = setup(hip_ctx, oracle, n=(2, 2, 2))
+    A = orc.csc(x)
+    d2, B2 = orc.blocks_from_tangents(orc.prepare(x))
+    Minv = oracle.block_inverse_matrix(d2, B2, sp.nfree)
+    lo_r, hi_r = oracle.arnoldi_bounds(lambda v: A @ (Minv @ v), oracle.chebyshev_start(sp.nfree), 10)
+    st = ChebyshevState(Chebyshev(degree=4), mf)
+    assert st.ok and not st.indefinite
+    assert st.lmax == pytest.approx(hi_r, rel=1e-8) and st.lmin == pytest.approx(lo_r, rel=1e-6)
+    assert st.b == pytest.approx(1.1 * st.lmax, rel=1e-15) and st.a == pytest.approx(st.b / 20, rel=1e-15)
+    lo, hi = eigenvalue_bounds(mf, 10)
+    assert hi == pytest.approx(hi_r, rel=1e-8)
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal(sp.nfree)
+    x0 = rng.standard_normal(sp.nfree)
+    for zero_guess in (True, False):
+        res = np.zeros(sp.nfree)
+        xg = st.smooth(x0.copy(), rhs, zero_guess=zero_guess, residual=res)
+        xr, rr = oracle.chebyshev_smooth(A, Minv, x0, rhs, st.a, st.b, 4, zero_guess=zero_guess, want_residual=True)
+        assert rel(xg, xr) < 1e-12 and rel(res, rr) < 1e-11
+        xg2 = st.smooth(x0.copy(), rhs, zero_guess=zero_guess)
+        assert rel(xg2, xr) < 1e-12
+
+
+@pytest.mark.parametrize("qc,qf", [(1, 2), (2, 3), (1, 3)])
+def test_transfers(hip_ctx, oracle, qc, qf):
+    n = (3, 2, 2) if qf < 3 else (2, 2, 1)
+    fs = hex_space(n, qf, dirichlet=dirichlet_closed_x0, perturb=0.1, seed=2)
+    cs = hex_space(n, qc, dirichlet=dirichlet_closed_x0, perturb=0.1, seed=2)
+    fop = KernelOperator(HyperelasticForm("svk", degree=2 * qf), fs, jacobian="matrixfree", ctx=hip_ctx)
+    cop = KernelOperator(HyperelasticForm("svk", degree=2 * qf), cs, jacobian="matrixfree", ctx=hip_ctx)
+    P = Prolongation(cop, fop)
+    Pm = oracle.prolongation_matrix(cs, fs)
+    rng = np.random.default_rng(3)
+    xc = rng.standard_normal(cs.nfree)
+    yf = rng.standard_normal(fs.nfree)
+    assert rel(P @ xc, Pm @ xc) < 1e-14
+    yc = P.restrict(np.zeros(cs.nfree), yf)
+    assert rel(yc, Pm.T @ yf) < 1e-13
+    if qc == 1:  # nested nodes: the injection of a prolongated field gives it back
+        xi, _ = P.inject(P @ xc, np.zeros(fs.ndir))
+        assert rel(xi, xc) < 1e-14
+
+
+def make_levels(hip_ctx, n, law="neohookean", seed=0, amp=0.01):
+    """Q2 working level and assembled Q1 coarse level on n^3 cells clamped on the closed face x = 0, at the
+    smooth random state of amplitude amp (1e-2 keeps |grad u| <= 0.19: a positive definite tangent)."""
+    from hfk.synthetic import smooth_field
+
+    fs = hex_space(n, 2, dirichlet=dirichlet_closed_x0)
+    cs = hex_space(n, 1, dirichlet=dirichlet_closed_x0)
+    u = smooth_field(seed + 2, amp=amp)
+    x = fs.free_values(u)
+    fop = KernelOperator(HyperelasticForm(law, degree=4), fs, jacobian="matrixfree",
+                         dirichlet_values=fs.dirichlet_values(u), ctx=hip_ctx)
+    # the coarse level keeps the fine quadrature (src/pmultigrid.jl:248-251)
+    cop = KernelOperator(HyperelasticForm(law, degree=4), cs, jacobian="assembled", ctx=hip_ctx)
+    mf = MatrixFreeJacobian(fop).prepare(x)
+    return fs, fop, cop, mf, x
+
+
+def test_pmultigrid_solution_equals_direct(hip_ctx, oracle):
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3).update()
+    assert pmg.ok
+    b = np.random.default_rng(11).standard_normal(fs.nfree)
+    sol = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg)
+    xs = np.zeros(fs.nfree)
+    res = sol.solve(xs, mf, b)
+    print("pmg fcg:", res, pmg.info(), flush=True)
+    assert res.converged and res.iterations <= 30
+    from hfk.synthetic import smooth_field
+
+    # the oracle's Jacobian at the same state and Dirichlet values
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    A = orc.csc(x)
+    xd = spla.spsolve(A.tocsc(), b)
+    assert rel(xs, xd) < 1e-7
+    # CG with the nodal blocks alone also reaches the solution (more iterations)
+    BlockJacobi(mf)
+    xs2 = np.zeros(fs.nfree)
+    res2 = MatrixFreeSolver("cg", rtol=1e-10, maxiter=2000, preconditioner="blockjacobi").solve(xs2, mf, b)
+    assert res2.converged and rel(xs2, xd) < 1e-7
+    assert res2.iterations > res.iterations
+
+
+def test_pmultigrid_h_independent(hip_ctx):
+    its = []
+    for n in (2, 4, 8):
+        fs, fop, cop, mf, x = make_levels(hip_ctx, (n, n, n))
+        pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-4).update()
+        b = np.random.default_rng(n).standard_normal(fs.nfree)
+        res = MatrixFreeSolver("fcg", rtol=1e-8, maxiter=100, preconditioner=pmg).solve(np.zeros(fs.nfree), mf, b)
+        assert res.converged
+        its.append(res.iterations)
+    print("iterations n=2,4,8:", its, flush=True)
+    assert max(its) <= 30 and its[2] <= its[1] + 6
+
+
+def test_nan_state_reports_failure(hip_ctx):
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (2, 2, 2))
+    xb = x.copy()
+    xb[5] = np.nan
+    mf.prepare(xb)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=2)).update()
+    assert not pmg.ok and not pmg.indefinite
+    res = MatrixFreeSolver("fcg", rtol=1e-8, maxiter=50, preconditioner=pmg).solve(np.zeros(fs.nfree), mf, np.ones(fs.nfree))
+    assert not res.converged and res.iterations == 0
+    z = pmg.ldiv(np.zeros(fs.nfree), np.ones(fs.nfree))
+    assert np.isnan(z).all()
+
+
+def test_indefinite_tangent_reported(hip_ctx):
+    """A large smooth state (|grad u| ~ 1) makes the tangent indefinite: the smoother finds a negative Ritz value,
+    the p-multigrid is not ok and reports indefinite, the solve returns at once (src/smoothers.jl:48,92-107,
+    src/solvers.jl:114-121)."""
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4), amp=0.05)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4)).update()
+    assert not pmg.ok and pmg.indefinite
+    res = MatrixFreeSolver("fcg", rtol=1e-8, maxiter=50, preconditioner=pmg).solve(np.zeros(fs.nfree), mf, np.ones(fs.nfree))
+    assert not res.converged and res.indefinite and res.iterations == 0
