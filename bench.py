@@ -63,6 +63,58 @@ def cpu_baseline(n=32, reps=3):
                       f"({sp.nfree} free dofs), min of {reps} after 1 warm-up, {t * 1e3:.1f} ms per Jv"}
 
 
+def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8):
+    """CG-type solve J(x) du = -R(x) to ||b - J du|| <= rtol ||b|| with one p-multigrid V-cycle per iteration:
+    Q2 -> assembled Q1 (fine quadrature), Chebyshev degree `degree` (range 20, inflate 1.1, 10 Arnoldi steps),
+    coarse block-Jacobi PCG to coarse_rtol; flexible CG outside (the inexact coarse solve makes the cycle vary).
+    Same law and state recipe as the Jv workload; both levels clamp the closed faces x = 0 and x = 1 (the p-MG
+    levels need one closed Dirichlet set).  Returns wall times of the update (prepare + levels + smoothers) and
+    of the solve, iterations and the final true residual."""
+    import numpy as np
+    import torch
+
+    from hfk import HyperelasticForm, KernelOperator, MatrixFreeJacobian
+    from hfk.mesh import dirichlet_closed_x_faces, hex_space
+    from hfk.solvers import Chebyshev, MatrixFreeSolver, PMultigrid
+    from hfk.synthetic import smooth_field
+
+    fs = hex_space((n, n, n), 2, dirichlet=dirichlet_closed_x_faces)
+    cs = hex_space((n, n, n), 1, dirichlet=dirichlet_closed_x_faces)
+    u = smooth_field(2)
+    x = fs.free_values(u)
+    form = HyperelasticForm("neohookean", 2.0, 1.0, degree=4)
+    fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u))
+    cop = KernelOperator(form, cs, jacobian="assembled")
+    b = -fop.residual(np.zeros(fs.nfree), x)
+    mf = MatrixFreeJacobian(fop, outer_storage)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=degree), coarse_rtol=coarse_rtol, coarse_maxiter=coarse_maxiter,
+                     smoother_storage=smoother_storage)
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(x).to(dev)
+    bd = torch.from_numpy(b).to(dev)
+    sol_d = torch.zeros_like(bd)
+    sptr = torch.cuda.current_stream(dev).cuda_stream
+    solver = MatrixFreeSolver("fcg", rtol=rtol, maxiter=200, preconditioner=pmg)
+    out = {}
+    for rep in range(2):  # the first pass also builds the coarse pattern and the work buffers
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mf.prepare_dev(xd.data_ptr(), sptr)
+        torch.cuda.synchronize()
+        pmg.update()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = solver.solve_dev(sol_d.data_ptr(), mf, bd.data_ptr(), sptr)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        info = pmg.info()
+        out = {"update_s": t1 - t0, "solve_s": t2 - t1, "iterations": res.iterations, "converged": res.converged,
+               "rel_residual": res.residual / res.rhs_norm, "coarse_iters_total": info["coarse_iters"],
+               "cycles": info["cycles"], "interval": [info["levels"][0]["a"], info["levels"][0]["b"]],
+               "free_dofs": fs.nfree, "coarse_free_dofs": cs.nfree}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +124,7 @@ def main():
     ap.add_argument("--storage", default="full64", choices=["full64", "sym64", "full32", "sym32"])
     ap.add_argument("--variants", default="sym64,full32,sym32", help="extra storages timed beside the headline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the Jv shards by cell partition; see DESIGN.md); "
@@ -177,6 +230,11 @@ def main():
                         "rel_err_vs_full64": float(np.linalg.norm(yv - y) / np.linalg.norm(y))}
         del mfv
     res["variants"] = variants
+    if not args.no_solve:
+        del mf
+        print("bench: solve legs (FP64 p-MG; mixed precision FP32 smoothers)", file=sys.stderr, flush=True)
+        res["solve_fp64_pmg"] = solve_leg(n, "full64", None)
+        res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32")
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

@@ -54,14 +54,19 @@ int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) 
 }
 
 // y = J dx (accumulate = false) or y += alpha J dx (accumulate = true)
-int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s, double alpha, bool accumulate) {
-  if (op->storage < 0)
+int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev, double* y_dev, hipStream_t s,
+                   double alpha, bool accumulate) {
+  if (storage < 0 || !T)
     return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: mul! before prepare! (no tangents stored)");
   if (!accumulate) HIPCHK(op->ctx, launch_fill(y_dev, op->nfree, 0.0, s));
   const OpDev d = op->dev();
   for (const BatchRange& r : op->color_batches)
-    HIPCHK(op->ctx, launch_apply(d, op->storage, op->tangents.p, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
+    HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
   return HFK_OK;
+}
+
+int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s, double alpha, bool accumulate) {
+  return op_run_apply_t(op, op->tangents.p, op->storage, dx_dev, y_dev, s, alpha, accumulate);
 }
 
 // sparsity (src/assembly.jl:773-818, full_keys!/cell_keys! :835-893) and cell_positions! (:864-880).

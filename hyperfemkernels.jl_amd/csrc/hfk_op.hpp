@@ -81,8 +81,11 @@ struct hfk_op {
   std::vector<int64_t> slot_of_cell;            // cell -> batch*C + position
   std::vector<int32_t> loc_of;                  // tensor index c*nn + t -> table-local dof
   hfk::DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64, state;
+  // a second copy of the tangents at the same state in another storage, for a smoother of another precision
+  hfk::DevBuf tangents_s;
   bool has_mu_field = false;
-  int storage = -1;  // storage of the prepared tangents, -1: not prepared
+  int storage = -1;    // storage of the prepared tangents, -1: not prepared
+  int storage_s = -1;  // storage of tangents_s
   bool pattern_built = false;
   std::vector<int64_t> colptr, rowval;
   std::vector<int32_t> nzpos;  // Gridap cell order [cell][ii + nd*jj]
@@ -117,6 +120,9 @@ namespace hfk {
 int op_bind(hfk_op* op);
 int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s);
 int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s, double alpha, bool accumulate);
+// the same with explicit tangents (op->tangents_s for a smoother of another precision)
+int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev, double* y_dev, hipStream_t s,
+                   double alpha, bool accumulate);
 int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s);
 int op_build_pattern(hfk_op* op);
 int op_build_blocks(hfk_op* op);

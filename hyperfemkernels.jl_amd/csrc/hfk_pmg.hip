@@ -135,15 +135,15 @@ struct Dots {
 
 // ---------------------------------------------------------------------------------------------------------
 // BlockJacobi on the stored tangents of op (block_diagonal!, src/preconditioners.jl:186-218)
-int blocks_update(hfk_op* op, hipStream_t s, bool* ok) {
+int blocks_update(hfk_op* op, const void* T, int storage, hipStream_t s, bool* ok) {
   int rc = op_build_blocks(op);
   if (rc) return rc;
-  if (op->storage < 0) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: BlockJacobi before prepare!");
+  if (storage < 0 || !T) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: BlockJacobi before prepare!");
   NodeBlocks& nb = op->blocks;
   HIPCHK(op->ctx, launch_fill(nb.B.as<double>(), nb.nblocks * 9, 0.0, s));
   const OpDev d = op->dev();
   for (const BatchRange& r : op->color_batches)
-    HIPCHK(op->ctx, launch_blocks(d, op->storage, op->tangents.p, nb.block_of.as<int32_t>(), nb.B.as<double>(), r, s));
+    HIPCHK(op->ctx, launch_blocks(d, storage, T, nb.block_of.as<int32_t>(), nb.B.as<double>(), r, s));
   HIPCHK(op->ctx, hipMemsetAsync(nb.bad.p, 0, sizeof(int), s));
   HIPCHK(op->ctx, blocks_invert(nb.nblocks, nb.dofs.as<int32_t>(), nb.B.as<double>(), nb.Binv.as<double>(), nb.bad.as<int>(), s));
   int bad = 0;
@@ -169,9 +169,12 @@ struct hfk_cheb {
   double range = 20, inflate = 1.1;
   double lmin = 0, lmax = 0, a = 0, b = 0;
   bool ok = false, blocks_ok = false;
+  bool secondary = false;  // apply the operator's second tangents (another precision) instead of its own
   int nbuild = 0;
   DevBuf r, d, tmp, start, V;
   Dots dots;
+  const void* T() const { return secondary ? op->tangents_s.p : op->tangents.p; }
+  int storage() const { return secondary ? op->storage_s : op->storage; }
 };
 
 namespace {
@@ -191,7 +194,7 @@ int arnoldi_bounds(hfk_op* op, hfk_cheb* ch, int k, hipStream_t s, double* lmin,
     double* vj = V + (size_t)j * n;
     double* w = V + (size_t)(j + 1) * n;
     if ((rc = blocks_apply(op, vj, ch->tmp.as<double>(), s))) return rc;
-    if ((rc = op_run_apply(op, ch->tmp.as<double>(), w, s, 1.0, false))) return rc;
+    if ((rc = op_run_apply_t(op, ch->T(), ch->storage(), ch->tmp.as<double>(), w, s, 1.0, false))) return rc;
     for (int i = 0; i <= j; ++i) {
       double hij;
       if ((rc = ch->dots.run(op->ctx, s, n, V + (size_t)i * n, w, &hij))) return rc;
@@ -240,7 +243,7 @@ int arnoldi_bounds(hfk_op* op, hfk_cheb* ch, int k, hipStream_t s, double* lmin,
 int cheb_update(hfk_cheb* ch, hipStream_t s) {
   hfk_op* op = ch->op;
   bool bok = false;
-  int rc = blocks_update(op, s, &bok);
+  int rc = blocks_update(op, ch->T(), ch->storage(), s, &bok);
   if (rc) return rc;
   ch->blocks_ok = bok;
   if (!bok) {
@@ -276,7 +279,7 @@ int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, dou
   if (zero_guess) {
     HIPCHK(op->ctx, launch_fill(x, n, 0.0, s));
   } else {
-    if ((rc = op_run_apply(op, x, r, s, -1.0, true))) return rc;  // r = rhs - A x
+    if ((rc = op_run_apply_t(op, ch->T(), ch->storage(), x, r, s, -1.0, true))) return rc;  // r = rhs - A x
   }
   // z = M^-1 r; d = z / theta; x += d
   HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, 0.0, 1.0 / th, s));
@@ -284,7 +287,7 @@ int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, dou
   const int k = ch->degree;
   for (int i = 1; i <= k; ++i) {
     if (i == k && !residual) break;
-    if ((rc = op_run_apply(op, d, r, s, -1.0, true))) return rc;  // r -= A d
+    if ((rc = op_run_apply_t(op, ch->T(), ch->storage(), d, r, s, -1.0, true))) return rc;  // r -= A d
     if (i == k) break;
     const double rn = 1.0 / (2.0 * sg - rho);
     HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, rn * rho, 2.0 * rn / de, s));
@@ -456,7 +459,8 @@ struct hfk_pmg {
   Dots dots;
   bool ok = false, indefinite = false, built = false, coarse_ok = false;
   int nupdate = 0, ncycle = 0, last_inner = 0, total_inner = 0;
-  int storage = 0;
+  int storage = 0;           // storage of the matrix-free coarse levels' tangents
+  int smoother_storage = -1;  // -1: the working operator's own tangents; else a copy in this storage
 };
 
 namespace {
@@ -555,6 +559,16 @@ int pmg_update(hfk_pmg* g, hipStream_t s) {
   g->nupdate++;
   int rc;
   bool ok = true;
+  // the working level's smoother: its own tangents, or a copy at the same state in the smoother's storage
+  const bool second = g->smoother_storage >= 0 && g->smoother_storage != f->storage;
+  if (second) {
+    const int st = g->smoother_storage;
+    const size_t bytes = (size_t)f->nbatches * store_entries(st) * f->C * f->np * store_bytes(st);
+    HIPCHK(g->ctx, f->tangents_s.alloc(bytes));
+    HIPCHK(g->ctx, launch_prepare(f->dev(), st, f->tangents_s.p, f->state.as<double>(), BatchRange{0, (int)f->nbatches}, s));
+    f->storage_s = st;
+  }
+  g->cheb[0]->secondary = second;
   for (size_t l = 1; l < g->ops.size(); ++l) {
     hfk_op* pf = g->ops[l - 1];
     hfk_op* pc = g->ops[l];
@@ -693,7 +707,7 @@ int hfk_block_jacobi_update(hfk_op* op, int* ok) {
   int rc = op_bind(op);
   if (rc) return rc;
   bool b = false;
-  if ((rc = blocks_update(op, op->ctx->stream, &b))) return rc;
+  if ((rc = blocks_update(op, op->tangents.p, op->storage, op->ctx->stream, &b))) return rc;
   if (ok) *ok = b ? 1 : 0;
   return HFK_OK;
 }
@@ -799,7 +813,7 @@ int hfk_eigenvalue_bounds(hfk_op* op, int k, double* lmin, double* lmax) {
   if ((rc = cheb_init(op, 2, 20.0, 1.1, std::max(k, 2), &ch))) return rc;
   if (!op->blocks.built) {
     bool ok;
-    if ((rc = blocks_update(op, op->ctx->stream, &ok))) return rc;
+    if ((rc = blocks_update(op, op->tangents.p, op->storage, op->ctx->stream, &ok))) return rc;
   }
   return arnoldi_bounds(op, &ch, k, op->ctx->stream, lmin, lmax);
 }
@@ -863,7 +877,7 @@ int hfk_inject(hfk_transfer* T, const double* xf, const double* df, double* xc, 
 
 // ---- p-multigrid ----------------------------------------------------------------------------------------
 int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, double inflate, int eig_iters,
-                   int coarse_kind, double coarse_rtol, int coarse_maxiter, hfk_pmg** out) {
+                   int smoother_storage, int coarse_kind, double coarse_rtol, int coarse_maxiter, hfk_pmg** out) {
   if (!levels || !out || nlevels < 2)
     return set_err(nullptr, HFK_EARG, "HyperFEMKernels: PMultigrid needs at least two levels");
   hfk_ctx* ctx = levels[0]->ctx;
@@ -872,8 +886,11 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
   if ((rc = check_cheb_args(ctx, degree, range, inflate, eig_iters))) return rc;
   if (coarse_kind != 0 || !(coarse_rtol > 0) || coarse_maxiter < 1)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unsupported coarse solver settings");
+  if (smoother_storage < -1 || smoother_storage > 3)
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unknown smoother storage");
   std::unique_ptr<hfk_pmg> g(new hfk_pmg());
   g->ctx = ctx;
+  g->smoother_storage = smoother_storage;
   g->coarse_kind = coarse_kind;
   g->coarse_rtol = coarse_rtol;
   g->coarse_maxiter = coarse_maxiter;
@@ -912,7 +929,7 @@ int hfk_pmg_update(hfk_pmg* g, int* status) {
   if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
   int rc = op_bind(g->ops[0]);
   if (rc) return rc;
-  g->storage = g->ops[0]->storage < 0 ? 0 : g->ops[0]->storage;
+  g->storage = g->smoother_storage >= 0 ? g->smoother_storage : (g->ops[0]->storage < 0 ? 0 : g->ops[0]->storage);
   if ((rc = pmg_update(g, g->ctx->stream))) return rc;
   if (status) {
     status[0] = g->ok ? 1 : 0;
@@ -976,7 +993,7 @@ int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const 
   if (op->storage < 0) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: solve! before prepare!");
   if (precond == 1 && !op->blocks.built) {
     bool ok;
-    if ((rc = blocks_update(op, op->stream(stream), &ok))) return rc;
+    if ((rc = blocks_update(op, op->tangents.p, op->storage, op->stream(stream), &ok))) return rc;
   }
   return krylov(op, pmg, kind, precond, b_dev, x_dev, rtol, atol, maxiter, status, rnorm, op->stream(stream));
 }

@@ -63,7 +63,7 @@ SIGNATURES = {
     "hfk_restrict": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_inject": (C.c_int, [_vp, _f64p, _f64p, _f64p, _f64p]),
     "hfk_pmg_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
-                                 C.c_double, C.c_int, C.POINTER(_vp)]),
+                                 C.c_int, C.c_double, C.c_int, C.POINTER(_vp)]),
     "hfk_pmg_destroy": (C.c_int, [_vp]),
     "hfk_pmg_update": (C.c_int, [_vp, _i32p]),
     "hfk_pmg_vcycle": (C.c_int, [_vp, _f64p, _f64p]),

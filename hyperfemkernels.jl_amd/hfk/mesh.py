@@ -100,6 +100,12 @@ def dirichlet_open_x_faces(I, J, K, L):
     return onx & inner
 
 
+def dirichlet_closed_x_faces(I, J, K, L):
+    """Nodes on the closed faces x = 0 and x = 1 (faces with their edges and vertices: the same closed set on
+    every p-multigrid level, which nested transfers require)."""
+    return (I == 0) | (I == L[0] - 1)
+
+
 def dirichlet_closed_x0(I, J, K, L):
     """Nodes on the closed face x = 0 (face, its edges and vertices)."""
     return I == 0

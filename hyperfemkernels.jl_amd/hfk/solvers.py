@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import check, ptr
+from ._lib import STORE, check, ptr
 from .operator import KernelOperator, MatrixFreeJacobian
 
 
@@ -166,15 +166,16 @@ class PMultigrid:
     finest first (the first is the Newton's, matrix-free; the last assembled)."""
 
     def __init__(self, levels, smoother: Chebyshev = Chebyshev(), coarse_rtol: float = 1e-2,
-                 coarse_maxiter: int = 500):
+                 coarse_maxiter: int = 500, smoother_storage: str | None = None):
         if len(levels) < 2:
             raise ValueError("HyperFEMKernels: PMultigrid needs at least two levels")
         self.levels = list(levels)
         arr = (C.c_void_p * len(levels))(*[op.handle.value for op in levels])
         h = C.c_void_p()
         lib = levels[0].lib
+        sst = -1 if smoother_storage is None else STORE[smoother_storage]
         check(lib.hfk_pmg_create(arr, len(levels), smoother.degree, smoother.range, smoother.inflate,
-                                 smoother.eig_iters, 0, float(coarse_rtol), int(coarse_maxiter), C.byref(h)),
+                                 smoother.eig_iters, sst, 0, float(coarse_rtol), int(coarse_maxiter), C.byref(h)),
               levels[0].ctx.handle)
         self.handle, self.lib, self.smoother = h, lib, smoother
         self.ok = self.indefinite = False
@@ -238,5 +239,18 @@ class MatrixFreeSolver:
         check(op.lib.hfk_krylov_solve(op.handle, g, 0 if self.krylov == "cg" else 1, prec, ptr(b, C.c_double),
                                       ptr(x, C.c_double), self.rtol, self.atol, self.maxiter, ptr(st, C.c_int32),
                                       ptr(rn, C.c_double)), op.ctx.handle)
+        self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
+        return self.last
+
+    def solve_dev(self, x_ptr: int, mf: MatrixFreeJacobian, b_ptr: int, stream: int = 0) -> SolveResult:
+        """The same on device vectors (raw addresses of nfree doubles); synchronises the stream at the end."""
+        op = mf.op
+        pc = self.preconditioner
+        prec, g = (0, None) if pc is None else ((1, None) if pc == "blockjacobi" else (2, pc.handle))
+        st = np.zeros(3, dtype=np.int32)
+        rn = np.zeros(2)
+        check(op.lib.hfk_krylov_solve_dev(op.handle, g, 0 if self.krylov == "cg" else 1, prec, C.c_void_p(b_ptr),
+                                          C.c_void_p(x_ptr), self.rtol, self.atol, self.maxiter, ptr(st, C.c_int32),
+                                          ptr(rn, C.c_double), C.c_void_p(stream)), op.ctx.handle)
         self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
         return self.last

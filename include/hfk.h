@@ -143,9 +143,12 @@ typedef struct hfk_pmg hfk_pmg;
 /* PMultigrid(spaces; smoother = Chebyshev(...), coarse): levels[0] is the Newton's operator (prepared with
  * hfk_mf_prepare*), levels[1..n-2] matrix-free, levels[n-1] assembled.  coarse_kind 0: block-Jacobi PCG on the
  * assembled coarse matrix to coarse_rtol (at most coarse_maxiter iterations; a variable cycle, so the outer
- * Krylov method should be the flexible CG).  The coarse matrix must be symmetric. */
+ * Krylov method should be the flexible CG).  The coarse matrix must be symmetric.  smoother_storage (an
+ * HFK_STORE_* value, or -1 for the working operator's own tangents) selects the tangents every smoother and
+ * matrix-free coarse level applies: HFK_STORE_SYM32 gives FP32 level applies and smoothers while the outer
+ * Krylov product keeps the working operator's FP64 tangents (mixed precision). */
 int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, double inflate, int eig_iters,
-                   int coarse_kind, double coarse_rtol, int coarse_maxiter, hfk_pmg** out);
+                   int smoother_storage, int coarse_kind, double coarse_rtol, int coarse_maxiter, hfk_pmg** out);
 int hfk_pmg_destroy(hfk_pmg* g);
 /* update_levels! + smoother_update!: inject the state of levels[0] (its last prepare), re-prepare / re-assemble
  * the levels, blocks and intervals.  status[0] = ok (preconditioner_ok), status[1] = indefinite */

@@ -136,6 +136,24 @@ def test_pmultigrid_solution_equals_direct(hip_ctx, oracle):
     assert res2.iterations > res.iterations
 
 
+@pytest.mark.parametrize("smoother_storage", ["sym32", "full32", "sym64"])
+def test_pmultigrid_mixed_precision(hip_ctx, oracle, smoother_storage):
+    """FP32 smoothers and level applies inside an FP64 flexible CG: the Krylov tolerance fixes the answer."""
+    from hfk.synthetic import smooth_field
+
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3, smoother_storage=smoother_storage).update()
+    assert pmg.ok
+    b = np.random.default_rng(12).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
+    print(f"mixed {smoother_storage}:", res, flush=True)
+    assert res.converged and res.iterations <= 30
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    xd = spla.spsolve(orc.csc(x).tocsc(), b)
+    assert rel(xs, xd) < 1e-7
+
+
 def test_pmultigrid_h_independent(hip_ctx):
     its = []
     for n in (2, 4, 8):
