@@ -184,6 +184,20 @@ def main():
         per = np.array([a.elapsed_time(b) * 1e-3 for a, b in evs])
         return mf, t_prep, per, wall
 
+    # a plain device copy of 1 GiB: the bandwidth an HBM-bound kernel can reach on this box (read + write bytes)
+    cpy_src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+    cpy_dst = torch.empty_like(cpy_src)
+    for _ in range(2):
+        cpy_dst.copy_(cpy_src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(5):
+        cpy_dst.copy_(cpy_src)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * cpy_src.numel() * 8 * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del cpy_src, cpy_dst
+
     mf, t_prep, per, wall = time_storage(args.storage, args.steps, args.warmup)
     y = yd.cpu().numpy()
     t_jv = float(per.mean())
@@ -210,7 +224,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "one Jv = 1 fill + 8 colour launches of k_apply<3,3>",
-                     "algorithmic_bytes": B, "stored_tangent_bytes": info["tangent_bytes"]},
+                     "algorithmic_bytes": B, "stored_tangent_bytes": info["tangent_bytes"],
+                     "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs},
         "prepare_ms": t_prep * 1e3,
         "jv_ms_min": float(per.min()) * 1e3,
         "jv_ms_median": float(np.median(per)) * 1e3,
@@ -218,17 +233,23 @@ def main():
         "checks": {"y_norm": float(np.linalg.norm(y)), "y_finite": bool(np.isfinite(y).all())},
     }
     variants = {}
-    for st in [s for s in args.variants.split(",") if s and s != args.storage]:
+    names = [s for s in args.variants.split(",") if s]
+    runs = [(st, True) for st in names if st != args.storage] + [(st, False) for st in [args.storage] + names]
+    for st, det in runs:
+        op.set_deterministic(det)
         mfv, tpv, perv, _ = time_storage(st, max(10, args.steps // 2), 2)
         yv = yd.cpu().numpy()
         tv = float(perv.mean())
         nent = 45 if st.startswith("sym") else 81
         s_t = 4 if st.endswith("32") else 8
         own = jv_bytes(ncells, nq, ndl, sp.nfree, s_t=s_t, nent=nent) - ncells * 10 * 8
-        variants[st] = {"gdofs": sp.nfree / tv / 1e9, "ms": tv * 1e3, "prepare_ms": tpv * 1e3,
-                        "own_bytes": own, "own_GBs": own / tv / 1e9, "own_frac": own / tv / 1e9 / HBM_PEAK_GBS,
-                        "rel_err_vs_full64": float(np.linalg.norm(yv - y) / np.linalg.norm(y))}
+        variants[st + ("" if det else "_atomic")] = {
+            "gdofs": sp.nfree / tv / 1e9, "ms": tv * 1e3, "prepare_ms": tpv * 1e3, "own_bytes": own,
+            "own_GBs": own / tv / 1e9, "own_frac": own / tv / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_frac": B / tv / 1e9 / HBM_PEAK_GBS,
+            "rel_err_vs_full64": float(np.linalg.norm(yv - y) / np.linalg.norm(y))}
         del mfv
+    op.set_deterministic(True)
     res["variants"] = variants
     if not args.no_solve:
         del mf

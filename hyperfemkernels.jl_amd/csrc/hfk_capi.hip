@@ -60,6 +60,10 @@ int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev,
     return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: mul! before prepare! (no tangents stored)");
   if (!accumulate) HIPCHK(op->ctx, launch_fill(y_dev, op->nfree, 0.0, s));
   const OpDev d = op->dev();
+  if (!op->deterministic) {
+    HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, BatchRange{0, 0}, s, accumulate ? alpha : 1.0, &op->amap));
+    return HFK_OK;
+  }
   for (const BatchRange& r : op->color_batches)
     HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
   return HFK_OK;
@@ -477,7 +481,29 @@ int hfk_op_create(hfk_ctx* ctx, int Q, int P, int64_t ncells, const int32_t* cel
   HIPCHK(ctx, hipMemcpy(op->loc_of_d.p, op->loc_of.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(ctx, op->dvals.alloc((size_t)std::max<int64_t>(ndir, 1) * sizeof(double)));
   HIPCHK(ctx, hipMemset(op->dvals.p, 0, (size_t)std::max<int64_t>(ndir, 1) * sizeof(double)));
+  // single-launch map of the atomic product: first batch and batch count per colour
+  {
+    std::vector<int> fc;
+    int maxc = 0;
+    for (const BatchRange& r : op->color_batches) {
+      fc.push_back(r.first);
+      fc.push_back(r.count);
+      maxc = std::max(maxc, r.count);
+    }
+    HIPCHK(ctx, op->colfc.alloc(fc.size() * sizeof(int)));
+    HIPCHK(ctx, hipMemcpy(op->colfc.p, fc.data(), fc.size() * sizeof(int), hipMemcpyHostToDevice));
+    op->amap.colfc = op->colfc.as<int>();
+    op->amap.ncol = (int)op->color_batches.size();
+    op->amap.gx = (maxc + 7) / 8;
+    op->amap.grid = 8 * op->amap.gx * op->amap.ncol;
+  }
   *out = op.release();
+  return HFK_OK;
+}
+
+int hfk_op_set_deterministic(hfk_op* op, int deterministic) {
+  if (!op) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  op->deterministic = deterministic != 0;
   return HFK_OK;
 }
 

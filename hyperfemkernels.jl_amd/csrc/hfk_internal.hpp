@@ -43,9 +43,19 @@ struct OpDev {
 
 // Launchers (hfk_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_prepare(const OpDev& op, int storage, void* tangents, const double* x, BatchRange r, hipStream_t s);
-// y[id] += alpha * (J dx)[id] over the batches r (the caller zeroes y first for a plain product)
+// Single-launch mode of the product: colfc[2c] = first batch and colfc[2c+1] = batch count of colour c (device),
+// gx = neighbourhoods per XCD, grid = 8 * gx * ncol work-groups.
+struct ApplyMap {
+  const int* colfc;
+  int ncol;
+  int gx;
+  int grid;
+};
+
+// y[id] += alpha * (J dx)[id] over the batches r of one colour (m == null), or over every batch in one launch
+// with atomic adds (m != null); the caller zeroes y first for a plain product
 hipError_t launch_apply(const OpDev& op, int storage, const void* tangents, const double* dx, double* y, BatchRange r,
-                        hipStream_t s, double alpha = 1.0);
+                        hipStream_t s, double alpha = 1.0, const ApplyMap* m = nullptr);
 // nodal 3x3 blocks of the tangent, accumulated into B[blk*9 + ci + 3cj] over the batches r of one colour
 hipError_t launch_blocks(const OpDev& op, int storage, const void* tangents, const int32_t* block_of, double* B,
                          BatchRange r, hipStream_t s);

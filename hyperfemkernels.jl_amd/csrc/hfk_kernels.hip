@@ -299,14 +299,32 @@ __device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* 
       s0[a] = in[a + P * jk];
       s1[a] = in[P * Q * Q + a + P * jk];
     }
+    double rl[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       double s = 0;
 #pragma unroll
       for (int a = 0; a < P; ++a) s += tb.N[a * Q + i] * s0[a] + tb.dN[a * Q + i] * s1[a];
-      emit(cell, c, i + Q * jk, s);
+      rl[i] = s;
     }
+    emit(cell, c, Q * jk, rl);  // the Q nodes t = Q jk + i of one x-line
   }
+}
+
+// y[id] += alpha * r for the Q nodes of one x-line of a cell: every load first, then every store (the ids of a
+// line are distinct, so the read-modify-writes are independent and their latencies overlap).
+template <int Q>
+__device__ __forceinline__ void scatter_line(const int32_t* __restrict__ id_line, const double* r, double alpha,
+                                             double* __restrict__ y) {
+  int id[Q];
+  double yv[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) id[i] = id_line[i];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) yv[i] = id[i] > 0 ? y[id[i] - 1] : 0.0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    if (id[i] > 0) y[id[i] - 1] = yv[i] + alpha * r[i];
 }
 
 // Geometry of the trilinear map at tensor point q (point_jacobian/point_geometry, src/assembly.jl:1073-1111):
@@ -425,19 +443,33 @@ __global__ __launch_bounds__(kThreads) void k_prepare(OpDev op, void* __restrict
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// mul!(y, mf, dx) for the batches of one colour: gather dx (0 at Dirichlet dofs), interpolate, sv = D su,
-// contract, add to the free rows of y (src/matrixfree.jl:341-396).
-template <int Q, int P, class TS, bool SYM>
+// Batch of a work-group in the single-launch (atomic) mode.  The hardware hands work-group b to XCD b % 8;
+// work-groups j = b / 8 of one XCD take the colours of one neighbourhood in turn: colour c = j % ncol of
+// neighbourhood g = xcd * gx + j / ncol (the g-th batch of every colour covers neighbouring cells, which share
+// nodes), so the dx entries they gather and the y entries they add to stay in that XCD's L2.  -1: idle.
+__device__ __forceinline__ int interleaved_batch(const int* __restrict__ colfc, int ncol, int gx) {
+  const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
+  const int c = j % ncol, g = xcd * gx + j / ncol;
+  return g < colfc[2 * c + 1] ? colfc[2 * c] + g : -1;
+}
+
+// mul!(y, mf, dx) (src/matrixfree.jl:341-396): gather dx (0 at Dirichlet dofs), interpolate, sv = D su,
+// contract, add to the free rows of y.  Deterministic mode (ATOMIC = false): the batches of one colour per
+// launch, plain read-modify-writes, the reference's colour order of the sums.  ATOMIC: every batch in one
+// launch, FP64 atomic adds (the order of the additions into one y entry varies from run to run).
+template <int Q, int P, class TS, bool SYM, bool ATOMIC>
 __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __restrict__ tangents,
                                                     const double* __restrict__ dx, double* __restrict__ y,
-                                                    int batch0, double alpha) {
+                                                    int batch0, double alpha, const int* __restrict__ colfc,
+                                                    int ncol, int gx) {
   using Dm = Dims<Q, P>;
   constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
   __shared__ double lds[Dm::LDS];
   double* A = lds;
   double* B = lds + C * SA;
   const int tid = threadIdx.x;
-  const int batch = batch0 + blockIdx.x;
+  const int batch = ATOMIC ? interleaved_batch(colfc, ncol, gx) : batch0 + blockIdx.x;
+  if (ATOMIC && batch < 0) return;  // uniform over the work-group
   const int32_t* ids = op.ids + (size_t)batch * C * 3 * Dm::NN;
   for (int it = tid; it < C * 3 * Dm::NN; it += kThreads) {
     const int id = ids[it];
@@ -474,9 +506,17 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
 #pragma unroll
     for (int i = 0; i < 9; ++i) s[i] = sv[i];
   }
-  contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t, double r) {
-    const int id = ids[cell * 3 * Dm::NN + c * Dm::NN + t];
-    if (id > 0) y[id - 1] += alpha * r;  // race-free: the cells of one colour share no free dof
+  contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t0, const double* r) {
+    const int32_t* idl = ids + cell * 3 * Dm::NN + c * Dm::NN + t0;
+    if (ATOMIC) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int id = idl[i];
+        if (id > 0) unsafeAtomicAdd(y + id - 1, alpha * r[i]);  // hardware FP64 add in L2 (device memory)
+      }
+    } else {
+      scatter_line<Q>(idl, r, alpha, y);  // race-free: the cells of one colour share no free dof
+    }
   });
 }
 
@@ -593,9 +633,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(OpDev op, const double* _
       }
     }
   }
-  contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t, double r) {
-    const int id = ids[cell * 3 * Dm::NN + c * Dm::NN + t];
-    if (id > 0) b[id - 1] += r;
+  contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t0, const double* r) {
+    scatter_line<Q>(ids + cell * 3 * Dm::NN + c * Dm::NN + t0, r, 1.0, b);
   });
 }
 
@@ -711,17 +750,26 @@ hipError_t prepare_law(const OpDev& op, int storage, void* T, const double* x, B
   }
 }
 
-template <int Q, int P>
-hipError_t apply_qp(const OpDev& op, int storage, const void* T, const double* dx, double* y, BatchRange r,
-                    hipStream_t s, double alpha) {
-  const dim3 g(r.count), b(kThreads);
+template <int Q, int P, bool ATOMIC>
+hipError_t apply_qpa(const OpDev& op, int storage, const void* T, const double* dx, double* y, BatchRange r,
+                     hipStream_t s, double alpha, const ApplyMap* m) {
+  const dim3 g(ATOMIC ? m->grid : r.count), b(kThreads);
+  const int* fc = ATOMIC ? m->colfc : nullptr;
+  const int nc = ATOMIC ? m->ncol : 0, gx = ATOMIC ? m->gx : 0;
   switch (storage) {
-    case 0: hipLaunchKernelGGL((k_apply<Q, P, double, false>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
-    case 1: hipLaunchKernelGGL((k_apply<Q, P, double, true>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
-    case 2: hipLaunchKernelGGL((k_apply<Q, P, float, false>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
-    default: hipLaunchKernelGGL((k_apply<Q, P, float, true>), g, b, 0, s, op, T, dx, y, r.first, alpha); break;
+    case 0: hipLaunchKernelGGL((k_apply<Q, P, double, false, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 1: hipLaunchKernelGGL((k_apply<Q, P, double, true, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 2: hipLaunchKernelGGL((k_apply<Q, P, float, false, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    default: hipLaunchKernelGGL((k_apply<Q, P, float, true, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
   }
   return hipGetLastError();
+}
+
+template <int Q, int P>
+hipError_t apply_qp(const OpDev& op, int storage, const void* T, const double* dx, double* y, BatchRange r,
+                    hipStream_t s, double alpha, const ApplyMap* m) {
+  return m ? apply_qpa<Q, P, true>(op, storage, T, dx, y, r, s, alpha, m)
+           : apply_qpa<Q, P, false>(op, storage, T, dx, y, r, s, alpha, m);
 }
 
 template <int Q, int P>
@@ -784,9 +832,9 @@ hipError_t launch_prepare(const OpDev& op, int storage, void* T, const double* x
 }
 
 hipError_t launch_apply(const OpDev& op, int storage, const void* T, const double* dx, double* y, BatchRange r,
-                        hipStream_t s, double alpha) {
-  if (r.count <= 0) return hipSuccess;
-#define CALL(q, p) apply_qp<q, p>(op, storage, T, dx, y, r, s, alpha)
+                        hipStream_t s, double alpha, const ApplyMap* m) {
+  if (!m && r.count <= 0) return hipSuccess;
+#define CALL(q, p) apply_qp<q, p>(op, storage, T, dx, y, r, s, alpha, m)
   HFK_DISPATCH_QP(op.Q, op.P, CALL);
 #undef CALL
 }

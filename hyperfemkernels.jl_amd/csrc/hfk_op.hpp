@@ -84,6 +84,11 @@ struct hfk_op {
   // a second copy of the tangents at the same state in another storage, for a smoother of another precision
   hfk::DevBuf tangents_s;
   bool has_mu_field = false;
+  // products: colour by colour with plain adds (the reference's summation order, bitwise reproducible), or one
+  // launch with atomic adds (amap)
+  bool deterministic = true;
+  hfk::DevBuf colfc;
+  hfk::ApplyMap amap{};
   int storage = -1;    // storage of the prepared tangents, -1: not prepared
   int storage_s = -1;  // storage of tangents_s
   bool pattern_built = false;

@@ -36,6 +36,7 @@ SIGNATURES = {
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_op_info": (C.c_int, [_vp, _i64p]),
     "hfk_op_colors": (C.c_int, [_vp, _i32p]),
+    "hfk_op_set_deterministic": (C.c_int, [_vp, C.c_int]),
     "hfk_residual": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_residual_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_mf_prepare": (C.c_int, [_vp, _f64p, C.c_int]),

@@ -78,7 +78,7 @@ class KernelOperator:
     """Residual and Jacobian of ``form`` on ``space`` (a ``mesh.HexSpace``), assembled on the GPU."""
 
     def __init__(self, form: HyperelasticForm, space, jacobian: str = "assembled", backend: str = "mi355x",
-                 dirichlet_values=None, ctx: Context | None = None):
+                 dirichlet_values=None, ctx: Context | None = None, deterministic: bool = True):
         if backend != "mi355x":
             raise ValueError(f"HyperFEMKernels: unknown backend {backend!r} (this build is mi355x only)")
         if jacobian not in ("assembled", "matrixfree"):
@@ -100,6 +100,9 @@ class KernelOperator:
               self.ctx.handle)
         self.handle = h
         self._pattern = None
+        self.deterministic = True
+        if not deterministic:
+            self.set_deterministic(False)
         if dirichlet_values is not None:
             self.set_dirichlet(dirichlet_values)
 
@@ -124,6 +127,13 @@ class KernelOperator:
         check(self.lib.hfk_op_info(self.handle, ptr(a, C.c_int64)), self.ctx.handle)
         keys = ["nfree", "ndir", "ncells", "nd", "np", "ncolors", "tangent_bytes", "nnz"]
         return dict(zip(keys, (int(v) for v in a)))
+
+    def set_deterministic(self, deterministic: bool = True):
+        """Products colour by colour (True: the reference's summation order, bitwise reproducible) or in one
+        launch with FP64 atomic adds (False: faster; the order of the additions varies between runs)."""
+        check(self.lib.hfk_op_set_deterministic(self.handle, int(bool(deterministic))), self.ctx.handle)
+        self.deterministic = bool(deterministic)
+        return self
 
     def colors(self) -> np.ndarray:
         c = np.zeros(self.space.ncells, dtype=np.int32)

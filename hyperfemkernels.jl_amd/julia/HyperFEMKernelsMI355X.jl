@@ -1,0 +1,155 @@
+# Julia side of the `backend = :mi355x` drop-in: thin `ccall` bindings of libhfk (include/hfk.h) and the methods
+# HyperFEMKernels.jl would route to them.  Not exercised in this repository (no Julia toolchain here); the same
+# C ABI is exercised by the Python mirror (hyperfemkernels.jl_amd/hfk) and its tests.
+#
+# Conventions: Gridap's cell dof ids are passed as they are (Int32, > 0 free, < 0 Dirichlet); table indices
+# (ldof, nodeperm) are passed 0-based (the 1-based Julia arrays minus one); CSC arrays come back 1-based, ready
+# for SparseMatrixCSC.
+module HyperFEMKernelsMI355X
+
+using SparseArrays
+
+const libhfk = get(ENV, "HFK_LIBHFK", joinpath(@__DIR__, "..", "hfk", "libhfk.so"))
+
+const HFK_STORE = Dict(:full64 => Cint(0), :sym64 => Cint(1), :full32 => Cint(2), :sym32 => Cint(3))
+const HFK_LAW = Dict(:svk => Cint(0), :neohookean => Cint(1), :linear => Cint(2))
+
+struct HFKError <: Exception
+    status::Cint
+    msg::String
+end
+
+function check(status::Cint, ctx::Ptr{Cvoid}=C_NULL)
+    status == 0 && return nothing
+    msg = unsafe_string(ccall((:hfk_last_error, libhfk), Cstring, (Ptr{Cvoid},), ctx))
+    status == -1 && throw(ArgumentError(msg))
+    status == -2 && throw(DimensionMismatch(msg))
+    status == -3 && error(msg)
+    throw(HFKError(status, msg))
+end
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+end
+function Context(device::Integer=0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hfk_open, libhfk), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, r))
+    c = Context(r[])
+    finalizer(c -> ccall((:hfk_close, libhfk), Cint, (Ptr{Cvoid},), c.ptr), c)
+    c
+end
+
+"Device operator of one volume group (what `KernelOperator(res, jac, U, V; backend = :mi355x)` builds)."
+mutable struct DeviceOperator
+    ptr::Ptr{Cvoid}
+    ctx::Context
+    nfree::Int
+end
+
+"""
+    DeviceOperator(ctx, cell_dofs, ldof, nodeperm, coords, nfree, ndir; law, λ, μ, P)
+
+`cell_dofs`: `Matrix{Int32}` nd × ncells from `cell_dof_matrix` (src/tables.jl:329-342); `ldof`: FieldTable.ldof
+(nn × 3, 1-based); `nodeperm`: TPTable.nodeperm (1-based); `coords`: 3 × 8 × ncells vertices in tensor order.
+"""
+function DeviceOperator(ctx::Context, cell_dofs::Matrix{Int32}, ldof::Matrix{Int}, nodeperm::Vector{Int},
+                        coords::Array{Float64,3}, nfree::Integer, ndir::Integer; law::Symbol=:neohookean,
+                        λ::Real=2.0, μ::Real=1.0, P::Integer)
+    nn = size(ldof, 1)
+    Q = round(Int, nn^(1 / 3))
+    ld = Int32[ldof[n, c] - 1 for n in 1:nn, c in 1:3]          # [c*nn + n] layout: column-major nn × 3
+    np_ = Int32.(nodeperm .- 1)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hfk_op_create, libhfk), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Int64, Int64, Cint,
+                 Cdouble, Cdouble, Ptr{Ptr{Cvoid}}),
+                ctx.ptr, Q, P, size(cell_dofs, 2), cell_dofs, ld, np_, coords, nfree, ndir, HFK_LAW[law], λ, μ, r), ctx.ptr)
+    op = DeviceOperator(r[], ctx, nfree)
+    finalizer(o -> ccall((:hfk_op_destroy, libhfk), Cint, (Ptr{Cvoid},), o.ptr), op)
+    op
+end
+
+set_dirichlet!(op::DeviceOperator, d::Vector{Float64}) =
+    check(ccall((:hfk_op_set_dirichlet, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, d), op.ctx.ptr)
+
+"Algebra.residual!(b, op, x) (src/assembly.jl:909-916)"
+function residual!(b::Vector{Float64}, op::DeviceOperator, x::Vector{Float64})
+    length(x) == op.nfree == length(b) || throw(DimensionMismatch("HyperFEMKernels: vector lengths"))
+    check(ccall((:hfk_residual, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op.ptr, x, b), op.ctx.ptr)
+    b
+end
+
+"sparsity_pattern(op) and nonzero_positions(op) (src/assembly.jl:768-771)"
+function sparsity_pattern(op::DeviceOperator, nd::Integer, ncells::Integer)
+    nnz = Ref{Int64}(0)
+    check(ccall((:hfk_pattern_nnz, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int64}), op.ptr, nnz), op.ctx.ptr)
+    colptr = Vector{Int64}(undef, op.nfree + 1)
+    rowval = Vector{Int64}(undef, nnz[])
+    nzpos = Matrix{Int32}(undef, nd * nd, ncells)
+    check(ccall((:hfk_pattern, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}), op.ptr, colptr, rowval,
+                nzpos), op.ctx.ptr)
+    SparseMatrixCSC(op.nfree, op.nfree, colptr, rowval, zeros(nnz[])), nzpos
+end
+
+"jacobian!(A, op, x) (src/assembly.jl:918-927): the nonzeros of A overwritten"
+function jacobian!(A::SparseMatrixCSC{Float64,Int}, op::DeviceOperator, x::Vector{Float64})
+    check(ccall((:hfk_jacobian, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op.ptr, x, nonzeros(A)), op.ctx.ptr)
+    A
+end
+
+"MatrixFreeJacobian on the device: prepare!(mf, x) and mul!(y, mf, dx) (src/matrixfree.jl:140-260)"
+mutable struct DeviceMatrixFree
+    op::DeviceOperator
+    storage::Symbol
+end
+function prepare!(mf::DeviceMatrixFree, x::Vector{Float64})
+    check(ccall((:hfk_mf_prepare, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), mf.op.ptr, x, HFK_STORE[mf.storage]),
+          mf.op.ctx.ptr)
+    mf
+end
+function mul!(y::Vector{Float64}, mf::DeviceMatrixFree, dx::Vector{Float64})
+    check(ccall((:hfk_mf_apply, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), mf.op.ptr, dx, y), mf.op.ctx.ptr)
+    y
+end
+
+"PMultigrid(spaces; smoother = Chebyshev(degree, range, inflate, eig_iters)) on device operators, finest first"
+mutable struct DevicePMultigrid
+    ptr::Ptr{Cvoid}
+    levels::Vector{DeviceOperator}
+end
+function DevicePMultigrid(levels::Vector{DeviceOperator}; degree=2, range=20.0, inflate=1.1, eig_iters=10,
+                          smoother_storage::Union{Nothing,Symbol}=nothing, coarse_rtol=1e-2, coarse_maxiter=500)
+    ptrs = [l.ptr for l in levels]
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    sst = smoother_storage === nothing ? Cint(-1) : HFK_STORE[smoother_storage]
+    check(ccall((:hfk_pmg_create, libhfk), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Cint, Cdouble, Cdouble, Cint, Cint, Cint, Cdouble, Cint, Ptr{Ptr{Cvoid}}),
+                ptrs, length(levels), degree, range, inflate, eig_iters, sst, 0, coarse_rtol, coarse_maxiter, r),
+          levels[1].ctx.ptr)
+    g = DevicePMultigrid(r[], levels)
+    finalizer(g -> ccall((:hfk_pmg_destroy, libhfk), Cint, (Ptr{Cvoid},), g.ptr), g)
+    g
+end
+"preconditioner_update!: returns (ok, indefinite)"
+function update!(g::DevicePMultigrid)
+    st = zeros(Cint, 2)
+    check(ccall((:hfk_pmg_update, libhfk), Cint, (Ptr{Cvoid}, Ptr{Cint}), g.ptr, st), g.levels[1].ctx.ptr)
+    (st[1] == 1, st[2] == 1)
+end
+
+"MatrixFreeSolver solve! (src/solvers.jl:111-158): returns (converged, iterations, indefinite, residual norm)"
+function solve!(x::Vector{Float64}, mf::DeviceMatrixFree, b::Vector{Float64}; pmg::Union{Nothing,DevicePMultigrid}=nothing,
+                krylov::Symbol=:fcg, rtol=1e-8, atol=0.0, maxiter=1000)
+    st = zeros(Cint, 3)
+    rn = zeros(2)
+    prec = pmg === nothing ? Cint(1) : Cint(2)
+    pmg === nothing && check(ccall((:hfk_block_jacobi_update, libhfk), Cint, (Ptr{Cvoid}, Ptr{Cint}), mf.op.ptr, st),
+                             mf.op.ctx.ptr)
+    check(ccall((:hfk_krylov_solve, libhfk), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cint, Ptr{Cint}, Ptr{Float64}),
+                mf.op.ptr, pmg === nothing ? C_NULL : pmg.ptr, krylov === :cg ? 0 : 1, prec, b, x, rtol, atol, maxiter, st, rn),
+          mf.op.ctx.ptr)
+    (st[1] == 1, Int(st[2]), st[3] == 1, rn[1])
+end
+
+end # module

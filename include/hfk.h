@@ -78,6 +78,10 @@ int hfk_op_set_mu_field(hfk_op* op, const double* mu_points);
 int hfk_op_info(hfk_op* op, int64_t* info);
 /* Colour of each cell, 1-based (the reference's `color[cell]`, src/coloring.jl:4-28). */
 int hfk_op_colors(hfk_op* op, int32_t* color_of_cell);
+/* Matrix-free products colour by colour (1, the default: the reference's summation order, bitwise
+ * reproducible, src/matrixfree.jl:246-260) or in one launch with FP64 atomic adds (0: faster, the order of the
+ * additions into one entry varies between runs). */
+int hfk_op_set_deterministic(hfk_op* op, int deterministic);
 
 /* ---- residual: Algebra.residual!(b, op, x)  (src/assembly.jl:909-916, 1219-1241) ----------------- */
 int hfk_residual(hfk_op* op, const double* x, double* b);

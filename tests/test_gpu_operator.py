@@ -114,6 +114,25 @@ def test_pattern_and_assembly(oracle, hip_ctx, order, law, n):
     assert rel(mf @ dx, A @ dx) < 1e-11
 
 
+@pytest.mark.parametrize("order,P,storage", [(2, 3, "full64"), (2, 3, "sym32"), (1, 2, "full64"), (3, 4, "sym64")])
+def test_atomic_single_launch(oracle, hip_ctx, order, P, storage):
+    """The single-launch product with atomic adds equals the oracle to the same bar (its summation order differs
+    from the colour order, so only to rounding)."""
+    sp, dv, x = problem(order, P, "neohookean", (5, 4, 3) if order < 3 else (3, 2, 2), 0.1)
+    op = KernelOperator(HyperelasticForm("neohookean", degree=2 * (P - 1)), sp, jacobian="matrixfree",
+                        dirichlet_values=dv, ctx=hip_ctx, deterministic=False)
+    orc = oracle.OracleOp(sp, "neohookean", 2.0, 1.0, points1d=P, dvals=dv)
+    dx = np.random.default_rng(17).standard_normal(sp.nfree)
+    yref = orc.jv(orc.prepare(x), dx)
+    mf = MatrixFreeJacobian(op, storage).prepare(x)
+    tol = TOL if storage.endswith("64") else 1e-5
+    y1 = mf @ dx
+    assert rel(y1, yref) < tol
+    op.set_deterministic(True)
+    y2 = mf @ dx
+    assert rel(y1, y2) < (1e-14 if storage.endswith("64") else 1e-12)
+
+
 def test_errors(hip_ctx):
     sp, dv, x = problem(2, 3, "svk", (2, 2, 2))
     op = KernelOperator(HyperelasticForm("svk"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
