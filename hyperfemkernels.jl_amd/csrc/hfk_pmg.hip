@@ -9,6 +9,7 @@
 #include <memory>
 #include <vector>
 
+#include "hfk_net.hpp"
 #include "hfk_op.hpp"
 #include "hfk_solver.hpp"
 
@@ -451,6 +452,8 @@ struct hfk_pmg {
   std::vector<std::unique_ptr<hfk_cheb>> cheb;   // smoothed levels 0..L-2
   std::vector<std::unique_ptr<DevBuf>> r, e, bc, xc;
   int coarse_kind = 0, coarse_maxiter = 500;
+  hfk_net* net = nullptr;  // learned coarse correction (not owned)
+  int net_mode = 0;
   double coarse_rtol = 1e-2;
   // coarse CSR (CSC of A, 0-based int32)
   DevBuf colptr, rowval, nzval;
@@ -497,7 +500,7 @@ int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
 }
 
 // coarse solve: block-Jacobi PCG on the assembled coarse matrix from zero to coarse_rtol (relative)
-int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s) {
+int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool allow_net, double rtol, int maxiter) {
   hfk_op* c = g->ops.back();
   const int64_t n = c->nfree;
   NodeBlocks& nb = c->blocks;
@@ -507,28 +510,35 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s) {
   double* q = g->cq.as<double>();
   double* sc = g->cs.as<double>();
   int rc;
-  HIPCHK(g->ctx, launch_fill(x, n, 0.0, s));
-  HIPCHK(g->ctx, vec_copy(n, b, r, s));
+  g->last_inner = 0;
+  if (allow_net && g->net && g->net_mode > 0) {  // learned coarse correction
+    if ((rc = net_apply_dev(g->net, b, x, s))) return rc;
+    if (g->net_mode == 2) return HFK_OK;
+    HIPCHK(g->ctx, vec_copy(n, b, r, s));  // r = b - A x0
+    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), x, r, -1.0, 1.0, s));
+  } else {
+    HIPCHK(g->ctx, launch_fill(x, n, 0.0, s));
+    HIPCHK(g->ctx, vec_copy(n, b, r, s));
+  }
   HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, z, nullptr, 0.0, 1.0, s));
   HIPCHK(g->ctx, vec_copy(n, z, p, s));
   double h[3];
-  if ((rc = g->dots.run(g->ctx, s, n, r, z, h, b, b))) return rc;
+  if ((rc = g->dots.run(g->ctx, s, n, r, z, h, b, b, r, r))) return rc;
   const double bn = std::sqrt(h[1]);
   HIPCHK(g->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZ, s));
-  g->last_inner = 0;
-  if (bn == 0) return HFK_OK;
-  for (int it = 1; it <= g->coarse_maxiter; ++it) {
+  if (bn == 0 || !(std::sqrt(h[2]) > rtol * bn)) return HFK_OK;
+  for (int it = 1; it <= maxiter; ++it) {
     HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), p, q, 1.0, 0.0, s));
     HIPCHK(g->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kPQ, s));
     HIPCHK(g->ctx, cg_update(n, sc, p, q, x, r, s));
     HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, z, nullptr, 0.0, 1.0, s));
     HIPCHK(g->ctx, vec_dots(n, r, z, r, r, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZN, s));
     g->last_inner = it;
-    if (it % 5 == 0 || it == g->coarse_maxiter) {
+    if (it % 5 == 0 || it == maxiter) {
       double rr[2];
       HIPCHK(g->ctx, hipMemcpyAsync(rr, sc + kRZN, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(g->ctx, hipStreamSynchronize(s));
-      if (!(std::sqrt(rr[1]) > g->coarse_rtol * bn)) break;  // converged (or NaN: leave it to the outer check)
+      if (!(std::sqrt(rr[1]) > rtol * bn)) break;  // converged (or NaN: leave it to the outer check)
     }
     HIPCHK(g->ctx, cg_direction(n, sc, z, p, 0, s));
   }
@@ -543,7 +553,9 @@ int vcycle(hfk_pmg* g, size_t l, double* x, const double* b, hipStream_t s) {
   if ((rc = cheb_smooth(S, x, b, true, g->r[l]->as<double>(), s))) return rc;
   if ((rc = do_restrict(g->P[l].get(), g->r[l]->as<double>(), g->bc[l]->as<double>(), s))) return rc;
   if (l + 2 == g->ops.size()) {
-    if ((rc = coarse_solve(g, g->bc[l]->as<double>(), g->xc[l]->as<double>(), s))) return rc;
+    if ((rc = coarse_solve(g, g->bc[l]->as<double>(), g->xc[l]->as<double>(), s, true, g->coarse_rtol,
+                           g->coarse_maxiter)))
+      return rc;
   } else {
     if ((rc = vcycle(g, l + 1, g->xc[l]->as<double>(), g->bc[l]->as<double>(), s))) return rc;
   }
@@ -979,6 +991,42 @@ int hfk_pmg_info(hfk_pmg* g, double* info) {
     o[4] = g->cheb[l]->ok ? 1.0 : 0.0;
     o[5] = g->cheb[l]->blocks_ok ? 1.0 : 0.0;
   }
+  return HFK_OK;
+}
+
+int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode) {
+  if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  if (mode < 0 || mode > 2) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: coarse network mode must be 0, 1 or 2");
+  if (mode > 0 && (!net || net_ncoarse(net) != g->ops.back()->nfree))
+    return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: the network is not built for this coarse level");
+  g->net = mode > 0 ? net : nullptr;
+  g->net_mode = mode;
+  return HFK_OK;
+}
+
+// Training pairs from the plain cycle (two-level view: the working level and the assembled coarse level): the
+// pre-smoothed residual of b_k restricted to the coarse level and its coarse solution.
+int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, double* bc_dev, double* ec_dev, double rtol,
+                             int maxiter, void* stream) {
+  if (!g || !b_dev || !bc_dev || !ec_dev || nsamples < 1 || !(rtol > 0) || maxiter < 1)
+    return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: invalid arguments");
+  if (g->ops.size() != 2) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: coarse pairs need a two-level hierarchy");
+  if (!g->built || !g->ok) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid not updated or not ok");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  hipStream_t s = g->ops[0]->stream(stream);
+  const int64_t nf = g->ops[0]->nfree, nc = g->ops[1]->nfree;
+  DevBuf xw;
+  if ((rc = vec_alloc(g->ctx, xw, nf))) return rc;
+  for (int k = 0; k < nsamples; ++k) {
+    const double* b = b_dev + (size_t)k * nf;
+    double* bc = bc_dev + (size_t)k * nc;
+    double* ec = ec_dev + (size_t)k * nc;
+    if ((rc = cheb_smooth(g->cheb[0].get(), xw.as<double>(), b, true, g->r[0]->as<double>(), s))) return rc;
+    if ((rc = do_restrict(g->P[0].get(), g->r[0]->as<double>(), bc, s))) return rc;
+    if ((rc = coarse_solve(g, bc, ec, s, false, rtol, maxiter))) return rc;
+  }
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
   return HFK_OK;
 }
 

@@ -1,0 +1,94 @@
+"""Learned coarse correction (no counterpart in the reference): host side over libhfk.
+
+``CoarseNetwork`` wraps the device U-Net of hfk_net.hip (BF16 on MFMA, FP32 master weights, trained on the
+device with the library's own forward/backward kernels and Adam).  It acts on the assembled Q1 level of a
+``PMultigrid``: ``PMultigrid.set_coarse_network(net, mode)`` makes it the coarse correction (mode "replace") or the
+initial guess of the coarse PCG (mode "warmstart"); the outer flexible CG's tolerance fixes the answer either way.
+Training pairs come from the plain p-multigrid itself (``coarse_pairs``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, ptr
+
+MODES = {"off": 0, "warmstart": 1, "replace": 2}
+
+
+class CoarseNetwork:
+    """U-Net on the vertex lattice of a Cartesian n^3 Q1 space (``coarse_space``: mesh.HexSpace of order 1)."""
+
+    def __init__(self, ctx, coarse_space, seed: int = 0):
+        n = coarse_space.n
+        if not (n[0] == n[1] == n[2]) or coarse_space.order != 1:
+            raise ValueError("HyperFEMKernels: the coarse network needs a cubic Q1 lattice")
+        self.ctx, self.lib = ctx, ctx.lib
+        self.N = n[0] + 1
+        self.ncoarse = coarse_space.nfree
+        dofs = np.ascontiguousarray(coarse_space.lattice_dofs(), dtype=np.int32)
+        h = (coarse_space.domain[1] - coarse_space.domain[0]) / n[0]
+        hd = C.c_void_p()
+        check(self.lib.hfk_net_create(ctx.handle, self.N, ptr(dofs, C.c_int32), self.ncoarse, h, seed, C.byref(hd)),
+              ctx.handle)
+        self.handle = hd
+        self.h = h
+        self.nparams = int(self.lib.hfk_net_num_params(hd))
+
+    def weights(self) -> np.ndarray:
+        w = np.zeros(self.nparams, dtype=np.float32)
+        check(self.lib.hfk_net_get_weights(self.handle, w.ctypes.data_as(C.POINTER(C.c_float))), self.ctx.handle)
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.nparams:
+            raise ValueError("HyperFEMKernels: wrong number of network weights")
+        check(self.lib.hfk_net_set_weights(self.handle, w.ctypes.data_as(C.POINTER(C.c_float))), self.ctx.handle)
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.ncoarse)
+        check(self.lib.hfk_net_apply(self.handle, ptr(r, C.c_double), ptr(z, C.c_double)), self.ctx.handle)
+        return z
+
+    def gradient(self, r, e):
+        """(loss, dloss/dweights) summed over the samples (rows of r and e)."""
+        r = np.ascontiguousarray(np.atleast_2d(r), dtype=np.float64)
+        e = np.ascontiguousarray(np.atleast_2d(e), dtype=np.float64)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        check(self.lib.hfk_net_gradient(self.handle, ptr(r, C.c_double), ptr(e, C.c_double), r.shape[0],
+                                        g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)), self.ctx.handle)
+        return loss.value, g
+
+    def train_dev(self, r_ptr: int, e_ptr: int, nsamples: int, steps: int, batch: int = 4, lr: float = 1e-3,
+                  seed: int = 0, stream: int = 0) -> np.ndarray:
+        """Adam on device-resident pairs ([k][ncoarse] doubles at r_ptr/e_ptr); returns the loss per step."""
+        hist = np.zeros(max(steps, 1))
+        check(self.lib.hfk_net_train_dev(self.handle, C.c_void_p(r_ptr), C.c_void_p(e_ptr), nsamples, steps, batch,
+                                         lr, seed, ptr(hist, C.c_double), C.c_void_p(stream)), self.ctx.handle)
+        return hist[:steps]
+
+    def __del__(self):
+        try:
+            self.lib.hfk_net_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def coarse_pairs_dev(pmg, b_ptr: int, nsamples: int, bc_ptr: int, ec_ptr: int, rtol: float = 1e-10,
+                     maxiter: int = 5000, stream: int = 0):
+    """Training pairs from the plain p-multigrid (device arrays [k][n])."""
+    check(pmg.lib.hfk_pmg_coarse_pairs_dev(pmg.handle, C.c_void_p(b_ptr), nsamples, C.c_void_p(bc_ptr),
+                                           C.c_void_p(ec_ptr), rtol, maxiter, C.c_void_p(stream)),
+          pmg.levels[0].ctx.handle)
+
+
+def set_coarse_network(pmg, net: CoarseNetwork | None, mode: str = "warmstart"):
+    if mode not in MODES:
+        raise ValueError(f"HyperFEMKernels: unknown coarse network mode {mode!r}")
+    check(pmg.lib.hfk_pmg_set_coarse_network(pmg.handle, None if net is None else net.handle, MODES[mode]),
+          pmg.levels[0].ctx.handle)
+    pmg.coarse_network = (net, mode)

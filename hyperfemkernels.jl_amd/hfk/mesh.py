@@ -86,6 +86,15 @@ class HexSpace:
         v = np.asarray(fun(self.node_xyz[self.dir_node_of]), dtype=np.float64).reshape(-1, 3)
         return v.reshape(-1).copy()
 
+    def lattice_dofs(self) -> np.ndarray:
+        """Signed dof ids of every lattice node and component, (nnodes, 3), lattice order (x fastest)."""
+        nn = self.node_xyz.shape[0]
+        out = np.zeros((nn, 3), dtype=np.int32)
+        for c in range(3):
+            out[self.free_node_of, c] = 3 * np.arange(self.free_node_of.size) + c + 1
+            out[self.dir_node_of, c] = -(3 * np.arange(self.dir_node_of.size) + c + 1)
+        return out
+
     def free_values(self, fun) -> np.ndarray:
         v = np.asarray(fun(self.node_xyz[self.free_node_of]), dtype=np.float64).reshape(-1, 3)
         return v.reshape(-1).copy()

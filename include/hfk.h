@@ -164,6 +164,36 @@ int hfk_pmg_vcycle_dev(hfk_pmg* g, const double* r_dev, double* z_dev, void* str
  * then per smoothed level (lambda_min, lambda_max, a, b, smoother ok, blocks ok): 6 + 6 (nlevels - 1) values */
 int hfk_pmg_info(hfk_pmg* g, double* info);
 
+/* ---- learned coarse correction (no counterpart in the reference) ------------------------------------- */
+/* A 4-level 3D convolutional U-Net (channels 16-32-64-64, bias-free 3x3x3 convolutions with ReLU, fixed
+ * full-weighting restriction and trilinear prolongation between levels, skip concatenations), BF16 weights and
+ * activations on MFMA with FP32 accumulation and FP32 master weights.  It maps a residual of the assembled Q1
+ * coarse level to an approximation of its coarse-level error: z = (s/h) N(r/s), s = RMS of r, h = cell size.
+ *   n_lattice  vertices per direction of the Q1 lattice (2^3 m + 1, e.g. 17, 33, 65)
+ *   node_dofs  n_lattice^3 * 3 signed coarse dof ids per vertex and component (x fastest; > 0 free) */
+typedef struct hfk_net hfk_net;
+int hfk_net_create(hfk_ctx* ctx, int n_lattice, const int32_t* node_dofs, int64_t ncoarse, double h, uint64_t seed,
+                   hfk_net** out);
+int hfk_net_destroy(hfk_net* net);
+int64_t hfk_net_num_params(const hfk_net* net);
+int hfk_net_get_weights(hfk_net* net, float* w);
+int hfk_net_set_weights(hfk_net* net, const float* w);
+int hfk_net_apply(hfk_net* net, const double* r, double* z);
+int hfk_net_apply_dev(hfk_net* net, const double* r_dev, double* z_dev, void* stream);
+/* loss (sum over samples of ||z_k - e_k||^2 / ||e_k||^2 in the network's units) and its gradient with respect to
+ * the FP32 weights, for samples [k][ncoarse] on the host; grad may be NULL */
+int hfk_net_gradient(hfk_net* net, const double* r, const double* e, int nsamples, float* grad, double* loss);
+/* Adam training on device-resident pairs; history[steps] receives the mean loss of each step (may be NULL) */
+int hfk_net_train_dev(hfk_net* net, const double* r_dev, const double* e_dev, int nsamples, int steps, int batch,
+                      double lr, uint64_t seed, double* history, void* stream);
+/* The network as the coarse level of a p-multigrid: mode 0 off (block-Jacobi PCG), 1 the network's output is the
+ * initial guess of the coarse PCG (same stopping rule), 2 the network's output is the coarse correction. */
+int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode);
+/* Training pairs from the plain p-multigrid: for each fine right-hand side b_k ([k][nfine], device) the
+ * pre-smoothed residual restricted to the coarse level (bc_k) and its coarse solution to rtol (ec_k). */
+int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, double* bc_dev, double* ec_dev, double rtol,
+                             int maxiter, void* stream);
+
 /* ---- MatrixFreeSolver solve! (src/solvers.jl:111-158) ------------------------------------------------- */
 /* kind 0 = CG, 1 = flexible CG; precond 0 = none, 1 = op's BlockJacobi (updated by hfk_block_jacobi_update),
  * 2 = one V-cycle of pmg.  From x = 0 to ||b - A x||_2 <= atol + rtol ||b||_2.  status = (converged,
