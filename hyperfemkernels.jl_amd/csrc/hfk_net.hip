@@ -221,8 +221,14 @@ __global__ void k_restrict_T(int Nf, View Cv, View F) {
   }
 }
 
-// fine = P coarse (BF16): fine node f takes the coarse nodes i with |f - 2i| <= 1, weights 1 and 1/2
-__global__ void k_prolong_bf(int Nf, View Cv, View F) {
+__device__ __forceinline__ float ld_act(const unsigned short* p, int64_t i) { return bf2f(p[i]); }
+__device__ __forceinline__ float ld_act(const float* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ void st_act(unsigned short* p, int64_t i, float v) { p[i] = f2bf(v); }
+__device__ __forceinline__ void st_act(float* p, int64_t i, float v) { p[i] = v; }
+
+// fine = P coarse (T: BF16 or FP32): fine node f takes the coarse nodes i with |f - 2i| <= 1, weights 1 and 1/2
+template <typename T>
+__global__ void k_prolong(int Nf, View Cv, View F) {
   const int Nc = (Nf + 1) / 2;
   const int64_t n = (int64_t)Nf * Nf * Nf * F.C;
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
@@ -239,9 +245,9 @@ __global__ void k_prolong_bf(int Nf, View Cv, View F) {
       for (int iy = lo[1]; iy <= hi[1]; ++iy)
         for (int ix = lo[0]; ix <= hi[0]; ++ix) {
           const float w = w_p(f[0] - 2 * ix) * w_p(f[1] - 2 * iy) * w_p(f[2] - 2 * iz);
-          s += w * bf2f(static_cast<const unsigned short*>(Cv.p)[(ix + (int64_t)Nc * (iy + (int64_t)Nc * iz)) * Cv.stride + c]);
+          s += w * ld_act(static_cast<const T*>(Cv.p), (ix + (int64_t)Nc * (iy + (int64_t)Nc * iz)) * Cv.stride + c);
         }
-    static_cast<unsigned short*>(F.p)[vf * F.stride + c] = f2bf(s);
+    st_act(static_cast<T*>(F.p), vf * F.stride + c, s);
   }
 }
 
@@ -266,9 +272,10 @@ __global__ void k_prolong_T(int Nf, View F, View Cv) {
   }
 }
 
-// input of the network: x[v][c] = r[dof(v, c)] / scale (c < 3), 0 elsewhere (16 channels)
+// input of the network: x[v][c] = r[dof(v, c)] / scale (c < 3), the auxiliary vertex field (c = 3, e.g. the
+// normalised log shear modulus; 0 when absent), 0 elsewhere (16 channels)
 __global__ void k_pack(int64_t nvox, const int32_t* __restrict__ node_dofs, const double* __restrict__ r,
-                       const double* __restrict__ scale, unsigned short* __restrict__ X) {
+                       const double* __restrict__ scale, const float* __restrict__ aux, unsigned short* __restrict__ X) {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nvox * 16; g += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = g / 16;
     const int c = (int)(g % 16);
@@ -276,8 +283,34 @@ __global__ void k_pack(int64_t nvox, const int32_t* __restrict__ node_dofs, cons
     if (c < 3) {
       const int id = node_dofs[v * 3 + c];
       if (id > 0) val = (float)(r[id - 1] / scale[0]);
+    } else if (c == 3 && aux) {
+      val = aux[v];
     }
     X[g] = f2bf(val);
+  }
+}
+
+// w[dof] = out[v][c] (the network's output in its own units) on the free dofs
+__global__ void k_out_to_vec(int64_t nvox, const int32_t* __restrict__ node_dofs, const float* __restrict__ out,
+                             double* __restrict__ w) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nvox * 3; g += (int64_t)gridDim.x * blockDim.x) {
+    const int id = node_dofs[g];
+    if (id > 0) w[id - 1] = (double)out[(g / 3) * 16 + g % 3];
+  }
+}
+
+// dOut[v][c] = gw[dof(v, c)] * scale_factor (c < 3, free), 0 elsewhere
+__global__ void k_vec_to_gout(int64_t nvox, const int32_t* __restrict__ node_dofs, const double* __restrict__ gw,
+                              double sc, float* __restrict__ dOut) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nvox * 16; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = g / 16;
+    const int c = (int)(g % 16);
+    float val = 0.f;
+    if (c < 3) {
+      const int id = node_dofs[v * 3 + c];
+      if (id > 0) val = (float)(gw[id - 1] * sc);
+    }
+    dOut[g] = val;
   }
 }
 
@@ -385,13 +418,21 @@ struct hfk_net {
   std::vector<hfk::Layer> layers;
   size_t nparam = 0;
   hfk::DevBuf w32, wbf, wt, grad, m1, m2;  // parameters (FP32 master, BF16), adjoint weights, gradient, Adam moments
-  hfk::DevBuf node_dofs;                   // [v][3] signed coarse dof ids
+  hfk::DevBuf node_dofs;                   // [v][3] signed coarse dof ids on the input lattice
   int64_t ncoarse = 0;
+  // input lattice: the coarse space's vertices (Nin^3).  With coarsen = 1 the U-Net runs on every other vertex
+  // (N[0] = (Nin + 1) / 2): its input is the full-weighting restriction of the packed input and its output is
+  // prolongated trilinearly back to the input lattice; otherwise N[0] = Nin.
+  int coarsen = 0, Nin = 0;
+  int64_t nvin = 0;
+  hfk::DevBuf xin, outf, g_outf;
   double h = 1.0;                          // cell size (output factor 1/h)
   // activations (BF16) and gradients (FP32)
   hfk::DevBuf x0, a0, s0, r1, a2, s1, r2, s2, r3, a5, a6, u2, a7, u1, a8, u0, a9, out;
   hfk::DevBuf g_out, g_a9, g_cat0, g_a8tmp, g_cat1, g_a7tmp, g_cat2, g_a6tmp, g_a5, g_r3, g_r2, g_a2, g_r1, g_a0, gbf;
   hfk::DevBuf scal, part, dots;
+  hfk::DevBuf aux;  // auxiliary input channel per vertex (FP32)
+  bool has_aux = false;
   int step = 0;
 };
 
@@ -443,11 +484,11 @@ int forward(hfk_net* nt, hipStream_t s) {
   hipLaunchKernelGGL(k_restrict_bf, grid1(nt->nvox[3] * 64), dim3(256), 0, s, nt->N[2], vb(nt->s2, 64, 64), vb(nt->r3, 64, 64));
   HIPCHK(ctx, conv(nt, 5, vb(nt->r3, 64, 64), kNone, Wl(5), vb(nt->a5, 64, 64), 0, s));
   HIPCHK(ctx, conv(nt, 6, vb(nt->a5, 64, 64), kNone, Wl(6), vb(nt->a6, 64, 64), 0, s));
-  hipLaunchKernelGGL(k_prolong_bf, grid1(nt->nvox[2] * 64), dim3(256), 0, s, nt->N[2], vb(nt->a6, 64, 64), vb(nt->u2, 64, 64));
+  hipLaunchKernelGGL(k_prolong<unsigned short>, grid1(nt->nvox[2] * 64), dim3(256), 0, s, nt->N[2], vb(nt->a6, 64, 64), vb(nt->u2, 64, 64));
   HIPCHK(ctx, conv(nt, 7, vb(nt->s2, 64, 64), vb(nt->u2, 64, 64), Wl(7), vb(nt->a7, 64, 64), 0, s));
-  hipLaunchKernelGGL(k_prolong_bf, grid1(nt->nvox[1] * 64), dim3(256), 0, s, nt->N[1], vb(nt->a7, 64, 64), vb(nt->u1, 64, 64));
+  hipLaunchKernelGGL(k_prolong<unsigned short>, grid1(nt->nvox[1] * 64), dim3(256), 0, s, nt->N[1], vb(nt->a7, 64, 64), vb(nt->u1, 64, 64));
   HIPCHK(ctx, conv(nt, 8, vb(nt->s1, 32, 32), vb(nt->u1, 64, 64), Wl(8), vb(nt->a8, 32, 32), 0, s));
-  hipLaunchKernelGGL(k_prolong_bf, grid1(nt->nvox[0] * 32), dim3(256), 0, s, nt->N[0], vb(nt->a8, 32, 32), vb(nt->u0, 32, 32));
+  hipLaunchKernelGGL(k_prolong<unsigned short>, grid1(nt->nvox[0] * 32), dim3(256), 0, s, nt->N[0], vb(nt->a8, 32, 32), vb(nt->u0, 32, 32));
   HIPCHK(ctx, conv(nt, 9, vb(nt->s0, 16, 16), vb(nt->u0, 32, 32), Wl(9), vb(nt->a9, 16, 16), 0, s));
   HIPCHK(ctx, conv(nt, 10, vb(nt->a9, 16, 16), kNone, Wl(10), vb(nt->out, 16, 16), 1, s));
   HIPCHK(ctx, hipGetLastError());
@@ -563,10 +604,29 @@ int pack_input(hfk_net* nt, const double* r, hipStream_t s) {
   double* sc = nt->scal.as<double>();
   HIPCHK(nt->ctx, vec_dots(nt->ncoarse, r, r, nullptr, nullptr, nullptr, nullptr, nt->dots.as<double>(), sc, s));
   hipLaunchKernelGGL(k_rms, dim3(1), dim3(1), 0, s, sc, nt->ncoarse, sc + 1);
-  hipLaunchKernelGGL(k_pack, grid1(nt->nvox[0] * 16), dim3(256), 0, s, nt->nvox[0], nt->node_dofs.as<int32_t>(), r, sc + 1,
-                     nt->x0.as<unsigned short>());
+  hipLaunchKernelGGL(k_pack, grid1(nt->nvin * 16), dim3(256), 0, s, nt->nvin, nt->node_dofs.as<int32_t>(), r, sc + 1,
+                     nt->has_aux ? nt->aux.as<float>() : nullptr, (nt->coarsen ? nt->xin : nt->x0).as<unsigned short>());
+  if (nt->coarsen)
+    hipLaunchKernelGGL(k_restrict_bf, grid1(nt->nvox[0] * 16), dim3(256), 0, s, nt->Nin, vb(nt->xin, 16, 16),
+                       vb(nt->x0, 16, 16));
   HIPCHK(nt->ctx, hipGetLastError());
   return HFK_OK;
+}
+
+// the network's output on the input lattice (FP32, 16-channel rows; channels 0..2 meaningful)
+const float* output_on_input(hfk_net* nt, hipStream_t s) {
+  if (!nt->coarsen) return nt->out.as<float>();
+  hipLaunchKernelGGL(k_prolong<float>, grid1(nt->nvin * 3), dim3(256), 0, s, nt->Nin, View{nt->out.p, 3, 16},
+                     View{nt->outf.p, 3, 16});
+  return nt->outf.as<float>();
+}
+
+// dLoss/dOut on the input lattice (written by the caller into input_gradient(nt)) -> nt->g_out
+float* input_gradient(hfk_net* nt) { return (nt->coarsen ? nt->g_outf : nt->g_out).as<float>(); }
+void gradient_to_network(hfk_net* nt, hipStream_t s) {
+  if (nt->coarsen)
+    hipLaunchKernelGGL(k_prolong_T, grid1(nt->nvox[0] * 16), dim3(256), 0, s, nt->Nin, View{nt->g_outf.p, 16, 16},
+                       View{nt->g_out.p, 16, 16});
 }
 
 int net_apply(hfk_net* nt, const double* r, double* z, hipStream_t s) {
@@ -574,8 +634,9 @@ int net_apply(hfk_net* nt, const double* r, double* z, hipStream_t s) {
   if ((rc = pack_input(nt, r, s))) return rc;
   if ((rc = forward(nt, s))) return rc;
   HIPCHK(nt->ctx, launch_fill(z, nt->ncoarse, 0.0, s));
-  hipLaunchKernelGGL(k_unpack, grid1(nt->nvox[0] * 3), dim3(256), 0, s, nt->nvox[0], nt->node_dofs.as<int32_t>(),
-                     nt->out.as<float>(), nt->scal.as<double>() + 1, 1.0 / nt->h, z);
+  const float* o = output_on_input(nt, s);
+  hipLaunchKernelGGL(k_unpack, grid1(nt->nvin * 3), dim3(256), 0, s, nt->nvin, nt->node_dofs.as<int32_t>(), o,
+                     nt->scal.as<double>() + 1, 1.0 / nt->h, z);
   HIPCHK(nt->ctx, hipGetLastError());
   return HFK_OK;
 }
@@ -595,8 +656,10 @@ int sample_grad(hfk_net* nt, const double* r, const double* e, hipStream_t s, do
   const double tt = hs[2] / (hs[1] * factor * hs[1] * factor);
   HIPCHK(nt->ctx, hipMemcpyAsync(sc + 3, &tt, sizeof(double), hipMemcpyHostToDevice, s));
   const int nb = 512;
-  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, nt->nvox[0], nt->node_dofs.as<int32_t>(), nt->out.as<float>(), e,
-                     sc + 1, factor, sc + 3, nt->g_out.as<float>(), nt->part.as<double>());
+  const float* o = output_on_input(nt, s);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, nt->nvin, nt->node_dofs.as<int32_t>(), o, e, sc + 1, factor,
+                     sc + 3, input_gradient(nt), nt->part.as<double>());
+  gradient_to_network(nt, s);
   std::vector<double> part(nb);
   HIPCHK(nt->ctx, hipMemcpyAsync(part.data(), nt->part.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(nt->ctx, hipStreamSynchronize(s));
@@ -643,6 +706,12 @@ int net_alloc(hfk_net* nt) {
   HIPCHK(ctx, fl(nt->g_r1, 1, 16));
   HIPCHK(ctx, fl(nt->g_a0, 0, 16));
   HIPCHK(ctx, nt->gbf.alloc((size_t)nt->nvox[0] * 64 * sizeof(unsigned short)));
+  if (nt->coarsen) {
+    HIPCHK(ctx, nt->xin.alloc((size_t)nt->nvin * 16 * sizeof(unsigned short)));
+    HIPCHK(ctx, nt->outf.alloc((size_t)nt->nvin * 16 * sizeof(float)));
+    HIPCHK(ctx, nt->g_outf.alloc((size_t)nt->nvin * 16 * sizeof(float)));
+    HIPCHK(ctx, hipMemset(nt->outf.p, 0, (size_t)nt->nvin * 16 * sizeof(float)));
+  }
   size_t maxw = 0;
   for (const Layer& L : nt->layers) maxw = std::max(maxw, (size_t)27 * L.cout * (L.cin1 + L.cin2));
   HIPCHK(ctx, nt->wt.alloc(maxw * sizeof(unsigned short)));
@@ -656,21 +725,61 @@ int net_alloc(hfk_net* nt) {
 
 int net_apply_dev(hfk_net* nt, const double* r, double* z, hipStream_t s) { return net_apply(nt, r, z, s); }
 int64_t net_ncoarse(const hfk_net* nt) { return nt->ncoarse; }
+double net_h(const hfk_net* nt) { return nt->h; }
+
+int net_forward_vec(hfk_net* nt, const double* r, double* w, hipStream_t s) {
+  int rc;
+  if ((rc = pack_input(nt, r, s))) return rc;
+  if ((rc = forward(nt, s))) return rc;
+  HIPCHK(nt->ctx, launch_fill(w, nt->ncoarse, 0.0, s));
+  const float* o = output_on_input(nt, s);
+  hipLaunchKernelGGL(k_out_to_vec, grid1(nt->nvin * 3), dim3(256), 0, s, nt->nvin, nt->node_dofs.as<int32_t>(), o, w);
+  HIPCHK(nt->ctx, hipGetLastError());
+  return HFK_OK;
+}
+
+int net_backward_vec(hfk_net* nt, const double* gw, double sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_to_gout, grid1(nt->nvin * 16), dim3(256), 0, s, nt->nvin, nt->node_dofs.as<int32_t>(), gw, sc,
+                     input_gradient(nt));
+  gradient_to_network(nt, s);
+  HIPCHK(nt->ctx, hipGetLastError());
+  return backward(nt, s);
+}
+
+int net_adam(hfk_net* nt, double lr, int batch, hipStream_t s) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  nt->step++;
+  const float c1 = 1.f - std::pow(b1, (float)nt->step), c2 = 1.f - std::pow(b2, (float)nt->step);
+  hipLaunchKernelGGL(k_adam, grid1((int64_t)nt->nparam), dim3(256), 0, s, (int64_t)nt->nparam, nt->w32.as<float>(),
+                     nt->grad.as<float>(), nt->m1.as<float>(), nt->m2.as<float>(), nt->wbf.as<unsigned short>(), (float)lr,
+                     b1, b2, eps, c1, c2, 1.0f / (float)batch);
+  HIPCHK(nt->ctx, hipGetLastError());
+  return HFK_OK;
+}
 
 }  // namespace hfk
 
 extern "C" {
 
-int hfk_net_create(hfk_ctx* ctx, int n_lattice, const int32_t* node_dofs, int64_t ncoarse, double h, uint64_t seed,
-                   hfk_net** out) {
+int hfk_net_create(hfk_ctx* ctx, int n_lattice, int coarsen, const int32_t* node_dofs, int64_t ncoarse, double h,
+                   uint64_t seed, hfk_net** out) {
   if (!ctx || !out || !node_dofs) return set_err(ctx, HFK_EARG, "HyperFEMKernels: null argument");
-  if (n_lattice < 9 || (n_lattice % 2) == 0 || ((n_lattice + 1) / 2) % 2 == 0 || ((((n_lattice + 1) / 2) + 1) / 2) % 2 == 0)
-    return set_err(ctx, HFK_EARG, "HyperFEMKernels: the lattice size must be 2^k m + 1 for three halvings (e.g. 17, 33, 65)");
+  if (coarsen != 0 && coarsen != 1) return set_err(ctx, HFK_EARG, "HyperFEMKernels: coarsen must be 0 or 1");
+  if (coarsen && (n_lattice % 2) == 0)
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: a coarsened network needs an odd lattice size");
+  const int n0 = coarsen ? (n_lattice + 1) / 2 : n_lattice;
+  if (n0 < 9 || (n0 % 2) == 0 || ((n0 + 1) / 2) % 2 == 0 || ((((n0 + 1) / 2) + 1) / 2) % 2 == 0)
+    return set_err(ctx, HFK_EARG,
+                   "HyperFEMKernels: the network lattice must be 2^k m + 1 for three halvings (e.g. 17, 33, 65) and "
+                   "at least 9");
   if (!(h > 0) || ncoarse <= 0) return set_err(ctx, HFK_EARG, "HyperFEMKernels: invalid cell size or dof count");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   std::unique_ptr<hfk_net> nt(new hfk_net());
   nt->ctx = ctx;
-  nt->N[0] = n_lattice;
+  nt->coarsen = coarsen;
+  nt->Nin = n_lattice;
+  nt->nvin = (int64_t)n_lattice * n_lattice * n_lattice;
+  nt->N[0] = n0;
   for (int l = 1; l < 4; ++l) nt->N[l] = (nt->N[l - 1] + 1) / 2;
   for (int l = 0; l < 4; ++l) nt->nvox[l] = (int64_t)nt->N[l] * nt->N[l] * nt->N[l];
   nt->ncoarse = ncoarse;
@@ -690,12 +799,12 @@ int hfk_net_create(hfk_ctx* ctx, int n_lattice, const int32_t* node_dofs, int64_
   for (int li = 0; li < kNL; ++li) {
     const Layer& L = nt->layers[li];
     const int cin = L.cin1 + L.cin2;
-    const float sd = std::sqrt((L.relu ? 2.0f : 1.0f) / (27.0f * (li == 0 ? 3 : cin)));
+    const float sd = std::sqrt((L.relu ? 2.0f : 1.0f) / (27.0f * (li == 0 ? 4 : cin)));
     for (int tap = 0; tap < 27; ++tap)
       for (int co = 0; co < L.cout; ++co)
         for (int ci = 0; ci < cin; ++ci) {
           float v = sd * nd(rng);
-          if (li == 0 && ci >= 3) v = 0.f;
+          if (li == 0 && ci >= 4) v = 0.f;
           if (li == kNL - 1 && co >= 3) v = 0.f;
           w[L.woff + ((size_t)tap * L.cout + co) * cin + ci] = v;
         }
@@ -709,8 +818,8 @@ int hfk_net_create(hfk_ctx* ctx, int n_lattice, const int32_t* node_dofs, int64_
   }
   hipLaunchKernelGGL(k_to_bf, grid1((int64_t)off), dim3(256), 0, ctx->stream, (int64_t)off, nt->w32.as<float>(),
                      nt->wbf.as<unsigned short>());
-  HIPCHK(ctx, nt->node_dofs.alloc((size_t)nt->nvox[0] * 3 * sizeof(int32_t)));
-  HIPCHK(ctx, hipMemcpy(nt->node_dofs.p, node_dofs, (size_t)nt->nvox[0] * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(ctx, nt->node_dofs.alloc((size_t)nt->nvin * 3 * sizeof(int32_t)));
+  HIPCHK(ctx, hipMemcpy(nt->node_dofs.p, node_dofs, (size_t)nt->nvin * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
   int rc = net_alloc(nt.get());
   if (rc) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -720,6 +829,18 @@ int hfk_net_create(hfk_ctx* ctx, int n_lattice, const int32_t* node_dofs, int64_
 
 int hfk_net_destroy(hfk_net* nt) {
   delete nt;
+  return HFK_OK;
+}
+
+int hfk_net_set_aux(hfk_net* nt, const float* aux) {
+  if (!nt) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null network");
+  if (!aux) {
+    nt->has_aux = false;
+    return HFK_OK;
+  }
+  HIPCHK(nt->ctx, nt->aux.alloc((size_t)nt->nvin * sizeof(float)));
+  HIPCHK(nt->ctx, hipMemcpy(nt->aux.p, aux, (size_t)nt->nvin * sizeof(float), hipMemcpyHostToDevice));
+  nt->has_aux = true;
   return HFK_OK;
 }
 

@@ -171,6 +171,10 @@ struct hfk_cheb {
   double lmin = 0, lmax = 0, a = 0, b = 0;
   bool ok = false, blocks_ok = false;
   bool secondary = false;  // apply the operator's second tangents (another precision) instead of its own
+  // an assembled operator instead of the stored tangents (CSC of A, 0-based; A symmetric): y = alpha A x + beta y
+  const int32_t* csr_colptr = nullptr;
+  const int32_t* csr_rowval = nullptr;
+  const double* csr_val = nullptr;
   int nbuild = 0;
   DevBuf r, d, tmp, start, V;
   Dots dots;
@@ -179,6 +183,16 @@ struct hfk_cheb {
 };
 
 namespace {
+
+// y = alpha A x (+ y when accumulate) with the smoother's operator
+int cheb_op_apply(hfk_cheb* ch, const double* x, double* y, double alpha, bool accumulate, hipStream_t s) {
+  hfk_op* op = ch->op;
+  if (ch->csr_val) {
+    HIPCHK(op->ctx, csc_t_spmv(op->nfree, ch->csr_colptr, ch->csr_rowval, ch->csr_val, x, y, alpha, accumulate ? 1.0 : 0.0, s));
+    return HFK_OK;
+  }
+  return op_run_apply_t(op, ch->T(), ch->storage(), x, y, s, alpha, accumulate);
+}
 
 // eigenvalue_bounds (src/preconditioners.jl:374-389): k Arnoldi steps (modified Gram-Schmidt) on J M^-1 from
 // `start`; extreme real parts of the Ritz values of the (truncated at a near breakdown) Hessenberg matrix
@@ -195,7 +209,7 @@ int arnoldi_bounds(hfk_op* op, hfk_cheb* ch, int k, hipStream_t s, double* lmin,
     double* vj = V + (size_t)j * n;
     double* w = V + (size_t)(j + 1) * n;
     if ((rc = blocks_apply(op, vj, ch->tmp.as<double>(), s))) return rc;
-    if ((rc = op_run_apply_t(op, ch->T(), ch->storage(), ch->tmp.as<double>(), w, s, 1.0, false))) return rc;
+    if ((rc = cheb_op_apply(ch, ch->tmp.as<double>(), w, 1.0, false, s))) return rc;
     for (int i = 0; i <= j; ++i) {
       double hij;
       if ((rc = ch->dots.run(op->ctx, s, n, V + (size_t)i * n, w, &hij))) return rc;
@@ -243,9 +257,10 @@ int arnoldi_bounds(hfk_op* op, hfk_cheb* ch, int k, hipStream_t s, double* lmin,
 
 int cheb_update(hfk_cheb* ch, hipStream_t s) {
   hfk_op* op = ch->op;
-  bool bok = false;
-  int rc = blocks_update(op, ch->T(), ch->storage(), s, &bok);
-  if (rc) return rc;
+  bool bok = ch->blocks_ok;
+  int rc;
+  // an assembled operator brings its nodal blocks (and their check) from its own set-up
+  if (!ch->csr_val && (rc = blocks_update(op, ch->T(), ch->storage(), s, &bok))) return rc;
   ch->blocks_ok = bok;
   if (!bok) {
     ch->ok = false;
@@ -280,7 +295,7 @@ int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, dou
   if (zero_guess) {
     HIPCHK(op->ctx, launch_fill(x, n, 0.0, s));
   } else {
-    if ((rc = op_run_apply_t(op, ch->T(), ch->storage(), x, r, s, -1.0, true))) return rc;  // r = rhs - A x
+    if ((rc = cheb_op_apply(ch, x, r, -1.0, true, s))) return rc;  // r = rhs - A x
   }
   // z = M^-1 r; d = z / theta; x += d
   HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, 0.0, 1.0 / th, s));
@@ -288,7 +303,7 @@ int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, dou
   const int k = ch->degree;
   for (int i = 1; i <= k; ++i) {
     if (i == k && !residual) break;
-    if ((rc = op_run_apply_t(op, ch->T(), ch->storage(), d, r, s, -1.0, true))) return rc;  // r -= A d
+    if ((rc = cheb_op_apply(ch, d, r, -1.0, true, s))) return rc;  // r -= A d
     if (i == k) break;
     const double rn = 1.0 / (2.0 * sg - rho);
     HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, rn * rho, 2.0 * rn / de, s));
@@ -458,7 +473,9 @@ struct hfk_pmg {
   // coarse CSR (CSC of A, 0-based int32)
   DevBuf colptr, rowval, nzval;
   int64_t nnz = 0;
-  DevBuf cr, cz, cp, cq, cs;  // inner CG work vectors and device scalars
+  DevBuf cr, cz, cp, cq, ct, cu, cs;  // inner CG work vectors and device scalars
+  std::unique_ptr<hfk_cheb> ccheb;      // Chebyshev on the assembled coarse matrix (network cycle, modes 3 and 4)
+  bool ccheb_stale = true;
   Dots dots;
   bool ok = false, indefinite = false, built = false, coarse_ok = false;
   int nupdate = 0, ncycle = 0, last_inner = 0, total_inner = 0;
@@ -499,7 +516,48 @@ int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
   return HFK_OK;
 }
 
-// coarse solve: block-Jacobi PCG on the assembled coarse matrix from zero to coarse_rtol (relative)
+// Chebyshev smoother of the assembled coarse level (same degree, range and inflation as the matrix-free levels);
+// set up at its first use after an update.  *usable = false when its interval is not usable.
+int coarse_smoother(hfk_pmg* g, hipStream_t s, bool* usable) {
+  hfk_op* c = g->ops.back();
+  int rc;
+  if (!g->ccheb) {
+    const hfk_cheb* f = g->cheb[0].get();
+    std::unique_ptr<hfk_cheb> ch(new hfk_cheb());
+    if ((rc = cheb_init(c, f->degree, f->range, f->inflate, f->eig_iters, ch.get()))) return rc;
+    ch->csr_colptr = g->colptr.as<int32_t>();
+    ch->csr_rowval = g->rowval.as<int32_t>();
+    ch->csr_val = g->nzval.as<double>();
+    g->ccheb = std::move(ch);
+  }
+  if (g->ccheb_stale) {
+    g->ccheb->blocks_ok = g->coarse_ok;
+    if ((rc = cheb_update(g->ccheb.get(), s))) return rc;
+    g->ccheb_stale = false;
+  }
+  *usable = g->ccheb->ok;
+  return HFK_OK;
+}
+
+// one two-grid cycle on the coarse level with the network as its coarse-grid correction:
+// x = S b; x += N(b - A x); x = x + S (b - A x)   (S: the coarse Chebyshev smoother)
+int net_cycle(hfk_pmg* g, const double* b, double* x, hipStream_t s) {
+  hfk_cheb* S = g->ccheb.get();
+  const int64_t n = g->ops.back()->nfree;
+  double* t = g->ct.as<double>();
+  double* u = g->cu.as<double>();
+  int rc;
+  if ((rc = cheb_smooth(S, x, b, true, t, s))) return rc;
+  if ((rc = net_apply_dev(g->net, t, u, s))) return rc;
+  HIPCHK(g->ctx, vec_axpby(n, 1.0, u, 1.0, x, s));
+  return cheb_smooth(S, x, b, false, nullptr, s);
+}
+
+// coarse solve on the assembled coarse matrix from zero to coarse_rtol (relative).  Without the network:
+// block-Jacobi PCG.  With it (set_coarse_network): mode 1 starts the PCG from the network's output, mode 2 takes
+// the network's output as the solution, mode 3 runs flexible CG preconditioned by the network cycle (net_cycle),
+// mode 4 takes one network cycle as the solution.  Modes 3 and 4 fall back to the PCG when the coarse smoother
+// has no usable interval.
 int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool allow_net, double rtol, int maxiter) {
   hfk_op* c = g->ops.back();
   const int64_t n = c->nfree;
@@ -509,18 +567,34 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
   double* p = g->cp.as<double>();
   double* q = g->cq.as<double>();
   double* sc = g->cs.as<double>();
+  const int32_t* cp = g->colptr.as<int32_t>();
+  const int32_t* rv = g->rowval.as<int32_t>();
+  const double* av = g->nzval.as<double>();
+  int mode = allow_net && g->net ? g->net_mode : 0;
   int rc;
   g->last_inner = 0;
-  if (allow_net && g->net && g->net_mode > 0) {  // learned coarse correction
+  if (mode >= 3) {
+    bool usable = false;
+    if ((rc = coarse_smoother(g, s, &usable))) return rc;
+    if (!usable) mode = 0;
+  }
+  if (mode == 4) return net_cycle(g, b, x, s);
+  const bool flexible = mode == 3;
+  auto precondition = [&](const double* rin, double* zout) -> int {
+    if (flexible) return net_cycle(g, rin, zout, s);
+    HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), rin, zout, nullptr, 0.0, 1.0, s));
+    return HFK_OK;
+  };
+  if (mode == 1 || mode == 2) {
     if ((rc = net_apply_dev(g->net, b, x, s))) return rc;
-    if (g->net_mode == 2) return HFK_OK;
+    if (mode == 2) return HFK_OK;
     HIPCHK(g->ctx, vec_copy(n, b, r, s));  // r = b - A x0
-    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), x, r, -1.0, 1.0, s));
+    HIPCHK(g->ctx, csc_t_spmv(n, cp, rv, av, x, r, -1.0, 1.0, s));
   } else {
     HIPCHK(g->ctx, launch_fill(x, n, 0.0, s));
     HIPCHK(g->ctx, vec_copy(n, b, r, s));
   }
-  HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, z, nullptr, 0.0, 1.0, s));
+  if ((rc = precondition(r, z))) return rc;
   HIPCHK(g->ctx, vec_copy(n, z, p, s));
   double h[3];
   if ((rc = g->dots.run(g->ctx, s, n, r, z, h, b, b, r, r))) return rc;
@@ -528,19 +602,22 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
   HIPCHK(g->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZ, s));
   if (bn == 0 || !(std::sqrt(h[2]) > rtol * bn)) return HFK_OK;
   for (int it = 1; it <= maxiter; ++it) {
-    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), p, q, 1.0, 0.0, s));
+    HIPCHK(g->ctx, csc_t_spmv(n, cp, rv, av, p, q, 1.0, 0.0, s));
     HIPCHK(g->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kPQ, s));
     HIPCHK(g->ctx, cg_update(n, sc, p, q, x, r, s));
-    HIPCHK(g->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, z, nullptr, 0.0, 1.0, s));
-    HIPCHK(g->ctx, vec_dots(n, r, z, r, r, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZN, s));
+    if ((rc = precondition(r, z))) return rc;
+    // sc[kRZN] = r.z, sc[kRZN+1] = r.r, sc[kRZN+2] = z.q (the flexible direction's projection)
+    HIPCHK(g->ctx, vec_dots(n, r, z, r, r, flexible ? z : nullptr, flexible ? q : nullptr, g->dots.part.as<double>(),
+                            sc + kRZN, s));
     g->last_inner = it;
-    if (it % 5 == 0 || it == maxiter) {
+    if (flexible || it % 5 == 0 || it == maxiter) {
       double rr[2];
       HIPCHK(g->ctx, hipMemcpyAsync(rr, sc + kRZN, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(g->ctx, hipStreamSynchronize(s));
       if (!(std::sqrt(rr[1]) > rtol * bn)) break;  // converged (or NaN: leave it to the outer check)
     }
-    HIPCHK(g->ctx, cg_direction(n, sc, z, p, 0, s));
+    if (flexible) HIPCHK(g->ctx, hipMemcpyAsync(sc + kZQ, sc + kRZN + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(g->ctx, cg_direction(n, sc, z, p, flexible ? 1 : 0, s));
   }
   g->total_inner += g->last_inner;
   return HFK_OK;
@@ -593,6 +670,7 @@ int pmg_update(hfk_pmg* g, hipStream_t s) {
       bool cok = false;
       if ((rc = coarse_setup(g, s, &cok))) return rc;
       g->coarse_ok = cok;
+      g->ccheb_stale = true;
       ok = ok && cok;
     }
   }
@@ -925,7 +1003,8 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
   for (int l = 1; l < nlevels; ++l) HIPCHK(ctx, g->ops[l]->state.alloc((size_t)g->ops[l]->nfree * sizeof(double)));
   const int64_t nc = g->ops.back()->nfree;
   if ((rc = vec_alloc(ctx, g->cr, nc)) || (rc = vec_alloc(ctx, g->cz, nc)) || (rc = vec_alloc(ctx, g->cp, nc)) ||
-      (rc = vec_alloc(ctx, g->cq, nc)) || (rc = g->dots.init(ctx)))
+      (rc = vec_alloc(ctx, g->cq, nc)) || (rc = vec_alloc(ctx, g->ct, nc)) || (rc = vec_alloc(ctx, g->cu, nc)) ||
+      (rc = g->dots.init(ctx)))
     return rc;
   HIPCHK(ctx, g->cs.alloc(kNS * sizeof(double)));
   *out = g.release();
@@ -996,7 +1075,7 @@ int hfk_pmg_info(hfk_pmg* g, double* info) {
 
 int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode) {
   if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
-  if (mode < 0 || mode > 2) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: coarse network mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 4) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: coarse network mode must be 0 to 4");
   if (mode > 0 && (!net || net_ncoarse(net) != g->ops.back()->nfree))
     return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: the network is not built for this coarse level");
   g->net = mode > 0 ? net : nullptr;
@@ -1006,9 +1085,9 @@ int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode) {
 
 // Training pairs from the plain cycle (two-level view: the working level and the assembled coarse level): the
 // pre-smoothed residual of b_k restricted to the coarse level and its coarse solution.
-int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, double* bc_dev, double* ec_dev, double rtol,
-                             int maxiter, void* stream) {
-  if (!g || !b_dev || !bc_dev || !ec_dev || nsamples < 1 || !(rtol > 0) || maxiter < 1)
+int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, double* bc_dev, double* ec_dev,
+                             int coarse_smoothed, double rtol, int maxiter, void* stream) {
+  if (!g || !b_dev || !bc_dev || nsamples < 1 || !(rtol > 0) || maxiter < 1)
     return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: invalid arguments");
   if (g->ops.size() != 2) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: coarse pairs need a two-level hierarchy");
   if (!g->built || !g->ok) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid not updated or not ok");
@@ -1018,15 +1097,93 @@ int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, doub
   const int64_t nf = g->ops[0]->nfree, nc = g->ops[1]->nfree;
   DevBuf xw;
   if ((rc = vec_alloc(g->ctx, xw, nf))) return rc;
+  if (coarse_smoothed) {
+    bool usable = false;
+    if ((rc = coarse_smoother(g, s, &usable))) return rc;
+    if (!usable) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: the coarse smoother has no usable interval");
+  }
   for (int k = 0; k < nsamples; ++k) {
     const double* b = b_dev + (size_t)k * nf;
     double* bc = bc_dev + (size_t)k * nc;
-    double* ec = ec_dev + (size_t)k * nc;
     if ((rc = cheb_smooth(g->cheb[0].get(), xw.as<double>(), b, true, g->r[0]->as<double>(), s))) return rc;
     if ((rc = do_restrict(g->P[0].get(), g->r[0]->as<double>(), bc, s))) return rc;
-    if ((rc = coarse_solve(g, bc, ec, s, false, rtol, maxiter))) return rc;
+    if (coarse_smoothed) {  // the network cycle's input: the residual of the coarse pre-smoothing
+      if ((rc = cheb_smooth(g->ccheb.get(), g->cu.as<double>(), bc, true, g->ct.as<double>(), s))) return rc;
+      HIPCHK(g->ctx, vec_copy(nc, g->ct.as<double>(), bc, s));
+    }
+    if (ec_dev && (rc = coarse_solve(g, bc, ec_dev + (size_t)k * nc, s, false, rtol, maxiter))) return rc;
   }
   HIPCHK(g->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+// Energy training of the coarse network on coarse right-hand sides r_k (device, [k][nc]): minimise
+// L(w) = w'(A/h) w - 2 w'x, x = r/s, whose minimiser is the coarse solve in the network's units (z = (s/h) w
+// = A^-1 r).  Needs one product with the assembled coarse matrix per sample and no exact solves.
+int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, int steps, int batch,
+                                 double lr, uint64_t seed, double* history, void* stream) {
+  if (!g || !net || !bc_dev || nsamples < 1 || steps < 0 || batch < 1 || !(lr > 0))
+    return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: invalid training arguments");
+  if (!g->built || !g->colptr.p) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid not updated");
+  const int64_t n = g->ops.back()->nfree;
+  if (net_ncoarse(net) != n) return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: the network is not built for this coarse level");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  hipStream_t s = g->ops[0]->stream(stream);
+  DevBuf w, q;
+  if ((rc = vec_alloc(g->ctx, w, n)) || (rc = vec_alloc(g->ctx, q, n))) return rc;
+  const double invh = 1.0 / net_h(net);
+  uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+  for (int step = 0; step < steps; ++step) {
+    double Lsum = 0;
+    for (int b = 0; b < batch; ++b) {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      const int k = (int)((st >> 33) % (uint64_t)nsamples);
+      const double* r = bc_dev + (size_t)k * n;
+      if ((rc = net_forward_vec(net, r, w.as<double>(), s))) return rc;
+      HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), w.as<double>(),
+                                q.as<double>(), invh, 0.0, s));
+      double h3[3];
+      if ((rc = g->dots.run(g->ctx, s, n, w.as<double>(), q.as<double>(), h3, w.as<double>(), r, r, r))) return rc;
+      const double sc = std::sqrt(h3[2] / (double)n);
+      const double sinv = sc > 0 ? 1.0 / sc : 1.0;
+      Lsum += (h3[0] - 2.0 * h3[1] * sinv) / (double)n;
+      HIPCHK(g->ctx, vec_axpby(n, -sinv, r, 1.0, q.as<double>(), s));  // q = (A/h) w - x
+      if ((rc = net_backward_vec(net, q.as<double>(), 2.0 / (double)n, s))) return rc;
+    }
+    if ((rc = net_adam(net, lr, batch, s))) return rc;
+    if (history) history[step] = Lsum / batch;
+  }
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+// Relative energy-norm error of the network's coarse correction against the coarse solve to 1e-10, averaged over
+// the right-hand sides r_k (device): sqrt((z - e)' A (z - e) / e' A e)
+int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, double* err,
+                                 void* stream) {
+  if (!g || !net || !bc_dev || nsamples < 1 || !err)
+    return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: invalid arguments");
+  const int64_t n = g->ops.back()->nfree;
+  if (net_ncoarse(net) != n) return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: the network is not built for this coarse level");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  hipStream_t s = g->ops[0]->stream(stream);
+  DevBuf e, z, q;
+  if ((rc = vec_alloc(g->ctx, e, n)) || (rc = vec_alloc(g->ctx, z, n)) || (rc = vec_alloc(g->ctx, q, n))) return rc;
+  double total = 0;
+  for (int k = 0; k < nsamples; ++k) {
+    const double* r = bc_dev + (size_t)k * n;
+    if ((rc = coarse_solve(g, r, e.as<double>(), s, false, 1e-10, 20000))) return rc;
+    if ((rc = net_apply_dev(net, r, z.as<double>(), s))) return rc;
+    HIPCHK(g->ctx, vec_axpby(n, -1.0, e.as<double>(), 1.0, z.as<double>(), s));  // z - e
+    double h2[2];
+    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), z.as<double>(),
+                              q.as<double>(), 1.0, 0.0, s));
+    if ((rc = g->dots.run(g->ctx, s, n, z.as<double>(), q.as<double>(), h2, e.as<double>(), r))) return rc;
+    total += std::sqrt(std::max(h2[0], 0.0) / std::max(h2[1], 1e-300));
+  }
+  *err = total / nsamples;
   return HFK_OK;
 }
 

@@ -70,7 +70,7 @@ SIGNATURES = {
     "hfk_pmg_vcycle": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_pmg_vcycle_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_pmg_info": (C.c_int, [_vp, _f64p]),
-    "hfk_net_create": (C.c_int, [_vp, C.c_int, _i32p, C.c_int64, C.c_double, C.c_uint64, C.POINTER(_vp)]),
+    "hfk_net_create": (C.c_int, [_vp, C.c_int, C.c_int, _i32p, C.c_int64, C.c_double, C.c_uint64, C.POINTER(_vp)]),
     "hfk_net_destroy": (C.c_int, [_vp]),
     "hfk_net_num_params": (C.c_int64, [_vp]),
     "hfk_net_get_weights": (C.c_int, [_vp, C.POINTER(C.c_float)]),
@@ -80,7 +80,11 @@ SIGNATURES = {
     "hfk_net_gradient": (C.c_int, [_vp, _f64p, _f64p, C.c_int, C.POINTER(C.c_float), _f64p]),
     "hfk_net_train_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p, _vp]),
     "hfk_pmg_set_coarse_network": (C.c_int, [_vp, _vp, C.c_int]),
-    "hfk_pmg_coarse_pairs_dev": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_double, C.c_int, _vp]),
+    "hfk_pmg_coarse_pairs_dev": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp]),
+    "hfk_net_set_aux": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "hfk_pmg_train_coarse_network": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p,
+                                               _vp]),
+    "hfk_pmg_coarse_network_error": (C.c_int, [_vp, _vp, _vp, C.c_int, _f64p, _vp]),
     "hfk_krylov_solve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _f64p, _f64p, C.c_double, C.c_double, C.c_int, _i32p,
                                    _f64p]),
     "hfk_krylov_solve_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_int, _i32p,

@@ -2,8 +2,11 @@
 
 ``CoarseNetwork`` wraps the device U-Net of hfk_net.hip (BF16 on MFMA, FP32 master weights, trained on the
 device with the library's own forward/backward kernels and Adam).  It acts on the assembled Q1 level of a
-``PMultigrid``: ``PMultigrid.set_coarse_network(net, mode)`` makes it the coarse correction (mode "replace") or the
-initial guess of the coarse PCG (mode "warmstart"); the outer flexible CG's tolerance fixes the answer either way.
+``PMultigrid``: ``set_coarse_network(pmg, net, mode)`` makes it the coarse correction (mode "replace"), the initial guess of the
+coarse PCG (mode "warmstart"), or the coarse-grid correction of a two-grid cycle on the assembled coarse level,
+between Chebyshev pre- and post-smoothing on that level: one such cycle as the coarse correction (mode "cycle") or
+as the preconditioner of a flexible coarse CG (mode "precondition").  The outer flexible CG's tolerance fixes the
+answer in every mode.
 Training pairs come from the plain p-multigrid itself (``coarse_pairs``).
 """
 from __future__ import annotations
@@ -14,13 +17,16 @@ import numpy as np
 
 from ._lib import check, ptr
 
-MODES = {"off": 0, "warmstart": 1, "replace": 2}
+MODES = {"off": 0, "warmstart": 1, "replace": 2, "precondition": 3, "cycle": 4}
 
 
 class CoarseNetwork:
-    """U-Net on the vertex lattice of a Cartesian n^3 Q1 space (``coarse_space``: mesh.HexSpace of order 1)."""
+    """U-Net on the vertex lattice of a Cartesian n^3 Q1 space (``coarse_space``: mesh.HexSpace of order 1).
 
-    def __init__(self, ctx, coarse_space, seed: int = 0):
+    With ``coarsen`` the U-Net runs on every other lattice vertex (fed by full weighting, prolongated back
+    trilinearly), which keeps its receptive field at the whole domain on fine coarse levels (n = 64)."""
+
+    def __init__(self, ctx, coarse_space, seed: int = 0, coarsen: bool = False):
         n = coarse_space.n
         if not (n[0] == n[1] == n[2]) or coarse_space.order != 1:
             raise ValueError("HyperFEMKernels: the coarse network needs a cubic Q1 lattice")
@@ -30,11 +36,22 @@ class CoarseNetwork:
         dofs = np.ascontiguousarray(coarse_space.lattice_dofs(), dtype=np.int32)
         h = (coarse_space.domain[1] - coarse_space.domain[0]) / n[0]
         hd = C.c_void_p()
-        check(self.lib.hfk_net_create(ctx.handle, self.N, ptr(dofs, C.c_int32), self.ncoarse, h, seed, C.byref(hd)),
-              ctx.handle)
+        check(self.lib.hfk_net_create(ctx.handle, self.N, int(bool(coarsen)), ptr(dofs, C.c_int32), self.ncoarse, h, seed,
+                                      C.byref(hd)), ctx.handle)
+        self.coarsen = bool(coarsen)
         self.handle = hd
         self.h = h
         self.nparams = int(self.lib.hfk_net_num_params(hd))
+
+    def set_aux(self, aux_vertex):
+        """Auxiliary input channel per lattice vertex (e.g. normalised log shear modulus), or None."""
+        if aux_vertex is None:
+            check(self.lib.hfk_net_set_aux(self.handle, None), self.ctx.handle)
+            return
+        a = np.ascontiguousarray(aux_vertex, dtype=np.float32)
+        if a.size != self.N ** 3:
+            raise ValueError("HyperFEMKernels: one auxiliary value per lattice vertex")
+        check(self.lib.hfk_net_set_aux(self.handle, a.ctypes.data_as(C.POINTER(C.c_float))), self.ctx.handle)
 
     def weights(self) -> np.ndarray:
         w = np.zeros(self.nparams, dtype=np.float32)
@@ -78,12 +95,33 @@ class CoarseNetwork:
             pass
 
 
-def coarse_pairs_dev(pmg, b_ptr: int, nsamples: int, bc_ptr: int, ec_ptr: int, rtol: float = 1e-10,
-                     maxiter: int = 5000, stream: int = 0):
-    """Training pairs from the plain p-multigrid (device arrays [k][n])."""
+def coarse_pairs_dev(pmg, b_ptr: int, nsamples: int, bc_ptr: int, ec_ptr: int | None, coarse_smoothed: bool = False,
+                     rtol: float = 1e-10, maxiter: int = 5000, stream: int = 0):
+    """Training data from the plain p-multigrid (device arrays [k][n]): restricted pre-smoothed residuals (with
+    ``coarse_smoothed``, the residuals after the coarse Chebyshev pre-smoothing: the network cycle's input), and
+    their coarse solutions when ec_ptr is given."""
     check(pmg.lib.hfk_pmg_coarse_pairs_dev(pmg.handle, C.c_void_p(b_ptr), nsamples, C.c_void_p(bc_ptr),
-                                           C.c_void_p(ec_ptr), rtol, maxiter, C.c_void_p(stream)),
+                                           C.c_void_p(ec_ptr) if ec_ptr else None, int(bool(coarse_smoothed)), rtol,
+                                           maxiter, C.c_void_p(stream)),
           pmg.levels[0].ctx.handle)
+
+
+def train_energy_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, steps: int, batch: int = 4,
+                     lr: float = 1e-3, seed: int = 0, stream: int = 0) -> np.ndarray:
+    """Energy training of the network on the coarse level of ``pmg`` (no exact solves); loss per step."""
+    hist = np.zeros(max(steps, 1))
+    check(pmg.lib.hfk_pmg_train_coarse_network(pmg.handle, net.handle, C.c_void_p(bc_ptr), nsamples, steps, batch, lr,
+                                               seed, ptr(hist, C.c_double), C.c_void_p(stream)),
+          pmg.levels[0].ctx.handle)
+    return hist[:steps]
+
+
+def coarse_error_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, stream: int = 0) -> float:
+    """Mean relative energy-norm error of the network's coarse correction (against the coarse solve)."""
+    err = C.c_double()
+    check(pmg.lib.hfk_pmg_coarse_network_error(pmg.handle, net.handle, C.c_void_p(bc_ptr), nsamples, C.byref(err),
+                                               C.c_void_p(stream)), pmg.levels[0].ctx.handle)
+    return err.value
 
 
 def set_coarse_network(pmg, net: CoarseNetwork | None, mode: str = "warmstart"):

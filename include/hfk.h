@@ -170,10 +170,12 @@ int hfk_pmg_info(hfk_pmg* g, double* info);
  * activations on MFMA with FP32 accumulation and FP32 master weights.  It maps a residual of the assembled Q1
  * coarse level to an approximation of its coarse-level error: z = (s/h) N(r/s), s = RMS of r, h = cell size.
  *   n_lattice  vertices per direction of the Q1 lattice (2^3 m + 1, e.g. 17, 33, 65)
+ *   coarsen    0: the U-Net runs on that lattice; 1: on every other vertex ((n_lattice + 1) / 2 per direction), fed
+ *              by full-weighting restriction of the input and prolongated trilinearly back
  *   node_dofs  n_lattice^3 * 3 signed coarse dof ids per vertex and component (x fastest; > 0 free) */
 typedef struct hfk_net hfk_net;
-int hfk_net_create(hfk_ctx* ctx, int n_lattice, const int32_t* node_dofs, int64_t ncoarse, double h, uint64_t seed,
-                   hfk_net** out);
+int hfk_net_create(hfk_ctx* ctx, int n_lattice, int coarsen, const int32_t* node_dofs, int64_t ncoarse, double h,
+                   uint64_t seed, hfk_net** out);
 int hfk_net_destroy(hfk_net* net);
 int64_t hfk_net_num_params(const hfk_net* net);
 int hfk_net_get_weights(hfk_net* net, float* w);
@@ -187,12 +189,27 @@ int hfk_net_gradient(hfk_net* net, const double* r, const double* e, int nsample
 int hfk_net_train_dev(hfk_net* net, const double* r_dev, const double* e_dev, int nsamples, int steps, int batch,
                       double lr, uint64_t seed, double* history, void* stream);
 /* The network as the coarse level of a p-multigrid: mode 0 off (block-Jacobi PCG), 1 the network's output is the
- * initial guess of the coarse PCG (same stopping rule), 2 the network's output is the coarse correction. */
+ * initial guess of the coarse PCG (same stopping rule), 2 the network's output is the coarse correction, 3 the
+ * coarse solve is a flexible CG (same stopping rule) preconditioned by one network cycle, 4 one network cycle is
+ * the coarse correction.  The network cycle is x = S b; x += N(b - A x); x += S(b - A x) with S the Chebyshev
+ * smoother of the assembled coarse matrix (the p-multigrid's degree, range and inflation); modes 3 and 4 fall back
+ * to mode 0 when that smoother has no usable interval. */
 int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode);
-/* Training pairs from the plain p-multigrid: for each fine right-hand side b_k ([k][nfine], device) the
- * pre-smoothed residual restricted to the coarse level (bc_k) and its coarse solution to rtol (ec_k). */
-int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, double* bc_dev, double* ec_dev, double rtol,
-                             int maxiter, void* stream);
+/* Training data from the plain p-multigrid: for each fine right-hand side b_k ([k][nfine], device) the
+ * pre-smoothed residual restricted to the coarse level (bc_k) -- with coarse_smoothed != 0, further the residual of
+ * the coarse Chebyshev pre-smoothing of it (the input of the network cycle) -- and, when ec_dev is not NULL, the
+ * coarse solution of bc_k to rtol (ec_k). */
+int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, double* bc_dev, double* ec_dev,
+                             int coarse_smoothed, double rtol, int maxiter, void* stream);
+/* An auxiliary input channel per lattice vertex (e.g. the normalised log shear modulus); NULL removes it. */
+int hfk_net_set_aux(hfk_net* net, const float* aux);
+/* Energy training on coarse right-hand sides bc_k (device): Adam on L(w) = w'(A/h)w - 2 w'x, x = r/s, whose
+ * minimiser is the coarse solve; one product with the assembled coarse matrix per sample, no exact solves. */
+int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, int steps, int batch,
+                                 double lr, uint64_t seed, double* history, void* stream);
+/* Mean relative energy-norm error of the network's coarse correction against the coarse solve to 1e-10. */
+int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, double* err,
+                                 void* stream);
 
 /* ---- MatrixFreeSolver solve! (src/solvers.jl:111-158) ------------------------------------------------- */
 /* kind 0 = CG, 1 = flexible CG; precond 0 = none, 1 = op's BlockJacobi (updated by hfk_block_jacobi_update),

@@ -2,6 +2,9 @@ import os
 import sys
 
 import pytest
+# torch ships its own HIP runtime; load it before libhfk (which links the system ROCm runtime) so that tests which
+# use torch device tensors as plumbing find the device (the bench imports torch first for the same reason)
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "hyperfemkernels.jl_amd")

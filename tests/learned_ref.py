@@ -66,8 +66,10 @@ def forward(Ws, x0):
     return conv(a9, 10)
 
 
-def loss_and_grad(w, node_dofs, N, h, r, e):
-    """Loss sum_free (out - t)^2 / sum t^2 with t = e h / s, s = RMS(r); and its gradient w.r.t. the weights."""
+def loss_and_grad(w, node_dofs, N, h, r, e, coarsen=False):
+    """Loss sum_free (out - t)^2 / sum t^2 with t = e h / s, s = RMS(r); and its gradient w.r.t. the weights.
+    With ``coarsen`` the U-Net runs on every other vertex of the N^3 lattice: BF16 full-weighting restriction of
+    the packed input, FP32 trilinear prolongation of the output."""
     Ws = [t.clone().requires_grad_(True) for t in unflatten(w)]
     s = np.sqrt(np.dot(r, r) / r.size)
     x0 = np.zeros((16, N ** 3), dtype=np.float32)
@@ -80,7 +82,10 @@ def loss_and_grad(w, node_dofs, N, h, r, e):
         t[c, f] = e[ids[f] - 1] * h / s
         mask[c, f] = True
     x0t = bf(torch.tensor(x0).reshape(1, 16, N, N, N))
-    out = forward(Ws, x0t)[0, :3].reshape(3, -1)
+    if coarsen:
+        out = prolong(forward(Ws, bf(restrict(x0t)))[:, :3])[0].reshape(3, -1)
+    else:
+        out = forward(Ws, x0t)[0, :3].reshape(3, -1)
     tt = torch.tensor(t, dtype=torch.float32)
     mk = torch.tensor(mask)
     d = (out - tt)[mk]

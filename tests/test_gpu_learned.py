@@ -2,9 +2,11 @@
 
 * forward output, loss and weight gradients of the device U-Net (BF16 MFMA kernels) against a torch FP32 CPU
   restatement with the same BF16 rounding points (tests/learned_ref.py);
-* training with the device kernels reduces the loss on pairs produced by the plain p-multigrid;
-* inside the p-multigrid the network (as the coarse PCG's initial guess, or as the whole coarse correction) leaves
-  the answer to the Krylov tolerance: the same solution as the plain cycle.
+* energy training with the device kernels on residuals produced by the plain p-multigrid lowers the held-out
+  energy error of the coarse correction;
+* inside the p-multigrid, every way of using the network leaves the answer to the Krylov tolerance (the plain
+  cycle's solution), and as the preconditioner of the coarse flexible CG it cuts the coarse iterations without
+  adding outer ones.
 """
 import numpy as np
 import pytest
@@ -22,17 +24,18 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
 
 
-def test_network_gradient_matches_torch(hip_ctx):
+@pytest.mark.parametrize("coarsen", [False, True])
+def test_network_gradient_matches_torch(hip_ctx, coarsen):
     import learned_ref
 
     cs = hex_space((16, 16, 16), 1, dirichlet=dirichlet_closed_x0)
-    net = CoarseNetwork(hip_ctx, cs, seed=3)
+    net = CoarseNetwork(hip_ctx, cs, seed=3, coarsen=coarsen)
     w = net.weights()
     rng = np.random.default_rng(0)
     r = rng.standard_normal(cs.nfree)
     e = rng.standard_normal(cs.nfree)
     loss, g = net.gradient(r, e)
-    Lr, gr, out = learned_ref.loss_and_grad(w, cs.lattice_dofs(), 17, 1.0 / 16, r, e)
+    Lr, gr, out = learned_ref.loss_and_grad(w, cs.lattice_dofs(), 17, 1.0 / 16, r, e, coarsen=coarsen)
     print(f"loss gpu {loss:.6e} torch {Lr:.6e}; grad rel {rel(g, gr):.3e}", flush=True)
     assert loss == pytest.approx(Lr, rel=2e-2)
     cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
@@ -74,43 +77,49 @@ def make_problem(hip_ctx, n, seed=0, mu_sigma=0.5):
 
 
 def test_training_and_learned_cycle(hip_ctx):
+    """Energy training on restricted, coarse-smoothed p-MG residuals; then the network inside the cycle: every mode
+    reaches the plain cycle's answer (the outer tolerance decides), and as the preconditioner of the coarse flexible
+    CG it cuts the coarse iterations without adding outer ones -- the gain of the trained network."""
     import torch
 
-    n = 16
+    from hfk.learned import coarse_error_dev, train_energy_dev
+
+    n = 32
     fs, cs, fop, cop, mf = make_problem(hip_ctx, n)
-    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-2).update()
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-2, coarse_maxiter=400).update()
     assert pmg.ok
     dev = torch.device("cuda:0")
-    ns = 48
-    rng = np.random.default_rng(1)
-    B = np.stack([smooth_field(100 + k, amp=1.0)(fs.node_xyz[fs.free_node_of]).reshape(-1)
-                  + 0.1 * rng.standard_normal(fs.nfree) for k in range(ns)])
-    bd = torch.from_numpy(B).to(dev)
+    ns, nh = 160, 16
+    gen = torch.Generator(device=dev).manual_seed(1)
+    bd = torch.randn((ns, fs.nfree), dtype=torch.float64, device=dev, generator=gen)
     bc = torch.zeros((ns, cs.nfree), dtype=torch.float64, device=dev)
-    ec = torch.zeros_like(bc)
     torch.cuda.synchronize()
-    coarse_pairs_dev(pmg, bd.data_ptr(), ns, bc.data_ptr(), ec.data_ptr(), rtol=1e-10)
-    net = CoarseNetwork(hip_ctx, cs, seed=7)
-    hist = net.train_dev(bc.data_ptr(), ec.data_ptr(), ns - 8, steps=300, batch=4, lr=1e-3)
-    print("loss first/last:", hist[:5].mean(), hist[-20:].mean(), flush=True)
-    assert hist[-20:].mean() < 0.5 * hist[:5].mean()
-    # held-out pairs: the network's relative error
-    bch = bc[ns - 8:].cpu().numpy()
-    ech = ec[ns - 8:].cpu().numpy()
-    errs = [rel(net.apply(bch[k]), ech[k]) for k in range(8)]
-    print("held-out relative error of the coarse correction:", np.mean(errs), flush=True)
-    # the cycle with the network: same answer (the Krylov tolerance decides)
+    coarse_pairs_dev(pmg, bd.data_ptr(), ns, bc.data_ptr(), None, coarse_smoothed=True)
+    del bd
+    net = CoarseNetwork(hip_ctx, cs, seed=7, coarsen=True)
+    err0 = coarse_error_dev(pmg, net, bc[ns - nh:].data_ptr(), nh)
+    hist = train_energy_dev(pmg, net, bc.data_ptr(), ns - nh, steps=500, batch=4, lr=1e-3)
+    err1 = coarse_error_dev(pmg, net, bc[ns - nh:].data_ptr(), nh)
+    print(f"energy loss first/last: {hist[:5].mean():.4f} {hist[-20:].mean():.4f}; held-out energy error "
+          f"{err0:.3f} -> {err1:.3f}", flush=True)
+    assert np.all(np.isfinite(hist))
+    assert hist[-20:].mean() < hist[:5].mean() - 1.0
+    assert err1 < 0.85 * err0
+    rng = np.random.default_rng(1)
     b = rng.standard_normal(fs.nfree)
     sols = {}
-    for mode in ("off", "warmstart", "replace"):
+    for mode in ("off", "warmstart", "cycle", "precondition"):
         set_coarse_network(pmg, None if mode == "off" else net, mode)
         x = np.zeros(fs.nfree)
         info0 = pmg.info()["coarse_iters"]
-        res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=200, preconditioner=pmg).solve(x, mf, b)
+        res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=400, preconditioner=pmg).solve(x, mf, b)
         inner = pmg.info()["coarse_iters"] - info0
         print(f"mode {mode}: {res}, coarse iterations {inner}", flush=True)
         assert res.converged
         sols[mode] = (x, res.iterations, inner)
-    assert rel(sols["warmstart"][0], sols["off"][0]) < 1e-8
-    assert rel(sols["replace"][0], sols["off"][0]) < 1e-8
-    assert sols["warmstart"][2] < sols["off"][2]
+    for mode in ("warmstart", "cycle", "precondition"):
+        assert rel(sols[mode][0], sols["off"][0]) < 1e-7
+    # the trained network as the coarse preconditioner: the outer iterations stay, the coarse work collapses
+    assert sols["precondition"][1] <= sols["off"][1] + 1
+    assert sols["precondition"][2] < 0.2 * sols["off"][2]
+    set_coarse_network(pmg, None, "off")
