@@ -63,13 +63,19 @@ def cpu_baseline(n=32, reps=3):
                       f"({sp.nfree} free dofs), min of {reps} after 1 warm-up, {t * 1e3:.1f} ms per Jv"}
 
 
-def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8):
+def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8,
+              learned=None):
     """CG-type solve J(x) du = -R(x) to ||b - J du|| <= rtol ||b|| with one p-multigrid V-cycle per iteration:
     Q2 -> assembled Q1 (fine quadrature), Chebyshev degree `degree` (range 20, inflate 1.1, 10 Arnoldi steps),
     coarse block-Jacobi PCG to coarse_rtol; flexible CG outside (the inexact coarse solve makes the cycle vary).
     Same law and state recipe as the Jv workload; both levels clamp the closed faces x = 0 and x = 1 (the p-MG
     levels need one closed Dirichlet set).  Returns wall times of the update (prepare + levels + smoothers) and
-    of the solve, iterations and the final true residual."""
+    of the solve, iterations and the final true residual.
+
+    ``learned`` (dict with samples, steps, seed): a coarse U-Net is trained on this operator with the library's
+    device kernels -- energy loss on coarse-smoothed residuals of `samples` random N(0,1) right-hand sides that
+    the plain p-multigrid produces, none of them the solve's own right-hand side -- and then preconditions the
+    coarse flexible CG (mode "precondition").  Data generation and training are timed apart from the solve."""
     import numpy as np
     import torch
 
@@ -96,6 +102,7 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     sptr = torch.cuda.current_stream(dev).cuda_stream
     solver = MatrixFreeSolver("fcg", rtol=rtol, maxiter=200, preconditioner=pmg)
     out = {}
+    train = None
     for rep in range(2):  # the first pass also builds the coarse pattern and the work buffers
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -103,16 +110,53 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
         torch.cuda.synchronize()
         pmg.update()
         torch.cuda.synchronize()
+        if learned and train is None:
+            train = train_coarse_network(pmg, fs, cs, dev, **learned)
+            torch.cuda.synchronize()
+        inner0 = pmg.info()["coarse_iters"]
         t1 = time.perf_counter()
         res = solver.solve_dev(sol_d.data_ptr(), mf, bd.data_ptr(), sptr)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         info = pmg.info()
         out = {"update_s": t1 - t0, "solve_s": t2 - t1, "iterations": res.iterations, "converged": res.converged,
-               "rel_residual": res.residual / res.rhs_norm, "coarse_iters_total": info["coarse_iters"],
+               "rel_residual": res.residual / res.rhs_norm, "coarse_iters": info["coarse_iters"] - inner0,
                "cycles": info["cycles"], "interval": [info["levels"][0]["a"], info["levels"][0]["b"]],
                "free_dofs": fs.nfree, "coarse_free_dofs": cs.nfree}
+    if train:
+        out.update(train)
     return out
+
+
+def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11):
+    """Train the coarse U-Net (every other Q1 vertex) on the p-multigrid's own coarse-smoothed residuals of random
+    right-hand sides and attach it as the coarse preconditioner; returns the timings and training figures."""
+    import torch
+
+    from hfk.learned import CoarseNetwork, coarse_error_dev, coarse_pairs_dev, set_coarse_network, train_energy_dev
+
+    t0 = time.perf_counter()
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    held = 16
+    bc = torch.zeros((samples + held, cs.nfree), dtype=torch.float64, device=dev)
+    chunk = 32  # fine right-hand sides in flight (6.3M doubles each at 64^3)
+    for k0 in range(0, samples + held, chunk):
+        k1 = min(k0 + chunk, samples + held)
+        bf = torch.randn((k1 - k0, fs.nfree), dtype=torch.float64, device=dev, generator=gen)
+        coarse_pairs_dev(pmg, bf.data_ptr(), k1 - k0, bc[k0:].data_ptr(), None, coarse_smoothed=True)
+        del bf
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    net = CoarseNetwork(pmg.levels[0].ctx, cs, seed=seed, coarsen=True)
+    hist = train_energy_dev(pmg, net, bc.data_ptr(), samples, steps, batch=4, lr=1e-3, seed=seed)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    err = coarse_error_dev(pmg, net, bc[samples:].data_ptr(), held)
+    set_coarse_network(pmg, net, "precondition")  # the hierarchy keeps a reference to the network
+    return {"train_data_s": t1 - t0, "train_s": t2 - t1, "train_samples": samples, "train_steps": steps,
+            "train_batch": 4, "train_loss_first_last": [float(hist[:5].mean()), float(hist[-20:].mean())],
+            "heldout_energy_error": err, "network_params": net.nparams,
+            "network_lattice": (net.N + 1) // 2, "coarse_mode": "precondition"}
 
 
 def main():
@@ -127,7 +171,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (the Jv shards by cell partition; see DESIGN.md); "
+        print("bench.py: this build measures one GPU only (the Jv would shard by cell partition across GPUs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
@@ -253,9 +297,12 @@ def main():
     res["variants"] = variants
     if not args.no_solve:
         del mf
-        print("bench: solve legs (FP64 p-MG; mixed precision FP32 smoothers)", file=sys.stderr, flush=True)
+        print("bench: solve legs (FP64 p-MG; mixed precision FP32 smoothers; mixed + learned coarse level)",
+              file=sys.stderr, flush=True)
         res["solve_fp64_pmg"] = solve_leg(n, "full64", None)
         res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32")
+        res["solve_mixed_learned"] = solve_leg(n, "sym64", "sym32", learned={"samples": 160, "steps": 500})
+        res["solve_learned_speedup_vs_fp64"] = res["solve_fp64_pmg"]["solve_s"] / res["solve_mixed_learned"]["solve_s"]
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()
