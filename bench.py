@@ -120,6 +120,7 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
         t2 = time.perf_counter()
         info = pmg.info()
         out = {"update_s": t1 - t0, "solve_s": t2 - t1, "iterations": res.iterations, "converged": res.converged,
+               "coarse_rtol": coarse_rtol,
                "rel_residual": res.residual / res.rhs_norm, "coarse_iters": info["coarse_iters"] - inner0,
                "cycles": info["cycles"], "interval": [info["levels"][0]["a"], info["levels"][0]["b"]],
                "free_dofs": fs.nfree, "coarse_free_dofs": cs.nfree}
@@ -128,7 +129,7 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     return out
 
 
-def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11):
+def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoother_degree=0):
     """Train the coarse U-Net (every other Q1 vertex) on the p-multigrid's own coarse-smoothed residuals of random
     right-hand sides and attach it as the coarse preconditioner; returns the timings and training figures."""
     import torch
@@ -152,11 +153,12 @@ def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     err = coarse_error_dev(pmg, net, bc[samples:].data_ptr(), held)
-    set_coarse_network(pmg, net, "precondition")  # the hierarchy keeps a reference to the network
+    set_coarse_network(pmg, net, "precondition", smoother_degree)  # the hierarchy keeps a reference to the network
     return {"train_data_s": t1 - t0, "train_s": t2 - t1, "train_samples": samples, "train_steps": steps,
             "train_batch": 4, "train_loss_first_last": [float(hist[:5].mean()), float(hist[-20:].mean())],
             "heldout_energy_error": err, "network_params": net.nparams,
-            "network_lattice": (net.N + 1) // 2, "coarse_mode": "precondition"}
+            "network_lattice": (net.N + 1) // 2, "coarse_mode": "precondition",
+            "coarse_smoother_degree": smoother_degree or pmg.smoother.degree}
 
 
 def main():
@@ -169,6 +171,7 @@ def main():
     ap.add_argument("--variants", default="sym64,full32,sym32", help="extra storages timed beside the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solve-legs", default="fp64,mixed,learned", help="solve legs: fp64, mixed, learned")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the Jv would shard by cell partition across GPUs); "
@@ -299,10 +302,18 @@ def main():
         del mf
         print("bench: solve legs (FP64 p-MG; mixed precision FP32 smoothers; mixed + learned coarse level)",
               file=sys.stderr, flush=True)
-        res["solve_fp64_pmg"] = solve_leg(n, "full64", None)
-        res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32")
-        res["solve_mixed_learned"] = solve_leg(n, "sym64", "sym32", learned={"samples": 160, "steps": 500})
-        res["solve_learned_speedup_vs_fp64"] = res["solve_fp64_pmg"]["solve_s"] / res["solve_mixed_learned"]["solve_s"]
+        legs = [s for s in args.solve_legs.split(",") if s]
+        # each leg at the coarse tolerance that was fastest for it over {1e-2, 3e-2, 1e-1, 3e-1} at 64^3
+        if "fp64" in legs:
+            res["solve_fp64_pmg"] = solve_leg(n, "full64", None, coarse_rtol=0.3)
+        if "mixed" in legs:
+            res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3)
+        if "learned" in legs:
+            res["solve_mixed_learned"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.1,
+                                                   learned={"samples": 160, "steps": 500, "smoother_degree": 3})
+        if "fp64" in legs and "learned" in legs:
+            res["solve_learned_speedup_vs_fp64"] = (res["solve_fp64_pmg"]["solve_s"] /
+                                                    res["solve_mixed_learned"]["solve_s"])
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

@@ -476,6 +476,7 @@ struct hfk_pmg {
   DevBuf cr, cz, cp, cq, ct, cu, cs;  // inner CG work vectors and device scalars
   std::unique_ptr<hfk_cheb> ccheb;      // Chebyshev on the assembled coarse matrix (network cycle, modes 3 and 4)
   bool ccheb_stale = true;
+  int ccheb_degree = 0;  // its degree (0: the matrix-free levels' degree)
   Dots dots;
   bool ok = false, indefinite = false, built = false, coarse_ok = false;
   int nupdate = 0, ncycle = 0, last_inner = 0, total_inner = 0;
@@ -524,7 +525,8 @@ int coarse_smoother(hfk_pmg* g, hipStream_t s, bool* usable) {
   if (!g->ccheb) {
     const hfk_cheb* f = g->cheb[0].get();
     std::unique_ptr<hfk_cheb> ch(new hfk_cheb());
-    if ((rc = cheb_init(c, f->degree, f->range, f->inflate, f->eig_iters, ch.get()))) return rc;
+    if ((rc = cheb_init(c, g->ccheb_degree > 0 ? g->ccheb_degree : f->degree, f->range, f->inflate, f->eig_iters, ch.get())))
+      return rc;
     ch->csr_colptr = g->colptr.as<int32_t>();
     ch->csr_rowval = g->rowval.as<int32_t>();
     ch->csr_val = g->nzval.as<double>();
@@ -1073,13 +1075,17 @@ int hfk_pmg_info(hfk_pmg* g, double* info) {
   return HFK_OK;
 }
 
-int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode) {
+int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree) {
   if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  if (smoother_degree < 0) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: smoother degree must be >= 0");
   if (mode < 0 || mode > 4) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: coarse network mode must be 0 to 4");
   if (mode > 0 && (!net || net_ncoarse(net) != g->ops.back()->nfree))
     return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: the network is not built for this coarse level");
   g->net = mode > 0 ? net : nullptr;
   g->net_mode = mode;
+  const int deg = smoother_degree > 0 ? smoother_degree : g->cheb[0]->degree;
+  if (g->ccheb && g->ccheb->degree != deg) g->ccheb->degree = deg;  // the interval does not depend on the degree
+  g->ccheb_degree = smoother_degree;
   return HFK_OK;
 }
 

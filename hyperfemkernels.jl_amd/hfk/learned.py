@@ -124,9 +124,11 @@ def coarse_error_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, stre
     return err.value
 
 
-def set_coarse_network(pmg, net: CoarseNetwork | None, mode: str = "warmstart"):
+def set_coarse_network(pmg, net: CoarseNetwork | None, mode: str = "warmstart", smoother_degree: int = 0):
+    """Attach ``net`` to the coarse level of ``pmg`` (modes in MODES); ``smoother_degree`` is the degree of the
+    coarse Chebyshev smoother of the network cycle (0: the p-multigrid's own degree)."""
     if mode not in MODES:
         raise ValueError(f"HyperFEMKernels: unknown coarse network mode {mode!r}")
-    check(pmg.lib.hfk_pmg_set_coarse_network(pmg.handle, None if net is None else net.handle, MODES[mode]),
-          pmg.levels[0].ctx.handle)
+    check(pmg.lib.hfk_pmg_set_coarse_network(pmg.handle, None if net is None else net.handle, MODES[mode],
+                                             int(smoother_degree)), pmg.levels[0].ctx.handle)
     pmg.coarse_network = (net, mode)
