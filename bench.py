@@ -64,13 +64,16 @@ def cpu_baseline(n=32, reps=3):
 
 
 def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8,
-              learned=None):
+              learned=None, order=2, mu=None, state_amp=1e-2):
     """CG-type solve J(x) du = -R(x) to ||b - J du|| <= rtol ||b|| with one p-multigrid V-cycle per iteration:
     Q2 -> assembled Q1 (fine quadrature), Chebyshev degree `degree` (range 20, inflate 1.1, 10 Arnoldi steps),
     coarse block-Jacobi PCG to coarse_rtol; flexible CG outside (the inexact coarse solve makes the cycle vary).
     Same law and state recipe as the Jv workload; both levels clamp the closed faces x = 0 and x = 1 (the p-MG
     levels need one closed Dirichlet set).  Returns wall times of the update (prepare + levels + smoothers) and
     of the solve, iterations and the final true residual.
+
+    ``order`` is the fine degree (Q2, Q3; quadrature degree 2 order), ``mu`` an optional lognormal shear modulus
+    (dict seed, sigma, ell: exp(sigma g), g a unit Gaussian field of correlation length ell).
 
     ``learned`` (dict with samples, steps, seed): a coarse U-Net is trained on this operator with the library's
     device kernels -- energy loss on coarse-smoothed residuals of `samples` random N(0,1) right-hand sides that
@@ -84,13 +87,17 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     from hfk.solvers import Chebyshev, MatrixFreeSolver, PMultigrid
     from hfk.synthetic import smooth_field
 
-    fs = hex_space((n, n, n), 2, dirichlet=dirichlet_closed_x_faces)
+    fs = hex_space((n, n, n), order, dirichlet=dirichlet_closed_x_faces)
     cs = hex_space((n, n, n), 1, dirichlet=dirichlet_closed_x_faces)
-    u = smooth_field(2)
+    u = smooth_field(2, amp=state_amp)
     x = fs.free_values(u)
-    form = HyperelasticForm("neohookean", 2.0, 1.0, degree=4)
+    form = HyperelasticForm("neohookean", 2.0, 1.0, degree=2 * order)
     fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u))
     cop = KernelOperator(form, cs, jacobian="assembled")
+    if mu is not None:  # lognormal shear modulus at the quadrature points (the coarse level keeps the fine rule)
+        mup = lognormal_points(mu, fs.quadrature_points(form.points1d).reshape(-1, 3))
+        fop.set_mu_field(mup)
+        cop.set_mu_field(mup)
     b = -fop.residual(np.zeros(fs.nfree), x)
     mf = MatrixFreeJacobian(fop, outer_storage)
     pmg = PMultigrid([fop, cop], Chebyshev(degree=degree), coarse_rtol=coarse_rtol, coarse_maxiter=coarse_maxiter,
@@ -120,12 +127,33 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
         t2 = time.perf_counter()
         info = pmg.info()
         out = {"update_s": t1 - t0, "solve_s": t2 - t1, "iterations": res.iterations, "converged": res.converged,
-               "coarse_rtol": coarse_rtol,
+               "coarse_rtol": coarse_rtol, "indefinite": res.indefinite, "pmg_ok": info["ok"],
+               "state_amp": state_amp,
                "rel_residual": res.residual / res.rhs_norm, "coarse_iters": info["coarse_iters"] - inner0,
                "cycles": info["cycles"], "interval": [info["levels"][0]["a"], info["levels"][0]["b"]],
                "free_dofs": fs.nfree, "coarse_free_dofs": cs.nfree}
     if train:
         out.update(train)
+    return out
+
+
+def lognormal_points(mu, xyz):
+    """exp(sigma g(x)) at the points xyz (m, 3), g the random-Fourier-feature field of hfk.synthetic, evaluated on
+    the device in chunks (synthetic input generation only)."""
+    import numpy as np
+    import torch
+
+    from hfk.synthetic import lognormal_features
+
+    w, b = lognormal_features(mu["seed"], mu["ell"])
+    dev = torch.device("cuda:0")
+    wt = torch.from_numpy(w.T.copy()).to(dev)
+    bt = torch.from_numpy(b).to(dev)
+    out = np.empty(xyz.shape[0])
+    for s in range(0, xyz.shape[0], 1 << 18):
+        xt = torch.from_numpy(np.ascontiguousarray(xyz[s:s + (1 << 18)])).to(dev)
+        g = np.sqrt(2.0 / w.shape[0]) * torch.cos(xt @ wt + bt).sum(dim=1)
+        out[s:s + (1 << 18)] = torch.exp(mu["sigma"] * g).cpu().numpy()
     return out
 
 
@@ -161,6 +189,71 @@ def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoo
             "coarse_smoother_degree": smoother_degree or pmg.smoother.degree}
 
 
+def time_jv(op, storage, xd, vd, yd, stream, steps, warmup):
+    """prepare! then `steps` products y = J v on `stream`, each bracketed by HIP events recorded on that stream;
+    returns the MatrixFreeJacobian, the prepare time, the per-product times and the wall time of the loop."""
+    import numpy as np
+    import torch
+
+    from hfk import MatrixFreeJacobian
+
+    sptr = stream.cuda_stream
+    mf = MatrixFreeJacobian(op, storage)
+    torch.cuda.synchronize()
+    # prepare (once per Newton step in a solve), timed with events too
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    mf.prepare_dev(xd.data_ptr(), sptr)
+    e0.record(stream)
+    mf.prepare_dev(xd.data_ptr(), sptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_prep = e0.elapsed_time(e1) * 1e-3
+    for _ in range(warmup):
+        mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t_start = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    per = np.array([a.elapsed_time(b) * 1e-3 for a, b in evs])
+    return mf, t_prep, per, wall
+
+
+def precision_leg(n, order, stream, storages=("full64", "full32", "sym32"), steps=20):
+    """The Q_order workload on the unit cube n^3 (neo-Hookean, the smooth state of seed 2, v ~ N(0,1) seed 3,
+    open x faces): Jv time per tangent storage and the relative error of each product against the FP64 one."""
+    import numpy as np
+    import torch
+
+    from hfk import HyperelasticForm, KernelOperator
+    from hfk.mesh import hex_space
+    from hfk.synthetic import normal, smooth_field
+
+    sp = hex_space((n, n, n), order)
+    u = smooth_field(2)
+    op = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=2 * order), sp, jacobian="matrixfree",
+                        dirichlet_values=sp.dirichlet_values(u))
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(sp.free_values(u)).to(dev)
+    vd = torch.from_numpy(normal(3, sp.nfree)).to(dev)
+    yd = torch.empty_like(vd)
+    out, y64 = {"free_dofs": sp.nfree, "cells": sp.ncells, "points_per_cell": (order + 1) ** 3}, None
+    for st in storages:
+        mf, tp, per, _ = time_jv(op, st, xd, vd, yd, stream, steps, 2)
+        y = yd.cpu().numpy()
+        if y64 is None:
+            y64 = y
+        t = float(per.mean())
+        out[st] = {"jv_ms": t * 1e3, "gdofs": sp.nfree / t / 1e9, "prepare_ms": tp * 1e3,
+                   "rel_err_vs_full64": float(np.linalg.norm(y - y64) / np.linalg.norm(y64))}
+        del mf
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +264,7 @@ def main():
     ap.add_argument("--variants", default="sym64,full32,sym32", help="extra storages timed beside the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--solve-legs", default="fp64,mixed,learned", help="solve legs: fp64, mixed, learned")
+    ap.add_argument("--solve-legs", default="fp64,mixed,learned,q3", help="solve legs: fp64, mixed, learned, q3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the Jv would shard by cell partition across GPUs); "
@@ -207,29 +300,7 @@ def main():
     yd = torch.empty_like(vd)
 
     def time_storage(storage, steps, warmup):
-        mf = MatrixFreeJacobian(op, storage)
-        torch.cuda.synchronize()
-        # prepare (once per Newton step in a solve), timed with events too
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        mf.prepare_dev(xd.data_ptr(), sptr)
-        e0.record(stream)
-        mf.prepare_dev(xd.data_ptr(), sptr)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        t_prep = e0.elapsed_time(e1) * 1e-3
-        for _ in range(warmup):
-            mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
-        torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-        t_start = time.perf_counter()
-        for a, b in evs:
-            a.record(stream)
-            mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
-            b.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t_start
-        per = np.array([a.elapsed_time(b) * 1e-3 for a, b in evs])
-        return mf, t_prep, per, wall
+        return time_jv(op, storage, xd, vd, yd, stream, steps, warmup)
 
     # a plain device copy of 1 GiB: the bandwidth an HBM-bound kernel can reach on this box (read + write bytes)
     cpy_src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
@@ -314,6 +385,22 @@ def main():
         if "fp64" in legs and "learned" in legs:
             res["solve_learned_speedup_vs_fp64"] = (res["solve_fp64_pmg"]["solve_s"] /
                                                     res["solve_mixed_learned"]["solve_s"])
+        if "q3" in legs:
+            # Q3 on 48^3 (64 points per cell): FP32 Jv against FP64; mixed-precision solves (FP32 smoothers, FP64
+            # outer FCG and coarse level) with mu = 1 and with a lognormal mu (sigma 1, correlation length 0.1)
+            print("bench: Q3 48^3 legs (FP32 Jv error; mixed-precision solves, mu = 1 and lognormal)",
+                  file=sys.stderr, flush=True)
+            res["q3_48_jv"] = precision_leg(48, 3, stream)
+            res["q3_48_solve_fp64_pmg"] = solve_leg(48, "full64", None, coarse_rtol=0.3, order=3)
+            res["q3_48_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3)
+            # with the lognormal mu the state of amplitude 1e-2 makes the tangent indefinite where mu is small
+            # (ln J > 2 mu / lam): the CG stops on negative curvature, as the reference's cg with linesearch.  The
+            # lognormal legs linearise at the same smooth state scaled to amplitude 3e-3 (positive definite).
+            lmu = {"seed": 4, "sigma": 1.0, "ell": 0.1}
+            res["q3_48_lognormal_solve_fp64_pmg"] = solve_leg(48, "full64", None, coarse_rtol=0.3, order=3, mu=lmu,
+                                                              state_amp=3e-3)
+            res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
+                                                               state_amp=3e-3)
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

@@ -95,6 +95,19 @@ class HexSpace:
             out[self.dir_node_of, c] = -(3 * np.arange(self.dir_node_of.size) + c + 1)
         return out
 
+    def quadrature_points(self, points1d: int) -> np.ndarray:
+        """Physical coordinates (ncells, P^3, 3) of the tensor Gauss-Legendre points of every cell (P = points1d
+        per direction, x fastest), through the trilinear map of the cell's vertices."""
+        g = (np.polynomial.legendre.leggauss(int(points1d))[0] + 1.0) / 2.0
+        P = g.size
+        xi = np.array([[g[a], g[b], g[c]] for c in range(P) for b in range(P) for a in range(P)])
+        xq = np.zeros((self.ncells, P ** 3, 3))
+        for v in range(8):
+            a, b, c = v & 1, (v >> 1) & 1, (v >> 2) & 1
+            w = (xi[:, 0] if a else 1 - xi[:, 0]) * (xi[:, 1] if b else 1 - xi[:, 1]) * (xi[:, 2] if c else 1 - xi[:, 2])
+            xq += w[None, :, None] * self.coords[:, v, None, :]
+        return xq
+
     def free_values(self, fun) -> np.ndarray:
         v = np.asarray(fun(self.node_xyz[self.free_node_of]), dtype=np.float64).reshape(-1, 3)
         return v.reshape(-1).copy()

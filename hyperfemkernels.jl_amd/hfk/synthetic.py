@@ -61,10 +61,17 @@ def smooth_field(seed: int, modes: int = 8, amp: float = 1e-2):
     return u
 
 
-def lognormal_field(seed: int, sigma: float = 1.0, ell: float = 0.1, features: int = 1024):
-    """``mu_factor(x) = exp(sigma g(x))`` (callable on (m, 3) -> (m,))."""
+def lognormal_features(seed: int, ell: float = 0.1, features: int = 1024):
+    """Random Fourier features of the unit-variance Gaussian field with squared-exponential correlation length
+    ``ell``: g(x) = sqrt(2/M) sum_m cos(w_m . x + b_m), w ~ N(0, ell^-2 I), b ~ U[0, 2 pi)."""
     w = normal(seed, 3 * features).reshape(features, 3) / ell
     b = uniform(seed + 1, features) * 2 * np.pi
+    return w, b
+
+
+def lognormal_field(seed: int, sigma: float = 1.0, ell: float = 0.1, features: int = 1024):
+    """``mu_factor(x) = exp(sigma g(x))`` (callable on (m, 3) -> (m,))."""
+    w, b = lognormal_features(seed, ell, features)
 
     def f(xyz):
         xyz = np.asarray(xyz, dtype=np.float64)

@@ -96,12 +96,16 @@ def test_mu_field(oracle, hip_ctx):
     assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < TOL
 
 
-@pytest.mark.parametrize("order,law,n", [(1, "neohookean", (4, 3, 3)), (2, "svk", (3, 2, 2)), (2, "neohookean", (4, 4, 3))])
-def test_pattern_and_assembly(oracle, hip_ctx, order, law, n):
-    print(f"assembly order={order} law={law} n={n}", flush=True)
-    sp, dv, x = problem(order, order + 1, law, n, 0.1)
-    op = KernelOperator(HyperelasticForm(law, degree=2 * order), sp, dirichlet_values=dv, ctx=hip_ctx)
-    orc = oracle.OracleOp(sp, law, 2.0, 1.0, dvals=dv)
+# (order, points per direction): the coarse Q1 levels of the p-multigrid keep the fine rule (3 points under Q2,
+# 4 under Q3)
+@pytest.mark.parametrize("order,P,law,n", [(1, 2, "neohookean", (4, 3, 3)), (1, 3, "neohookean", (3, 3, 2)),
+                                           (1, 4, "svk", (3, 2, 2)), (2, 3, "svk", (3, 2, 2)),
+                                           (2, 3, "neohookean", (4, 4, 3)), (3, 4, "neohookean", (2, 2, 1))])
+def test_pattern_and_assembly(oracle, hip_ctx, order, P, law, n):
+    print(f"assembly order={order} P={P} law={law} n={n}", flush=True)
+    sp, dv, x = problem(order, P, law, n, 0.1)
+    op = KernelOperator(HyperelasticForm(law, degree=2 * P - 2), sp, dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, points1d=P, dvals=dv)
     colptr, rowval, nzpos = op.pattern()
     c2, r2, p2 = orc.pattern()
     assert np.array_equal(colptr, c2) and np.array_equal(rowval, r2) and np.array_equal(nzpos, p2)

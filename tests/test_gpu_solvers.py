@@ -136,6 +136,37 @@ def test_pmultigrid_solution_equals_direct(hip_ctx, oracle):
     assert res2.iterations > res.iterations
 
 
+def test_pmultigrid_q3_lognormal(hip_ctx, oracle):
+    """Q3 working level over an assembled Q1 coarse level with the fine 4-point rule, heterogeneous (lognormal)
+    shear modulus, FP32 smoothers: the flexible CG reaches the direct solution."""
+    from hfk.synthetic import lognormal_field, smooth_field
+
+    n = (3, 3, 3)
+    fs = hex_space(n, 3, dirichlet=dirichlet_closed_x0)
+    cs = hex_space(n, 1, dirichlet=dirichlet_closed_x0)
+    u = smooth_field(2, amp=0.01)
+    x = fs.free_values(u)
+    form = HyperelasticForm("neohookean", degree=6)
+    assert form.points1d == 4
+    fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u), ctx=hip_ctx)
+    cop = KernelOperator(form, cs, jacobian="assembled", ctx=hip_ctx)
+    mup = lognormal_field(4, sigma=1.0, ell=0.3)(fs.quadrature_points(4).reshape(-1, 3))
+    fop.set_mu_field(mup)
+    cop.set_mu_field(mup)
+    mf = MatrixFreeJacobian(fop).prepare(x)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-2, smoother_storage="sym32").update()
+    assert pmg.ok
+    b = np.random.default_rng(12).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
+    print("q3 pmg fcg:", res, flush=True)
+    assert res.converged and res.iterations <= 30
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(u))
+    orc.set_mu_field(mup)
+    xd = spla.spsolve(orc.csc(x).tocsc(), b)
+    assert rel(xs, xd) < 1e-7
+
+
 @pytest.mark.parametrize("smoother_storage", ["sym32", "full32", "sym64"])
 def test_pmultigrid_mixed_precision(hip_ctx, oracle, smoother_storage):
     """FP32 smoothers and level applies inside an FP64 flexible CG: the Krylov tolerance fixes the answer."""
