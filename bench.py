@@ -58,9 +58,86 @@ def cpu_baseline(n=32, reps=3):
         op.jv(prep, v, nthreads=cores)
         ts.append(time.perf_counter() - t0)
     t = min(ts)
+    # the quick-start problem's residual and assembled Jacobian on the host (the oracle's serial FP64 path)
+    qs, qdv = svk_quickstart_space()
+    qop = orc.OracleOp(qs, "svk", 2.0, 1.0, dvals=qdv)
+    qx = 0.01 * normal(1, qs.nfree)
+    t0 = time.perf_counter()
+    qop.residual(qx)
+    t_res = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, rowval, nzpos = qop.pattern()
+    t_pat = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    qop.jacobian(qx, nzpos, rowval.size)
+    t_jac = time.perf_counter() - t0
     return {"value": sp.nfree / t / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "port",
             "sample": f"oracle stored-T sum-factorised Jv (OpenMP, colour chunks), Q2 neo-Hookean {n}^3 "
-                      f"({sp.nfree} free dofs), min of {reps} after 1 warm-up, {t * 1e3:.1f} ms per Jv"}
+                      f"({sp.nfree} free dofs), min of {reps} after 1 warm-up, {t * 1e3:.1f} ms per Jv",
+            "svk_quickstart_host": {"residual_ms": t_res * 1e3, "pattern_ms": t_pat * 1e3,
+                                    "jacobian_ms": t_jac * 1e3, "cores": 1,
+                                    "sample": "oracle residual, pattern and Jacobian of the 20x10x10 Q2 SVK "
+                                              "quick start at x = 0.01 N(0,1) (seed 1), one call each"}}
+
+
+def svk_quickstart_space():
+    """The reference's quick start (README.md:61-83): unit cube 20 x 10 x 10 Q2 hexes, quadrature degree 4,
+    SVK lam = 2, mu = 1; face x = 0 clamped, face x = 1 moved by (0.05, 0, 0) (open faces)."""
+    import numpy as np
+
+    from hfk.mesh import dirichlet_open_x_faces, hex_space
+
+    sp = hex_space((20, 10, 10), 2, dirichlet=dirichlet_open_x_faces)
+    dv = sp.dirichlet_values(lambda X: np.stack([0.05 * X[:, 0], 0.0 * X[:, 0], 0.0 * X[:, 0]], axis=1))
+    return sp, dv
+
+
+def newton_leg(rtol=1e-10, maxit=20):
+    """Newton on the quick-start problem from x = 0 to ||R(x)|| <= rtol ||R(0)||: residual! and jacobian! on the
+    GPU through the host-array boundary (as the reference's Algebra interface), the linear systems by a host
+    sparse LU (SuperLU) of the device-assembled CSC Jacobian.  Times each call; the first jacobian! also builds
+    the sparsity pattern (timed apart)."""
+    import numpy as np
+    import scipy.sparse.linalg as spla
+
+    from hfk import HyperelasticForm, KernelOperator
+
+    sp, dv = svk_quickstart_space()
+    t0 = time.perf_counter()
+    op = KernelOperator(HyperelasticForm("svk", 2.0, 1.0, degree=4), sp, jacobian="assembled", dirichlet_values=dv)
+    t_create = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    A = op.allocate_jacobian()
+    t_pattern = time.perf_counter() - t0
+    x = np.zeros(sp.nfree)
+    b = np.zeros(sp.nfree)
+    tr, tj, tl, hist = [], [], [], []
+    t_all = time.perf_counter()
+    t0 = time.perf_counter()
+    op.residual(b, x)
+    tr.append(time.perf_counter() - t0)
+    r0 = float(np.linalg.norm(b))
+    hist.append(r0)
+    it = 0
+    while it < maxit and hist[-1] > rtol * r0:
+        t0 = time.perf_counter()
+        op.jacobian(A, x)
+        tj.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        dx = spla.splu(A.tocsc()).solve(-b)
+        tl.append(time.perf_counter() - t0)
+        x += dx
+        t0 = time.perf_counter()
+        op.residual(b, x)
+        tr.append(time.perf_counter() - t0)
+        hist.append(float(np.linalg.norm(b)))
+        it += 1
+    t_all = time.perf_counter() - t_all
+    return {"free_dofs": sp.nfree, "cells": sp.ncells, "nnz": int(A.nnz), "newton_iterations": it,
+            "converged": hist[-1] <= rtol * r0, "residual_history": hist, "create_s": t_create,
+            "pattern_s": t_pattern, "residual_ms_mean": 1e3 * float(np.mean(tr)),
+            "jacobian_ms_mean": 1e3 * float(np.mean(tj)), "lu_ms_mean": 1e3 * float(np.mean(tl)),
+            "newton_total_s": t_all, "note": "residual!/jacobian! timed through host arrays (PCIe included)"}
 
 
 def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8,
@@ -401,6 +478,8 @@ def main():
                                                               state_amp=3e-3)
             res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
                                                                state_amp=3e-3)
+    print("bench: SVK quick start 20x10x10 Q2, Newton with host LU", file=sys.stderr, flush=True)
+    res["newton_svk_20x10x10"] = newton_leg()
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

@@ -164,3 +164,35 @@ def test_benchmark_size_properties(oracle, hip_ctx):
     assert rel(mf @ (2.0 * v - 3.0 * w), 2.0 * Jv - 3.0 * Jw) < 1e-13
     orc = oracle.OracleOp(sp, "neohookean", 2.0, 1.0, dvals=dvals)
     assert rel(Jv, orc.jv(orc.prepare(x), v)) < TOL
+
+
+def test_newton_quickstart_matches_oracle(oracle, hip_ctx):
+    """The reference's quick start (README.md:61-83) on a 10x5x5 mesh: Newton from x = 0 with host LU on the
+    device residual and CSC Jacobian follows the oracle's Newton iterate by iterate, to 1e-10 of the residual."""
+    import scipy.sparse.linalg as spla
+
+    from hfk.mesh import dirichlet_open_x_faces
+
+    sp = hex_space((10, 5, 5), 2, dirichlet=dirichlet_open_x_faces)
+    dv = sp.dirichlet_values(lambda X: np.stack([0.05 * X[:, 0], 0.0 * X[:, 0], 0.0 * X[:, 0]], axis=1))
+    op = KernelOperator(HyperelasticForm("svk", 2.0, 1.0, degree=4), sp, dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, "svk", 2.0, 1.0, dvals=dv)
+    A = op.allocate_jacobian()
+    x = np.zeros(sp.nfree)
+    xo = np.zeros(sp.nfree)
+    b = op.residual(np.zeros(sp.nfree), x)
+    bo = orc.residual(xo)
+    r0 = np.linalg.norm(bo)
+    assert rel(b, bo) < TOL
+    for k in range(12):
+        op.jacobian(A, x)
+        x += spla.splu(A.tocsc()).solve(-b)
+        xo += spla.splu(orc.csc(xo).tocsc()).solve(-bo)
+        op.residual(b, x)
+        bo = orc.residual(xo)
+        print(f"newton {k}: |R|/|R0| gpu {np.linalg.norm(b) / r0:.3e} oracle {np.linalg.norm(bo) / r0:.3e}", flush=True)
+        assert np.linalg.norm(b - bo) <= 1e-10 * r0
+        assert rel(x, xo) < 1e-10
+        if np.linalg.norm(bo) <= 1e-10 * r0:
+            break
+    assert np.linalg.norm(b) <= 1e-10 * r0 and k <= 8
