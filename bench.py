@@ -234,7 +234,114 @@ def lognormal_points(mu, xyz):
     return out
 
 
-def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoother_degree=0):
+def lognormal_problem(n, seed, state_amp=0.0):
+    """One problem of the learned-preconditioner workload: Q2 neo-Hookean (lam = 2) on the unit cube n^3, both
+    x faces clamped, shear modulus exp(sigma g) with sigma in [0.5, 2] and correlation length ell in [0.05, 0.3]
+    drawn from SplitMix64(seed); linearised at the smooth state of amplitude state_amp (0: the reference
+    configuration, where the tangent is positive definite for any modulus); right-hand side a smooth random field
+    of seed + 1 (unit amplitude) at the free dofs.  Returns the spaces, operators, state, right-hand side and the
+    auxiliary network input log(mu)/2 at the coarse lattice vertices."""
+    import numpy as np
+
+    from hfk import HyperelasticForm, KernelOperator
+    from hfk.mesh import dirichlet_closed_x_faces, hex_space
+    from hfk.synthetic import smooth_field, uniform
+
+    u01 = uniform(seed, 2)
+    mu = {"seed": seed, "sigma": 0.5 + 1.5 * float(u01[0]), "ell": 0.05 + 0.25 * float(u01[1])}
+    fs = hex_space((n, n, n), 2, dirichlet=dirichlet_closed_x_faces)
+    cs = hex_space((n, n, n), 1, dirichlet=dirichlet_closed_x_faces)
+    u = smooth_field(2, amp=state_amp)
+    form = HyperelasticForm("neohookean", 2.0, 1.0, degree=4)
+    fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u))
+    cop = KernelOperator(form, cs, jacobian="assembled")
+    mup = lognormal_points(mu, fs.quadrature_points(form.points1d).reshape(-1, 3))
+    fop.set_mu_field(mup)
+    cop.set_mu_field(mup)
+    b = smooth_field(seed + 1, amp=1.0)(fs.node_xyz[fs.free_node_of]).reshape(-1)
+    aux = np.log(lognormal_points(mu, cs.node_xyz)) / 2.0
+    return {"fs": fs, "cs": cs, "fop": fop, "cop": cop, "x": fs.free_values(u), "b": b, "aux": aux, "mu": mu}
+
+
+def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(10000, 10010), samples=24,
+                       steps=3000, rtol=1e-8, mode="cycle", scaled=True):
+    """The learned coarse correction across modulus fields: one network (U-Net on every other coarse vertex, log mu
+    as auxiliary input, acting on the diagonally scaled coarse system) trained on the plain p-MG's coarse-smoothed
+    residuals of `samples` random right-hand sides on each training field, then on every held-out field (seeds never
+    trained on) FCG + p-MG with the network's two-grid cycle as the coarse correction (mode "cycle"; FP32
+    smoothers, FP64 outer FCG and coarse matrix) against the plain FP64 p-MG (coarse PCG to 0.3, its best in the
+    64^3 sweep), both to rtol.  Solve times are the second of two identical solves."""
+    import numpy as np
+    import torch
+
+    from hfk import MatrixFreeJacobian
+    from hfk.learned import CoarseNetwork, coarse_pairs_dev, set_coarse_network, train_energy_problems
+    from hfk.solvers import Chebyshev, MatrixFreeSolver, PMultigrid
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    train, keep = [], []
+    for sd in train_seeds:
+        p = lognormal_problem(n, sd)
+        mf = MatrixFreeJacobian(p["fop"], "sym64").prepare(p["x"])
+        pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=0.1, coarse_maxiter=400,
+                         smoother_storage="sym32").update()
+        if not pmg.ok:
+            continue
+        gen = torch.Generator(device=dev).manual_seed(sd)
+        bf = torch.randn((samples, p["fs"].nfree), dtype=torch.float64, device=dev, generator=gen)
+        bc = torch.zeros((samples, p["cs"].nfree), dtype=torch.float64, device=dev)
+        set_coarse_network(pmg, None, "off", 3, scaled)
+        coarse_pairs_dev(pmg, bf.data_ptr(), samples, bc.data_ptr(), None, coarse_smoothed=True)
+        del bf
+        train.append((pmg, bc.data_ptr(), samples, p["aux"]))
+        keep.append((p, mf, pmg, bc))
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    cs0 = keep[0][0]["cs"]
+    net = CoarseNetwork(keep[0][0]["fop"].ctx, cs0, seed=5, coarsen=True)
+    t0 = time.perf_counter()
+    hist = train_energy_problems(train, net, steps, chunk=25, batch=4, lr=1e-3, seed=5)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del train, keep
+    rows = []
+    for sd in test_seeds:
+        p = lognormal_problem(n, sd)
+        row = {"seed": sd, "sigma": p["mu"]["sigma"], "ell": p["mu"]["ell"]}
+        for leg in ("fp64", "learned"):
+            st, sst, crt = ("full64", None, 0.3) if leg == "fp64" else ("sym64", "sym32", 0.1)
+            mf = MatrixFreeJacobian(p["fop"], st).prepare(p["x"])
+            pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=crt, coarse_maxiter=400,
+                             smoother_storage=sst).update()
+            if leg == "learned":
+                net.set_aux(p["aux"])
+                set_coarse_network(pmg, net, mode, 3, scaled)
+            solver = MatrixFreeSolver("fcg", rtol=rtol, maxiter=300, preconditioner=pmg)
+            for _ in range(2):
+                xs = np.zeros(p["fs"].nfree)
+                t1 = time.perf_counter()
+                res = solver.solve(xs, mf, p["b"])
+                ts = time.perf_counter() - t1
+            row[leg] = {"solve_s": ts, "iterations": res.iterations, "converged": res.converged,
+                        "rel_residual": res.residual / res.rhs_norm}
+            set_coarse_network(pmg, None, "off")
+            del pmg, mf
+        rows.append(row)
+    t_fp = np.array([r["fp64"]["solve_s"] for r in rows])
+    t_ln = np.array([r["learned"]["solve_s"] for r in rows])
+    return {"n": n, "free_dofs": int(3 * (2 * n + 1) ** 2 * (2 * n - 1)), "coarse_mode": mode, "scaled": scaled,
+            "train_fields": len(list(train_seeds)),
+            "samples_per_field": samples, "train_steps": steps, "train_data_s": t_data, "train_s": t_train,
+            "train_loss_first_last": [float(hist[:25].mean()), float(hist[-50:].mean())],
+            "heldout_fields": len(rows), "fp64_solve_s_mean": float(t_fp.mean()),
+            "learned_solve_s_mean": float(t_ln.mean()),
+            "speedup_mean": float((t_fp / t_ln).mean()), "learned_faster_on": int((t_ln < t_fp).sum()),
+            "all_converged": all(r["fp64"]["converged"] and r["learned"]["converged"] for r in rows),
+            "fields": rows}
+
+
+def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoother_degree=0, scaled=True):
     """Train the coarse U-Net (every other Q1 vertex) on the p-multigrid's own coarse-smoothed residuals of random
     right-hand sides and attach it as the coarse preconditioner; returns the timings and training figures."""
     import torch
@@ -242,6 +349,7 @@ def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoo
     from hfk.learned import CoarseNetwork, coarse_error_dev, coarse_pairs_dev, set_coarse_network, train_energy_dev
 
     t0 = time.perf_counter()
+    set_coarse_network(pmg, None, "off", smoother_degree, scaled)  # the coarse smoother and scaling of the cycle
     gen = torch.Generator(device=dev).manual_seed(seed)
     held = 16
     bc = torch.zeros((samples + held, cs.nfree), dtype=torch.float64, device=dev)
@@ -258,12 +366,12 @@ def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoo
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     err = coarse_error_dev(pmg, net, bc[samples:].data_ptr(), held)
-    set_coarse_network(pmg, net, "precondition", smoother_degree)  # the hierarchy keeps a reference to the network
+    set_coarse_network(pmg, net, "precondition", smoother_degree, scaled)  # the hierarchy keeps the network
     return {"train_data_s": t1 - t0, "train_s": t2 - t1, "train_samples": samples, "train_steps": steps,
             "train_batch": 4, "train_loss_first_last": [float(hist[:5].mean()), float(hist[-20:].mean())],
             "heldout_energy_error": err, "network_params": net.nparams,
             "network_lattice": (net.N + 1) // 2, "coarse_mode": "precondition",
-            "coarse_smoother_degree": smoother_degree or pmg.smoother.degree}
+            "coarse_smoother_degree": smoother_degree or pmg.smoother.degree, "scaled": scaled}
 
 
 def time_jv(op, storage, xd, vd, yd, stream, steps, warmup):
@@ -341,7 +449,9 @@ def main():
     ap.add_argument("--variants", default="sym64,full32,sym32", help="extra storages timed beside the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--solve-legs", default="fp64,mixed,learned,q3", help="solve legs: fp64, mixed, learned, q3")
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--solve-legs", default="fp64,mixed,learned,fields,q3",
+                    help="solve legs: fp64, mixed, learned, fields, q3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the Jv would shard by cell partition across GPUs); "
@@ -458,10 +568,15 @@ def main():
             res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3)
         if "learned" in legs:
             res["solve_mixed_learned"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.1,
-                                                   learned={"samples": 160, "steps": 500, "smoother_degree": 3})
+                                                   learned={"samples": 160, "steps": 500, "smoother_degree": 3,
+                                                            "scaled": False})
         if "fp64" in legs and "learned" in legs:
             res["solve_learned_speedup_vs_fp64"] = (res["solve_fp64_pmg"]["solve_s"] /
                                                     res["solve_mixed_learned"]["solve_s"])
+        if "fields" in legs:
+            print("bench: learned coarse correction across lognormal modulus fields (32^3 Q2)", file=sys.stderr,
+                  flush=True)
+            res["learned_fields_32"] = learned_fields_leg()
         if "q3" in legs:
             # Q3 on 48^3 (64 points per cell): FP32 Jv against FP64; mixed-precision solves (FP32 smoothers, FP64
             # outer FCG and coarse level) with mu = 1 and with a lognormal mu (sigma 1, correlation length 0.1)
@@ -478,8 +593,9 @@ def main():
                                                               state_amp=3e-3)
             res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
                                                                state_amp=3e-3)
-    print("bench: SVK quick start 20x10x10 Q2, Newton with host LU", file=sys.stderr, flush=True)
-    res["newton_svk_20x10x10"] = newton_leg()
+    if not args.no_newton:
+        print("bench: SVK quick start 20x10x10 Q2, Newton with host LU", file=sys.stderr, flush=True)
+        res["newton_svk_20x10x10"] = newton_leg()
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 32^3 sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()

@@ -473,7 +473,9 @@ struct hfk_pmg {
   // coarse CSR (CSC of A, 0-based int32)
   DevBuf colptr, rowval, nzval;
   int64_t nnz = 0;
-  DevBuf cr, cz, cp, cq, ct, cu, cs;  // inner CG work vectors and device scalars
+  DevBuf cr, cz, cp, cq, ct, cu, cv, cs;  // inner CG work vectors and device scalars
+  DevBuf cdscale;                         // D^-1/2 of the coarse matrix (network scaling)
+  bool net_scaled = true;                 // the network acts on D^-1/2 A D^-1/2
   std::unique_ptr<hfk_cheb> ccheb;      // Chebyshev on the assembled coarse matrix (network cycle, modes 3 and 4)
   bool ccheb_stale = true;
   int ccheb_degree = 0;  // its degree (0: the matrix-free levels' degree)
@@ -504,6 +506,8 @@ int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
     if ((rc = op_build_blocks(c))) return rc;
   }
   if ((rc = op_run_jacobian(c, c->state.as<double>(), g->nzval.as<double>(), s))) return rc;
+  HIPCHK(g->ctx, csc_diag_rsqrt(c->nfree, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(),
+                                g->cdscale.as<double>(), s));
   NodeBlocks& nb = c->blocks;
   HIPCHK(g->ctx, csr_blocks(nb.nblocks, nb.dofs.as<int32_t>(), g->colptr.as<int32_t>(), g->rowval.as<int32_t>(),
                             g->nzval.as<double>(), nb.B.as<double>(), s));
@@ -541,6 +545,22 @@ int coarse_smoother(hfk_pmg* g, hipStream_t s, bool* usable) {
   return HFK_OK;
 }
 
+// the network's approximation z of A^-1 r on the coarse level: z = (s/h) N(r/s) (s: RMS of the input, h: the
+// network's cell size); with net_scaled the network acts on the diagonally scaled system D^-1/2 A D^-1/2
+// (D = diag(A), unit diagonal, so no 1/h factor): z = s D^-1/2 N(D^-1/2 r / s), s the RMS of D^-1/2 r
+int pmg_net_apply(hfk_pmg* g, hfk_net* net, const double* r, double* z, hipStream_t s) {
+  if (!g->net_scaled) return net_apply_dev(net, r, z, s);
+  const int64_t n = g->ops.back()->nfree;
+  const double* d = g->cdscale.as<double>();
+  double* v = g->cv.as<double>();
+  HIPCHK(g->ctx, vec_mul(n, d, r, v, s));
+  int rc = net_apply_dev(net, v, z, s);  // (s/h) N(v/s)
+  if (rc) return rc;
+  HIPCHK(g->ctx, vec_mul(n, d, z, z, s));
+  HIPCHK(g->ctx, vec_axpby(n, 0.0, z, net_h(net), z, s));  // times h
+  return HFK_OK;
+}
+
 // one two-grid cycle on the coarse level with the network as its coarse-grid correction:
 // x = S b; x += N(b - A x); x = x + S (b - A x)   (S: the coarse Chebyshev smoother)
 int net_cycle(hfk_pmg* g, const double* b, double* x, hipStream_t s) {
@@ -550,7 +570,7 @@ int net_cycle(hfk_pmg* g, const double* b, double* x, hipStream_t s) {
   double* u = g->cu.as<double>();
   int rc;
   if ((rc = cheb_smooth(S, x, b, true, t, s))) return rc;
-  if ((rc = net_apply_dev(g->net, t, u, s))) return rc;
+  if ((rc = pmg_net_apply(g, g->net, t, u, s))) return rc;
   HIPCHK(g->ctx, vec_axpby(n, 1.0, u, 1.0, x, s));
   return cheb_smooth(S, x, b, false, nullptr, s);
 }
@@ -588,7 +608,7 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
     return HFK_OK;
   };
   if (mode == 1 || mode == 2) {
-    if ((rc = net_apply_dev(g->net, b, x, s))) return rc;
+    if ((rc = pmg_net_apply(g, g->net, b, x, s))) return rc;
     if (mode == 2) return HFK_OK;
     HIPCHK(g->ctx, vec_copy(n, b, r, s));  // r = b - A x0
     HIPCHK(g->ctx, csc_t_spmv(n, cp, rv, av, x, r, -1.0, 1.0, s));
@@ -1006,7 +1026,7 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
   const int64_t nc = g->ops.back()->nfree;
   if ((rc = vec_alloc(ctx, g->cr, nc)) || (rc = vec_alloc(ctx, g->cz, nc)) || (rc = vec_alloc(ctx, g->cp, nc)) ||
       (rc = vec_alloc(ctx, g->cq, nc)) || (rc = vec_alloc(ctx, g->ct, nc)) || (rc = vec_alloc(ctx, g->cu, nc)) ||
-      (rc = g->dots.init(ctx)))
+      (rc = vec_alloc(ctx, g->cv, nc)) || (rc = vec_alloc(ctx, g->cdscale, nc)) || (rc = g->dots.init(ctx)))
     return rc;
   HIPCHK(ctx, g->cs.alloc(kNS * sizeof(double)));
   *out = g.release();
@@ -1075,8 +1095,10 @@ int hfk_pmg_info(hfk_pmg* g, double* info) {
   return HFK_OK;
 }
 
-int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree) {
+int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree, int scaled) {
   if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  if (scaled != 0 && scaled != 1) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: scaled must be 0 or 1");
+  g->net_scaled = scaled == 1;
   if (smoother_degree < 0) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: smoother degree must be >= 0");
   if (mode < 0 || mode > 4) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: coarse network mode must be 0 to 4");
   if (mode > 0 && (!net || net_ncoarse(net) != g->ops.back()->nfree))
@@ -1125,7 +1147,8 @@ int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, doub
 
 // Energy training of the coarse network on coarse right-hand sides r_k (device, [k][nc]): minimise
 // L(w) = w'(A/h) w - 2 w'x, x = r/s, whose minimiser is the coarse solve in the network's units (z = (s/h) w
-// = A^-1 r).  Needs one product with the assembled coarse matrix per sample and no exact solves.
+// = A^-1 r); in the scaled mode A/h -> D^-1/2 A D^-1/2 and r -> D^-1/2 r (pmg_net_apply).  Needs one product
+// with the assembled coarse matrix per sample and no exact solves.
 int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, int steps, int batch,
                                  double lr, uint64_t seed, double* history, void* stream) {
   if (!g || !net || !bc_dev || nsamples < 1 || steps < 0 || batch < 1 || !(lr > 0))
@@ -1138,7 +1161,7 @@ int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev,
   hipStream_t s = g->ops[0]->stream(stream);
   DevBuf w, q;
   if ((rc = vec_alloc(g->ctx, w, n)) || (rc = vec_alloc(g->ctx, q, n))) return rc;
-  const double invh = 1.0 / net_h(net);
+  const double invh = g->net_scaled ? 1.0 : 1.0 / net_h(net);  // the loss's operator: D^-1/2 A D^-1/2 or A/h
   uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
   for (int step = 0; step < steps; ++step) {
     double Lsum = 0;
@@ -1146,9 +1169,19 @@ int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev,
       st = st * 6364136223846793005ull + 1442695040888963407ull;
       const int k = (int)((st >> 33) % (uint64_t)nsamples);
       const double* r = bc_dev + (size_t)k * n;
+      if (g->net_scaled) {  // train on D^-1/2 A D^-1/2 and D^-1/2 r
+        HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), r, g->cv.as<double>(), s));
+        r = g->cv.as<double>();
+      }
       if ((rc = net_forward_vec(net, r, w.as<double>(), s))) return rc;
-      HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), w.as<double>(),
+      const double* wa = w.as<double>();
+      if (g->net_scaled) {
+        HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), w.as<double>(), g->cu.as<double>(), s));
+        wa = g->cu.as<double>();
+      }
+      HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), wa,
                                 q.as<double>(), invh, 0.0, s));
+      if (g->net_scaled) HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), q.as<double>(), q.as<double>(), s));
       double h3[3];
       if ((rc = g->dots.run(g->ctx, s, n, w.as<double>(), q.as<double>(), h3, w.as<double>(), r, r, r))) return rc;
       const double sc = std::sqrt(h3[2] / (double)n);
@@ -1181,7 +1214,7 @@ int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev,
   for (int k = 0; k < nsamples; ++k) {
     const double* r = bc_dev + (size_t)k * n;
     if ((rc = coarse_solve(g, r, e.as<double>(), s, false, 1e-10, 20000))) return rc;
-    if ((rc = net_apply_dev(net, r, z.as<double>(), s))) return rc;
+    if ((rc = pmg_net_apply(g, net, r, z.as<double>(), s))) return rc;
     HIPCHK(g->ctx, vec_axpby(n, -1.0, e.as<double>(), 1.0, z.as<double>(), s));  // z - e
     double h2[2];
     HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), z.as<double>(),

@@ -64,6 +64,31 @@ __global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, doubl
     y[i] = a * x[i] + b * y[i];
 }
 
+__global__ void k_mul(int64_t n, const double* __restrict__ a, const double* x, double* y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a[i] * x[i];
+}
+
+// d[c] = 1 / sqrt(A[c][c]) for a positive finite diagonal entry, 1 otherwise (CSC, sorted rows)
+__global__ void k_csc_diag_rsqrt(int64_t ncols, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                                 const double* __restrict__ val, double* __restrict__ d) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncols; c += (int64_t)gridDim.x * blockDim.x) {
+    int lo = colptr[c], hi = colptr[c + 1] - 1;
+    double a = 0.0;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int r = rowval[mid];
+      if (r == (int)c) {
+        a = val[mid];
+        break;
+      }
+      if (r < (int)c) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    d[c] = (a > 0.0 && isfinite(a)) ? 1.0 / sqrt(a) : 1.0;
+  }
+}
+
 __global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = x[i];
 }
@@ -284,6 +309,15 @@ size_t dot_partial_doubles() { return 3 * kRedBlocks; }
 
 hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(256), 0, s, n, a, x, b, y);
+  return hipGetLastError();
+}
+hipError_t vec_mul(int64_t n, const double* a, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_mul, grid_for(n), dim3(256), 0, s, n, a, x, y);
+  return hipGetLastError();
+}
+hipError_t csc_diag_rsqrt(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, double* d,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_csc_diag_rsqrt, grid_for(ncols), dim3(256), 0, s, ncols, colptr, rowval, val, d);
   return hipGetLastError();
 }
 hipError_t vec_copy(int64_t n, const double* x, double* y, hipStream_t s) {

@@ -17,6 +17,10 @@ hipError_t vec_dots(int64_t n, const double* a0, const double* b0, const double*
 size_t dot_partial_doubles();
 hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);  // y = a x + b y
 hipError_t vec_copy(int64_t n, const double* x, double* y, hipStream_t s);
+hipError_t vec_mul(int64_t n, const double* a, const double* x, double* y, hipStream_t s);  // y = a .* x (y may be x)
+// d = 1/sqrt(diag(A)) of a CSC matrix with sorted rows (1 where the diagonal is missing, non-positive or not finite)
+hipError_t csc_diag_rsqrt(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, double* d,
+                          hipStream_t s);
 hipError_t cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, hipStream_t s);
 hipError_t cg_direction(int64_t n, double* sc, const double* z, double* p, int flexible, hipStream_t s);
 hipError_t blocks_invert(int64_t nb, const int32_t* dofs, const double* B, double* Binv, int* bad, hipStream_t s);

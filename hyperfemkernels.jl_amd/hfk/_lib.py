@@ -79,7 +79,7 @@ SIGNATURES = {
     "hfk_net_apply_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_net_gradient": (C.c_int, [_vp, _f64p, _f64p, C.c_int, C.POINTER(C.c_float), _f64p]),
     "hfk_net_train_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p, _vp]),
-    "hfk_pmg_set_coarse_network": (C.c_int, [_vp, _vp, C.c_int, C.c_int]),
+    "hfk_pmg_set_coarse_network": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int]),
     "hfk_pmg_coarse_pairs_dev": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp]),
     "hfk_net_set_aux": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "hfk_pmg_train_coarse_network": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p,

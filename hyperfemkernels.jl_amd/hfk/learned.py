@@ -116,6 +116,24 @@ def train_energy_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, step
     return hist[:steps]
 
 
+def train_energy_problems(problems, net: "CoarseNetwork", steps: int, chunk: int = 25, batch: int = 4,
+                          lr: float = 1e-3, seed: int = 0, stream: int = 0) -> np.ndarray:
+    """Energy training of one network over several operators: ``problems`` is a list of (pmg, bc_ptr, nsamples,
+    aux) -- the p-multigrid whose coarse matrix defines the energy, its device samples, and the auxiliary vertex
+    field of that operator (or None).  The problems take turns, ``chunk`` Adam steps each (the optimiser state
+    carries over); returns the loss per step."""
+    hist = []
+    done, turn = 0, 0
+    while done < steps:
+        pmg, bc_ptr, ns, aux = problems[turn % len(problems)]
+        k = min(chunk, steps - done)
+        net.set_aux(aux)
+        hist.append(train_energy_dev(pmg, net, bc_ptr, ns, k, batch=batch, lr=lr, seed=seed + turn, stream=stream))
+        done += k
+        turn += 1
+    return np.concatenate(hist) if hist else np.zeros(0)
+
+
 def coarse_error_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, stream: int = 0) -> float:
     """Mean relative energy-norm error of the network's coarse correction (against the coarse solve)."""
     err = C.c_double()
@@ -124,11 +142,13 @@ def coarse_error_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, stre
     return err.value
 
 
-def set_coarse_network(pmg, net: CoarseNetwork | None, mode: str = "warmstart", smoother_degree: int = 0):
+def set_coarse_network(pmg, net: CoarseNetwork | None, mode: str = "warmstart", smoother_degree: int = 0,
+                       scaled: bool = True):
     """Attach ``net`` to the coarse level of ``pmg`` (modes in MODES); ``smoother_degree`` is the degree of the
-    coarse Chebyshev smoother of the network cycle (0: the p-multigrid's own degree)."""
+    coarse Chebyshev smoother of the network cycle (0: the p-multigrid's own degree); with ``scaled`` the network
+    acts on the diagonally scaled coarse system D^-1/2 A D^-1/2 (training and error included)."""
     if mode not in MODES:
         raise ValueError(f"HyperFEMKernels: unknown coarse network mode {mode!r}")
     check(pmg.lib.hfk_pmg_set_coarse_network(pmg.handle, None if net is None else net.handle, MODES[mode],
-                                             int(smoother_degree)), pmg.levels[0].ctx.handle)
+                                             int(smoother_degree), int(bool(scaled))), pmg.levels[0].ctx.handle)
     pmg.coarse_network = (net, mode)

@@ -193,8 +193,10 @@ int hfk_net_train_dev(hfk_net* net, const double* r_dev, const double* e_dev, in
  * coarse solve is a flexible CG (same stopping rule) preconditioned by one network cycle, 4 one network cycle is
  * the coarse correction.  The network cycle is x = S b; x += N(b - A x); x += S(b - A x) with S the Chebyshev
  * smoother of the assembled coarse matrix (degree smoother_degree, 0 for the p-multigrid's; the p-multigrid's range
- * and inflation); modes 3 and 4 fall back to mode 0 when that smoother has no usable interval. */
-int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree);
+ * and inflation); modes 3 and 4 fall back to mode 0 when that smoother has no usable interval.  scaled = 1: the
+ * network acts on the diagonally scaled coarse system, z = D^-1/2 N(D^-1/2 r) with D = diag(A) (also in training
+ * and in hfk_pmg_coarse_network_error); 0: z = N(r).  The setting stays with the p-multigrid. */
+int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree, int scaled);
 /* Training data from the plain p-multigrid: for each fine right-hand side b_k ([k][nfine], device) the
  * pre-smoothed residual restricted to the coarse level (bc_k) -- with coarse_smoothed != 0, further the residual of
  * the coarse Chebyshev pre-smoothing of it (the input of the network cycle) -- and, when ec_dev is not NULL, the

@@ -76,7 +76,8 @@ def make_problem(hip_ctx, n, seed=0, mu_sigma=0.5):
     return fs, cs, fop, cop, mf
 
 
-def test_training_and_learned_cycle(hip_ctx):
+@pytest.mark.parametrize("scaled", [False, True])
+def test_training_and_learned_cycle(hip_ctx, scaled):
     """Energy training on restricted, coarse-smoothed p-MG residuals; then the network inside the cycle: every mode
     reaches the plain cycle's answer (the outer tolerance decides), and as the preconditioner of the coarse flexible
     CG it cuts the coarse iterations without adding outer ones -- the gain of the trained network."""
@@ -96,6 +97,7 @@ def test_training_and_learned_cycle(hip_ctx):
     torch.cuda.synchronize()
     coarse_pairs_dev(pmg, bd.data_ptr(), ns, bc.data_ptr(), None, coarse_smoothed=True)
     del bd
+    set_coarse_network(pmg, None, "off", 0, scaled)
     net = CoarseNetwork(hip_ctx, cs, seed=7, coarsen=True)
     err0 = coarse_error_dev(pmg, net, bc[ns - nh:].data_ptr(), nh)
     hist = train_energy_dev(pmg, net, bc.data_ptr(), ns - nh, steps=500, batch=4, lr=1e-3)
@@ -109,7 +111,7 @@ def test_training_and_learned_cycle(hip_ctx):
     b = rng.standard_normal(fs.nfree)
     sols = {}
     for mode in ("off", "warmstart", "cycle", "precondition"):
-        set_coarse_network(pmg, None if mode == "off" else net, mode)
+        set_coarse_network(pmg, None if mode == "off" else net, mode, 0, scaled)
         x = np.zeros(fs.nfree)
         info0 = pmg.info()["coarse_iters"]
         res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=400, preconditioner=pmg).solve(x, mf, b)
@@ -121,5 +123,5 @@ def test_training_and_learned_cycle(hip_ctx):
         assert rel(sols[mode][0], sols["off"][0]) < 1e-7
     # the trained network as the coarse preconditioner: the outer iterations stay, the coarse work collapses
     assert sols["precondition"][1] <= sols["off"][1] + 1
-    assert sols["precondition"][2] < 0.2 * sols["off"][2]
+    assert sols["precondition"][2] < 0.25 * sols["off"][2]
     set_coarse_network(pmg, None, "off")
