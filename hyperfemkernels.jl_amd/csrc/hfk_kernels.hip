@@ -356,9 +356,13 @@ __device__ __forceinline__ void point_geometry(const Tab& tb, const double* __re
 // Index of the packed upper triangle (i <= j) of a symmetric 9x9 matrix.
 __host__ __device__ constexpr int sym_index(int i, int j) { return j * (j + 1) / 2 + i; }
 
-template <class TS>
+// the tangents are written once per prepare! and read by the next products, never from cache.  Non-temporal
+// stores measured faster for the packed symmetric storages (-12 % sym64, -20 % sym32 at 64^3 Q2) and slower for
+// the full ones (+15 % full64), so they are used for the former only
+template <class TS, bool NT>
 __device__ __forceinline__ void store_entry(void* base, size_t idx, double v) {
-  static_cast<TS*>(base)[idx] = static_cast<TS>(v);
+  if (NT) __builtin_nontemporal_store(static_cast<TS>(v), static_cast<TS*>(base) + idx);
+  else static_cast<TS*>(base)[idx] = static_cast<TS>(v);
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -431,11 +435,11 @@ __global__ __launch_bounds__(kThreads) void k_prepare(OpDev op, void* __restrict
             for (int L = 0; L < 3; ++L) s += W[K + 3 * L] * iJt[L + 3 * M];
             const int i = K + 3 * c, j = M + 3 * k;
             if (!SYM) {
-              store_entry<TS>(tangents, base + (size_t)(i + 9 * j) * NSLOT, dV * s);
+              store_entry<TS, false>(tangents, base + (size_t)(i + 9 * j) * NSLOT, dV * s);
             } else if (c < k || K <= M) {
               // c < k: the whole block lies above the diagonal; c == k: its upper triangle.  The tangents of
               // the built-in laws are symmetric, so the upper triangle determines D.
-              store_entry<TS>(tangents, base + (size_t)sym_index(i, j) * NSLOT, dV * s);
+              store_entry<TS, true>(tangents, base + (size_t)sym_index(i, j) * NSLOT, dV * s);
             }
           }
       }
@@ -457,6 +461,13 @@ __device__ __forceinline__ int interleaved_batch(const int* __restrict__ colfc, 
 // contract, add to the free rows of y.  Deterministic mode (ATOMIC = false): the batches of one colour per
 // launch, plain read-modify-writes, the reference's colour order of the sums.  ATOMIC: every batch in one
 // launch, FP64 atomic adds (the order of the additions into one y entry varies from run to run).
+// the stored tangents are read once per product and never reused within it: non-temporal loads keep them from
+// evicting the gathered dx and the scattered y from the caches
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 template <int Q, int P, class TS, bool SYM, bool ATOMIC>
 __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __restrict__ tangents,
                                                     const double* __restrict__ dx, double* __restrict__ y,
@@ -492,13 +503,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
-        for (int i = 0; i < 9; ++i) sv[i] += static_cast<double>(Dp[(size_t)(i + 9 * j) * NSLOT]) * su[j];
+        for (int i = 0; i < 9; ++i) sv[i] += static_cast<double>(ld_stream(Dp + (size_t)(i + 9 * j) * NSLOT)) * su[j];
     } else {
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
         for (int i = 0; i <= j; ++i) {
-          const double d = static_cast<double>(Dp[(size_t)sym_index(i, j) * NSLOT]);
+          const double d = static_cast<double>(ld_stream(Dp + (size_t)sym_index(i, j) * NSLOT));
           sv[i] += d * su[j];
           if (i != j) sv[j] += d * su[i];
         }
@@ -545,7 +556,7 @@ __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __res
     for (int it = tid; it < C * 81; it += kThreads) {
       const int c2 = it / 81, e = it - c2 * 81, ii = e % 9, jj = e / 9;
       const int se = SYM ? (ii <= jj ? sym_index(ii, jj) : sym_index(jj, ii)) : e;
-      Dl[it] = static_cast<double>(Dg[(size_t)se * NSLOT + c2 * NP + q]);
+      Dl[it] = static_cast<double>(ld_stream(Dg + (size_t)se * NSLOT + c2 * NP + q));
     }
     __syncthreads();
     if (active) {

@@ -175,6 +175,13 @@ struct hfk_cheb {
   const int32_t* csr_colptr = nullptr;
   const int32_t* csr_rowval = nullptr;
   const double* csr_val = nullptr;
+  // the same matrix in node-block ELL form when available (values FP64, or FP32 with bell_fp32)
+  int64_t bell_nb = 0;
+  int bell_W = 0;
+  const int32_t* bell_nbr = nullptr;
+  const int32_t* bell_dofs = nullptr;
+  const void* bell_val = nullptr;
+  bool bell_fp32 = false;
   int nbuild = 0;
   DevBuf r, d, tmp, start, V;
   Dots dots;
@@ -187,6 +194,11 @@ namespace {
 // y = alpha A x (+ y when accumulate) with the smoother's operator
 int cheb_op_apply(hfk_cheb* ch, const double* x, double* y, double alpha, bool accumulate, hipStream_t s) {
   hfk_op* op = ch->op;
+  if (ch->bell_val) {
+    HIPCHK(op->ctx, bell_spmv(ch->bell_nb, ch->bell_W, ch->bell_nbr, ch->bell_val, ch->bell_fp32, ch->bell_dofs, x, y, alpha,
+                              accumulate ? 1.0 : 0.0, s));
+    return HFK_OK;
+  }
   if (ch->csr_val) {
     HIPCHK(op->ctx, csc_t_spmv(op->nfree, ch->csr_colptr, ch->csr_rowval, ch->csr_val, x, y, alpha, accumulate ? 1.0 : 0.0, s));
     return HFK_OK;
@@ -475,6 +487,13 @@ struct hfk_pmg {
   int64_t nnz = 0;
   DevBuf cr, cz, cp, cq, ct, cu, cv, cs;  // inner CG work vectors and device scalars
   DevBuf cdscale;                         // D^-1/2 of the coarse matrix (network scaling)
+  // the coarse matrix in node-block ELL form (3x3 blocks of the nodal dofs), filled from the CSC at every setup
+  struct BlockEll {
+    bool built = false, has32 = false;
+    int64_t nb = 0;
+    int W = 0;
+    DevBuf nbr, pos, v64, v32;
+  } bell;
   bool net_scaled = true;                 // the network acts on D^-1/2 A D^-1/2
   std::unique_ptr<hfk_cheb> ccheb;      // Chebyshev on the assembled coarse matrix (network cycle, modes 3 and 4)
   bool ccheb_stale = true;
@@ -487,6 +506,86 @@ struct hfk_pmg {
 };
 
 namespace {
+
+// The node-block ELL structure of the coarse matrix from its CSC pattern and nodal blocks (host, once): the
+// neighbour blocks of every block row (sorted, padded to the widest row with the row's own block) and, for every
+// CSC entry, its position in the ELL values.  Left unbuilt (the scalar CSC product is used) when the positions
+// would not fit in 32 bits.
+int bell_build(hfk_pmg* g) {
+  hfk_op* c = g->ops.back();
+  const NodeBlocks& nbk = c->blocks;
+  const int64_t n = c->nfree, nb = nbk.nblocks;
+  if (nb <= 0) return HFK_OK;
+  std::vector<int32_t> blk((size_t)n, -1), comp((size_t)n, 0);
+  for (int64_t b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      const int32_t d = nbk.dofs_h[3 * b + k];
+      if (d > 0) {
+        blk[d - 1] = (int32_t)b;
+        comp[d - 1] = k;
+      }
+    }
+  for (int64_t d = 0; d < n; ++d)
+    if (blk[d] < 0) return HFK_OK;  // a free dof outside the nodal blocks: keep the scalar product
+  // neighbour lists: block j (of column c) is a neighbour of block i (of row r)
+  constexpr int kMaxW = 64;
+  std::vector<int32_t> lst((size_t)nb * kMaxW), cnt((size_t)nb, 0), last((size_t)nb, -1);
+  for (int64_t col = 0; col < n; ++col) {
+    const int32_t j = blk[col];
+    for (int64_t p = c->colptr[col] - 1; p < c->colptr[col + 1] - 1; ++p) {
+      const int32_t i = blk[c->rowval[p] - 1];
+      if (last[i] == j) continue;
+      bool found = false;
+      for (int k = 0; k < cnt[i]; ++k) found = found || lst[(size_t)i * kMaxW + k] == j;
+      if (found) continue;
+      if (cnt[i] >= kMaxW) return HFK_OK;  // unusually wide block row: keep the scalar product
+      lst[(size_t)i * kMaxW + cnt[i]++] = j;
+      last[i] = j;
+    }
+  }
+  int W = 1;
+  for (int64_t i = 0; i < nb; ++i) {
+    std::sort(lst.begin() + (size_t)i * kMaxW, lst.begin() + (size_t)i * kMaxW + cnt[i]);
+    W = std::max(W, cnt[i]);
+  }
+  if ((int64_t)9 * W * nb >= ((int64_t)1 << 31)) return HFK_OK;
+  std::vector<int32_t> nbr((size_t)W * nb);
+  for (int64_t i = 0; i < nb; ++i)
+    for (int k = 0; k < W; ++k) nbr[(size_t)k * nb + i] = k < cnt[i] ? lst[(size_t)i * kMaxW + k] : (int32_t)i;
+  std::vector<int32_t> pos((size_t)g->nnz);
+  for (int64_t col = 0; col < n; ++col) {
+    const int32_t j = blk[col];
+    for (int64_t p = c->colptr[col] - 1; p < c->colptr[col + 1] - 1; ++p) {
+      const int64_t r = c->rowval[p] - 1;
+      const int32_t i = blk[r];
+      const int32_t* L = lst.data() + (size_t)i * kMaxW;
+      const int k = (int)(std::lower_bound(L, L + cnt[i], j) - L);
+      pos[p] = (int32_t)(((int64_t)9 * k + comp[r] + 3 * comp[col]) * nb + i);
+    }
+  }
+  auto& B = g->bell;
+  HIPCHK(g->ctx, B.nbr.alloc(nbr.size() * sizeof(int32_t)));
+  HIPCHK(g->ctx, hipMemcpy(B.nbr.p, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(g->ctx, B.pos.alloc(std::max<size_t>(pos.size(), 1) * sizeof(int32_t)));
+  if (!pos.empty()) HIPCHK(g->ctx, hipMemcpy(B.pos.p, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  const size_t nv = (size_t)9 * W * nb;
+  HIPCHK(g->ctx, B.v64.alloc(nv * sizeof(double)));
+  B.has32 = g->smoother_storage == HFK_STORE_FULL32 || g->smoother_storage == HFK_STORE_SYM32;
+  if (B.has32) HIPCHK(g->ctx, B.v32.alloc(nv * sizeof(float)));
+  B.nb = nb;
+  B.W = W;
+  B.built = true;
+  return HFK_OK;
+}
+
+// y = alpha A x + beta y with the assembled coarse matrix
+hipError_t coarse_spmv(hfk_pmg* g, const double* x, double* y, double alpha, double beta, hipStream_t s) {
+  if (g->bell.built)
+    return bell_spmv(g->bell.nb, g->bell.W, g->bell.nbr.as<int32_t>(), g->bell.v64.p, false,
+                     g->ops.back()->blocks.dofs.as<int32_t>(), x, y, alpha, beta, s);
+  return csc_t_spmv(g->ops.back()->nfree, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), x, y,
+                    alpha, beta, s);
+}
 
 int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
   hfk_op* c = g->ops.back();
@@ -504,10 +603,18 @@ int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
     HIPCHK(g->ctx, hipMemcpy(g->rowval.p, rv.data(), rv.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(g->ctx, g->nzval.alloc(std::max<size_t>(rv.size(), 1) * sizeof(double)));
     if ((rc = op_build_blocks(c))) return rc;
+    if ((rc = bell_build(g))) return rc;
   }
   if ((rc = op_run_jacobian(c, c->state.as<double>(), g->nzval.as<double>(), s))) return rc;
   HIPCHK(g->ctx, csc_diag_rsqrt(c->nfree, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(),
                                 g->cdscale.as<double>(), s));
+  if (g->bell.built) {
+    const size_t nv = (size_t)9 * g->bell.W * g->bell.nb;
+    HIPCHK(g->ctx, hipMemsetAsync(g->bell.v64.p, 0, nv * sizeof(double), s));
+    if (g->bell.has32) HIPCHK(g->ctx, hipMemsetAsync(g->bell.v32.p, 0, nv * sizeof(float), s));
+    HIPCHK(g->ctx, bell_fill(g->nnz, g->bell.pos.as<int32_t>(), g->nzval.as<double>(), g->bell.v64.as<double>(),
+                             g->bell.has32 ? g->bell.v32.as<float>() : nullptr, s));
+  }
   NodeBlocks& nb = c->blocks;
   HIPCHK(g->ctx, csr_blocks(nb.nblocks, nb.dofs.as<int32_t>(), g->colptr.as<int32_t>(), g->rowval.as<int32_t>(),
                             g->nzval.as<double>(), nb.B.as<double>(), s));
@@ -534,6 +641,14 @@ int coarse_smoother(hfk_pmg* g, hipStream_t s, bool* usable) {
     ch->csr_colptr = g->colptr.as<int32_t>();
     ch->csr_rowval = g->rowval.as<int32_t>();
     ch->csr_val = g->nzval.as<double>();
+    if (g->bell.built) {  // FP32 values when the p-multigrid smooths in FP32
+      ch->bell_nb = g->bell.nb;
+      ch->bell_W = g->bell.W;
+      ch->bell_nbr = g->bell.nbr.as<int32_t>();
+      ch->bell_dofs = c->blocks.dofs.as<int32_t>();
+      ch->bell_fp32 = g->bell.has32;
+      ch->bell_val = g->bell.has32 ? g->bell.v32.p : g->bell.v64.p;
+    }
     g->ccheb = std::move(ch);
   }
   if (g->ccheb_stale) {
@@ -589,9 +704,6 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
   double* p = g->cp.as<double>();
   double* q = g->cq.as<double>();
   double* sc = g->cs.as<double>();
-  const int32_t* cp = g->colptr.as<int32_t>();
-  const int32_t* rv = g->rowval.as<int32_t>();
-  const double* av = g->nzval.as<double>();
   int mode = allow_net && g->net ? g->net_mode : 0;
   int rc;
   g->last_inner = 0;
@@ -611,7 +723,7 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
     if ((rc = pmg_net_apply(g, g->net, b, x, s))) return rc;
     if (mode == 2) return HFK_OK;
     HIPCHK(g->ctx, vec_copy(n, b, r, s));  // r = b - A x0
-    HIPCHK(g->ctx, csc_t_spmv(n, cp, rv, av, x, r, -1.0, 1.0, s));
+    HIPCHK(g->ctx, coarse_spmv(g, x, r, -1.0, 1.0, s));
   } else {
     HIPCHK(g->ctx, launch_fill(x, n, 0.0, s));
     HIPCHK(g->ctx, vec_copy(n, b, r, s));
@@ -624,7 +736,7 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
   HIPCHK(g->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZ, s));
   if (bn == 0 || !(std::sqrt(h[2]) > rtol * bn)) return HFK_OK;
   for (int it = 1; it <= maxiter; ++it) {
-    HIPCHK(g->ctx, csc_t_spmv(n, cp, rv, av, p, q, 1.0, 0.0, s));
+    HIPCHK(g->ctx, coarse_spmv(g, p, q, 1.0, 0.0, s));
     HIPCHK(g->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kPQ, s));
     HIPCHK(g->ctx, cg_update(n, sc, p, q, x, r, s));
     if ((rc = precondition(r, z))) return rc;
@@ -1095,6 +1207,26 @@ int hfk_pmg_info(hfk_pmg* g, double* info) {
   return HFK_OK;
 }
 
+int hfk_pmg_coarse_matvec(hfk_pmg* g, const double* x, double* y, int scalar_csc) {
+  if (!g || !x || !y) return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  if (!g->built || !g->colptr.p) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid not updated");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = g->ops.back()->nfree;
+  DevBuf xd, yd;
+  if ((rc = vec_alloc(g->ctx, xd, n)) || (rc = vec_alloc(g->ctx, yd, n))) return rc;
+  HIPCHK(g->ctx, hipMemcpyAsync(xd.p, x, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (scalar_csc)
+    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), xd.as<double>(),
+                              yd.as<double>(), 1.0, 0.0, s));
+  else
+    HIPCHK(g->ctx, coarse_spmv(g, xd.as<double>(), yd.as<double>(), 1.0, 0.0, s));
+  HIPCHK(g->ctx, hipMemcpyAsync(y, yd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
 int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree, int scaled) {
   if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
   if (scaled != 0 && scaled != 1) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: scaled must be 0 or 1");
@@ -1179,8 +1311,7 @@ int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev,
         HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), w.as<double>(), g->cu.as<double>(), s));
         wa = g->cu.as<double>();
       }
-      HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), wa,
-                                q.as<double>(), invh, 0.0, s));
+      HIPCHK(g->ctx, coarse_spmv(g, wa, q.as<double>(), invh, 0.0, s));
       if (g->net_scaled) HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), q.as<double>(), q.as<double>(), s));
       double h3[3];
       if ((rc = g->dots.run(g->ctx, s, n, w.as<double>(), q.as<double>(), h3, w.as<double>(), r, r, r))) return rc;
@@ -1217,8 +1348,7 @@ int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev,
     if ((rc = pmg_net_apply(g, net, r, z.as<double>(), s))) return rc;
     HIPCHK(g->ctx, vec_axpby(n, -1.0, e.as<double>(), 1.0, z.as<double>(), s));  // z - e
     double h2[2];
-    HIPCHK(g->ctx, csc_t_spmv(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(), z.as<double>(),
-                              q.as<double>(), 1.0, 0.0, s));
+    HIPCHK(g->ctx, coarse_spmv(g, z.as<double>(), q.as<double>(), 1.0, 0.0, s));
     if ((rc = g->dots.run(g->ctx, s, n, z.as<double>(), q.as<double>(), h2, e.as<double>(), r))) return rc;
     total += std::sqrt(std::max(h2[0], 0.0) / std::max(h2[1], 1e-300));
   }

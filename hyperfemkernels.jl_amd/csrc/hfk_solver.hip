@@ -216,6 +216,43 @@ __global__ __launch_bounds__(256) void k_csc_t_spmv(int64_t ncols, const int32_t
   }
 }
 
+// y = alpha A x + beta y with A in node-block ELL form: block row i (the up to 3 free dofs of node block i) has W
+// slots of 3x3 blocks; slot k couples it with block nbr[k nb + i] (padding slots point at i and hold zeros) and
+// holds A[dof(i,ci), dof(j,cj)] at val[(9k + ci + 3cj) nb + i]: one thread per block row, coalesced values, one
+// index per 9 values (the scalar CSC above reads one index per value)
+template <class TV>
+__global__ __launch_bounds__(256) void k_bell_spmv(int64_t nb, int W, const int32_t* __restrict__ nbr,
+                                                   const TV* __restrict__ val, const int32_t* __restrict__ dofs,
+                                                   const double* __restrict__ x, double* __restrict__ y, double alpha,
+                                                   double beta) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = 0; k < W; ++k) {
+      const int64_t j = nbr[(size_t)k * nb + i];
+      const int d0 = dofs[3 * j], d1 = dofs[3 * j + 1], d2 = dofs[3 * j + 2];
+      const double x0 = d0 > 0 ? x[d0 - 1] : 0.0, x1 = d1 > 0 ? x[d1 - 1] : 0.0, x2 = d2 > 0 ? x[d2 - 1] : 0.0;
+      const TV* v = val + (size_t)9 * k * nb + i;
+      s0 += (double)v[0] * x0 + (double)v[3 * nb] * x1 + (double)v[6 * nb] * x2;
+      s1 += (double)v[nb] * x0 + (double)v[4 * nb] * x1 + (double)v[7 * nb] * x2;
+      s2 += (double)v[2 * nb] * x0 + (double)v[5 * nb] * x1 + (double)v[8 * nb] * x2;
+    }
+    const int o[3] = {dofs[3 * i], dofs[3 * i + 1], dofs[3 * i + 2]};
+    const double sv[3] = {s0, s1, s2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (o[c] > 0) y[o[c] - 1] = alpha * sv[c] + (beta == 0.0 ? 0.0 : beta * y[o[c] - 1]);
+  }
+}
+
+// the block-ELL values from the CSC values: v64[pos[p]] = val[p] (and the FP32 copy when v32 is given)
+__global__ void k_bell_fill(int64_t nnz, const int32_t* __restrict__ pos, const double* __restrict__ val,
+                            double* __restrict__ v64, float* __restrict__ v32) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+    v64[pos[p]] = val[p];
+    if (v32) v32[pos[p]] = (float)val[p];
+  }
+}
+
 // --- transfers (src/transfers.jl:188-221, 295-313) ---------------------------------------------------------
 // fine ids: [slot][c][tf] (the fine operator's batch layout); coarse ids of the same cells: [slot][c][tc]
 __global__ void k_prolongate(int64_t nslot, int NF, int NC, const int32_t* __restrict__ fid,
@@ -350,6 +387,20 @@ hipError_t csr_blocks(int64_t nb, const int32_t* dofs, const int32_t* colptr, co
 hipError_t csc_t_spmv(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, const double* x,
                       double* y, double alpha, double beta, hipStream_t s) {
   hipLaunchKernelGGL(k_csc_t_spmv, grid_for(ncols * 16), dim3(256), 0, s, ncols, colptr, rowval, val, x, y, alpha, beta);
+  return hipGetLastError();
+}
+hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,
+                     const double* x, double* y, double alpha, double beta, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_bell_spmv<float>, grid_for(nb), dim3(256), 0, s, nb, W, nbr, static_cast<const float*>(val), dofs,
+                       x, y, alpha, beta);
+  else
+    hipLaunchKernelGGL(k_bell_spmv<double>, grid_for(nb), dim3(256), 0, s, nb, W, nbr, static_cast<const double*>(val),
+                       dofs, x, y, alpha, beta);
+  return hipGetLastError();
+}
+hipError_t bell_fill(int64_t nnz, const int32_t* pos, const double* val, double* v64, float* v32, hipStream_t s) {
+  hipLaunchKernelGGL(k_bell_fill, grid_for(nnz), dim3(256), 0, s, nnz, pos, val, v64, v32);
   return hipGetLastError();
 }
 hipError_t transfer_prolongate(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* E,

@@ -17,6 +17,10 @@ hipError_t vec_dots(int64_t n, const double* a0, const double* b0, const double*
 size_t dot_partial_doubles();
 hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);  // y = a x + b y
 hipError_t vec_copy(int64_t n, const double* x, double* y, hipStream_t s);
+// node-block ELL product y = alpha A x + beta y (val FP64, or FP32 when fp32) and its value fill from the CSC
+hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,
+                     const double* x, double* y, double alpha, double beta, hipStream_t s);
+hipError_t bell_fill(int64_t nnz, const int32_t* pos, const double* val, double* v64, float* v32, hipStream_t s);
 hipError_t vec_mul(int64_t n, const double* a, const double* x, double* y, hipStream_t s);  // y = a .* x (y may be x)
 // d = 1/sqrt(diag(A)) of a CSC matrix with sorted rows (1 where the diagonal is missing, non-positive or not finite)
 hipError_t csc_diag_rsqrt(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, double* d,

@@ -70,6 +70,7 @@ SIGNATURES = {
     "hfk_pmg_vcycle": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_pmg_vcycle_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_pmg_info": (C.c_int, [_vp, _f64p]),
+    "hfk_pmg_coarse_matvec": (C.c_int, [_vp, _f64p, _f64p, C.c_int]),
     "hfk_net_create": (C.c_int, [_vp, C.c_int, C.c_int, _i32p, C.c_int64, C.c_double, C.c_uint64, C.POINTER(_vp)]),
     "hfk_net_destroy": (C.c_int, [_vp]),
     "hfk_net_num_params": (C.c_int64, [_vp]),

@@ -200,6 +200,16 @@ class PMultigrid:
         return dict(ok=bool(a[1]), cycles=int(a[2]), last_coarse_iters=int(a[3]), coarse_iters=int(a[4]),
                     coarse_ok=bool(a[5]), levels=lv)
 
+    def coarse_matvec(self, x, scalar_csc: bool = False):
+        """A_c x with the assembled coarse matrix of the last update (the solver's block product, or the scalar CSC
+        product with ``scalar_csc``)."""
+        n = self.levels[-1].nfree
+        x = _vec(x, n)
+        y = np.zeros(n)
+        check(self.lib.hfk_pmg_coarse_matvec(self.handle, ptr(x, C.c_double), ptr(y, C.c_double), int(bool(scalar_csc))),
+              self.levels[0].ctx.handle)
+        return y
+
     def __del__(self):
         try:
             self.lib.hfk_pmg_destroy(self.handle)

@@ -152,4 +152,52 @@ function solve!(x::Vector{Float64}, mf::DeviceMatrixFree, b::Vector{Float64}; pm
     (st[1] == 1, Int(st[2]), st[3] == 1, rn[1])
 end
 
+# --- learned coarse correction (no counterpart in the reference) ---------------------------------------------------
+# Device buffers (training samples) are passed as raw device pointers, e.g. `pointer(::ROCArray{Float64})` from
+# AMDGPU.jl; `stream = C_NULL` runs on the context's stream.
+
+"U-Net on the vertex lattice of the Q1 coarse space (n_lattice^3 vertices, lattice_dofs: 3 x n_lattice^3 signed ids)"
+mutable struct CoarseNetwork
+    ptr::Ptr{Cvoid}
+    ctx::Context
+end
+function CoarseNetwork(ctx::Context, n_lattice::Integer, lattice_dofs::Matrix{Int32}, ncoarse::Integer, h::Real;
+                       seed::Integer=0, coarsen::Bool=false)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hfk_net_create, libhfk), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Ptr{Int32}, Int64, Cdouble, UInt64, Ptr{Ptr{Cvoid}}),
+                ctx.ptr, n_lattice, coarsen ? 1 : 0, lattice_dofs, ncoarse, h, seed, r), ctx.ptr)
+    net = CoarseNetwork(r[], ctx)
+    finalizer(n -> ccall((:hfk_net_destroy, libhfk), Cint, (Ptr{Cvoid},), n.ptr), net)
+    net
+end
+set_aux!(net::CoarseNetwork, aux::Union{Nothing,Vector{Float32}}) =
+    check(ccall((:hfk_net_set_aux, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float32}), net.ptr,
+                aux === nothing ? Ptr{Float32}(C_NULL) : pointer(aux)), net.ctx.ptr)
+
+"restricted (and coarse-smoothed) p-MG residuals of the device right-hand sides b_dev ([k][nfine]) into bc_dev"
+coarse_pairs!(g::DevicePMultigrid, b_dev::Ptr{Float64}, nsamples::Integer, bc_dev::Ptr{Float64};
+              coarse_smoothed::Bool=true, stream::Ptr{Cvoid}=C_NULL) =
+    check(ccall((:hfk_pmg_coarse_pairs_dev, libhfk), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Cint, Cdouble, Cint, Ptr{Cvoid}),
+                g.ptr, b_dev, nsamples, bc_dev, C_NULL, coarse_smoothed ? 1 : 0, 1e-10, 5000, stream), g.levels[1].ctx.ptr)
+
+"energy training of net on the coarse level of g; returns the loss per step"
+function train!(g::DevicePMultigrid, net::CoarseNetwork, bc_dev::Ptr{Float64}, nsamples::Integer, steps::Integer;
+                batch::Integer=4, lr=1e-3, seed::Integer=0, stream::Ptr{Cvoid}=C_NULL)
+    hist = zeros(max(steps, 1))
+    check(ccall((:hfk_pmg_train_coarse_network, libhfk), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Cint, Cdouble, UInt64, Ptr{Float64}, Ptr{Cvoid}),
+                g.ptr, net.ptr, bc_dev, nsamples, steps, batch, lr, seed, hist, stream), g.levels[1].ctx.ptr)
+    hist[1:steps]
+end
+
+const COARSE_MODES = Dict(:off => 0, :warmstart => 1, :replace => 2, :precondition => 3, :cycle => 4)
+"attach net to the coarse level of g (mode in COARSE_MODES); ldiv!/solve! are unchanged"
+set_coarse_network!(g::DevicePMultigrid, net::Union{Nothing,CoarseNetwork}, mode::Symbol=:precondition;
+                    smoother_degree::Integer=0, scaled::Bool=false) =
+    check(ccall((:hfk_pmg_set_coarse_network, libhfk), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint),
+                g.ptr, net === nothing ? C_NULL : net.ptr, COARSE_MODES[mode], smoother_degree, scaled ? 1 : 0),
+          g.levels[1].ctx.ptr)
+
 end # module

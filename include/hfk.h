@@ -163,6 +163,9 @@ int hfk_pmg_vcycle_dev(hfk_pmg* g, const double* r_dev, double* z_dev, void* str
 /* info[0..5] = nlevels, ok, cycles, inner coarse iterations of the last cycle, their total, coarse level ok;
  * then per smoothed level (lambda_min, lambda_max, a, b, smoother ok, blocks ok): 6 + 6 (nlevels - 1) values */
 int hfk_pmg_info(hfk_pmg* g, double* info);
+/* y = A_c x with the assembled coarse matrix of the last update (host arrays of the coarse level's free dofs):
+ * through the node-block ELL product the solver uses (scalar_csc = 0) or the scalar CSC product (1). */
+int hfk_pmg_coarse_matvec(hfk_pmg* g, const double* x, double* y, int scalar_csc);
 
 /* ---- learned coarse correction (no counterpart in the reference) ------------------------------------- */
 /* A 4-level 3D convolutional U-Net (channels 16-32-64-64, bias-free 3x3x3 convolutions with ReLU, fixed

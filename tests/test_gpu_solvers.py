@@ -115,6 +115,9 @@ def test_pmultigrid_solution_equals_direct(hip_ctx, oracle):
     fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
     pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3).update()
     assert pmg.ok
+    # the coarse solver's node-block ELL product = the scalar CSC product of the same matrix
+    xc = np.random.default_rng(10).standard_normal(cop.nfree)
+    assert rel(pmg.coarse_matvec(xc), pmg.coarse_matvec(xc, scalar_csc=True)) < 1e-14
     b = np.random.default_rng(11).standard_normal(fs.nfree)
     sol = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg)
     xs = np.zeros(fs.nfree)
@@ -156,6 +159,8 @@ def test_pmultigrid_q3_lognormal(hip_ctx, oracle):
     mf = MatrixFreeJacobian(fop).prepare(x)
     pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-2, smoother_storage="sym32").update()
     assert pmg.ok
+    xc = np.random.default_rng(10).standard_normal(cs.nfree)
+    assert rel(pmg.coarse_matvec(xc), pmg.coarse_matvec(xc, scalar_csc=True)) < 1e-14
     b = np.random.default_rng(12).standard_normal(fs.nfree)
     xs = np.zeros(fs.nfree)
     res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
