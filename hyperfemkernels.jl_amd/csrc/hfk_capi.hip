@@ -76,16 +76,15 @@ int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s,
 // sparsity (src/assembly.jl:773-818, full_keys!/cell_keys! :835-893) and cell_positions! (:864-880).
 // The reference sorts the keys (col << 32) | row of every free pair of every cell and removes duplicates; the
 // result is, per column, the sorted set of rows coupled to it.  Here each column's set is formed directly from
-// the cells that contain the column's dof (same CSC, bit for bit), in parallel over columns.
-int op_build_pattern(hfk_op* op) {
-  if (op->pattern_built) return HFK_OK;
-  const int nd = op->nd;
-  const int64_t ncols = op->nfree, ncells = op->ncells;
+// the cells that contain the column's dof (same CSC, bit for bit), in parallel over columns.  Generic over the
+// cell dof table (any element, any number of fields: every (test, trial) pair of a cell is coupled).
+void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
+                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos) {
   // cells of every free dof
   std::vector<int64_t> cstart((size_t)ncols + 1, 0);
   for (int64_t cell = 0; cell < ncells; ++cell)
     for (int j = 0; j < nd; ++j) {
-      const int32_t id = op->cell_dofs[(size_t)cell * nd + j];
+      const int32_t id = cell_dofs[(size_t)cell * nd + j];
       if (id > 0) cstart[id]++;
     }
   for (int64_t c = 0; c < ncols; ++c) cstart[c + 1] += cstart[c];
@@ -93,14 +92,14 @@ int op_build_pattern(hfk_op* op) {
   std::vector<int32_t> cells_of((size_t)cstart[ncols]);
   for (int64_t cell = 0; cell < ncells; ++cell)
     for (int j = 0; j < nd; ++j) {
-      const int32_t id = op->cell_dofs[(size_t)cell * nd + j];
+      const int32_t id = cell_dofs[(size_t)cell * nd + j];
       if (id > 0) cells_of[fillp[id - 1]++] = (int32_t)cell;
     }
   const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   auto rows_of = [&](int64_t c, std::vector<int32_t>& buf) {
     buf.clear();
     for (int64_t k = cstart[c]; k < cstart[c + 1]; ++k) {
-      const int32_t* ids = &op->cell_dofs[(size_t)cells_of[k] * nd];
+      const int32_t* ids = &cell_dofs[(size_t)cells_of[k] * nd];
       for (int ii = 0; ii < nd; ++ii)
         if (ids[ii] > 0) buf.push_back(ids[ii]);
     }
@@ -120,9 +119,9 @@ int op_build_pattern(hfk_op* op) {
       });
     for (auto& th : pool) th.join();
   }
-  op->colptr.assign((size_t)ncols + 1, 1);
-  for (int64_t c = 0; c < ncols; ++c) op->colptr[c + 1] = op->colptr[c] + cnt[c];
-  op->rowval.assign((size_t)(op->colptr[ncols] - 1), 0);
+  colptr.assign((size_t)ncols + 1, 1);
+  for (int64_t c = 0; c < ncols; ++c) colptr[c + 1] = colptr[c] + cnt[c];
+  rowval.assign((size_t)(colptr[ncols] - 1), 0);
   {
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; ++t)
@@ -130,38 +129,65 @@ int op_build_pattern(hfk_op* op) {
         std::vector<int32_t> buf;
         for (int64_t c = t; c < ncols; c += nth) {
           rows_of(c, buf);
-          int64_t* out = &op->rowval[(size_t)(op->colptr[c] - 1)];
+          int64_t* out = &rowval[(size_t)(colptr[c] - 1)];
           for (size_t k = 0; k < buf.size(); ++k) out[k] = buf[k];
         }
       });
     for (auto& th : pool) th.join();
   }
-  op->nzpos.assign((size_t)ncells * nd * nd, 0);
+  nzpos.assign((size_t)ncells * nd * nd, 0);
   {
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; ++t)
-      pool.emplace_back([op, nd, t, nth]() {
-        for (int64_t cell = t; cell < op->ncells; cell += nth) {
-          const int32_t* ids = &op->cell_dofs[(size_t)cell * nd];
-          int32_t* pos = &op->nzpos[(size_t)cell * nd * nd];
+      pool.emplace_back([&, t]() {
+        for (int64_t cell = t; cell < ncells; cell += nth) {
+          const int32_t* ids = &cell_dofs[(size_t)cell * nd];
+          int32_t* pos = &nzpos[(size_t)cell * nd * nd];
           for (int jj = 0; jj < nd; ++jj) {
             const int32_t c = ids[jj];
             if (c <= 0) continue;
-            const int64_t lo = op->colptr[c - 1] - 1, hi = op->colptr[c] - 1;
+            const int64_t lo = colptr[c - 1] - 1, hi = colptr[c] - 1;
             for (int ii = 0; ii < nd; ++ii) {
               const int32_t r = ids[ii];
               if (r <= 0) continue;
-              const int64_t* it = std::lower_bound(op->rowval.data() + lo, op->rowval.data() + hi, (int64_t)r);
-              pos[ii + nd * jj] = (int32_t)(it - op->rowval.data() + 1);
+              const int64_t* it = std::lower_bound(rowval.data() + lo, rowval.data() + hi, (int64_t)r);
+              pos[ii + nd * jj] = (int32_t)(it - rowval.data() + 1);
             }
           }
         }
       });
     for (auto& th : pool) th.join();
   }
+}
+
+int op_build_pattern(hfk_op* op) {
+  if (op->pattern_built) return HFK_OK;
+  csc_pattern(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos);
   op->pattern_built = true;
   return HFK_OK;
 }
+
+// src/coloring.jl:4-28: greedy first fit over the free (test) dofs with 128-bit masks; colour = trailing_ones.
+// Generic over the cell dof table; returns false when more than 128 colours are needed.
+bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color) {
+  std::vector<unsigned __int128> masks((size_t)nfree, 0);
+  color.assign((size_t)ncells, 0);
+  for (int64_t cell = 0; cell < ncells; ++cell) {
+    unsigned __int128 used = 0;
+    const int32_t* ids = &cell_dofs[(size_t)cell * nd];
+    for (int j = 0; j < nd; ++j)
+      if (ids[j] > 0) used |= masks[ids[j] - 1];
+    int c = 0;
+    while (c < 128 && ((used >> c) & 1)) ++c;
+    if (c >= 128) return false;
+    color[cell] = c + 1;
+    const unsigned __int128 bit = (unsigned __int128)1 << c;
+    for (int j = 0; j < nd; ++j)
+      if (ids[j] > 0) masks[ids[j] - 1] |= bit;
+  }
+  return true;
+}
+
 
 // field_blocks (src/preconditioners.jl:151-171): a block per node with a free component, first-seen order over
 // (cell, table node); block_of in the batch layout, tensor node order
@@ -255,23 +281,9 @@ void gauss01(int P, double* x, double* w) {
   }
 }
 
-// src/coloring.jl:4-28: greedy first fit over the free (test) dofs with 128-bit masks; colour = trailing_ones
 int color_cells(hfk_op* op) {
-  std::vector<unsigned __int128> masks((size_t)op->nfree, 0);
-  op->color.assign((size_t)op->ncells, 0);
-  for (int64_t cell = 0; cell < op->ncells; ++cell) {
-    unsigned __int128 used = 0;
-    const int32_t* ids = &op->cell_dofs[(size_t)cell * op->nd];
-    for (int j = 0; j < op->nd; ++j)
-      if (ids[j] > 0) used |= masks[ids[j] - 1];
-    int c = 0;
-    while (c < 128 && ((used >> c) & 1)) ++c;
-    if (c >= 128) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: more than 128 colours needed");
-    op->color[cell] = c + 1;
-    const unsigned __int128 bit = (unsigned __int128)1 << c;
-    for (int j = 0; j < op->nd; ++j)
-      if (ids[j] > 0) masks[ids[j] - 1] |= bit;
-  }
+  if (!greedy_colors(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->color))
+    return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: more than 128 colours needed");
   return HFK_OK;
 }
 

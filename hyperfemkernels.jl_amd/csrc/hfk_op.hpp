@@ -130,5 +130,10 @@ int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev,
                    double alpha, bool accumulate);
 int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s);
 int op_build_pattern(hfk_op* op);
+// the reference's CSC sparsity and cell positions (1-based) of a cell dof table (> 0 free ids)
+void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
+                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos);
+// the reference's greedy colouring (1-based colours); false when more than 128 colours are needed
+bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color);
 int op_build_blocks(hfk_op* op);
 }  // namespace hfk

@@ -1,0 +1,127 @@
+"""The Oseen-BDF2 step on tetrahedra (Taylor-Hood P2-P1, SUPG/PSPG/LSIC) as an operator with the reference's
+Algebra interface: residual!, sparsity_pattern / nonzero_positions, allocate_jacobian, jacobian!
+(src/assembly.jl:768-933), over libhfk's hfk_oseen_* entry points.  The weak form is the volume part of
+test/common.jl:165-228; the point data (advecting velocity w, stabilisation velocity un, history f) come from the
+host, as Gridap's FE-function leaves would.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, ptr
+from .operator import Context, default_context
+
+
+class OseenOperator:
+    """``OseenOperator(space, rho, mu, dt; bdf2)``: space is a tetmesh.TetP2P1Space (or any object with the same
+    fields); c0 = 1.5/dt (BDF2) or 1/dt (backward Euler)."""
+
+    def __init__(self, space, rho: float = 1.0, mu: float = 0.01, dt: float = 0.01, bdf2: bool = True,
+                 quadrature=None, dirichlet_values=None, ctx: Context | None = None):
+        from .tetmesh import tet_quadrature_deg5
+
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.space = space
+        self.rho, self.mu, self.dt, self.bdf2 = float(rho), float(mu), float(dt), bool(bdf2)
+        self.c0 = (1.5 if bdf2 else 1.0) / dt
+        qp, qw = quadrature if quadrature is not None else tet_quadrature_deg5()
+        self.qpoints = np.ascontiguousarray(qp, dtype=np.float64)
+        self.qweights = np.ascontiguousarray(qw, dtype=np.float64)
+        self.np = self.qweights.size
+        self.nfree, self.ndir = int(space.nfree), int(space.ndir)
+        cd = np.ascontiguousarray(space.cell_dofs, dtype=np.int32)
+        xy = np.ascontiguousarray(space.coords, dtype=np.float64)
+        h = C.c_void_p()
+        check(self.lib.hfk_oseen_create(self.ctx.handle, space.ncells, ptr(cd, C.c_int32), ptr(xy, C.c_double), self.np,
+                                        ptr(self.qpoints, C.c_double), ptr(self.qweights, C.c_double), self.nfree,
+                                        self.ndir, self.rho, self.mu, self.c0, C.byref(h)), self.ctx.handle)
+        self.handle = h
+        self._pattern = None
+        if dirichlet_values is not None:
+            self.set_dirichlet(dirichlet_values)
+
+    def set_dirichlet(self, dvals):
+        d = np.ascontiguousarray(dvals, dtype=np.float64)
+        if d.size != self.ndir:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: Dirichlet values")
+        check(self.lib.hfk_oseen_set_dirichlet(self.handle, ptr(d, C.c_double)), self.ctx.handle)
+
+    def point_coordinates(self) -> np.ndarray:
+        return self.space.quadrature_points(self.qpoints)
+
+    def set_fields(self, w, un, f):
+        """Point data (ncells, np, 3) each: advecting velocity, stabilisation velocity, history term."""
+        arr = np.concatenate([np.asarray(a, dtype=np.float64).reshape(self.space.ncells, self.np, 3) for a in (w, un, f)],
+                             axis=2)
+        arr = np.ascontiguousarray(arr)
+        check(self.lib.hfk_oseen_set_fields(self.handle, ptr(arr, C.c_double)), self.ctx.handle)
+
+    def set_steady_advection(self, wfun):
+        """The step of a steady advecting field w: un = un-1 = w, so w_adv = w, the stabilisation velocity is w
+        and the BDF history is f = c0 w (c1 un - c2 un-1 with c1 - c2 = c0)."""
+        wq = np.asarray(wfun(self.point_coordinates().reshape(-1, 3))).reshape(self.space.ncells, self.np, 3)
+        self.set_fields(wq, wq, self.c0 * wq)
+
+    def info(self) -> dict:
+        a = np.zeros(6, dtype=np.int64)
+        check(self.lib.hfk_oseen_info(self.handle, ptr(a, C.c_int64)), self.ctx.handle)
+        return dict(zip(["nfree", "ndir", "ncells", "np", "ncolors", "nnz"], (int(v) for v in a)))
+
+    def colors(self) -> np.ndarray:
+        c = np.zeros(self.space.ncells, dtype=np.int32)
+        check(self.lib.hfk_oseen_colors(self.handle, ptr(c, C.c_int32)), self.ctx.handle)
+        return c
+
+    def _vec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.nfree:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: state vector")
+        return x
+
+    def residual(self, b, x):
+        x = self._vec(x)
+        if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != self.nfree:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: residual vector")
+        check(self.lib.hfk_oseen_residual(self.handle, ptr(x, C.c_double), ptr(b, C.c_double)), self.ctx.handle)
+        return b
+
+    def pattern(self):
+        if self._pattern is None:
+            nnz = C.c_int64()
+            check(self.lib.hfk_oseen_pattern_nnz(self.handle, C.byref(nnz)), self.ctx.handle)
+            colptr = np.zeros(self.nfree + 1, dtype=np.int64)
+            rowval = np.zeros(nnz.value, dtype=np.int64)
+            nzpos = np.zeros((self.space.ncells, 34 * 34), dtype=np.int32)
+            check(self.lib.hfk_oseen_pattern(self.handle, ptr(colptr, C.c_int64), ptr(rowval, C.c_int64),
+                                             ptr(nzpos, C.c_int32)), self.ctx.handle)
+            self._pattern = (colptr, rowval, nzpos)
+        return self._pattern
+
+    def allocate_jacobian(self, x=None):
+        import scipy.sparse as sp
+
+        colptr, rowval, _ = self.pattern()
+        return sp.csc_matrix((np.zeros(rowval.size), rowval - 1, colptr - 1), shape=(self.nfree, self.nfree))
+
+    def jacobian(self, A, x=None):
+        colptr, rowval, _ = self.pattern()
+        if A.shape != (self.nfree, self.nfree) or A.nnz != rowval.size or not np.array_equal(A.indptr, colptr - 1):
+            raise RuntimeError("HyperFEMKernels: jacobian! on a matrix with another sparsity pattern")
+        if A.data.dtype != np.float64 or not A.data.flags.c_contiguous:
+            A.data = np.ascontiguousarray(A.data, dtype=np.float64)
+        check(self.lib.hfk_oseen_jacobian(self.handle, None, ptr(A.data, C.c_double)), self.ctx.handle)
+        return A
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.hfk_oseen_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -226,6 +226,43 @@ int hfk_krylov_solve(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const doub
 int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b_dev, double* x_dev,
                          double rtol, double atol, int maxiter, int* status, double* rnorm, void* stream);
 
+/* ------------------------------------------------------------------------------------------------------------
+ * Oseen-BDF2 step on tetrahedra, Taylor-Hood P2-P1 with SUPG/PSPG/LSIC (the weak form of test/common.jl:165-228,
+ * volume terms): coloured FP64 assembly of the residual and the CSC Jacobian, as residual!/jacobian!
+ * (src/assembly.jl:909-927) do for that form, with the reference's sparsity (src/assembly.jl:773-899) and colouring
+ * (src/coloring.jl:4-28) over the 34 dofs of each cell (velocity and pressure blocks all coupled).
+ *   cell_dofs  ncells x 34 signed ids: [c*10 + a] velocity component c at P2 node a (vertices 0..3, then the
+ *              midpoints of the edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)), [30 + a] pressure at vertex a;
+ *              > 0 free (1-based, velocity and pressure in one numbering), < 0 Dirichlet (index into dvals)
+ *   coords     ncells x 4 x 3 vertex coordinates (affine cells)
+ *   qpoints    np x 3 reference coordinates of the quadrature points on the unit tetrahedron, qweights np weights
+ *   rho, mu    density and dynamic viscosity; c0 the time-derivative coefficient (1/dt Euler, 1.5/dt BDF2)
+ * Point data (hfk_oseen_set_fields): ncells x np x 9 = advecting velocity w, velocity un of the stabilisation
+ * parameters, history term f. */
+typedef struct hfk_oseen hfk_oseen;
+int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, const double* coords, int np,
+                     const double* qpoints, const double* qweights, int64_t nfree, int64_t ndir, double rho, double mu,
+                     double c0, hfk_oseen** out);
+int hfk_oseen_destroy(hfk_oseen* op);
+int hfk_oseen_set_fields(hfk_oseen* op, const double* fields);
+/* refresh_dirichlet_values! (src/assembly.jl:393-410) */
+int hfk_oseen_set_dirichlet(hfk_oseen* op, const double* dvals);
+/* info[6]: nfree, ndir, ncells, np, ncolors, nnz (-1 before the pattern) */
+int hfk_oseen_info(hfk_oseen* op, int64_t* info);
+int hfk_oseen_colors(hfk_oseen* op, int32_t* colors);
+/* sparsity_pattern / nonzero_positions (src/assembly.jl:768-899): 1-based CSC and nzpos [cell][ii + 34 jj] */
+int hfk_oseen_pattern_nnz(hfk_oseen* op, int64_t* nnz);
+int hfk_oseen_pattern(hfk_oseen* op, int64_t* colptr, int64_t* rowval, int32_t* nzpos);
+/* residual! (src/assembly.jl:909-916): b = a(x, .) - l(.) on the free test dofs, Dirichlet values lifted */
+int hfk_oseen_residual(hfk_oseen* op, const double* x, double* b);
+int hfk_oseen_residual_dev(hfk_oseen* op, const double* x_dev, double* b_dev, void* stream);
+/* jacobian! (src/assembly.jl:918-927): the CSC values (the step is linear; x is not read) */
+int hfk_oseen_jacobian(hfk_oseen* op, const double* x, double* nzval);
+int hfk_oseen_jacobian_dev(hfk_oseen* op, const double* x_dev, double* nzval_dev, void* stream);
+/* residual_and_jacobian! (src/assembly.jl:929-933) in one pass */
+int hfk_oseen_residual_jacobian_dev(hfk_oseen* op, const double* x_dev, double* b_dev, double* nzval_dev,
+                                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

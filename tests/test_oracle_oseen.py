@@ -1,0 +1,200 @@
+"""CPU pins of the oracle's Oseen-BDF2 restatement (tetrahedral P2-P1, SUPG/PSPG/LSIC; test/common.jl:165-228).
+
+Each term of the element matrix is checked against what the mathematics fixes, on cells of a real mesh:
+* basis: Kronecker property at the P2 nodes, partition of unity, exact reproduction of quadratics;
+* quadrature: exact for every monomial of total degree <= 5;
+* mass: the textbook P2 tetrahedron mass matrix |K|/420 [6, 1, -4, -6, 32, 16, 8];
+* viscous 2 mu eps:eps: rigid motions (translations, rotations) are in its kernel, and it is not the full
+  gradient form (a rotation is not in the kernel of grad:grad);
+* pressure / continuity: a linear velocity field of trace t gives -t |K|/4 per pressure vertex;
+* convection with constant w on a linear field u = A x: rho (A w)_i int N_a;
+* LSIC: a divergence-free linear field is in its kernel; PSPG pressure block: -tp |K| grad M_a . grad M_b;
+* assembly: the sparsity equals the set of coupled pairs; the greedy colours never share a free dof within a
+  colour; the residual is affine, r(x) = J x + r(0).
+"""
+import numpy as np
+import pytest
+
+from hfk.tetmesh import EDGES, divfree_field, tet_p2p1_space, tet_quadrature_deg5
+from oracle import oseen as O
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    sp = tet_p2p1_space((2, 2, 1), domain=(0.0, 1.0, 0.0, 0.8, 0.0, 0.6))
+    qp, qw = tet_quadrature_deg5()
+    return sp, qp, qw
+
+
+def nodes_ref():
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
+    E = np.array([(V[a] + V[b]) / 2 for a, b in EDGES])
+    return np.vstack([V, E])
+
+
+def test_p2_basis():
+    X = nodes_ref()
+    N, G = O.basis(X)
+    assert np.allclose(N[:, :10], np.eye(10), atol=1e-15)
+    rng = np.random.default_rng(0)
+    pts = rng.dirichlet(np.ones(4), size=20)[:, 1:]
+    N, G = O.basis(pts)
+    assert np.allclose(N[:, :10].sum(axis=1), 1.0) and np.allclose(G[:, :10].sum(axis=1), 0.0)
+    assert np.allclose(N[:, 10:].sum(axis=1), 1.0)
+    # quadratic reproduction: f = 1 + 2x - y + 3xz + z^2
+    f = lambda P: 1 + 2 * P[:, 0] - P[:, 1] + 3 * P[:, 0] * P[:, 2] + P[:, 2] ** 2
+    assert np.allclose(N[:, :10] @ f(X), f(pts), atol=1e-14)
+
+
+def test_quadrature_degree5():
+    from math import factorial
+
+    qp, qw = tet_quadrature_deg5()
+    assert np.all(qw > 0) and np.all(qp >= 0) and np.all(qp.sum(axis=1) <= 1)
+    for a in range(6):
+        for b in range(6 - a):
+            for c in range(6 - a - b):
+                exact = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+                assert (qw * qp[:, 0] ** a * qp[:, 1] ** b * qp[:, 2] ** c).sum() == pytest.approx(exact, rel=1e-13)
+
+
+def terms(sp, qp, qw, w=None, un=None, f=None, rho=1.3, mu=0.07, c0=20.0):
+    npt = len(qw)
+    fields = np.zeros((sp.ncells, npt, 9))
+    for k, v in enumerate((w, un, f)):
+        if v is not None:
+            fields[:, :, 3 * k:3 * k + 3] = v
+    return O.element_terms(sp.coords, qp, qw, fields, rho, mu, c0)
+
+
+def test_mass_matrix_closed_form(mesh):
+    sp, qp, qw = mesh
+    T, _ = terms(sp, qp, qw, rho=1.3, c0=20.0)
+    M = np.zeros((10, 10))
+    for a in range(10):
+        for b in range(10):
+            if a < 4 and b < 4:
+                M[a, b] = 6 if a == b else 1
+            elif a < 4 or b < 4:
+                v, e = (a, b - 4) if a < 4 else (b, a - 4)
+                M[a, b] = -4 if v in EDGES[e] else -6
+            else:
+                ea, eb = EDGES[a - 4], EDGES[b - 4]
+                M[a, b] = 32 if a == b else (16 if set(ea) & set(eb) else 8)
+    vol = sp.volumes()
+    for c in range(3):
+        blk = T["mass"][:, c * 10:(c + 1) * 10, c * 10:(c + 1) * 10]
+        assert np.allclose(blk, 1.3 * 20.0 * vol[:, None, None] * M[None] / 420, rtol=1e-12, atol=1e-16)
+    # no coupling between components or with pressure
+    assert np.abs(T["mass"][:, :10, 10:]).max() < 1e-16
+
+
+def nodal_field(sp, cell_fun):
+    """Element dof vectors (ncells, 34) of a velocity field given at the P2 nodes of every cell."""
+    out = np.zeros((sp.ncells, 34))
+    X = nodes_ref()
+    for c in range(sp.ncells):
+        x0 = sp.coords[c, 0]
+        J = (sp.coords[c, 1:] - x0).T
+        P = x0[None, :] + X @ J.T
+        u = cell_fun(P)  # (10, 3)
+        for k in range(3):
+            out[c, k * 10:(k + 1) * 10] = u[:, k]
+    return out
+
+
+def test_viscous_kernel_rigid_motions(mesh):
+    sp, qp, qw = mesh
+    T, _ = terms(sp, qp, qw, mu=0.07)
+    Kv = T["viscous"]
+    for fun in (lambda P: np.tile([1.0, -2.0, 0.5], (len(P), 1)),               # translation
+                lambda P: np.cross(np.array([0.3, -0.7, 1.1])[None, :], P)):    # rotation
+        xe = nodal_field(sp, fun)
+        assert np.abs(np.einsum("cab,cb->ca", Kv, xe)).max() < 1e-13 * np.abs(Kv).max()
+    # a pure shear has energy 2 mu |eps|^2 |K|
+    A = np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    xe = nodal_field(sp, lambda P: P @ A.T)
+    e = 0.5 * (A + A.T)
+    energy = np.einsum("ca,cab,cb->c", xe, Kv, xe)
+    assert np.allclose(energy, 2 * 0.07 * (e * e).sum() * sp.volumes(), rtol=1e-12)
+
+
+def test_divergence_terms_linear_field(mesh):
+    sp, qp, qw = mesh
+    T, _ = terms(sp, qp, qw)
+    A = np.array([[0.4, -0.2, 0.1], [0.3, 0.7, -0.5], [0.2, 0.1, -0.3]])
+    xe = nodal_field(sp, lambda P: P @ A.T)
+    tr = np.trace(A)
+    q_rows = np.einsum("cab,cb->ca", T["continuity"], xe)[:, 30:]
+    assert np.allclose(q_rows, -tr * sp.volumes()[:, None] / 4, rtol=1e-12)
+    # the pressure term is its transpose
+    assert np.allclose(T["pressure"], np.transpose(T["continuity"], (0, 2, 1)), atol=1e-16)
+
+
+def test_convection_constant_w(mesh):
+    sp, qp, qw = mesh
+    w = np.array([0.5, -1.0, 2.0])
+    T, _ = terms(sp, qp, qw, w=w, rho=1.3)
+    A = np.array([[0.4, -0.2, 0.1], [0.3, 0.7, -0.5], [0.2, 0.1, -0.3]])
+    xe = nodal_field(sp, lambda P: P @ A.T)
+    out = np.einsum("cab,cb->ca", T["convection"], xe)
+    intN = np.array([-1 / 20] * 4 + [1 / 5] * 6)  # int N_a / |K| of the P2 functions
+    Aw = A @ w
+    for i in range(3):
+        assert np.allclose(out[:, i * 10:(i + 1) * 10], 1.3 * Aw[i] * sp.volumes()[:, None] * intN[None], rtol=1e-12)
+
+
+def test_lsic_and_pspg_blocks(mesh):
+    sp, qp, qw = mesh
+    un = np.array([0.3, 0.4, 1.2])
+    T, _ = terms(sp, qp, qw, w=un, un=un, rho=1.3, mu=0.07)
+    A = np.array([[0.4, -0.2, 0.1], [0.3, 0.7, -0.5], [0.2, 0.1, -1.1]])  # trace 0: divergence free
+    xe = nodal_field(sp, lambda P: P @ A.T)
+    assert np.abs(np.einsum("cab,cb->ca", T["lsic"], xe)).max() < 1e-12 * np.abs(T["lsic"]).max()
+    vol = sp.volumes()
+    h = (6 * np.sqrt(2) * vol) ** (1 / 3)
+    ts = ((2 * np.linalg.norm(un) / h) ** 2 + 9 * (4 * 0.07 / (1.3 * h ** 2)) ** 2) ** -0.5
+    tp = ts / 1.3
+    G = np.zeros((sp.ncells, 4, 3))
+    for c in range(sp.ncells):
+        J = (sp.coords[c, 1:] - sp.coords[c, 0]).T
+        G[c] = O.GL @ np.linalg.inv(J)
+    ref = -tp[:, None, None] * vol[:, None, None] * np.einsum("cai,cbi->cab", G, G)
+    assert np.allclose(T["pspg"][:, 30:, 30:], ref, rtol=1e-12)
+
+
+def test_pattern_colors_and_affine_residual(mesh):
+    sp, qp, qw = mesh
+    colptr, rowval, nzpos = O.pattern(sp.cell_dofs, sp.nfree)
+    pairs = set()
+    for ids in sp.cell_dofs:
+        f = [int(i) for i in ids if i > 0]
+        pairs.update((c, r) for c in f for r in f)
+    col = np.repeat(np.arange(1, sp.nfree + 1), np.diff(colptr))
+    assert set(zip(col.tolist(), rowval.tolist())) == pairs
+    assert np.all(np.diff(rowval)[np.diff(col) == 0] > 0)  # sorted within columns
+    colors = O.greedy_colors(sp.cell_dofs, sp.nfree)
+    for k in range(1, colors.max() + 1):
+        ids = sp.cell_dofs[colors == k]
+        f = ids[ids > 0]
+        assert f.size == np.unique(f).size
+    rng = np.random.default_rng(3)
+    fields = rng.standard_normal((sp.ncells, len(qw), 9))
+    K, load = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.01, 150.0)
+    nz = O.jacobian_values(K, nzpos, rowval.size)
+    import scipy.sparse as ssp
+
+    Am = ssp.csc_matrix((nz, rowval - 1, colptr - 1), shape=(sp.nfree, sp.nfree))
+    d = rng.standard_normal(sp.ndir)
+    x = rng.standard_normal(sp.nfree)
+    r0 = O.residual(sp, np.zeros(sp.nfree), d, K, load)
+    r = O.residual(sp, x, d, K, load)
+    assert np.linalg.norm(r - r0 - Am @ x) < 1e-13 * np.linalg.norm(r)
+
+
+def test_divfree_field():
+    w = divfree_field(5)
+    x = np.random.default_rng(1).random((50, 3))
+    h = 1e-5
+    div = sum((w(x + h * e)[:, i] - w(x - h * e)[:, i]) / (2 * h) for i, e in enumerate(np.eye(3)))
+    assert np.abs(div).max() < 1e-6 * np.abs(w(x)).max()
