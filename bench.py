@@ -140,6 +140,75 @@ def newton_leg(rtol=1e-10, maxit=20):
             "newton_total_s": t_all, "note": "residual!/jacobian! timed through host arrays (PCIe included)"}
 
 
+def oseen_leg(n=40, reps=10):
+    """The Oseen-BDF2 step on tetrahedra: unit cube n^3 split into 6 tetrahedra per cell, Taylor-Hood P2-P1,
+    rho = 1, mu = 0.01, dt = 0.01 (BDF2), SUPG/PSPG/LSIC; inlet x = 0 with the parabolic profile
+    16 y(1-y) z(1-z), walls on the y and z faces, outlet x = 1; advecting velocity the 12-mode divergence-free
+    field of seed 5 scaled to unit RMS over the velocity nodes (steady: un = w, history c0 w).  Times the device
+    residual!, jacobian! and the fused pass (state and outputs on the device, HIP events on the stream), and the
+    host-side pattern build; the 14-point degree-5 rule."""
+    import numpy as np
+    import torch
+
+    from hfk._lib import check
+    from hfk.oseen import OseenOperator
+    from hfk.tetmesh import divfree_field, tet_p2p1_space
+
+    import ctypes as C
+
+    t0 = time.perf_counter()
+    sp = tet_p2p1_space((n, n, n))
+    wraw = divfree_field(5)
+    scale = 1.0 / np.sqrt((wraw(sp.node_xyz) ** 2).sum(axis=1).mean())
+    w = lambda X: scale * wraw(X)
+    gin = lambda X: np.stack([16 * X[:, 1] * (1 - X[:, 1]) * X[:, 2] * (1 - X[:, 2]), 0 * X[:, 0], 0 * X[:, 0]], 1)
+    xd = sp.node_xyz[sp.dir_node_of]
+    dv = np.where((xd[:, 0] == 0.0)[:, None], gin(xd), 0.0).reshape(-1)
+    t_mesh = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    op = OseenOperator(sp, rho=1.0, mu=0.01, dt=0.01, bdf2=True, dirichlet_values=dv)
+    t_create = time.perf_counter() - t0
+    op.set_steady_advection(w)
+    t0 = time.perf_counter()
+    colptr, rowval, nzpos = op.pattern()
+    t_pattern = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    x = torch.zeros(sp.nfree, dtype=torch.float64, device=dev)
+    b = torch.zeros(sp.nfree, dtype=torch.float64, device=dev)
+    nz = torch.zeros(rowval.size, dtype=torch.float64, device=dev)
+    sptr = C.c_void_p(stream.cuda_stream or None)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+
+    xp, bp, np_ = C.c_void_p(x.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(nz.data_ptr())
+    t_res = timed(lambda: check(op.lib.hfk_oseen_residual_dev(op.handle, xp, bp, sptr), op.ctx.handle))
+    t_jac = timed(lambda: check(op.lib.hfk_oseen_jacobian_dev(op.handle, xp, np_, sptr), op.ctx.handle))
+    t_both = timed(lambda: check(op.lib.hfk_oseen_residual_jacobian_dev(op.handle, xp, bp, np_, sptr), op.ctx.handle))
+    bb = b.cpu().numpy()
+    info = op.info()
+    ncells = sp.ncells
+    nnz = int(rowval.size)
+    # bytes of one Jacobian assembly: nzpos read once, every stored value read and written once (its other
+    # updates hit the caches), cell ids and point data read once
+    jac_bytes = ncells * 34 * 34 * 4 + 2 * nnz * 8 + ncells * (34 * 4 + 12 * 8 + op.np * 9 * 8)
+    return {"free_dofs": sp.nfree, "total_dofs": sp.nfree + sp.ndir, "cells": ncells, "nnz": nnz,
+            "colors": info["ncolors"], "points_per_cell": op.np, "residual_ms": t_res * 1e3,
+            "jacobian_ms": t_jac * 1e3, "residual_and_jacobian_ms": t_both * 1e3,
+            "jacobian_Mcells_per_s": ncells / t_jac / 1e6, "jacobian_min_bytes": jac_bytes,
+            "jacobian_GBs_of_min_bytes": jac_bytes / t_jac / 1e9, "pattern_s": t_pattern, "create_s": t_create,
+            "mesh_s": t_mesh, "residual_at_zero_norm": float(np.linalg.norm(bb))}
+
+
 def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8,
               learned=None, order=2, mu=None, state_amp=1e-2):
     """CG-type solve J(x) du = -R(x) to ||b - J du|| <= rtol ||b|| with one p-multigrid V-cycle per iteration:
@@ -450,6 +519,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--no-oseen", action="store_true")
     ap.add_argument("--solve-legs", default="fp64,mixed,learned,fields,q3",
                     help="solve legs: fp64, mixed, learned, fields, q3")
     args = ap.parse_args()
@@ -593,6 +663,10 @@ def main():
                                                               state_amp=3e-3)
             res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
                                                                state_amp=3e-3)
+    if not args.no_oseen:
+        print("bench: Oseen-BDF2 step on 40^3 x 6 tetrahedra (P2-P1, SUPG/PSPG/LSIC): device assembly",
+              file=sys.stderr, flush=True)
+        res["oseen_tet_40"] = oseen_leg()
     if not args.no_newton:
         print("bench: SVK quick start 20x10x10 Q2, Newton with host LU", file=sys.stderr, flush=True)
         res["newton_svk_20x10x10"] = newton_leg()
