@@ -173,11 +173,12 @@ def oseen_leg(n=40, reps=10):
     colptr, rowval, nzpos = op.pattern()
     t_pattern = time.perf_counter() - t0
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    stream = torch.cuda.Stream(dev)  # a stream of our own: the events and the library's launches share it
     x = torch.zeros(sp.nfree, dtype=torch.float64, device=dev)
     b = torch.zeros(sp.nfree, dtype=torch.float64, device=dev)
     nz = torch.zeros(rowval.size, dtype=torch.float64, device=dev)
-    sptr = C.c_void_p(stream.cuda_stream or None)
+    assert stream.cuda_stream != 0
+    sptr = C.c_void_p(stream.cuda_stream)
 
     def timed(fn):
         fn()

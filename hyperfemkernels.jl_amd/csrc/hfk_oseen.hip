@@ -31,7 +31,7 @@ namespace hfk {
 namespace {
 
 constexpr int kND = 34;             // dofs per cell
-constexpr int kCellsPerGroup = 7;   // 7 x 34 = 238 of 256 threads: one thread per element-matrix row
+constexpr int kCellsPerGroup = 3;   // 3 x 2 x 34 = 204 of 256 threads: two threads per element-matrix row
 constexpr int kMaxPoints = 64;
 
 struct OseenDev {
@@ -46,30 +46,84 @@ struct OseenDev {
   const int32_t* nzpos;   // [cell][ii + 34 jj], 1-based, 0: not stored
 };
 
-__device__ __forceinline__ void p2_basis(const double l[4], const double gl[4][3], double N[10], double G[10][3]) {
+// barycentric coordinate k (0..3) of a reference point and its (constant) reference gradient
+__device__ __forceinline__ double bary(int k, double xi, double et, double ze) {
+  return k == 0 ? 1.0 - xi - et - ze : (k == 1 ? xi : (k == 2 ? et : ze));
+}
+__device__ __forceinline__ void bary_grad(int k, double g[3]) {
+  g[0] = k == 0 ? -1.0 : (k == 1 ? 1.0 : 0.0);
+  g[1] = k == 0 ? -1.0 : (k == 2 ? 1.0 : 0.0);
+  g[2] = k == 0 ? -1.0 : (k == 3 ? 1.0 : 0.0);
+}
+
+// shape function f of the cell at a reference point: f = 0..3 P2 vertex functions l(2l - 1), 4..9 P2 edge
+// functions 4 l_a l_b (edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)), 10..13 the P1 functions l; value and
+// reference gradient
+__device__ __forceinline__ void shape(int f, double xi, double et, double ze, double& n, double g[3]) {
+  if (f < 4 || f >= 10) {
+    const int k = f < 4 ? f : f - 10;
+    const double l = bary(k, xi, et, ze);
+    double gl[3];
+    bary_grad(k, gl);
+    const double s = f < 4 ? 4.0 * l - 1.0 : 1.0;
+    n = f < 4 ? l * (2.0 * l - 1.0) : l;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    N[a] = l[a] * (2.0 * l[a] - 1.0);
+    for (int d = 0; d < 3; ++d) g[d] = s * gl[d];
+  } else {
+    const int e = f - 4;
+    const int a = e < 3 ? 0 : (e < 5 ? 1 : 2);
+    const int b = e < 3 ? e + 1 : (e < 5 ? e - 1 : 3);
+    const double la = bary(a, xi, et, ze), lb = bary(b, xi, et, ze);
+    double ga[3], gb[3];
+    bary_grad(a, ga);
+    bary_grad(b, gb);
+    n = 4.0 * la * lb;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) G[a][d] = (4.0 * l[a] - 1.0) * gl[a][d];
-  }
-  const int ea[6] = {0, 0, 0, 1, 1, 2}, eb[6] = {1, 2, 3, 2, 3, 3};
-#pragma unroll
-  for (int e = 0; e < 6; ++e) {
-    const int a = ea[e], b = eb[e];
-    N[4 + e] = 4.0 * l[a] * l[b];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) G[4 + e][d] = 4.0 * (l[b] * gl[a][d] + l[a] * gl[b][d]);
+    for (int d = 0; d < 3; ++d) g[d] = 4.0 * (lb * ga[d] + la * gb[d]);
   }
 }
 
-// One work-group: kCellsPerGroup cells of one colour, one thread per element-matrix row (test function).  Per
-// quadrature point the cell's 14 shape functions (value, physical gradient, w . gradient) are staged in LDS;
-// each thread accumulates its row of K (34 entries) and its load.  Then the row is scattered: into the CSC values
-// through nzpos (free trial dofs), and/or r = K x_e - l into the residual (free test dof).  The cells of one colour
-// share no free dof, so the plain read-modify-writes do not race and the sums follow the colour order.
+// The 17 entries [J0, J0 + 17) of the element-matrix row of test function (vel: velocity component ic of P2
+// node a; else pressure vertex a - 10) at one point, added to acc with the weight dV.  S: the cell's 14 shape
+// functions at the point (value, physical gradient, w . gradient).
+template <int J0>
+__device__ __forceinline__ void accumulate(const double (*S)[5], bool vel, int ic, int a, double dV, double ts,
+                                           double tp, double tl, double c, double rho, double mu, double* acc) {
+  const double Na = S[a][0], ga[3] = {S[a][1], S[a][2], S[a][3]}, wga = S[a][4];
+  const double gai = ic == 0 ? ga[0] : (ic == 1 ? ga[1] : ga[2]);
+#pragma unroll
+  for (int t = 0; t < 17; ++t) {
+    const int jj = J0 + t;
+    double v;
+    if (jj < 30) {  // trial velocity component jc at P2 node nb
+      const int jc = jj / 10, nb = jj % 10;
+      const double Nb = S[nb][0], wgb = S[nb][4], gb_j = S[nb][1 + jc];
+      if (vel) {
+        v = mu * ga[jc] * S[nb][1 + ic] + tl * gai * gb_j;
+        if (jc == ic) {
+          const double gab = ga[0] * S[nb][1] + ga[1] * S[nb][2] + ga[2] * S[nb][3];
+          v += rho * c * Na * Nb + mu * gab + rho * Na * wgb + ts * rho * wga * (c * Nb + wgb);
+        }
+      } else {
+        v = -Na * gb_j - tp * rho * ga[jc] * (c * Nb + wgb);
+      }
+    } else {  // trial pressure at vertex nb
+      const int nb = 10 + jj - 30;
+      if (vel) v = -S[nb][0] * gai + ts * wga * S[nb][1 + ic];
+      else v = -tp * (ga[0] * S[nb][1] + ga[1] * S[nb][2] + ga[2] * S[nb][3]);
+    }
+    acc[t] += dV * v;
+  }
+}
+
+// One work-group: kCellsPerGroup cells of one colour, two threads per element-matrix row (test function), each
+// for 17 of its 34 entries.  Per quadrature point the cell's 14 shape functions (value, physical gradient,
+// w . gradient) and the point scalars are staged in LDS; each thread accumulates its 17 entries (and the first of
+// the two the row's load).  Then the entries are scattered into the CSC values through nzpos (free trial dofs),
+// and/or the two halves of K x_e - l are added in LDS and written to the residual (free test dof).  The cells of
+// one colour share no free dof, so the plain read-modify-writes do not race and the sums follow the colour order.
 template <bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(256) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, int64_t ncell_color,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, int64_t ncell_color,
                                                const double* __restrict__ x, double* __restrict__ b,
                                                double* __restrict__ nzval) {
   __shared__ double S[kCellsPerGroup][14][5];  // per cell: N, grad N (3), w . grad N
@@ -77,12 +131,14 @@ __global__ __launch_bounds__(256) void k_oseen(OseenDev op, const int32_t* __res
   __shared__ double Jit[kCellsPerGroup][9];    // inverse transposed Jacobian (physical gradient = Jit * ref)
   __shared__ double Cm[kCellsPerGroup][2];     // |det J|, h
   __shared__ double Xe[kCellsPerGroup][kND];
+  __shared__ double Rp[kCellsPerGroup][2][kND];  // the two halves of the residual rows
   const int tid = threadIdx.x;
-  const int lc = tid / kND, row = tid - lc * kND;
+  const int lc = tid / (2 * kND), rem = tid - lc * 2 * kND;
+  const int half = rem / kND, row = rem - half * kND;
   const int64_t k = (int64_t)blockIdx.x * kCellsPerGroup + lc;
   const bool active = lc < kCellsPerGroup && k < ncell_color;
   const int64_t cell = active ? cells[k] : 0;
-  if (active && row == 0) {
+  if (active && rem == 0) {
     const double* X = op.coords + cell * 12;
     double J[9];  // column d: x_{d+1} - x_0
 #pragma unroll
@@ -114,33 +170,20 @@ __global__ __launch_bounds__(256) void k_oseen(OseenDev op, const int32_t* __res
   int32_t myid = 0;
   if (active) {
     myid = op.ids[cell * kND + row];
-    Xe[lc][row] = myid > 0 ? (x ? x[myid - 1] : 0.0) : op.dvals[-myid - 1];
+    if (half == 0) Xe[lc][row] = myid > 0 ? (x ? x[myid - 1] : 0.0) : op.dvals[-myid - 1];
   }
   // the row's test function: velocity component ic of node a, or pressure node a
   const bool vel = row < 30;
   const int ic = vel ? row / 10 : 0, a = vel ? row % 10 : 10 + (row - 30);
-  double acc[kND];
+  double acc[17];
 #pragma unroll
-  for (int j = 0; j < kND; ++j) acc[j] = 0.0;
+  for (int j = 0; j < 17; ++j) acc[j] = 0.0;
   double load = 0.0;
   for (int q = 0; q < op.np; ++q) {
     __syncthreads();
-    if (active && row < 14) {  // shape function `row` of this cell at point q
-      const double xi = op.qp[4 * q], et = op.qp[4 * q + 1], ze = op.qp[4 * q + 2];
-      const double l[4] = {1.0 - xi - et - ze, xi, et, ze};
-      const double gl[4][3] = {{-1.0, -1.0, -1.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    if (active && rem < 14) {  // shape function `rem` of this cell at point q
       double n, g[3];
-      if (row < 10) {
-        double N[10], G[10][3];
-        p2_basis(l, gl, N, G);
-        n = N[row];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[d] = G[row][d];
-      } else {
-        n = l[row - 10];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[d] = gl[row - 10][d];
-      }
+      shape(rem, op.qp[4 * q], op.qp[4 * q + 1], op.qp[4 * q + 2], n, g);
       const double* Ji = Jit[lc];
       const double* fq = op.fields + ((size_t)cell * op.np + q) * 9;
       double gx[3], wg = 0.0;
@@ -149,13 +192,13 @@ __global__ __launch_bounds__(256) void k_oseen(OseenDev op, const int32_t* __res
         gx[i] = Ji[i] * g[0] + Ji[i + 3] * g[1] + Ji[i + 6] * g[2];
         wg += fq[i] * gx[i];
       }
-      S[lc][row][0] = n;
-      S[lc][row][1] = gx[0];
-      S[lc][row][2] = gx[1];
-      S[lc][row][3] = gx[2];
-      S[lc][row][4] = wg;
+      S[lc][rem][0] = n;
+      S[lc][rem][1] = gx[0];
+      S[lc][rem][2] = gx[1];
+      S[lc][rem][3] = gx[2];
+      S[lc][rem][4] = wg;
     }
-    if (active && row == 14) {  // point scalars: dV, stabilisation parameters, w, f
+    if (active && rem == 14) {  // point scalars: dV, stabilisation parameters, w, f
       const double* fq = op.fields + ((size_t)cell * op.np + q) * 9;
       const double un = sqrt(fq[3] * fq[3] + fq[4] * fq[4] + fq[5] * fq[5]);
       const double h = Cm[lc][1];
@@ -174,62 +217,40 @@ __global__ __launch_bounds__(256) void k_oseen(OseenDev op, const int32_t* __res
     }
     __syncthreads();
     if (!active) continue;
-    const double dV = P[lc][0], ts = P[lc][1], tp = P[lc][2], tl = P[lc][3], c = P[lc][4], rho = op.rho, mu = op.mu;
-    const double Na = S[lc][a][0], ga[3] = {S[lc][a][1], S[lc][a][2], S[lc][a][3]}, wga = S[lc][a][4];
-    const double f[3] = {P[lc][8], P[lc][9], P[lc][10]};
-    if (vel) {
-      // trial velocity (jc, nb)
-#pragma unroll
-      for (int jc = 0; jc < 3; ++jc)
-#pragma unroll
-        for (int nb = 0; nb < 10; ++nb) {
-          const double Nb = S[lc][nb][0], wgb = S[lc][nb][4];
-          const double gb_i = S[lc][nb][1 + ic], gb_j = S[lc][nb][1 + jc];
-          double v = mu * ga[jc] * gb_i + tl * ga[ic] * gb_j;
-          if (jc == ic) {
-            const double gab = ga[0] * S[lc][nb][1] + ga[1] * S[lc][nb][2] + ga[2] * S[lc][nb][3];
-            v += rho * c * Na * Nb + mu * gab + rho * Na * wgb + ts * rho * wga * (c * Nb + wgb);
-          }
-          acc[jc * 10 + nb] += dV * v;
-        }
-      // trial pressure nb
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        const double Mb = S[lc][10 + nb][0], gMb_i = S[lc][10 + nb][1 + ic];
-        acc[30 + nb] += dV * (-Mb * ga[ic] + ts * wga * gMb_i);
+    const double dV = P[lc][0], ts = P[lc][1], tp = P[lc][2], tl = P[lc][3], c = P[lc][4];
+    if (half == 0) {
+      accumulate<0>(S[lc], vel, ic, a, dV, ts, tp, tl, c, op.rho, op.mu, acc);
+      const double Na = S[lc][a][0], wga = S[lc][a][4];
+      const double f[3] = {P[lc][8], P[lc][9], P[lc][10]};
+      if (vel) {
+        const double fi = ic == 0 ? f[0] : (ic == 1 ? f[1] : f[2]);
+        load += dV * (op.rho * fi * Na + ts * wga * op.rho * fi);
+      } else {
+        load += dV * (-tp * op.rho * (S[lc][a][1] * f[0] + S[lc][a][2] * f[1] + S[lc][a][3] * f[2]));
       }
-      load += dV * (rho * f[ic] * Na + ts * wga * rho * f[ic]);
     } else {
-      // pressure test: trial velocity (jc, nb)
-#pragma unroll
-      for (int jc = 0; jc < 3; ++jc)
-#pragma unroll
-        for (int nb = 0; nb < 10; ++nb) {
-          const double Nb = S[lc][nb][0], wgb = S[lc][nb][4], gb_j = S[lc][nb][1 + jc];
-          acc[jc * 10 + nb] += dV * (-Na * gb_j - tp * rho * ga[jc] * (c * Nb + wgb));
-        }
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        const double gMb[3] = {S[lc][10 + nb][1], S[lc][10 + nb][2], S[lc][10 + nb][3]};
-        acc[30 + nb] += dV * (-tp * (ga[0] * gMb[0] + ga[1] * gMb[1] + ga[2] * gMb[2]));
-      }
-      load += dV * (-tp * rho * (ga[0] * f[0] + ga[1] * f[1] + ga[2] * f[2]));
+      accumulate<17>(S[lc], vel, ic, a, dV, ts, tp, tl, c, op.rho, op.mu, acc);
     }
   }
-  if (!active || myid <= 0) return;  // Dirichlet test rows are not assembled
-  if (WANT_K) {
-    const int32_t* pos = op.nzpos + cell * kND * kND;
+  const bool assemble = active && myid > 0;  // Dirichlet test rows are not assembled
+  const int j0 = half * 17;
+  if (WANT_K && assemble) {
+    const int32_t* pos = op.nzpos + cell * kND * kND + row;
 #pragma unroll
-    for (int jj = 0; jj < kND; ++jj) {
-      const int32_t p = pos[row + kND * jj];
-      if (p > 0) nzval[p - 1] += acc[jj];
+    for (int t = 0; t < 17; ++t) {
+      const int32_t p = pos[kND * (j0 + t)];
+      if (p > 0) nzval[p - 1] += acc[t];
     }
   }
   if (WANT_R) {
-    double r = -load;
+    if (active) {
+      double r = half == 0 ? -load : 0.0;
 #pragma unroll
-    for (int jj = 0; jj < kND; ++jj) r += acc[jj] * Xe[lc][jj];
-    b[myid - 1] += r;
+      for (int t = 0; t < 17; ++t) r += acc[t] * Xe[lc][j0 + t];
+      Rp[lc][half][row] = r;
+    }
+    __syncthreads();
+    if (assemble && half == 0) b[myid - 1] += Rp[lc][0][row] + Rp[lc][1][row];
   }
 }
 
