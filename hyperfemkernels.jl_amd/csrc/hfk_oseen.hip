@@ -31,8 +31,7 @@ namespace hfk {
 namespace {
 
 constexpr int kND = 34;             // dofs per cell
-constexpr int kCellsPerGroup = 3;   // 3 x 2 x 34 = 204 of 256 threads: two threads per element-matrix row
-constexpr int kMaxPoints = 64;
+constexpr int kMaxPoints = 32;
 
 struct OseenDev {
   int64_t ncells;
@@ -83,62 +82,30 @@ __device__ __forceinline__ void shape(int f, double xi, double et, double ze, do
   }
 }
 
-// The 17 entries [J0, J0 + 17) of the element-matrix row of test function (vel: velocity component ic of P2
-// node a; else pressure vertex a - 10) at one point, added to acc with the weight dV.  S: the cell's 14 shape
-// functions at the point (value, physical gradient, w . gradient).
-template <int J0>
-__device__ __forceinline__ void accumulate(const double (*S)[5], bool vel, int ic, int a, double dV, double ts,
-                                           double tp, double tl, double c, double rho, double mu, double* acc) {
-  const double Na = S[a][0], ga[3] = {S[a][1], S[a][2], S[a][3]}, wga = S[a][4];
-  const double gai = ic == 0 ? ga[0] : (ic == 1 ? ga[1] : ga[2]);
-#pragma unroll
-  for (int t = 0; t < 17; ++t) {
-    const int jj = J0 + t;
-    double v;
-    if (jj < 30) {  // trial velocity component jc at P2 node nb
-      const int jc = jj / 10, nb = jj % 10;
-      const double Nb = S[nb][0], wgb = S[nb][4], gb_j = S[nb][1 + jc];
-      if (vel) {
-        v = mu * ga[jc] * S[nb][1 + ic] + tl * gai * gb_j;
-        if (jc == ic) {
-          const double gab = ga[0] * S[nb][1] + ga[1] * S[nb][2] + ga[2] * S[nb][3];
-          v += rho * c * Na * Nb + mu * gab + rho * Na * wgb + ts * rho * wga * (c * Nb + wgb);
-        }
-      } else {
-        v = -Na * gb_j - tp * rho * ga[jc] * (c * Nb + wgb);
-      }
-    } else {  // trial pressure at vertex nb
-      const int nb = 10 + jj - 30;
-      if (vel) v = -S[nb][0] * gai + ts * wga * S[nb][1 + ic];
-      else v = -tp * (ga[0] * S[nb][1] + ga[1] * S[nb][2] + ga[2] * S[nb][3]);
-    }
-    acc[t] += dV * v;
-  }
-}
+// One work-group per cell, the element matrix in factorised form.  The velocity-velocity block of test (i, a)
+// and trial (j, b) is  delta_ij S_ab + P_ab[j][i] + Q_ab[i][j]  with the point sums
+//   S_ab = sum dV (rho c Na Nb + mu ga.gb + rho Na (w.gb) + ts rho (w.ga)(c Nb + w.gb)),
+//   P_ab = sum dV mu ga (x) gb   (the second half of 2 mu eps:eps),   Q_ab = sum dV tl ga (x) gb   (LSIC),
+// so one thread per node pair (a, b) accumulates 19 numbers instead of 9 entries of the full form; the
+// pressure-velocity, velocity-pressure and pressure-pressure blocks take one thread per node pair as well (3, 3
+// and 1 numbers), and 34 further threads the load.  All points of the cell are staged in LDS first (one
+// barrier), the entries go to the CSC values through nzpos, and the element matrix is kept in LDS for the
+// residual rows K x_e - l.
+constexpr int kVV = 100, kVP = 40, kPV = 40, kPP = 16, kPairs = kVV + kVP + kPV + kPP;  // 196 pair threads
 
-// One work-group: kCellsPerGroup cells of one colour, two threads per element-matrix row (test function), each
-// for 17 of its 34 entries.  Per quadrature point the cell's 14 shape functions (value, physical gradient,
-// w . gradient) and the point scalars are staged in LDS; each thread accumulates its 17 entries (and the first of
-// the two the row's load).  Then the entries are scattered into the CSC values through nzpos (free trial dofs),
-// and/or the two halves of K x_e - l are added in LDS and written to the residual (free test dof).  The cells of
-// one colour share no free dof, so the plain read-modify-writes do not race and the sums follow the colour order.
 template <bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, int64_t ncell_color,
-                                               const double* __restrict__ x, double* __restrict__ b,
-                                               double* __restrict__ nzval) {
-  __shared__ double S[kCellsPerGroup][14][5];  // per cell: N, grad N (3), w . grad N
-  __shared__ double P[kCellsPerGroup][12];     // per cell and point: dV, ts, tp, tl, c, w(3), f(3), spare
-  __shared__ double Jit[kCellsPerGroup][9];    // inverse transposed Jacobian (physical gradient = Jit * ref)
-  __shared__ double Cm[kCellsPerGroup][2];     // |det J|, h
-  __shared__ double Xe[kCellsPerGroup][kND];
-  __shared__ double Rp[kCellsPerGroup][2][kND];  // the two halves of the residual rows
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, const double* __restrict__ x,
+                                               double* __restrict__ b, double* __restrict__ nzval) {
+  __shared__ double Sh[kMaxPoints][14][5];  // per point and shape function: N, grad N (3), w . grad N
+  __shared__ double Pq[kMaxPoints][7];      // per point: dV, ts, tp, tl, f (3)
+  __shared__ double Ke[kND][kND + 1];       // element matrix [test][trial]
+  __shared__ double Xe[kND], Ld[kND];
+  __shared__ double Jit[9], Cm[2];
+  __shared__ int32_t Id[kND];
   const int tid = threadIdx.x;
-  const int lc = tid / (2 * kND), rem = tid - lc * 2 * kND;
-  const int half = rem / kND, row = rem - half * kND;
-  const int64_t k = (int64_t)blockIdx.x * kCellsPerGroup + lc;
-  const bool active = lc < kCellsPerGroup && k < ncell_color;
-  const int64_t cell = active ? cells[k] : 0;
-  if (active && rem == 0) {
+  const int64_t cell = cells[blockIdx.x];
+  const int np = op.np;
+  if (tid == 0) {
     const double* X = op.coords + cell * 12;
     double J[9];  // column d: x_{d+1} - x_0
 #pragma unroll
@@ -148,7 +115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[3] * (J[1] * J[8] - J[7] * J[2]) +
                        J[6] * (J[1] * J[5] - J[4] * J[2]);
     const double id = 1.0 / det;
-    // inverse (column-major), then its transpose
+    // inverse (column-major), stored transposed
     double I[9];
     I[0] = (J[4] * J[8] - J[7] * J[5]) * id;
     I[3] = -(J[3] * J[8] - J[6] * J[5]) * id;
@@ -162,95 +129,134 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) Jit[lc][i + 3 * j] = I[j + 3 * i];
-    const double vol = fabs(det) / 6.0;
-    Cm[lc][0] = fabs(det);
-    Cm[lc][1] = cbrt(6.0 * sqrt(2.0) * vol);
+      for (int j = 0; j < 3; ++j) Jit[i + 3 * j] = I[j + 3 * i];
+    Cm[0] = fabs(det);
+    Cm[1] = cbrt(6.0 * sqrt(2.0) * fabs(det) / 6.0);
   }
-  int32_t myid = 0;
-  if (active) {
-    myid = op.ids[cell * kND + row];
-    if (half == 0) Xe[lc][row] = myid > 0 ? (x ? x[myid - 1] : 0.0) : op.dvals[-myid - 1];
+  if (tid < kND) {
+    const int32_t id = op.ids[cell * kND + tid];
+    Id[tid] = id;
+    Xe[tid] = id > 0 ? (x ? x[id - 1] : 0.0) : op.dvals[-id - 1];
   }
-  // the row's test function: velocity component ic of node a, or pressure node a
-  const bool vel = row < 30;
-  const int ic = vel ? row / 10 : 0, a = vel ? row % 10 : 10 + (row - 30);
-  double acc[17];
-#pragma unroll
-  for (int j = 0; j < 17; ++j) acc[j] = 0.0;
-  double load = 0.0;
-  for (int q = 0; q < op.np; ++q) {
-    __syncthreads();
-    if (active && rem < 14) {  // shape function `rem` of this cell at point q
+  __syncthreads();
+  // stage every (point, shape function) and the point scalars
+  for (int it = tid; it < np * 15; it += blockDim.x) {
+    const int q = it / 15, f = it - q * 15;
+    const double* fq = op.fields + ((size_t)cell * np + q) * 9;
+    if (f < 14) {
       double n, g[3];
-      shape(rem, op.qp[4 * q], op.qp[4 * q + 1], op.qp[4 * q + 2], n, g);
-      const double* Ji = Jit[lc];
-      const double* fq = op.fields + ((size_t)cell * op.np + q) * 9;
+      shape(f, op.qp[4 * q], op.qp[4 * q + 1], op.qp[4 * q + 2], n, g);
       double gx[3], wg = 0.0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        gx[i] = Ji[i] * g[0] + Ji[i + 3] * g[1] + Ji[i + 6] * g[2];
+        gx[i] = Jit[i] * g[0] + Jit[i + 3] * g[1] + Jit[i + 6] * g[2];
         wg += fq[i] * gx[i];
       }
-      S[lc][rem][0] = n;
-      S[lc][rem][1] = gx[0];
-      S[lc][rem][2] = gx[1];
-      S[lc][rem][3] = gx[2];
-      S[lc][rem][4] = wg;
-    }
-    if (active && rem == 14) {  // point scalars: dV, stabilisation parameters, w, f
-      const double* fq = op.fields + ((size_t)cell * op.np + q) * 9;
+      Sh[q][f][0] = n;
+      Sh[q][f][1] = gx[0];
+      Sh[q][f][2] = gx[1];
+      Sh[q][f][3] = gx[2];
+      Sh[q][f][4] = wg;
+    } else {
       const double un = sqrt(fq[3] * fq[3] + fq[4] * fq[4] + fq[5] * fq[5]);
-      const double h = Cm[lc][1];
+      const double h = Cm[1];
       const double r1 = 2.0 * un / h, r2 = 3.0 * (4.0 * op.mu / (op.rho * h * h));
       const double ts = 1.0 / sqrt(r1 * r1 + r2 * r2);
-      P[lc][0] = Cm[lc][0] * op.qp[4 * q + 3];
-      P[lc][1] = ts;
-      P[lc][2] = ts / op.rho;
-      P[lc][3] = op.rho * un * h * 0.5;
-      P[lc][4] = op.c0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        P[lc][5 + i] = fq[i];
-        P[lc][8 + i] = fq[6 + i];
-      }
+      Pq[q][0] = Cm[0] * op.qp[4 * q + 3];
+      Pq[q][1] = ts;
+      Pq[q][2] = ts / op.rho;
+      Pq[q][3] = op.rho * un * h * 0.5;
+      Pq[q][4] = fq[6];
+      Pq[q][5] = fq[7];
+      Pq[q][6] = fq[8];
     }
-    __syncthreads();
-    if (!active) continue;
-    const double dV = P[lc][0], ts = P[lc][1], tp = P[lc][2], tl = P[lc][3], c = P[lc][4];
-    if (half == 0) {
-      accumulate<0>(S[lc], vel, ic, a, dV, ts, tp, tl, c, op.rho, op.mu, acc);
-      const double Na = S[lc][a][0], wga = S[lc][a][4];
-      const double f[3] = {P[lc][8], P[lc][9], P[lc][10]};
-      if (vel) {
-        const double fi = ic == 0 ? f[0] : (ic == 1 ? f[1] : f[2]);
-        load += dV * (op.rho * fi * Na + ts * wga * op.rho * fi);
-      } else {
-        load += dV * (-tp * op.rho * (S[lc][a][1] * f[0] + S[lc][a][2] * f[1] + S[lc][a][3] * f[2]));
+  }
+  __syncthreads();
+  const double rho = op.rho, mu = op.mu, c = op.c0;
+  if (tid < kVV) {  // velocity-velocity, nodes (a, b)
+    const int a = tid / 10, bn = tid - a * 10;
+    double S = 0.0, P[9], Q[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) P[e] = Q[e] = 0.0;
+    for (int q = 0; q < np; ++q) {
+      const double dV = Pq[q][0], ts = Pq[q][1], tl = Pq[q][3];
+      const double Na = Sh[q][a][0], ga[3] = {Sh[q][a][1], Sh[q][a][2], Sh[q][a][3]}, wga = Sh[q][a][4];
+      const double Nb = Sh[q][bn][0], gb[3] = {Sh[q][bn][1], Sh[q][bn][2], Sh[q][bn][3]}, wgb = Sh[q][bn][4];
+      S += dV * (rho * c * Na * Nb + mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]) + rho * Na * wgb +
+                 ts * rho * wga * (c * Nb + wgb));
+      const double wm = dV * mu, wl = dV * tl;
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          P[u * 3 + v] += wm * ga[u] * gb[v];
+          Q[u * 3 + v] += wl * ga[u] * gb[v];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Ke[i * 10 + a][j * 10 + bn] = (i == j ? S : 0.0) + P[j * 3 + i] + Q[i * 3 + j];
+  } else if (tid < kVV + kVP) {  // velocity test node a, pressure trial vertex bp
+    const int t = tid - kVV, a = t / 4, bp = 10 + (t - (t / 4) * 4);
+    double V[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < np; ++q) {
+      const double dV = Pq[q][0], ts = Pq[q][1];
+      const double wga = Sh[q][a][4], Mb = Sh[q][bp][0];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) V[i] += dV * (-Mb * Sh[q][a][1 + i] + ts * wga * Sh[q][bp][1 + i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Ke[i * 10 + a][20 + bp] = V[i];
+  } else if (tid < kVV + kVP + kPV) {  // pressure test vertex ap, velocity trial node b
+    const int t = tid - kVV - kVP, ap = 10 + t / 10, bn = t - (t / 10) * 10;
+    double W[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < np; ++q) {
+      const double dV = Pq[q][0], tp = Pq[q][2];
+      const double Ma = Sh[q][ap][0], Nb = Sh[q][bn][0], wgb = Sh[q][bn][4];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) W[j] += dV * (-Ma * Sh[q][bn][1 + j] - tp * rho * Sh[q][ap][1 + j] * (c * Nb + wgb));
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Ke[20 + ap][j * 10 + bn] = W[j];
+  } else if (tid < kPairs) {  // pressure-pressure
+    const int t = tid - kVV - kVP - kPV, ap = 10 + t / 4, bp = 10 + (t - (t / 4) * 4);
+    double Z = 0.0;
+    for (int q = 0; q < np; ++q)
+      Z -= Pq[q][0] * Pq[q][2] *
+           (Sh[q][ap][1] * Sh[q][bp][1] + Sh[q][ap][2] * Sh[q][bp][2] + Sh[q][ap][3] * Sh[q][bp][3]);
+    Ke[20 + ap][20 + bp] = Z;
+  } else if (tid < kPairs + kND) {  // load of row r
+    const int r = tid - kPairs;
+    double L = 0.0;
+    if (r < 30) {
+      const int i = r / 10, a = r - i * 10;
+      for (int q = 0; q < np; ++q) {
+        const double dV = Pq[q][0], ts = Pq[q][1], fi = Pq[q][4 + i];
+        L += dV * (rho * fi * Sh[q][a][0] + ts * Sh[q][a][4] * rho * fi);
       }
     } else {
-      accumulate<17>(S[lc], vel, ic, a, dV, ts, tp, tl, c, op.rho, op.mu, acc);
+      const int ap = 10 + (r - 30);
+      for (int q = 0; q < np; ++q)
+        L -= Pq[q][0] * Pq[q][2] * rho *
+             (Sh[q][ap][1] * Pq[q][4] + Sh[q][ap][2] * Pq[q][5] + Sh[q][ap][3] * Pq[q][6]);
+    }
+    Ld[r] = L;
+  }
+  __syncthreads();
+  if (WANT_K) {  // the free (test, trial) entries into the CSC values, column by column
+    const int32_t* pos = op.nzpos + cell * kND * kND;
+    for (int e = tid; e < kND * kND; e += blockDim.x) {
+      const int jj = e / kND, ii = e - jj * kND;
+      const int32_t p = pos[ii + kND * jj];
+      if (p > 0) nzval[p - 1] += Ke[ii][jj];
     }
   }
-  const bool assemble = active && myid > 0;  // Dirichlet test rows are not assembled
-  const int j0 = half * 17;
-  if (WANT_K && assemble) {
-    const int32_t* pos = op.nzpos + cell * kND * kND + row;
+  if (WANT_R && tid < kND && Id[tid] > 0) {
+    double r = -Ld[tid];
 #pragma unroll
-    for (int t = 0; t < 17; ++t) {
-      const int32_t p = pos[kND * (j0 + t)];
-      if (p > 0) nzval[p - 1] += acc[t];
-    }
-  }
-  if (WANT_R) {
-    if (active) {
-      double r = half == 0 ? -load : 0.0;
-#pragma unroll
-      for (int t = 0; t < 17; ++t) r += acc[t] * Xe[lc][j0 + t];
-      Rp[lc][half][row] = r;
-    }
-    __syncthreads();
-    if (assemble && half == 0) b[myid - 1] += Rp[lc][0][row] + Rp[lc][1][row];
+    for (int jj = 0; jj < kND; ++jj) r += Ke[tid][jj] * Xe[jj];
+    b[Id[tid] - 1] += r;
   }
 }
 
@@ -315,10 +321,10 @@ int oseen_run(hfk_oseen* op, const double* x, double* b, double* nzval, hipStrea
     const int64_t n = op->color_start[c + 1] - op->color_start[c];
     if (n <= 0) continue;
     const int32_t* cells = op->cells_d.as<int32_t>() + op->color_start[c];
-    const dim3 g((unsigned)((n + kCellsPerGroup - 1) / kCellsPerGroup)), t(256);
-    if (b && nzval) hipLaunchKernelGGL((k_oseen<true, true>), g, t, 0, s, d, cells, n, x, b, nzval);
-    else if (b) hipLaunchKernelGGL((k_oseen<false, true>), g, t, 0, s, d, cells, n, x, b, nzval);
-    else hipLaunchKernelGGL((k_oseen<true, false>), g, t, 0, s, d, cells, n, x, b, nzval);
+    const dim3 g((unsigned)n), t(256);  // one work-group per cell of the colour
+    if (b && nzval) hipLaunchKernelGGL((k_oseen<true, true>), g, t, 0, s, d, cells, x, b, nzval);
+    else if (b) hipLaunchKernelGGL((k_oseen<false, true>), g, t, 0, s, d, cells, x, b, nzval);
+    else hipLaunchKernelGGL((k_oseen<true, false>), g, t, 0, s, d, cells, x, b, nzval);
     HIPCHK(op->ctx, hipGetLastError());
   }
   return HFK_OK;
@@ -334,7 +340,7 @@ int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, con
   if (!ctx || !out || !cell_dofs || !coords || !qpoints || !qweights)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: null argument");
   if (ncells <= 0 || nfree <= 0 || ndir < 0 || np < 1 || np > kMaxPoints)
-    return set_err(ctx, HFK_EARG, "HyperFEMKernels: invalid Oseen sizes (cells, dofs, 1..64 points)");
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: invalid Oseen sizes (cells, dofs, 1..32 points)");
   if (!(rho > 0) || !(mu > 0) || !(c0 >= 0)) return set_err(ctx, HFK_EARG, "HyperFEMKernels: invalid rho, mu or c0");
   for (int64_t i = 0; i < ncells * kND; ++i) {
     const int32_t id = cell_dofs[i];
