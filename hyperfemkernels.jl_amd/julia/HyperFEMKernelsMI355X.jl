@@ -200,4 +200,54 @@ set_coarse_network!(g::DevicePMultigrid, net::Union{Nothing,CoarseNetwork}, mode
                 g.ptr, net === nothing ? C_NULL : net.ptr, COARSE_MODES[mode], smoother_degree, scaled ? 1 : 0),
           g.levels[1].ctx.ptr)
 
+# --- Oseen-BDF2 step on tetrahedra (P2-P1, SUPG/PSPG/LSIC; test/common.jl:165-228) ---------------------------------
+"KernelOperator of the Oseen step on the device: cell_dofs 34 x ncells (Gridap's MultiFieldFESpace ids, velocity
+components then pressure), coords 3 x 4 x ncells, quadrature points 3 x np and weights np of the unit tetrahedron"
+mutable struct DeviceOseen
+    ptr::Ptr{Cvoid}
+    ctx::Context
+    nfree::Int
+    pattern::Union{Nothing,Tuple{Vector{Int},Vector{Int},Matrix{Int32}}}
+end
+function DeviceOseen(ctx::Context, cell_dofs::Matrix{Int32}, coords::Array{Float64,3}, qpoints::Matrix{Float64},
+                     qweights::Vector{Float64}, nfree::Integer, ndir::Integer; rho=1.0, mu=0.01, c0=150.0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hfk_oseen_create, libhfk), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Cdouble,
+                 Cdouble, Cdouble, Ptr{Ptr{Cvoid}}),
+                ctx.ptr, size(cell_dofs, 2), cell_dofs, coords, length(qweights), qpoints, qweights, nfree, ndir, rho,
+                mu, c0, r), ctx.ptr)
+    op = DeviceOseen(r[], ctx, nfree, nothing)
+    finalizer(o -> ccall((:hfk_oseen_destroy, libhfk), Cint, (Ptr{Cvoid},), o.ptr), op)
+    op
+end
+"point data 9 x np x ncells: advecting velocity w, stabilisation velocity un, history f"
+set_fields!(op::DeviceOseen, fields::Array{Float64,3}) =
+    check(ccall((:hfk_oseen_set_fields, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, fields), op.ctx.ptr)
+set_dirichlet!(op::DeviceOseen, dvals::Vector{Float64}) =
+    check(ccall((:hfk_oseen_set_dirichlet, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, dvals), op.ctx.ptr)
+function residual!(b::Vector{Float64}, op::DeviceOseen, x::Vector{Float64})
+    check(ccall((:hfk_oseen_residual, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op.ptr, x, b), op.ctx.ptr)
+    b
+end
+function sparsity_pattern(op::DeviceOseen)
+    if op.pattern === nothing
+        nnz = Ref{Int64}(0)
+        check(ccall((:hfk_oseen_pattern_nnz, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int64}), op.ptr, nnz), op.ctx.ptr)
+        info = zeros(Int64, 6)
+        check(ccall((:hfk_oseen_info, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int64}), op.ptr, info), op.ctx.ptr)
+        colptr = zeros(Int64, op.nfree + 1); rowval = zeros(Int64, nnz[]); nzpos = zeros(Int32, 34 * 34, info[3])
+        check(ccall((:hfk_oseen_pattern, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}),
+                    op.ptr, colptr, rowval, nzpos), op.ctx.ptr)
+        op.pattern = (colptr, rowval, nzpos)
+    end
+    colptr, rowval, _ = op.pattern
+    SparseMatrixCSC(op.nfree, op.nfree, colptr, rowval, zeros(length(rowval)))
+end
+function jacobian!(A::SparseMatrixCSC{Float64,Int}, op::DeviceOseen, x::Vector{Float64})
+    check(ccall((:hfk_oseen_jacobian, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op.ptr, x, nonzeros(A)),
+          op.ctx.ptr)
+    A
+end
+
 end # module
