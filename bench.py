@@ -24,6 +24,9 @@ for p in (ROOT, os.path.join(ROOT, "hyperfemkernels.jl_amd")):
 
 METRIC = "Q2 hyperelastic Jv GDoF/s + solve time to 1e-8: learned/FP32-MG vs FP64 p-MG"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per deterministic Jv at 64^3 Q2 full64 (8 colour launches), from the PMC counters FETCH_SIZE x 2.0 and
+# WRITE_SIZE x 1.0 (the factors a 1 GiB copy of known size reads): 8 x (324,203 x 2 + 27,479) KiB
+JV_TRAFFIC_64 = 8 * (324203 * 2 + 27479) * 1024
 
 
 def jv_bytes(ncells, nq, ndl, nfree, s_t=8, s_v=8, nent=81):
@@ -598,7 +601,10 @@ def main():
                                f"FP64, tangent storage {args.storage}",
                    "free_dofs": sp.nfree, "cells": ncells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": JV_TRAFFIC_64 if (n == 64 and args.storage == "full64") else None,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this command (64^3, full64), "
+                                       "calibrated x2.0 / x1.0 on the 1 GiB copy; HBM bytes per Jv",
                      "kernel": "one Jv = 1 fill + 8 colour launches of k_apply<3,3>",
                      "algorithmic_bytes": B, "stored_tangent_bytes": info["tangent_bytes"],
                      "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs},
