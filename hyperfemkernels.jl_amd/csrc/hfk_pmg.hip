@@ -922,6 +922,152 @@ int krylov(hfk_op* op, hfk_pmg* g, int kind, int precond, const double* b, doubl
   return HFK_OK;
 }
 
+// Krylov.jl's gmres! / fgmres! as MatrixFreeSolver calls them (src/solvers.jl:128-133): right preconditioning,
+// restarted every `restart` iterations, converged when the residual estimate |g_{j+1}| <= atol + rtol ||b|| (the
+// true residual of right preconditioning).  Arnoldi with classical Gram-Schmidt done twice (two batched
+// projections per iteration), Givens rotations on the host.  GMRES stores the basis V and forms
+// x += M^-1 (V y) at each restart; FGMRES stores Z_j = M^-1 v_j as well (a variable preconditioner, e.g. the
+// p-multigrid with an inexact coarse solve) and forms x += Z y.  When the method stops without its test (iteration
+// limit, breakdown), the decision is taken on the true residual (src/solvers.jl:149-157).
+int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, double* x, double rtol, double atol,
+          int maxiter, int restart, int* status, double* rnorm, hipStream_t s) {
+  const int64_t n = op->nfree;
+  const int m = restart;
+  DevBuf V, Z, t, r, hd, part;
+  Dots dots;
+  int rc;
+  if ((rc = dots.init(op->ctx))) return rc;
+  HIPCHK(op->ctx, V.alloc((size_t)(m + 1) * n * sizeof(double)));
+  if (flexible) HIPCHK(op->ctx, Z.alloc((size_t)m * n * sizeof(double)));
+  if ((rc = vec_alloc(op->ctx, t, n)) || (rc = vec_alloc(op->ctx, r, n))) return rc;
+  HIPCHK(op->ctx, hd.alloc((size_t)(m + 1) * sizeof(double)));
+  HIPCHK(op->ctx, part.alloc(multi_dot_partial_doubles(m + 1) * sizeof(double)));
+  double* Vp = V.as<double>();
+  double* Zp = Z.as<double>();
+  double* tp = t.as<double>();
+  double* rp = r.as<double>();
+  double* hp = hd.as<double>();
+  auto M = [&](const double* rin, double* zout) -> int {
+    if (precond == 0) {
+      HIPCHK(op->ctx, vec_copy(n, rin, zout, s));
+      return HFK_OK;
+    }
+    if (precond == 1) return blocks_apply(op, rin, zout, s);
+    return pmg_apply(g, rin, zout, s);
+  };
+  status[0] = status[1] = status[2] = 0;
+  HIPCHK(op->ctx, launch_fill(x, n, 0.0, s));
+  double h[1];
+  if ((rc = dots.run(op->ctx, s, n, b, b, h))) return rc;
+  const double bn = std::sqrt(h[0]);
+  rnorm[1] = bn;
+  if (bn == 0) {
+    status[0] = 1;
+    rnorm[0] = 0;
+    return HFK_OK;
+  }
+  const double tol = atol + rtol * bn;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), hcol(m + 1);
+  int it = 0;
+  bool solved = false, stop = false;
+  HIPCHK(op->ctx, vec_copy(n, b, rp, s));  // r = b - A x, x = 0
+  double beta = bn;
+  while (!stop && it < maxiter) {
+    HIPCHK(op->ctx, vec_axpby(n, 1.0 / beta, rp, 0.0, Vp, s));  // v_0 = r / beta
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int k = 0;  // columns of this cycle
+    for (int j = 0; j < m && it < maxiter; ++j) {
+      double* vj = Vp + (size_t)j * n;
+      double* w = Vp + (size_t)(j + 1) * n;
+      double* zj = flexible ? Zp + (size_t)j * n : tp;
+      if ((rc = M(vj, zj))) return rc;
+      if ((rc = op_run_apply(op, zj, w, s, 1.0, false))) return rc;
+      // two passes of classical Gram-Schmidt against v_0..v_j
+      std::fill(hcol.begin(), hcol.end(), 0.0);
+      for (int pass = 0; pass < 2; ++pass) {
+        HIPCHK(op->ctx, multi_dot(n, Vp, n, j + 1, w, part.as<double>(), hp, s));
+        HIPCHK(op->ctx, multi_axpy(n, Vp, n, j + 1, hp, -1.0, w, s));
+        std::vector<double> hh(j + 1);
+        HIPCHK(op->ctx, hipMemcpyAsync(hh.data(), hp, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(op->ctx, hipStreamSynchronize(s));
+        for (int i = 0; i <= j; ++i) hcol[i] += hh[i];
+      }
+      double ww;
+      if ((rc = dots.run(op->ctx, s, n, w, w, &ww))) return rc;
+      const double hn = std::sqrt(ww);
+      ++it;
+      if (!std::isfinite(hn)) {
+        stop = true;  // NaN: keep the columns before it, leave the decision to the true residual
+        break;
+      }
+      k = j + 1;
+      // apply the previous rotations, then the new one
+      for (int i = 0; i < j; ++i) {
+        const double a0 = hcol[i], a1 = hcol[i + 1];
+        hcol[i] = cs[i] * a0 + sn[i] * a1;
+        hcol[i + 1] = -sn[i] * a0 + cs[i] * a1;
+      }
+      const double den = std::hypot(hcol[j], hn);
+      cs[j] = den > 0 ? hcol[j] / den : 1.0;
+      sn[j] = den > 0 ? hn / den : 0.0;
+      hcol[j] = den;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i];
+      const double est = std::fabs(gv[j + 1]);
+      if (est <= tol) {
+        solved = true;
+        stop = true;
+        break;
+      }
+      if (!(hn > 1e-14 * den)) {  // breakdown: the Krylov space holds the solution
+        stop = true;
+        break;
+      }
+      HIPCHK(op->ctx, vec_axpby(n, 0.0, w, 1.0 / hn, w, s));  // v_{j+1} = w / hn
+    }
+    if (k == 0) break;
+    // y = R^-1 g (upper triangular k x k), then x += Z y (FGMRES) or x += M^-1 (V y) (GMRES)
+    for (int i = k - 1; i >= 0; --i) {
+      double v = gv[i];
+      for (int l = i + 1; l < k; ++l) v -= H[(size_t)i * m + l] * y[l];
+      y[i] = v / H[(size_t)i * m + i];
+    }
+    HIPCHK(op->ctx, hipMemcpyAsync(hp, y.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+    if (flexible) {
+      HIPCHK(op->ctx, multi_axpy(n, Zp, n, k, hp, 1.0, x, s));
+    } else {
+      HIPCHK(op->ctx, launch_fill(rp, n, 0.0, s));
+      HIPCHK(op->ctx, multi_axpy(n, Vp, n, k, hp, 1.0, rp, s));
+      if ((rc = M(rp, tp))) return rc;
+      HIPCHK(op->ctx, vec_axpby(n, 1.0, tp, 1.0, x, s));
+    }
+    HIPCHK(op->ctx, hipStreamSynchronize(s));  // y is read from host memory by the copy above
+    if (stop) break;
+    // restart from the true residual
+    if ((rc = op_run_apply(op, x, rp, s, 1.0, false))) return rc;
+    HIPCHK(op->ctx, vec_axpby(n, 1.0, b, -1.0, rp, s));
+    double rr;
+    if ((rc = dots.run(op->ctx, s, n, rp, rp, &rr))) return rc;
+    beta = std::sqrt(rr);
+    if (!(beta > tol)) {
+      solved = std::isfinite(beta);
+      break;
+    }
+  }
+  // the true residual
+  if ((rc = op_run_apply(op, x, rp, s, 1.0, false))) return rc;
+  HIPCHK(op->ctx, vec_axpby(n, 1.0, b, -1.0, rp, s));
+  double rt;
+  if ((rc = dots.run(op->ctx, s, n, rp, rp, &rt))) return rc;
+  rnorm[0] = std::sqrt(rt);
+  status[0] = (solved || rnorm[0] <= tol) && std::isfinite(rnorm[0]) ? 1 : 0;
+  status[1] = it;
+  status[2] = 0;
+  return HFK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1358,22 +1504,37 @@ int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev,
 
 // ---- Krylov ---------------------------------------------------------------------------------------------
 int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b_dev, double* x_dev, double rtol,
-                         double atol, int maxiter, int* status, double* rnorm, void* stream) {
+                         double atol, int maxiter, int restart, int* status, double* rnorm, void* stream) {
   int rc = op_bind(op);
   if (rc) return rc;
   if (!b_dev || !x_dev || !status || !rnorm) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
-  if (kind < 0 || kind > 1 || precond < 0 || precond > 2 || maxiter < 0 || rtol < 0 || atol < 0)
+  if (kind < 0 || kind > 3 || precond < 0 || precond > 2 || maxiter < 0 || rtol < 0 || atol < 0)
     return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: MatrixFreeSolver: invalid settings");
+  if (kind >= 2 && (restart < 1 || restart > 1024))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: MatrixFreeSolver: restart must be in 1..1024");
   if (op->storage < 0) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: solve! before prepare!");
   if (precond == 1 && !op->blocks.built) {
     bool ok;
     if ((rc = blocks_update(op, op->tangents.p, op->storage, op->stream(stream), &ok))) return rc;
   }
+  if (kind >= 2) {
+    if (precond == 2 && (!pmg || pmg->ops[0] != op))
+      return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the p-multigrid is not built on this operator");
+    if (precond == 2 && !pmg->built) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid used before its update");
+    if (precond == 2 && !pmg->ok) {  // src/solvers.jl:114-121
+      status[0] = status[1] = 0;
+      status[2] = pmg->indefinite ? 1 : 0;
+      rnorm[0] = rnorm[1] = NAN;
+      return HFK_OK;
+    }
+    return gmres(op, pmg, kind == 3, precond, b_dev, x_dev, rtol, atol, maxiter, restart, status, rnorm,
+                 op->stream(stream));
+  }
   return krylov(op, pmg, kind, precond, b_dev, x_dev, rtol, atol, maxiter, status, rnorm, op->stream(stream));
 }
 
 int hfk_krylov_solve(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b, double* x, double rtol,
-                     double atol, int maxiter, int* status, double* rnorm) {
+                     double atol, int maxiter, int restart, int* status, double* rnorm) {
   int rc = op_bind(op);
   if (rc) return rc;
   if (!b || !x) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
@@ -1382,8 +1543,8 @@ int hfk_krylov_solve(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const doub
   DevBuf bd, xd;
   if ((rc = vec_alloc(op->ctx, bd, n)) || (rc = vec_alloc(op->ctx, xd, n))) return rc;
   HIPCHK(op->ctx, hipMemcpyAsync(bd.p, b, n * sizeof(double), hipMemcpyHostToDevice, s));
-  if ((rc = hfk_krylov_solve_dev(op, pmg, kind, precond, bd.as<double>(), xd.as<double>(), rtol, atol, maxiter, status,
-                                 rnorm, nullptr)))
+  if ((rc = hfk_krylov_solve_dev(op, pmg, kind, precond, bd.as<double>(), xd.as<double>(), rtol, atol, maxiter, restart,
+                                 status, rnorm, nullptr)))
     return rc;
   HIPCHK(op->ctx, hipMemcpyAsync(x, xd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(op->ctx, hipStreamSynchronize(s));

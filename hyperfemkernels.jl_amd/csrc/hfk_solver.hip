@@ -59,6 +59,49 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restr
   }
 }
 
+// Gram-Schmidt helpers of GMRES: partial sums of the K dot products V[k] . w (k = 8 tile + t), tile by tile;
+// part[k][kRedBlocks]
+__global__ __launch_bounds__(kRedThreads) void k_multi_dot_partial(int64_t n, const double* __restrict__ V, int64_t ldv,
+                                                                   int K, const double* __restrict__ w,
+                                                                   double* __restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  const int k0 = 8 * blockIdx.y;
+  double s[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) s[t] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads) {
+    const double wi = w[i];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (k0 + t < K) s[t] += V[(size_t)(k0 + t) * ldv + i] * wi;
+  }
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    if (k0 + t >= K) break;  // uniform over the block
+    const double v = block_sum(s[t], sh);
+    if (threadIdx.x == 0) part[(size_t)(k0 + t) * kRedBlocks + blockIdx.x] = v;
+  }
+}
+
+// out[k] = sum_b part[k][b] (fixed tree order), one block per dot product
+__global__ __launch_bounds__(kRedThreads) void k_multi_dot_final(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  const int k = blockIdx.x;
+  double v = part[(size_t)k * kRedBlocks + threadIdx.x] + part[(size_t)k * kRedBlocks + threadIdx.x + kRedThreads];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) out[k] = v;
+}
+
+// w += alpha sum_k coef[k] V[k]
+__global__ void k_multi_axpy(int64_t n, const double* __restrict__ V, int64_t ldv, int K, const double* __restrict__ coef,
+                             double alpha, double* __restrict__ w) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s += coef[k] * V[(size_t)k * ldv + i];
+    w[i] += alpha * s;
+  }
+}
+
 __global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = a * x[i] + b * y[i];
@@ -343,6 +386,18 @@ hipError_t vec_dots(int64_t n, const double* a0, const double* b0, const double*
   return hipGetLastError();
 }
 size_t dot_partial_doubles() { return 3 * kRedBlocks; }
+size_t multi_dot_partial_doubles(int K) { return (size_t)K * kRedBlocks; }
+hipError_t multi_dot(int64_t n, const double* V, int64_t ldv, int K, const double* w, double* part, double* out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_multi_dot_partial, dim3(kRedBlocks, (K + 7) / 8), dim3(kRedThreads), 0, s, n, V, ldv, K, w, part);
+  hipLaunchKernelGGL(k_multi_dot_final, dim3(K), dim3(kRedThreads), 0, s, part, out);
+  return hipGetLastError();
+}
+hipError_t multi_axpy(int64_t n, const double* V, int64_t ldv, int K, const double* coef, double alpha, double* w,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_multi_axpy, grid_for(n), dim3(256), 0, s, n, V, ldv, K, coef, alpha, w);
+  return hipGetLastError();
+}
 
 hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(256), 0, s, n, a, x, b, y);

@@ -17,6 +17,13 @@ hipError_t vec_dots(int64_t n, const double* a0, const double* b0, const double*
 size_t dot_partial_doubles();
 hipError_t vec_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);  // y = a x + b y
 hipError_t vec_copy(int64_t n, const double* x, double* y, hipStream_t s);
+// GMRES: out[k] = V[k] . w for k < K (V: K vectors at stride ldv; part: multi_dot_partial_doubles(K)), and
+// w += alpha sum_k coef[k] V[k] (coef on the device)
+size_t multi_dot_partial_doubles(int K);
+hipError_t multi_dot(int64_t n, const double* V, int64_t ldv, int K, const double* w, double* part, double* out,
+                     hipStream_t s);
+hipError_t multi_axpy(int64_t n, const double* V, int64_t ldv, int K, const double* coef, double alpha, double* w,
+                      hipStream_t s);
 // node-block ELL product y = alpha A x + beta y (val FP64, or FP32 when fp32) and its value fill from the CSC
 hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,
                      const double* x, double* y, double alpha, double beta, hipStream_t s);

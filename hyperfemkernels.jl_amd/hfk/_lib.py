@@ -100,10 +100,10 @@ SIGNATURES = {
     "hfk_oseen_jacobian": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_oseen_jacobian_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_oseen_residual_jacobian_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
-    "hfk_krylov_solve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _f64p, _f64p, C.c_double, C.c_double, C.c_int, _i32p,
-                                   _f64p]),
-    "hfk_krylov_solve_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_int, _i32p,
-                                       _f64p, _vp]),
+    "hfk_krylov_solve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _f64p, _f64p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   _i32p, _f64p]),
+    "hfk_krylov_solve_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       _i32p, _f64p, _vp]),
 }
 
 _lib = None

@@ -226,17 +226,24 @@ class SolveResult:
     rhs_norm: float
 
 
-class MatrixFreeSolver:
-    """Krylov solve on a prepared MatrixFreeJacobian (src/solvers.jl:33-174): ``krylov`` "cg" or "fcg" (flexible
-    CG); ``preconditioner`` None, "blockjacobi" or a PMultigrid."""
+KRYLOV = {"cg": 0, "fcg": 1, "gmres": 2, "fgmres": 3}
 
-    def __init__(self, krylov: str = "cg", rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 1000,
-                 preconditioner=None):
-        if krylov not in ("cg", "fcg"):
-            raise ValueError("HyperFEMKernels: MatrixFreeSolver: krylov must be 'cg' or 'fcg'")
-        if rtol < 0 or atol < 0 or maxiter < 0:
-            raise ValueError("HyperFEMKernels: MatrixFreeSolver: tolerances and maxiter must be >= 0")
+
+class MatrixFreeSolver:
+    """Krylov solve on a prepared MatrixFreeJacobian (src/solvers.jl:33-174): ``krylov`` "gmres" (the reference's
+    default), "fgmres", "cg" or "fcg" (flexible CG); GMRES and FGMRES restart every ``restart`` iterations;
+    ``preconditioner`` None, "blockjacobi" or a PMultigrid."""
+
+    def __init__(self, krylov: str = "gmres", rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 500,
+                 preconditioner=None, restart: int = 50):
+        if krylov not in KRYLOV:
+            raise ValueError(f"HyperFEMKernels: MatrixFreeSolver: krylov must be one of {sorted(KRYLOV)} (got {krylov!r})")
+        if not (0 <= rtol < 1) or atol < 0:
+            raise ValueError("HyperFEMKernels: MatrixFreeSolver: rtol must be in [0, 1) and atol >= 0")
+        if maxiter <= 0 or restart <= 0:
+            raise ValueError("HyperFEMKernels: MatrixFreeSolver: maxiter and restart must be positive")
         self.krylov, self.rtol, self.atol, self.maxiter, self.preconditioner = krylov, rtol, atol, maxiter, preconditioner
+        self.restart = restart
         self.last = None
 
     def solve(self, x, mf: MatrixFreeJacobian, b) -> SolveResult:
@@ -246,8 +253,8 @@ class MatrixFreeSolver:
         prec, g = (0, None) if pc is None else ((1, None) if pc == "blockjacobi" else (2, pc.handle))
         st = np.zeros(3, dtype=np.int32)
         rn = np.zeros(2)
-        check(op.lib.hfk_krylov_solve(op.handle, g, 0 if self.krylov == "cg" else 1, prec, ptr(b, C.c_double),
-                                      ptr(x, C.c_double), self.rtol, self.atol, self.maxiter, ptr(st, C.c_int32),
+        check(op.lib.hfk_krylov_solve(op.handle, g, KRYLOV[self.krylov], prec, ptr(b, C.c_double), ptr(x, C.c_double),
+                                      self.rtol, self.atol, self.maxiter, self.restart, ptr(st, C.c_int32),
                                       ptr(rn, C.c_double)), op.ctx.handle)
         self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
         return self.last
@@ -259,8 +266,8 @@ class MatrixFreeSolver:
         prec, g = (0, None) if pc is None else ((1, None) if pc == "blockjacobi" else (2, pc.handle))
         st = np.zeros(3, dtype=np.int32)
         rn = np.zeros(2)
-        check(op.lib.hfk_krylov_solve_dev(op.handle, g, 0 if self.krylov == "cg" else 1, prec, C.c_void_p(b_ptr),
-                                          C.c_void_p(x_ptr), self.rtol, self.atol, self.maxiter, ptr(st, C.c_int32),
+        check(op.lib.hfk_krylov_solve_dev(op.handle, g, KRYLOV[self.krylov], prec, C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                          self.rtol, self.atol, self.maxiter, self.restart, ptr(st, C.c_int32),
                                           ptr(rn, C.c_double), C.c_void_p(stream)), op.ctx.handle)
         self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
         return self.last

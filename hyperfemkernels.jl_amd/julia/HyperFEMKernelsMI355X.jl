@@ -138,16 +138,19 @@ function update!(g::DevicePMultigrid)
 end
 
 "MatrixFreeSolver solve! (src/solvers.jl:111-158): returns (converged, iterations, indefinite, residual norm)"
+const KRYLOV = Dict(:cg => 0, :fcg => 1, :gmres => 2, :fgmres => 3)
 function solve!(x::Vector{Float64}, mf::DeviceMatrixFree, b::Vector{Float64}; pmg::Union{Nothing,DevicePMultigrid}=nothing,
-                krylov::Symbol=:fcg, rtol=1e-8, atol=0.0, maxiter=1000)
+                krylov::Symbol=:gmres, rtol=1e-8, atol=0.0, maxiter=500, restart=50)
     st = zeros(Cint, 3)
     rn = zeros(2)
     prec = pmg === nothing ? Cint(1) : Cint(2)
     pmg === nothing && check(ccall((:hfk_block_jacobi_update, libhfk), Cint, (Ptr{Cvoid}, Ptr{Cint}), mf.op.ptr, st),
                              mf.op.ctx.ptr)
     check(ccall((:hfk_krylov_solve, libhfk), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cint, Ptr{Cint}, Ptr{Float64}),
-                mf.op.ptr, pmg === nothing ? C_NULL : pmg.ptr, krylov === :cg ? 0 : 1, prec, b, x, rtol, atol, maxiter, st, rn),
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cint, Cint, Ptr{Cint},
+                 Ptr{Float64}),
+                mf.op.ptr, pmg === nothing ? C_NULL : pmg.ptr, KRYLOV[krylov], prec, b, x, rtol, atol, maxiter, restart,
+                st, rn),
           mf.op.ctx.ptr)
     (st[1] == 1, Int(st[2]), st[3] == 1, rn[1])
 end

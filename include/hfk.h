@@ -217,14 +217,17 @@ int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev,
                                  void* stream);
 
 /* ---- MatrixFreeSolver solve! (src/solvers.jl:111-158) ------------------------------------------------- */
-/* kind 0 = CG, 1 = flexible CG; precond 0 = none, 1 = op's BlockJacobi (updated by hfk_block_jacobi_update),
- * 2 = one V-cycle of pmg.  From x = 0 to ||b - A x||_2 <= atol + rtol ||b||_2.  status = (converged,
- * iterations, indefinite); rnorm = (true residual norm, ||b||).  A failed preconditioner (pmg not ok) returns at
- * once with converged = 0. */
+/* kind 0 = CG (left preconditioned, stops on a non-positive curvature as cg! with linesearch), 1 = flexible CG,
+ * 2 = GMRES, 3 = FGMRES (right preconditioned, restarted every `restart` iterations, as gmres!/fgmres! with
+ * restart = true; the reference's default is GMRES with restart 50); precond 0 = none, 1 = op's BlockJacobi
+ * (updated by hfk_block_jacobi_update), 2 = one V-cycle of pmg.  From x = 0 to ||b - A x||_2 <= atol +
+ * rtol ||b||_2.  status = (converged, iterations, indefinite); rnorm = (true residual norm, ||b||).  A failed
+ * preconditioner (pmg not ok) returns at once with converged = 0; a stop without the method's own test is decided
+ * on the true residual. */
 int hfk_krylov_solve(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b, double* x, double rtol,
-                     double atol, int maxiter, int* status, double* rnorm);
+                     double atol, int maxiter, int restart, int* status, double* rnorm);
 int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const double* b_dev, double* x_dev,
-                         double rtol, double atol, int maxiter, int* status, double* rnorm, void* stream);
+                         double rtol, double atol, int maxiter, int restart, int* status, double* rnorm, void* stream);
 
 /* ------------------------------------------------------------------------------------------------------------
  * Oseen-BDF2 step on tetrahedra, Taylor-Hood P2-P1 with SUPG/PSPG/LSIC (the weak form of test/common.jl:165-228,

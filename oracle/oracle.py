@@ -318,6 +318,64 @@ def chebyshev_smooth(A, Minv, x, rhs, a, b, degree, zero_guess=False, want_resid
     return (x, r) if want_residual else x
 
 
+def gmres_right(A, b, Minv, rtol, restart, maxiter, atol=0.0, flexible=False):
+    """Restarted right-preconditioned GMRES (Krylov.jl's gmres!/fgmres! with restart = true, as MatrixFreeSolver
+    calls them, src/solvers.jl:128-133): Arnoldi on A M^-1 from r0 = b (x0 = 0) with modified Gram-Schmidt, Givens
+    rotations, converged when the residual estimate |g_{j+1}| <= atol + rtol ||b||; x += M^-1 V y (or Z y).
+    Returns (x, iterations, converged)."""
+    n = b.size
+    bn = np.linalg.norm(b)
+    tol = atol + rtol * bn
+    x = np.zeros(n)
+    r = b.copy()
+    it = 0
+    beta = bn
+    if bn == 0:
+        return x, 0, True
+    while it < maxiter:
+        V = np.zeros((restart + 1, n))
+        Z = np.zeros((restart, n))
+        H = np.zeros((restart + 1, restart))
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        V[0] = r / beta
+        k = 0
+        done = False
+        for j in range(restart):
+            if it >= maxiter:
+                break
+            Z[j] = Minv(V[j])
+            w = A @ Z[j]
+            for i in range(j + 1):
+                H[i, j] = V[i] @ w
+                w = w - H[i, j] * V[i]
+            H[j + 1, j] = np.linalg.norm(w)
+            it += 1
+            k = j + 1
+            for i in range(j):
+                a0, a1 = H[i, j], H[i + 1, j]
+                H[i, j], H[i + 1, j] = cs[i] * a0 + sn[i] * a1, -sn[i] * a0 + cs[i] * a1
+            den = np.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = H[j, j] / den, H[j + 1, j] / den
+            H[j, j] = den
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            if abs(g[j + 1]) <= tol or H[j + 1, j] == 0:
+                done = True
+                break
+            V[j + 1] = w / np.linalg.norm(w)
+        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+        x = x + (Z[:k].T @ y if flexible else Minv(V[:k].T @ y))
+        if done:
+            return x, it, True
+        r = b - A @ x
+        beta = np.linalg.norm(r)
+        if beta <= tol:
+            return x, it, True
+    return x, it, np.linalg.norm(b - A @ x) <= tol
+
+
 def lagrange_embedding(Qc, Qf):
     """E[tf, tc]: coarse tensor shape function tc at fine tensor node tf (equispaced nodes on [0,1]^3), from the
     oracle's 1D Lagrange basis."""

@@ -172,6 +172,64 @@ def test_pmultigrid_q3_lognormal(hip_ctx, oracle):
     assert rel(xs, xd) < 1e-7
 
 
+@pytest.mark.parametrize("restart", [50, 10])
+def test_gmres_blockjacobi_matches_oracle(hip_ctx, oracle, restart):
+    """GMRES (the reference's default MatrixFreeSolver method) right-preconditioned by the nodal blocks: the same
+    iteration count as the oracle's restatement on the assembled matrix (+-1 for rounding at the tolerance), the
+    direct solution to 1e-7, restarts included (GMRES(10) needs ~400 iterations here, GMRES(50) ~220)."""
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (3, 3, 3))
+    from hfk.synthetic import smooth_field
+
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    A = orc.csc(x).tocsc()
+    bj = BlockJacobi(mf)
+    dofs, Binv = bj.blocks()
+    Minv = oracle.block_inverse_matrix(dofs, Binv, fs.nfree)
+    b = np.random.default_rng(13).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("gmres", rtol=1e-9, maxiter=1000, restart=restart, preconditioner="blockjacobi").solve(xs, mf, b)
+    xo, ito, oko = oracle.gmres_right(A, b, lambda v: Minv @ v, 1e-9, restart, 1000)
+    print(f"gmres(restart={restart}): device {res}, oracle {ito} iterations", flush=True)
+    assert res.converged and oko
+    assert abs(res.iterations - ito) <= 1
+    xd = spla.spsolve(A, b)
+    assert rel(xs, xd) < 1e-7 and rel(xs, xo) < 1e-7
+    assert res.residual <= 1e-9 * res.rhs_norm * 1.01
+
+
+@pytest.mark.parametrize("krylov", ["fgmres", "gmres"])
+def test_gmres_pmultigrid(hip_ctx, oracle, krylov):
+    """FGMRES with one p-multigrid V-cycle as right preconditioner reaches the direct solution.  Plain GMRES
+    needs a fixed preconditioner: with the coarse level solved to 1e-12 it does too (with an inexact coarse solve
+    the cycle varies, its residual estimate no longer equals the true residual, and the flexible method is the
+    one to use, as src/pmultigrid.jl:237-239 says)."""
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3 if krylov == "fgmres" else 1e-12,
+                     coarse_maxiter=2000).update()
+    b = np.random.default_rng(11).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver(krylov, rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
+    print(f"{krylov} + pmg: {res}", flush=True)
+    assert res.converged and res.iterations <= 30
+    from hfk.synthetic import smooth_field
+
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    assert rel(xs, spla.spsolve(orc.csc(x).tocsc(), b)) < 1e-7
+
+
+def test_gmres_failed_preconditioner_returns_at_once(hip_ctx):
+    """A p-multigrid that is not ok (NaN state) makes GMRES return at once, not converged (src/solvers.jl:114-121)."""
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (3, 3, 3))
+    xb = x.copy()
+    xb[5] = np.nan
+    mf.prepare(xb)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=2)).update()
+    assert not pmg.ok
+    res = MatrixFreeSolver("fgmres", rtol=1e-8, maxiter=50, preconditioner=pmg).solve(np.zeros(fs.nfree), mf,
+                                                                                     np.ones(fs.nfree))
+    assert not res.converged and res.iterations == 0
+
+
 @pytest.mark.parametrize("smoother_storage", ["sym32", "full32", "sym64"])
 def test_pmultigrid_mixed_precision(hip_ctx, oracle, smoother_storage):
     """FP32 smoothers and level applies inside an FP64 flexible CG: the Krylov tolerance fixes the answer."""

@@ -89,3 +89,22 @@ def test_prolongation_exact_on_linear_fields(oracle, qc, qf):
     A = np.array([[0.3, -0.1, 0.2], [0.05, 0.4, 0.0], [0.1, 0.2, -0.3]])
     f = lambda xyz: xyz[:, :1] * (A[:, 0][None, :] + 0.0 * xyz)  # linear, zero on the clamped face x = 0
     assert np.allclose(P @ cs.free_values(f), fs.free_values(f), atol=1e-13)
+
+
+def test_gmres_restatement_solves_and_restarts():
+    """The oracle's restarted right-preconditioned GMRES: on a nonsymmetric system it reaches the tolerance on the
+    true residual, with and without restarts, and with a Jacobi preconditioner; unrestarted GMRES terminates in at
+    most n steps on an n x n system (the Krylov space is exhausted)."""
+    from oracle.oracle import gmres_right
+
+    rng = np.random.default_rng(4)
+    n = 40
+    A = np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3
+    b = rng.standard_normal(n)
+    d = np.diag(A).copy()
+    for restart in (5, 50):
+        for Minv in (lambda v: v, lambda v: v / d):
+            x, it, ok = gmres_right(A, b, Minv, 1e-10, restart, 500)
+            assert ok and np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
+    x, it, ok = gmres_right(A, b, lambda v: v, 1e-14, n, 500)
+    assert ok and it <= n
