@@ -16,6 +16,8 @@
 // Q x Q x Q (or P x P x P) cube of one (cell, component).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 
 #include "hfk_internal.hpp"
@@ -143,25 +145,25 @@ struct LawLinear {  // sigma = lam tr(G) I + mu (G + G'), G = F - I
 // Sum-factorised interpolation of the 3 components of the C cells of a batch.
 // In : A[cell*SA + c*NN + t]              nodal values, tensor order
 // Out: B[cell*SB + q*9 + K + 3c]          d u_c / d xi_K at tensor point q = a + P b + P^2 cc
-template <int Q, int P>
-__device__ __forceinline__ void interp_sweeps(double* __restrict__ A, double* __restrict__ B, const Tab& tb, int tid) {
+template <int Q, int P, class R = double>
+__device__ __forceinline__ void interp_sweeps(R* __restrict__ A, R* __restrict__ B, const Tab& tb, int tid) {
   using Dm = Dims<Q, P>;
   constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB;
   // sweep 1 (x): items (cell, c, jk); value and x-derivative at (a, j, k)
   for (int it = tid; it < C * 3 * Q * Q; it += kThreads) {
     const int cell = it / (3 * Q * Q), r = it - cell * 3 * Q * Q, c = r / (Q * Q), jk = r - c * Q * Q;
-    const double* u = A + cell * SA + c * Dm::NN + jk * Q;
-    double ui[Q];
+    const R* u = A + cell * SA + c * Dm::NN + jk * Q;
+    R ui[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) ui[i] = u[i];
-    double* o = B + cell * SB + c * 2 * P * Q * Q;
+    R* o = B + cell * SB + c * 2 * P * Q * Q;
 #pragma unroll
     for (int a = 0; a < P; ++a) {
-      double s = 0, ds = 0;
+      R s = 0, ds = 0;
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        s += tb.N[a * Q + i] * ui[i];
-        ds += tb.dN[a * Q + i] * ui[i];
+        s += (R)tb.N[a * Q + i] * ui[i];
+        ds += (R)tb.dN[a * Q + i] * ui[i];
       }
       o[a + P * jk] = s;
       o[P * Q * Q + a + P * jk] = ds;
@@ -172,22 +174,22 @@ __device__ __forceinline__ void interp_sweeps(double* __restrict__ A, double* __
   for (int it = tid; it < C * 3 * P * Q; it += kThreads) {
     const int cell = it / (3 * P * Q), r = it - cell * 3 * P * Q, c = r / (P * Q), ak = r - c * P * Q;
     const int a = ak % P, k = ak / P;
-    const double* in = B + cell * SB + c * 2 * P * Q * Q;
-    double av[Q], bv[Q];
+    const R* in = B + cell * SB + c * 2 * P * Q * Q;
+    R av[Q], bv[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       av[j] = in[a + P * (j + Q * k)];
       bv[j] = in[P * Q * Q + a + P * (j + Q * k)];
     }
-    double* o = A + cell * SA + c * 3 * P * P * Q;
+    R* o = A + cell * SA + c * 3 * P * P * Q;
 #pragma unroll
     for (int b = 0; b < P; ++b) {
-      double v = 0, g1 = 0, g2 = 0;
+      R v = 0, g1 = 0, g2 = 0;
 #pragma unroll
       for (int j = 0; j < Q; ++j) {
-        v += tb.N[b * Q + j] * av[j];
-        g1 += tb.N[b * Q + j] * bv[j];
-        g2 += tb.dN[b * Q + j] * av[j];
+        v += (R)tb.N[b * Q + j] * av[j];
+        g1 += (R)tb.N[b * Q + j] * bv[j];
+        g2 += (R)tb.dN[b * Q + j] * av[j];
       }
       const int idx = a + P * b + P * P * k;
       o[idx] = v;
@@ -199,23 +201,23 @@ __device__ __forceinline__ void interp_sweeps(double* __restrict__ A, double* __
   // sweep 3 (z): items (cell, c, ab); the 3 reference derivatives at (a, b, cc)
   for (int it = tid; it < C * 3 * P * P; it += kThreads) {
     const int cell = it / (3 * P * P), r = it - cell * 3 * P * P, c = r / (P * P), ab = r - c * P * P;
-    const double* in = A + cell * SA + c * 3 * P * P * Q;
-    double vv[Q], g1v[Q], g2v[Q];
+    const R* in = A + cell * SA + c * 3 * P * P * Q;
+    R vv[Q], g1v[Q], g2v[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
       vv[k] = in[ab + P * P * k];
       g1v[k] = in[P * P * Q + ab + P * P * k];
       g2v[k] = in[2 * P * P * Q + ab + P * P * k];
     }
-    double* o = B + cell * SB;
+    R* o = B + cell * SB;
 #pragma unroll
     for (int cc = 0; cc < P; ++cc) {
-      double x = 0, y = 0, z = 0;
+      R x = 0, y = 0, z = 0;
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        x += tb.N[cc * Q + k] * g1v[k];
-        y += tb.N[cc * Q + k] * g2v[k];
-        z += tb.dN[cc * Q + k] * vv[k];
+        x += (R)tb.N[cc * Q + k] * g1v[k];
+        y += (R)tb.N[cc * Q + k] * g2v[k];
+        z += (R)tb.dN[cc * Q + k] * vv[k];
       }
       const int q = ab + P * P * cc;
       o[q * 9 + 3 * c + 0] = x;
@@ -229,8 +231,8 @@ __device__ __forceinline__ void interp_sweeps(double* __restrict__ A, double* __
 // Sum-factorised contraction (transpose of the interpolation, gradient coefficients only).
 // In : B[cell*SB + q*9 + K + 3c]   coefficient of d/dxi_K of the test function of component c at point q
 // Out: emit(cell, c, t, r)          r = sum_q sum_K dN_t/dxi_K(q) coef   for every tensor node t
-template <int Q, int P, class Emit>
-__device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* __restrict__ B, const Tab& tb, int tid,
+template <int Q, int P, class R = double, class Emit>
+__device__ __forceinline__ void contract_sweeps(R* __restrict__ A, R* __restrict__ B, const Tab& tb, int tid,
                                                 Emit&& emit) {
   using Dm = Dims<Q, P>;
   constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB;
@@ -238,8 +240,8 @@ __device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* 
   // sweep 3T (z): items (cell, c, ab)
   for (int it = tid; it < C * 3 * P * P; it += kThreads) {
     const int cell = it / (3 * P * P), r = it - cell * 3 * P * P, c = r / (P * P), ab = r - c * P * P;
-    const double* in = B + cell * SB;
-    double cx[P], cy[P], cz[P];
+    const R* in = B + cell * SB;
+    R cx[P], cy[P], cz[P];
 #pragma unroll
     for (int cc = 0; cc < P; ++cc) {
       const int q = ab + P * P * cc;
@@ -247,15 +249,15 @@ __device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* 
       cy[cc] = in[q * 9 + 3 * c + 1];
       cz[cc] = in[q * 9 + 3 * c + 2];
     }
-    double* o = A + cell * SA + c * 3 * P * P * Q;
+    R* o = A + cell * SA + c * 3 * P * P * Q;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      double t0 = 0, t1 = 0, t2 = 0;
+      R t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
       for (int cc = 0; cc < P; ++cc) {
-        t0 += tb.dN[cc * Q + k] * cz[cc];
-        t1 += tb.N[cc * Q + k] * cx[cc];
-        t2 += tb.N[cc * Q + k] * cy[cc];
+        t0 += (R)tb.dN[cc * Q + k] * cz[cc];
+        t1 += (R)tb.N[cc * Q + k] * cx[cc];
+        t2 += (R)tb.N[cc * Q + k] * cy[cc];
       }
       o[ab + P * P * k] = t0;
       o[P * P * Q + ab + P * P * k] = t1;
@@ -267,22 +269,22 @@ __device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* 
   for (int it = tid; it < C * 3 * P * Q; it += kThreads) {
     const int cell = it / (3 * P * Q), r = it - cell * 3 * P * Q, c = r / (P * Q), ak = r - c * P * Q;
     const int a = ak % P, k = ak / P;
-    const double* in = A + cell * SA + c * 3 * P * P * Q;
-    double t0[P], t1[P], t2[P];
+    const R* in = A + cell * SA + c * 3 * P * P * Q;
+    R t0[P], t1[P], t2[P];
 #pragma unroll
     for (int b = 0; b < P; ++b) {
       t0[b] = in[a + P * b + P * P * k];
       t1[b] = in[P * P * Q + a + P * b + P * P * k];
       t2[b] = in[2 * P * P * Q + a + P * b + P * P * k];
     }
-    double* o = B + cell * SB + c * 2 * P * Q * Q;
+    R* o = B + cell * SB + c * 2 * P * Q * Q;
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
-      double s0 = 0, s1 = 0;
+      R s0 = 0, s1 = 0;
 #pragma unroll
       for (int b = 0; b < P; ++b) {
-        s0 += tb.N[b * Q + j] * t0[b] + tb.dN[b * Q + j] * t2[b];
-        s1 += tb.N[b * Q + j] * t1[b];
+        s0 += (R)tb.N[b * Q + j] * t0[b] + (R)tb.dN[b * Q + j] * t2[b];
+        s1 += (R)tb.N[b * Q + j] * t1[b];
       }
       o[a + P * (j + Q * k)] = s0;
       o[P * Q * Q + a + P * (j + Q * k)] = s1;
@@ -292,19 +294,19 @@ __device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* 
   // sweep 1T (x): items (cell, c, jk)
   for (int it = tid; it < C * 3 * Q * Q; it += kThreads) {
     const int cell = it / (3 * Q * Q), r = it - cell * 3 * Q * Q, c = r / (Q * Q), jk = r - c * Q * Q;
-    const double* in = B + cell * SB + c * 2 * P * Q * Q;
-    double s0[P], s1[P];
+    const R* in = B + cell * SB + c * 2 * P * Q * Q;
+    R s0[P], s1[P];
 #pragma unroll
     for (int a = 0; a < P; ++a) {
       s0[a] = in[a + P * jk];
       s1[a] = in[P * Q * Q + a + P * jk];
     }
-    double rl[Q];
+    R rl[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      double s = 0;
+      R s = 0;
 #pragma unroll
-      for (int a = 0; a < P; ++a) s += tb.N[a * Q + i] * s0[a] + tb.dN[a * Q + i] * s1[a];
+      for (int a = 0; a < P; ++a) s += (R)tb.N[a * Q + i] * s0[a] + (R)tb.dN[a * Q + i] * s1[a];
       rl[i] = s;
     }
     emit(cell, c, Q * jk, rl);  // the Q nodes t = Q jk + i of one x-line
@@ -313,8 +315,8 @@ __device__ __forceinline__ void contract_sweeps(double* __restrict__ A, double* 
 
 // y[id] += alpha * r for the Q nodes of one x-line of a cell: every load first, then every store (the ids of a
 // line are distinct, so the read-modify-writes are independent and their latencies overlap).
-template <int Q>
-__device__ __forceinline__ void scatter_line(const int32_t* __restrict__ id_line, const double* r, double alpha,
+template <int Q, class R = double>
+__device__ __forceinline__ void scatter_line(const int32_t* __restrict__ id_line, const R* r, double alpha,
                                              double* __restrict__ y) {
   int id[Q];
   double yv[Q];
@@ -324,7 +326,7 @@ __device__ __forceinline__ void scatter_line(const int32_t* __restrict__ id_line
   for (int i = 0; i < Q; ++i) yv[i] = id[i] > 0 ? y[id[i] - 1] : 0.0;
 #pragma unroll
   for (int i = 0; i < Q; ++i)
-    if (id[i] > 0) y[id[i] - 1] = yv[i] + alpha * r[i];
+    if (id[i] > 0) y[id[i] - 1] = yv[i] + alpha * static_cast<double>(r[i]);
 }
 
 // Geometry of the trilinear map at tensor point q (point_jacobian/point_geometry, src/assembly.jl:1073-1111):
@@ -474,10 +476,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
                                                     int batch0, double alpha, const int* __restrict__ colfc,
                                                     int ncol, int gx) {
   using Dm = Dims<Q, P>;
+  // FP32 tangents (the smoothers' storages): the sweeps and the point product run in FP32 too, which halves the
+  // LDS and register footprint of the work-group; y accumulates in FP64
+  using CT = typename std::conditional<std::is_same<TS, float>::value, float, double>::type;
   constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
-  __shared__ double lds[Dm::LDS];
-  double* A = lds;
-  double* B = lds + C * SA;
+  __shared__ CT lds[Dm::LDS];
+  CT* A = lds;
+  CT* B = lds + C * SA;
   const int tid = threadIdx.x;
   const int batch = ATOMIC ? interleaved_batch(colfc, ncol, gx) : batch0 + blockIdx.x;
   if (ATOMIC && batch < 0) return;  // uniform over the work-group
@@ -485,14 +490,14 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
   for (int it = tid; it < C * 3 * Dm::NN; it += kThreads) {
     const int id = ids[it];
     const int cell = it / (3 * Dm::NN);
-    A[cell * SA + (it - cell * 3 * Dm::NN)] = id > 0 ? dx[id - 1] : 0.0;
+    A[cell * SA + (it - cell * 3 * Dm::NN)] = id > 0 ? static_cast<CT>(dx[id - 1]) : CT(0);
   }
   __syncthreads();
-  interp_sweeps<Q, P>(A, B, op.tab, tid);
+  interp_sweeps<Q, P, CT>(A, B, op.tab, tid);
   if (tid < NSLOT) {
     const int cell = tid / Dm::NP, q = tid - cell * Dm::NP;
-    double* s = B + cell * SB + q * 9;
-    double su[9], sv[9];
+    CT* s = B + cell * SB + q * 9;
+    CT su[9], sv[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       su[j] = s[j];
@@ -503,13 +508,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
-        for (int i = 0; i < 9; ++i) sv[i] += static_cast<double>(ld_stream(Dp + (size_t)(i + 9 * j) * NSLOT)) * su[j];
+        for (int i = 0; i < 9; ++i) sv[i] += static_cast<CT>(ld_stream(Dp + (size_t)(i + 9 * j) * NSLOT)) * su[j];
     } else {
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
         for (int i = 0; i <= j; ++i) {
-          const double d = static_cast<double>(ld_stream(Dp + (size_t)sym_index(i, j) * NSLOT));
+          const CT d = static_cast<CT>(ld_stream(Dp + (size_t)sym_index(i, j) * NSLOT));
           sv[i] += d * su[j];
           if (i != j) sv[j] += d * su[i];
         }
@@ -517,13 +522,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
 #pragma unroll
     for (int i = 0; i < 9; ++i) s[i] = sv[i];
   }
-  contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t0, const double* r) {
+  contract_sweeps<Q, P, CT>(A, B, op.tab, tid, [&](int cell, int c, int t0, const CT* r) {
     const int32_t* idl = ids + cell * 3 * Dm::NN + c * Dm::NN + t0;
     if (ATOMIC) {
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         const int id = idl[i];
-        if (id > 0) unsafeAtomicAdd(y + id - 1, alpha * r[i]);  // hardware FP64 add in L2 (device memory)
+        if (id > 0) unsafeAtomicAdd(y + id - 1, alpha * static_cast<double>(r[i]));  // hardware FP64 add in L2
       }
     } else {
       scatter_line<Q>(idl, r, alpha, y);  // race-free: the cells of one colour share no free dof
