@@ -247,6 +247,9 @@ class MatrixFreeSolver:
         self.last = None
 
     def solve(self, x, mf: MatrixFreeJacobian, b) -> SolveResult:
+        if not isinstance(mf, MatrixFreeJacobian):  # symbolic_setup on a matrix, src/solvers.jl:66-67
+            raise RuntimeError("HyperFEMKernels: MatrixFreeSolver applies a MatrixFreeJacobian; build the "
+                               "KernelOperator with jacobian = 'matrixfree'")
         op = mf.op
         b = _vec(b, op.nfree)
         pc = self.preconditioner
