@@ -547,6 +547,73 @@ def precision_leg(n, order, stream, storages=("full64", "full32", "sym32"), step
     return out
 
 
+def compiled_law_leg(n, stream, steps=5):
+    """The 64^3 Q2 neo-Hookean workload with the law compiled at run time from source (hiprtc) beside the
+    built-in kernels: compile time, prepare! per storage, residual! and the relative difference of Jv and of the
+    residual (the compiled path stages F, iJt, dV and mu per point through HBM; the built-in kernels keep them in
+    registers)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from hfk import HyperelasticForm, KernelOperator, MatrixFreeJacobian
+    from hfk._lib import check
+    from hfk.laws import SOURCES, CompiledLaw
+    from hfk.mesh import hex_space
+    from hfk.synthetic import normal, smooth_field
+
+    sp = hex_space((n, n, n), 2)
+    u = smooth_field(2)
+    op = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=4), sp, jacobian="matrixfree",
+                        dirichlet_values=sp.dirichlet_values(u))
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(sp.free_values(u)).to(dev)
+    vd = torch.from_numpy(normal(3, sp.nfree)).to(dev)
+    yd, bd = torch.empty_like(vd), torch.empty_like(vd)
+    sptr = stream.cuda_stream
+    t0 = time.perf_counter()
+    law = CompiledLaw(*SOURCES["neohookean"], ctx=op.ctx)
+    out = {"law": "neohookean from source", "compile_s": time.perf_counter() - t0, "free_dofs": sp.nfree}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+    def residual():
+        check(op.lib.hfk_residual_dev(op.handle, C.c_void_p(xd.data_ptr()), C.c_void_p(bd.data_ptr()),
+                                      C.c_void_p(sptr)), op.ctx.handle)
+
+    res = {}
+    for tag, lw in (("builtin", None), ("compiled", law)):
+        op.set_law(lw)
+        r = {"residual_ms": timed(residual)}
+        b = bd.cpu().numpy()
+        for st in ("full64", "sym32"):
+            mf = MatrixFreeJacobian(op, st)
+            r[f"prepare_{st}_ms"] = timed(lambda: mf.prepare_dev(xd.data_ptr(), sptr))
+            mf.mul_dev(yd.data_ptr(), vd.data_ptr(), sptr)
+            torch.cuda.synchronize()
+            r[f"jv_{st}"] = yd.cpu().numpy()
+        r["b"] = b
+        res[tag] = r
+    for tag in ("builtin", "compiled"):
+        out[tag] = {k: v for k, v in res[tag].items() if k.endswith("_ms")}
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["rel_diff_residual"] = rel(res["compiled"]["b"], res["builtin"]["b"])
+    out["rel_diff_jv_full64"] = rel(res["compiled"]["jv_full64"], res["builtin"]["jv_full64"])
+    out["rel_diff_jv_sym32"] = rel(res["compiled"]["jv_sym32"], res["builtin"]["jv_sym32"])
+    op.set_law(None)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +626,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--no-oseen", action="store_true")
+    ap.add_argument("--no-law", action="store_true", help="skip the run-time compiled law leg")
     ap.add_argument("--solve-legs", default="fp64,mixed,learned,fields,q3",
                     help="solve legs: fp64, mixed, learned, fields, q3")
     args = ap.parse_args()
@@ -705,6 +773,10 @@ def main():
                                                               state_amp=3e-3)
             res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
                                                                state_amp=3e-3)
+    if not args.no_law:
+        print("bench: 64^3 Q2 neo-Hookean with the law compiled at run time (hiprtc) beside the built-in kernels",
+              file=sys.stderr, flush=True)
+        res["compiled_law_q2_64"] = compiled_law_leg(n, stream)
     if not args.no_oseen:
         print("bench: Oseen-BDF2 step on 40^3 x 6 tetrahedra (P2-P1, SUPG/PSPG/LSIC): device assembly",
               file=sys.stderr, flush=True)

@@ -48,8 +48,15 @@ int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) 
   HIPCHK(op->ctx, op->state.alloc((size_t)op->nfree * sizeof(double)));
   if (x_dev != op->state.as<double>())
     HIPCHK(op->ctx, hipMemcpyAsync(op->state.p, x_dev, (size_t)op->nfree * sizeof(double), hipMemcpyDeviceToDevice, s));
-  HIPCHK(op->ctx, launch_prepare(op->dev(), storage, op->tangents.p, x_dev, BatchRange{0, (int)op->nbatches}, s));
+  const int rc = op_prepare_into(op, storage, op->tangents.p, x_dev, s);
+  if (rc) return rc;
   op->storage = storage;
+  return HFK_OK;
+}
+
+int op_prepare_into(hfk_op* op, int storage, void* T, const double* x_dev, hipStream_t s) {
+  if (op->rtc) return rtc_prepare(op, storage, T, x_dev, s);
+  HIPCHK(op->ctx, launch_prepare(op->dev(), storage, T, x_dev, BatchRange{0, (int)op->nbatches}, s));
   return HFK_OK;
 }
 
@@ -303,6 +310,7 @@ int run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s) {
 }
 
 int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) {
+  if (op->rtc) return rtc_residual(op, x_dev, b_dev, s);
   HIPCHK(op->ctx, launch_fill(b_dev, op->nfree, 0.0, s));
   const OpDev d = op->dev();
   for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_residual(d, x_dev, b_dev, r, s));
@@ -339,7 +347,7 @@ int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t
   const size_t bytes = (size_t)op->nbatches * 81 * op->C * op->np * sizeof(double);
   HIPCHK(op->ctx, op->D64.alloc(bytes));
   const OpDev d = op->dev();
-  HIPCHK(op->ctx, launch_prepare(d, 0, op->D64.p, x_dev, BatchRange{0, (int)op->nbatches}, s));
+  if ((rc = op_prepare_into(op, 0, op->D64.p, x_dev, s))) return rc;
   HIPCHK(op->ctx, launch_fill(nz_dev, (int64_t)op->rowval.size(), 0.0, s));
   for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_assemble(d, op->D64.as<double>(), nz_dev, r, s));
   return HFK_OK;

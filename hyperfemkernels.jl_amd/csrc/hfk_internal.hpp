@@ -60,6 +60,11 @@ hipError_t launch_apply(const OpDev& op, int storage, const void* tangents, cons
 hipError_t launch_blocks(const OpDev& op, int storage, const void* tangents, const int32_t* block_of, double* B,
                          BatchRange r, hipStream_t s);
 hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRange r, hipStream_t s);
+// pointwise laws compiled at run time: the point state (F, iJt, dV, mu: kStateEntries entries per point,
+// entry-major per batch) and the residual from point coefficients (9 per point, entry-major per batch)
+constexpr int kStateEntries = 20;
+hipError_t launch_point_state(const OpDev& op, const double* x, double* st, BatchRange r, hipStream_t s);
+hipError_t launch_residual_coefs(const OpDev& op, const double* coef, double* b, BatchRange r, hipStream_t s);
 hipError_t launch_assemble(const OpDev& op, const double* tangents64, double* nzval, BatchRange r, hipStream_t s);
 hipError_t launch_fill(double* y, int64_t n, double v, hipStream_t s);
 hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* tangents, double* D, int64_t nbatches,

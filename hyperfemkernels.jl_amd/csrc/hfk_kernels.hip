@@ -655,6 +655,74 @@ __global__ __launch_bounds__(kThreads) void k_residual(OpDev op, const double* _
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// Point state for a pointwise law compiled at run time: F (9), iJt (9), dV and mu of every point, entry-major
+// st[(batch * kStateEntries + e) * NSLOT + slot] (the gather, interpolation and geometry of k_prepare).
+template <int Q, int P>
+__global__ __launch_bounds__(kThreads) void k_point_state(OpDev op, const double* __restrict__ x, double* __restrict__ st,
+                                                          int batch0) {
+  using Dm = Dims<Q, P>;
+  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT;
+  __shared__ double lds[Dm::LDS];
+  double* A = lds;
+  double* B = lds + C * SA;
+  const int tid = threadIdx.x;
+  const int batch = batch0 + blockIdx.x;
+  const int32_t* ids = op.ids + (size_t)batch * C * 3 * Dm::NN;
+  for (int it = tid; it < C * 3 * Dm::NN; it += kThreads) {
+    const int id = ids[it];
+    const int cell = it / (3 * Dm::NN);
+    A[cell * SA + (it - cell * 3 * Dm::NN)] = id > 0 ? x[id - 1] : (id < 0 ? op.dvals[-id - 1] : 0.0);
+  }
+  __syncthreads();
+  interp_sweeps<Q, P>(A, B, op.tab, tid);
+  if (tid < NSLOT) {
+    const int cell = tid / Dm::NP, q = tid - cell * Dm::NP;
+    const double* X = op.coords + ((size_t)batch * C + cell) * 24;
+    double iJt[9], dV;
+    point_geometry<P>(op.tab, X, q, iJt, dV);
+    const double* su = B + cell * SB + q * 9;
+    double* o = st + (size_t)batch * kStateEntries * NSLOT + tid;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int J = 0; J < 3; ++J) {
+        double g = 0;
+#pragma unroll
+        for (int K = 0; K < 3; ++K) g += iJt[J + 3 * K] * su[3 * c + K];
+        o[(size_t)(c + 3 * J) * NSLOT] = (c == J ? 1.0 : 0.0) + g;
+      }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) o[(size_t)(9 + e) * NSLOT] = iJt[e];
+    o[(size_t)18 * NSLOT] = dV;
+    o[(size_t)19 * NSLOT] = op.mu_pts ? op.mu_pts[(size_t)batch * NSLOT + tid] : op.mu;
+  }
+}
+
+// residual! from point coefficients coef[(batch * 9 + K + 3c) * NSLOT + slot] (dV sum_J iJt[J,K] P_cJ, written
+// by a law compiled at run time): the contraction and scatter of k_residual, colour by colour
+template <int Q, int P>
+__global__ __launch_bounds__(kThreads) void k_residual_coefs(OpDev op, const double* __restrict__ coef,
+                                                             double* __restrict__ b, int batch0) {
+  using Dm = Dims<Q, P>;
+  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT;
+  __shared__ double lds[Dm::LDS];
+  double* A = lds;
+  double* B = lds + C * SA;
+  const int tid = threadIdx.x;
+  const int batch = batch0 + blockIdx.x;
+  const int32_t* ids = op.ids + (size_t)batch * C * 3 * Dm::NN;
+  if (tid < NSLOT) {
+    const int cell = tid / Dm::NP, q = tid - cell * Dm::NP;
+    double* s = B + cell * SB + q * 9;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) s[e] = coef[((size_t)batch * 9 + e) * NSLOT + tid];
+  }
+  contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t0, const double* r) {
+    scatter_line<Q>(ids + cell * 3 * Dm::NN + c * Dm::NN + t0, r, 1.0, b);
+  });
+}
+
+// ---------------------------------------------------------------------------------------------------------
 // jacobian!: element matrices from the prepared FP64 tangents, one work-group per cell of a colour, summed
 // into the nonzeros through nzpos (element_jacobian_mm!/scatter_jacobian!, src/assembly.jl:1316-1408).
 // Ke[(n,c),(m,k)] = sum_q sum_K sum_M dN_n/dxi_K(q) D_q[(K,c),(M,k)] dN_m/dxi_M(q), in two steps per point:
@@ -813,6 +881,18 @@ hipError_t residual_law(const OpDev& op, const double* x, double* bvec, BatchRan
 }
 
 template <int Q, int P>
+hipError_t point_state_qp(const OpDev& op, const double* x, double* st, BatchRange r, hipStream_t s) {
+  hipLaunchKernelGGL((k_point_state<Q, P>), dim3(r.count), dim3(kThreads), 0, s, op, x, st, r.first);
+  return hipGetLastError();
+}
+
+template <int Q, int P>
+hipError_t residual_coefs_qp(const OpDev& op, const double* coef, double* b, BatchRange r, hipStream_t s) {
+  hipLaunchKernelGGL((k_residual_coefs<Q, P>), dim3(r.count), dim3(kThreads), 0, s, op, coef, b, r.first);
+  return hipGetLastError();
+}
+
+template <int Q, int P>
 hipError_t assemble_qp(const OpDev& op, const double* D, double* nzval, BatchRange r, hipStream_t s) {
   constexpr int C = Dims<Q, P>::C;
   constexpr int NT = Q >= 4 ? 1024 : 256;
@@ -866,6 +946,20 @@ hipError_t launch_blocks(const OpDev& op, int storage, const void* T, const int3
 hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRange r, hipStream_t s) {
   if (r.count <= 0) return hipSuccess;
 #define CALL(q, p) residual_law<q, p>(op, x, b, r, s)
+  HFK_DISPATCH_QP(op.Q, op.P, CALL);
+#undef CALL
+}
+
+hipError_t launch_point_state(const OpDev& op, const double* x, double* st, BatchRange r, hipStream_t s) {
+  if (r.count <= 0) return hipSuccess;
+#define CALL(q, p) point_state_qp<q, p>(op, x, st, r, s)
+  HFK_DISPATCH_QP(op.Q, op.P, CALL);
+#undef CALL
+}
+
+hipError_t launch_residual_coefs(const OpDev& op, const double* coef, double* b, BatchRange r, hipStream_t s) {
+  if (r.count <= 0) return hipSuccess;
+#define CALL(q, p) residual_coefs_qp<q, p>(op, coef, b, r, s)
   HFK_DISPATCH_QP(op.Q, op.P, CALL);
 #undef CALL
 }

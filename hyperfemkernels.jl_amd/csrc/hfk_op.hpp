@@ -95,6 +95,9 @@ struct hfk_op {
   std::vector<int64_t> colptr, rowval;
   std::vector<int32_t> nzpos;  // Gridap cell order [cell][ii + nd*jj]
   hfk::NodeBlocks blocks;
+  // a pointwise law compiled at run time (nullptr: the built-in law), with its per-point staging buffers
+  hfk_law* rtc = nullptr;
+  hfk::DevBuf rtc_state, rtc_coef;
 
   hfk::OpDev dev() const {
     hfk::OpDev d;
@@ -136,4 +139,9 @@ void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols
 // the reference's greedy colouring (1-based colours); false when more than 128 colours are needed
 bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color);
 int op_build_blocks(hfk_op* op);
+// tangents of the operator's law at state x into T in `storage` (built-in kernel or the run-time law)
+int op_prepare_into(hfk_op* op, int storage, void* T, const double* x_dev, hipStream_t s);
+// the run-time law's prepare! and residual! (hfk_rtc.hip)
+int rtc_prepare(hfk_op* op, int storage, void* T, const double* x_dev, hipStream_t s);
+int rtc_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s);
 }  // namespace hfk

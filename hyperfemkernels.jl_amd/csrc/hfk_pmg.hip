@@ -788,7 +788,7 @@ int pmg_update(hfk_pmg* g, hipStream_t s) {
     const int st = g->smoother_storage;
     const size_t bytes = (size_t)f->nbatches * store_entries(st) * f->C * f->np * store_bytes(st);
     HIPCHK(g->ctx, f->tangents_s.alloc(bytes));
-    HIPCHK(g->ctx, launch_prepare(f->dev(), st, f->tangents_s.p, f->state.as<double>(), BatchRange{0, (int)f->nbatches}, s));
+    if ((rc = op_prepare_into(f, st, f->tangents_s.p, f->state.as<double>(), s))) return rc;
     f->storage_s = st;
   }
   g->cheb[0]->secondary = second;

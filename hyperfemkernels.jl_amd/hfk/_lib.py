@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhfk.so")
@@ -37,6 +38,11 @@ SIGNATURES = {
     "hfk_op_info": (C.c_int, [_vp, _i64p]),
     "hfk_op_colors": (C.c_int, [_vp, _i32p]),
     "hfk_op_set_deterministic": (C.c_int, [_vp, C.c_int]),
+    "hfk_law_create": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
+    "hfk_law_destroy": (C.c_int, [_vp]),
+    "hfk_law_check": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64]),
+    "hfk_law_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    "hfk_op_set_law": (C.c_int, [_vp, _vp]),
     "hfk_residual": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_residual_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_mf_prepare": (C.c_int, [_vp, _f64p, C.c_int]),
@@ -123,6 +129,12 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"libhfk.so not found at {path}: build it with hfk.build.build() "
                           "(there is no CPU fallback)")
+    # hiprtc of the HIP runtime already in the process: torch bundles both (same soname as /opt/rocm's)
+    torch = sys.modules.get("torch")
+    if torch is not None:
+        rtc = os.path.join(os.path.dirname(torch.__file__), "lib", "libhiprtc.so")
+        if os.path.exists(rtc):
+            C.CDLL(rtc, mode=C.RTLD_GLOBAL)
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")
 LIB = os.path.join(HERE, "libhfk.so")
-SOURCES = ["hfk_kernels.hip", "hfk_capi.hip", "hfk_solver.hip", "hfk_pmg.hip", "hfk_net.hip", "hfk_oseen.hip"]
+SOURCES = ["hfk_kernels.hip", "hfk_capi.hip", "hfk_solver.hip", "hfk_pmg.hip", "hfk_net.hip", "hfk_oseen.hip",
+           "hfk_rtc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # packed FP32 instructions stay off (-packed-fp32-ops): the compiler may not form v_pk_* FP32 arithmetic.
@@ -40,7 +41,7 @@ def build(verbose: bool = False) -> str:
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-L/opt/rocm/lib", "-lhiprtc", "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

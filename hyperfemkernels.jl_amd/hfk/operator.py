@@ -122,6 +122,15 @@ class KernelOperator:
             raise ValueError("HyperFEMKernels: mu field must have np values per cell")
         check(self.lib.hfk_op_set_mu_field(self.handle, ptr(m, C.c_double)), self.ctx.handle)
 
+    def set_law(self, law):
+        """Evaluate the residual, tangents and Jacobian with a law compiled at run time (``laws.CompiledLaw``;
+        ``None``: the form's built-in law).  Prepared tangents are invalidated (prepare again)."""
+        if law is not None and law.ctx is not self.ctx:
+            raise ValueError("HyperFEMKernels: the law was compiled for another context")
+        check(self.lib.hfk_op_set_law(self.handle, law.handle if law is not None else None), self.ctx.handle)
+        self.law = law  # kept alive while the operator uses it
+        return self
+
     def info(self) -> dict:
         a = np.zeros(8, dtype=np.int64)
         check(self.lib.hfk_op_info(self.handle, ptr(a, C.c_int64)), self.ctx.handle)

@@ -44,6 +44,7 @@ mutable struct DeviceOperator
     ptr::Ptr{Cvoid}
     ctx::Context
     nfree::Int
+    law::Any      # a CompiledLaw in use (kept alive with the operator), or nothing
 end
 
 """
@@ -64,13 +65,46 @@ function DeviceOperator(ctx::Context, cell_dofs::Matrix{Int32}, ldof::Matrix{Int
                 (Ptr{Cvoid}, Cint, Cint, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Int64, Int64, Cint,
                  Cdouble, Cdouble, Ptr{Ptr{Cvoid}}),
                 ctx.ptr, Q, P, size(cell_dofs, 2), cell_dofs, ld, np_, coords, nfree, ndir, HFK_LAW[law], λ, μ, r), ctx.ptr)
-    op = DeviceOperator(r[], ctx, nfree)
+    op = DeviceOperator(r[], ctx, nfree, nothing)
     finalizer(o -> ccall((:hfk_op_destroy, libhfk), Cint, (Ptr{Cvoid},), o.ptr), op)
     op
 end
 
 set_dirichlet!(op::DeviceOperator, d::Vector{Float64}) =
     check(ccall((:hfk_op_set_dirichlet, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, d), op.ctx.ptr)
+
+"""
+    CompiledLaw(ctx, source, name)
+
+A pointwise constitutive law compiled at run time with hiprtc (the role of the generated `PointForm`,
+src/codegen.jl:166-198): `source` is HIP C++ defining a struct `name` with `setup(F, λ, μ)`, `stress(c, J)` and
+`tangent(c, J, k, L)` (`__device__`, F column-major, indices 0-based).  Code objects are cached per process by
+source hash.  A law that does not compile throws an `ArgumentError` with the compiler log.
+"""
+mutable struct CompiledLaw
+    ptr::Ptr{Cvoid}
+    ctx::Context
+end
+function CompiledLaw(ctx::Context, source::AbstractString, name::AbstractString)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hfk_law_create, libhfk), Cint, (Ptr{Cvoid}, Cstring, Cstring, Ptr{Ptr{Cvoid}}), ctx.ptr, source, name, r),
+          ctx.ptr)
+    l = CompiledLaw(r[], ctx)
+    finalizer(l -> ccall((:hfk_law_destroy, libhfk), Cint, (Ptr{Cvoid},), l.ptr), l)
+    l
+end
+
+"Compile a law without a device: `nothing`, or an `ArgumentError` with the compiler log."
+check_law(source::AbstractString, name::AbstractString) =
+    check(ccall((:hfk_law_check, libhfk), Cint, (Cstring, Cstring, Ptr{UInt8}, Int64), source, name, C_NULL, 0))
+
+"Evaluate `op` with a compiled law (`nothing`: its built-in law); prepared tangents are invalidated."
+function set_law!(op::DeviceOperator, law::Union{Nothing,CompiledLaw})
+    check(ccall((:hfk_op_set_law, libhfk), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), op.ptr, law === nothing ? C_NULL : law.ptr),
+          op.ctx.ptr)
+    op.law = law
+    op
+end
 
 "Algebra.residual!(b, op, x) (src/assembly.jl:909-916)"
 function residual!(b::Vector{Float64}, op::DeviceOperator, x::Vector{Float64})

@@ -83,6 +83,27 @@ int hfk_op_colors(hfk_op* op, int32_t* color_of_cell);
  * additions into one entry varies between runs). */
 int hfk_op_set_deterministic(hfk_op* op, int deterministic);
 
+/* ---- pointwise laws compiled at run time (src/codegen.jl:166-198 PointForm, compiled per structural
+ * signature, src/tree.jl:244-283) ------------------------------------------------------------------ */
+/* Compile a constitutive law given as HIP source with hiprtc for gfx950: `source` defines a struct `name` with
+ *   __device__ void setup(const double* F, double lam, double mu);   F[c + 3J] = d x_c / d X_J
+ *   __device__ double stress(int c, int J) const;                     first Piola-Kirchhoff P_cJ
+ *   __device__ double tangent(int c, int J, int k, int L) const;     d P_cJ / d F_kL
+ * (helpers hfk_det3(a) and hfk_inv3(a, r, det) on column-major 3x3 arrays are in scope).  Code objects are
+ * cached per process by a hash of the full source: a second law of the same text is not compiled again.
+ * Compile errors return HFK_EARG with the compiler log in hfk_last_error. */
+typedef struct hfk_law hfk_law;
+int hfk_law_create(hfk_ctx* ctx, const char* source, const char* name, hfk_law** law);
+int hfk_law_destroy(hfk_law* law);
+/* Compile only (no device): HFK_OK, or HFK_EARG with the compiler log (also copied into log[0..log_size)). */
+int hfk_law_check(const char* source, const char* name, char* log, int64_t log_size);
+/* key: the source hash; from_cache: 1 when the code object came from the cache */
+int hfk_law_info(hfk_law* law, uint64_t* key, int* from_cache);
+/* Evaluate the operator's residual!, prepare! and jacobian! with `law` (NULL: the built-in law of
+ * hfk_op_create).  The law must outlive its use by the operator; the stored tangents are invalidated.  The
+ * packed symmetric storages assume a symmetric tangent, as for the built-in laws. */
+int hfk_op_set_law(hfk_op* op, hfk_law* law);
+
 /* ---- residual: Algebra.residual!(b, op, x)  (src/assembly.jl:909-916, 1219-1241) ----------------- */
 int hfk_residual(hfk_op* op, const double* x, double* b);
 int hfk_residual_dev(hfk_op* op, const double* x_dev, double* b_dev, void* stream);
