@@ -7,20 +7,9 @@
 #include <string>
 #include <vector>
 
+#include "hfk_types.hpp"
+
 namespace hfk {
-
-// 1D tables of one (Q, P) pair: Lagrange values/derivatives on Q equispaced nodes of [0,1] at the P Gauss
-// points (TPTable.N1/dN1, src/tensorproduct.jl:13-47), with the points and weights.  N[a*Q + i].
-struct Tab {
-  double N[16];
-  double dN[16];
-  double xq[4];
-  double wq[4];
-};
-
-// Cells per work-group batch: the point phase maps one thread to one (cell, point) pair of the batch.
-constexpr int kThreads = 256;
-constexpr int cells_per_batch(int P) { return kThreads / (P * P * P); }
 
 // Geometry of the launch of one colour (or of all batches when the kernel has no scatter).
 struct BatchRange {
@@ -28,18 +17,7 @@ struct BatchRange {
   int count;  // number of batches
 };
 
-struct OpDev {
-  int Q, P, C, nn, np, nd;
-  int law;
-  double lam, mu;
-  Tab tab;
-  const int32_t* ids;     // [batch][cell][c][t] tensor order, 0 for padding cells
-  const double* coords;   // [batch][cell][v][i]
-  const double* mu_pts;   // [batch][cell*np + q] or null
-  const double* dvals;    // Dirichlet values
-  const int32_t* loc_of;  // tensor index c*nn + t -> table-local dof (assembly)
-  const int32_t* nzpos;   // [cell (batch order)][ii + nd*jj] table-local, 1-based, or null
-};
+
 
 // Launchers (hfk_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_prepare(const OpDev& op, int storage, void* tangents, const double* x, BatchRange r, hipStream_t s);
@@ -60,11 +38,6 @@ hipError_t launch_apply(const OpDev& op, int storage, const void* tangents, cons
 hipError_t launch_blocks(const OpDev& op, int storage, const void* tangents, const int32_t* block_of, double* B,
                          BatchRange r, hipStream_t s);
 hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRange r, hipStream_t s);
-// pointwise laws compiled at run time: the point state (F, iJt, dV, mu: kStateEntries entries per point,
-// entry-major per batch) and the residual from point coefficients (9 per point, entry-major per batch)
-constexpr int kStateEntries = 20;
-hipError_t launch_point_state(const OpDev& op, const double* x, double* st, BatchRange r, hipStream_t s);
-hipError_t launch_residual_coefs(const OpDev& op, const double* coef, double* b, BatchRange r, hipStream_t s);
 hipError_t launch_assemble(const OpDev& op, const double* tangents64, double* nzval, BatchRange r, hipStream_t s);
 hipError_t launch_fill(double* y, int64_t n, double v, hipStream_t s);
 hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* tangents, double* D, int64_t nbatches,

@@ -95,9 +95,8 @@ struct hfk_op {
   std::vector<int64_t> colptr, rowval;
   std::vector<int32_t> nzpos;  // Gridap cell order [cell][ii + nd*jj]
   hfk::NodeBlocks blocks;
-  // a pointwise law compiled at run time (nullptr: the built-in law), with its per-point staging buffers
+  // a pointwise law compiled at run time (nullptr: the built-in law)
   hfk_law* rtc = nullptr;
-  hfk::DevBuf rtc_state, rtc_coef;
 
   hfk::OpDev dev() const {
     hfk::OpDev d;

@@ -28,14 +28,33 @@ def _run(cmd):
     return r.stdout
 
 
+# device headers compiled again at run time by hiprtc (laws given as source): embedded in the library as text
+EMBEDDED = (("kTypesSrc", "hfk_types.hpp"), ("kPointSrc", "hfk_point.hpp"))
+
+
+def write_embedded(objdir: str) -> None:
+    parts = ["// generated by hfk/build.py from the device headers; do not edit\n"]
+    for var, name in EMBEDDED:
+        text = open(os.path.join(CSRC, name)).read()
+        if ")HFKSRC\"" in text:
+            raise RuntimeError(f"{name} contains the raw-string delimiter")
+        parts.append(f'static const char {var}[] = R"HFKSRC({text})HFKSRC";\n')
+    path = os.path.join(objdir, "hfk_embedded.inc")
+    new = "".join(parts)
+    if not os.path.exists(path) or open(path).read() != new:
+        with open(path, "w") as f:
+            f.write(new)
+
+
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    write_embedded(objdir)
     objs, cmds = [], []
     for s in SOURCES:
         o = os.path.join(objdir, s.replace(".hip", ".o"))
         objs.append(o)
-        cmds.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        cmds.append([HIPCC, *FLAGS, f"-I{objdir}", "-c", os.path.join(CSRC, s), "-o", o])
     with ThreadPoolExecutor(len(cmds)) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():
