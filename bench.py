@@ -371,14 +371,15 @@ def lognormal_problem(n, seed, state_amp=0.0):
     return {"fs": fs, "cs": cs, "fop": fop, "cop": cop, "x": fs.free_values(u), "b": b, "aux": aux, "mu": mu}
 
 
-def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(10000, 10010), samples=24,
-                       steps=3000, rtol=1e-8, mode="cycle", scaled=True):
+def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(10000, 10016), samples=96,
+                       steps=12000, rtol=1e-8, mode="cycle", scaled=True, decay="cosine", progress=False):
     """The learned coarse correction across modulus fields: one network (U-Net on every other coarse vertex, log mu
     as auxiliary input, acting on the diagonally scaled coarse system) trained on the plain p-MG's coarse-smoothed
-    residuals of `samples` random right-hand sides on each training field, then on every held-out field (seeds never
-    trained on) FCG + p-MG with the network's two-grid cycle as the coarse correction (mode "cycle"; FP32
-    smoothers, FP64 outer FCG and coarse matrix) against the plain FP64 p-MG (coarse PCG to 0.3, its best in the
-    64^3 sweep), both to rtol.  Solve times are the second of two identical solves."""
+    residuals of `samples` random right-hand sides on each training field (Adam, step size 1e-3 decayed along a
+    half cosine), then on every held-out field (seeds never trained on) FCG + p-MG with the network's two-grid
+    cycle as the coarse correction (mode "cycle"; FP32 smoothers, FP64 outer FCG and coarse matrix), against the
+    plain FP64 p-MG (coarse PCG to 0.3, its best in the 64^3 sweep) and against the same mixed-precision p-MG
+    without the network, all to rtol.  Solve times are the second of two identical solves."""
     import numpy as np
     import torch
 
@@ -397,32 +398,46 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
         if not pmg.ok:
             continue
         gen = torch.Generator(device=dev).manual_seed(sd)
-        bf = torch.randn((samples, p["fs"].nfree), dtype=torch.float64, device=dev, generator=gen)
         bc = torch.zeros((samples, p["cs"].nfree), dtype=torch.float64, device=dev)
         set_coarse_network(pmg, None, "off", 3, scaled)
-        coarse_pairs_dev(pmg, bf.data_ptr(), samples, bc.data_ptr(), None, coarse_smoothed=True)
-        del bf
+        for k0 in range(0, samples, 32):  # fine right-hand sides in flight
+            k1 = min(k0 + 32, samples)
+            bf = torch.randn((k1 - k0, p["fs"].nfree), dtype=torch.float64, device=dev, generator=gen)
+            coarse_pairs_dev(pmg, bf.data_ptr(), k1 - k0, bc[k0:].data_ptr(), None, coarse_smoothed=True)
+            del bf
         train.append((pmg, bc.data_ptr(), samples, p["aux"]))
         keep.append((p, mf, pmg, bc))
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
+    if progress:
+        print(f"bench: fields: {len(train)} training fields x {samples} samples in {t_data:.1f} s", file=sys.stderr,
+              flush=True)
     cs0 = keep[0][0]["cs"]
     net = CoarseNetwork(keep[0][0]["fop"].ctx, cs0, seed=5, coarsen=True)
     t0 = time.perf_counter()
-    hist = train_energy_problems(train, net, steps, chunk=25, batch=4, lr=1e-3, seed=5)
-    torch.cuda.synchronize()
+    hist = []
+    for k0 in range(0, steps, 4000):  # in slices, so that a long run reports progress
+        k = min(4000, steps - k0)
+        hist.append(train_energy_problems(train, net, k, chunk=25, batch=4, lr=1e-3, seed=5, decay=decay, step0=k0,
+                                          total=steps))
+        torch.cuda.synchronize()
+        if progress:
+            print(f"bench: fields: {k0 + k}/{steps} Adam steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+    hist = np.concatenate(hist)
     t_train = time.perf_counter() - t0
     del train, keep
     rows = []
+    legs = (("fp64", "full64", None, 0.3, False), ("mixed", "sym64", "sym32", 0.3, False),
+            ("learned", "sym64", "sym32", 0.1, True))
     for sd in test_seeds:
         p = lognormal_problem(n, sd)
         row = {"seed": sd, "sigma": p["mu"]["sigma"], "ell": p["mu"]["ell"]}
-        for leg in ("fp64", "learned"):
-            st, sst, crt = ("full64", None, 0.3) if leg == "fp64" else ("sym64", "sym32", 0.1)
+        for leg, st, sst, crt, use_net in legs:
             mf = MatrixFreeJacobian(p["fop"], st).prepare(p["x"])
             pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=crt, coarse_maxiter=400,
                              smoother_storage=sst).update()
-            if leg == "learned":
+            if use_net:
                 net.set_aux(p["aux"])
                 set_coarse_network(pmg, net, mode, 3, scaled)
             solver = MatrixFreeSolver("fcg", rtol=rtol, maxiter=300, preconditioner=pmg)
@@ -436,16 +451,25 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
             set_coarse_network(pmg, None, "off")
             del pmg, mf
         rows.append(row)
+        if progress and len(rows) % 10 == 0:
+            print(f"bench: fields: {len(rows)}/{len(test_seeds)} held-out fields solved", file=sys.stderr, flush=True)
     t_fp = np.array([r["fp64"]["solve_s"] for r in rows])
+    t_mx = np.array([r["mixed"]["solve_s"] for r in rows])
     t_ln = np.array([r["learned"]["solve_s"] for r in rows])
+    sig = np.array([r["sigma"] for r in rows])
     return {"n": n, "free_dofs": int(3 * (2 * n + 1) ** 2 * (2 * n - 1)), "coarse_mode": mode, "scaled": scaled,
-            "train_fields": len(list(train_seeds)),
-            "samples_per_field": samples, "train_steps": steps, "train_data_s": t_data, "train_s": t_train,
+            "train_fields": len(list(train_seeds)), "samples_per_field": samples,
+            "train_pairs": len(list(train_seeds)) * samples, "train_steps": steps, "lr_decay": decay,
+            "train_data_s": t_data, "train_s": t_train,
             "train_loss_first_last": [float(hist[:25].mean()), float(hist[-50:].mean())],
             "heldout_fields": len(rows), "fp64_solve_s_mean": float(t_fp.mean()),
-            "learned_solve_s_mean": float(t_ln.mean()),
-            "speedup_mean": float((t_fp / t_ln).mean()), "learned_faster_on": int((t_ln < t_fp).sum()),
-            "all_converged": all(r["fp64"]["converged"] and r["learned"]["converged"] for r in rows),
+            "mixed_solve_s_mean": float(t_mx.mean()), "learned_solve_s_mean": float(t_ln.mean()),
+            "speedup_mean": float((t_fp / t_ln).mean()), "speedup_median": float(np.median(t_fp / t_ln)),
+            "speedup_min": float((t_fp / t_ln).min()), "learned_faster_on": int((t_ln < t_fp).sum()),
+            "network_gain_vs_mixed_mean": float((t_mx / t_ln).mean()),
+            "network_gain_vs_mixed_sigma_ge_1": float((t_mx / t_ln)[sig >= 1.0].mean()) if (sig >= 1.0).any() else None,
+            "network_gain_vs_mixed_sigma_lt_1": float((t_mx / t_ln)[sig < 1.0].mean()) if (sig < 1.0).any() else None,
+            "all_converged": all(r[k]["converged"] for r in rows for k in ("fp64", "mixed", "learned")),
             "fields": rows}
 
 
@@ -631,6 +655,8 @@ def main():
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--no-oseen", action="store_true")
     ap.add_argument("--no-law", action="store_true", help="skip the run-time compiled law leg")
+    ap.add_argument("--fields-full", action="store_true",
+                    help="learned fields leg at the configured scale (20k training pairs, 100 held-out fields)")
     ap.add_argument("--solve-legs", default="fp64,mixed,learned,fields,q3",
                     help="solve legs: fp64, mixed, learned, fields, q3")
     args = ap.parse_args()
@@ -760,7 +786,12 @@ def main():
         if "fields" in legs:
             print("bench: learned coarse correction across lognormal modulus fields (32^3 Q2)", file=sys.stderr,
                   flush=True)
-            res["learned_fields_32"] = learned_fields_leg()
+            if args.fields_full:
+                # the configured scale: 64 training fields x 320 = 20,480 (residual, error) pairs, 100 held-out
+                res["learned_fields_32"] = learned_fields_leg(train_seeds=range(1000, 1064), samples=320, steps=48000,
+                                                              test_seeds=range(10000, 10100), progress=True)
+            else:
+                res["learned_fields_32"] = learned_fields_leg(progress=True)
         if "q3" in legs:
             # Q3 on 48^3 (64 points per cell): FP32 Jv against FP64; mixed-precision solves (FP32 smoothers, FP64
             # outer FCG and coarse level) with mu = 1 and with a lognormal mu (sigma 1, correlation length 0.1)

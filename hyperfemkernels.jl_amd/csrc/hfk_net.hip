@@ -54,16 +54,19 @@ __device__ __forceinline__ bool neighbour(int v, int tap, int N, int& nb) {
 __device__ __forceinline__ bf16x4 load4(const unsigned short* p) { return *reinterpret_cast<const bf16x4*>(p); }
 
 // Y[v][co] = act( sum_tap sum_ci X[v + off(tap)][ci] W[tap][co][ci] ), X = [X1 | X2] (zero outside the lattice).
-// Implicit GEMM on v_mfma_f32_16x16x16_bf16: M = voxels, N = output channels, K = 27 x input channels.  One wave
-// computes 16 voxels x all output channels (NT tiles of 16).  Lane l: A rows m = l % 16 (a voxel), k = 4 (l / 16)
-// + i; B columns n = l % 16 (an output channel); D rows 4 (l / 16) + i, column l % 16.
+// Implicit GEMM on v_mfma_f32_16x16x16_bf16: M = voxels, N = output channels, K = 27 x input channels.  One
+// work-group computes 16 voxels x all output channels (NT tiles of 16); its 4 waves split the 27 taps (wave w takes
+// taps w, w + 4, ...) and their partial sums are added through LDS in wave order.  On the small lattices of the
+// coarse levels (5^3 .. 33^3 voxels) this gives 4x the waves of a one-wave-per-tile mapping and a quarter of the
+// dependent MFMA chain.  Lane l: A rows m = l % 16 (a voxel), k = 4 (l / 16) + i; B columns n = l % 16 (an output
+// channel); D rows 4 (l / 16) + i, column l % 16.
 // OUT: 0 = BF16 with ReLU, 1 = FP32 (overwrite), 2 = FP32 accumulate.
 template <int NT, int OUT>
 __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const unsigned short* __restrict__ W, View Y) {
+  __shared__ f32x4 part[4][NT][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nvox = N * N * N;
-  const int v0 = (blockIdx.x * 4 + wave) * 16;
-  if (v0 >= nvox) return;  // uniform over the wave (no barrier below)
+  const int v0 = blockIdx.x * 16;  // the grid has ceil(nvox / 16) work-groups: v0 < nvox
   const int m = lane & 15, kg = lane >> 4;
   const int vm = v0 + m;
   const int Cin = X1.C + (X2.p ? X2.C : 0);
@@ -71,14 +74,14 @@ __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const uns
   f32x4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int tap = 0; tap < 27; ++tap) {
+  for (int tap = wave; tap < 27; tap += 4) {
     int nb = 0;
     const bool in = vm < nvox && neighbour(vm, tap, N, nb);
     const unsigned short* wt = W + (size_t)tap * Cout * Cin;
-    for (int part = 0; part < 2; ++part) {
-      const View& X = part == 0 ? X1 : X2;
+    for (int part_ = 0; part_ < 2; ++part_) {
+      const View& X = part_ == 0 ? X1 : X2;
       if (!X.p) continue;
-      const int cbase = part == 0 ? 0 : X1.C;
+      const int cbase = part_ == 0 ? 0 : X1.C;
       const unsigned short* xp = static_cast<const unsigned short*>(X.p) + (size_t)nb * X.stride + 4 * kg;
       for (int c0 = 0; c0 < X.C; c0 += 16) {
         bf16x4 a = in ? load4(xp + c0) : bf16x4{0, 0, 0, 0};
@@ -91,13 +94,16 @@ __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const uns
     }
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
+  for (int t = 0; t < NT; ++t) part[wave][t][lane] = acc[t];
+  __syncthreads();
+  for (int t = wave; t < NT; t += 4) {
+    const f32x4 sum = ((part[0][t][lane] + part[1][t][lane]) + part[2][t][lane]) + part[3][t][lane];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int v = v0 + 4 * kg + i;
       if (v >= nvox) continue;
       const int co = t * 16 + m;
-      const float val = acc[t][i];
+      const float val = sum[i];
       if (OUT == 0) {
         static_cast<unsigned short*>(Y.p)[(size_t)v * Y.stride + co] = f2bf(val > 0.f ? val : 0.f);
       } else if (OUT == 1) {
@@ -106,6 +112,7 @@ __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const uns
         static_cast<float*>(Y.p)[(size_t)v * Y.stride + co] += val;
       }
     }
+  }
 }
 
 // dW[tap][co][ci] += sum_v G[v][co] X[v + off(tap)][ci]: per (tap, co tile, ci tile) a GEMM over the voxels
@@ -450,7 +457,7 @@ hipError_t conv(hfk_net* nt, int li, View X1, View X2, const unsigned short* W, 
   const Layer& L = nt->layers[li];
   const int N = nt->N[L.level];
   const int64_t nv = nt->nvox[L.level];
-  const dim3 g((unsigned)((nv + 63) / 64)), b(256);
+  const dim3 g((unsigned)((nv + 15) / 16)), b(256);
   const int nt16 = Y.C / 16;
 #define LAUNCH(NT_)                                                                                   \
   if (out == 0) hipLaunchKernelGGL((k_conv<NT_, 0>), g, b, 0, s, N, X1, X2, W, Y);                     \
@@ -513,7 +520,7 @@ int conv_backward(hfk_net* nt, int li, View dY, View Yact, View X1, View X2, Vie
     hipLaunchKernelGGL(k_weight_adjoint, grid1((int64_t)nw), dim3(256), 0, s, nt->wbf.as<unsigned short>() + L.woff,
                        nt->wt.as<unsigned short>(), L.cout, cin);
     // the adjoint conv: input G (cout channels), weights Wt [tap][cin][cout], output dX (cin channels)
-    const dim3 g((unsigned)((nv + 63) / 64)), b(256);
+    const dim3 g((unsigned)((nv + 15) / 16)), b(256);
     const int nt16 = cin / 16;
 #define LAUNCH(NT_)                                                                                                   \
   if (dx_out == 1) hipLaunchKernelGGL((k_conv<NT_, 1>), g, b, 0, s, N, G, kNone, nt->wt.as<unsigned short>(), dX);    \

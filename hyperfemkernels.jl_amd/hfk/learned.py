@@ -117,18 +117,28 @@ def train_energy_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, step
 
 
 def train_energy_problems(problems, net: "CoarseNetwork", steps: int, chunk: int = 25, batch: int = 4,
-                          lr: float = 1e-3, seed: int = 0, stream: int = 0) -> np.ndarray:
+                          lr: float = 1e-3, seed: int = 0, stream: int = 0, decay: str = "none", step0: int = 0,
+                          total: int | None = None) -> np.ndarray:
     """Energy training of one network over several operators: ``problems`` is a list of (pmg, bc_ptr, nsamples,
     aux) -- the p-multigrid whose coarse matrix defines the energy, its device samples, and the auxiliary vertex
     field of that operator (or None).  The problems take turns, ``chunk`` Adam steps each (the optimiser state
-    carries over); returns the loss per step."""
+    carries over); ``decay="cosine"`` lowers the step size from ``lr`` to lr/20 along a half cosine over a run of
+    ``total`` steps (default ``steps``) of which this call does steps ``step0 .. step0 + steps`` (a long run can be
+    done in slices).  Returns the loss per step."""
+    if decay not in ("none", "cosine"):
+        raise ValueError(f"HyperFEMKernels: unknown learning-rate decay {decay!r}")
+    total = steps if total is None else total
     hist = []
-    done, turn = 0, 0
+    done, turn = 0, step0 // chunk
     while done < steps:
         pmg, bc_ptr, ns, aux = problems[turn % len(problems)]
         k = min(chunk, steps - done)
         net.set_aux(aux)
-        hist.append(train_energy_dev(pmg, net, bc_ptr, ns, k, batch=batch, lr=lr, seed=seed + turn, stream=stream))
+        step_lr = lr
+        if decay == "cosine":
+            step_lr = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * (step0 + done + 0.5 * k) / total)))
+        hist.append(train_energy_dev(pmg, net, bc_ptr, ns, k, batch=batch, lr=step_lr, seed=seed + turn,
+                                     stream=stream))
         done += k
         turn += 1
     return np.concatenate(hist) if hist else np.zeros(0)
