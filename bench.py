@@ -147,15 +147,17 @@ def oseen_leg(n=40, reps=10):
     """The Oseen-BDF2 step on tetrahedra: unit cube n^3 split into 6 tetrahedra per cell, Taylor-Hood P2-P1,
     rho = 1, mu = 0.01, dt = 0.01 (BDF2), SUPG/PSPG/LSIC; inlet x = 0 with the parabolic profile
     16 y(1-y) z(1-z), walls on the y and z faces, outlet x = 1; advecting velocity the 12-mode divergence-free
-    field of seed 5 scaled to unit RMS over the velocity nodes (steady: un = w, history c0 w).  Times the device
-    residual!, jacobian! and the fused pass (state and outputs on the device, HIP events on the stream), and the
-    host-side pattern build; the 14-point degree-5 rule."""
+    field of seed 5 scaled to unit RMS over the velocity nodes (steady: un = w, history c0 w); the full step of
+    test/common.jl:165-228: the elementwise Laplacian of un (the P2 interpolant of w) in the stabilisation load and
+    the outlet terms (backflow with w, Robin ZA = 2.3, pressure Pc = 0.7; collapsed Gauss rules of degree 4 and 5 on
+    the faces).  Times the device residual!, jacobian! and the fused pass (state and outputs on the device, HIP events
+    on the stream), and the host-side pattern build; the 14-point degree-5 rule in the cells."""
     import numpy as np
     import torch
 
     from hfk._lib import check
     from hfk.oseen import OseenOperator
-    from hfk.tetmesh import divfree_field, tet_p2p1_space
+    from hfk.tetmesh import divfree_field, outlet_faces, p2_laplacian, tet_p2p1_space
 
     import ctypes as C
 
@@ -172,6 +174,11 @@ def oseen_leg(n=40, reps=10):
     op = OseenOperator(sp, rho=1.0, mu=0.01, dt=0.01, bdf2=True, dirichlet_values=dv)
     t_create = time.perf_counter() - t0
     op.set_steady_advection(w)
+    un = np.zeros(sp.nfree)
+    un[:sp.nfree_u] = w(sp.node_xyz[sp.free_node_of]).reshape(-1)
+    op.set_laplacian(p2_laplacian(sp, un, sp.dirichlet_values(w)))
+    faces = outlet_faces(sp)
+    op.set_outlet(faces, w, Pc=0.7, ZA=2.3)
     t0 = time.perf_counter()
     colptr, rowval, nzpos = op.pattern()
     t_pattern = time.perf_counter() - t0
@@ -206,7 +213,9 @@ def oseen_leg(n=40, reps=10):
     # updates hit the caches), cell ids and point data read once
     jac_bytes = ncells * 34 * 34 * 4 + 2 * nnz * 8 + ncells * (34 * 4 + 12 * 8 + op.np * 9 * 8)
     return {"free_dofs": sp.nfree, "total_dofs": sp.nfree + sp.ndir, "cells": ncells, "nnz": nnz,
-            "colors": info["ncolors"], "points_per_cell": op.np, "residual_ms": t_res * 1e3,
+            "colors": info["ncolors"], "points_per_cell": op.np, "outlet_faces": int(len(faces.cell)),
+            "terms": "volume + Laplacian in the stabilisation load + outlet backflow/Robin/pressure",
+            "residual_ms": t_res * 1e3,
             "jacobian_ms": t_jac * 1e3, "residual_and_jacobian_ms": t_both * 1e3,
             "jacobian_Mcells_per_s": ncells / t_jac / 1e6, "jacobian_min_bytes": jac_bytes,
             "jacobian_GBs_of_min_bytes": jac_bytes / t_jac / 1e9, "pattern_s": t_pattern, "create_s": t_create,
@@ -470,7 +479,15 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
             "network_gain_vs_mixed_sigma_ge_1": float((t_mx / t_ln)[sig >= 1.0].mean()) if (sig >= 1.0).any() else None,
             "network_gain_vs_mixed_sigma_lt_1": float((t_mx / t_ln)[sig < 1.0].mean()) if (sig < 1.0).any() else None,
             "all_converged": all(r[k]["converged"] for r in rows for k in ("fp64", "mixed", "learned")),
+            "converged": {k: int(sum(r[k]["converged"] for r in rows)) for k in ("fp64", "mixed", "learned")},
+            "speedup_mean_all_converged": _mean_ratio(rows, ("fp64", "mixed", "learned"), "fp64", "learned"),
             "fields": rows}
+
+
+def _mean_ratio(rows, legs, num, den):
+    """Mean solve-time ratio num/den over the fields on which every leg converged (None if there are none)."""
+    r = [x[num]["solve_s"] / x[den]["solve_s"] for x in rows if all(x[k]["converged"] for k in legs)]
+    return float(sum(r) / len(r)) if r else None
 
 
 def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoother_degree=0, scaled=True):

@@ -6,13 +6,17 @@
 //   a((u,p),(v,q)) = rho c u.v + 2 mu eps(u):eps(v) + rho ((grad u)' w).v - p div v - q div u
 //                  + ts ((grad v)' w).Ru + ts ((grad v)' w).grad p - tp grad q.Ru - tp grad q.grad p
 //                  + tl div v div u,                     Ru = rho c u + rho (grad u)' w
-//   l((v,q))       = rho f.v + ts ((grad v)' w).(rho f) - tp grad q.(rho f)
+//   l((v,q))       = rho f.v + ts ((grad v)' w).rk - tp grad q.rk,        rk = rho f + mu lap
 //   ts = ((2|un|/h)^2 + 9 (4 mu/(rho h^2))^2)^(-1/2), tp = ts/rho, tl = rho |un| h / 2, h = (6 sqrt(2) |K|)^(1/3)
+// and on the outlet faces (backflow on dGb, Robin and Windkessel pressure on dGo)
+//   a += -rho negpart(w.n) u.v + ZA (u.n)(v.n),   l += -Pc v.n,          negpart(s) = (s - |s|) / 2
 //
-// (test/common.jl:201-205, Gridap's gradient convention grad u[J,c] = d u_c / d x_J).  The residual is
+// (test/common.jl:201-220, Gridap's gradient convention grad u[J,c] = d u_c / d x_J).  The residual is
 // r(x) = a(x, .) - l(.) on the free test dofs with the Dirichlet values lifted; the Jacobian is a(., .) on the free
-// (test, trial) pairs.  w (advecting velocity), un (velocity of the stabilisation parameters) and f (BDF history)
-// are point data given per cell and quadrature point.
+// (test, trial) pairs.  w (advecting velocity), un (velocity of the stabilisation parameters), f (BDF history) and
+// lap (elementwise Laplacian of un, optional) are point data given per cell and quadrature point; the outlet faces
+// come with their points in the reference coordinates of their cell, the weights dS, the outward normal and w at
+// the backflow points.
 //
 // Element: 4 vertices (affine map x = x0 + J xi), P2 velocity nodes 0..3 the vertices and 4..9 the midpoints of
 // the edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3), P1 pressure on the vertices; cell dofs [c*10 + a] (velocity
@@ -41,6 +45,7 @@ struct OseenDev {
   const double* coords;   // [cell][4][3]
   const double* qp;       // [q][4]: xi, eta, zeta, weight
   const double* fields;   // [cell][q][9]: w, un, f
+  const double* lap;      // [cell][q][3] or null
   const double* dvals;
   const int32_t* nzpos;   // [cell][ii + 34 jj], 1-based, 0: not stored
 };
@@ -97,7 +102,7 @@ template <bool WANT_K, bool WANT_R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, const double* __restrict__ x,
                                                double* __restrict__ b, double* __restrict__ nzval) {
   __shared__ double Sh[kMaxPoints][14][5];  // per point and shape function: N, grad N (3), w . grad N
-  __shared__ double Pq[kMaxPoints][7];      // per point: dV, ts, tp, tl, f (3)
+  __shared__ double Pq[kMaxPoints][10];     // per point: dV, ts, tp, tl, f (3), lap (3)
   __shared__ double Ke[kND][kND + 1];       // element matrix [test][trial]
   __shared__ double Xe[kND], Ld[kND];
   __shared__ double Jit[9], Cm[2];
@@ -169,6 +174,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       Pq[q][4] = fq[6];
       Pq[q][5] = fq[7];
       Pq[q][6] = fq[8];
+      const double* lq = op.lap ? op.lap + ((size_t)cell * np + q) * 3 : nullptr;
+      Pq[q][7] = lq ? lq[0] : 0.0;
+      Pq[q][8] = lq ? lq[1] : 0.0;
+      Pq[q][9] = lq ? lq[2] : 0.0;
     }
   }
   __syncthreads();
@@ -233,13 +242,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       const int i = r / 10, a = r - i * 10;
       for (int q = 0; q < np; ++q) {
         const double dV = Pq[q][0], ts = Pq[q][1], fi = Pq[q][4 + i];
-        L += dV * (rho * fi * Sh[q][a][0] + ts * Sh[q][a][4] * rho * fi);
+        L += dV * (rho * fi * Sh[q][a][0] + ts * Sh[q][a][4] * (rho * fi + mu * Pq[q][7 + i]));
       }
     } else {
       const int ap = 10 + (r - 30);
       for (int q = 0; q < np; ++q)
-        L -= Pq[q][0] * Pq[q][2] * rho *
-             (Sh[q][ap][1] * Pq[q][4] + Sh[q][ap][2] * Pq[q][5] + Sh[q][ap][3] * Pq[q][6]);
+        L -= Pq[q][0] * Pq[q][2] *
+             (Sh[q][ap][1] * (rho * Pq[q][4] + mu * Pq[q][7]) + Sh[q][ap][2] * (rho * Pq[q][5] + mu * Pq[q][8]) +
+              Sh[q][ap][3] * (rho * Pq[q][6] + mu * Pq[q][9]));
     }
     Ld[r] = L;
   }
@@ -260,6 +270,93 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
 }
 
+// Outlet faces (test/common.jl:212-213, 219).  Per face: its cell, the outward unit normal, the points of the
+// Robin/pressure measure dGo [q][4] (cell reference coordinates, dS) and of the backflow measure dGb [q][7]
+// (reference coordinates, dS, w).
+constexpr int kMaxFacePoints = 32;
+struct OseenFaceDev {
+  int nqo, nqb;
+  double Pc, ZA;
+  const int32_t* cell;    // [face]
+  const double* normal;   // [face][3]
+  const double* xo;       // [face][nqo][4]
+  const double* xb;       // [face][nqb][7]
+};
+
+// One work-group per face of one colour (the colour of its cell, so faces of a launch share no free test dof):
+// the velocity block Kf[(i,a)][(j,b)] = delta_ij Mb_ab + ZA n_i n_j Mo_ab with
+//   Mb_ab = sum_dGb dS (-rho negpart(w.n)) Na Nb,   Mo_ab = sum_dGo dS Na Nb,
+// and the load -Pc n_i sum_dGo dS Na, added to the CSC values through the cell's nzpos and to the residual rows.
+// Velocity shape functions vanish on the faces they are not attached to, so all 10 nodes are summed.
+template <bool WANT_K, bool WANT_R>
+__global__ __launch_bounds__(128) void k_oseen_face(OseenDev op, OseenFaceDev fd, const int32_t* __restrict__ faces,
+                                                    const double* __restrict__ x, double* __restrict__ b,
+                                                    double* __restrict__ nzval) {
+  __shared__ double Nb[kMaxFacePoints][10], No[kMaxFacePoints][10], Wb[kMaxFacePoints], Wo[kMaxFacePoints];
+  __shared__ double Mb[10][10], Mo[10][10], Lo[10], Xe[30], Nrm[3];
+  __shared__ int32_t Id[30];
+  const int tid = threadIdx.x;
+  const int64_t f = faces[blockIdx.x];
+  const int64_t cell = fd.cell[f];
+  if (tid < 3) Nrm[tid] = fd.normal[f * 3 + tid];
+  if (tid < 30) {
+    const int32_t id = op.ids[cell * kND + tid];
+    Id[tid] = id;
+    Xe[tid] = id > 0 ? (x ? x[id - 1] : 0.0) : op.dvals[-id - 1];
+  }
+  __syncthreads();
+  for (int it = tid; it < (fd.nqb + fd.nqo) * 11; it += blockDim.x) {
+    const int q = it / 11, a = it - q * 11;
+    const bool back = q < fd.nqb;
+    const double* P = back ? fd.xb + ((size_t)f * fd.nqb + q) * 7 : fd.xo + ((size_t)f * fd.nqo + (q - fd.nqb)) * 4;
+    if (a < 10) {
+      double n, g[3];
+      shape(a, P[0], P[1], P[2], n, g);
+      if (back) Nb[q][a] = n;
+      else No[q - fd.nqb][a] = n;
+    } else if (back) {
+      const double wn = P[4] * Nrm[0] + P[5] * Nrm[1] + P[6] * Nrm[2];
+      Wb[q] = -op.rho * 0.5 * (wn - fabs(wn)) * P[3];
+    } else {
+      Wo[q - fd.nqb] = P[3];
+    }
+  }
+  __syncthreads();
+  if (tid < 100) {
+    const int a = tid / 10, bn = tid - a * 10;
+    double sb = 0.0, so = 0.0;
+    for (int q = 0; q < fd.nqb; ++q) sb += Wb[q] * Nb[q][a] * Nb[q][bn];
+    for (int q = 0; q < fd.nqo; ++q) so += Wo[q] * No[q][a] * No[q][bn];
+    Mb[a][bn] = sb;
+    Mo[a][bn] = so;
+  } else if (tid < 110) {
+    const int a = tid - 100;
+    double l = 0.0;
+    for (int q = 0; q < fd.nqo; ++q) l += Wo[q] * No[q][a];
+    Lo[a] = l;
+  }
+  __syncthreads();
+  if (WANT_K) {
+    const int32_t* pos = op.nzpos + cell * kND * kND;
+    for (int e = tid; e < 900; e += blockDim.x) {
+      const int jj = e / 30, ii = e - jj * 30;
+      const int32_t p = pos[ii + kND * jj];
+      if (p <= 0) continue;
+      const int i = ii / 10, a = ii - i * 10, j = jj / 10, bn = jj - j * 10;
+      nzval[p - 1] += (i == j ? Mb[a][bn] : 0.0) + fd.ZA * Nrm[i] * Nrm[j] * Mo[a][bn];
+    }
+  }
+  if (WANT_R && tid < 30 && Id[tid] > 0) {
+    const int i = tid / 10, a = tid - i * 10;
+    double r = fd.Pc * Nrm[i] * Lo[a];  // minus the load -Pc n_i int Na
+    for (int jj = 0; jj < 30; ++jj) {
+      const int j = jj / 10, bn = jj - j * 10;
+      r += ((i == j ? Mb[a][bn] : 0.0) + fd.ZA * Nrm[i] * Nrm[j] * Mo[a][bn]) * Xe[jj];
+    }
+    b[Id[tid] - 1] += r;
+  }
+}
+
 }  // namespace
 }  // namespace hfk
 
@@ -275,7 +372,14 @@ struct hfk_oseen {
   bool pattern_built = false, has_fields = false;
   std::vector<int64_t> colptr, rowval;
   std::vector<int32_t> nzpos;
-  DevBuf ids, coords, qp, fields, dvals, cells_d, nzpos_d, x_stage, y_stage, nz_stage;
+  DevBuf ids, coords, qp, fields, lap, dvals, cells_d, nzpos_d, x_stage, y_stage, nz_stage;
+  bool has_lap = false;
+  // outlet faces, ordered by the colour of their cell
+  int64_t nfaces = 0;
+  int nqo = 0, nqb = 0;
+  double Pc = 0, ZA = 0;
+  std::vector<int64_t> face_start;  // faces of colour c: faces_d[face_start[c] .. face_start[c+1])
+  DevBuf face_cell, face_normal, face_xo, face_xb, faces_d;
   hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
   OseenDev dev() const {
     OseenDev d;
@@ -288,8 +392,21 @@ struct hfk_oseen {
     d.coords = coords.as<double>();
     d.qp = qp.as<double>();
     d.fields = fields.as<double>();
+    d.lap = has_lap ? lap.as<double>() : nullptr;
     d.dvals = dvals.as<double>();
     d.nzpos = nzpos_d.as<int32_t>();
+    return d;
+  }
+  OseenFaceDev face_dev() const {
+    OseenFaceDev d;
+    d.nqo = nqo;
+    d.nqb = nqb;
+    d.Pc = Pc;
+    d.ZA = ZA;
+    d.cell = face_cell.as<int32_t>();
+    d.normal = face_normal.as<double>();
+    d.xo = face_xo.as<double>();
+    d.xb = face_xb.as<double>();
     return d;
   }
 };
@@ -326,6 +443,19 @@ int oseen_run(hfk_oseen* op, const double* x, double* b, double* nzval, hipStrea
     else if (b) hipLaunchKernelGGL((k_oseen<false, true>), g, t, 0, s, d, cells, x, b, nzval);
     else hipLaunchKernelGGL((k_oseen<true, false>), g, t, 0, s, d, cells, x, b, nzval);
     HIPCHK(op->ctx, hipGetLastError());
+  }
+  if (op->nfaces > 0) {
+    const OseenFaceDev fd = op->face_dev();
+    for (size_t c = 0; c + 1 < op->face_start.size(); ++c) {
+      const int64_t n = op->face_start[c + 1] - op->face_start[c];
+      if (n <= 0) continue;
+      const int32_t* faces = op->faces_d.as<int32_t>() + op->face_start[c];
+      const dim3 g((unsigned)n), t(128);  // one work-group per outlet face of the colour
+      if (b && nzval) hipLaunchKernelGGL((k_oseen_face<true, true>), g, t, 0, s, d, fd, faces, x, b, nzval);
+      else if (b) hipLaunchKernelGGL((k_oseen_face<false, true>), g, t, 0, s, d, fd, faces, x, b, nzval);
+      else hipLaunchKernelGGL((k_oseen_face<true, false>), g, t, 0, s, d, fd, faces, x, b, nzval);
+      HIPCHK(op->ctx, hipGetLastError());
+    }
   }
   return HFK_OK;
 }
@@ -399,6 +529,68 @@ int hfk_oseen_set_fields(hfk_oseen* op, const double* fields) {
   HIPCHK(op->ctx, op->fields.alloc(n * sizeof(double)));
   HIPCHK(op->ctx, hipMemcpy(op->fields.p, fields, n * sizeof(double), hipMemcpyHostToDevice));
   op->has_fields = true;
+  return HFK_OK;
+}
+
+int hfk_oseen_set_laplacian(hfk_oseen* op, const double* lap_points) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (!lap_points) {
+    op->has_lap = false;
+    return HFK_OK;
+  }
+  const size_t n = (size_t)op->ncells * op->np * 3;
+  HIPCHK(op->ctx, op->lap.alloc(n * sizeof(double)));
+  HIPCHK(op->ctx, hipMemcpy(op->lap.p, lap_points, n * sizeof(double), hipMemcpyHostToDevice));
+  op->has_lap = true;
+  return HFK_OK;
+}
+
+int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell, const double* normals, int nqo,
+                         const double* xo, int nqb, const double* xb, double Pc, double ZA) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (nfaces == 0) {
+    op->nfaces = 0;
+    return HFK_OK;
+  }
+  if (nfaces < 0 || !face_cell || !normals || !xo || !xb || nqo < 1 || nqb < 1 || nqo > kMaxFacePoints ||
+      nqb > kMaxFacePoints)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: invalid outlet faces (sizes, 1..32 points per measure)");
+  for (int64_t f = 0; f < nfaces; ++f)
+    if (face_cell[f] < 0 || face_cell[f] >= op->ncells)
+      return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: outlet face cell out of range");
+  // faces ordered by the colour of their cell
+  const size_t nc = op->color_start.size() - 1;
+  std::vector<int64_t> start(nc + 1, 0);
+  for (int64_t f = 0; f < nfaces; ++f) start[op->color[face_cell[f]]]++;
+  for (size_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+  std::vector<int64_t> fill(start.begin(), start.end() - 1);
+  std::vector<int32_t> order((size_t)nfaces);
+  for (int64_t f = 0; f < nfaces; ++f) order[fill[op->color[face_cell[f]] - 1]++] = (int32_t)f;
+  HIPCHK(op->ctx, op->face_cell.alloc((size_t)nfaces * sizeof(int32_t)));
+  HIPCHK(op->ctx, hipMemcpy(op->face_cell.p, face_cell, (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, op->face_normal.alloc((size_t)nfaces * 3 * sizeof(double)));
+  HIPCHK(op->ctx, hipMemcpy(op->face_normal.p, normals, (size_t)nfaces * 3 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, op->face_xo.alloc((size_t)nfaces * nqo * 4 * sizeof(double)));
+  HIPCHK(op->ctx, hipMemcpy(op->face_xo.p, xo, (size_t)nfaces * nqo * 4 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, op->face_xb.alloc((size_t)nfaces * nqb * 7 * sizeof(double)));
+  HIPCHK(op->ctx, hipMemcpy(op->face_xb.p, xb, (size_t)nfaces * nqb * 7 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, op->faces_d.alloc((size_t)nfaces * sizeof(int32_t)));
+  HIPCHK(op->ctx, hipMemcpy(op->faces_d.p, order.data(), (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice));
+  op->face_start = start;
+  op->nfaces = nfaces;
+  op->nqo = nqo;
+  op->nqb = nqb;
+  op->Pc = Pc;
+  op->ZA = ZA;
+  return HFK_OK;
+}
+
+int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  op->Pc = Pc;
   return HFK_OK;
 }
 

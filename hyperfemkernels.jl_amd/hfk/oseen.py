@@ -1,8 +1,9 @@
 """The Oseen-BDF2 step on tetrahedra (Taylor-Hood P2-P1, SUPG/PSPG/LSIC) as an operator with the reference's
 Algebra interface: residual!, sparsity_pattern / nonzero_positions, allocate_jacobian, jacobian!
-(src/assembly.jl:768-933), over libhfk's hfk_oseen_* entry points.  The weak form is the volume part of
-test/common.jl:165-228; the point data (advecting velocity w, stabilisation velocity un, history f) come from the
-host, as Gridap's FE-function leaves would.
+(src/assembly.jl:768-933), over libhfk's hfk_oseen_* entry points.  The weak form is that of test/common.jl:165-228:
+volume terms, the elementwise Laplacian of un in the stabilisation load (``set_laplacian``) and the outlet terms
+(``set_outlet``: backflow, Robin, Windkessel pressure).  The point data (advecting velocity w, stabilisation
+velocity un, history f, Laplacian) come from the host, as Gridap's FE-function leaves would.
 """
 from __future__ import annotations
 
@@ -64,6 +65,48 @@ class OseenOperator:
         and the BDF history is f = c0 w (c1 un - c2 un-1 with c1 - c2 = c0)."""
         wq = np.asarray(wfun(self.point_coordinates().reshape(-1, 3))).reshape(self.space.ncells, self.np, 3)
         self.set_fields(wq, wq, self.c0 * wq)
+
+    def set_laplacian(self, lap=None):
+        """Elementwise Laplacian of un at the points, (ncells, np, 3) or per cell (ncells, 3) (``tetmesh.
+        p2_laplacian``); None removes the term from the stabilisation load."""
+        if lap is None:
+            check(self.lib.hfk_oseen_set_laplacian(self.handle, None), self.ctx.handle)
+            return
+        lap = np.asarray(lap, dtype=np.float64)
+        if lap.shape == (self.space.ncells, 3):
+            lap = np.repeat(lap[:, None, :], self.np, axis=1)
+        if lap.shape != (self.space.ncells, self.np, 3):
+            raise ValueError("HyperFEMKernels: DimensionMismatch: Laplacian point data")
+        lap = np.ascontiguousarray(lap)
+        check(self.lib.hfk_oseen_set_laplacian(self.handle, ptr(lap, C.c_double)), self.ctx.handle)
+
+    def set_outlet(self, faces, w=None, Pc: float = 0.0, ZA: float = 0.0):
+        """Outlet terms on ``faces`` (``tetmesh.OutletFaces``): backflow -rho negpart(w.n) u.v, Robin ZA (u.n)(v.n),
+        pressure load -Pc v.n (test/common.jl:212-213, 219).  ``w``: the advecting velocity at the backflow points,
+        (nfaces, nqb, 3) or a callable of the physical coordinates; None: no backflow.  ``faces=None`` removes the
+        terms."""
+        if faces is None:
+            check(self.lib.hfk_oseen_set_outlet(self.handle, 0, None, None, 1, None, 1, None, 0.0, 0.0),
+                  self.ctx.handle)
+            return
+        nf, nqb = faces.xb.shape[0], faces.xb.shape[1]
+        if w is None:
+            wb = np.zeros((nf, nqb, 3))
+        elif callable(w):
+            wb = np.asarray(w(faces.xb_phys.reshape(-1, 3)), dtype=np.float64).reshape(nf, nqb, 3)
+        else:
+            wb = np.asarray(w, dtype=np.float64).reshape(nf, nqb, 3)
+        xb = np.ascontiguousarray(np.concatenate([faces.xb, wb], axis=2))
+        xo = np.ascontiguousarray(faces.xo, dtype=np.float64)
+        cell = np.ascontiguousarray(faces.cell, dtype=np.int32)
+        nrm = np.ascontiguousarray(faces.normal, dtype=np.float64)
+        check(self.lib.hfk_oseen_set_outlet(self.handle, nf, ptr(cell, C.c_int32), ptr(nrm, C.c_double),
+                                            xo.shape[1], ptr(xo, C.c_double), nqb, ptr(xb, C.c_double), float(Pc),
+                                            float(ZA)), self.ctx.handle)
+        self.outlet = (faces, xb)
+
+    def set_outlet_pressure(self, Pc: float):
+        check(self.lib.hfk_oseen_set_outlet_pressure(self.handle, float(Pc)), self.ctx.handle)
 
     def info(self) -> dict:
         a = np.zeros(6, dtype=np.int64)

@@ -135,6 +135,104 @@ def tet_quadrature_deg5():
     return np.ascontiguousarray(bary[:, 1:]), np.array(wts)
 
 
+def tri_quadrature(degree: int):
+    """Collapsed Gauss rule on the reference triangle {s, t >= 0, s + t <= 1}: Gauss-Legendre in a and Gauss-Jacobi
+    with weight (1 - b) in b, s = a (1 - b), t = b; n = floor(degree / 2) + 1 points per direction, exact for total
+    degree <= 2n - 1.  Points (n^2, 2) and weights summing to 1/2."""
+    from scipy.special import roots_jacobi
+
+    n = degree // 2 + 1
+    ga, wa = np.polynomial.legendre.leggauss(n)
+    a, wa = (ga + 1.0) / 2.0, wa / 2.0
+    tb, wb = roots_jacobi(n, 1.0, 0.0)
+    bb, wb = (tb + 1.0) / 2.0, wb / 4.0
+    pts = np.array([(ai * (1.0 - bj), bj) for bj in bb for ai in a])
+    wts = np.array([wai * wbj for wbj in wb for wai in wa])
+    return pts, wts
+
+
+REF_VERTS = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+
+
+@dataclass
+class OutletFaces:
+    """Boundary faces of one plane with their quadratures, in the layout of hfk_oseen_set_outlet.  ``xo``/``xb``:
+    (nfaces, nq, 4) = reference coordinates of the point in the face's cell and the weight dS; ``xb_phys``: the
+    physical coordinates of the dGb points (where the host evaluates w)."""
+    cell: np.ndarray
+    normal: np.ndarray
+    xo: np.ndarray
+    xb: np.ndarray
+    xb_phys: np.ndarray
+    area: float
+
+
+def _face_points(space, cell, fv, tri_pts, tri_w):
+    """Reference coordinates in the cell, physical coordinates and weights dS of the triangle rule mapped onto the
+    faces with local vertices fv (nfaces, 3)."""
+    R = REF_VERTS[fv]  # (nf, 3, 3)
+    xi = R[:, None, 0, :] + tri_pts[None, :, 0, None] * (R[:, None, 1, :] - R[:, None, 0, :]) + \
+        tri_pts[None, :, 1, None] * (R[:, None, 2, :] - R[:, None, 0, :])
+    X = space.coords[cell]  # (nf, 4, 3)
+    P = X[np.arange(len(cell))[:, None], fv]  # face vertices (nf, 3, 3)
+    area = 0.5 * np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1)
+    J = X[:, 1:, :] - X[:, :1, :]
+    phys = X[:, None, 0, :] + np.einsum("fqd,fdi->fqi", xi, J)
+    dS = 2.0 * area[:, None] * tri_w[None, :]
+    return xi, phys, dS, area
+
+
+def outlet_faces(space, degree_o: int = 4, degree_b: int = 5, axis: int = 0, side: int = 1) -> OutletFaces:
+    """The faces of the tetrahedra on the box plane x_axis = (domain max if side else min), with the triangle rules
+    of degree degree_o (Robin and pressure terms, Measure(Γo, 4)) and degree_b (backflow, Measure(Γo, 5)),
+    test/common.jl:197."""
+    plane = space.domain[2 * axis + (1 if side else 0)]
+    h = (space.domain[2 * axis + 1] - space.domain[2 * axis]) / space.n[axis]
+    on = np.abs(space.coords[:, :, axis] - plane) < 1e-9 * max(1.0, abs(plane), h)
+    cell = np.nonzero(on.sum(axis=1) == 3)[0]
+    fv = np.array([np.nonzero(r)[0] for r in on[cell]], dtype=np.int64).reshape(-1, 3)
+    opp = np.array([np.nonzero(~r)[0][0] for r in on[cell]], dtype=np.int64)
+    X = space.coords[cell]
+    P = X[np.arange(len(cell))[:, None], fv]
+    nrm = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    inward = np.einsum("fi,fi->f", X[np.arange(len(cell)), opp] - P[:, 0], nrm)
+    nrm *= np.where(inward > 0, -1.0, 1.0)[:, None]
+    po, wo = tri_quadrature(degree_o)
+    pb, wb = tri_quadrature(degree_b)
+    xio, _, dSo, area = _face_points(space, cell, fv, po, wo)
+    xib, physb, dSb, _ = _face_points(space, cell, fv, pb, wb)
+    xo = np.concatenate([xio, dSo[..., None]], axis=2)
+    xb = np.concatenate([xib, dSb[..., None]], axis=2)
+    return OutletFaces(cell=cell.astype(np.int32), normal=np.ascontiguousarray(nrm), xo=np.ascontiguousarray(xo),
+                       xb=np.ascontiguousarray(xb), xb_phys=physb, area=float(area.sum()))
+
+
+def element_velocity(space, x, dvals) -> np.ndarray:
+    """Nodal velocity values (ncells, 3, 10) of the cells from the free values x and the Dirichlet values."""
+    ids = np.asarray(space.cell_dofs)[:, :30]
+    x, dvals = np.asarray(x), np.asarray(dvals)
+    ue = np.where(ids > 0, x[np.maximum(ids, 1) - 1], dvals[np.maximum(-ids, 1) - 1])
+    return ue.reshape(-1, 3, 10)
+
+
+def p2_laplacian(space, x, dvals) -> np.ndarray:
+    """Elementwise Laplacian (ncells, 3) of the P2 velocity with free values x and Dirichlet values dvals: constant
+    on each affine cell (Laplacians of the shape functions: 4 |grad l_a|^2 at vertex a, 8 grad l_a . grad l_b at the
+    midpoint of edge (a, b)).  The reference obtains the same field by interpolating grad(un) . e_k into a
+    discontinuous P1 space, which holds the (linear) gradient of a P2 field exactly, and taking its divergence
+    (test/common.jl:191-195)."""
+    GL = np.array([[-1.0, -1.0, -1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    J = np.transpose(space.coords[:, 1:, :] - space.coords[:, :1, :], (0, 2, 1))  # J[c][i][d]
+    G = np.einsum("kd,cdi->cki", GL, np.linalg.inv(J))  # physical gradients of the barycentrics (ncells, 4, 3)
+    lapN = np.zeros((space.ncells, 10))
+    for a in range(4):
+        lapN[:, a] = 4.0 * np.einsum("ci,ci->c", G[:, a], G[:, a])
+    for e, (a, b) in enumerate(EDGES):
+        lapN[:, 4 + e] = 8.0 * np.einsum("ci,ci->c", G[:, a], G[:, b])
+    return np.einsum("cka,ca->ck", element_velocity(space, x, dvals), lapN)
+
+
 def divfree_field(seed: int, modes: int = 12, kmax: int = 2):
     """w(x) = sum_m (a_m x k_m/|k_m|) cos(2 pi k_m . x + phi_m): k_m uniform in {-kmax..kmax}^3 without 0,
     a_m ~ N(0, I), phi_m ~ U[0, 2 pi), all from SplitMix64(seed); divergence free term by term.  Returns the

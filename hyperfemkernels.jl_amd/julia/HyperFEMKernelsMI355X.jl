@@ -263,6 +263,28 @@ set_fields!(op::DeviceOseen, fields::Array{Float64,3}) =
     check(ccall((:hfk_oseen_set_fields, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, fields), op.ctx.ptr)
 set_dirichlet!(op::DeviceOseen, dvals::Vector{Float64}) =
     check(ccall((:hfk_oseen_set_dirichlet, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, dvals), op.ctx.ptr)
+"Elementwise Laplacian of uⁿ at the points (3 × np × ncells), the `lapn` of the stabilisation load (test/common.jl:191-195); `nothing` removes it."
+set_laplacian!(op::DeviceOseen, lap::Union{Nothing,Array{Float64,3}}) =
+    check(ccall((:hfk_oseen_set_laplacian, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr,
+                lap === nothing ? C_NULL : lap), op.ctx.ptr)
+
+"""
+    set_outlet!(op, face_cell, normals, xo, xb, Pc, ZA)
+
+Outlet terms (test/common.jl:212-213, 219) on the faces of `BoundaryTriangulation(model, tags="outlet")`:
+`face_cell` 0-based cells, `normals` 3 × nfaces, `xo` 4 × nqo × nfaces (reference coordinates in the cell and dS of
+`Measure(Γo, 4)`), `xb` 7 × nqb × nfaces (the same for `Measure(Γo, 5)` with w appended).
+"""
+set_outlet!(op::DeviceOseen, face_cell::Vector{Int32}, normals::Matrix{Float64}, xo::Array{Float64,3},
+            xb::Array{Float64,3}, Pc::Real, ZA::Real) =
+    check(ccall((:hfk_oseen_set_outlet, libhfk), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cdouble, Cdouble),
+                op.ptr, length(face_cell), face_cell, normals, size(xo, 2), xo, size(xb, 2), xb, Pc, ZA), op.ctx.ptr)
+
+"The Windkessel pressure `Pc[]` of the next step."
+set_outlet_pressure!(op::DeviceOseen, Pc::Real) =
+    check(ccall((:hfk_oseen_set_outlet_pressure, libhfk), Cint, (Ptr{Cvoid}, Cdouble), op.ptr, Pc), op.ctx.ptr)
+
 function residual!(b::Vector{Float64}, op::DeviceOseen, x::Vector{Float64})
     check(ccall((:hfk_oseen_residual, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op.ptr, x, b), op.ctx.ptr)
     b

@@ -251,8 +251,9 @@ int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const 
                          double rtol, double atol, int maxiter, int restart, int* status, double* rnorm, void* stream);
 
 /* ------------------------------------------------------------------------------------------------------------
- * Oseen-BDF2 step on tetrahedra, Taylor-Hood P2-P1 with SUPG/PSPG/LSIC (the weak form of test/common.jl:165-228,
- * volume terms): coloured FP64 assembly of the residual and the CSC Jacobian, as residual!/jacobian!
+ * Oseen-BDF2 step on tetrahedra, Taylor-Hood P2-P1 with SUPG/PSPG/LSIC (the weak form of test/common.jl:165-228:
+ * volume terms, elementwise Laplacian in the stabilisation load, outlet backflow/Robin/pressure terms): coloured FP64
+ * assembly of the residual and the CSC Jacobian, as residual!/jacobian!
  * (src/assembly.jl:909-927) do for that form, with the reference's sparsity (src/assembly.jl:773-899) and colouring
  * (src/coloring.jl:4-28) over the 34 dofs of each cell (velocity and pressure blocks all coupled).
  *   cell_dofs  ncells x 34 signed ids: [c*10 + a] velocity component c at P2 node a (vertices 0..3, then the
@@ -271,6 +272,17 @@ int hfk_oseen_destroy(hfk_oseen* op);
 int hfk_oseen_set_fields(hfk_oseen* op, const double* fields);
 /* refresh_dirichlet_values! (src/assembly.jl:393-410) */
 int hfk_oseen_set_dirichlet(hfk_oseen* op, const double* dvals);
+/* The elementwise Laplacian lapn of un at the points, ncells x np x 3 (test/common.jl:191-195: rk = rho f + mu lapn
+ * in the SUPG/PSPG load); NULL removes the term. */
+int hfk_oseen_set_laplacian(hfk_oseen* op, const double* lap_points);
+/* Outlet faces (test/common.jl:197-200, 212-213, 219): a += -rho negpart(w.n) u.v dGb + ZA (u.n)(v.n) dGo and
+ * l += -Pc v.n dGo.  Per face: its cell (0-based), the outward unit normal (nfaces x 3), the nqo points of dGo as
+ * [face][q][4] = reference coordinates in the cell (3) and weight dS, the nqb points of dGb as [face][q][7] =
+ * reference coordinates (3), dS, w at the point (3).  1..32 points per measure; nfaces = 0 removes the terms. */
+int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell, const double* normals, int nqo,
+                         const double* xo, int nqb, const double* xb, double Pc, double ZA);
+/* The outlet pressure Pc alone (a number of the integrand, updated every step by a Windkessel model). */
+int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc);
 /* info[6]: nfree, ndir, ncells, np, ncolors, nnz (-1 before the pattern) */
 int hfk_oseen_info(hfk_oseen* op, int64_t* info);
 int hfk_oseen_colors(hfk_oseen* op, int32_t* colors);

@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE ONLY -- the oracle's restatement of the Oseen-BDF2 step on tetrahedra (Taylor-Hood P2-P1,
 SUPG/PSPG/LSIC), used by tests/ to check libhfk's hfk_oseen_* path.  Never imported by the product.
 
-It evaluates the weak form of test/common.jl:165-228 (volume terms) literally, with tensor algebra on every
+It evaluates the weak form of test/common.jl:165-228 (volume terms, the Laplacian in the stabilisation load, the
+outlet face terms) literally, with tensor algebra on every
 (trial, test) pair of basis functions -- grad u with Gridap's convention grad u[J,c] = d u_c/d x_J, eps(u),
 (grad u)' . w, div -- instead of the per-entry reduced formulas of the device kernel.  Assembly follows the
 reference: residual! (src/assembly.jl:909-916), sparsity as the sorted unique keys (col << 32) | row of the free
@@ -34,9 +35,10 @@ def basis(qp):
     return N, G
 
 
-def element_terms(coords, qp, qw, fields, rho, mu, c0):
+def element_terms(coords, qp, qw, fields, rho, mu, c0, lap=None):
     """Per cell, the matrices of every term of a((u,p),(v,q)) (dict name -> (ncells, 34 test, 34 trial)) and the
-    load vector (ncells, 34).  fields (ncells, np, 9): w, un, f."""
+    load vector (ncells, 34).  fields (ncells, np, 9): w, un, f; lap (ncells, np, 3): the Laplacian of un in the
+    stabilisation load rk = rho f + mu lap (test/common.jl:216-218), or None."""
     coords = np.asarray(coords)
     nc = coords.shape[0]
     Nr, Gr = basis(qp)
@@ -79,15 +81,45 @@ def element_terms(coords, qp, qw, fields, rho, mu, c0):
     T["supg"] = np.einsum("cq,cq,cqai,cqbi->cab", dV, ts, conv, Ru) + np.einsum("cq,cq,cqai,cqbi->cab", dV, ts, conv, GP)
     T["pspg"] = -np.einsum("cq,cq,cqai,cqbi->cab", dV, tp, GP, Ru) - np.einsum("cq,cq,cqai,cqbi->cab", dV, tp, GP, GP)
     T["lsic"] = np.einsum("cq,cq,cqa,cqb->cab", dV, tl, div, div)
-    rk = rho * f
+    rk = rho * f if lap is None else rho * f + mu * np.asarray(lap)
     load = (np.einsum("cq,cqi,cqai->ca", dV, rho * f, U) + np.einsum("cq,cq,cqai,cqi->ca", dV, ts, conv, rk)
             - np.einsum("cq,cq,cqai,cqi->ca", dV, tp, GP, rk))
     return T, load
 
 
-def element_system(coords, qp, qw, fields, rho, mu, c0):
-    T, load = element_terms(coords, qp, qw, fields, rho, mu, c0)
+def element_system(coords, qp, qw, fields, rho, mu, c0, lap=None):
+    T, load = element_terms(coords, qp, qw, fields, rho, mu, c0, lap)
     return sum(T.values()), load
+
+
+def outlet_terms(xo, xb, normals, rho, Pc, ZA):
+    """Per outlet face, the terms of test/common.jl:212-213 and 219 over the velocity dofs of its cell:
+    Kf (nfaces, 34, 34) = -rho negpart(w.n) u.v on dGb + ZA (u.n)(v.n) on dGo, and the load Lf (nfaces, 34) =
+    -Pc v.n on dGo.  xo [face][q] = (reference coordinates in the cell, dS); xb the same with w appended."""
+    xo, xb, nrm = np.asarray(xo), np.asarray(xb), np.asarray(normals)
+    nf = xo.shape[0]
+    No = basis(xo[..., :3].reshape(-1, 3))[0][:, :10].reshape(nf, xo.shape[1], 10)
+    Nb = basis(xb[..., :3].reshape(-1, 3))[0][:, :10].reshape(nf, xb.shape[1], 10)
+    # velocity test/trial functions as vectors: U[f, q, (c, a), i] = N_a delta_ci
+    eye = np.eye(3)
+    Uo = np.einsum("fqa,ci->fqcai", No, eye).reshape(nf, xo.shape[1], 30, 3)
+    Ub = np.einsum("fqa,ci->fqcai", Nb, eye).reshape(nf, xb.shape[1], 30, 3)
+    wn = np.einsum("fqi,fi->fq", xb[..., 4:7], nrm)
+    negpart = 0.5 * (wn - np.abs(wn))
+    Kf = np.zeros((nf, 34, 34))
+    Kf[:, :30, :30] = (np.einsum("fq,fqai,fqbi->fab", -rho * negpart * xb[..., 3], Ub, Ub) +
+                       ZA * np.einsum("fq,fqai,fi,fqbj,fj->fab", xo[..., 3], Uo, nrm, Uo, nrm))
+    Lf = np.zeros((nf, 34))
+    Lf[:, :30] = -Pc * np.einsum("fq,fqai,fi->fa", xo[..., 3], Uo, nrm)
+    return Kf, Lf
+
+
+def add_faces(K, load, face_cell, Kf, Lf):
+    """Element matrices and loads with the face terms added to their cells."""
+    K, load = K.copy(), load.copy()
+    np.add.at(K, np.asarray(face_cell), Kf)
+    np.add.at(load, np.asarray(face_cell), Lf)
+    return K, load
 
 
 def element_values(cell_dofs, x, dvals):

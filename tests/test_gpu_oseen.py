@@ -1,8 +1,8 @@
 """GPU parity of the Oseen-BDF2 step on tetrahedra (P2-P1, SUPG/PSPG/LSIC; test/common.jl:165-228) against the
 oracle (tests run as test/test_cfd.jl does against Gridap): sparsity and nzpos bit-exact, colours equal to the
 greedy first fit, residual and CSC Jacobian values within 1e-12 relative, the residual affine in the state
-(r(x) = J x + r(0), the lifted system A = J, b = -r(0)), and the fused residual+Jacobian pass equal to the two
-separate ones.
+(r(x) = J x + r(0), the lifted system A = J, b = -r(0)), the fused residual+Jacobian pass equal to the two
+separate ones, and the full step with the Laplacian in the stabilisation load and the outlet terms.
 """
 import numpy as np
 import pytest
@@ -87,3 +87,41 @@ def test_oseen_fused_and_steady_advection(hip_ctx):
     assert rel(bt.cpu().numpy(), ref_b) < TOL
     assert rel(nt.cpu().numpy(), ref_nz) < TOL
     assert rel(op.residual(np.zeros(sp.nfree), x), ref_b) < TOL
+
+
+@pytest.mark.parametrize("n,bdf2", [((3, 2, 2), True), ((4, 2, 3), False)])
+def test_oseen_full_step_with_outlet_and_laplacian(hip_ctx, n, bdf2):
+    """Every term of the reference's step: volume, the Laplacian of un in the stabilisation load, and the outlet
+    backflow (w random: inflow and outflow points), Robin and pressure terms; then the pressure alone updated."""
+    from oracle import oseen as O
+
+    from hfk.tetmesh import outlet_faces
+
+    print(f"oseen full step n={n} bdf2={bdf2}", flush=True)
+    sp, qp, qw, fields, dv, op, rng = setup(hip_ctx, n, bdf2, seed=11)
+    lap = rng.standard_normal((sp.ncells, len(qw), 3))
+    op.set_laplacian(lap)
+    of = outlet_faces(sp)
+    wb = rng.standard_normal((len(of.cell), of.xb.shape[1], 3))
+    op.set_outlet(of, wb, Pc=0.7, ZA=2.3)
+    K, load = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.05, op.c0, lap=lap)
+    xb = np.concatenate([of.xb, wb], axis=2)
+    Kf, Lf = O.outlet_terms(of.xo, xb, of.normal, 1.0, 0.7, 2.3)
+    Kt, lt = O.add_faces(K, load, of.cell, Kf, Lf)
+    colptr, rowval, nzpos = op.pattern()
+    A = op.jacobian(op.allocate_jacobian())
+    assert rel(A.data, O.jacobian_values(Kt, nzpos, rowval.size)) < TOL
+    x = rng.standard_normal(sp.nfree)
+    b = op.residual(np.zeros(sp.nfree), x)
+    assert rel(b, O.residual(sp, x, dv, Kt, lt)) < TOL
+    assert rel(b - op.residual(np.zeros(sp.nfree), np.zeros(sp.nfree)), A @ x) < 1e-11
+    # the Windkessel pressure of the next step
+    op.set_outlet_pressure(1.9)
+    _, Lf2 = O.outlet_terms(of.xo, xb, of.normal, 1.0, 1.9, 2.3)
+    _, lt2 = O.add_faces(K, load, of.cell, Kf, Lf2)
+    assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, Kt, lt2)) < TOL
+    # terms removed again: the volume step alone
+    op.set_outlet(None)
+    op.set_laplacian(None)
+    K0, l0 = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.05, op.c0)
+    assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, K0, l0)) < TOL

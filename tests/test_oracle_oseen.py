@@ -10,7 +10,12 @@ Each term of the element matrix is checked against what the mathematics fixes, o
 * convection with constant w on a linear field u = A x: rho (A w)_i int N_a;
 * LSIC: a divergence-free linear field is in its kernel; PSPG pressure block: -tp |K| grad M_a . grad M_b;
 * assembly: the sparsity equals the set of coupled pairs; the greedy colours never share a free dof within a
-  colour; the residual is affine, r(x) = J x + r(0).
+  colour; the residual is affine, r(x) = J x + r(0);
+* outlet faces: the triangle rules are exact to their degree; the faces cover the outlet (area, normal); with
+  w.n = -1 the backflow block of each face is the textbook P2 face mass matrix |T|/180 [6, -1, 0, -4, 32, 16]; the
+  Robin block sums to ZA |outlet| and the pressure load to -Pc |outlet|;
+* Laplacian: the P2 elementwise Laplacian of quadratic fields; its load rows in closed form (PSPG
+  -mu tp |K| lap . grad l_a; SUPG zero at the vertices, ts mu lap_i |K| w . (grad l_a + grad l_b) at edge (a, b)).
 """
 import numpy as np
 import pytest
@@ -198,3 +203,133 @@ def test_divfree_field():
     h = 1e-5
     div = sum((w(x + h * e)[:, i] - w(x - h * e)[:, i]) / (2 * h) for i, e in enumerate(np.eye(3)))
     assert np.abs(div).max() < 1e-6 * np.abs(w(x)).max()
+
+
+# ---- outlet terms and the Laplacian of the stabilisation load (test/common.jl:191-219) ----------------------
+
+@pytest.fixture(scope="module")
+def channel():
+    from hfk.tetmesh import outlet_faces
+
+    sp = tet_p2p1_space((3, 2, 2), domain=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0))
+    return sp, outlet_faces(sp)
+
+
+@pytest.mark.parametrize("degree", [4, 5])
+def test_triangle_quadrature(degree):
+    from math import factorial
+
+    from hfk.tetmesh import tri_quadrature
+
+    p, w = tri_quadrature(degree)
+    assert np.all(w > 0) and np.all(p >= 0) and np.all(p.sum(axis=1) <= 1)
+    for i in range(degree + 1):
+        for j in range(degree + 1 - i):
+            exact = factorial(i) * factorial(j) / factorial(i + j + 2)
+            assert (w * p[:, 0] ** i * p[:, 1] ** j).sum() == pytest.approx(exact, rel=1e-13)
+
+
+def test_outlet_faces_geometry(channel):
+    sp, of = channel
+    assert len(of.cell) == 2 * 2 * 2  # two tetrahedra of each of the 2 x 2 boundary boxes
+    assert of.area == pytest.approx(1.0, rel=1e-14) and of.xo[..., 3].sum() == pytest.approx(1.0, rel=1e-14)
+    assert np.allclose(of.normal, [1.0, 0.0, 0.0], atol=1e-15)
+    X = sp.coords[of.cell]
+    phys = X[:, None, 0, :] + np.einsum("fqd,fdi->fqi", of.xo[..., :3], X[:, 1:, :] - X[:, :1, :])
+    assert np.abs(phys[..., 0] - 2.0).max() < 1e-14
+
+
+def face_nodes(sp, of):
+    """Cell-local P2 nodes on each outlet face: the 3 vertices and the 3 edges joining them."""
+    out = []
+    for f, c in enumerate(of.cell):
+        on = np.abs(sp.coords[c, :, 0] - 2.0) < 1e-12
+        verts = [k for k in range(4) if on[k]]
+        edges = [4 + e for e, (a, b) in enumerate(EDGES) if on[a] and on[b]]
+        out.append((verts, edges))
+    return out
+
+
+def test_outlet_backflow_is_the_p2_face_mass_matrix(channel):
+    sp, of = channel
+    nf, nqb = of.xb.shape[:2]
+    inflow = np.concatenate([of.xb, np.tile([-1.0, 0.0, 0.0], (nf, nqb, 1))], axis=2)  # w.n = -1
+    Kf, Lf = O.outlet_terms(of.xo, inflow, of.normal, 1.3, 0.0, 0.0)
+    assert np.abs(Lf).max() == 0.0
+    for f, (verts, edges) in enumerate(face_nodes(sp, of)):
+        P = sp.coords[of.cell[f]][verts]
+        area = 0.5 * np.linalg.norm(np.cross(P[1] - P[0], P[2] - P[0]))
+        nodes = verts + edges
+        M = np.zeros((6, 6))
+        for r, a in enumerate(nodes):
+            for s_, b in enumerate(nodes):
+                if a < 4 and b < 4:
+                    M[r, s_] = 6 if a == b else -1
+                elif a >= 4 and b >= 4:
+                    M[r, s_] = 32 if a == b else 16
+                else:
+                    v, e = (a, b - 4) if a < 4 else (b, a - 4)
+                    M[r, s_] = 0 if v in EDGES[e] else -4
+        for c in range(3):
+            idx = [c * 10 + a for a in nodes]
+            assert np.allclose(Kf[f][np.ix_(idx, idx)], 1.3 * area * M / 180, rtol=1e-12, atol=1e-15)
+        others = [k for k in range(30) if k % 10 not in nodes]
+        assert np.abs(Kf[f][others]).max() < 1e-15
+    outflow = np.concatenate([of.xb, np.tile([1.0, 0.0, 0.0], (nf, nqb, 1))], axis=2)
+    Kf, _ = O.outlet_terms(of.xo, outflow, of.normal, 1.3, 0.0, 0.0)
+    assert np.abs(Kf).max() == 0.0
+
+
+def test_outlet_robin_and_pressure_totals(channel):
+    sp, of = channel
+    nf, nqb = of.xb.shape[:2]
+    xb = np.concatenate([of.xb, np.zeros((nf, nqb, 3))], axis=2)
+    Kf, Lf = O.outlet_terms(of.xo, xb, of.normal, 1.0, 0.7, 2.3)
+    # sum_a N_a = 1 on a face: the x-x block sums to ZA x area, the load to -Pc x area; n has no y, z part
+    assert Kf[:, :10, :10].sum() == pytest.approx(2.3 * of.area, rel=1e-13)
+    assert np.abs(Kf[:, 10:30, :]).max() == 0.0 and np.abs(Kf[:, :, 10:30]).max() == 0.0
+    assert Lf[:, :10].sum() == pytest.approx(-0.7 * of.area, rel=1e-13)
+    assert np.abs(Lf[:, 10:]).max() == 0.0
+
+
+def test_p2_laplacian_quadratics():
+    from hfk.tetmesh import p2_laplacian
+
+    sp = tet_p2p1_space((2, 3, 2), domain=(0.0, 1.0, 0.0, 0.8, 0.0, 0.6))
+    for fun, lap in ((lambda X: np.stack([X[:, 0] ** 2, X[:, 1] * X[:, 2], X[:, 0] ** 2 + X[:, 1] ** 2], 1), [2, 0, 4]),
+                     (lambda X: np.stack([X[:, 0] * X[:, 1], X[:, 2] ** 2 - X[:, 0], 3 * X[:, 1] ** 2], 1), [0, 2, 6])):
+        x = np.zeros(sp.nfree)
+        x[:sp.nfree_u] = fun(sp.node_xyz[sp.free_node_of]).reshape(-1)
+        assert np.abs(p2_laplacian(sp, x, sp.dirichlet_values(fun)) - lap).max() < 1e-11
+
+
+def test_laplacian_load_closed_forms(mesh):
+    sp, qp, qw = mesh
+    npt = len(qw)
+    lap = np.array([0.4, -1.1, 0.7])
+    w = np.array([0.5, -1.0, 2.0])
+    un = np.array([0.3, 0.4, 1.2])
+    rho, mu = 1.3, 0.07
+    fields = np.zeros((sp.ncells, npt, 9))
+    fields[:, :, 3:6] = un
+    L = np.tile(lap, (sp.ncells, npt, 1))
+    _, l0 = O.element_terms(sp.coords, qp, qw, fields, rho, mu, 20.0, lap=L)
+    vol = sp.volumes()
+    h = (6 * np.sqrt(2) * vol) ** (1 / 3)
+    ts = ((2 * np.linalg.norm(un) / h) ** 2 + 9 * (4 * mu / (rho * h ** 2)) ** 2) ** -0.5
+    G = np.zeros((sp.ncells, 4, 3))
+    for c in range(sp.ncells):
+        G[c] = O.GL @ np.linalg.inv((sp.coords[c, 1:] - sp.coords[c, 0]).T)
+    # w = 0, f = 0: no SUPG; the PSPG rows -tp int grad M_a . (mu lap) = -mu tp |K| lap . grad l_a
+    assert np.abs(l0[:, :30]).max() == 0.0
+    assert np.allclose(l0[:, 30:], -mu * (ts / rho)[:, None] * vol[:, None] * (G @ lap), rtol=1e-12)
+    fields[:, :, 0:3] = w
+    _, l1 = O.element_terms(sp.coords, qp, qw, fields, rho, mu, 20.0, lap=L)
+    # SUPG rows ts int (grad N_a . w) mu lap_i: zero at the vertices (int grad N_a = 0), ts mu lap_i |K| w.(grad l_a
+    # + grad l_b) at the midpoint of edge (a, b)
+    for i in range(3):
+        rows = l1[:, i * 10:(i + 1) * 10]
+        assert np.abs(rows[:, :4]).max() < 1e-13 * np.abs(rows).max()
+        for e, (a, b) in enumerate(EDGES):
+            ref = ts * mu * lap[i] * vol * ((G[:, a] + G[:, b]) @ w)
+            assert np.allclose(rows[:, 4 + e], ref, rtol=1e-11, atol=1e-15)
