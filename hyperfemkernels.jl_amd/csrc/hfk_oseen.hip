@@ -98,9 +98,19 @@ __device__ __forceinline__ void shape(int f, double xi, double et, double ze, do
 // residual rows K x_e - l.
 constexpr int kVV = 100, kVP = 40, kPV = 40, kPP = 16, kPairs = kVV + kVP + kPV + kPP;  // 196 pair threads
 
-template <bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, const double* __restrict__ x,
-                                               double* __restrict__ b, double* __restrict__ nzval) {
+// Cell of a work-group: the cells of one colour (cells != null), or, in the single-launch mode (cells == null),
+// every cell with XCD x (work-groups b = x mod 8) taking the contiguous block [x chunk, (x + 1) chunk) in mesh
+// order, so that neighbouring cells, which add into the same CSC lines, meet in one L2.  -1: idle.
+__device__ __forceinline__ int64_t work_item(const int32_t* cells, int64_t n, int64_t chunk) {
+  if (cells) return cells[blockIdx.x];
+  const int64_t c = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+  return c < n ? c : -1;
+}
+
+template <bool WANT_K, bool WANT_R, bool ATOMIC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, int64_t chunk,
+                                               const double* __restrict__ x, double* __restrict__ b,
+                                               double* __restrict__ nzval) {
   __shared__ double Sh[kMaxPoints][14][5];  // per point and shape function: N, grad N (3), w . grad N
   __shared__ double Pq[kMaxPoints][10];     // per point: dV, ts, tp, tl, f (3), lap (3)
   __shared__ double Ke[kND][kND + 1];       // element matrix [test][trial]
@@ -108,7 +118,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ double Jit[9], Cm[2];
   __shared__ int32_t Id[kND];
   const int tid = threadIdx.x;
-  const int64_t cell = cells[blockIdx.x];
+  const int64_t cell = work_item(cells, op.ncells, chunk);
+  if (cell < 0) return;  // uniform over the work-group, before any barrier
   const int np = op.np;
   if (tid == 0) {
     const double* X = op.coords + cell * 12;
@@ -259,14 +270,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int e = tid; e < kND * kND; e += blockDim.x) {
       const int jj = e / kND, ii = e - jj * kND;
       const int32_t p = pos[ii + kND * jj];
-      if (p > 0) nzval[p - 1] += Ke[ii][jj];
+      if (p <= 0) continue;
+      if (ATOMIC) unsafeAtomicAdd(nzval + p - 1, Ke[ii][jj]);  // hardware FP64 add in L2
+      else nzval[p - 1] += Ke[ii][jj];
     }
   }
   if (WANT_R && tid < kND && Id[tid] > 0) {
     double r = -Ld[tid];
 #pragma unroll
     for (int jj = 0; jj < kND; ++jj) r += Ke[tid][jj] * Xe[jj];
-    b[Id[tid] - 1] += r;
+    if (ATOMIC) unsafeAtomicAdd(b + Id[tid] - 1, r);
+    else b[Id[tid] - 1] += r;
   }
 }
 
@@ -288,15 +302,16 @@ struct OseenFaceDev {
 //   Mb_ab = sum_dGb dS (-rho negpart(w.n)) Na Nb,   Mo_ab = sum_dGo dS Na Nb,
 // and the load -Pc n_i sum_dGo dS Na, added to the CSC values through the cell's nzpos and to the residual rows.
 // Velocity shape functions vanish on the faces they are not attached to, so all 10 nodes are summed.
-template <bool WANT_K, bool WANT_R>
+template <bool WANT_K, bool WANT_R, bool ATOMIC>
 __global__ __launch_bounds__(128) void k_oseen_face(OseenDev op, OseenFaceDev fd, const int32_t* __restrict__ faces,
-                                                    const double* __restrict__ x, double* __restrict__ b,
-                                                    double* __restrict__ nzval) {
+                                                    int64_t nfaces, int64_t chunk, const double* __restrict__ x,
+                                                    double* __restrict__ b, double* __restrict__ nzval) {
   __shared__ double Nb[kMaxFacePoints][10], No[kMaxFacePoints][10], Wb[kMaxFacePoints], Wo[kMaxFacePoints];
   __shared__ double Mb[10][10], Mo[10][10], Lo[10], Xe[30], Nrm[3];
   __shared__ int32_t Id[30];
   const int tid = threadIdx.x;
-  const int64_t f = faces[blockIdx.x];
+  const int64_t f = work_item(faces, nfaces, chunk);
+  if (f < 0) return;  // uniform over the work-group, before any barrier
   const int64_t cell = fd.cell[f];
   if (tid < 3) Nrm[tid] = fd.normal[f * 3 + tid];
   if (tid < 30) {
@@ -343,7 +358,9 @@ __global__ __launch_bounds__(128) void k_oseen_face(OseenDev op, OseenFaceDev fd
       const int32_t p = pos[ii + kND * jj];
       if (p <= 0) continue;
       const int i = ii / 10, a = ii - i * 10, j = jj / 10, bn = jj - j * 10;
-      nzval[p - 1] += (i == j ? Mb[a][bn] : 0.0) + fd.ZA * Nrm[i] * Nrm[j] * Mo[a][bn];
+      const double v = (i == j ? Mb[a][bn] : 0.0) + fd.ZA * Nrm[i] * Nrm[j] * Mo[a][bn];
+      if (ATOMIC) unsafeAtomicAdd(nzval + p - 1, v);
+      else nzval[p - 1] += v;
     }
   }
   if (WANT_R && tid < 30 && Id[tid] > 0) {
@@ -353,7 +370,8 @@ __global__ __launch_bounds__(128) void k_oseen_face(OseenDev op, OseenFaceDev fd
       const int j = jj / 10, bn = jj - j * 10;
       r += ((i == j ? Mb[a][bn] : 0.0) + fd.ZA * Nrm[i] * Nrm[j] * Mo[a][bn]) * Xe[jj];
     }
-    b[Id[tid] - 1] += r;
+    if (ATOMIC) unsafeAtomicAdd(b + Id[tid] - 1, r);
+    else b[Id[tid] - 1] += r;
   }
 }
 
@@ -374,6 +392,8 @@ struct hfk_oseen {
   std::vector<int32_t> nzpos;
   DevBuf ids, coords, qp, fields, lap, dvals, cells_d, nzpos_d, x_stage, y_stage, nz_stage;
   bool has_lap = false;
+  // colour by colour with plain adds (bitwise reproducible), or one launch with FP64 atomic adds
+  bool deterministic = true;
   // outlet faces, ordered by the colour of their cell
   int64_t nfaces = 0;
   int nqo = 0, nqb = 0;
@@ -428,36 +448,59 @@ int oseen_pattern(hfk_oseen* op) {
   return HFK_OK;
 }
 
-// colour by colour; x may be null (zero free values) when only K is wanted
-int oseen_run(hfk_oseen* op, const double* x, double* b, double* nzval, hipStream_t s) {
-  if (!op->has_fields) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: Oseen fields (w, un, f) not set");
+// colour by colour (or in one launch with atomic adds); x may be null (zero free values) when only K is wanted
+template <bool A>
+int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipStream_t s) {
   const OseenDev d = op->dev();
-  if (b) HIPCHK(op->ctx, launch_fill(b, op->nfree, 0.0, s));
-  if (nzval) HIPCHK(op->ctx, launch_fill(nzval, (int64_t)op->rowval.size(), 0.0, s));
+  const dim3 t(256), tf(128);
+  auto cells_launch = [&](const int32_t* cells, int64_t n, int64_t chunk, unsigned grid) {
+    const dim3 g(grid);
+    if (b && nzval) hipLaunchKernelGGL((k_oseen<true, true, A>), g, t, 0, s, d, cells, chunk, x, b, nzval);
+    else if (b) hipLaunchKernelGGL((k_oseen<false, true, A>), g, t, 0, s, d, cells, chunk, x, b, nzval);
+    else hipLaunchKernelGGL((k_oseen<true, false, A>), g, t, 0, s, d, cells, chunk, x, b, nzval);
+    (void)n;
+  };
+  auto faces_launch = [&](const int32_t* faces, int64_t chunk, unsigned grid) {
+    const OseenFaceDev fd = op->face_dev();
+    const dim3 g(grid);
+    const int64_t nf = op->nfaces;
+    if (b && nzval) hipLaunchKernelGGL((k_oseen_face<true, true, A>), g, tf, 0, s, d, fd, faces, nf, chunk, x, b, nzval);
+    else if (b) hipLaunchKernelGGL((k_oseen_face<false, true, A>), g, tf, 0, s, d, fd, faces, nf, chunk, x, b, nzval);
+    else hipLaunchKernelGGL((k_oseen_face<true, false, A>), g, tf, 0, s, d, fd, faces, nf, chunk, x, b, nzval);
+  };
+  if (A) {
+    const int64_t chunk = (op->ncells + 7) / 8;
+    cells_launch(nullptr, op->ncells, chunk, (unsigned)(8 * chunk));
+    HIPCHK(op->ctx, hipGetLastError());
+    if (op->nfaces > 0) {
+      const int64_t fchunk = (op->nfaces + 7) / 8;
+      faces_launch(nullptr, fchunk, (unsigned)(8 * fchunk));
+      HIPCHK(op->ctx, hipGetLastError());
+    }
+    return HFK_OK;
+  }
   for (size_t c = 0; c + 1 < op->color_start.size(); ++c) {
     const int64_t n = op->color_start[c + 1] - op->color_start[c];
     if (n <= 0) continue;
-    const int32_t* cells = op->cells_d.as<int32_t>() + op->color_start[c];
-    const dim3 g((unsigned)n), t(256);  // one work-group per cell of the colour
-    if (b && nzval) hipLaunchKernelGGL((k_oseen<true, true>), g, t, 0, s, d, cells, x, b, nzval);
-    else if (b) hipLaunchKernelGGL((k_oseen<false, true>), g, t, 0, s, d, cells, x, b, nzval);
-    else hipLaunchKernelGGL((k_oseen<true, false>), g, t, 0, s, d, cells, x, b, nzval);
+    cells_launch(op->cells_d.as<int32_t>() + op->color_start[c], n, 0, (unsigned)n);  // one work-group per cell
     HIPCHK(op->ctx, hipGetLastError());
   }
   if (op->nfaces > 0) {
-    const OseenFaceDev fd = op->face_dev();
     for (size_t c = 0; c + 1 < op->face_start.size(); ++c) {
       const int64_t n = op->face_start[c + 1] - op->face_start[c];
       if (n <= 0) continue;
-      const int32_t* faces = op->faces_d.as<int32_t>() + op->face_start[c];
-      const dim3 g((unsigned)n), t(128);  // one work-group per outlet face of the colour
-      if (b && nzval) hipLaunchKernelGGL((k_oseen_face<true, true>), g, t, 0, s, d, fd, faces, x, b, nzval);
-      else if (b) hipLaunchKernelGGL((k_oseen_face<false, true>), g, t, 0, s, d, fd, faces, x, b, nzval);
-      else hipLaunchKernelGGL((k_oseen_face<true, false>), g, t, 0, s, d, fd, faces, x, b, nzval);
+      faces_launch(op->faces_d.as<int32_t>() + op->face_start[c], 0, (unsigned)n);  // one work-group per face
       HIPCHK(op->ctx, hipGetLastError());
     }
   }
   return HFK_OK;
+}
+
+int oseen_run(hfk_oseen* op, const double* x, double* b, double* nzval, hipStream_t s) {
+  if (!op->has_fields) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: Oseen fields (w, un, f) not set");
+  if (b) HIPCHK(op->ctx, launch_fill(b, op->nfree, 0.0, s));
+  if (nzval) HIPCHK(op->ctx, launch_fill(nzval, (int64_t)op->rowval.size(), 0.0, s));
+  return op->deterministic ? oseen_launch<false>(op, x, b, nzval, s) : oseen_launch<true>(op, x, b, nzval, s);
 }
 
 }  // namespace
@@ -584,6 +627,13 @@ int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell
   op->nqb = nqb;
   op->Pc = Pc;
   op->ZA = ZA;
+  return HFK_OK;
+}
+
+int hfk_oseen_set_deterministic(hfk_oseen* op, int deterministic) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  op->deterministic = deterministic != 0;
   return HFK_OK;
 }
 

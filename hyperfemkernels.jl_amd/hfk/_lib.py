@@ -101,6 +101,7 @@ SIGNATURES = {
     "hfk_oseen_set_outlet": (C.c_int, [_vp, C.c_int64, _i32p, _f64p, C.c_int, _f64p, C.c_int, _f64p, C.c_double,
                                        C.c_double]),
     "hfk_oseen_set_outlet_pressure": (C.c_int, [_vp, C.c_double]),
+    "hfk_oseen_set_deterministic": (C.c_int, [_vp, C.c_int]),
     "hfk_oseen_info": (C.c_int, [_vp, _i64p]),
     "hfk_oseen_colors": (C.c_int, [_vp, _i32p]),
     "hfk_oseen_pattern_nnz": (C.c_int, [_vp, _i64p]),

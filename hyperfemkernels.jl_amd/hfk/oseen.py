@@ -105,6 +105,11 @@ class OseenOperator:
                                             float(ZA)), self.ctx.handle)
         self.outlet = (faces, xb)
 
+    def set_deterministic(self, deterministic: bool = True):
+        """Colour by colour with plain adds (True, bitwise reproducible) or one launch with FP64 atomic adds."""
+        check(self.lib.hfk_oseen_set_deterministic(self.handle, int(bool(deterministic))), self.ctx.handle)
+        return self
+
     def set_outlet_pressure(self, Pc: float):
         check(self.lib.hfk_oseen_set_outlet_pressure(self.handle, float(Pc)), self.ctx.handle)
 

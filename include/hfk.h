@@ -283,6 +283,10 @@ int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell
                          const double* xo, int nqb, const double* xb, double Pc, double ZA);
 /* The outlet pressure Pc alone (a number of the integrand, updated every step by a Windkessel model). */
 int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc);
+/* Assembly colour by colour with plain adds (1, the default: the reference's colour order, bitwise reproducible,
+ * src/assembly.jl:1413-1432) or in one launch over the cells in mesh order with FP64 atomic adds (0: the CSC values
+ * that neighbouring cells share stay in L2; the order of the additions into one entry varies between runs). */
+int hfk_oseen_set_deterministic(hfk_oseen* op, int deterministic);
 /* info[6]: nfree, ndir, ncells, np, ncolors, nnz (-1 before the pattern) */
 int hfk_oseen_info(hfk_oseen* op, int64_t* info);
 int hfk_oseen_colors(hfk_oseen* op, int32_t* colors);
