@@ -93,9 +93,8 @@ __device__ __forceinline__ void shape(int f, double xi, double et, double ze, do
 //   P_ab = sum dV mu ga (x) gb   (the second half of 2 mu eps:eps),   Q_ab = sum dV tl ga (x) gb   (LSIC),
 // so one thread per node pair (a, b) accumulates 19 numbers instead of 9 entries of the full form; the
 // pressure-velocity, velocity-pressure and pressure-pressure blocks take one thread per node pair as well (3, 3
-// and 1 numbers), and 34 further threads the load.  All points of the cell are staged in LDS first (one
-// barrier), the entries go to the CSC values through nzpos, and the element matrix is kept in LDS for the
-// residual rows K x_e - l.
+// and 1 numbers).  All points of the cell are staged in LDS first (one barrier), and the entries go to the CSC
+// values through nzpos.  The residual does not form the element matrix (k_oseen_res).
 constexpr int kVV = 100, kVP = 40, kPV = 40, kPP = 16, kPairs = kVV + kVP + kPV + kPP;  // 196 pair threads
 
 // Cell of a work-group: the cells of one colour (cells != null), or, in the single-launch mode (cells == null),
@@ -107,16 +106,13 @@ __device__ __forceinline__ int64_t work_item(const int32_t* cells, int64_t n, in
   return c < n ? c : -1;
 }
 
-template <bool WANT_K, bool WANT_R, bool ATOMIC>
+template <bool ATOMIC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, int64_t chunk,
-                                               const double* __restrict__ x, double* __restrict__ b,
                                                double* __restrict__ nzval) {
   __shared__ double Sh[kMaxPoints][14][5];  // per point and shape function: N, grad N (3), w . grad N
-  __shared__ double Pq[kMaxPoints][10];     // per point: dV, ts, tp, tl, f (3), lap (3)
+  __shared__ double Pq[kMaxPoints][4];      // per point: dV, ts, tp, tl
   __shared__ double Ke[kND][kND + 1];       // element matrix [test][trial]
-  __shared__ double Xe[kND], Ld[kND];
   __shared__ double Jit[9], Cm[2];
-  __shared__ int32_t Id[kND];
   const int tid = threadIdx.x;
   const int64_t cell = work_item(cells, op.ncells, chunk);
   if (cell < 0) return;  // uniform over the work-group, before any barrier
@@ -149,12 +145,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     Cm[0] = fabs(det);
     Cm[1] = cbrt(6.0 * sqrt(2.0) * fabs(det) / 6.0);
   }
-  if (tid < kND) {
-    const int32_t id = op.ids[cell * kND + tid];
-    Id[tid] = id;
-    Xe[tid] = id > 0 ? (x ? x[id - 1] : 0.0) : op.dvals[-id - 1];
-  }
   __syncthreads();
+  // CSC positions of this thread's element-matrix entries e = ii + 34 jj (entry Ke[ii][jj]); the values they add to
+  // are loaded after the staging below (loads complete in issue order, so that their wait does not hold the
+  // staging) and arrive while the pair phase, which reads only LDS, runs.  Within one colour launch no other
+  // work-group touches these entries.
+  constexpr int kPerThread = (kND * kND + 255) / 256;  // the launch has 256 threads per work-group
+  int32_t pk[kPerThread];
+  double old[kPerThread];
+  {
+    const int32_t* pos = op.nzpos + cell * kND * kND;
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int e = tid + 256 * k;
+      pk[k] = e < kND * kND ? pos[e] : 0;
+    }
+  }
   // stage every (point, shape function) and the point scalars
   for (int it = tid; it < np * 15; it += blockDim.x) {
     const int q = it / 15, f = it - q * 15;
@@ -182,16 +188,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       Pq[q][1] = ts;
       Pq[q][2] = ts / op.rho;
       Pq[q][3] = op.rho * un * h * 0.5;
-      Pq[q][4] = fq[6];
-      Pq[q][5] = fq[7];
-      Pq[q][6] = fq[8];
-      const double* lq = op.lap ? op.lap + ((size_t)cell * np + q) * 3 : nullptr;
-      Pq[q][7] = lq ? lq[0] : 0.0;
-      Pq[q][8] = lq ? lq[1] : 0.0;
-      Pq[q][9] = lq ? lq[2] : 0.0;
     }
   }
   __syncthreads();
+  if (!ATOMIC) {
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) old[k] = pk[k] > 0 ? nzval[pk[k] - 1] : 0.0;
+  }
   const double rho = op.rho, mu = op.mu, c = op.c0;
   if (tid < kVV) {  // velocity-velocity, nodes (a, b)
     const int a = tid / 10, bn = tid - a * 10;
@@ -246,41 +249,167 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       Z -= Pq[q][0] * Pq[q][2] *
            (Sh[q][ap][1] * Sh[q][bp][1] + Sh[q][ap][2] * Sh[q][bp][2] + Sh[q][ap][3] * Sh[q][bp][3]);
     Ke[20 + ap][20 + bp] = Z;
-  } else if (tid < kPairs + kND) {  // load of row r
-    const int r = tid - kPairs;
-    double L = 0.0;
-    if (r < 30) {
-      const int i = r / 10, a = r - i * 10;
-      for (int q = 0; q < np; ++q) {
-        const double dV = Pq[q][0], ts = Pq[q][1], fi = Pq[q][4 + i];
-        L += dV * (rho * fi * Sh[q][a][0] + ts * Sh[q][a][4] * (rho * fi + mu * Pq[q][7 + i]));
-      }
-    } else {
-      const int ap = 10 + (r - 30);
-      for (int q = 0; q < np; ++q)
-        L -= Pq[q][0] * Pq[q][2] *
-             (Sh[q][ap][1] * (rho * Pq[q][4] + mu * Pq[q][7]) + Sh[q][ap][2] * (rho * Pq[q][5] + mu * Pq[q][8]) +
-              Sh[q][ap][3] * (rho * Pq[q][6] + mu * Pq[q][9]));
-    }
-    Ld[r] = L;
   }
   __syncthreads();
-  if (WANT_K) {  // the free (test, trial) entries into the CSC values, column by column
-    const int32_t* pos = op.nzpos + cell * kND * kND;
-    for (int e = tid; e < kND * kND; e += blockDim.x) {
+  {  // the free (test, trial) entries into the CSC values
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int e = tid + 256 * k;
+      if (pk[k] <= 0) continue;
       const int jj = e / kND, ii = e - jj * kND;
-      const int32_t p = pos[ii + kND * jj];
-      if (p <= 0) continue;
-      if (ATOMIC) unsafeAtomicAdd(nzval + p - 1, Ke[ii][jj]);  // hardware FP64 add in L2
-      else nzval[p - 1] += Ke[ii][jj];
+      if (ATOMIC) unsafeAtomicAdd(nzval + pk[k] - 1, Ke[ii][jj]);  // hardware FP64 add in L2
+      else nzval[pk[k] - 1] = old[k] + Ke[ii][jj];
     }
   }
-  if (WANT_R && tid < kND && Id[tid] > 0) {
-    double r = -Ld[tid];
+}
+
+// residual! without the element matrix.  Per point, from the cell's values x_e (Dirichlet values lifted):
+// u, grad u, p, grad p, then the coefficients of the test functions and of their gradients,
+//   velocity rows (i, a):  A_i Na + B_iJ dNa/dx_J,   A_i  = dV (rho c u_i + rho conv_i - rho f_i),
+//                          B_iJ = dV (2 mu eps_Ji + (tl div u - p) delta_iJ + ts w_J (Ru_i + dp/dx_i - rk_i)),
+//   pressure rows a:       C Ma + D_J dMa/dx_J,      C = -dV div u,   D_J = -dV tp (Ru_J + dp/dx_J - rk_J),
+// with conv = (grad u)' w, Ru = rho c u + rho conv, rk = rho f + mu lap: the terms of a((u,p),(v,q)) - l((v,q))
+// for one test function, summed over the points (about a tenth of the arithmetic of forming K x_e).  A work-group
+// takes 256 / np cells of one colour, one thread per (cell, point), then one thread per (cell, row).
+constexpr int kResCoef = 16;  // A (3), B (9), C, D (3)
+
+template <bool ATOMIC>
+__global__ __launch_bounds__(256) void k_oseen_res(OseenDev op, const int32_t* __restrict__ cells, int64_t ncells_launch,
+                                                   const double* __restrict__ x, double* __restrict__ b) {
+  constexpr int kMaxCells = 256;  // np >= 1
+  __shared__ double Co[256][kResCoef + 1];       // per (cell, point) slot
+  __shared__ double Nr[kMaxPoints][14][4];       // reference value and gradient of each shape function
+  __shared__ double JitS[kMaxCells / 8][9];      // per cell (np >= 8 in practice; bounded below)
+  __shared__ double Xs[kMaxCells / 8][kND];
+  __shared__ double Hs[kMaxCells / 8][2];         // |det J|, h
+  __shared__ int32_t Is[kMaxCells / 8][kND];
+  const int tid = threadIdx.x;
+  const int np = op.np;
+  const int cpw = 256 / np < kMaxCells / 8 ? 256 / np : kMaxCells / 8;  // cells per work-group
+  const int64_t c0 = (int64_t)blockIdx.x * cpw;
+  const int nc = (int)((ncells_launch - c0) < cpw ? (ncells_launch - c0) : cpw);
+  // reference shape table, cell geometry and values
+  for (int it = tid; it < np * 14; it += 256) {
+    const int q = it / 14, f = it - q * 14;
+    double n, g[3];
+    shape(f, op.qp[4 * q], op.qp[4 * q + 1], op.qp[4 * q + 2], n, g);
+    Nr[q][f][0] = n;
+    Nr[q][f][1] = g[0];
+    Nr[q][f][2] = g[1];
+    Nr[q][f][3] = g[2];
+  }
+  for (int it = tid; it < nc * kND; it += 256) {
+    const int lc = it / kND, d = it - lc * kND;
+    const int64_t cell = ATOMIC ? c0 + lc : cells[c0 + lc];
+    const int32_t id = op.ids[cell * kND + d];
+    Is[lc][d] = id;
+    Xs[lc][d] = id > 0 ? x[id - 1] : op.dvals[-id - 1];
+  }
+  if (tid < nc) {
+    const int64_t cell = ATOMIC ? c0 + tid : cells[c0 + tid];
+    const double* X = op.coords + cell * 12;
+    double J[9];
 #pragma unroll
-    for (int jj = 0; jj < kND; ++jj) r += Ke[tid][jj] * Xe[jj];
-    if (ATOMIC) unsafeAtomicAdd(b + Id[tid] - 1, r);
-    else b[Id[tid] - 1] += r;
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) J[i + 3 * d] = X[3 * (d + 1) + i] - X[i];
+    const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[3] * (J[1] * J[8] - J[7] * J[2]) +
+                       J[6] * (J[1] * J[5] - J[4] * J[2]);
+    const double id = 1.0 / det;
+    double I[9];
+    I[0] = (J[4] * J[8] - J[7] * J[5]) * id;
+    I[3] = -(J[3] * J[8] - J[6] * J[5]) * id;
+    I[6] = (J[3] * J[7] - J[6] * J[4]) * id;
+    I[1] = -(J[1] * J[8] - J[7] * J[2]) * id;
+    I[4] = (J[0] * J[8] - J[6] * J[2]) * id;
+    I[7] = -(J[0] * J[7] - J[6] * J[1]) * id;
+    I[2] = (J[1] * J[5] - J[4] * J[2]) * id;
+    I[5] = -(J[0] * J[5] - J[3] * J[2]) * id;
+    I[8] = (J[0] * J[4] - J[3] * J[1]) * id;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) JitS[tid][i + 3 * j] = I[j + 3 * i];
+    Hs[tid][0] = fabs(det);
+    Hs[tid][1] = cbrt(6.0 * sqrt(2.0) * fabs(det) / 6.0);
+  }
+  __syncthreads();
+  const double rho = op.rho, mu = op.mu, cc = op.c0;
+  if (tid < nc * np) {  // point phase
+    const int lc = tid / np, q = tid - lc * np;
+    const int64_t cell = ATOMIC ? c0 + lc : cells[c0 + lc];
+    const double* Jt = JitS[lc];
+    const double* xe = Xs[lc];
+    double u[3] = {0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, p = 0, gp[3] = {0, 0, 0};  // G[J + 3 i] = du_i/dx_J
+    for (int f = 0; f < 14; ++f) {
+      const double n = Nr[q][f][0];
+      double g[3];
+#pragma unroll
+      for (int J = 0; J < 3; ++J) g[J] = Jt[J] * Nr[q][f][1] + Jt[J + 3] * Nr[q][f][2] + Jt[J + 6] * Nr[q][f][3];
+      if (f < 10) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double v = xe[i * 10 + f];
+          u[i] += n * v;
+#pragma unroll
+          for (int J = 0; J < 3; ++J) G[J + 3 * i] += g[J] * v;
+        }
+      } else {
+        const double v = xe[30 + f - 10];
+        p += n * v;
+#pragma unroll
+        for (int J = 0; J < 3; ++J) gp[J] += g[J] * v;
+      }
+    }
+    const double* fq = op.fields + ((size_t)cell * np + q) * 9;
+    const double* lq = op.lap ? op.lap + ((size_t)cell * np + q) * 3 : nullptr;
+    const double w[3] = {fq[0], fq[1], fq[2]};
+    const double unn = sqrt(fq[3] * fq[3] + fq[4] * fq[4] + fq[5] * fq[5]);
+    const double h = Hs[lc][1];
+    const double r1 = 2.0 * unn / h, r2 = 3.0 * (4.0 * mu / (rho * h * h));
+    const double ts = 1.0 / sqrt(r1 * r1 + r2 * r2), tp = ts / rho, tl = rho * unn * h * 0.5;
+    const double dV = Hs[lc][0] * op.qp[4 * q + 3];
+    const double div = G[0] + G[4] + G[8];
+    double* co = Co[tid];
+    double ev[3];  // Ru + grad p - rk
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double conv = G[0 + 3 * i] * w[0] + G[1 + 3 * i] * w[1] + G[2 + 3 * i] * w[2];
+      const double ru = rho * cc * u[i] + rho * conv;
+      const double rk = rho * fq[6 + i] + (lq ? mu * lq[i] : 0.0);
+      co[i] = dV * (ru - rho * fq[6 + i]);
+      ev[i] = ru + gp[i] - rk;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int J = 0; J < 3; ++J) {
+        const double eps = 0.5 * (G[J + 3 * i] + G[i + 3 * J]);
+        co[3 + 3 * i + J] = dV * (2.0 * mu * eps + (i == J ? tl * div - p : 0.0) + ts * w[J] * ev[i]);
+      }
+    co[12] = -dV * div;
+#pragma unroll
+    for (int J = 0; J < 3; ++J) co[13 + J] = -dV * tp * ev[J];
+  }
+  __syncthreads();
+  for (int it = tid; it < nc * kND; it += 256) {  // row phase
+    const int lc = it / kND, r = it - lc * kND;
+    const int32_t id = Is[lc][r];
+    if (id <= 0) continue;
+    const double* Jt = JitS[lc];
+    const int f = r < 30 ? r % 10 : 10 + (r - 30);
+    const int i = r < 30 ? r / 10 : 0;
+    double acc = 0.0;
+    for (int q = 0; q < np; ++q) {
+      const double* co = Co[lc * np + q];
+      double g[3];
+#pragma unroll
+      for (int J = 0; J < 3; ++J) g[J] = Jt[J] * Nr[q][f][1] + Jt[J + 3] * Nr[q][f][2] + Jt[J + 6] * Nr[q][f][3];
+      if (r < 30) acc += co[i] * Nr[q][f][0] + co[3 + 3 * i] * g[0] + co[4 + 3 * i] * g[1] + co[5 + 3 * i] * g[2];
+      else acc += co[12] * Nr[q][f][0] + co[13] * g[0] + co[14] * g[1] + co[15] * g[2];
+    }
+    if (ATOMIC) unsafeAtomicAdd(b + id - 1, acc);
+    else b[id - 1] += acc;
   }
 }
 
@@ -453,13 +582,7 @@ template <bool A>
 int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipStream_t s) {
   const OseenDev d = op->dev();
   const dim3 t(256), tf(128);
-  auto cells_launch = [&](const int32_t* cells, int64_t n, int64_t chunk, unsigned grid) {
-    const dim3 g(grid);
-    if (b && nzval) hipLaunchKernelGGL((k_oseen<true, true, A>), g, t, 0, s, d, cells, chunk, x, b, nzval);
-    else if (b) hipLaunchKernelGGL((k_oseen<false, true, A>), g, t, 0, s, d, cells, chunk, x, b, nzval);
-    else hipLaunchKernelGGL((k_oseen<true, false, A>), g, t, 0, s, d, cells, chunk, x, b, nzval);
-    (void)n;
-  };
+  const int cpw = std::min(256 / op->np, 32);  // cells per work-group of k_oseen_res
   auto faces_launch = [&](const int32_t* faces, int64_t chunk, unsigned grid) {
     const OseenFaceDev fd = op->face_dev();
     const dim3 g(grid);
@@ -468,9 +591,12 @@ int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipSt
     else if (b) hipLaunchKernelGGL((k_oseen_face<false, true, A>), g, tf, 0, s, d, fd, faces, nf, chunk, x, b, nzval);
     else hipLaunchKernelGGL((k_oseen_face<true, false, A>), g, tf, 0, s, d, fd, faces, nf, chunk, x, b, nzval);
   };
-  if (A) {
+  if (A) {  // one launch each, cells in mesh order
     const int64_t chunk = (op->ncells + 7) / 8;
-    cells_launch(nullptr, op->ncells, chunk, (unsigned)(8 * chunk));
+    if (nzval) hipLaunchKernelGGL((k_oseen<true>), dim3((unsigned)(8 * chunk)), t, 0, s, d, nullptr, chunk, nzval);
+    if (b)
+      hipLaunchKernelGGL((k_oseen_res<true>), dim3((unsigned)((op->ncells + cpw - 1) / cpw)), t, 0, s, d, nullptr,
+                         op->ncells, x, b);
     HIPCHK(op->ctx, hipGetLastError());
     if (op->nfaces > 0) {
       const int64_t fchunk = (op->nfaces + 7) / 8;
@@ -482,7 +608,9 @@ int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipSt
   for (size_t c = 0; c + 1 < op->color_start.size(); ++c) {
     const int64_t n = op->color_start[c + 1] - op->color_start[c];
     if (n <= 0) continue;
-    cells_launch(op->cells_d.as<int32_t>() + op->color_start[c], n, 0, (unsigned)n);  // one work-group per cell
+    const int32_t* cells = op->cells_d.as<int32_t>() + op->color_start[c];
+    if (nzval) hipLaunchKernelGGL((k_oseen<false>), dim3((unsigned)n), t, 0, s, d, cells, 0, nzval);  // a cell each
+    if (b) hipLaunchKernelGGL((k_oseen_res<false>), dim3((unsigned)((n + cpw - 1) / cpw)), t, 0, s, d, cells, n, x, b);
     HIPCHK(op->ctx, hipGetLastError());
   }
   if (op->nfaces > 0) {
