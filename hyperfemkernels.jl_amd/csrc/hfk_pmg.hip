@@ -1110,7 +1110,13 @@ int hfk_block_jacobi_blocks(hfk_op* op, int64_t* nblocks, int32_t* dofs, double*
   if ((rc = op_build_blocks(op))) return rc;
   if (nblocks) *nblocks = op->blocks.nblocks;
   if (dofs) std::memcpy(dofs, op->blocks.dofs_h.data(), op->blocks.dofs_h.size() * sizeof(int32_t));
-  if (Binv) HIPCHK(op->ctx, hipMemcpy(Binv, op->blocks.Binv.p, op->blocks.nblocks * 9 * sizeof(double), hipMemcpyDeviceToHost));
+  if (Binv) {  // the device stores entry-major [e][block]; the ABI returns [block][e]
+    const int64_t nb = op->blocks.nblocks;
+    std::vector<double> t((size_t)nb * 9);
+    HIPCHK(op->ctx, hipMemcpy(t.data(), op->blocks.Binv.p, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < nb; ++k)
+      for (int e = 0; e < 9; ++e) Binv[9 * k + e] = t[(size_t)e * nb + k];
+  }
   return HFK_OK;
 }
 

@@ -161,7 +161,8 @@ __device__ __forceinline__ double det3d(const double* a) {
 }
 
 // _invert_blocks! (src/preconditioners.jl:292-307): fixed rows/columns -> identity; a non-finite block is kept
-// as it is and counted in bad[0]
+// as it is and counted in bad[0].  The inverses are stored entry-major, Binv[(i + 3j) * nb + k], so that the
+// block step reads each entry with one coalesced access per wave.
 __global__ void k_invert_blocks(int64_t nb, const int32_t* __restrict__ dofs, const double* __restrict__ B,
                                 double* __restrict__ Binv, int* __restrict__ bad) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
@@ -190,7 +191,7 @@ __global__ void k_invert_blocks(int64_t nb, const int32_t* __restrict__ dofs, co
     }
     bool ok = true;
     for (int e = 0; e < 9; ++e) {
-      Binv[9 * k + e] = R[e];
+      Binv[(size_t)e * nb + k] = R[e];
       ok = ok && isfinite(R[e]);
     }
     if (!ok) atomicAdd(bad, 1);
@@ -214,7 +215,7 @@ __global__ void k_block_step(int64_t nb, const int32_t* __restrict__ dofs, const
       if (id[i] <= 0) continue;
       double z = 0;
       for (int j = 0; j < 3; ++j)
-        if (id[j] > 0) z += Binv[9 * k + i + 3 * j] * rv[j];
+        if (id[j] > 0) z += Binv[(size_t)(i + 3 * j) * nb + k] * rv[j];
       const double dn = (cd == 0.0 ? 0.0 : cd * d[id[i] - 1]) + cz * z;
       d[id[i] - 1] = dn;
       if (x) x[id[i] - 1] += dn;
