@@ -606,9 +606,10 @@ def precision_leg(n, order, stream, storages=("full64", "full32", "sym32"), step
 
 
 def compiled_law_leg(n, stream, steps=5):
-    """The 64^3 Q2 neo-Hookean workload with the law compiled at run time from source (hiprtc) beside the
-    built-in kernels: compile times (the law's check, then its prepare!/residual! kernels at first use),
-    prepare! per storage, residual! and the relative difference of Jv and of the residual."""
+    """The 64^3 Q2 neo-Hookean workload with the law compiled at run time (hiprtc) beside the built-in kernels: the
+    law written as HIP source, and the law generated from its energy by hfk.codegen (stress and tangent derived
+    from the expression tree); compile times, prepare! per storage, residual! and the relative difference of Jv
+    and of the residual against the built-in kernels."""
     import ctypes as C
 
     import numpy as np
@@ -653,8 +654,13 @@ def compiled_law_leg(n, stream, steps=5):
     residual()  # first use on this (Q, P): the prepare!/residual! kernels are compiled with the law
     torch.cuda.synchronize()
     out["first_use_compile_s"] = time.perf_counter() - t0
+    from hfk import codegen
+
+    t0 = time.perf_counter()
+    gen = CompiledLaw(codegen.from_energy(codegen.neohooke_energy).source("GenNeoHooke"), "GenNeoHooke", ctx=op.ctx)
+    out["generated_check_compile_s"] = time.perf_counter() - t0
     res = {}
-    for tag, lw in (("builtin", None), ("compiled", law)):
+    for tag, lw in (("builtin", None), ("compiled", law), ("generated", gen)):
         op.set_law(lw)
         r = {"residual_ms": timed(residual)}
         b = bd.cpu().numpy()
@@ -666,12 +672,13 @@ def compiled_law_leg(n, stream, steps=5):
             r[f"jv_{st}"] = yd.cpu().numpy()
         r["b"] = b
         res[tag] = r
-    for tag in ("builtin", "compiled"):
+    for tag in ("builtin", "compiled", "generated"):
         out[tag] = {k: v for k, v in res[tag].items() if k.endswith("_ms")}
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    out["rel_diff_residual"] = rel(res["compiled"]["b"], res["builtin"]["b"])
-    out["rel_diff_jv_full64"] = rel(res["compiled"]["jv_full64"], res["builtin"]["jv_full64"])
-    out["rel_diff_jv_sym32"] = rel(res["compiled"]["jv_sym32"], res["builtin"]["jv_sym32"])
+    for tag in ("compiled", "generated"):
+        out[tag]["rel_diff_residual"] = rel(res[tag]["b"], res["builtin"]["b"])
+        out[tag]["rel_diff_jv_full64"] = rel(res[tag]["jv_full64"], res["builtin"]["jv_full64"])
+        out[tag]["rel_diff_jv_sym32"] = rel(res[tag]["jv_sym32"], res["builtin"]["jv_sym32"])
     op.set_law(None)
     return out
 

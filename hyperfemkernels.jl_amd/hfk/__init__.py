@@ -2,7 +2,7 @@
 
 Modules: ``mesh`` (Cartesian hex Q_k spaces, the stand-in for Gridap's spaces), ``synthetic`` (SplitMix64
 inputs), ``operator`` (KernelOperator, MatrixFreeJacobian), ``laws``
-(pointwise laws compiled at run time with hiprtc), ``build`` (hipcc build of libhfk.so), ``_lib``
+(pointwise laws compiled at run time with hiprtc), ``codegen`` (laws generated from an expression tree), ``build`` (hipcc build of libhfk.so), ``_lib``
 (ctypes binding).  Importing this package does not load the library; the first operator does, and fails loudly
 if it has not been built.
 """

@@ -141,3 +141,62 @@ def test_cache_and_compile_error(hip_ctx):
     assert c.key != a.key and not c.from_cache
     with pytest.raises(ValueError, match="no member named"):
         CompiledLaw("struct Empty {};", "Empty", ctx=hip_ctx)
+
+
+# ---- laws generated from an expression tree (hfk.codegen, the analogue of src/codegen.jl:122-198) ------------
+
+@pytest.mark.parametrize("law,energy", [("svk", "svk_energy"), ("neohookean", "neohooke_energy")])
+def test_generated_law_matches_builtin(hip_ctx, law, energy):
+    from hfk import codegen as cg
+
+    print(f"generated law {law}", flush=True)
+    sp, dv, x, rng = problem(2, (3, 2, 2), 0.1, seed=21)
+    form = HyperelasticForm(law, 2.0, 1.0, degree=4)
+    ref = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    name = "Gen" + law.capitalize()
+    op.set_law(CompiledLaw(cg.from_energy(getattr(cg, energy)).source(name), name, ctx=hip_ctx))
+    assert rel(op.residual(np.zeros(sp.nfree), x), ref.residual(np.zeros(sp.nfree), x)) < TOL
+    dx = rng.standard_normal(sp.nfree)
+    for storage in ("full64", "sym64"):
+        y = MatrixFreeJacobian(op, storage).prepare(x) @ dx
+        assert rel(y, MatrixFreeJacobian(ref, storage).prepare(x) @ dx) < TOL, storage
+
+
+def test_generated_extra_law_invariants(hip_ctx):
+    """Compressible Mooney-Rivlin from its energy W = mu/2 (I1-3) + mu/4 (I2-3) - 2 mu lnJ + lam/2 lnJ^2: stress free
+    at F = I with the linear law's tangent there (linearising: shear modulus 3 mu / 2, first Lame parameter
+    lam + mu), Jv equal to the central difference of the residual, zero residual under a rigid rotation."""
+    from hfk import codegen as cg
+
+    def mooney(F, lam, mu):
+        C = F.T() @ F
+        I1 = cg.tr(C)
+        I2 = 0.5 * (I1 * I1 - cg.tr(C @ C))
+        lnJ = cg.log(cg.det(F))
+        return mu * 0.5 * (I1 - 3.0) + mu * 0.25 * (I2 - 3.0) - 2.0 * mu * lnJ + lam * 0.5 * lnJ * lnJ
+
+    law = CompiledLaw(cg.from_energy(mooney).source("GenMooney"), "GenMooney", ctx=hip_ctx)
+    sp, _, _, rng = problem(2, (3, 2, 2), 0.1, seed=23)
+    op = KernelOperator(HyperelasticForm("svk", 2.0, 0.8, degree=4), sp, jacobian="matrixfree",
+                        dirichlet_values=np.zeros(sp.ndir), ctx=hip_ctx)
+    op.set_law(law)
+    lin = KernelOperator(HyperelasticForm("linear", 2.0 + 0.8, 1.5 * 0.8, degree=4), sp, jacobian="matrixfree",
+                         dirichlet_values=np.zeros(sp.ndir), ctx=hip_ctx)
+    z = np.zeros(sp.nfree)
+    v = rng.standard_normal(sp.nfree)
+    assert np.abs(op.residual(np.zeros(sp.nfree), z)).max() < 1e-14
+    assert rel(MatrixFreeJacobian(op, "full64").prepare(z) @ v, MatrixFreeJacobian(lin, "full64").prepare(z) @ v) < TOL
+    R = rotation([0.2, 0.9, -0.4], np.deg2rad(35.0))
+    rot = lambda X: X @ (R - np.eye(3)).T
+    op.set_dirichlet(sp.dirichlet_values(rot))
+    br = op.residual(np.zeros(sp.nfree), sp.free_values(rot))
+    stretch = lambda X: 0.1 * X * X[:, [1, 2, 0]]
+    op.set_dirichlet(sp.dirichlet_values(stretch))
+    xs = sp.free_values(stretch)
+    bs = op.residual(np.zeros(sp.nfree), xs)
+    assert np.linalg.norm(br) < 1e-12 * np.linalg.norm(bs)
+    h = 1e-6
+    y = MatrixFreeJacobian(op, "full64").prepare(xs) @ v
+    fd = (op.residual(np.zeros(sp.nfree), xs + h * v) - op.residual(np.zeros(sp.nfree), xs - h * v)) / (2 * h)
+    assert rel(fd, y) < 1e-7
