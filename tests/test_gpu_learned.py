@@ -124,4 +124,10 @@ def test_training_and_learned_cycle(hip_ctx, scaled):
     # the trained network as the coarse preconditioner: the outer iterations stay, the coarse work collapses
     assert sols["precondition"][1] <= sols["off"][1] + 1
     assert sols["precondition"][2] < 0.25 * sols["off"][2]
+    # the network cycle inside restarted flexible GMRES (right preconditioned, the preconditioner may vary)
+    set_coarse_network(pmg, net, "cycle", 0, scaled)
+    x = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fgmres", rtol=1e-10, maxiter=400, preconditioner=pmg, restart=30).solve(x, mf, b)
+    print(f"fgmres + network cycle: {res}", flush=True)
+    assert res.converged and rel(x, sols["off"][0]) < 1e-7
     set_coarse_network(pmg, None, "off")
