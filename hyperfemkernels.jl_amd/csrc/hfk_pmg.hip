@@ -500,6 +500,10 @@ struct hfk_pmg {
   int ccheb_degree = 0;  // its degree (0: the matrix-free levels' degree)
   Dots dots;
   bool ok = false, indefinite = false, built = false, coarse_ok = false;
+  // coarse re-assembly every coarse_every updates (PLevels coarse_every, src/pmultigrid.jl:200-216)
+  int coarse_every = 1, since = 0;
+  bool coarse_built = false;
+  int64_t coarse_builds = 0;
   int nupdate = 0, ncycle = 0, last_inner = 0, total_inner = 0;
   int storage = 0;           // storage of the matrix-free coarse levels' tangents
   int smoother_storage = -1;  // -1: the working operator's own tangents; else a copy in this storage
@@ -801,11 +805,21 @@ int pmg_update(hfk_pmg* g, hipStream_t s) {
     if (l + 1 < g->ops.size()) {
       if ((rc = op_run_prepare(pc, pc->state.as<double>(), g->storage, s))) return rc;
     } else {
-      bool cok = false;
-      if ((rc = coarse_setup(g, s, &cok))) return rc;
-      g->coarse_ok = cok;
-      g->ccheb_stale = true;
-      ok = ok && cok;
+      // the coarse state is injected at every update; the matrix is re-assembled and set up on the first update,
+      // every coarse_every updates, and after one that was not finite (src/pmultigrid.jl:200-216)
+      g->since++;
+      if (!g->coarse_built || g->since >= g->coarse_every) {
+        bool cok = false;
+        if ((rc = coarse_setup(g, s, &cok))) return rc;
+        g->coarse_ok = cok;
+        g->ccheb_stale = true;
+        g->coarse_built = cok;
+        if (cok) {
+          g->since = 0;
+          g->coarse_builds++;
+        }
+      }
+      ok = ok && g->coarse_ok;
     }
   }
   bool indef = false;
@@ -1299,6 +1313,19 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
 
 int hfk_pmg_destroy(hfk_pmg* g) {
   delete g;
+  return HFK_OK;
+}
+
+int hfk_pmg_set_coarse_every(hfk_pmg* g, int coarse_every) {
+  if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  if (coarse_every < 1) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: coarse_every must be >= 1");
+  g->coarse_every = coarse_every;
+  return HFK_OK;
+}
+
+int hfk_pmg_coarse_builds(hfk_pmg* g, int64_t* builds) {
+  if (!g || !builds) return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  *builds = g->coarse_builds;
   return HFK_OK;
 }
 

@@ -166,9 +166,11 @@ class PMultigrid:
     finest first (the first is the Newton's, matrix-free; the last assembled)."""
 
     def __init__(self, levels, smoother: Chebyshev = Chebyshev(), coarse_rtol: float = 1e-2,
-                 coarse_maxiter: int = 500, smoother_storage: str | None = None):
+                 coarse_maxiter: int = 500, smoother_storage: str | None = None, coarse_every: int = 1):
         if len(levels) < 2:
             raise ValueError("HyperFEMKernels: PMultigrid needs at least two levels")
+        if coarse_every < 1:
+            raise ValueError("HyperFEMKernels: PMultigrid: coarse_every must be >= 1")
         self.levels = list(levels)
         arr = (C.c_void_p * len(levels))(*[op.handle.value for op in levels])
         h = C.c_void_p()
@@ -179,6 +181,15 @@ class PMultigrid:
               levels[0].ctx.handle)
         self.handle, self.lib, self.smoother = h, lib, smoother
         self.ok = self.indefinite = False
+        self.coarse_every = int(coarse_every)
+        if coarse_every != 1:
+            check(lib.hfk_pmg_set_coarse_every(h, int(coarse_every)), levels[0].ctx.handle)
+
+    def coarse_builds(self) -> int:
+        """Coarse set-ups so far (the reference's ``nbuild``)."""
+        n = C.c_int64()
+        check(self.lib.hfk_pmg_coarse_builds(self.handle, C.byref(n)), self.levels[0].ctx.handle)
+        return int(n.value)
 
     def update(self):
         """preconditioner_update!: follow the state of the working operator's last prepare."""

@@ -152,7 +152,8 @@ mutable struct DevicePMultigrid
     levels::Vector{DeviceOperator}
 end
 function DevicePMultigrid(levels::Vector{DeviceOperator}; degree=2, range=20.0, inflate=1.1, eig_iters=10,
-                          smoother_storage::Union{Nothing,Symbol}=nothing, coarse_rtol=1e-2, coarse_maxiter=500)
+                          smoother_storage::Union{Nothing,Symbol}=nothing, coarse_rtol=1e-2, coarse_maxiter=500,
+                          coarse_every::Integer=1)
     ptrs = [l.ptr for l in levels]
     r = Ref{Ptr{Cvoid}}(C_NULL)
     sst = smoother_storage === nothing ? Cint(-1) : HFK_STORE[smoother_storage]
@@ -162,6 +163,7 @@ function DevicePMultigrid(levels::Vector{DeviceOperator}; degree=2, range=20.0, 
           levels[1].ctx.ptr)
     g = DevicePMultigrid(r[], levels)
     finalizer(g -> ccall((:hfk_pmg_destroy, libhfk), Cint, (Ptr{Cvoid},), g.ptr), g)
+    check(ccall((:hfk_pmg_set_coarse_every, libhfk), Cint, (Ptr{Cvoid}, Cint), g.ptr, coarse_every), levels[1].ctx.ptr)
     g
 end
 "preconditioner_update!: returns (ok, indefinite)"

@@ -178,6 +178,12 @@ int hfk_pmg_destroy(hfk_pmg* g);
 /* update_levels! + smoother_update!: inject the state of levels[0] (its last prepare), re-prepare / re-assemble
  * the levels, blocks and intervals.  status[0] = ok (preconditioner_ok), status[1] = indefinite */
 int hfk_pmg_update(hfk_pmg* g, int* status);
+/* PMultigrid(...; coarse_every) (src/pmultigrid.jl:229-262, 200-216): the assembled coarse level is re-assembled and
+ * set up on the first update and then every coarse_every updates (its state is injected at every update; a lagged
+ * coarse matrix only changes the preconditioner, the outer tolerance still fixes the answer).  Default 1. */
+int hfk_pmg_set_coarse_every(hfk_pmg* g, int coarse_every);
+/* number of coarse set-ups so far (the reference's nbuild) */
+int hfk_pmg_coarse_builds(hfk_pmg* g, int64_t* builds);
 /* ldiv!: one V-cycle z = M^-1 r (NaN-filled when not ok) */
 int hfk_pmg_vcycle(hfk_pmg* g, const double* r, double* z);
 int hfk_pmg_vcycle_dev(hfk_pmg* g, const double* r_dev, double* z_dev, void* stream);

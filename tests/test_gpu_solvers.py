@@ -139,6 +139,49 @@ def test_pmultigrid_solution_equals_direct(hip_ctx, oracle):
     assert res2.iterations > res.iterations
 
 
+def test_pmultigrid_coarse_every(hip_ctx, oracle):
+    """coarse_every = 3 (test_pmultigrid_levels.jl:98-107): four updates at changing states build the coarse level
+    at updates 1 and 4; between them the coarse matrix is the lagged one, and FCG with the lagged coarse level
+    still reaches the direct solution at the new state (test_pmultigrid.jl:107-114)."""
+    from hfk.synthetic import smooth_field
+
+    with pytest.raises(ValueError):
+        PMultigrid([None, None], coarse_every=0)
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3, coarse_every=3)
+    # the every-update hierarchy gets its own coarse operator: a coarse operator's node blocks belong to the
+    # hierarchy that assembles it
+    cop1 = KernelOperator(HyperelasticForm("neohookean", degree=4), hex_space((4, 4, 4), 1, dirichlet=dirichlet_closed_x0),
+                          jacobian="assembled", ctx=hip_ctx)
+    every1 = PMultigrid([fop, cop1], Chebyshev(degree=4), coarse_rtol=1e-3)
+    xc = np.random.default_rng(10).standard_normal(cop.nfree)
+    b = np.random.default_rng(11).standard_normal(fs.nfree)
+    dvals = fs.dirichlet_values(smooth_field(2, amp=0.01))
+    ac = []
+    for k in range(4):
+        xk = (1.0 + 0.2 * k) * x
+        mf.prepare(xk)
+        pmg.update()
+        assert pmg.ok
+        ac.append(pmg.coarse_matvec(xc))
+        assert pmg.coarse_builds() == (1 if k < 3 else 2)
+        if k == 2:
+            # the lagged coarse level of update 1 preconditions the tangent at this state
+            xs = np.zeros(fs.nfree)
+            res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
+            assert res.converged and res.iterations <= 30
+            orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=dvals)
+            assert rel(xs, spla.spsolve(orc.csc(xk).tocsc(), b)) < 1e-7
+            every1.update()
+            assert every1.coarse_builds() == 1
+            fresh = every1.coarse_matvec(xc)
+    assert np.array_equal(ac[0], ac[1]) and np.array_equal(ac[0], ac[2])
+    assert rel(ac[2], fresh) > 1e-6 and rel(ac[3], ac[2]) > 1e-6
+    # at update 4 the rebuilt coarse matrix is the one an every-update hierarchy assembles at that state
+    every1.update()
+    assert every1.coarse_builds() == 2 and rel(ac[3], every1.coarse_matvec(xc)) < 1e-14
+
+
 def test_pmultigrid_q3_lognormal(hip_ctx, oracle):
     """Q3 working level over an assembled Q1 coarse level with the fine 4-point rule, heterogeneous (lognormal)
     shear modulus, FP32 smoothers: the flexible CG reaches the direct solution."""
