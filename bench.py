@@ -155,7 +155,8 @@ def oseen_leg(n=40, reps=10):
     test/common.jl:165-228: the elementwise Laplacian of un (the P2 interpolant of w) in the stabilisation load and
     the outlet terms (backflow with w, Robin ZA = 2.3, pressure Pc = 0.7; collapsed Gauss rules of degree 4 and 5 on
     the faces).  Times the device residual!, jacobian! and the fused pass (state and outputs on the device, HIP events
-    on the stream), and the host-side pattern build; the 14-point degree-5 rule in the cells."""
+    on the stream), the step's point data made on the device from the FE functions of the previous steps, and the
+    host-side pattern build; the 14-point degree-5 rule in the cells."""
     import numpy as np
     import torch
 
@@ -218,6 +219,22 @@ def oseen_leg(n=40, reps=10):
                                    op.ctx.handle))
     atomic_rel = float(np.linalg.norm(nz.cpu().numpy() - nz_det) / np.linalg.norm(nz_det))
     op.set_deterministic(True)
+    # the step's point data made on the device from the FE functions u^n = u^{n-1} = the P2 interpolant of w (free
+    # and Dirichlet values resident on the device): w, un, f at the points, the Laplacian of un, w at the outlet
+    # backflow points; then the whole step (point data, residual and Jacobian)
+    fe = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (un, sp.dirichlet_values(w))]
+    fptr = [C.c_void_p(fe[0].data_ptr()), C.c_void_p(fe[1].data_ptr())]
+    aw, bw, c1, c2 = op.step_coefficients()
+    torch.cuda.synchronize()
+
+    def fields_dev():
+        check(op.lib.hfk_oseen_step_fields_dev(op.handle, fptr[0], fptr[1], fptr[0], fptr[1], aw, bw, c1, c2, 1, sptr),
+              op.ctx.handle)
+
+    t_fields = timed(fields_dev)
+    t_step = timed(lambda: (fields_dev(), check(op.lib.hfk_oseen_residual_jacobian_dev(op.handle, xp, bp, np_, sptr),
+                                                op.ctx.handle)))
+    fields_bytes = sp.ncells * op.np * 12 * 8 + len(faces.cell) * faces.xb.shape[1] * 3 * 8
     info = op.info()
     ncells = sp.ncells
     nnz = int(rowval.size)
@@ -235,6 +252,8 @@ def oseen_leg(n=40, reps=10):
                                      "residual_and_jacobian_ms": t_both_a * 1e3,
                                      "jacobian_GBs_of_min_bytes": jac_bytes / t_jac_a / 1e9,
                                      "rel_diff_vs_colour_order": atomic_rel},
+            "step_fields_ms": t_fields * 1e3, "step_fields_GBs_written": fields_bytes / t_fields / 1e9,
+            "step_with_fields_ms": t_step * 1e3,
             "pattern_s": t_pattern, "create_s": t_create,
             "mesh_s": t_mesh, "residual_at_zero_norm": float(np.linalg.norm(bb))}
 

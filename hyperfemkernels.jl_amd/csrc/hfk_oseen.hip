@@ -504,6 +504,116 @@ __global__ __launch_bounds__(128) void k_oseen_face(OseenDev op, OseenFaceDev fd
   }
 }
 
+// The FE functions un = u^n and unm1 = u^{n-1} of a step: free values in the operator's numbering and Dirichlet
+// values, and the coefficients of the point data made from them.
+struct StepFunctions {
+  const double* un;
+  const double* un_dir;
+  const double* unm1;
+  const double* unm1_dir;
+  double aw, bw, c1, c2;
+};
+
+__device__ __forceinline__ double dof_value(int32_t id, const double* fr, const double* dr) {
+  return id > 0 ? fr[id - 1] : dr[-id - 1];
+}
+
+// The point data of a step from the FE functions of the two previous steps (the FE-function leaves of
+// test/common.jl:183-196, which src/fefunctions.jl materialises on the CPU): one thread per quadrature point of a
+// cell, then one per dGb point of an outlet face.  At a cell point: w = aw un + bw unm1, un, f = c1 un - c2 unm1
+// into fields[cell][q][9], and with lap != null the elementwise Laplacian of un into lap[cell][q][3]: constant on
+// an affine cell, sum_a un_a lap(N_a) with lap(N_a) = 4 |grad l_a|^2 at vertex a and 8 grad l_a . grad l_b at the
+// midpoint of edge (a, b), grad l the physical gradients of the barycentric coordinates (J^-T of their reference
+// gradients; J^-T = [e2 x e3, e3 x e1, e1 x e2] / det for the edge vectors e_d = x_d - x_0).  At a face point: w
+// into xb[face][q][4..6].
+__global__ __launch_bounds__(256) void k_oseen_fields(OseenDev op, StepFunctions sf, int64_t nfaces, int nqb,
+                                                      const int32_t* __restrict__ face_cell, double* __restrict__ fields,
+                                                      double* __restrict__ lap, double* __restrict__ xb) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t ncp = op.ncells * op.np;
+  int64_t cell;
+  double xi, et, ze;
+  double* xq = nullptr;
+  if (t < ncp) {
+    cell = t / op.np;
+    const double* p = op.qp + (t - cell * op.np) * 4;
+    xi = p[0];
+    et = p[1];
+    ze = p[2];
+  } else if (t - ncp < nfaces * nqb) {
+    const int64_t f = (t - ncp) / nqb;
+    cell = face_cell[f];
+    xq = xb + (t - ncp) * 7;
+    xi = xq[0];
+    et = xq[1];
+    ze = xq[2];
+  } else {
+    return;
+  }
+  const int32_t* ids = op.ids + cell * kND;
+  double u[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < 10; ++a) {
+    double n, g[3];
+    shape(a, xi, et, ze, n, g);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int32_t id = ids[c * 10 + a];
+      u[c] += n * dof_value(id, sf.un, sf.un_dir);
+      v[c] += n * dof_value(id, sf.unm1, sf.unm1_dir);
+    }
+  }
+  if (xq) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xq[4 + c] = sf.aw * u[c] + sf.bw * v[c];
+    return;
+  }
+  double* fq = fields + t * 9;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    fq[c] = sf.aw * u[c] + sf.bw * v[c];
+    fq[3 + c] = u[c];
+    fq[6 + c] = sf.c1 * u[c] - sf.c2 * v[c];
+  }
+  if (!lap) return;
+  const double* X = op.coords + cell * 12;
+  double e[3][3];  // e[d] = x_{d+1} - x_0
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) e[d][i] = X[3 * (d + 1) + i] - X[i];
+  double G[4][3];  // physical gradients of the barycentric coordinates
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double* p = e[(d + 1) % 3];
+    const double* q = e[(d + 2) % 3];
+    G[d + 1][0] = p[1] * q[2] - p[2] * q[1];
+    G[d + 1][1] = p[2] * q[0] - p[0] * q[2];
+    G[d + 1][2] = p[0] * q[1] - p[1] * q[0];
+  }
+  const double det = e[0][0] * G[1][0] + e[0][1] * G[1][1] + e[0][2] * G[1][2];
+#pragma unroll
+  for (int k = 1; k < 4; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) G[k][i] /= det;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) G[0][i] = -(G[1][i] + G[2][i] + G[3][i]);
+  double L[10];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) L[a] = 4.0 * (G[a][0] * G[a][0] + G[a][1] * G[a][1] + G[a][2] * G[a][2]);
+#pragma unroll
+  for (int ed = 0; ed < 6; ++ed) {
+    const int a = ed < 3 ? 0 : (ed < 5 ? 1 : 2);
+    const int b = ed < 3 ? ed + 1 : (ed < 5 ? ed - 1 : 3);
+    L[4 + ed] = 8.0 * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int a = 0; a < 10; ++a) s += L[a] * dof_value(ids[c * 10 + a], sf.un, sf.un_dir);
+    lap[t * 3 + c] = s;
+  }
+}
+
 }  // namespace
 }  // namespace hfk
 
@@ -529,6 +639,7 @@ struct hfk_oseen {
   double Pc = 0, ZA = 0;
   std::vector<int64_t> face_start;  // faces of colour c: faces_d[face_start[c] .. face_start[c+1])
   DevBuf face_cell, face_normal, face_xo, face_xb, faces_d;
+  DevBuf step_stage;  // host FE-function values of hfk_oseen_step_fields
   hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
   OseenDev dev() const {
     OseenDev d;
@@ -631,6 +742,22 @@ int oseen_run(hfk_oseen* op, const double* x, double* b, double* nzval, hipStrea
   return op->deterministic ? oseen_launch<false>(op, x, b, nzval, s) : oseen_launch<true>(op, x, b, nzval, s);
 }
 
+// the point data of a step on the device (k_oseen_fields); enables the Laplacian term when with_lap, removes it
+// otherwise
+int oseen_step_fields(hfk_oseen* op, const StepFunctions& sf, bool with_lap, hipStream_t s) {
+  const int64_t ncp = op->ncells * op->np;
+  HIPCHK(op->ctx, op->fields.alloc((size_t)ncp * 9 * sizeof(double)));
+  if (with_lap) HIPCHK(op->ctx, op->lap.alloc((size_t)ncp * 3 * sizeof(double)));
+  const int64_t total = ncp + op->nfaces * op->nqb;
+  hipLaunchKernelGGL(k_oseen_fields, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, op->dev(), sf, op->nfaces,
+                     op->nqb, op->face_cell.as<int32_t>(), op->fields.as<double>(),
+                     with_lap ? op->lap.as<double>() : nullptr, op->face_xb.as<double>());
+  HIPCHK(op->ctx, hipGetLastError());
+  op->has_fields = true;
+  op->has_lap = with_lap;
+  return HFK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -700,6 +827,39 @@ int hfk_oseen_set_fields(hfk_oseen* op, const double* fields) {
   HIPCHK(op->ctx, op->fields.alloc(n * sizeof(double)));
   HIPCHK(op->ctx, hipMemcpy(op->fields.p, fields, n * sizeof(double), hipMemcpyHostToDevice));
   op->has_fields = true;
+  return HFK_OK;
+}
+
+int hfk_oseen_step_fields_dev(hfk_oseen* op, const double* un, const double* un_dir, const double* unm1,
+                              const double* unm1_dir, double aw, double bw, double c1, double c2, int laplacian,
+                              void* stream) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (!un || !unm1 || (op->ndir > 0 && (!un_dir || !unm1_dir)))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null FE-function values");
+  const StepFunctions sf{un, un_dir, unm1, unm1_dir, aw, bw, c1, c2};
+  return oseen_step_fields(op, sf, laplacian != 0, op->stream(stream));
+}
+
+int hfk_oseen_step_fields(hfk_oseen* op, const double* un, const double* un_dir, const double* unm1,
+                          const double* unm1_dir, double aw, double bw, double c1, double c2, int laplacian) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (!un || !unm1 || (op->ndir > 0 && (!un_dir || !unm1_dir)))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null FE-function values");
+  const size_t nf = (size_t)op->nfree, nd = (size_t)op->ndir;
+  HIPCHK(op->ctx, op->step_stage.alloc((2 * (nf + nd)) * sizeof(double)));
+  double* d = op->step_stage.as<double>();
+  hipStream_t s = op->ctx->stream;
+  HIPCHK(op->ctx, hipMemcpyAsync(d, un, nf * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(d + nf + nd, unm1, nf * sizeof(double), hipMemcpyHostToDevice, s));
+  if (nd) {
+    HIPCHK(op->ctx, hipMemcpyAsync(d + nf, un_dir, nd * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(op->ctx, hipMemcpyAsync(d + 2 * nf + nd, unm1_dir, nd * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  const StepFunctions sf{d, d + nf, d + nf + nd, d + 2 * nf + nd, aw, bw, c1, c2};
+  if ((rc = oseen_step_fields(op, sf, laplacian != 0, s))) return rc;
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
   return HFK_OK;
 }
 

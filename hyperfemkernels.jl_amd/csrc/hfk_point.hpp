@@ -338,6 +338,21 @@ __device__ __forceinline__ void point_geometry(const Tab& tb, const double* __re
   dV = tb.wq[a] * tb.wq[b] * tb.wq[cc] * fabs(d);
 }
 
+// Physical coordinates of tensor point q (P points per direction) under the trilinear map of the vertices X[v][i]:
+// the points at which a function of x is materialised (src/coordfunctions.jl:186-213).
+__device__ __forceinline__ void point_coordinates(const Tab& tb, const double* __restrict__ X, int q, int P, double* x) {
+  const int a = q % P, b = (q / P) % P, cc = q / (P * P);
+  const double xi[3] = {tb.xq[a], tb.xq[b], tb.xq[cc]};
+  x[0] = x[1] = x[2] = 0.0;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int bx = v & 1, by = (v >> 1) & 1, bz = (v >> 2) & 1;
+    const double m = (bx ? xi[0] : 1 - xi[0]) * (by ? xi[1] : 1 - xi[1]) * (bz ? xi[2] : 1 - xi[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[i] += m * X[v * 3 + i];
+  }
+}
+
 // Index of the packed upper triangle (i <= j) of a symmetric 9x9 matrix.
 __host__ __device__ constexpr int sym_index(int i, int j) { return j * (j + 1) / 2 + i; }
 

@@ -52,6 +52,27 @@ std::string probe_program(const char* source, const char* name) {
          " law;\n  law.setup(F, F[9], F[10]);\n  out[0] = law.stress(0, 0) + law.tangent(0, 0, 0, 0);\n}\n";
 }
 
+// a function of the physical coordinates, `double name(const double* x, const double* p)`, and the kernel that
+// materialises it at the quadrature points of every cell of the operator's batches into mu_pts' layout
+// [batch][cell*np + q]; padding cells (ids 0) take the constant `fill`
+std::string coord_program(const char* source, const char* name) {
+  return law_source(source) +
+         "extern \"C\" __global__ void __launch_bounds__(256) hfk_coord_fill(hfk::OpDev op, const double* p, "
+         "double fill, double* out) {\n"
+         "  const int ns = op.C * op.np;\n"
+         "  const int tid = threadIdx.x;\n"
+         "  if (tid >= ns) return;\n"
+         "  const size_t slot = (size_t)blockIdx.x * op.C + tid / op.np;\n"
+         "  double v = fill;\n"
+         "  if (op.ids[slot * op.nd] != 0) {\n"
+         "    double x[3];\n"
+         "    hfk::point_coordinates(op.tab, op.coords + slot * 24, tid % op.np, op.P, x);\n"
+         "    v = " + std::string(name) + "(x, p);\n"
+         "  }\n"
+         "  out[(size_t)blockIdx.x * ns + tid] = v;\n"
+         "}\n";
+}
+
 std::vector<std::string> kernel_names(const std::string& name, int Q, int P) {
   const std::string qp = std::to_string(Q) + ", " + std::to_string(P) + ", " + name;
   return {"hfk::k_prepare<" + qp + ", double, false>", "hfk::k_prepare<" + qp + ", double, true>",
@@ -269,6 +290,59 @@ int hfk_law_info(hfk_law* law, uint64_t* key, int* from_cache) {
   if (!law) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null law");
   if (key) *key = law->key;
   if (from_cache) *from_cache = law->from_cache ? 1 : 0;
+  return HFK_OK;
+}
+
+int hfk_op_set_mu_function(hfk_op* op, const char* source, const char* name, const double* params, int64_t nparams) {
+  if (!op || !op->ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  if (!source) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null source");
+  if (!valid_identifier(name)) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the function name must be a C identifier");
+  if (nparams < 0 || (nparams > 0 && !params)) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: invalid parameters");
+  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
+  CacheEntry e;
+  bool cached = false;
+  std::string log;
+  if (!compile_cached(coord_program(source, name), {}, e, cached, log))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the function of x does not compile: " + log);
+  hipStream_t s = op->ctx->stream;
+  DevBuf p;
+  HIPCHK(op->ctx, p.alloc((size_t)std::max<int64_t>(nparams, 1) * sizeof(double)));
+  if (nparams > 0) HIPCHK(op->ctx, hipMemcpyAsync(p.p, params, (size_t)nparams * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, op->mu_pts.alloc((size_t)op->nbatches * op->C * op->np * sizeof(double)));
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  HIPCHK(op->ctx, hipModuleLoadData(&mod, e.code.data()));
+  hipError_t err = hipModuleGetFunction(&fn, mod, "hfk_coord_fill");
+  if (err == hipSuccess) {
+    OpDev d = op->dev();
+    double* pp = p.as<double>();
+    double fill = op->mu;
+    double* out = op->mu_pts.as<double>();
+    void* args[] = {&d, &pp, &fill, &out};
+    err = hipModuleLaunchKernel(fn, (unsigned)op->nbatches, 1, 1, kThreads, 1, 1, 0, s, args, nullptr);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  (void)hipModuleUnload(mod);
+  HIPCHK(op->ctx, err);
+  op->has_mu_field = true;
+  return HFK_OK;
+}
+
+int hfk_op_mu_field(hfk_op* op, double* mu_points) {
+  if (!op || !op->ctx || !mu_points) return set_err(op ? op->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
+  const size_t nslot = (size_t)op->nbatches * op->C;
+  if (!op->has_mu_field) {
+    std::fill(mu_points, mu_points + (size_t)op->ncells * op->np, op->mu);
+    return HFK_OK;
+  }
+  std::vector<double> h(nslot * op->np);
+  HIPCHK(op->ctx, hipStreamSynchronize(op->ctx->stream));
+  HIPCHK(op->ctx, hipMemcpy(h.data(), op->mu_pts.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t sl = 0; sl < nslot; ++sl) {
+    const int64_t cell = op->batch_cell[sl];
+    if (cell >= 0) std::memcpy(&mu_points[(size_t)cell * op->np], &h[sl * op->np], op->np * sizeof(double));
+  }
   return HFK_OK;
 }
 

@@ -35,6 +35,8 @@ SIGNATURES = {
     "hfk_op_destroy": (C.c_int, [_vp]),
     "hfk_op_set_dirichlet": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
+    "hfk_op_set_mu_function": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int64]),
+    "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_op_info": (C.c_int, [_vp, _i64p]),
     "hfk_op_colors": (C.c_int, [_vp, _i32p]),
     "hfk_op_set_deterministic": (C.c_int, [_vp, C.c_int]),
@@ -100,6 +102,10 @@ SIGNATURES = {
     "hfk_oseen_set_fields": (C.c_int, [_vp, _f64p]),
     "hfk_oseen_set_dirichlet": (C.c_int, [_vp, _f64p]),
     "hfk_oseen_set_laplacian": (C.c_int, [_vp, _f64p]),
+    "hfk_oseen_step_fields": (C.c_int, [_vp, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int]),
+    "hfk_oseen_step_fields_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_int, _vp]),
     "hfk_oseen_set_outlet": (C.c_int, [_vp, C.c_int64, _i32p, _f64p, C.c_int, _f64p, C.c_int, _f64p, C.c_double,
                                        C.c_double]),
     "hfk_oseen_set_outlet_pressure": (C.c_int, [_vp, C.c_double]),

@@ -122,6 +122,22 @@ class KernelOperator:
             raise ValueError("HyperFEMKernels: mu field must have np values per cell")
         check(self.lib.hfk_op_set_mu_field(self.handle, ptr(m, C.c_double)), self.ctx.handle)
 
+    def set_mu_function(self, source: str, name: str, params=None):
+        """Shear modulus mu(x), a function of the physical coordinates given as HIP source
+        ``__device__ double name(const double* x, const double* p)``, compiled with hiprtc and evaluated on the
+        device at the quadrature points (the reference's function-of-x leaves, src/coordfunctions.jl:150-213);
+        ``params``: the array ``p`` (may be None)."""
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        check(self.lib.hfk_op_set_mu_function(self.handle, source.encode(), name.encode(),
+                                              None if p is None or p.size == 0 else ptr(p, C.c_double),
+                                              0 if p is None else int(p.size)), self.ctx.handle)
+
+    def mu_field(self) -> np.ndarray:
+        """The shear modulus at the quadrature points, (ncells, np) in tensor point order."""
+        m = np.zeros((self.space.ncells, self.form.points1d ** 3))
+        check(self.lib.hfk_op_mu_field(self.handle, ptr(m, C.c_double)), self.ctx.handle)
+        return m
+
     def set_law(self, law):
         """Evaluate the residual, tangents and Jacobian with a law compiled at run time (``laws.CompiledLaw``;
         ``None``: the form's built-in law).  Prepared tangents are invalidated (prepare again)."""

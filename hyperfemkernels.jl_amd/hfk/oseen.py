@@ -3,7 +3,8 @@ Algebra interface: residual!, sparsity_pattern / nonzero_positions, allocate_jac
 (src/assembly.jl:768-933), over libhfk's hfk_oseen_* entry points.  The weak form is that of test/common.jl:165-228:
 volume terms, the elementwise Laplacian of un in the stabilisation load (``set_laplacian``) and the outlet terms
 (``set_outlet``: backflow, Robin, Windkessel pressure).  The point data (advecting velocity w, stabilisation
-velocity un, history f, Laplacian) come from the host, as Gridap's FE-function leaves would.
+velocity un, history f, Laplacian) are either given at the points by the host (``set_fields``, ``set_laplacian``)
+or computed on the device from the FE functions of the two previous steps (``step_fields``).
 """
 from __future__ import annotations
 
@@ -79,6 +80,35 @@ class OseenOperator:
             raise ValueError("HyperFEMKernels: DimensionMismatch: Laplacian point data")
         lap = np.ascontiguousarray(lap)
         check(self.lib.hfk_oseen_set_laplacian(self.handle, ptr(lap, C.c_double)), self.ctx.handle)
+
+    def step_coefficients(self):
+        """(aw, bw, c1, c2) of the step (test/common.jl:185-190): w = aw un + bw unm1 and f = c1 un - c2 unm1; BDF2
+        (2, -1, 2/dt, 1/(2 dt)) or backward Euler (1, 0, 1/dt, 0)."""
+        if self.bdf2:
+            return 2.0, -1.0, 2.0 / self.dt, 0.5 / self.dt
+        return 1.0, 0.0, 1.0 / self.dt, 0.0
+
+    def step_fields(self, un, un_dir, unm1, unm1_dir, laplacian: bool = True, coefficients=None):
+        """The point data of the step computed on the device from the FE functions un = u^n and unm1 = u^{n-1}:
+        free values (nfree; pressure entries not read) and Dirichlet values (ndir) of each.  Sets w, un, f at the
+        points, the Laplacian of un (``laplacian``; False removes the term) and w at the outlet backflow points;
+        ``coefficients`` (aw, bw, c1, c2) default to ``step_coefficients()``."""
+        aw, bw, c1, c2 = coefficients if coefficients is not None else self.step_coefficients()
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (un, un_dir, unm1, unm1_dir)]
+        if a[0].size != self.nfree or a[2].size != self.nfree or a[1].size != self.ndir or a[3].size != self.ndir:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: FE-function values")
+        check(self.lib.hfk_oseen_step_fields(self.handle, *[ptr(v, C.c_double) for v in a], float(aw), float(bw),
+                                             float(c1), float(c2), int(bool(laplacian))), self.ctx.handle)
+
+    def step_fields_dev(self, un_ptr: int, un_dir_ptr: int, unm1_ptr: int, unm1_dir_ptr: int, laplacian: bool = True,
+                        coefficients=None, stream: int = 0):
+        """``step_fields`` with device pointers (torch tensors' data_ptr() or raw addresses), asynchronous on
+        ``stream``."""
+        aw, bw, c1, c2 = coefficients if coefficients is not None else self.step_coefficients()
+        check(self.lib.hfk_oseen_step_fields_dev(self.handle, C.c_void_p(un_ptr), C.c_void_p(un_dir_ptr),
+                                                 C.c_void_p(unm1_ptr), C.c_void_p(unm1_dir_ptr), float(aw), float(bw),
+                                                 float(c1), float(c2), int(bool(laplacian)), C.c_void_p(stream)),
+              self.ctx.handle)
 
     def set_outlet(self, faces, w=None, Pc: float = 0.0, ZA: float = 0.0):
         """Outlet terms on ``faces`` (``tetmesh.OutletFaces``): backflow -rho negpart(w.n) u.v, Robin ZA (u.n)(v.n),

@@ -82,3 +82,23 @@ def lognormal_field(seed: int, sigma: float = 1.0, ell: float = 0.1, features: i
         return out
 
     return f
+
+
+# mu(x) = p[0] exp(p[1] sqrt(2/M) sum_m cos(w_m . x + b_m)), M = p[2], w = p[3 .. 3+3M) (m-major), b = p[3+3M ..):
+# the lognormal field above as a function of x for KernelOperator.set_mu_function (evaluated on the device)
+LOGNORMAL_MU_SOURCE = r"""
+__device__ double lognormal_mu(const double* x, const double* p) {
+  const int M = (int)p[2];
+  const double* w = p + 3;
+  const double* b = p + 3 + 3 * M;
+  double g = 0.0;
+  for (int m = 0; m < M; ++m) g += cos(w[3 * m] * x[0] + w[3 * m + 1] * x[1] + w[3 * m + 2] * x[2] + b[m]);
+  return p[0] * exp(p[1] * sqrt(2.0 / M) * g);
+}
+"""
+
+
+def lognormal_mu_params(seed: int, sigma: float = 1.0, ell: float = 0.1, features: int = 1024, mu0: float = 1.0):
+    """The parameter array of LOGNORMAL_MU_SOURCE for mu0 * lognormal_field(seed, sigma, ell, features)."""
+    w, b = lognormal_features(seed, ell, features)
+    return np.concatenate([[mu0, sigma, float(features)], w.reshape(-1), b])

@@ -73,6 +73,22 @@ end
 set_dirichlet!(op::DeviceOperator, d::Vector{Float64}) =
     check(ccall((:hfk_op_set_dirichlet, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, d), op.ctx.ptr)
 
+"Pointwise shear modulus at the quadrature points (np × ncells, tensor point order); `nothing`: the constant μ."
+set_mu_field!(op::DeviceOperator, μ::Union{Nothing,Matrix{Float64}}) =
+    check(ccall((:hfk_op_set_mu_field, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, μ === nothing ? C_NULL : μ),
+          op.ctx.ptr)
+
+"""
+    set_mu_function!(op, source, name, params)
+
+A function-of-x leaf (`CoordArray`, src/coordfunctions.jl:150-213) as the shear modulus, evaluated on the device at the
+physical quadrature points: `source` is HIP C++ defining `__device__ double name(const double* x, const double* p)`,
+compiled with hiprtc (cached by source text); `params` is the array `p`.
+"""
+set_mu_function!(op::DeviceOperator, source::AbstractString, name::AbstractString, params::Vector{Float64}=Float64[]) =
+    check(ccall((:hfk_op_set_mu_function, libhfk), Cint, (Ptr{Cvoid}, Cstring, Cstring, Ptr{Float64}, Int64), op.ptr,
+                source, name, isempty(params) ? C_NULL : params, length(params)), op.ctx.ptr)
+
 """
     CompiledLaw(ctx, source, name)
 
@@ -269,6 +285,21 @@ set_dirichlet!(op::DeviceOseen, dvals::Vector{Float64}) =
 set_laplacian!(op::DeviceOseen, lap::Union{Nothing,Array{Float64,3}}) =
     check(ccall((:hfk_oseen_set_laplacian, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr,
                 lap === nothing ? C_NULL : lap), op.ctx.ptr)
+
+"""
+    step_fields!(op, un, un_dir, unm1, unm1_dir; aw, bw, c1, c2, laplacian=true)
+
+The point data of a step from the FE functions uⁿ (`xh⁻`) and uⁿ⁻¹ (`xh⁻⁻`) of test/common.jl:183-196, computed on
+the device: their free values (`get_free_dof_values`, pressure entries not read) and the Dirichlet values of their
+velocity spaces; w = aw uⁿ + bw uⁿ⁻¹, f = c1 uⁿ - c2 uⁿ⁻¹, the elementwise Laplacian of uⁿ when `laplacian`, and w at the
+backflow points of the outlet faces.
+"""
+step_fields!(op::DeviceOseen, un::Vector{Float64}, un_dir::Vector{Float64}, unm1::Vector{Float64},
+             unm1_dir::Vector{Float64}; aw::Real, bw::Real, c1::Real, c2::Real, laplacian::Bool=true) =
+    check(ccall((:hfk_oseen_step_fields, libhfk), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cdouble,
+                 Cint),
+                op.ptr, un, un_dir, unm1, unm1_dir, aw, bw, c1, c2, laplacian), op.ctx.ptr)
 
 """
     set_outlet!(op, face_cell, normals, xo, xb, Pc, ZA)

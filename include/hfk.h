@@ -74,6 +74,16 @@ int hfk_op_set_dirichlet(hfk_op* op, const double* dvals);
 /* Pointwise shear modulus mu(x) at the quadrature points, np*ncells in tensor point order, [cell][q]
  * (dense auxiliary point data; NULL restores the constant mu). */
 int hfk_op_set_mu_field(hfk_op* op, const double* mu_points);
+/* A function of the physical coordinates materialised on the device at the quadrature points as the pointwise
+ * shear modulus (the function-of-x leaves of src/coordfunctions.jl:150-213, materialize_coord!: M[q, cell] =
+ * f(X[q, cell]) at the physical points X of the cell's quadrature points).  `source` (HIP) defines
+ *   __device__ double name(const double* x, const double* p);     x[3] the physical point, p the parameters
+ * compiled with hiprtc for gfx950 (cached per process by the source text, like the laws); params: nparams doubles
+ * copied to the device (nparams may be 0).  The points are the trilinear images of the tensor Gauss points.  Replaces
+ * a field of hfk_op_set_mu_field; compile errors return HFK_EARG with the compiler log. */
+int hfk_op_set_mu_function(hfk_op* op, const char* source, const char* name, const double* params, int64_t nparams);
+/* The operator's shear modulus at the points, np*ncells, [cell][q] (the constant mu without a field). */
+int hfk_op_mu_field(hfk_op* op, double* mu_points);
 /* info[0..7] = nfree, ndir, ncells, nd, np, ncolors, tangent bytes of the current storage, nnz (or -1). */
 int hfk_op_info(hfk_op* op, int64_t* info);
 /* Colour of each cell, 1-based (the reference's `color[cell]`, src/coloring.jl:4-28). */
@@ -281,6 +291,20 @@ int hfk_oseen_set_dirichlet(hfk_oseen* op, const double* dvals);
 /* The elementwise Laplacian lapn of un at the points, ncells x np x 3 (test/common.jl:191-195: rk = rho f + mu lapn
  * in the SUPG/PSPG load); NULL removes the term. */
 int hfk_oseen_set_laplacian(hfk_oseen* op, const double* lap_points);
+/* The step's point data materialised on the device from the FE functions of the two previous steps (the FE-function
+ * leaves of test/common.jl:183-196, which src/fefunctions.jl:156+ and src/assembly.jl:936-1052 materialise on the CPU;
+ * the time loop of test/test_cfd.jl:47-75 overwrites them in place every step): un = u^n and unm1 = u^{n-1}, each
+ * given by its free values (nfree, the operator's numbering; pressure entries are not read) and its Dirichlet values
+ * (ndir).  At every quadrature point w = aw un + bw unm1 (advecting velocity: aw = 1, bw = 0 for backward Euler,
+ * 2, -1 for BDF2), un, and f = c1 un - c2 unm1 (history); with laplacian != 0 the elementwise Laplacian of un
+ * (test/common.jl:191-195), otherwise that term is removed; and w at the dGb points of the outlet faces.  Replaces
+ * hfk_oseen_set_fields, hfk_oseen_set_laplacian and the w of hfk_oseen_set_outlet.  _dev: device pointers,
+ * asynchronous on `stream`. */
+int hfk_oseen_step_fields(hfk_oseen* op, const double* un, const double* un_dir, const double* unm1,
+                          const double* unm1_dir, double aw, double bw, double c1, double c2, int laplacian);
+int hfk_oseen_step_fields_dev(hfk_oseen* op, const double* un, const double* un_dir, const double* unm1,
+                              const double* unm1_dir, double aw, double bw, double c1, double c2, int laplacian,
+                              void* stream);
 /* Outlet faces (test/common.jl:197-200, 212-213, 219): a += -rho negpart(w.n) u.v dGb + ZA (u.n)(v.n) dGo and
  * l += -Pc v.n dGo.  Per face: its cell (0-based), the outward unit normal (nfaces x 3), the nqo points of dGo as
  * [face][q][4] = reference coordinates in the cell (3) and weight dS, the nqb points of dGb as [face][q][7] =

@@ -79,6 +79,23 @@ def gauss01(P):
     return x, w
 
 
+def physical_points(coords, P):
+    """Physical coordinates (ncells, P^3, 3) of the tensor Gauss points q = a + P b + P^2 c under the trilinear maps
+    of the cells' vertices coords (ncells, 8, 3), vertex v = i + 2j + 4k (x = sum_v M_v(xi) x_v, src/assembly.jl:
+    1073-1111): the points at which the reference materialises a function of x (src/coordfunctions.jl:186-213)."""
+    g, _ = gauss01(P)
+    coords = np.asarray(coords, dtype=np.float64)
+    out = np.zeros((coords.shape[0], P ** 3, 3))
+    for q in range(P ** 3):
+        xi = (g[q % P], g[(q // P) % P], g[q // (P * P)])
+        for v in range(8):
+            m = 1.0
+            for d in range(3):
+                m *= xi[d] if (v >> d) & 1 else 1.0 - xi[d]
+            out[:, q, :] += m * coords[:, v, :]
+    return out
+
+
 def lagrange1d(Q, xi):
     N, dN = np.zeros(Q), np.zeros(Q)
     load().orc_lagrange1d(Q, float(xi), _p(N, C.c_double), _p(dN, C.c_double))

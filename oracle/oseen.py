@@ -128,6 +128,43 @@ def element_values(cell_dofs, x, dvals):
     return xe
 
 
+REF_VERTS = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+
+
+def step_fields(cell_dofs, coords, qp, un, un_dir, unm1, unm1_dir, aw, bw, c1, c2):
+    """The point data of a step built from the FE functions of the two previous steps as the reference's forms
+    build them (test/common.jl:183-196): un, unm1 evaluated at the points through the P2 basis, w = aw un + bw unm1
+    (wadv), f = c1 un - c2 unm1 (hist); the Laplacian by the reference's own route (:191-195): grad(un) . e_k
+    interpolated into the discontinuous P1 space (its values at the 4 vertices of each cell), then the divergence
+    of that P1 field.  Returns fields (ncells, np, 9) = w, un, f and lap (ncells, np, 3)."""
+    ids = np.asarray(cell_dofs)[:, :30]
+    coords = np.asarray(coords)
+    ue = element_values(ids, un, un_dir).reshape(-1, 3, 10)
+    ve = element_values(ids, unm1, unm1_dir).reshape(-1, 3, 10)
+    N = basis(qp)[0][:, :10]
+    u = np.einsum("qa,cka->cqk", N, ue)
+    v = np.einsum("qa,cka->cqk", N, ve)
+    fields = np.concatenate([aw * u + bw * v, u, c1 * u - c2 * v], axis=2)
+    Jinv = np.linalg.inv(np.transpose(coords[:, 1:, :] - coords[:, :1, :], (0, 2, 1)))  # [cell][d][i]
+    Gv = basis(REF_VERTS)[1][:, :10]  # reference gradients of the P2 functions at the vertices (4, 10, 3)
+    gu = np.einsum("cka,mad,cdi->cmki", ue, Gv, Jinv)  # d un_k / d x_i at vertex m
+    dlam = np.einsum("md,cdi->cmi", GL, Jinv)  # physical gradients of the P1 functions
+    lapc = np.einsum("cmki,cmi->ck", gu, dlam)  # div of sum_m gu[m, k, :] lambda_m
+    lap = np.repeat(lapc[:, None, :], np.asarray(qp).shape[0], axis=1)
+    return fields, lap
+
+
+def face_velocity(cell_dofs, face_cell, xref, un, un_dir, unm1, unm1_dir, aw, bw):
+    """w = aw un + bw unm1 at face points given by reference coordinates in their cell, xref (nfaces, nq, 3)."""
+    xref = np.asarray(xref)
+    nf, nq = xref.shape[:2]
+    ids = np.asarray(cell_dofs)[np.asarray(face_cell), :30]
+    ue = element_values(ids, un, un_dir).reshape(nf, 3, 10)
+    ve = element_values(ids, unm1, unm1_dir).reshape(nf, 3, 10)
+    N = basis(xref.reshape(-1, 3))[0][:, :10].reshape(nf, nq, 10)
+    return np.einsum("fqa,fka->fqk", N, aw * ue + bw * ve)
+
+
 def residual(space, x, dvals, K, load):
     """b[free test] = sum over cells of (K x_e - l) (Dirichlet values lifted)."""
     ids = space.cell_dofs

@@ -200,3 +200,36 @@ def test_generated_extra_law_invariants(hip_ctx):
     y = MatrixFreeJacobian(op, "full64").prepare(xs) @ v
     fd = (op.residual(np.zeros(sp.nfree), xs + h * v) - op.residual(np.zeros(sp.nfree), xs - h * v)) / (2 * h)
     assert rel(fd, y) < 1e-7
+
+
+# ---- functions of x materialised on the device (src/coordfunctions.jl:150-213) ------------------------------
+
+def test_mu_function_of_x(oracle, hip_ctx):
+    """A shear modulus given as a function of x (HIP source, compiled with hiprtc) is evaluated at the physical
+    points of the quadrature points, on non-affine cells: the lognormal random-Fourier-feature field and a
+    polynomial without parameters equal the same functions at the oracle's physical points (1e-13), and the
+    tangents prepared with the device field equal those with the same field given point by point (1e-12)."""
+    from hfk.synthetic import LOGNORMAL_MU_SOURCE, lognormal_mu_params
+
+    sp, dv, x, rng = problem(2, (3, 2, 2), 0.1, seed=9)
+    form = HyperelasticForm("neohookean", 2.0, 1.0, degree=4)
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    ref = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    assert np.array_equal(op.mu_field(), np.full((sp.ncells, 27), 1.0))
+    X = oracle.physical_points(sp.coords, 3)
+    p = lognormal_mu_params(4, sigma=1.0, ell=0.3, features=64, mu0=1.5)
+    op.set_mu_function(LOGNORMAL_MU_SOURCE, "lognormal_mu", p)
+    w, b = p[3:3 + 192].reshape(64, 3), p[3 + 192:]
+    expect = 1.5 * np.exp(np.sqrt(2.0 / 64) * np.cos(X @ w.T + b).sum(axis=2))
+    got = op.mu_field()
+    assert np.abs(got / expect - 1.0).max() < 1e-13
+    ref.set_mu_field(expect)
+    dx = rng.standard_normal(sp.nfree)
+    y = MatrixFreeJacobian(op, "full64").prepare(x) @ dx
+    assert rel(y, MatrixFreeJacobian(ref, "full64").prepare(x) @ dx) < TOL
+    assert rel(op.residual(np.zeros(sp.nfree), x), ref.residual(np.zeros(sp.nfree), x)) < TOL
+    poly = "__device__ double poly(const double* x, const double*) { return 1.0 + x[0] * x[0] + 2.0 * x[1] * x[2]; }"
+    op.set_mu_function(poly, "poly")
+    assert np.abs(op.mu_field() - (1.0 + X[..., 0] ** 2 + 2.0 * X[..., 1] * X[..., 2])).max() < 1e-14
+    with pytest.raises(RuntimeError, match="does not compile"):
+        op.set_mu_function("__device__ double bad(const double* x, const double* p) { return y; }", "bad")

@@ -145,3 +145,65 @@ def test_oseen_atomic_single_launch(hip_ctx):
     assert rel(b1, b0) < TOL and rel(A1.data, A0.data) < TOL
     op.set_deterministic(True)
     assert np.array_equal(op.residual(np.zeros(sp.nfree), x), b0)  # colour order: bitwise reproducible
+
+
+@pytest.mark.parametrize("bdf2", [True, False])
+def test_oseen_step_fields_time_loop(hip_ctx, bdf2):
+    """The point data of each step made on the device from the FE functions of the two previous steps (w, un, f,
+    the Laplacian of un and w at the outlet backflow points), over two steps of the time loop of
+    test/test_cfd.jl:47-75 (previous states shifted, Dirichlet values of the previous velocity rescaled, the
+    Windkessel pressure changed): residual and CSC values equal to the oracle's system built from the same FE
+    functions (1e-12); the Laplacian term removed on request; the device-pointer variant gives the same values."""
+    import torch
+
+    from oracle import oseen as O
+
+    from hfk.tetmesh import outlet_faces
+
+    print(f"oseen step fields bdf2={bdf2}", flush=True)
+    sp = tet_p2p1_space((3, 2, 2), domain=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0))
+    qp, qw = tet_quadrature_deg5()
+    rng = np.random.default_rng(17)
+    dv = rng.standard_normal(sp.ndir)
+    op = OseenOperator(sp, rho=1.0, mu=0.05, dt=0.01, bdf2=bdf2, dirichlet_values=dv, ctx=hip_ctx)
+    of = outlet_faces(sp)
+    op.set_outlet(of, None, Pc=0.7, ZA=2.3)
+    aw, bw, c1, c2 = op.step_coefficients()
+    un, unm1 = rng.standard_normal(sp.nfree), rng.standard_normal(sp.nfree)
+    und, unm1d = rng.standard_normal(sp.ndir), rng.standard_normal(sp.ndir)
+    x = rng.standard_normal(sp.nfree)
+    colptr, rowval, nzpos = op.pattern()
+
+    def oracle_system(Pc, with_lap=True):
+        fields, lap = O.step_fields(sp.cell_dofs, sp.coords, qp, un, und, unm1, unm1d, aw, bw, c1, c2)
+        K, load = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.05, op.c0, lap=lap if with_lap else None)
+        wb = O.face_velocity(sp.cell_dofs, of.cell, of.xb[..., :3], un, und, unm1, unm1d, aw, bw)
+        Kf, Lf = O.outlet_terms(of.xo, np.concatenate([of.xb, wb], axis=2), of.normal, 1.0, Pc, 2.3)
+        return O.add_faces(K, load, of.cell, Kf, Lf)
+
+    for step, Pc in enumerate((0.7, -1.3)):
+        op.set_outlet_pressure(Pc)
+        op.step_fields(un, und, unm1, unm1d)
+        Kt, lt = oracle_system(Pc)
+        b = op.residual(np.zeros(sp.nfree), x)
+        assert rel(b, O.residual(sp, x, dv, Kt, lt)) < TOL
+        A = op.jacobian(op.allocate_jacobian())
+        assert rel(A.data, O.jacobian_values(Kt, nzpos, rowval.size)) < TOL
+        # the next step: previous states shifted, new u^n, Dirichlet values of the previous velocities rescaled
+        unm1, un = un, rng.standard_normal(sp.nfree)
+        unm1d, und = und, 0.4 * und
+    # the Laplacian term removed
+    op.step_fields(un, und, unm1, unm1d, laplacian=False)
+    Kt, lt = oracle_system(-1.3, with_lap=False)
+    b0 = op.residual(np.zeros(sp.nfree), x)
+    assert rel(b0, O.residual(sp, x, dv, Kt, lt)) < TOL
+    # device pointers, asynchronous on the context's stream
+    dev = torch.device("cuda:0")
+    t = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (un, und, unm1, unm1d)]
+    torch.cuda.synchronize()
+    op.step_fields_dev(*[v.data_ptr() for v in t])
+    Kt, lt = oracle_system(-1.3)
+    b1 = op.residual(np.zeros(sp.nfree), x)
+    assert rel(b1, O.residual(sp, x, dv, Kt, lt)) < TOL
+    op.step_fields(un, und, unm1, unm1d)
+    assert np.array_equal(op.residual(np.zeros(sp.nfree), x), b1)

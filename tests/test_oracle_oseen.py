@@ -16,6 +16,8 @@ Each term of the element matrix is checked against what the mathematics fixes, o
   Robin block sums to ZA |outlet| and the pressure load to -Pc |outlet|;
 * Laplacian: the P2 elementwise Laplacian of quadratic fields; its load rows in closed form (PSPG
   -mu tp |K| lap . grad l_a; SUPG zero at the vertices, ts mu lap_i |K| w . (grad l_a + grad l_b) at edge (a, b)).
+* step point data from FE functions: for quadratic u^n, u^{n-1} the point values, w, f, the Laplacian (by the
+  reference's discontinuous-P1 route) and the outlet-face w equal the polynomials' closed forms.
 """
 import numpy as np
 import pytest
@@ -333,3 +335,41 @@ def test_laplacian_load_closed_forms(mesh):
         for e, (a, b) in enumerate(EDGES):
             ref = ts * mu * lap[i] * vol * ((G[:, a] + G[:, b]) @ w)
             assert np.allclose(rows[:, 4 + e], ref, rtol=1e-11, atol=1e-15)
+
+
+def test_step_fields_closed_forms(channel):
+    """The point data of a step from two quadratic velocity fields p (u^n) and q (u^{n-1}), which P2 holds exactly:
+    the point values are the polynomials at the physical points, w = 2p - q and f = c1 p - c2 q (BDF2), the
+    Laplacian (the reference's route through the discontinuous P1 gradient) is the polynomials' own, and the face
+    velocity at the outlet points is 2p - q there."""
+    sp, of = channel
+    qp, qw = tet_quadrature_deg5()
+    A = np.array([[[1.0, 0.3, -0.2], [0.3, -0.5, 0.1], [-0.2, 0.1, 0.7]],
+                  [[0.2, 0.0, 0.4], [0.0, 1.1, -0.3], [0.4, -0.3, -0.6]],
+                  [[-0.8, 0.5, 0.0], [0.5, 0.2, 0.2], [0.0, 0.2, 0.9]]])
+    B = np.array([[0.5, -1.0, 0.2], [0.0, 0.3, 1.0], [1.5, 0.1, -0.4]])
+
+    def p(X):
+        return np.einsum("cij,mi,mj->mc", A, X, X) + X @ B.T + np.array([0.1, -0.2, 0.3])
+
+    def q(X):
+        return np.stack([X[:, 1] * X[:, 2], X[:, 0] ** 2 - X[:, 2], 2.0 - X[:, 1] ** 2], 1)
+
+    rng = np.random.default_rng(3)
+    un = rng.standard_normal(sp.nfree)  # the pressure entries are not read
+    unm1 = rng.standard_normal(sp.nfree)
+    un[:sp.nfree_u] = p(sp.node_xyz[sp.free_node_of]).reshape(-1)
+    unm1[:sp.nfree_u] = q(sp.node_xyz[sp.free_node_of]).reshape(-1)
+    c1, c2 = 200.0, 50.0
+    fields, lap = O.step_fields(sp.cell_dofs, sp.coords, qp, un, sp.dirichlet_values(p), unm1, sp.dirichlet_values(q),
+                                2.0, -1.0, c1, c2)
+    X = sp.quadrature_points(qp).reshape(-1, 3)
+    P, Q = p(X).reshape(sp.ncells, -1, 3), q(X).reshape(sp.ncells, -1, 3)
+    assert np.abs(fields[..., 3:6] - P).max() < 1e-13
+    assert np.abs(fields[..., 0:3] - (2 * P - Q)).max() < 1e-13
+    assert np.abs(fields[..., 6:9] - (c1 * P - c2 * Q)).max() < 1e-10
+    assert np.abs(lap - 2.0 * np.trace(A, axis1=1, axis2=2)).max() < 1e-11
+    wb = O.face_velocity(sp.cell_dofs, of.cell, of.xb[..., :3], un, sp.dirichlet_values(p), unm1,
+                         sp.dirichlet_values(q), 2.0, -1.0)
+    Xb = of.xb_phys.reshape(-1, 3)
+    assert np.abs(wb - (2 * p(Xb) - q(Xb)).reshape(wb.shape)).max() < 1e-13

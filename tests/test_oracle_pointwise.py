@@ -4,6 +4,8 @@
   exactness up to degree 2P - 1.
 * 1D Lagrange (src/tensorproduct.jl:24-47): the reference's known answers lagrange1d(3, 0.5) = [0, 1, 0],
   partition of unity and derivative sum 0 (test/test_matrixfree.jl:9-17), derivatives vs finite differences.
+* Physical points of the tensor Gauss points: affine images on an affine cell (x fastest), the vertex mean for
+  the one-point rule.
 * Laws: P = dW/dF and T = dP/dF by central differences of the energy, the SVK closed forms of
   test/common.jl:2-8 written with numpy matrices, and C_svk at F = I (test/common.jl:11-20).
 """
@@ -28,6 +30,20 @@ def test_gauss_exactness(oracle, P):
         assert np.dot(w, x ** deg) == pytest.approx(1.0 / (deg + 1), abs=1e-14)
     # and not exact one degree higher (a wrong rule that happens to be exact further would be suspicious)
     assert abs(np.dot(w, x ** (2 * P)) - 1.0 / (2 * P + 1)) > 1e-8
+
+
+def test_physical_points_closed_forms(oracle):
+    """An affine (sheared, scaled) cell maps the Gauss points affinely, x fastest; the one-point rule of any
+    trilinear cell is the mean of its vertices."""
+    A = np.array([[0.5, 0.1, 0.0], [0.2, 0.3, -0.1], [0.0, 0.05, 0.7]])
+    b = np.array([1.0, -2.0, 0.5])
+    V = np.array([[v & 1, (v >> 1) & 1, (v >> 2) & 1] for v in range(8)], dtype=float)
+    coords = (V @ A.T + b)[None]
+    g, _ = oracle.gauss01(3)
+    xi = np.array([[g[a], g[bb], g[c]] for c in range(3) for bb in range(3) for a in range(3)])
+    assert np.abs(oracle.physical_points(coords, 3)[0] - (xi @ A.T + b)).max() < 1e-15
+    X = np.random.default_rng(2).standard_normal((4, 8, 3))
+    assert np.abs(oracle.physical_points(X, 1)[:, 0] - X.mean(axis=1)).max() < 1e-15
 
 
 def test_lagrange_known_answers(oracle):
