@@ -324,10 +324,11 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     form = HyperelasticForm("neohookean", 2.0, 1.0, degree=2 * order)
     fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u))
     cop = KernelOperator(form, cs, jacobian="assembled")
+    t_mu = None
     if mu is not None:  # lognormal shear modulus at the quadrature points (the coarse level keeps the fine rule)
-        mup = lognormal_points(mu, fs.quadrature_points(form.points1d).reshape(-1, 3))
-        fop.set_mu_field(mup)
-        cop.set_mu_field(mup)
+        t0 = time.perf_counter()
+        set_lognormal_mu(mu, fop, cop)
+        t_mu = time.perf_counter() - t0
     b = -fop.residual(np.zeros(fs.nfree), x)
     mf = MatrixFreeJacobian(fop, outer_storage)
     pmg = PMultigrid([fop, cop], Chebyshev(degree=degree), coarse_rtol=coarse_rtol, coarse_maxiter=coarse_maxiter,
@@ -364,6 +365,8 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
                "free_dofs": fs.nfree, "coarse_free_dofs": cs.nfree}
     if train:
         out.update(train)
+    if t_mu is not None:  # both levels, first call: includes the hiprtc compile of the function (cached after)
+        out["mu_function_s"] = t_mu
     return out
 
 
@@ -387,6 +390,16 @@ def lognormal_points(mu, xyz):
     return out
 
 
+def set_lognormal_mu(mu, *ops):
+    """The lognormal shear modulus exp(sigma g(x)) of hfk.synthetic as a function of x, compiled with hiprtc and
+    evaluated by the library at every operator's quadrature points on the device."""
+    from hfk.synthetic import LOGNORMAL_MU_SOURCE, lognormal_mu_params
+
+    p = lognormal_mu_params(mu["seed"], sigma=mu["sigma"], ell=mu["ell"])
+    for op in ops:
+        op.set_mu_function(LOGNORMAL_MU_SOURCE, "lognormal_mu", p)
+
+
 def lognormal_problem(n, seed, state_amp=0.0):
     """One problem of the learned-preconditioner workload: Q2 neo-Hookean (lam = 2) on the unit cube n^3, both
     x faces clamped, shear modulus exp(sigma g) with sigma in [0.5, 2] and correlation length ell in [0.05, 0.3]
@@ -408,9 +421,7 @@ def lognormal_problem(n, seed, state_amp=0.0):
     form = HyperelasticForm("neohookean", 2.0, 1.0, degree=4)
     fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u))
     cop = KernelOperator(form, cs, jacobian="assembled")
-    mup = lognormal_points(mu, fs.quadrature_points(form.points1d).reshape(-1, 3))
-    fop.set_mu_field(mup)
-    cop.set_mu_field(mup)
+    set_lognormal_mu(mu, fop, cop)
     b = smooth_field(seed + 1, amp=1.0)(fs.node_xyz[fs.free_node_of]).reshape(-1)
     aux = np.log(lognormal_points(mu, cs.node_xyz)) / 2.0
     return {"fs": fs, "cs": cs, "fop": fop, "cop": cop, "x": fs.free_values(u), "b": b, "aux": aux, "mu": mu}
@@ -627,8 +638,9 @@ def precision_leg(n, order, stream, storages=("full64", "full32", "sym32"), step
 def compiled_law_leg(n, stream, steps=5):
     """The 64^3 Q2 neo-Hookean workload with the law compiled at run time (hiprtc) beside the built-in kernels: the
     law written as HIP source, and the law generated from its energy by hfk.codegen (stress and tangent derived
-    from the expression tree); compile times, prepare! per storage, residual! and the relative difference of Jv
-    and of the residual against the built-in kernels."""
+    from the expression tree); compile times, prepare! per storage, residual!, the strain energy (the functional of
+    a line search) and the relative difference of Jv, of the residual and of the energy against the built-in
+    kernels."""
     import ctypes as C
 
     import numpy as np
@@ -679,9 +691,16 @@ def compiled_law_leg(n, stream, steps=5):
     gen = CompiledLaw(codegen.from_energy(codegen.neohooke_energy).source("GenNeoHooke"), "GenNeoHooke", ctx=op.ctx)
     out["generated_check_compile_s"] = time.perf_counter() - t0
     res = {}
+    ed = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    def energy():  # the strain energy (KernelFunctional, reduce = sum): per-batch sums, then their sum
+        check(op.lib.hfk_energy_dev(op.handle, C.c_void_p(xd.data_ptr()), C.c_void_p(ed.data_ptr()), C.c_void_p(sptr)),
+              op.ctx.handle)
+
     for tag, lw in (("builtin", None), ("compiled", law), ("generated", gen)):
         op.set_law(lw)
-        r = {"residual_ms": timed(residual)}
+        r = {"residual_ms": timed(residual), "energy_ms": timed(energy)}
+        r["energy"] = float(ed.cpu().numpy()[0])
         b = bd.cpu().numpy()
         for st in ("full64", "sym32"):
             mf = MatrixFreeJacobian(op, st)
@@ -698,6 +717,7 @@ def compiled_law_leg(n, stream, steps=5):
         out[tag]["rel_diff_residual"] = rel(res[tag]["b"], res["builtin"]["b"])
         out[tag]["rel_diff_jv_full64"] = rel(res[tag]["jv_full64"], res["builtin"]["jv_full64"])
         out[tag]["rel_diff_jv_sym32"] = rel(res[tag]["jv_sym32"], res["builtin"]["jv_sym32"])
+        out[tag]["rel_diff_energy"] = abs(res[tag]["energy"] / res["builtin"]["energy"] - 1.0)
     op.set_law(None)
     return out
 

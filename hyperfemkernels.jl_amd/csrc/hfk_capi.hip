@@ -317,6 +317,20 @@ int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) 
   return HFK_OK;
 }
 
+// strain energy into out_dev[0]: per-batch sums, then their sum, both in a fixed order
+int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) {
+  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches, 1) * sizeof(double)));
+  double* part = op->part.as<double>();
+  if (op->rtc) {
+    int rc = rtc_energy(op, x_dev, part, s);
+    if (rc) return rc;
+  } else {
+    HIPCHK(op->ctx, launch_energy(op->dev(), x_dev, part, op->nbatches, s));
+  }
+  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches, 0, out_dev, s));
+  return HFK_OK;
+}
+
 int stage_in(hfk_op* op, DevBuf& buf, const double* h, int64_t n) {
   HIPCHK(op->ctx, buf.alloc((size_t)std::max<int64_t>(n, 1) * sizeof(double)));
   if (n > 0) HIPCHK(op->ctx, hipMemcpyAsync(buf.p, h, (size_t)n * sizeof(double), hipMemcpyHostToDevice, op->ctx->stream));
@@ -597,6 +611,24 @@ int hfk_residual_dev(hfk_op* op, const double* x_dev, double* b_dev, void* strea
   if (rc) return rc;
   if (!x_dev || !b_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null vector");
   return run_residual(op, x_dev, b_dev, op->stream(stream));
+}
+
+int hfk_energy(hfk_op* op, const double* x, double* energy) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x || !energy) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  if ((rc = stage_in(op, op->x_stage, x, op->nfree))) return rc;
+  HIPCHK(op->ctx, op->y_stage.alloc((size_t)std::max<int64_t>(op->nfree, 1) * sizeof(double)));
+  if ((rc = run_energy(op, op->x_stage.as<double>(), op->y_stage.as<double>(), op->ctx->stream))) return rc;
+  HIPCHK(op->ctx, hipMemcpyAsync(energy, op->y_stage.p, sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+  return sync(op);
+}
+
+int hfk_energy_dev(hfk_op* op, const double* x_dev, double* energy_dev, void* stream) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x_dev || !energy_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  return run_energy(op, x_dev, energy_dev, op->stream(stream));
 }
 
 int hfk_mf_prepare(hfk_op* op, const double* x, int storage) {

@@ -38,6 +38,10 @@ hipError_t launch_apply(const OpDev& op, int storage, const void* tangents, cons
 hipError_t launch_blocks(const OpDev& op, int storage, const void* tangents, const int32_t* block_of, double* B,
                          BatchRange r, hipStream_t s);
 hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRange r, hipStream_t s);
+// strain energy of the built-in law per batch into part[nbatches] (k_energy)
+hipError_t launch_energy(const OpDev& op, const double* x, double* part, int64_t nbatches, hipStream_t s);
+// out[0] = sum (mode 0), min (1) or max (2) of part[0..n) in a fixed order
+hipError_t launch_reduce_partials(const double* part, int64_t n, int mode, double* out, hipStream_t s);
 hipError_t launch_assemble(const OpDev& op, const double* tangents64, double* nzval, BatchRange r, hipStream_t s);
 hipError_t launch_fill(double* y, int64_t n, double v, hipStream_t s);
 hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* tangents, double* D, int64_t nbatches,

@@ -331,6 +331,17 @@ hipError_t residual_law(const OpDev& op, const double* x, double* bvec, BatchRan
 }
 
 template <int Q, int P>
+hipError_t energy_law(const OpDev& op, const double* x, double* part, int64_t nbatches, hipStream_t s) {
+  const dim3 g((unsigned)nbatches), b(kThreads);
+  switch (op.law) {
+    case 0: hipLaunchKernelGGL((k_energy<Q, P, LawSVK>), g, b, 0, s, op, x, part); break;
+    case 1: hipLaunchKernelGGL((k_energy<Q, P, LawNeoHooke>), g, b, 0, s, op, x, part); break;
+    default: hipLaunchKernelGGL((k_energy<Q, P, LawLinear>), g, b, 0, s, op, x, part); break;
+  }
+  return hipGetLastError();
+}
+
+template <int Q, int P>
 hipError_t assemble_qp(const OpDev& op, const double* D, double* nzval, BatchRange r, hipStream_t s) {
   constexpr int C = Dims<Q, P>::C;
   constexpr int NT = Q >= 4 ? 1024 : 256;
@@ -386,6 +397,21 @@ hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRan
 #define CALL(q, p) residual_law<q, p>(op, x, b, r, s)
   HFK_DISPATCH_QP(op.Q, op.P, CALL);
 #undef CALL
+}
+
+hipError_t launch_energy(const OpDev& op, const double* x, double* part, int64_t nbatches, hipStream_t s) {
+  if (nbatches <= 0) return hipSuccess;
+#define CALL(q, p) energy_law<q, p>(op, x, part, nbatches, s)
+  HFK_DISPATCH_QP(op.Q, op.P, CALL);
+#undef CALL
+}
+
+hipError_t launch_reduce_partials(const double* part, int64_t n, int mode, double* out, hipStream_t s) {
+  const dim3 g(1), b(kThreads);
+  if (mode == 0) hipLaunchKernelGGL(k_reduce_partials<0>, g, b, 0, s, part, n, out);
+  else if (mode == 1) hipLaunchKernelGGL(k_reduce_partials<1>, g, b, 0, s, part, n, out);
+  else hipLaunchKernelGGL(k_reduce_partials<2>, g, b, 0, s, part, n, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_assemble(const OpDev& op, const double* D, double* nzval, BatchRange r, hipStream_t s) {

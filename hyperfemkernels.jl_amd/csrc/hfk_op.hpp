@@ -81,6 +81,7 @@ struct hfk_op {
   std::vector<int64_t> slot_of_cell;            // cell -> batch*C + position
   std::vector<int32_t> loc_of;                  // tensor index c*nn + t -> table-local dof
   hfk::DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64, state;
+  hfk::DevBuf part;  // per-batch partial results of the functionals
   // a second copy of the tangents at the same state in another storage, for a smoother of another precision
   hfk::DevBuf tangents_s;
   bool has_mu_field = false;
@@ -143,4 +144,6 @@ int op_prepare_into(hfk_op* op, int storage, void* T, const double* x_dev, hipSt
 // the run-time law's prepare! and residual! (hfk_rtc.hip)
 int rtc_prepare(hfk_op* op, int storage, void* T, const double* x_dev, hipStream_t s);
 int rtc_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s);
+// strain energy per batch with the operator's compiled law into part[nbatches] (HFK_EARG if it has no energy())
+int rtc_energy(hfk_op* op, const double* x_dev, double* part, hipStream_t s);
 }  // namespace hfk

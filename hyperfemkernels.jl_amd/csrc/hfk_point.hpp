@@ -80,6 +80,22 @@ struct LawSVK {  // St Venant-Kirchhoff (test/common.jl:2-8): P = F S, S = lam t
     for (int I = 0; I < 3; ++I) s += F[c + 3 * I] * S[I + 3 * J];
     return s;
   }
+  // W = lam/2 tr(E)^2 + mu E:E
+  __device__ double energy() const {
+    double tr = 0, ee = 0;
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int J = 0; J < 3; ++J) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s += F[k + 3 * I] * F[k + 3 * J];
+        const double e = 0.5 * (s - (I == J ? 1.0 : 0.0));
+        ee += e * e;
+        if (I == J) tr += e;
+      }
+    return 0.5 * lam * tr * tr + mu * ee;
+  }
   // dP_cJ/dF_kL = delta_ck S_LJ + lam F_cJ F_kL + mu (F_cL F_kJ + (F F')_ck delta_JL)
   __device__ double tangent(int c, int J, int k, int L) const {
     return (c == k ? S[L + 3 * J] : 0.0) + lam * F[c + 3 * J] * F[k + 3 * L] +
@@ -99,6 +115,12 @@ struct LawNeoHooke {  // W = mu/2 (I1 - 3) - mu lnJ + lam/2 lnJ^2:  P = mu (F - 
     lnJ = log(d);
   }
   __device__ double stress(int c, int J) const { return mu * F[c + 3 * J] + (lam * lnJ - mu) * A[J + 3 * c]; }
+  __device__ double energy() const {
+    double I1 = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) I1 += F[i] * F[i];
+    return 0.5 * mu * (I1 - 3.0) - mu * lnJ + 0.5 * lam * lnJ * lnJ;
+  }
   // mu d_ck d_JL + (mu - lam lnJ) A_Jk A_Lc + lam A_Jc A_Lk
   __device__ double tangent(int c, int J, int k, int L) const {
     return (c == k && J == L ? mu : 0.0) + (mu - lam * lnJ) * A[J + 3 * k] * A[L + 3 * c] + lam * A[J + 3 * c] * A[L + 3 * k];
@@ -118,6 +140,18 @@ struct LawLinear {  // sigma = lam tr(G) I + mu (G + G'), G = F - I
   }
   __device__ double stress(int c, int J) const {
     return (c == J ? lam * tr : 0.0) + mu * (G[c + 3 * J] + G[J + 3 * c]);
+  }
+  // W = lam/2 tr(eps)^2 + mu eps:eps, eps = (G + G')/2
+  __device__ double energy() const {
+    double ee = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int J = 0; J < 3; ++J) {
+        const double e = 0.5 * (G[c + 3 * J] + G[J + 3 * c]);
+        ee += e * e;
+      }
+    return 0.5 * lam * tr * tr + mu * ee;
   }
   __device__ double tangent(int c, int J, int k, int L) const {
     return (c == J && k == L ? lam : 0.0) + (c == k && J == L ? mu : 0.0) + (J == k && c == L ? mu : 0.0);
@@ -504,6 +538,163 @@ __global__ __launch_bounds__(kThreads) void k_residual(OpDev op, const double* _
   contract_sweeps<Q, P>(A, B, op.tab, tid, [&](int cell, int c, int t0, const double* r) {
     scatter_line<Q>(ids + cell * 3 * Dm::NN + c * Dm::NN + t0, r, 1.0, b);
   });
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Scalar functionals of the unknown (KernelFunctional, src/functionals.jl:9-154): the strain energy
+// sum_cells sum_q W(F) dV (reduce = :sum) and the extremum over the quadrature points of a pointwise function of
+// the displacement gradient and the gradient of an increment (reduce = :min / :max).  Each work-group reduces its
+// batch in a fixed order into part[batch]; k_reduce_partials reduces the batches in a fixed order.
+
+// 1 when the law defines `double energy() const` (detection by partial specialisation; no standard headers)
+template <class L, class = void>
+struct HasEnergy {
+  static constexpr int value = 0;
+};
+template <class L>
+struct HasEnergy<L, decltype((void)static_cast<const L*>(nullptr)->energy())> {
+  static constexpr int value = 1;
+};
+template <class L>
+__device__ __forceinline__ auto law_energy(const L& l, int) -> decltype(l.energy()) {
+  return l.energy();
+}
+template <class L>
+__device__ __forceinline__ double law_energy(const L&, long) {
+  return __builtin_nan("");
+}
+
+// combine for MODE 0 (sum), 1 (min), 2 (max); min and max propagate NaN
+template <int MODE>
+__device__ __forceinline__ double combine(double a, double b) {
+  if (MODE == 0) return a + b;
+  if (a != a || b != b) return __builtin_nan("");
+  return MODE == 1 ? (a < b ? a : b) : (a > b ? a : b);
+}
+template <int MODE>
+__device__ __forceinline__ double identity_of() {
+  return MODE == 0 ? 0.0 : (MODE == 1 ? __builtin_inf() : -__builtin_inf());
+}
+
+// fixed-order tree over the kThreads values of a work-group; the result is returned to every thread
+template <int MODE>
+__device__ __forceinline__ double block_reduce(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = combine<MODE>(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// the nodal values of the C cells of a batch into A: free values x, Dirichlet values d (null: zero), 0 on padding
+template <int Q, int P>
+__device__ __forceinline__ void gather_batch(const int32_t* __restrict__ ids, const double* __restrict__ x,
+                                             const double* __restrict__ d, double* A, int tid) {
+  using Dm = Dims<Q, P>;
+  for (int it = tid; it < Dm::C * 3 * Dm::NN; it += kThreads) {
+    const int id = ids[it];
+    const int cell = it / (3 * Dm::NN);
+    A[cell * Dm::SA + (it - cell * 3 * Dm::NN)] = id > 0 ? x[id - 1] : (id < 0 && d ? d[-id - 1] : 0.0);
+  }
+}
+
+// physical gradient G[c + 3J] = du_c/dx_J from the reference gradients s[3c + K] = du_c/dxi_K
+__device__ __forceinline__ void physical_gradient(const double* iJt, const double* s, double* G) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int J = 0; J < 3; ++J) {
+      double g = 0;
+#pragma unroll
+      for (int K = 0; K < 3; ++K) g += iJt[J + 3 * K] * s[3 * c + K];
+      G[c + 3 * J] = g;
+    }
+}
+
+template <int Q, int P, class Law>
+__global__ __launch_bounds__(kThreads) void k_energy(OpDev op, const double* __restrict__ x, double* __restrict__ part) {
+  using Dm = Dims<Q, P>;
+  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT;
+  __shared__ double lds[Dm::LDS];
+  __shared__ double red[kThreads];
+  double* A = lds;
+  double* B = lds + C * SA;
+  const int tid = threadIdx.x;
+  const int batch = blockIdx.x;
+  const int32_t* ids = op.ids + (size_t)batch * C * 3 * Dm::NN;
+  gather_batch<Q, P>(ids, x, op.dvals, A, tid);
+  __syncthreads();
+  interp_sweeps<Q, P>(A, B, op.tab, tid);
+  double e = 0.0;
+  if (tid < NSLOT) {
+    const int cell = tid / Dm::NP, q = tid - cell * Dm::NP;
+    if (ids[cell * 3 * Dm::NN] != 0) {  // not a padding cell
+      double iJt[9], dV, F[9];
+      point_geometry<P>(op.tab, op.coords + ((size_t)batch * C + cell) * 24, q, iJt, dV);
+      physical_gradient(iJt, B + cell * SB + q * 9, F);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c + 3 * c] += 1.0;
+      const double mu = op.mu_pts ? op.mu_pts[(size_t)batch * NSLOT + tid] : op.mu;
+      Law law;
+      law.setup(F, op.lam, mu);
+      e = dV * law_energy(law, 0);
+    }
+  }
+  e = block_reduce<0>(e, red);
+  if (tid == 0) part[batch] = e;
+}
+
+// Fn::eval(Gu, Gdu, p): u the free values x with the Dirichlet values of the operator, du the free values dx with
+// the Dirichlet values ddir (null: zero); MODE 1 min, 2 max over the points, no weights
+template <int Q, int P, class Fn, int MODE>
+__global__ __launch_bounds__(kThreads) void k_extremum(OpDev op, const double* __restrict__ x,
+                                                       const double* __restrict__ dx, const double* __restrict__ ddir,
+                                                       const double* __restrict__ prm, double* __restrict__ part) {
+  using Dm = Dims<Q, P>;
+  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT;
+  __shared__ double lds[Dm::LDS];
+  __shared__ double red[kThreads];
+  double* A = lds;
+  double* B = lds + C * SA;
+  const int tid = threadIdx.x;
+  const int batch = blockIdx.x;
+  const int32_t* ids = op.ids + (size_t)batch * C * 3 * Dm::NN;
+  const int cell = tid / Dm::NP, q = tid - cell * Dm::NP;
+  const bool real = tid < NSLOT && ids[cell * 3 * Dm::NN] != 0;
+  double iJt[9], dV, Gu[9];
+  gather_batch<Q, P>(ids, x, op.dvals, A, tid);
+  __syncthreads();
+  interp_sweeps<Q, P>(A, B, op.tab, tid);
+  if (real) {
+    point_geometry<P>(op.tab, op.coords + ((size_t)batch * C + cell) * 24, q, iJt, dV);
+    physical_gradient(iJt, B + cell * SB + q * 9, Gu);
+  }
+  __syncthreads();  // B read before the increment overwrites A and B
+  gather_batch<Q, P>(ids, dx, ddir, A, tid);
+  __syncthreads();
+  interp_sweeps<Q, P>(A, B, op.tab, tid);
+  double v = identity_of<MODE>();
+  if (real) {
+    double Gdu[9];
+    physical_gradient(iJt, B + cell * SB + q * 9, Gdu);
+    v = Fn::eval(Gu, Gdu, prm);
+  }
+  v = block_reduce<MODE>(v, red);
+  if (tid == 0) part[batch] = v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_reduce_partials(const double* __restrict__ part, int64_t n,
+                                                              double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double v = identity_of<MODE>();
+  for (int64_t i = threadIdx.x; i < n; i += kThreads) v = combine<MODE>(v, part[i]);
+  v = block_reduce<MODE>(v, red);
+  if (threadIdx.x == 0) out[0] = v;
 }
 
 }  // namespace hfk

@@ -77,7 +77,26 @@ std::vector<std::string> kernel_names(const std::string& name, int Q, int P) {
   const std::string qp = std::to_string(Q) + ", " + std::to_string(P) + ", " + name;
   return {"hfk::k_prepare<" + qp + ", double, false>", "hfk::k_prepare<" + qp + ", double, true>",
           "hfk::k_prepare<" + qp + ", float, false>", "hfk::k_prepare<" + qp + ", float, true>",
-          "hfk::k_residual<" + qp + ">"};
+          "hfk::k_residual<" + qp + ">", "hfk::k_energy<" + qp + ">"};
+}
+
+// the module of a law's kernels: its source and whether it defines an energy (a global the host reads)
+std::string law_module(const char* source, const std::string& name) {
+  return law_source(source) + "extern \"C\" __device__ int hfk_law_has_energy = hfk::HasEnergy<" + name + ">::value;\n";
+}
+
+// a pointwise function `double name(const double* Gu, const double* Gdu, const double* p)` wrapped for
+// k_extremum, with a probe kernel that calls it
+std::string pointfn_program(const char* source, const char* name) {
+  return law_source(source) + "struct HfkPointFn {\n  __device__ static double eval(const double* a, const double* b, "
+         "const double* p) { return " + std::string(name) + "(a, b, p); }\n};\n"
+         "extern \"C\" __global__ void hfk_pointfn_probe(const double* g, double* out) { out[0] = "
+         "HfkPointFn::eval(g, g + 9, g + 18); }\n";
+}
+
+std::vector<std::string> extremum_names(int Q, int P) {
+  const std::string qp = std::to_string(Q) + ", " + std::to_string(P) + ", HfkPointFn, ";
+  return {"hfk::k_extremum<" + qp + "1>", "hfk::k_extremum<" + qp + "2>"};
 }
 
 // hiprtc for gfx950 (no device needed); false with the compiler log on failure.  `names`: template
@@ -165,11 +184,31 @@ bool valid_identifier(const char* s) {
 
 using namespace hfk;
 
-// the kernels of one (Q, P): prepare! per storage 0..3, residual!
+// the kernels of one (Q, P): prepare! per storage 0..3, residual!, the energy
 struct RtcInstance {
   hipModule_t mod = nullptr;
   hipFunction_t prep[4] = {};
   hipFunction_t res = nullptr;
+  hipFunction_t energy = nullptr;
+  int has_energy = 0;
+};
+
+// the extremum kernels (min, max) of a pointwise function for one (Q, P)
+struct FnInstance {
+  hipModule_t mod = nullptr;
+  hipFunction_t ext[2] = {};
+};
+
+struct hfk_pointfn {
+  hfk_ctx* ctx = nullptr;
+  std::string source, name;
+  std::mutex m;
+  std::map<int, FnInstance> inst;  // by Q * 16 + P
+  ~hfk_pointfn() {
+    if (inst.empty()) return;
+    if (ctx && hipSetDevice(ctx->device) == hipSuccess) (void)hipDeviceSynchronize();
+    for (auto& kv : inst) (void)hipModuleUnload(kv.second.mod);
+  }
 };
 
 struct hfk_law {
@@ -202,13 +241,38 @@ int law_instance(hfk_op* op, RtcInstance** out) {
   CacheEntry e;
   bool cached = false;
   std::string log;
-  if (!compile_cached(law_source(law->source.c_str()), kernel_names(law->name, op->Q, op->P), e, cached, log))
+  if (!compile_cached(law_module(law->source.c_str(), law->name), kernel_names(law->name, op->Q, op->P), e, cached, log))
     return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the law's kernels do not compile: " + log);
   RtcInstance ri;
   HIPCHK(op->ctx, hipModuleLoadData(&ri.mod, e.code.data()));
   for (int s = 0; s < 4; ++s) HIPCHK(op->ctx, hipModuleGetFunction(&ri.prep[s], ri.mod, e.lowered[s].c_str()));
   HIPCHK(op->ctx, hipModuleGetFunction(&ri.res, ri.mod, e.lowered[4].c_str()));
+  HIPCHK(op->ctx, hipModuleGetFunction(&ri.energy, ri.mod, e.lowered[5].c_str()));
+  hipDeviceptr_t flag = nullptr;
+  size_t flag_bytes = 0;
+  HIPCHK(op->ctx, hipModuleGetGlobal(&flag, &flag_bytes, ri.mod, "hfk_law_has_energy"));
+  HIPCHK(op->ctx, hipMemcpyDtoH(&ri.has_energy, flag, sizeof(int)));
   *out = &(law->inst[k] = ri);
+  return HFK_OK;
+}
+
+int fn_instance(hfk_op* op, hfk_pointfn* fn, FnInstance** out) {
+  std::lock_guard<std::mutex> lk(fn->m);
+  const int k = op->Q * 16 + op->P;
+  auto it = fn->inst.find(k);
+  if (it != fn->inst.end()) {
+    *out = &it->second;
+    return HFK_OK;
+  }
+  CacheEntry e;
+  bool cached = false;
+  std::string log;
+  if (!compile_cached(pointfn_program(fn->source.c_str(), fn->name.c_str()), extremum_names(op->Q, op->P), e, cached, log))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the extremum kernels do not compile: " + log);
+  FnInstance fi;
+  HIPCHK(op->ctx, hipModuleLoadData(&fi.mod, e.code.data()));
+  for (int m = 0; m < 2; ++m) HIPCHK(op->ctx, hipModuleGetFunction(&fi.ext[m], fi.mod, e.lowered[m].c_str()));
+  *out = &(fn->inst[k] = fi);
   return HFK_OK;
 }
 
@@ -240,9 +304,82 @@ int rtc_residual(hfk_op* op, const double* x, double* b, hipStream_t s) {
   return HFK_OK;
 }
 
+int rtc_energy(hfk_op* op, const double* x, double* part, hipStream_t s) {
+  RtcInstance* ri = nullptr;
+  int rc = law_instance(op, &ri);
+  if (rc) return rc;
+  if (!ri->has_energy)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the compiled law defines no `double energy() const`");
+  OpDev d = op->dev();
+  void* args[] = {&d, &x, &part};
+  HIPCHK(op->ctx, hipModuleLaunchKernel(ri->energy, (unsigned)op->nbatches, 1, 1, kThreads, 1, 1, 0, s, args, nullptr));
+  return HFK_OK;
+}
+
 }  // namespace hfk
 
 extern "C" {
+
+int hfk_pointfn_create(hfk_ctx* ctx, const char* source, const char* name, hfk_pointfn** out) {
+  if (!ctx || !source || !out) return set_err(ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  if (!valid_identifier(name)) return set_err(ctx, HFK_EARG, "HyperFEMKernels: the function name must be a C identifier");
+  CacheEntry e;
+  bool cached = false;
+  std::string log;
+  if (!compile_cached(pointfn_program(source, name), {}, e, cached, log))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: the pointwise function does not compile: " + log);
+  std::unique_ptr<hfk_pointfn> fn(new hfk_pointfn());
+  fn->ctx = ctx;
+  fn->source = source;
+  fn->name = name;
+  *out = fn.release();
+  return HFK_OK;
+}
+
+int hfk_pointfn_destroy(hfk_pointfn* fn) {
+  delete fn;
+  return HFK_OK;
+}
+
+int hfk_extremum(hfk_op* op, hfk_pointfn* fn, const double* x, const double* dx, const double* dx_dir,
+                 const double* params, int64_t nparams, int reduce, double* value) {
+  if (!op || !op->ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  if (!fn || !x || !dx || !value || (nparams > 0 && !params) || nparams < 0)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  if (fn->ctx != op->ctx) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the function belongs to another context");
+  if (reduce != 0 && reduce != 1) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: reduce must be 0 (min) or 1 (max)");
+  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
+  FnInstance* fi = nullptr;
+  int rc = fn_instance(op, fn, &fi);
+  if (rc) return rc;
+  hipStream_t s = op->ctx->stream;
+  const size_t nf = (size_t)op->nfree, nd = dx_dir ? (size_t)op->ndir : 0, np = (size_t)std::max<int64_t>(nparams, 0);
+  // staging: x, dx, dx_dir, params, result
+  DevBuf buf;
+  HIPCHK(op->ctx, buf.alloc((2 * nf + nd + np + 1) * sizeof(double)));
+  double* xd = buf.as<double>();
+  double* dxd = xd + nf;
+  double* ddd = dxd + nf;
+  double* pd = ddd + nd;
+  double* res = pd + np;
+  HIPCHK(op->ctx, hipMemcpyAsync(xd, x, nf * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(dxd, dx, nf * sizeof(double), hipMemcpyHostToDevice, s));
+  if (nd) HIPCHK(op->ctx, hipMemcpyAsync(ddd, dx_dir, nd * sizeof(double), hipMemcpyHostToDevice, s));
+  if (np) HIPCHK(op->ctx, hipMemcpyAsync(pd, params, np * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches, 1) * sizeof(double)));
+  OpDev d = op->dev();
+  const double* xa = xd;
+  const double* dxa = dxd;
+  const double* dda = nd ? ddd : nullptr;
+  const double* pa = pd;
+  double* part = op->part.as<double>();
+  void* args[] = {&d, &xa, &dxa, &dda, &pa, &part};
+  HIPCHK(op->ctx, hipModuleLaunchKernel(fi->ext[reduce], (unsigned)op->nbatches, 1, 1, kThreads, 1, 1, 0, s, args, nullptr));
+  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches, reduce + 1, res, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(value, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
 
 int hfk_law_create(hfk_ctx* ctx, const char* source, const char* name, hfk_law** out) {
   if (!ctx || !source || !out) return set_err(ctx, HFK_EARG, "HyperFEMKernels: null argument");

@@ -37,6 +37,11 @@ SIGNATURES = {
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_mu_function": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int64]),
     "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),
+    "hfk_energy": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_energy_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_pointfn_create": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
+    "hfk_pointfn_destroy": (C.c_int, [_vp]),
+    "hfk_extremum": (C.c_int, [_vp, _vp, _f64p, _f64p, _f64p, _f64p, C.c_int64, C.c_int, _f64p]),
     "hfk_op_info": (C.c_int, [_vp, _i64p]),
     "hfk_op_colors": (C.c_int, [_vp, _i32p]),
     "hfk_op_set_deterministic": (C.c_int, [_vp, C.c_int]),

@@ -280,15 +280,19 @@ def ddot(A: Mat3, B: Mat3) -> Node:
 
 class PointLaw:
     """The derived point quantities of a law: P (9 nodes, P[c + 3J]) and T (81 nodes, T[(c + 3J) + 9 (k + 3L)] =
-    dP_cJ / dF_kL) over the variables F (9), lam and mu."""
+    dP_cJ / dF_kL) over the variables F (9), lam and mu; W the energy node of a law given by its energy (else
+    None)."""
 
-    def __init__(self, F, lam, mu, P, T):
-        self.F, self.lam, self.mu, self.P, self.T = F, lam, mu, P, T
+    def __init__(self, F, lam, mu, P, T, W=None):
+        self.F, self.lam, self.mu, self.P, self.T, self.W = F, lam, mu, P, T, W
+
+    def outputs(self):
+        return self.P + self.T + ([self.W] if self.W is not None else [])
 
     def nodes(self):
         """Every node reachable from the outputs, in an order where arguments come first."""
         seen, order = set(), []
-        stack = [(n, False) for n in reversed(self.P + self.T)]
+        stack = [(n, False) for n in reversed(self.outputs())]
         while stack:
             n, done = stack.pop()
             if id(n) in seen:
@@ -319,6 +323,7 @@ class PointLaw:
                 val[id(n)] = {"add": lambda: a[0] + a[1], "sub": lambda: a[0] - a[1], "mul": lambda: a[0] * a[1],
                               "div": lambda: a[0] / a[1], "neg": lambda: -a[0], "log": lambda: math.log(a[0]),
                               "exp": lambda: math.exp(a[0]), "sqrt": lambda: math.sqrt(a[0])}[n.op]()
+        self._last_energy = val[id(self.W)] if self.W is not None else None
         P = np.array([[val[id(self.P[c + 3 * J])] for J in range(3)] for c in range(3)])
         T = np.zeros((3, 3, 3, 3))
         for c in range(3):
@@ -328,8 +333,16 @@ class PointLaw:
                         T[c, J, k, L] = val[id(self.T[(c + 3 * J) + 9 * (k + 3 * L)])]
         return P, T
 
+    def energy(self, F: np.ndarray, lam: float, mu: float) -> float:
+        """W at F with numpy (a law given by its energy)."""
+        if self.W is None:
+            raise ValueError("HyperFEMKernels: the law was not given by an energy")
+        self.evaluate(F, lam, mu)
+        return self._last_energy
+
     def source(self, name: str) -> str:
-        """HIP source of the law struct ``name`` (setup / stress / tangent) for ``laws.CompiledLaw``."""
+        """HIP source of the law struct ``name`` (setup / stress / tangent, and energy for a law given by its energy)
+        for ``laws.CompiledLaw``."""
         names = {}
         for j in range(3):
             for i in range(3):
@@ -354,14 +367,21 @@ class PointLaw:
             body.append(f"    const double {t} = {expr};")
         outs = [f"    P[{i}] = {ref(n)};" for i, n in enumerate(self.P)]
         outs += [f"    T[{i}] = {ref(n)};" for i, n in enumerate(self.T)]
+        members = "  double P[9], T[81];"
+        extra = []
+        if self.W is not None:
+            outs.append(f"    W = {ref(self.W)};")
+            members = "  double P[9], T[81], W;"
+            extra = ["  __device__ double energy() const { return W; }"]
         return "\n".join([
             f"struct {name} {{  // generated by hfk.codegen: {len(body)} operations",
-            "  double P[9], T[81];",
+            members,
             "  __device__ void setup(const double* Fin, double lam, double mu) {",
             *body, *outs,
             "  }",
             "  __device__ double stress(int c, int J) const { return P[c + 3 * J]; }",
             "  __device__ double tangent(int c, int J, int k, int L) const { return T[(c + 3 * J) + 9 * (k + 3 * L)]; }",
+            *extra,
             "};",
             ""])
 
@@ -376,7 +396,9 @@ def from_energy(W: Callable[[Mat3, Node, Node], Node]) -> PointLaw:
     F, lam, mu = _variables()
     w = wrap(W(F, lam, mu))
     P = [diff(w, F.e[i], {}) for i in range(9)]
-    return _with_tangent(F, lam, mu, P)
+    law = _with_tangent(F, lam, mu, P)
+    law.W = w
+    return law
 
 
 def from_stress(Pf: Callable[[Mat3, Node, Node], Mat3]) -> PointLaw:

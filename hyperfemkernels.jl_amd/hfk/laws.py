@@ -7,6 +7,7 @@ Here the pointwise part is the constitutive law -- HIP source of a struct with
     __device__ void setup(const double* F, double lam, double mu);   // F[c + 3J] = d x_c / d X_J
     __device__ double stress(int c, int J) const;                     // first Piola-Kirchhoff P_cJ
     __device__ double tangent(int c, int J, int k, int L) const;     // d P_cJ / d F_kL
+    __device__ double energy() const;                                 // W (optional: the energy functional)
 
 -- compiled by libhfk (``hfk_law_create``) into the pointwise kernels of ``prepare!`` (stored tangents in the
 operator's layout) and ``residual!`` (point coefficients), between the library's own staging and contraction
@@ -52,6 +53,18 @@ struct SVK {  // P = F S, S = lam tr(E) I + 2 mu E, E = (F'F - I)/2
     return (c == k ? S[L + 3 * J] : 0.0) + lam * F[c + 3 * J] * F[k + 3 * L] +
            mu * (F[c + 3 * L] * F[k + 3 * J] + (J == L ? b : 0.0));
   }
+  __device__ double energy() const {  // lam/2 tr(E)^2 + mu E:E
+    double tr = 0, ee = 0;
+    for (int I = 0; I < 3; ++I)
+      for (int J = 0; J < 3; ++J) {
+        double s = 0;
+        for (int k = 0; k < 3; ++k) s += F[k + 3 * I] * F[k + 3 * J];
+        const double e = 0.5 * (s - (I == J ? 1.0 : 0.0));
+        ee += e * e;
+        if (I == J) tr += e;
+      }
+    return 0.5 * lam * tr * tr + mu * ee;
+  }
 };
 """
 
@@ -71,6 +84,11 @@ struct NeoHooke {  // W = mu/2 (I1 - 3) - mu lnJ + lam/2 lnJ^2
     return (c == k && J == L ? mu : 0.0) + (mu - lam * lnJ) * A[J + 3 * k] * A[L + 3 * c] +
            lam * A[J + 3 * c] * A[L + 3 * k];
   }
+  __device__ double energy() const {
+    double I1 = 0;
+    for (int i = 0; i < 9; ++i) I1 += F[i] * F[i];
+    return 0.5 * mu * (I1 - 3.0) - mu * lnJ + 0.5 * lam * lnJ * lnJ;
+  }
 };
 """
 
@@ -88,6 +106,12 @@ struct Linear {  // sigma = lam tr(G) I + mu (G + G'), G = F - I
   __device__ double tangent(int c, int J, int k, int L) const {
     return (c == J && k == L ? lam : 0.0) + (c == k && J == L ? mu : 0.0) + (J == k && c == L ? mu : 0.0);
   }
+  __device__ double energy() const {  // lam/2 tr^2 + mu eps:eps
+    double ee = 0;
+    for (int c = 0; c < 3; ++c)
+      for (int J = 0; J < 3; ++J) ee += 0.25 * (G[c + 3 * J] + G[J + 3 * c]) * (G[c + 3 * J] + G[J + 3 * c]);
+    return 0.5 * lam * tr * tr + mu * ee;
+  }
 };
 """
 
@@ -96,15 +120,21 @@ struct Linear {  // sigma = lam tr(G) I + mu (G + G'), G = F - I
 # At F = I it reduces to the linear law's tangent (g(1) = 0, g'(1) = lam).
 NEOHOOKE_QUADVOL_SOURCE = r"""
 struct NeoHookeQuadVol {
-  double F[9], A[9], lam, mu, g, gJ;
+  double F[9], A[9], lam, mu, g, gJ, J;
   __device__ void setup(const double* Fin, double l, double m) {
     lam = l;
     mu = m;
     for (int i = 0; i < 9; ++i) F[i] = Fin[i];
     const double d = hfk_det3(F);
     hfk_inv3(F, A, d);
+    J = d;
     g = lam * (d - 1.0) * d;
     gJ = lam * (2.0 * d - 1.0) * d;
+  }
+  __device__ double energy() const {
+    double I1 = 0;
+    for (int i = 0; i < 9; ++i) I1 += F[i] * F[i];
+    return 0.5 * mu * (I1 - 3.0) - mu * log(J) + 0.5 * lam * (J - 1.0) * (J - 1.0);
   }
   __device__ double stress(int c, int J) const { return mu * F[c + 3 * J] + (g - mu) * A[J + 3 * c]; }
   __device__ double tangent(int c, int J, int k, int L) const {

@@ -122,6 +122,52 @@ function set_law!(op::DeviceOperator, law::Union{Nothing,CompiledLaw})
     op
 end
 
+"""
+    DeviceFunctional(op; reduce = :sum)
+    DeviceFunctional(op, source, name; reduce = :min, params = Float64[])
+
+`KernelFunctional` (src/functionals.jl:9-154) on the device: the strain energy of `op`'s law (`:sum`), or the
+extremum over the quadrature points of a pointwise function `g(Gu, Gdu, p)` given as HIP source (`:min`/`:max`), with
+the increment set by `set_increment!` (free values and Dirichlet values, changeable between calls like the
+reference's `duh`).  Call it on the free values: `fun(x) -> Float64`.
+"""
+mutable struct DeviceFunctional
+    op::Any
+    reduce::Symbol
+    fn::Ptr{Cvoid}
+    params::Vector{Float64}
+    dx::Vector{Float64}
+    dx_dir::Union{Nothing,Vector{Float64}}
+end
+function DeviceFunctional(op; reduce::Symbol=:sum)
+    reduce === :sum || throw(ArgumentError("HyperFEMKernels: a :min/:max functional needs a pointwise function"))
+    DeviceFunctional(op, :sum, C_NULL, Float64[], zeros(op.nfree), nothing)
+end
+function DeviceFunctional(op, source::AbstractString, name::AbstractString; reduce::Symbol=:min,
+                          params::Vector{Float64}=Float64[])
+    reduce in (:min, :max) || throw(ArgumentError("HyperFEMKernels: reduce must be :min or :max"))
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hfk_pointfn_create, libhfk), Cint, (Ptr{Cvoid}, Cstring, Cstring, Ptr{Ptr{Cvoid}}), op.ctx.ptr, source,
+                name, r), op.ctx.ptr)
+    f = DeviceFunctional(op, reduce, r[], params, zeros(op.nfree), nothing)
+    finalizer(f -> ccall((:hfk_pointfn_destroy, libhfk), Cint, (Ptr{Cvoid},), f.fn), f)
+    f
+end
+set_increment!(f::DeviceFunctional, dx::Vector{Float64}, dx_dir::Union{Nothing,Vector{Float64}}=nothing) =
+    (f.dx = dx; f.dx_dir = dx_dir; f)
+function (f::DeviceFunctional)(x::Vector{Float64})
+    v = Ref{Float64}(0.0)
+    if f.reduce === :sum
+        check(ccall((:hfk_energy, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), f.op.ptr, x, v), f.op.ctx.ptr)
+    else
+        check(ccall((:hfk_extremum, libhfk), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Cint, Ptr{Float64}),
+                    f.op.ptr, f.fn, x, f.dx, f.dx_dir === nothing ? C_NULL : f.dx_dir,
+                    isempty(f.params) ? C_NULL : f.params, length(f.params), f.reduce === :min ? 0 : 1, v), f.op.ctx.ptr)
+    end
+    v[]
+end
+
 "Algebra.residual!(b, op, x) (src/assembly.jl:909-916)"
 function residual!(b::Vector{Float64}, op::DeviceOperator, x::Vector{Float64})
     length(x) == op.nfree == length(b) || throw(DimensionMismatch("HyperFEMKernels: vector lengths"))

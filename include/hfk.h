@@ -118,6 +118,27 @@ int hfk_op_set_law(hfk_op* op, hfk_law* law);
 int hfk_residual(hfk_op* op, const double* x, double* b);
 int hfk_residual_dev(hfk_op* op, const double* x_dev, double* b_dev, void* stream);
 
+/* ---- scalar functionals: KernelFunctional(f, U; reduce)  (src/functionals.jl:9-154) --------------------------- */
+/* The strain energy sum_cells sum_q W(F(x)) dV of the operator's law at the free values x (Dirichlet values lifted):
+ * KernelFunctional(u -> ∫(Ψ ∘ F(∇u))dΩ, U) with reduce = :sum, the energy of a Newton line search.  Built-in laws, or
+ * a compiled law that defines `__device__ double energy() const` (W at the F of setup; HFK_EARG otherwise).  Summed
+ * per batch and then over the batches in a fixed order (bitwise reproducible).  _dev: device pointers, asynchronous
+ * on `stream`, the energy written to energy_dev[0]. */
+int hfk_energy(hfk_op* op, const double* x, double* energy);
+int hfk_energy_dev(hfk_op* op, const double* x_dev, double* energy_dev, void* stream);
+/* A pointwise function of the displacement gradient and the gradient of an increment, compiled with hiprtc:
+ * `source` defines  __device__ double name(const double* Gu, const double* Gdu, const double* p);
+ * G[c + 3J] = d u_c / d x_J (Gridap's ∇(u)'), p the parameters.  Cached per process by the source text. */
+typedef struct hfk_pointfn hfk_pointfn;
+int hfk_pointfn_create(hfk_ctx* ctx, const char* source, const char* name, hfk_pointfn** fn);
+int hfk_pointfn_destroy(hfk_pointfn* fn);
+/* KernelFunctional(u -> ∫(g ∘ (∇(u)', ∇(duh)'))dΩ, U; reduce = :min / :max) (src/functionals.jl:12-21; the
+ * injectivity step of a line search): the minimum (reduce 0) or maximum (1) of g over every quadrature point, no
+ * weights, with u the free values x and the operator's Dirichlet values and du the free values dx with the Dirichlet
+ * values dx_dir (ndir; NULL: zero).  params: nparams doubles (p).  NaN at any point gives NaN. */
+int hfk_extremum(hfk_op* op, hfk_pointfn* fn, const double* x, const double* dx, const double* dx_dir,
+                 const double* params, int64_t nparams, int reduce, double* value);
+
 /* ---- matrix-free Jacobian (src/matrixfree.jl) ---------------------------------------------------- */
 /* prepare!(mf, x)  (src/matrixfree.jl:140-159, 199-228): point tangents at state x, stored per `storage`. */
 int hfk_mf_prepare(hfk_op* op, const double* x, int storage);

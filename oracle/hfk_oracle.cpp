@@ -555,6 +555,22 @@ int orc_jv(orc_op* op, const double* T, const double* iJt, const double* dV, con
   return 0;
 }
 
+// the displacement gradient at every quadrature point, G[(cell np + q) 9 + J + 3c] = d u_c / d x_J (Gridap's
+// grad u[J, c]): the pointwise input of a functional's integrand (functionals.jl:9-21)
+int orc_point_gradients(orc_op* op, const double* x, double* G) {
+  int np = op->np, nd = op->nd;
+  std::vector<double> xl(nd);
+  for (int cell = 0; cell < op->ncells; ++cell) {
+    gather_state(op, cell, x, xl.data());
+    for (int q = 0; q < np; ++q) {
+      double ij[9], dv;
+      point_geometry(op, cell, q, ij, &dv);
+      point_gradient(op, xl.data(), q, ij, &G[((size_t)cell * np + q) * 9]);
+    }
+  }
+  return 0;
+}
+
 double orc_energy(orc_op* op, const double* x) {
   int np = op->np, nd = op->nd;
   double total = 0;

@@ -49,6 +49,7 @@ def load():
         "orc_prepare": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _f64p]),
         "orc_jv": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int]),
         "orc_energy": (C.c_double, [C.c_void_p, _f64p]),
+        "orc_point_gradients": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_residual": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_element_jacobians": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_pattern_nnz": (C.c_int64, [C.c_void_p]),
@@ -184,6 +185,14 @@ class OracleOp:
     def energy(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         return load().orc_energy(self.h, _p(x, C.c_double))
+
+    def point_gradients(self, x):
+        """(ncells, np, 3, 3) with [cell, q, c, J] = d u_c / d x_J at every quadrature point (Dirichlet values of
+        the operator)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        G = np.zeros((self.ncells, self.np, 9))
+        load().orc_point_gradients(self.h, _p(x, C.c_double), _p(G, C.c_double))
+        return G.reshape(self.ncells, self.np, 3, 3)  # stored [J + 3c]: row c, column J
 
     def residual(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -1,0 +1,129 @@
+"""GPU parity of the scalar functionals (KernelFunctional, src/functionals.jl:9-154) against the oracle and the
+reference's own functional tests (test/solvers/test_functionals.jl).
+
+* Energy (reduce = sum): equal to the oracle's total energy (1e-12) for the three built-in laws on Q1-Q3 non-affine
+  cells with Dirichlet values, and with a shear-modulus field; its directional derivative equals residual . dx
+  (central differences, 1e-6, test_functionals.jl:33-37); bitwise reproducible.  Compiled laws (sources and a law
+  generated from its energy) give the built-in law's energy; a compiled law without energy() is refused.
+* Extremum (reduce = min / max) of g(Gu, Gdu) = det(I + Gu) / (p0 + |tr Gdu| + p1 Gdu:Gdu) (the function of
+  test_functionals.jl:59-78, with its numbers as parameters) over the quadrature points: equal to the same function
+  of the oracle's point gradients (1e-13), with the increment changing between calls and with zero or the
+  operator's Dirichlet values for the increment.
+"""
+import numpy as np
+import pytest
+
+from hfk import HyperelasticForm, KernelOperator
+from hfk.functionals import KernelFunctional
+from hfk.mesh import dirichlet_closed_x0, hex_space
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(hip_ctx, oracle, order, law, n, seed, amp=0.02):
+    sp = hex_space(n, order, dirichlet=dirichlet_closed_x0, perturb=0.1, seed=seed)
+    rng = np.random.default_rng(seed + 50)
+    dv = amp / order * rng.standard_normal(sp.ndir)
+    x = amp / order * rng.standard_normal(sp.nfree)
+    op = KernelOperator(HyperelasticForm(law, 2.0, 1.0, degree=2 * order), sp, jacobian="matrixfree",
+                        dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, dvals=dv)
+    return sp, dv, x, op, orc, rng
+
+
+@pytest.mark.parametrize("order,law,n", [(1, "svk", (3, 3, 2)), (2, "neohookean", (3, 2, 2)), (2, "linear", (2, 2, 2)),
+                                         (3, "svk", (2, 2, 1))])
+def test_energy_matches_oracle_and_residual(hip_ctx, oracle, order, law, n):
+    print(f"energy Q{order} {law}", flush=True)
+    sp, dv, x, op, orc, rng = problem(hip_ctx, oracle, order, law, n, seed=4)
+    fun = KernelFunctional(op)
+    E = fun(x)
+    assert E == pytest.approx(orc.energy(x), rel=1e-12)
+    assert fun(x) == E  # fixed summation order
+    dx = rng.standard_normal(sp.nfree)
+    h = 1e-6
+    fd = (fun(x + h * dx) - fun(x - h * dx)) / (2 * h)
+    assert fd == pytest.approx(op.residual(np.zeros(sp.nfree), x) @ dx, rel=1e-6)
+    mu = np.exp(0.5 * rng.standard_normal((sp.ncells, (order + 1) ** 3)))
+    op.set_mu_field(mu)
+    orc.set_mu_field(mu)
+    assert fun(x) == pytest.approx(orc.energy(x), rel=1e-12)
+
+
+NO_ENERGY = r"""
+struct LinearNoEnergy {
+  double G[9], lam, mu, tr;
+  __device__ void setup(const double* Fin, double l, double m) {
+    lam = l; mu = m;
+    for (int c = 0; c < 3; ++c)
+      for (int J = 0; J < 3; ++J) G[c + 3 * J] = Fin[c + 3 * J] - (c == J ? 1.0 : 0.0);
+    tr = G[0] + G[4] + G[8];
+  }
+  __device__ double stress(int c, int J) const { return (c == J ? lam * tr : 0.0) + mu * (G[c + 3 * J] + G[J + 3 * c]); }
+  __device__ double tangent(int c, int J, int k, int L) const {
+    return (c == J && k == L ? lam : 0.0) + (c == k && J == L ? mu : 0.0) + (J == k && c == L ? mu : 0.0);
+  }
+};
+"""
+
+
+@pytest.mark.parametrize("law", ["svk", "neohookean", "linear"])
+def test_energy_of_compiled_laws(hip_ctx, oracle, law):
+    from hfk import codegen as cg
+    from hfk.laws import SOURCES, CompiledLaw
+
+    sp, dv, x, op, orc, rng = problem(hip_ctx, oracle, 2, law, (3, 2, 2), seed=8)
+    fun = KernelFunctional(op)
+    E = fun(x)
+    src, name = SOURCES[law]
+    op.set_law(CompiledLaw(src, name, ctx=hip_ctx))
+    assert fun(x) == pytest.approx(E, rel=1e-13)
+    if law != "linear":
+        energy = cg.svk_energy if law == "svk" else cg.neohooke_energy
+        op.set_law(CompiledLaw(cg.from_energy(energy).source("Gen" + name), "Gen" + name, ctx=hip_ctx))
+        assert fun(x) == pytest.approx(E, rel=1e-12)
+    else:
+        op.set_law(CompiledLaw(NO_ENERGY, "LinearNoEnergy", ctx=hip_ctx))
+        with pytest.raises(RuntimeError, match="defines no"):
+            fun(x)
+    op.set_law(None)
+    assert fun(x) == E
+
+
+G_SOURCE = r"""
+__device__ double injectivity_probe(const double* Gu, const double* Gdu, const double* p) {
+  double F[9], dd = 0;
+  for (int i = 0; i < 9; ++i) {
+    F[i] = Gu[i] + (i % 4 == 0 ? 1.0 : 0.0);
+    dd += Gdu[i] * Gdu[i];
+  }
+  return hfk_det3(F) / (p[0] + fabs(Gdu[0] + Gdu[4] + Gdu[8]) + p[1] * dd);
+}
+"""
+
+
+def g_of(Gu, Gdu, p):
+    F = np.eye(3) + Gu
+    return np.linalg.det(F) / (p[0] + np.abs(np.trace(Gdu, axis1=-2, axis2=-1)) + p[1] * (Gdu * Gdu).sum(axis=(-2, -1)))
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_extremum_matches_oracle(hip_ctx, oracle, order):
+    print(f"extremum Q{order}", flush=True)
+    n = (3, 2, 2) if order == 2 else (2, 2, 1)
+    sp, dv, x, op, orc, rng = problem(hip_ctx, oracle, order, "svk", n, seed=12, amp=0.1)
+    p = np.array([1.0, 0.1])
+    fmin = KernelFunctional(op, "min", G_SOURCE, "injectivity_probe", params=p)
+    fmax = KernelFunctional(op, "max", G_SOURCE, "injectivity_probe", params=p)
+    orc0 = oracle.OracleOp(sp, "svk", 2.0, 1.0, dvals=np.zeros(sp.ndir))
+    for k in range(3):
+        x = 0.1 / order * rng.standard_normal(sp.nfree)
+        dx = rng.standard_normal(sp.nfree)
+        # the increment with the operator's Dirichlet values (an FE function on the same space), then with zero ones
+        for dd, o in ((dv, orc), (None, orc0)):
+            fmin.set_increment(dx, dd)
+            fmax.set_increment(dx, dd)
+            g = g_of(orc.point_gradients(x), o.point_gradients(dx), p)
+            vmin, vmax = fmin(x), fmax(x)
+            assert vmin == pytest.approx(g.min(), rel=1e-13) and vmax == pytest.approx(g.max(), rel=1e-13)
+            assert vmin < vmax

@@ -45,6 +45,8 @@ def test_energy_laws_match_oracle(oracle, law, energy):
         assert np.abs(P - Pr).max() <= 1e-13 * np.abs(Pr).max()
         assert np.abs(T - Tr).max() <= 1e-13 * np.abs(Tr).max()
         assert np.allclose(T, T.transpose(2, 3, 0, 1), atol=1e-13)  # hyperelastic: symmetric tangent
+        W = pl.energy(F, 2.0, 1.3)
+        assert W == pytest.approx(oracle.law_W(law, 2.0, 1.3, oracle_layout(F)), rel=1e-13)
 
 
 def test_stress_defined_law_gets_the_same_tangent():
@@ -99,7 +101,7 @@ def test_extra_law_against_finite_differences():
 def test_generated_source_compiles():
     for name, energy in (("GenSVK", cg.svk_energy), ("GenNeoHooke", cg.neohooke_energy), ("GenMooney", mooney_rivlin)):
         src = cg.from_energy(energy).source(name)
-        assert f"struct {name}" in src and "stress" in src and "tangent" in src
+        assert f"struct {name}" in src and "stress" in src and "tangent" in src and "energy()" in src
         check_law(src, name)
 
 

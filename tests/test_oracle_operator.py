@@ -1,6 +1,7 @@
 """Pins of the oracle's operator-level functions on tiny meshes (brute force and invariants).
 
 * residual = gradient of the total energy (central differences), with Dirichlet values;
+* point gradients of affine fields = their constant gradient on trilinear cells;
 * element Jacobians assembled densely = derivative of the residual (central differences);
 * sum-factorised stored-T product (src/matrixfree.jl:341-424) = dense assembled matrix times dx
   (test/test_matrixfree.jl:74-96), serial = threaded (coloured determinism, test/test_assembly.jl:40-47);
@@ -51,6 +52,22 @@ def test_residual_is_energy_gradient(oracle, order, law):
         e[i] = h
         fd = (op.energy(x + e) - op.energy(x - e)) / (2 * h)
         assert fd == pytest.approx(b[i], abs=1e-9, rel=1e-6)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_point_gradients_of_affine_fields(oracle, order):
+    """An affine displacement u = A x + b is held exactly by Q1-Q3 on trilinear cells: its gradient at every
+    quadrature point is A (row c, column J = d u_c / d x_J), with the field's Dirichlet values."""
+    sp = hex_space((2, 2, 1), order, perturb=0.15, seed=3)
+    A = np.array([[0.1, -0.2, 0.05], [0.3, 0.0, -0.1], [0.02, 0.15, -0.25]])
+
+    def u(X):
+        return X @ A.T + np.array([0.5, -1.0, 0.25])
+
+    op = oracle.OracleOp(sp, "svk", 2.0, 1.0, dvals=sp.dirichlet_values(u))
+    G = op.point_gradients(sp.free_values(u))
+    assert G.shape == (sp.ncells, (order + 1) ** 3, 3, 3)
+    assert np.abs(G - A).max() < 1e-13
 
 
 @pytest.mark.parametrize("order,law", [(1, "neohookean"), (2, "svk"), (2, "linear")])
