@@ -557,6 +557,16 @@ int hfk_op_set_dirichlet(hfk_op* op, const double* dvals) {
   return HFK_OK;
 }
 
+int hfk_op_set_parameters(hfk_op* op, double lam, double mu) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!std::isfinite(lam) || !std::isfinite(mu)) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: lam and mu must be finite");
+  op->lam = lam;
+  op->mu = mu;
+  op->storage = -1;  // the stored tangents belong to the previous parameters
+  return HFK_OK;
+}
+
 int hfk_op_set_mu_field(hfk_op* op, const double* mu_points) {
   int rc = bind(op);
   if (rc) return rc;

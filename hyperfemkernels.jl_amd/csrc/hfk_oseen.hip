@@ -925,6 +925,16 @@ int hfk_oseen_set_deterministic(hfk_oseen* op, int deterministic) {
   return HFK_OK;
 }
 
+int hfk_oseen_set_parameters(hfk_oseen* op, double rho, double mu, double c0) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (!(rho > 0) || !(mu > 0) || !(c0 >= 0)) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: invalid rho, mu or c0");
+  op->rho = rho;
+  op->mu = mu;
+  op->c0 = c0;
+  return HFK_OK;
+}
+
 int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc) {
   int rc = oseen_bind(op);
   if (rc) return rc;

@@ -34,6 +34,7 @@ SIGNATURES = {
                                 C.c_int, C.c_double, C.c_double, C.POINTER(_vp)]),
     "hfk_op_destroy": (C.c_int, [_vp]),
     "hfk_op_set_dirichlet": (C.c_int, [_vp, _f64p]),
+    "hfk_op_set_parameters": (C.c_int, [_vp, C.c_double, C.c_double]),
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_mu_function": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int64]),
     "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),
@@ -114,6 +115,7 @@ SIGNATURES = {
     "hfk_oseen_set_outlet": (C.c_int, [_vp, C.c_int64, _i32p, _f64p, C.c_int, _f64p, C.c_int, _f64p, C.c_double,
                                        C.c_double]),
     "hfk_oseen_set_outlet_pressure": (C.c_int, [_vp, C.c_double]),
+    "hfk_oseen_set_parameters": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
     "hfk_oseen_set_deterministic": (C.c_int, [_vp, C.c_int]),
     "hfk_oseen_info": (C.c_int, [_vp, _i64p]),
     "hfk_oseen_colors": (C.c_int, [_vp, _i32p]),

@@ -112,6 +112,20 @@ class KernelOperator:
         if d.size != self.ndir:
             raise ValueError("HyperFEMKernels: Dirichlet values of the wrong length")
         check(self.lib.hfk_op_set_dirichlet(self.handle, ptr(d, C.c_double)), self.ctx.handle)
+        self._dvals = d
+
+    def update(self, form: HyperelasticForm):
+        """``update(op, res, jac)`` (src/assembly.jl:693-744): the operator of ``form``.  With the structure of this
+        operator's form (same law and quadrature) the numbers are rebound in place (``hfk_op_set_parameters``:
+        colouring, pattern, tables and compiled laws kept) and ``self`` is returned; otherwise a new operator on the
+        same space, with the same Jacobian kind and Dirichlet values, is built.  Prepared tangents are invalidated
+        either way."""
+        if form.law == self.form.law and form.degree == self.form.degree:
+            check(self.lib.hfk_op_set_parameters(self.handle, float(form.lam), float(form.mu)), self.ctx.handle)
+            self.form = form
+            return self
+        return KernelOperator(form, self.space, jacobian=self.jacobian_kind, dirichlet_values=getattr(self, "_dvals", None),
+                              ctx=self.ctx)
 
     def set_mu_field(self, mu_points):
         if mu_points is None:

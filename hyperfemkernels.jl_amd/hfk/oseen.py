@@ -140,6 +140,19 @@ class OseenOperator:
         check(self.lib.hfk_oseen_set_deterministic(self.handle, int(bool(deterministic))), self.ctx.handle)
         return self
 
+    def update(self, rho: float | None = None, mu: float | None = None, dt: float | None = None,
+               bdf2: bool | None = None):
+        """``update(op, res, jac)`` of the step with other numbers (test/test_cfd.jl:47-75): density, viscosity,
+        time step, or the switch from backward Euler to BDF2 (c0 = 1/dt -> 1.5/dt, and the step coefficients of
+        ``step_fields``); colouring, pattern, point data and outlet data are kept.  Returns self."""
+        self.rho = self.rho if rho is None else float(rho)
+        self.mu = self.mu if mu is None else float(mu)
+        self.dt = self.dt if dt is None else float(dt)
+        self.bdf2 = self.bdf2 if bdf2 is None else bool(bdf2)
+        self.c0 = (1.5 if self.bdf2 else 1.0) / self.dt
+        check(self.lib.hfk_oseen_set_parameters(self.handle, self.rho, self.mu, self.c0), self.ctx.handle)
+        return self
+
     def set_outlet_pressure(self, Pc: float):
         check(self.lib.hfk_oseen_set_outlet_pressure(self.handle, float(Pc)), self.ctx.handle)
 

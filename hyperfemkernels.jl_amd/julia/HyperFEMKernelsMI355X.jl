@@ -73,6 +73,10 @@ end
 set_dirichlet!(op::DeviceOperator, d::Vector{Float64}) =
     check(ccall((:hfk_op_set_dirichlet, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, d), op.ctx.ptr)
 
+"The fast path of `update(op, res, jac)` for a form of the same structure (src/assembly.jl:693-744): λ and μ rebound, nothing rebuilt; prepare! again."
+update!(op::DeviceOperator; λ::Real, μ::Real) =
+    (check(ccall((:hfk_op_set_parameters, libhfk), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), op.ptr, λ, μ), op.ctx.ptr); op)
+
 "Pointwise shear modulus at the quadrature points (np × ncells, tensor point order); `nothing`: the constant μ."
 set_mu_field!(op::DeviceOperator, μ::Union{Nothing,Matrix{Float64}}) =
     check(ccall((:hfk_op_set_mu_field, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, μ === nothing ? C_NULL : μ),
@@ -359,6 +363,11 @@ set_outlet!(op::DeviceOseen, face_cell::Vector{Int32}, normals::Matrix{Float64},
     check(ccall((:hfk_oseen_set_outlet, libhfk), Cint,
                 (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cdouble, Cdouble),
                 op.ptr, length(face_cell), face_cell, normals, size(xo, 2), xo, size(xb, 2), xb, Pc, ZA), op.ctx.ptr)
+
+"`update(op, res, jac)` of the Oseen step with other numbers (ρ, μ, c0: load and time stepping, the BDF2 switch)."
+update!(op::DeviceOseen; rho::Real, mu::Real, c0::Real) =
+    (check(ccall((:hfk_oseen_set_parameters, libhfk), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble), op.ptr, rho, mu, c0),
+           op.ctx.ptr); op)
 
 "The Windkessel pressure `Pc[]` of the next step."
 set_outlet_pressure!(op::DeviceOseen, Pc::Real) =

@@ -71,6 +71,10 @@ int hfk_op_create(hfk_ctx* ctx, int Q, int P, int64_t ncells, const int32_t* cel
 int hfk_op_destroy(hfk_op* op);
 /* Dirichlet values, re-read by every assembly (refresh_dirichlet_values!, src/assembly.jl:393-410). */
 int hfk_op_set_dirichlet(hfk_op* op, const double* dvals);
+/* The numbers of the form, rebound without rebuilding anything (the fast path of update(op, res, jac) for a capture
+ * of the same structure, src/assembly.jl:693-744 and src/tree.jl:270-283: the colouring, the pattern, the tables and
+ * compiled laws, which take lam and mu as arguments, are kept).  The stored tangents are invalidated (prepare! again). */
+int hfk_op_set_parameters(hfk_op* op, double lam, double mu);
 /* Pointwise shear modulus mu(x) at the quadrature points, np*ncells in tensor point order, [cell][q]
  * (dense auxiliary point data; NULL restores the constant mu). */
 int hfk_op_set_mu_field(hfk_op* op, const double* mu_points);
@@ -332,6 +336,10 @@ int hfk_oseen_step_fields_dev(hfk_oseen* op, const double* un, const double* un_
  * reference coordinates (3), dS, w at the point (3).  1..32 points per measure; nfaces = 0 removes the terms. */
 int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell, const double* normals, int nqo,
                          const double* xo, int nqb, const double* xb, double Pc, double ZA);
+/* rho, mu and c0 rebound for the next assemblies (update(op, res, jac) with the same structure: load and time
+ * stepping, e.g. the switch from backward Euler to BDF2, c0 = 1/dt -> 1.5/dt, test/test_cfd.jl:66-75); colouring,
+ * pattern and point data are kept. */
+int hfk_oseen_set_parameters(hfk_oseen* op, double rho, double mu, double c0);
 /* The outlet pressure Pc alone (a number of the integrand, updated every step by a Windkessel model). */
 int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc);
 /* Assembly colour by colour with plain adds (1, the default: the reference's colour order, bitwise reproducible,

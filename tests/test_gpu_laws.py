@@ -233,3 +233,32 @@ def test_mu_function_of_x(oracle, hip_ctx):
     assert np.abs(op.mu_field() - (1.0 + X[..., 0] ** 2 + 2.0 * X[..., 1] * X[..., 2])).max() < 1e-14
     with pytest.raises(RuntimeError, match="does not compile"):
         op.set_mu_function("__device__ double bad(const double* x, const double* p) { return y; }", "bad")
+
+
+# ---- update with the same structure: numbers rebound, nothing rebuilt (src/assembly.jl:693-744) ----------------
+
+def test_update_fast_path(hip_ctx):
+    """update(op, res, jac) with other lam, mu: the same operator object with its pattern, the compiled law's module
+    kept, and residual, tangents (Jv) and assembled values bitwise equal to a fresh operator of the new numbers;
+    another law builds a new operator."""
+    sp, dv, x, rng = problem(2, (3, 2, 2), 0.1, seed=21)
+    f1, f2 = HyperelasticForm("neohookean", 2.0, 1.0, degree=4), HyperelasticForm("neohookean", 3.0, 0.5, degree=4)
+    law = CompiledLaw.builtin("neohookean", ctx=hip_ctx)
+    dx = rng.standard_normal(sp.nfree)
+    for compiled in (False, True):
+        op = KernelOperator(f1, sp, jacobian="assembled", dirichlet_values=dv, ctx=hip_ctx)
+        ref = KernelOperator(f2, sp, jacobian="assembled", dirichlet_values=dv, ctx=hip_ctx)
+        if compiled:
+            op.set_law(law)
+            ref.set_law(law)
+        A1 = op.jacobian(op.allocate_jacobian(), x)
+        pat = op.pattern()
+        assert op.update(f2) is op and op.pattern() is pat
+        assert np.array_equal(op.residual(np.zeros(sp.nfree), x), ref.residual(np.zeros(sp.nfree), x))
+        A2 = op.jacobian(op.allocate_jacobian(), x)
+        assert np.array_equal(A2.data, ref.jacobian(ref.allocate_jacobian(), x).data)
+        assert not np.array_equal(A2.data, A1.data)
+        y = MatrixFreeJacobian(op, "full64").prepare(x) @ dx
+        assert np.array_equal(y, MatrixFreeJacobian(ref, "full64").prepare(x) @ dx)
+    op2 = op.update(HyperelasticForm("svk", 3.0, 0.5, degree=4))
+    assert op2 is not op and op2.form.law == "svk"

@@ -207,3 +207,20 @@ def test_oseen_step_fields_time_loop(hip_ctx, bdf2):
     assert rel(b1, O.residual(sp, x, dv, Kt, lt)) < TOL
     op.step_fields(un, und, unm1, unm1d)
     assert np.array_equal(op.residual(np.zeros(sp.nfree), x), b1)
+
+
+def test_oseen_update_bdf2_switch(hip_ctx):
+    """The switch from backward Euler to BDF2 through update (test/test_cfd.jl:66-75): the same operator and pattern,
+    residual and CSC values bitwise equal to an operator created with BDF2."""
+    print("oseen update bdf2 switch", flush=True)
+    sp, qp, qw, fields, dv, op, rng = setup(hip_ctx, (3, 2, 2), False, seed=19)
+    ref = OseenOperator(sp, rho=1.0, mu=0.05, dt=0.01, bdf2=True, dirichlet_values=dv, ctx=hip_ctx)
+    ref.set_fields(fields[..., 0:3], fields[..., 3:6], fields[..., 6:9])
+    x = rng.standard_normal(sp.nfree)
+    pat = op.pattern()
+    b1 = op.residual(np.zeros(sp.nfree), x)
+    assert op.update(bdf2=True) is op and op.pattern() is pat
+    b2 = op.residual(np.zeros(sp.nfree), x)
+    assert np.array_equal(b2, ref.residual(np.zeros(sp.nfree), x)) and not np.array_equal(b1, b2)
+    assert np.array_equal(op.jacobian(op.allocate_jacobian()).data, ref.jacobian(ref.allocate_jacobian()).data)
+    assert op.step_coefficients() == ref.step_coefficients()
