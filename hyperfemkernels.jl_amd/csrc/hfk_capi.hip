@@ -310,16 +310,26 @@ int run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s) {
 }
 
 int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) {
-  if (op->rtc) return rtc_residual(op, x_dev, b_dev, s);
-  HIPCHK(op->ctx, launch_fill(b_dev, op->nfree, 0.0, s));
-  const OpDev d = op->dev();
-  for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_residual(d, x_dev, b_dev, r, s));
+  if (op->rtc) {
+    int rc = rtc_residual(op, x_dev, b_dev, s);
+    if (rc) return rc;
+  } else {
+    HIPCHK(op->ctx, launch_fill(b_dev, op->nfree, 0.0, s));
+    const OpDev d = op->dev();
+    for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_residual(d, x_dev, b_dev, r, s));
+  }
+  // the boundary groups after the volume group: minus the traction load on the free rows
+  if (op->has_traction)
+    HIPCHK(op->ctx, launch_sub_scaled(b_dev, op->face_load.as<double>(), op->load_factor, op->nfree, s));
   return HFK_OK;
 }
 
 // strain energy into out_dev[0]: per-batch sums, then their sum, both in a fixed order
+constexpr int kDotBlocks = 256;  // work-groups of the load's dot products (a fixed grid: a fixed order)
+
 int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) {
-  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches, 1) * sizeof(double)));
+  const int64_t nload = op->has_traction ? 2 * kDotBlocks : 0;
+  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches + nload, 1) * sizeof(double)));
   double* part = op->part.as<double>();
   if (op->rtc) {
     int rc = rtc_energy(op, x_dev, part, s);
@@ -327,7 +337,13 @@ int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) 
   } else {
     HIPCHK(op->ctx, launch_energy(op->dev(), x_dev, part, op->nbatches, s));
   }
-  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches, 0, out_dev, s));
+  if (op->has_traction) {  // minus the work of the load: f . u over the free and the Dirichlet values
+    const double* f = op->face_load.as<double>();
+    HIPCHK(op->ctx, launch_dot_partials(f, x_dev, op->nfree, -op->load_factor, part + op->nbatches, kDotBlocks, s));
+    HIPCHK(op->ctx, launch_dot_partials(f + op->nfree, op->dvals.as<double>(), op->ndir, -op->load_factor,
+                                        part + op->nbatches + kDotBlocks, kDotBlocks, s));
+  }
+  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches + nload, 0, out_dev, s));
   return HFK_OK;
 }
 
@@ -564,6 +580,82 @@ int hfk_op_set_parameters(hfk_op* op, double lam, double mu) {
   op->lam = lam;
   op->mu = mu;
   op->storage = -1;  // the stored tangents belong to the previous parameters
+  return HFK_OK;
+}
+
+int hfk_op_set_traction(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                        const double* traction) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (nfaces == 0) {
+    op->has_traction = false;
+    return HFK_OK;
+  }
+  if (nfaces < 0 || !face_cell || !face_local || !traction)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: invalid traction faces");
+  for (int64_t f = 0; f < nfaces; ++f) {
+    if (face_cell[f] < 0 || face_cell[f] >= op->ncells || face_local[f] < 0 || face_local[f] > 5)
+      return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: traction face out of range (cell, local face 0..5)");
+    for (int c = 0; c < 3; ++c)
+      if (!std::isfinite(traction[3 * f + c])) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: traction not finite");
+  }
+  // faces grouped by the colour of their cell and their local face: the faces of one launch belong to different
+  // cells of one colour, so they share no dof (two faces of one cell share the nodes of their common edge)
+  const int nc = *std::max_element(op->color.begin(), op->color.end());
+  const int ng = 6 * nc;
+  auto group = [&](int64_t f) { return 6 * (op->color[face_cell[f]] - 1) + face_local[f]; };
+  std::vector<int64_t> start((size_t)ng + 1, 0);
+  for (int64_t f = 0; f < nfaces; ++f) start[group(f) + 1]++;
+  for (int g = 0; g < ng; ++g) start[g + 1] += start[g];
+  std::vector<int64_t> fill(start.begin(), start.end() - 1);
+  std::vector<int32_t> order((size_t)nfaces);
+  std::vector<int64_t> slot((size_t)nfaces);
+  for (int64_t f = 0; f < nfaces; ++f) {
+    order[fill[group(f)]++] = (int32_t)f;
+    slot[f] = op->slot_of_cell[face_cell[f]];
+  }
+  hipStream_t s = op->ctx->stream;
+  DevBuf slot_d, local_d, trac_d, order_d;
+  HIPCHK(op->ctx, slot_d.alloc((size_t)nfaces * sizeof(int64_t)));
+  HIPCHK(op->ctx, local_d.alloc((size_t)nfaces * sizeof(int32_t)));
+  HIPCHK(op->ctx, trac_d.alloc((size_t)nfaces * 3 * sizeof(double)));
+  HIPCHK(op->ctx, order_d.alloc((size_t)nfaces * sizeof(int32_t)));
+  HIPCHK(op->ctx, hipMemcpyAsync(slot_d.p, slot.data(), (size_t)nfaces * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(local_d.p, face_local, (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(trac_d.p, traction, (size_t)nfaces * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(order_d.p, order.data(), (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  const int64_t nf = op->nfree + op->ndir;
+  HIPCHK(op->ctx, op->face_load.alloc((size_t)std::max<int64_t>(nf, 1) * sizeof(double)));
+  HIPCHK(op->ctx, launch_fill(op->face_load.as<double>(), nf, 0.0, s));
+  const OpDev d = op->dev();
+  for (int g = 0; g < ng; ++g)
+    HIPCHK(op->ctx, launch_face_load(d, slot_d.as<int64_t>(), local_d.as<int32_t>(), trac_d.as<double>(),
+                                     order_d.as<int32_t>() + start[g], start[g + 1] - start[g], op->nfree,
+                                     op->face_load.as<double>(), s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  op->has_traction = true;
+  return HFK_OK;
+}
+
+int hfk_op_set_load_factor(hfk_op* op, double load_factor) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!std::isfinite(load_factor)) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: load factor not finite");
+  op->load_factor = load_factor;
+  return HFK_OK;
+}
+
+int hfk_op_traction_load(hfk_op* op, double* f) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!f) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  const int64_t nf = op->nfree + op->ndir;
+  if (!op->has_traction) {
+    std::fill(f, f + nf, 0.0);
+    return HFK_OK;
+  }
+  HIPCHK(op->ctx, hipStreamSynchronize(op->ctx->stream));
+  HIPCHK(op->ctx, hipMemcpy(f, op->face_load.p, (size_t)nf * sizeof(double), hipMemcpyDeviceToHost));
   return HFK_OK;
 }
 

@@ -40,6 +40,14 @@ hipError_t launch_blocks(const OpDev& op, int storage, const void* tangents, con
 hipError_t launch_residual(const OpDev& op, const double* x, double* b, BatchRange r, hipStream_t s);
 // strain energy of the built-in law per batch into part[nbatches] (k_energy)
 hipError_t launch_energy(const OpDev& op, const double* x, double* part, int64_t nbatches, hipStream_t s);
+// boundary traction load of the faces `faces[0..nfaces)` (indices into face_slot/face_local/traction) added into f
+hipError_t launch_face_load(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* traction,
+                            const int32_t* faces, int64_t nfaces, int64_t nfree, double* f, hipStream_t s);
+// y -= a x
+hipError_t launch_sub_scaled(double* y, const double* x, double a, int64_t n, hipStream_t s);
+// part[0..nblocks) = scale * (fixed-order partial sums of a . b)
+hipError_t launch_dot_partials(const double* a, const double* b, int64_t n, double scale, double* part, int nblocks,
+                               hipStream_t s);
 // out[0] = sum (mode 0), min (1) or max (2) of part[0..n) in a fixed order
 hipError_t launch_reduce_partials(const double* part, int64_t n, int mode, double* out, hipStream_t s);
 hipError_t launch_assemble(const OpDev& op, const double* tangents64, double* nzval, BatchRange r, hipStream_t s);

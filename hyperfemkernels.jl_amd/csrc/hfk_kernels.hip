@@ -244,6 +244,79 @@ __global__ void k_fill(double* __restrict__ y, int64_t n, double v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = v;
 }
 
+// Boundary traction (a Neumann load on faces of the cells): one work-group per face, one thread per (component,
+// node of the face); the load of that dof, the sum over the P x P Gauss points of the face of dS t_c N_node, is added
+// into f (free ids at f[id - 1], Dirichlet ids at f[nfree - id - 1]).  The faces of one launch belong to different
+// cells of one colour, so no two of them share a dof.  Local face 2d + s is the reference-cube face xi_d = s; e1 < e2 are the
+// other directions, dS = w_a w_b |dx/dxi_e1 x dx/dxi_e2| with the trilinear map.
+__global__ __launch_bounds__(64) void k_face_load(OpDev op, const int64_t* __restrict__ face_slot,
+                                                  const int32_t* __restrict__ face_local,
+                                                  const double* __restrict__ traction, const int32_t* __restrict__ faces,
+                                                  int64_t nfree, double* __restrict__ f) {
+  const int64_t fi = faces[blockIdx.x];
+  const int64_t slot = face_slot[fi];
+  const int lf = face_local[fi];
+  const int d = lf >> 1, side = lf & 1;
+  const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  const int Q = op.Q, P = op.P, NN = Q * Q * Q;
+  const int tid = threadIdx.x;
+  if (tid >= 3 * Q * Q) return;
+  const int c = tid / (Q * Q), m = tid - c * Q * Q, i1 = m % Q, i2 = m / Q;
+  int idx[3];
+  idx[d] = side ? Q - 1 : 0;
+  idx[e1] = i1;
+  idx[e2] = i2;
+  const int t = idx[0] + Q * (idx[1] + Q * idx[2]);
+  const double* X = op.coords + slot * 24;
+  double acc = 0.0;
+  for (int b = 0; b < P; ++b)
+    for (int a = 0; a < P; ++a) {
+      double xi[3];
+      xi[d] = side;
+      xi[e1] = op.tab.xq[a];
+      xi[e2] = op.tab.xq[b];
+      double g1[3] = {0.0, 0.0, 0.0}, g2[3] = {0.0, 0.0, 0.0};
+      for (int v = 0; v < 8; ++v) {
+        double mv[3], dv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int bit = (v >> k) & 1;
+          mv[k] = bit ? xi[k] : 1.0 - xi[k];
+          dv[k] = bit ? 1.0 : -1.0;
+        }
+        const double d1 = dv[e1] * mv[e2] * mv[d], d2 = dv[e2] * mv[e1] * mv[d];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          g1[k] += d1 * X[v * 3 + k];
+          g2[k] += d2 * X[v * 3 + k];
+        }
+      }
+      const double n0 = g1[1] * g2[2] - g1[2] * g2[1], n1 = g1[2] * g2[0] - g1[0] * g2[2], n2 = g1[0] * g2[1] - g1[1] * g2[0];
+      const double dS = op.tab.wq[a] * op.tab.wq[b] * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      acc += dS * op.tab.N[a * Q + i1] * op.tab.N[b * Q + i2];
+    }
+  const int32_t id = op.ids[slot * 3 * NN + c * NN + t];
+  const double v = acc * traction[fi * 3 + c];
+  if (id > 0) f[id - 1] += v;
+  else if (id < 0) f[nfree - id - 1] += v;
+}
+
+__global__ void k_sub_scaled(double* __restrict__ y, const double* __restrict__ x, double a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] -= a * x[i];
+}
+
+// scale * sum a_i b_i per work-group into part[blockIdx.x], a fixed order for a fixed grid
+__global__ __launch_bounds__(kThreads) void k_dot_partials(const double* __restrict__ a, const double* __restrict__ b,
+                                                           int64_t n, double scale, double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    s += a[i] * b[i];
+  s = block_reduce<0>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = scale * s;
+}
+
 // Expand the stored tangents of every batch to D[(batch*C + cell)*np + q][81] (FP64), for inspection.
 template <int P, class TS, bool SYM>
 __global__ void k_expand(const void* __restrict__ tangents, double* __restrict__ out, int64_t nbatches) {
@@ -404,6 +477,28 @@ hipError_t launch_energy(const OpDev& op, const double* x, double* part, int64_t
 #define CALL(q, p) energy_law<q, p>(op, x, part, nbatches, s)
   HFK_DISPATCH_QP(op.Q, op.P, CALL);
 #undef CALL
+}
+
+hipError_t launch_face_load(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* traction,
+                            const int32_t* faces, int64_t nfaces, int64_t nfree, double* f, hipStream_t s) {
+  if (nfaces <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_face_load, dim3((unsigned)nfaces), dim3(64), 0, s, op, face_slot, face_local, traction, faces,
+                     nfree, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_sub_scaled(double* y, const double* x, double a, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_sub_scaled, dim3((unsigned)blocks), dim3(256), 0, s, y, x, a, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_partials(const double* a, const double* b, int64_t n, double scale, double* part, int nblocks,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partials, dim3((unsigned)nblocks), dim3(kThreads), 0, s, a, b, n, scale, part);
+  return hipGetLastError();
 }
 
 hipError_t launch_reduce_partials(const double* part, int64_t n, int mode, double* out, hipStream_t s) {

@@ -82,6 +82,11 @@ struct hfk_op {
   std::vector<int32_t> loc_of;                  // tensor index c*nn + t -> table-local dof
   hfk::DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64, state;
   hfk::DevBuf part;  // per-batch partial results of the functionals
+  // boundary traction (Neumann load): its load vector over the free then the Dirichlet ids, scaled by load_factor in
+  // residual! and the energy
+  bool has_traction = false;
+  double load_factor = 1.0;
+  hfk::DevBuf face_load;
   // a second copy of the tangents at the same state in another storage, for a smoother of another precision
   hfk::DevBuf tangents_s;
   bool has_mu_field = false;

@@ -637,7 +637,7 @@ struct hfk_oseen {
   int64_t nfaces = 0;
   int nqo = 0, nqb = 0;
   double Pc = 0, ZA = 0;
-  std::vector<int64_t> face_start;  // faces of colour c: faces_d[face_start[c] .. face_start[c+1])
+  std::vector<int64_t> face_start;  // faces of launch group g: faces_d[face_start[g] .. face_start[g+1])
   DevBuf face_cell, face_normal, face_xo, face_xb, faces_d;
   DevBuf step_stage;  // host FE-function values of hfk_oseen_step_fields
   hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
@@ -891,14 +891,27 @@ int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell
   for (int64_t f = 0; f < nfaces; ++f)
     if (face_cell[f] < 0 || face_cell[f] >= op->ncells)
       return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: outlet face cell out of range");
-  // faces ordered by the colour of their cell
+  // faces grouped by the colour of their cell and, within it, by their rank among the faces of that cell (0 for
+  // the first face of a cell, 1 for a second one, ...): the faces of one launch belong to different cells of one
+  // colour, so no two of them add into the same CSC entries
   const size_t nc = op->color_start.size() - 1;
-  std::vector<int64_t> start(nc + 1, 0);
-  for (int64_t f = 0; f < nfaces; ++f) start[op->color[face_cell[f]]]++;
-  for (size_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+  std::vector<int> rank((size_t)nfaces);
+  int maxrank = 0;
+  {
+    std::vector<int> seen((size_t)op->ncells, 0);
+    for (int64_t f = 0; f < nfaces; ++f) {
+      rank[f] = seen[face_cell[f]]++;
+      maxrank = std::max(maxrank, rank[f]);
+    }
+  }
+  const size_t nr = (size_t)maxrank + 1, ng = nc * nr;
+  auto group = [&](int64_t f) { return (size_t)(op->color[face_cell[f]] - 1) * nr + (size_t)rank[f]; };
+  std::vector<int64_t> start(ng + 1, 0);
+  for (int64_t f = 0; f < nfaces; ++f) start[group(f) + 1]++;
+  for (size_t g = 0; g < ng; ++g) start[g + 1] += start[g];
   std::vector<int64_t> fill(start.begin(), start.end() - 1);
   std::vector<int32_t> order((size_t)nfaces);
-  for (int64_t f = 0; f < nfaces; ++f) order[fill[op->color[face_cell[f]] - 1]++] = (int32_t)f;
+  for (int64_t f = 0; f < nfaces; ++f) order[fill[group(f)]++] = (int32_t)f;
   HIPCHK(op->ctx, op->face_cell.alloc((size_t)nfaces * sizeof(int32_t)));
   HIPCHK(op->ctx, hipMemcpy(op->face_cell.p, face_cell, (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(op->ctx, op->face_normal.alloc((size_t)nfaces * 3 * sizeof(double)));

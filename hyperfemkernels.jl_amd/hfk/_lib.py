@@ -36,6 +36,9 @@ SIGNATURES = {
     "hfk_op_set_dirichlet": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_parameters": (C.c_int, [_vp, C.c_double, C.c_double]),
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
+    "hfk_op_set_traction": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),
+    "hfk_op_set_load_factor": (C.c_int, [_vp, C.c_double]),
+    "hfk_op_traction_load": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_mu_function": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int64]),
     "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_energy": (C.c_int, [_vp, _f64p, _f64p]),

@@ -219,3 +219,16 @@ def hex_space(n, order: int, domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), dirichlet=di
                     ldof=ldof, nodeperm=nodeperm, coords=coords, nfree=3 * nfree_nodes, ndir=3 * ndir_nodes,
                     node_xyz=node_xyz, node_free=~isdir, free_node_of=np.nonzero(~isdir)[0],
                     dir_node_of=np.nonzero(isdir)[0], lattice=L)
+
+
+def boundary_faces(space: HexSpace, axis: int, side: int):
+    """The cell faces on the box face ``axis`` = min (side 0) or max (side 1), the boundary entity a Gridap tag
+    names (e.g. the x = 1 face): (face_cell, face_local) with the cells numbered x fastest and the local face
+    2 axis + side of the reference cube."""
+    nx, ny, nz = space.n
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cell = (I + nx * (J + ny * K)).transpose(2, 1, 0).reshape(-1)
+    idx = (I, J, K)[axis].transpose(2, 1, 0).reshape(-1)
+    on = idx == (0 if side == 0 else space.n[axis] - 1)
+    face_cell = cell[on].astype(np.int32)
+    return face_cell, np.full(face_cell.size, 2 * axis + side, dtype=np.int32)

@@ -127,6 +127,31 @@ class KernelOperator:
         return KernelOperator(form, self.space, jacobian=self.jacobian_kind, dirichlet_values=getattr(self, "_dvals", None),
                               ctx=self.ctx)
 
+    def set_traction(self, face_cell, face_local, traction):
+        """A dead traction on faces of the cells (``mesh.boundary_faces``): the Neumann load -Λ ∫ t·v dΓ in the
+        residual and -Λ ∫ t·u dΓ in the energy; ``traction`` (3,) for every face or (nfaces, 3); ``face_cell=None``
+        removes it."""
+        if face_cell is None:
+            check(self.lib.hfk_op_set_traction(self.handle, 0, None, None, None), self.ctx.handle)
+            return
+        fc = np.ascontiguousarray(face_cell, dtype=np.int32)
+        fl = np.ascontiguousarray(face_local, dtype=np.int32)
+        tr = np.ascontiguousarray(np.broadcast_to(np.asarray(traction, dtype=np.float64), (fc.size, 3)))
+        if fl.size != fc.size:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: traction faces")
+        check(self.lib.hfk_op_set_traction(self.handle, fc.size, ptr(fc, C.c_int32), ptr(fl, C.c_int32),
+                                           ptr(tr, C.c_double)), self.ctx.handle)
+
+    def set_load_factor(self, load_factor: float):
+        """The load factor Λ of the traction (a number of the form: nothing is recomputed)."""
+        check(self.lib.hfk_op_set_load_factor(self.handle, float(load_factor)), self.ctx.handle)
+
+    def traction_load(self):
+        """The unscaled traction load vector: (f on the free ids, f on the Dirichlet ids)."""
+        f = np.zeros(self.nfree + self.ndir)
+        check(self.lib.hfk_op_traction_load(self.handle, ptr(f, C.c_double)), self.ctx.handle)
+        return f[:self.nfree], f[self.nfree:]
+
     def set_mu_field(self, mu_points):
         if mu_points is None:
             check(self.lib.hfk_op_set_mu_field(self.handle, None), self.ctx.handle)

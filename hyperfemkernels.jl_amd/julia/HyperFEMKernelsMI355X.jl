@@ -77,6 +77,20 @@ set_dirichlet!(op::DeviceOperator, d::Vector{Float64}) =
 update!(op::DeviceOperator; λ::Real, μ::Real) =
     (check(ccall((:hfk_op_set_parameters, libhfk), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), op.ptr, λ, μ), op.ctx.ptr); op)
 
+"""
+    set_traction!(op, face_cell, face_local, traction)
+
+A dead traction on faces of the cells (the Neumann load `-Λ ∫ t⋅v dΓ` of a boundary group, src/assembly.jl:539-578):
+`face_cell` 0-based cells of the faces of `BoundaryTriangulation(model, tags=...)`, `face_local` the local face 0..5
+(ξ_d = s ↦ 2d + s), `traction` 3 × nfaces.  Enters residual! and the energy; the Jacobian does not change.
+"""
+set_traction!(op::DeviceOperator, face_cell::Vector{Int32}, face_local::Vector{Int32}, traction::Matrix{Float64}) =
+    check(ccall((:hfk_op_set_traction, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}), op.ptr,
+                length(face_cell), face_cell, face_local, traction), op.ctx.ptr)
+"The load factor Λ of the traction (a number of the form, as in `en(Λ)` of test/solvers/test_functionals.jl:40-45)."
+set_load_factor!(op::DeviceOperator, Λ::Real) =
+    check(ccall((:hfk_op_set_load_factor, libhfk), Cint, (Ptr{Cvoid}, Cdouble), op.ptr, Λ), op.ctx.ptr)
+
 "Pointwise shear modulus at the quadrature points (np × ncells, tensor point order); `nothing`: the constant μ."
 set_mu_field!(op::DeviceOperator, μ::Union{Nothing,Matrix{Float64}}) =
     check(ccall((:hfk_op_set_mu_field, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, μ === nothing ? C_NULL : μ),

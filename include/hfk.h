@@ -75,6 +75,18 @@ int hfk_op_set_dirichlet(hfk_op* op, const double* dvals);
  * of the same structure, src/assembly.jl:693-744 and src/tree.jl:270-283: the colouring, the pattern, the tables and
  * compiled laws, which take lam and mu as arguments, are kept).  The stored tangents are invalidated (prepare! again). */
 int hfk_op_set_parameters(hfk_op* op, double lam, double mu);
+/* A boundary traction, the Neumann load  -Λ ∫ t·v dΓ  of a boundary group (src/assembly.jl:539-578; the load term of
+ * test/solvers/test_functionals.jl:40-45), on nfaces faces of the cells: face_cell (0-based cell), face_local (0..5:
+ * the reference-cube face xi_d = s is 2d + s, vertex order v = i + 2j + 4k), traction nfaces x 3 (a dead load per
+ * unit area of the physical face), integrated with the operator's P x P Gauss rule on the face.  residual! subtracts
+ * Λ f on the free rows and the energy subtracts Λ (f·x + f_D·u_D); the Jacobian does not change.  The load vector f
+ * is computed on the device when the faces are set; nfaces = 0 removes it. */
+int hfk_op_set_traction(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                        const double* traction);
+/* The load factor Λ of the traction (1 by default; a number of the form, rebound without recomputing f). */
+int hfk_op_set_load_factor(hfk_op* op, double load_factor);
+/* The traction load vector f (unscaled): nfree values on the free ids, then ndir on the Dirichlet ids. */
+int hfk_op_traction_load(hfk_op* op, double* f);
 /* Pointwise shear modulus mu(x) at the quadrature points, np*ncells in tensor point order, [cell][q]
  * (dense auxiliary point data; NULL restores the constant mu). */
 int hfk_op_set_mu_field(hfk_op* op, const double* mu_points);

@@ -555,6 +555,65 @@ int orc_jv(orc_op* op, const double* T, const double* iJt, const double* dV, con
   return 0;
 }
 
+// A dead traction on faces of the cells (a boundary group, assembly.jl:539-578): f[dof] += sum_q dS t_c N_n(xi_q)
+// over the P x P Gauss points of each face (local face 2d + s: xi_d = s), with dS = w_a w_b |dx/dxi_e1 x dx/dxi_e2|
+// from the cell's trilinear map and N_n the tensor-product Lagrange function of table node n evaluated at the face
+// point; free ids into f[id - 1], Dirichlet ids into f[nfree - id - 1].
+int orc_face_load(orc_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local, const double* traction,
+                  double* f) {
+  const int Q = op->Q, P = op->P, nn = op->nn;
+  std::fill(f, f + op->nfree + op->ndir, 0.0);
+  std::vector<double> N[3], dN[3];
+  for (int k = 0; k < 3; ++k) {
+    N[k].resize(Q);
+    dN[k].resize(Q);
+  }
+  for (int64_t fc = 0; fc < nfaces; ++fc) {
+    const int cell = face_cell[fc], d = face_local[fc] / 2, side = face_local[fc] % 2;
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    for (int b = 0; b < P; ++b)
+      for (int a = 0; a < P; ++a) {
+        double xi[3];
+        xi[d] = side;
+        xi[e1] = op->xq1[a];
+        xi[e2] = op->xq1[b];
+        double Jt[9] = {0};  // Jt[K + 3i] = d x_i / d xi_K
+        for (int v = 0; v < 8; ++v) {
+          double m[3], dm[3];
+          for (int k = 0; k < 3; ++k) {
+            const int bit = (v >> k) & 1;
+            m[k] = bit ? xi[k] : 1 - xi[k];
+            dm[k] = bit ? 1.0 : -1.0;
+          }
+          const double dM[3] = {dm[0] * m[1] * m[2], m[0] * dm[1] * m[2], m[0] * m[1] * dm[2]};
+          const double* x = &op->coords[((size_t)cell * 8 + v) * 3];
+          for (int K = 0; K < 3; ++K)
+            for (int i = 0; i < 3; ++i) Jt[K + 3 * i] += dM[K] * x[i];
+        }
+        double t1[3], t2[3];
+        for (int i = 0; i < 3; ++i) {
+          t1[i] = Jt[e1 + 3 * i];
+          t2[i] = Jt[e2 + 3 * i];
+        }
+        const double nrm[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2], t1[0] * t2[1] - t1[1] * t2[0]};
+        const double dS = op->wq1[a] * op->wq1[b] * std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        for (int k = 0; k < 3; ++k) orc_lagrange1d(Q, xi[k], N[k].data(), dN[k].data());
+        for (int n = 0; n < nn; ++n) {
+          const int t = op->tensor_of[n];
+          const double Nn = N[0][t % Q] * N[1][(t / Q) % Q] * N[2][t / (Q * Q)];
+          if (Nn == 0.0) continue;
+          for (int c = 0; c < 3; ++c) {
+            const int32_t id = op->dofs[(size_t)cell * op->nd + op->ldof[c * nn + n]];
+            const double v = dS * Nn * traction[3 * fc + c];
+            if (id > 0) f[id - 1] += v;
+            else f[op->nfree - id - 1] += v;
+          }
+        }
+      }
+  }
+  return 0;
+}
+
 // the displacement gradient at every quadrature point, G[(cell np + q) 9 + J + 3c] = d u_c / d x_J (Gridap's
 // grad u[J, c]): the pointwise input of a functional's integrand (functionals.jl:9-21)
 int orc_point_gradients(orc_op* op, const double* x, double* G) {

@@ -50,6 +50,7 @@ def load():
         "orc_jv": (C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int]),
         "orc_energy": (C.c_double, [C.c_void_p, _f64p]),
         "orc_point_gradients": (C.c_int, [C.c_void_p, _f64p, _f64p]),
+        "orc_face_load": (C.c_int, [C.c_void_p, C.c_int64, _i32p, _i32p, _f64p, _f64p]),
         "orc_residual": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_element_jacobians": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_pattern_nnz": (C.c_int64, [C.c_void_p]),
@@ -185,6 +186,16 @@ class OracleOp:
     def energy(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         return load().orc_energy(self.h, _p(x, C.c_double))
+
+    def face_load(self, face_cell, face_local, traction):
+        """The load vector of a dead traction on faces (nfaces x 3): (f on the free ids, f on the Dirichlet ids)."""
+        fc = np.ascontiguousarray(face_cell, dtype=np.int32)
+        fl = np.ascontiguousarray(face_local, dtype=np.int32)
+        tr = np.ascontiguousarray(traction, dtype=np.float64).reshape(-1, 3)
+        f = np.zeros(self.nfree + self.space.ndir)
+        load().orc_face_load(self.h, fc.size, _p(fc, C.c_int32), _p(fl, C.c_int32), _p(tr, C.c_double),
+                             _p(f, C.c_double))
+        return f[:self.nfree], f[self.nfree:]
 
     def point_gradients(self, x):
         """(ncells, np, 3, 3) with [cell, q, c, J] = d u_c / d x_J at every quadrature point (Dirichlet values of

@@ -127,3 +127,43 @@ def test_extremum_matches_oracle(hip_ctx, oracle, order):
             vmin, vmax = fmin(x), fmax(x)
             assert vmin == pytest.approx(g.min(), rel=1e-13) and vmax == pytest.approx(g.max(), rel=1e-13)
             assert vmin < vmax
+
+
+@pytest.mark.parametrize("order,law,n", [(1, "svk", (3, 3, 2)), (2, "neohookean", (3, 2, 2)), (3, "svk", (2, 2, 1))])
+def test_traction_load(hip_ctx, oracle, order, law, n):
+    """A dead traction on two boundary faces (a boundary group of the residual and of the energy): the load vector
+    equals the oracle's (1e-13); residual! = residual without the load - Λ f on the free rows; the energy = the
+    oracle's energy - Λ (f . x + f_D . u_D) (1e-12) and its central difference = residual . dx (1e-6); the assembled
+    Jacobian does not change; Λ = 0 and removing the faces give back the volume terms."""
+    from hfk.mesh import boundary_faces
+
+    print(f"traction Q{order} {law}", flush=True)
+    sp, dv, x, op, orc, rng = problem(hip_ctx, oracle, order, law, n, seed=31)
+    fc1, fl1 = boundary_faces(sp, 0, 1)
+    fc2, fl2 = boundary_faces(sp, 2, 0)
+    fc, fl = np.concatenate([fc1, fc2]), np.concatenate([fl1, fl2])
+    tr = np.concatenate([np.tile([0.3, -0.2, 0.7], (fc1.size, 1)), np.tile([-1.1, 0.4, 0.25], (fc2.size, 1))])
+    fun = KernelFunctional(op)
+    b0 = op.residual(np.zeros(sp.nfree), x)
+    A0 = op.jacobian(op.allocate_jacobian(), x).data.copy()
+    E0 = fun(x)
+    op.set_traction(fc, fl, tr)
+    lam = 1.7
+    op.set_load_factor(lam)
+    ff, fd = orc.face_load(fc, fl, tr)
+    gf, gd = op.traction_load()
+    scale = np.abs(np.concatenate([ff, fd])).max()
+    assert np.abs(gf - ff).max() <= 1e-13 * scale and np.abs(gd - fd).max() <= 1e-13 * scale
+    b = op.residual(np.zeros(sp.nfree), x)
+    assert np.abs(b - (b0 - lam * gf)).max() <= 1e-14 * np.abs(b).max()
+    E = fun(x)
+    assert E == pytest.approx(orc.energy(x) - lam * (ff @ x + fd @ dv), rel=1e-12, abs=1e-14)
+    dx = rng.standard_normal(sp.nfree)
+    h = 1e-6
+    assert (fun(x + h * dx) - fun(x - h * dx)) / (2 * h) == pytest.approx(b @ dx, rel=1e-6)
+    assert np.array_equal(op.jacobian(op.allocate_jacobian(), x).data, A0)
+    op.set_load_factor(0.0)
+    assert np.array_equal(op.residual(np.zeros(sp.nfree), x), b0) and fun(x) == E0
+    op.set_load_factor(1.0)
+    op.set_traction(None, None, None)
+    assert np.array_equal(op.residual(np.zeros(sp.nfree), x), b0) and fun(x) == E0

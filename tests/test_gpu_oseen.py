@@ -224,3 +224,29 @@ def test_oseen_update_bdf2_switch(hip_ctx):
     assert np.array_equal(b2, ref.residual(np.zeros(sp.nfree), x)) and not np.array_equal(b1, b2)
     assert np.array_equal(op.jacobian(op.allocate_jacobian()).data, ref.jacobian(ref.allocate_jacobian()).data)
     assert op.step_coefficients() == ref.step_coefficients()
+
+
+def test_oseen_two_faces_of_one_cell(hip_ctx):
+    """Faces of one cell go to different launches: with every outlet face listed twice (two faces per cell) the
+    residual and CSC values equal the oracle's system with each face term counted twice."""
+    import dataclasses
+
+    from oracle import oseen as O
+
+    from hfk.tetmesh import outlet_faces
+
+    print("oseen two faces per cell", flush=True)
+    sp, qp, qw, fields, dv, op, rng = setup(hip_ctx, (3, 2, 2), True, seed=23)
+    of = outlet_faces(sp)
+    twice = dataclasses.replace(of, cell=np.concatenate([of.cell, of.cell]), normal=np.concatenate([of.normal] * 2),
+                                xo=np.concatenate([of.xo] * 2), xb=np.concatenate([of.xb] * 2),
+                                xb_phys=np.concatenate([of.xb_phys] * 2))
+    wb = rng.standard_normal((len(of.cell), of.xb.shape[1], 3))
+    op.set_outlet(twice, np.concatenate([wb, wb]), Pc=0.7, ZA=2.3)
+    K, load = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.05, op.c0)
+    Kf, Lf = O.outlet_terms(of.xo, np.concatenate([of.xb, wb], axis=2), of.normal, 1.0, 0.7, 2.3)
+    Kt, lt = O.add_faces(K, load, np.concatenate([of.cell, of.cell]), np.concatenate([Kf, Kf]), np.concatenate([Lf, Lf]))
+    colptr, rowval, nzpos = op.pattern()
+    x = rng.standard_normal(sp.nfree)
+    assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, Kt, lt)) < TOL
+    assert rel(op.jacobian(op.allocate_jacobian()).data, O.jacobian_values(Kt, nzpos, rowval.size)) < TOL

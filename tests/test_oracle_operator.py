@@ -2,6 +2,7 @@
 
 * residual = gradient of the total energy (central differences), with Dirichlet values;
 * point gradients of affine fields = their constant gradient on trilinear cells;
+* boundary traction load: total force t x area and first moment t x area x centroid per component;
 * element Jacobians assembled densely = derivative of the residual (central differences);
 * sum-factorised stored-T product (src/matrixfree.jl:341-424) = dense assembled matrix times dx
   (test/test_matrixfree.jl:74-96), serial = threaded (coloured determinism, test/test_assembly.jl:40-47);
@@ -68,6 +69,32 @@ def test_point_gradients_of_affine_fields(oracle, order):
     G = op.point_gradients(sp.free_values(u))
     assert G.shape == (sp.ncells, (order + 1) ** 3, 3, 3)
     assert np.abs(G - A).max() < 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_face_load_closed_forms(oracle, order):
+    """A dead traction on two box faces with different tractions: per component the load sums to t x area (the
+    face's functions sum to one) and its first moment sum_n f_n x_n to t x area x centroid (the nodes reproduce the
+    trilinear map); interior vertices perturbed (boundary faces stay planar), Dirichlet dofs included."""
+    from hfk.mesh import boundary_faces
+
+    sp = hex_space((3, 2, 2), order, domain=(0.0, 2.0, 0.0, 1.5, 0.0, 0.5), perturb=0.15, seed=1,
+                   dirichlet=dirichlet_closed_x0)
+    op = oracle.OracleOp(sp, "svk", 2.0, 1.0, dvals=np.zeros(sp.ndir))
+    fc1, fl1 = boundary_faces(sp, 0, 1)
+    fc2, fl2 = boundary_faces(sp, 1, 0)
+    t1, t2 = np.array([0.3, -0.2, 0.7]), np.array([-1.1, 0.4, 0.25])
+    tr = np.concatenate([np.tile(t1, (fc1.size, 1)), np.tile(t2, (fc2.size, 1))])
+    ff, fd = op.face_load(np.concatenate([fc1, fc2]), np.concatenate([fl1, fl2]), tr)
+    A1, A2 = 1.5 * 0.5, 2.0 * 0.5
+    m1, m2 = A1 * np.array([2.0, 0.75, 0.25]), A2 * np.array([1.0, 0.0, 0.25])
+    f = np.concatenate([ff, fd])
+    comp = np.concatenate([np.arange(sp.nfree) % 3, np.arange(sp.ndir) % 3])
+    xyz = np.concatenate([sp.free_xyz(), np.repeat(sp.node_xyz[sp.dir_node_of], 3, axis=0)])
+    for c in range(3):
+        sel = comp == c
+        assert f[sel].sum() == pytest.approx(t1[c] * A1 + t2[c] * A2, rel=1e-13)
+        assert np.allclose((f[sel][:, None] * xyz[sel]).sum(axis=0), t1[c] * m1 + t2[c] * m2, rtol=1e-13, atol=1e-14)
 
 
 @pytest.mark.parametrize("order,law", [(1, "neohookean"), (2, "svk"), (2, "linear")])
