@@ -69,10 +69,42 @@ int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev,
   const OpDev d = op->dev();
   if (!op->deterministic) {
     HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, BatchRange{0, 0}, s, accumulate ? alpha : 1.0, &op->amap));
-    return HFK_OK;
+  } else {
+    for (const BatchRange& r : op->color_batches)
+      HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
   }
-  for (const BatchRange& r : op->color_batches)
-    HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
+  // the boundary group of the Robin faces (its Jacobian is the face mass matrix; increments vanish on the
+  // Dirichlet dofs)
+  return robin_apply(op, dx_dev, nullptr, y_dev, accumulate ? alpha : 1.0, s);
+}
+
+int robin_apply(hfk_op* op, const double* uf, const double* ud, double* y, double alpha, hipStream_t s) {
+  if (op->robin_n == 0) return HFK_OK;
+  const OpDev d = op->dev();
+  for (size_t g = 0; g + 1 < op->robin_start.size(); ++g)
+    HIPCHK(op->ctx, launch_face_apply(d, op->robin_slot.as<int64_t>(), op->robin_local.as<int32_t>(),
+                                      op->robin_M.as<double>(), op->robin_order.as<int32_t>() + op->robin_start[g],
+                                      op->robin_start[g + 1] - op->robin_start[g], uf, ud, y, alpha, s));
+  return HFK_OK;
+}
+
+int robin_blocks(hfk_op* op, const int32_t* block_of, double* B, hipStream_t s) {
+  if (op->robin_n == 0) return HFK_OK;
+  const OpDev d = op->dev();
+  for (size_t g = 0; g + 1 < op->robin_start.size(); ++g)
+    HIPCHK(op->ctx, launch_face_blocks(d, op->robin_slot.as<int64_t>(), op->robin_local.as<int32_t>(),
+                                       op->robin_M.as<double>(), op->robin_order.as<int32_t>() + op->robin_start[g],
+                                       op->robin_start[g + 1] - op->robin_start[g], block_of, B, s));
+  return HFK_OK;
+}
+
+int robin_assemble(hfk_op* op, double* nzval, hipStream_t s) {
+  if (op->robin_n == 0) return HFK_OK;
+  const OpDev d = op->dev();
+  for (size_t g = 0; g + 1 < op->robin_start.size(); ++g)
+    HIPCHK(op->ctx, launch_face_assemble(d, op->robin_slot.as<int64_t>(), op->robin_local.as<int32_t>(),
+                                         op->robin_M.as<double>(), op->robin_order.as<int32_t>() + op->robin_start[g],
+                                         op->robin_start[g + 1] - op->robin_start[g], nzval, s));
   return HFK_OK;
 }
 
@@ -318,10 +350,10 @@ int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) 
     const OpDev d = op->dev();
     for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_residual(d, x_dev, b_dev, r, s));
   }
-  // the boundary groups after the volume group: minus the traction load on the free rows
+  // the boundary groups after the volume group: minus the traction load on the free rows, plus the springs
   if (op->has_traction)
     HIPCHK(op->ctx, launch_sub_scaled(b_dev, op->face_load.as<double>(), op->load_factor, op->nfree, s));
-  return HFK_OK;
+  return robin_apply(op, x_dev, op->dvals.as<double>(), b_dev, 1.0, s);
 }
 
 // strain energy into out_dev[0]: per-batch sums, then their sum, both in a fixed order
@@ -329,7 +361,7 @@ constexpr int kDotBlocks = 256;  // work-groups of the load's dot products (a fi
 
 int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) {
   const int64_t nload = op->has_traction ? 2 * kDotBlocks : 0;
-  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches + nload, 1) * sizeof(double)));
+  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches + nload + op->robin_n, 1) * sizeof(double)));
   double* part = op->part.as<double>();
   if (op->rtc) {
     int rc = rtc_energy(op, x_dev, part, s);
@@ -343,7 +375,10 @@ int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) 
     HIPCHK(op->ctx, launch_dot_partials(f + op->nfree, op->dvals.as<double>(), op->ndir, -op->load_factor,
                                         part + op->nbatches + kDotBlocks, kDotBlocks, s));
   }
-  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches + nload, 0, out_dev, s));
+  if (op->robin_n > 0)  // plus the energy of the springs, one partial per face
+    HIPCHK(op->ctx, launch_face_energy(op->dev(), op->robin_slot.as<int64_t>(), op->robin_local.as<int32_t>(),
+                                       op->robin_M.as<double>(), op->robin_n, x_dev, part + op->nbatches + nload, s));
+  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches + nload + op->robin_n, 0, out_dev, s));
   return HFK_OK;
 }
 
@@ -380,7 +415,7 @@ int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t
   if ((rc = op_prepare_into(op, 0, op->D64.p, x_dev, s))) return rc;
   HIPCHK(op->ctx, launch_fill(nz_dev, (int64_t)op->rowval.size(), 0.0, s));
   for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_assemble(d, op->D64.as<double>(), nz_dev, r, s));
-  return HFK_OK;
+  return robin_assemble(op, nz_dev, s);
 }
 
 }  // namespace hfk
@@ -390,6 +425,42 @@ int run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s)
   return op_run_jacobian(op, x_dev, nz_dev, s);
 }
 }  // namespace
+namespace {
+
+// Faces grouped by the colour of their cell and their local face: the faces of one launch group belong to different
+// cells of one colour, so they share no dof (two faces of one cell share the nodes of their common edge).  order:
+// face indices group by group, start: group bounds, slot: the batch slot of each face's cell.
+void face_groups(const hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                 std::vector<int32_t>& order, std::vector<int64_t>& start, std::vector<int64_t>& slot) {
+  const int nc = *std::max_element(op->color.begin(), op->color.end());
+  const int ng = 6 * nc;
+  auto group = [&](int64_t f) { return 6 * (op->color[face_cell[f]] - 1) + face_local[f]; };
+  start.assign((size_t)ng + 1, 0);
+  for (int64_t f = 0; f < nfaces; ++f) start[group(f) + 1]++;
+  for (int g = 0; g < ng; ++g) start[g + 1] += start[g];
+  std::vector<int64_t> fill(start.begin(), start.end() - 1);
+  order.assign((size_t)nfaces, 0);
+  slot.assign((size_t)nfaces, 0);
+  for (int64_t f = 0; f < nfaces; ++f) {
+    order[fill[group(f)]++] = (int32_t)f;
+    slot[f] = op->slot_of_cell[face_cell[f]];
+  }
+}
+
+int check_faces(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local, const double* v,
+                int nv) {
+  if (nfaces < 0 || !face_cell || !face_local || !v) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: invalid faces");
+  for (int64_t f = 0; f < nfaces; ++f) {
+    if (face_cell[f] < 0 || face_cell[f] >= op->ncells || face_local[f] < 0 || face_local[f] > 5)
+      return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: face out of range (cell, local face 0..5)");
+    for (int c = 0; c < nv; ++c)
+      if (!std::isfinite(v[nv * f + c])) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: face data not finite");
+  }
+  return HFK_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int hfk_version(void) { return 1; }
@@ -583,6 +654,39 @@ int hfk_op_set_parameters(hfk_op* op, double lam, double mu) {
   return HFK_OK;
 }
 
+int hfk_op_set_robin(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                     const double* stiffness) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (nfaces == 0) {
+    op->robin_n = 0;
+    op->storage = -1;
+    return HFK_OK;
+  }
+  if ((rc = check_faces(op, nfaces, face_cell, face_local, stiffness, 1))) return rc;
+  std::vector<int32_t> order;
+  std::vector<int64_t> start, slot;
+  face_groups(op, nfaces, face_cell, face_local, order, start, slot);
+  hipStream_t s = op->ctx->stream;
+  DevBuf k_d;
+  HIPCHK(op->ctx, k_d.alloc((size_t)nfaces * sizeof(double)));
+  HIPCHK(op->ctx, op->robin_slot.alloc((size_t)nfaces * sizeof(int64_t)));
+  HIPCHK(op->ctx, op->robin_local.alloc((size_t)nfaces * sizeof(int32_t)));
+  HIPCHK(op->ctx, op->robin_order.alloc((size_t)nfaces * sizeof(int32_t)));
+  HIPCHK(op->ctx, op->robin_M.alloc((size_t)nfaces * op->Q * op->Q * op->Q * op->Q * sizeof(double)));
+  HIPCHK(op->ctx, hipMemcpyAsync(k_d.p, stiffness, (size_t)nfaces * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(op->robin_slot.p, slot.data(), (size_t)nfaces * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(op->robin_local.p, face_local, (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, hipMemcpyAsync(op->robin_order.p, order.data(), (size_t)nfaces * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(op->ctx, launch_face_mass(op->dev(), op->robin_slot.as<int64_t>(), op->robin_local.as<int32_t>(),
+                                   k_d.as<double>(), nfaces, op->robin_M.as<double>(), s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  op->robin_start = start;
+  op->robin_n = nfaces;
+  op->storage = -1;  // the tangents of a matrix-free Jacobian belong to the operator without the springs
+  return HFK_OK;
+}
+
 int hfk_op_set_traction(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
                         const double* traction) {
   int rc = bind(op);
@@ -591,29 +695,11 @@ int hfk_op_set_traction(hfk_op* op, int64_t nfaces, const int32_t* face_cell, co
     op->has_traction = false;
     return HFK_OK;
   }
-  if (nfaces < 0 || !face_cell || !face_local || !traction)
-    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: invalid traction faces");
-  for (int64_t f = 0; f < nfaces; ++f) {
-    if (face_cell[f] < 0 || face_cell[f] >= op->ncells || face_local[f] < 0 || face_local[f] > 5)
-      return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: traction face out of range (cell, local face 0..5)");
-    for (int c = 0; c < 3; ++c)
-      if (!std::isfinite(traction[3 * f + c])) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: traction not finite");
-  }
-  // faces grouped by the colour of their cell and their local face: the faces of one launch belong to different
-  // cells of one colour, so they share no dof (two faces of one cell share the nodes of their common edge)
-  const int nc = *std::max_element(op->color.begin(), op->color.end());
-  const int ng = 6 * nc;
-  auto group = [&](int64_t f) { return 6 * (op->color[face_cell[f]] - 1) + face_local[f]; };
-  std::vector<int64_t> start((size_t)ng + 1, 0);
-  for (int64_t f = 0; f < nfaces; ++f) start[group(f) + 1]++;
-  for (int g = 0; g < ng; ++g) start[g + 1] += start[g];
-  std::vector<int64_t> fill(start.begin(), start.end() - 1);
-  std::vector<int32_t> order((size_t)nfaces);
-  std::vector<int64_t> slot((size_t)nfaces);
-  for (int64_t f = 0; f < nfaces; ++f) {
-    order[fill[group(f)]++] = (int32_t)f;
-    slot[f] = op->slot_of_cell[face_cell[f]];
-  }
+  if ((rc = check_faces(op, nfaces, face_cell, face_local, traction, 3))) return rc;
+  std::vector<int32_t> order;
+  std::vector<int64_t> start, slot;
+  face_groups(op, nfaces, face_cell, face_local, order, start, slot);
+  const int ng = (int)start.size() - 1;
   hipStream_t s = op->ctx->stream;
   DevBuf slot_d, local_d, trac_d, order_d;
   HIPCHK(op->ctx, slot_d.alloc((size_t)nfaces * sizeof(int64_t)));

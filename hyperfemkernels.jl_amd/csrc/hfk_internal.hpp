@@ -43,6 +43,18 @@ hipError_t launch_energy(const OpDev& op, const double* x, double* part, int64_t
 // boundary traction load of the faces `faces[0..nfaces)` (indices into face_slot/face_local/traction) added into f
 hipError_t launch_face_load(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* traction,
                             const int32_t* faces, int64_t nfaces, int64_t nfree, double* f, hipStream_t s);
+// Robin faces: face mass matrices M (nfaces x Q^4, k included) and the terms built from them, per launch group
+hipError_t launch_face_mass(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* kf,
+                            int64_t nfaces, double* M, hipStream_t s);
+hipError_t launch_face_apply(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                             const int32_t* faces, int64_t n, const double* uf, const double* ud, double* y, double alpha,
+                             hipStream_t s);
+hipError_t launch_face_blocks(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                              const int32_t* faces, int64_t n, const int32_t* block_of, double* B, hipStream_t s);
+hipError_t launch_face_assemble(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                                const int32_t* faces, int64_t n, double* nzval, hipStream_t s);
+hipError_t launch_face_energy(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                              int64_t nfaces, const double* x, double* part, hipStream_t s);
 // y -= a x
 hipError_t launch_sub_scaled(double* y, const double* x, double a, int64_t n, hipStream_t s);
 // part[0..nblocks) = scale * (fixed-order partial sums of a . b)

@@ -244,6 +244,160 @@ __global__ void k_fill(double* __restrict__ y, int64_t n, double v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = v;
 }
 
+// dS of Gauss point (a, b) of the reference-cube face xi_d = side (other directions e1 < e2) under the trilinear map
+__device__ __forceinline__ double face_dS(const Tab& tab, const double* __restrict__ X, int d, int side, int e1, int e2,
+                                          int a, int b) {
+  double xi[3];
+  xi[d] = side;
+  xi[e1] = tab.xq[a];
+  xi[e2] = tab.xq[b];
+  double g1[3] = {0.0, 0.0, 0.0}, g2[3] = {0.0, 0.0, 0.0};
+  for (int v = 0; v < 8; ++v) {
+    double mv[3], dv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int bit = (v >> k) & 1;
+      mv[k] = bit ? xi[k] : 1.0 - xi[k];
+      dv[k] = bit ? 1.0 : -1.0;
+    }
+    const double d1 = dv[e1] * mv[e2] * mv[d], d2 = dv[e2] * mv[e1] * mv[d];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      g1[k] += d1 * X[v * 3 + k];
+      g2[k] += d2 * X[v * 3 + k];
+    }
+  }
+  const double n0 = g1[1] * g2[2] - g1[2] * g2[1], n1 = g1[2] * g2[0] - g1[0] * g2[2], n2 = g1[0] * g2[1] - g1[1] * g2[0];
+  return tab.wq[a] * tab.wq[b] * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+}
+
+// tensor node of face node m = i1 + Q i2 (i1 along e1, i2 along e2) of the face xi_d = side
+__device__ __forceinline__ int face_tnode(int Q, int d, int side, int e1, int e2, int m) {
+  int idx[3];
+  idx[d] = side ? Q - 1 : 0;
+  idx[e1] = m % Q;
+  idx[e2] = m / Q;
+  return idx[0] + Q * (idx[1] + Q * idx[2]);
+}
+
+__device__ __forceinline__ void face_dirs(int lf, int& d, int& side, int& e1, int& e2) {
+  d = lf >> 1;
+  side = lf & 1;
+  e1 = d == 0 ? 1 : 0;
+  e2 = d == 2 ? 1 : 2;
+}
+
+// Robin (spring) term k int u.v dGamma on faces of the cells.  Per face the scalar face mass matrix
+// M[m1][m2] = k sum_q dS N_m1 N_m2 over the face's Q^2 nodes (the same for every component), computed once; the
+// residual, Jv, the node blocks, the assembled values and the energy are built from it.  Launch groups hold faces of
+// different cells of one colour (no shared dof).
+__global__ __launch_bounds__(256) void k_face_mass(OpDev op, const int64_t* __restrict__ face_slot,
+                                                   const int32_t* __restrict__ face_local, const double* __restrict__ kf,
+                                                   double* __restrict__ M) {
+  __shared__ double dS[16];
+  const int64_t f = blockIdx.x;
+  const int Q = op.Q, P = op.P, Q2 = Q * Q;
+  int d, side, e1, e2;
+  face_dirs(face_local[f], d, side, e1, e2);
+  const int tid = threadIdx.x;
+  if (tid < P * P) dS[tid] = face_dS(op.tab, op.coords + face_slot[f] * 24, d, side, e1, e2, tid % P, tid / P);
+  __syncthreads();
+  if (tid >= Q2 * Q2) return;
+  const int m1 = tid / Q2, m2 = tid - m1 * Q2;
+  const int i1 = m1 % Q, j1 = m1 / Q, i2 = m2 % Q, j2 = m2 / Q;
+  double s = 0.0;
+  for (int b = 0; b < P; ++b)
+    for (int a = 0; a < P; ++a)
+      s += dS[a + P * b] * op.tab.N[a * Q + i1] * op.tab.N[b * Q + j1] * op.tab.N[a * Q + i2] * op.tab.N[b * Q + j2];
+  M[f * Q2 * Q2 + tid] = kf[f] * s;
+}
+
+__device__ __forceinline__ double dof_value(int32_t id, const double* __restrict__ uf, const double* __restrict__ ud) {
+  return id > 0 ? uf[id - 1] : (id < 0 && ud ? ud[-id - 1] : 0.0);
+}
+
+// y[row] += alpha sum_m2 M[m1][m2] u[(c, m2)] on the free rows (c, m1) of the faces of one group; u from the free
+// values uf and the Dirichlet values ud (null: zero)
+__global__ __launch_bounds__(64) void k_face_apply(OpDev op, const int64_t* __restrict__ face_slot,
+                                                   const int32_t* __restrict__ face_local, const double* __restrict__ M,
+                                                   const int32_t* __restrict__ faces, const double* __restrict__ uf,
+                                                   const double* __restrict__ ud, double* __restrict__ y, double alpha) {
+  const int64_t f = faces[blockIdx.x];
+  const int Q = op.Q, Q2 = Q * Q, NN = Q * Q2;
+  int d, side, e1, e2;
+  face_dirs(face_local[f], d, side, e1, e2);
+  const int tid = threadIdx.x;
+  if (tid >= 3 * Q2) return;
+  const int c = tid / Q2, m1 = tid - c * Q2;
+  const int32_t* ids = op.ids + face_slot[f] * 3 * NN + c * NN;
+  const int32_t row = ids[face_tnode(Q, d, side, e1, e2, m1)];
+  if (row <= 0) return;
+  const double* Mr = M + f * Q2 * Q2 + m1 * Q2;
+  double s = 0.0;
+  for (int m2 = 0; m2 < Q2; ++m2) s += Mr[m2] * dof_value(ids[face_tnode(Q, d, side, e1, e2, m2)], uf, ud);
+  y[row - 1] += alpha * s;
+}
+
+// the diagonal of the node blocks: B[blk][c + 3c] += M[m1][m1]
+__global__ __launch_bounds__(64) void k_face_blocks(OpDev op, const int64_t* __restrict__ face_slot,
+                                                    const int32_t* __restrict__ face_local, const double* __restrict__ M,
+                                                    const int32_t* __restrict__ faces, const int32_t* __restrict__ block_of,
+                                                    double* __restrict__ B) {
+  const int64_t f = faces[blockIdx.x];
+  const int Q = op.Q, Q2 = Q * Q, NN = Q * Q2;
+  int d, side, e1, e2;
+  face_dirs(face_local[f], d, side, e1, e2);
+  const int m1 = threadIdx.x;
+  if (m1 >= Q2) return;
+  const int blk = block_of[face_slot[f] * NN + face_tnode(Q, d, side, e1, e2, m1)];
+  if (blk < 0) return;
+  const double v = M[f * Q2 * Q2 + m1 * Q2 + m1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) B[(size_t)blk * 9 + 4 * c] += v;
+}
+
+// the assembled values: nzval[pos(c m1, c m2)] += M[m1][m2]
+__global__ __launch_bounds__(256) void k_face_assemble(OpDev op, const int64_t* __restrict__ face_slot,
+                                                       const int32_t* __restrict__ face_local, const double* __restrict__ M,
+                                                       const int32_t* __restrict__ faces, double* __restrict__ nzval) {
+  const int64_t f = faces[blockIdx.x];
+  const int Q = op.Q, Q2 = Q * Q, NN = Q * Q2, ND = 3 * NN;
+  int d, side, e1, e2;
+  face_dirs(face_local[f], d, side, e1, e2);
+  const int32_t* pos = op.nzpos + face_slot[f] * ND * ND;
+  for (int it = threadIdx.x; it < 3 * Q2 * Q2; it += 256) {
+    const int c = it / (Q2 * Q2), r = it - c * Q2 * Q2, m1 = r / Q2, m2 = r - m1 * Q2;
+    const int ii = op.loc_of[c * NN + face_tnode(Q, d, side, e1, e2, m1)];
+    const int jj = op.loc_of[c * NN + face_tnode(Q, d, side, e1, e2, m2)];
+    const int p = pos[ii + ND * jj];
+    if (p > 0) nzval[p - 1] += M[f * Q2 * Q2 + r];
+  }
+}
+
+// the energy of the springs: part[f] = 1/2 sum_c sum_m1,m2 u_c,m1 M[m1][m2] u_c,m2 (free and Dirichlet values)
+__global__ __launch_bounds__(kThreads) void k_face_energy(OpDev op, const int64_t* __restrict__ face_slot,
+                                                          const int32_t* __restrict__ face_local,
+                                                          const double* __restrict__ M, const double* __restrict__ x,
+                                                          double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  const int64_t f = blockIdx.x;
+  const int Q = op.Q, Q2 = Q * Q, NN = Q * Q2;
+  int d, side, e1, e2;
+  face_dirs(face_local[f], d, side, e1, e2);
+  const int tid = threadIdx.x;
+  double e = 0.0;
+  if (tid < 3 * Q2) {
+    const int c = tid / Q2, m1 = tid - c * Q2;
+    const int32_t* ids = op.ids + face_slot[f] * 3 * NN + c * NN;
+    const double* Mr = M + f * Q2 * Q2 + m1 * Q2;
+    double s = 0.0;
+    for (int m2 = 0; m2 < Q2; ++m2) s += Mr[m2] * dof_value(ids[face_tnode(Q, d, side, e1, e2, m2)], x, op.dvals);
+    e = 0.5 * dof_value(ids[face_tnode(Q, d, side, e1, e2, m1)], x, op.dvals) * s;
+  }
+  e = block_reduce<0>(e, red);
+  if (tid == 0) part[f] = e;
+}
+
 // Boundary traction (a Neumann load on faces of the cells): one work-group per face, one thread per (component,
 // node of the face); the load of that dof, the sum over the P x P Gauss points of the face of dS t_c N_node, is added
 // into f (free ids at f[id - 1], Dirichlet ids at f[nfree - id - 1]).  The faces of one launch belong to different
@@ -255,47 +409,17 @@ __global__ __launch_bounds__(64) void k_face_load(OpDev op, const int64_t* __res
                                                   int64_t nfree, double* __restrict__ f) {
   const int64_t fi = faces[blockIdx.x];
   const int64_t slot = face_slot[fi];
-  const int lf = face_local[fi];
-  const int d = lf >> 1, side = lf & 1;
-  const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  int d, side, e1, e2;
+  face_dirs(face_local[fi], d, side, e1, e2);
   const int Q = op.Q, P = op.P, NN = Q * Q * Q;
   const int tid = threadIdx.x;
   if (tid >= 3 * Q * Q) return;
   const int c = tid / (Q * Q), m = tid - c * Q * Q, i1 = m % Q, i2 = m / Q;
-  int idx[3];
-  idx[d] = side ? Q - 1 : 0;
-  idx[e1] = i1;
-  idx[e2] = i2;
-  const int t = idx[0] + Q * (idx[1] + Q * idx[2]);
   const double* X = op.coords + slot * 24;
   double acc = 0.0;
   for (int b = 0; b < P; ++b)
-    for (int a = 0; a < P; ++a) {
-      double xi[3];
-      xi[d] = side;
-      xi[e1] = op.tab.xq[a];
-      xi[e2] = op.tab.xq[b];
-      double g1[3] = {0.0, 0.0, 0.0}, g2[3] = {0.0, 0.0, 0.0};
-      for (int v = 0; v < 8; ++v) {
-        double mv[3], dv[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const int bit = (v >> k) & 1;
-          mv[k] = bit ? xi[k] : 1.0 - xi[k];
-          dv[k] = bit ? 1.0 : -1.0;
-        }
-        const double d1 = dv[e1] * mv[e2] * mv[d], d2 = dv[e2] * mv[e1] * mv[d];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          g1[k] += d1 * X[v * 3 + k];
-          g2[k] += d2 * X[v * 3 + k];
-        }
-      }
-      const double n0 = g1[1] * g2[2] - g1[2] * g2[1], n1 = g1[2] * g2[0] - g1[0] * g2[2], n2 = g1[0] * g2[1] - g1[1] * g2[0];
-      const double dS = op.tab.wq[a] * op.tab.wq[b] * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-      acc += dS * op.tab.N[a * Q + i1] * op.tab.N[b * Q + i2];
-    }
-  const int32_t id = op.ids[slot * 3 * NN + c * NN + t];
+    for (int a = 0; a < P; ++a) acc += face_dS(op.tab, X, d, side, e1, e2, a, b) * op.tab.N[a * Q + i1] * op.tab.N[b * Q + i2];
+  const int32_t id = op.ids[slot * 3 * NN + c * NN + face_tnode(Q, d, side, e1, e2, m)];
   const double v = acc * traction[fi * 3 + c];
   if (id > 0) f[id - 1] += v;
   else if (id < 0) f[nfree - id - 1] += v;
@@ -484,6 +608,43 @@ hipError_t launch_face_load(const OpDev& op, const int64_t* face_slot, const int
   if (nfaces <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_face_load, dim3((unsigned)nfaces), dim3(64), 0, s, op, face_slot, face_local, traction, faces,
                      nfree, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_mass(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* kf,
+                            int64_t nfaces, double* M, hipStream_t s) {
+  if (nfaces <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_face_mass, dim3((unsigned)nfaces), dim3(256), 0, s, op, face_slot, face_local, kf, M);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_apply(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                             const int32_t* faces, int64_t n, const double* uf, const double* ud, double* y, double alpha,
+                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_face_apply, dim3((unsigned)n), dim3(64), 0, s, op, face_slot, face_local, M, faces, uf, ud, y,
+                     alpha);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_blocks(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                              const int32_t* faces, int64_t n, const int32_t* block_of, double* B, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_face_blocks, dim3((unsigned)n), dim3(64), 0, s, op, face_slot, face_local, M, faces, block_of, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_assemble(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                                const int32_t* faces, int64_t n, double* nzval, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_face_assemble, dim3((unsigned)n), dim3(256), 0, s, op, face_slot, face_local, M, faces, nzval);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_energy(const OpDev& op, const int64_t* face_slot, const int32_t* face_local, const double* M,
+                              int64_t nfaces, const double* x, double* part, hipStream_t s) {
+  if (nfaces <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_face_energy, dim3((unsigned)nfaces), dim3(kThreads), 0, s, op, face_slot, face_local, M, x, part);
   return hipGetLastError();
 }
 

@@ -87,6 +87,11 @@ struct hfk_op {
   bool has_traction = false;
   double load_factor = 1.0;
   hfk::DevBuf face_load;
+  // Robin (spring) faces: face mass matrices (k included), launched in groups of faces of different cells of one
+  // colour: robin_order[robin_start[g] .. robin_start[g+1])
+  int64_t robin_n = 0;
+  std::vector<int64_t> robin_start;
+  hfk::DevBuf robin_slot, robin_local, robin_order, robin_M;
   // a second copy of the tangents at the same state in another storage, for a smoother of another precision
   hfk::DevBuf tangents_s;
   bool has_mu_field = false;
@@ -149,6 +154,11 @@ int op_prepare_into(hfk_op* op, int storage, void* T, const double* x_dev, hipSt
 // the run-time law's prepare! and residual! (hfk_rtc.hip)
 int rtc_prepare(hfk_op* op, int storage, void* T, const double* x_dev, hipStream_t s);
 int rtc_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s);
+// the Robin term: y += alpha M u on the free rows (u: free values uf, Dirichlet values ud or zero); the node-block
+// diagonals; the assembled values
+int robin_apply(hfk_op* op, const double* uf, const double* ud, double* y, double alpha, hipStream_t s);
+int robin_blocks(hfk_op* op, const int32_t* block_of, double* B, hipStream_t s);
+int robin_assemble(hfk_op* op, double* nzval, hipStream_t s);
 // strain energy per batch with the operator's compiled law into part[nbatches] (HFK_EARG if it has no energy())
 int rtc_energy(hfk_op* op, const double* x_dev, double* part, hipStream_t s);
 }  // namespace hfk

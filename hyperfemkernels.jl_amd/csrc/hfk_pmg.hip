@@ -145,6 +145,7 @@ int blocks_update(hfk_op* op, const void* T, int storage, hipStream_t s, bool* o
   const OpDev d = op->dev();
   for (const BatchRange& r : op->color_batches)
     HIPCHK(op->ctx, launch_blocks(d, storage, T, nb.block_of.as<int32_t>(), nb.B.as<double>(), r, s));
+  if ((rc = robin_blocks(op, nb.block_of.as<int32_t>(), nb.B.as<double>(), s))) return rc;  // the Robin faces
   HIPCHK(op->ctx, hipMemsetAsync(nb.bad.p, 0, sizeof(int), s));
   HIPCHK(op->ctx, blocks_invert(nb.nblocks, nb.dofs.as<int32_t>(), nb.B.as<double>(), nb.Binv.as<double>(), nb.bad.as<int>(), s));
   int bad = 0;

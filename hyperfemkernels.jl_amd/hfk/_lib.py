@@ -38,6 +38,7 @@ SIGNATURES = {
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_traction": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),
     "hfk_op_set_load_factor": (C.c_int, [_vp, C.c_double]),
+    "hfk_op_set_robin": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),
     "hfk_op_traction_load": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_mu_function": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int64]),
     "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),

@@ -142,6 +142,21 @@ class KernelOperator:
         check(self.lib.hfk_op_set_traction(self.handle, fc.size, ptr(fc, C.c_int32), ptr(fl, C.c_int32),
                                            ptr(tr, C.c_double)), self.ctx.handle)
 
+    def set_robin(self, face_cell, face_local, stiffness):
+        """A Robin (spring) term k ∫ u·v dΓ on faces of the cells: in residual!, Jv, the assembled Jacobian, the
+        node blocks and the energy; ``stiffness`` a number or one per face; ``face_cell=None`` removes it.  Prepared
+        tangents are invalidated (prepare again)."""
+        if face_cell is None:
+            check(self.lib.hfk_op_set_robin(self.handle, 0, None, None, None), self.ctx.handle)
+            return
+        fc = np.ascontiguousarray(face_cell, dtype=np.int32)
+        fl = np.ascontiguousarray(face_local, dtype=np.int32)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(stiffness, dtype=np.float64), (fc.size,)))
+        if fl.size != fc.size:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: Robin faces")
+        check(self.lib.hfk_op_set_robin(self.handle, fc.size, ptr(fc, C.c_int32), ptr(fl, C.c_int32),
+                                        ptr(k, C.c_double)), self.ctx.handle)
+
     def set_load_factor(self, load_factor: float):
         """The load factor Λ of the traction (a number of the form: nothing is recomputed)."""
         check(self.lib.hfk_op_set_load_factor(self.handle, float(load_factor)), self.ctx.handle)

@@ -87,6 +87,15 @@ A dead traction on faces of the cells (the Neumann load `-Λ ∫ t⋅v dΓ` of a
 set_traction!(op::DeviceOperator, face_cell::Vector{Int32}, face_local::Vector{Int32}, traction::Matrix{Float64}) =
     check(ccall((:hfk_op_set_traction, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}), op.ptr,
                 length(face_cell), face_cell, face_local, traction), op.ctx.ptr)
+"""
+    set_robin!(op, face_cell, face_local, k)
+
+A Robin (spring) term `k ∫ u⋅v dΓ` on faces of the cells (faces as in `set_traction!`, `k` one stiffness per face):
+in residual!, Jv, the assembled Jacobian, the node blocks of BlockJacobi and the energy; prepare! again afterwards.
+"""
+set_robin!(op::DeviceOperator, face_cell::Vector{Int32}, face_local::Vector{Int32}, k::Vector{Float64}) =
+    check(ccall((:hfk_op_set_robin, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}), op.ptr,
+                length(face_cell), face_cell, face_local, k), op.ctx.ptr)
 "The load factor Λ of the traction (a number of the form, as in `en(Λ)` of test/solvers/test_functionals.jl:40-45)."
 set_load_factor!(op::DeviceOperator, Λ::Real) =
     check(ccall((:hfk_op_set_load_factor, libhfk), Cint, (Ptr{Cvoid}, Cdouble), op.ptr, Λ), op.ctx.ptr)

@@ -83,6 +83,14 @@ int hfk_op_set_parameters(hfk_op* op, double lam, double mu);
  * is computed on the device when the faces are set; nfaces = 0 removes it. */
 int hfk_op_set_traction(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
                         const double* traction);
+/* A Robin (spring) term  k ∫ u·v dΓ  on nfaces faces of the cells (a boundary group whose bilinear form enters the
+ * Jacobian, src/assembly.jl:539-578; the Robin outlet term ZA (u·n)(v·n) of test/common.jl:212-213 is its normal
+ * part): faces as in hfk_op_set_traction, stiffness k per face.  The face mass matrices are computed on the device
+ * when the faces are set; residual! adds k M u (Dirichlet values lifted), Jv, the assembled Jacobian and the node
+ * blocks add k M, and the energy adds k/2 u·M u.  The stored tangents are invalidated (prepare! again); nfaces = 0
+ * removes the term. */
+int hfk_op_set_robin(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                     const double* stiffness);
 /* The load factor Λ of the traction (1 by default; a number of the form, rebound without recomputing f). */
 int hfk_op_set_load_factor(hfk_op* op, double load_factor);
 /* The traction load vector f (unscaled): nfree values on the free ids, then ndir on the Dirichlet ids. */

@@ -614,6 +614,71 @@ int orc_face_load(orc_op* op, int64_t nfaces, const int32_t* face_cell, const in
   return 0;
 }
 
+// A Robin (spring) term k int u.v dGamma on faces of the cells, as a dense matrix over all dofs (free ids first, then
+// the Dirichlet ids: index id - 1, or nfree - id - 1): M[i + ntot j] += k dS N_n1 N_n2 for the same component, over
+// the P x P Gauss points of each face, with the tensor Lagrange functions of every table node evaluated at the point.
+int orc_face_mass(orc_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local, const double* k,
+                  double* M) {
+  const int Q = op->Q, P = op->P, nn = op->nn;
+  const int64_t ntot = op->nfree + op->ndir;
+  std::fill(M, M + ntot * ntot, 0.0);
+  std::vector<double> N[3], dN[3], Nn(nn);
+  for (int kk = 0; kk < 3; ++kk) {
+    N[kk].resize(Q);
+    dN[kk].resize(Q);
+  }
+  auto index = [&](int32_t id) { return id > 0 ? (int64_t)id - 1 : op->nfree - id - 1; };
+  for (int64_t fc = 0; fc < nfaces; ++fc) {
+    const int cell = face_cell[fc], d = face_local[fc] / 2, side = face_local[fc] % 2;
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    for (int b = 0; b < P; ++b)
+      for (int a = 0; a < P; ++a) {
+        double xi[3];
+        xi[d] = side;
+        xi[e1] = op->xq1[a];
+        xi[e2] = op->xq1[b];
+        double Jt[9] = {0};
+        for (int v = 0; v < 8; ++v) {
+          double m[3], dm[3];
+          for (int kk = 0; kk < 3; ++kk) {
+            const int bit = (v >> kk) & 1;
+            m[kk] = bit ? xi[kk] : 1 - xi[kk];
+            dm[kk] = bit ? 1.0 : -1.0;
+          }
+          const double dM[3] = {dm[0] * m[1] * m[2], m[0] * dm[1] * m[2], m[0] * m[1] * dm[2]};
+          const double* x = &op->coords[((size_t)cell * 8 + v) * 3];
+          for (int K = 0; K < 3; ++K)
+            for (int i = 0; i < 3; ++i) Jt[K + 3 * i] += dM[K] * x[i];
+        }
+        double t1[3], t2[3];
+        for (int i = 0; i < 3; ++i) {
+          t1[i] = Jt[e1 + 3 * i];
+          t2[i] = Jt[e2 + 3 * i];
+        }
+        const double nrm[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2], t1[0] * t2[1] - t1[1] * t2[0]};
+        const double w = k[fc] * op->wq1[a] * op->wq1[b] *
+                         std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        for (int kk = 0; kk < 3; ++kk) orc_lagrange1d(Q, xi[kk], N[kk].data(), dN[kk].data());
+        for (int n = 0; n < nn; ++n) {
+          const int t = op->tensor_of[n];
+          Nn[n] = N[0][t % Q] * N[1][(t / Q) % Q] * N[2][t / (Q * Q)];
+        }
+        for (int n1 = 0; n1 < nn; ++n1) {
+          if (Nn[n1] == 0.0) continue;
+          for (int n2 = 0; n2 < nn; ++n2) {
+            if (Nn[n2] == 0.0) continue;
+            for (int c = 0; c < 3; ++c) {
+              const int64_t i = index(op->dofs[(size_t)cell * op->nd + op->ldof[c * nn + n1]]);
+              const int64_t j = index(op->dofs[(size_t)cell * op->nd + op->ldof[c * nn + n2]]);
+              M[i + ntot * j] += w * Nn[n1] * Nn[n2];
+            }
+          }
+        }
+      }
+  }
+  return 0;
+}
+
 // the displacement gradient at every quadrature point, G[(cell np + q) 9 + J + 3c] = d u_c / d x_J (Gridap's
 // grad u[J, c]): the pointwise input of a functional's integrand (functionals.jl:9-21)
 int orc_point_gradients(orc_op* op, const double* x, double* G) {

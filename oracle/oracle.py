@@ -51,6 +51,7 @@ def load():
         "orc_energy": (C.c_double, [C.c_void_p, _f64p]),
         "orc_point_gradients": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_face_load": (C.c_int, [C.c_void_p, C.c_int64, _i32p, _i32p, _f64p, _f64p]),
+        "orc_face_mass": (C.c_int, [C.c_void_p, C.c_int64, _i32p, _i32p, _f64p, _f64p]),
         "orc_residual": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_element_jacobians": (C.c_int, [C.c_void_p, _f64p, _f64p]),
         "orc_pattern_nnz": (C.c_int64, [C.c_void_p]),
@@ -196,6 +197,18 @@ class OracleOp:
         load().orc_face_load(self.h, fc.size, _p(fc, C.c_int32), _p(fl, C.c_int32), _p(tr, C.c_double),
                              _p(f, C.c_double))
         return f[:self.nfree], f[self.nfree:]
+
+    def face_mass(self, face_cell, face_local, stiffness):
+        """The Robin term k int u.v dGamma on faces as a dense matrix over all dofs, (nfree + ndir)^2: free ids first,
+        then the Dirichlet ids."""
+        fc = np.ascontiguousarray(face_cell, dtype=np.int32)
+        fl = np.ascontiguousarray(face_local, dtype=np.int32)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(stiffness, dtype=np.float64), (fc.size,)))
+        n = self.nfree + self.space.ndir
+        M = np.zeros((n, n), order="F")
+        load().orc_face_mass(self.h, fc.size, _p(fc, C.c_int32), _p(fl, C.c_int32), _p(k, C.c_double),
+                             M.ctypes.data_as(C.POINTER(C.c_double)))
+        return np.ascontiguousarray(M)
 
     def point_gradients(self, x):
         """(ncells, np, 3, 3) with [cell, q, c, J] = d u_c / d x_J at every quadrature point (Dirichlet values of

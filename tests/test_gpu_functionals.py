@@ -167,3 +167,59 @@ def test_traction_load(hip_ctx, oracle, order, law, n):
     op.set_load_factor(1.0)
     op.set_traction(None, None, None)
     assert np.array_equal(op.residual(np.zeros(sp.nfree), x), b0) and fun(x) == E0
+
+
+@pytest.mark.parametrize("order,law,n", [(1, "svk", (3, 3, 2)), (2, "neohookean", (3, 2, 2)), (3, "linear", (2, 2, 1))])
+def test_robin_term(hip_ctx, oracle, order, law, n):
+    """A Robin (spring) term k int u.v dGamma on two boundary faces with different stiffnesses, against the oracle's
+    face mass matrix M over all dofs: residual = volume + (M u)_free with the Dirichlet values lifted; assembled
+    Jacobian = volume + M_ff; Jv (full64 and sym64 storages) = volume + M_ff dx; the inverted node blocks = the
+    inverses of the 3 x 3 node blocks of the oracle's Jacobian + M_ff; energy = volume + u.M u / 2 and its central
+    difference = residual . dx; removing the faces gives back the volume terms (all 1e-13 / 1e-12)."""
+    from hfk import MatrixFreeJacobian
+    from hfk.mesh import boundary_faces
+    from hfk.solvers import BlockJacobi
+
+    print(f"robin Q{order} {law}", flush=True)
+    sp, dv, x, op, orc, rng = problem(hip_ctx, oracle, order, law, n, seed=37)
+    fc1, fl1 = boundary_faces(sp, 0, 1)
+    fc2, fl2 = boundary_faces(sp, 2, 0)
+    fc, fl = np.concatenate([fc1, fc2]), np.concatenate([fl1, fl2])
+    k = np.concatenate([np.full(fc1.size, 1.3), np.full(fc2.size, 0.4)])
+    nf = sp.nfree
+    M = orc.face_mass(fc, fl, k)
+    Mff = M[:nf, :nf]
+    u = np.concatenate([x, dv])
+    fun = KernelFunctional(op)
+    dx = rng.standard_normal(nf)
+    b0 = op.residual(np.zeros(nf), x)
+    A0 = op.jacobian(op.allocate_jacobian(), x).toarray()
+    E0 = fun(x)
+    y0 = {st: MatrixFreeJacobian(op, st).prepare(x) @ dx for st in ("full64", "sym64")}
+    op.set_robin(fc, fl, k)
+    b = op.residual(np.zeros(nf), x)
+    assert np.abs(b - (b0 + (M @ u)[:nf])).max() <= 1e-13 * np.abs(b).max()
+    A = op.jacobian(op.allocate_jacobian(), x).toarray()
+    assert np.abs(A - (A0 + Mff)).max() <= 1e-13 * np.abs(A).max()
+    for st in ("full64", "sym64"):
+        mf = MatrixFreeJacobian(op, st).prepare(x)
+        y = mf @ dx
+        assert np.abs(y - (y0[st] + Mff @ dx)).max() <= 1e-13 * np.abs(y).max()
+    mf = MatrixFreeJacobian(op, "full64").prepare(x)
+    bj = BlockJacobi(mf)
+    assert bj.ok
+    dofs, Binv = bj.blocks()
+    Atot = orc.csc(x).toarray() + Mff
+    for blk in range(dofs.shape[0]):
+        ids = dofs[blk]
+        B = np.eye(3)
+        fr = ids > 0
+        B[np.ix_(fr, fr)] = Atot[np.ix_(ids[fr] - 1, ids[fr] - 1)]
+        Bi = np.linalg.inv(B).reshape(-1, order="F")
+        assert np.abs(Binv[blk] - Bi).max() <= 1e-12 * np.abs(Bi).max()
+    E = fun(x)
+    assert E == pytest.approx(E0 + 0.5 * u @ M @ u, rel=1e-12)
+    h = 1e-6
+    assert (fun(x + h * dx) - fun(x - h * dx)) / (2 * h) == pytest.approx(b @ dx, rel=1e-6)
+    op.set_robin(None, None, None)
+    assert np.array_equal(op.residual(np.zeros(nf), x), b0) and fun(x) == E0

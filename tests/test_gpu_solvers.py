@@ -182,6 +182,31 @@ def test_pmultigrid_coarse_every(hip_ctx, oracle):
     assert every1.coarse_builds() == 2 and rel(ac[3], every1.coarse_matvec(xc)) < 1e-14
 
 
+def test_pmultigrid_with_robin_faces(hip_ctx, oracle):
+    """Springs k int u.v dGamma on the face x = 1 of both levels (each level's operator carries its own Robin term,
+    the coarse one in its assembled matrix and node blocks): FCG with the p-multigrid reaches the direct solution of
+    the oracle's Jacobian plus the oracle's face mass matrix."""
+    from hfk.mesh import boundary_faces
+    from hfk.synthetic import smooth_field
+
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    cs = hex_space((4, 4, 4), 1, dirichlet=dirichlet_closed_x0)
+    fc, fl = boundary_faces(fs, 0, 1)
+    fop.set_robin(fc, fl, 5.0)
+    cop.set_robin(*boundary_faces(cs, 0, 1), 5.0)
+    mf.prepare(x)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3).update()
+    assert pmg.ok
+    b = np.random.default_rng(13).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
+    print("pmg fcg with robin:", res, flush=True)
+    assert res.converged and res.iterations <= 30
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    A = orc.csc(x).toarray() + orc.face_mass(fc, fl, 5.0)[:fs.nfree, :fs.nfree]
+    assert rel(xs, np.linalg.solve(A, b)) < 1e-7
+
+
 def test_pmultigrid_q3_lognormal(hip_ctx, oracle):
     """Q3 working level over an assembled Q1 coarse level with the fine 4-point rule, heterogeneous (lognormal)
     shear modulus, FP32 smoothers: the flexible CG reaches the direct solution."""

@@ -3,6 +3,7 @@
 * residual = gradient of the total energy (central differences), with Dirichlet values;
 * point gradients of affine fields = their constant gradient on trilinear cells;
 * boundary traction load: total force t x area and first moment t x area x centroid per component;
+* Robin face mass: symmetric, k x area per component, k int x^2 dGamma for u = x e_c, no component coupling;
 * element Jacobians assembled densely = derivative of the residual (central differences);
 * sum-factorised stored-T product (src/matrixfree.jl:341-424) = dense assembled matrix times dx
   (test/test_matrixfree.jl:74-96), serial = threaded (coloured determinism, test/test_assembly.jl:40-47);
@@ -95,6 +96,32 @@ def test_face_load_closed_forms(oracle, order):
         sel = comp == c
         assert f[sel].sum() == pytest.approx(t1[c] * A1 + t2[c] * A2, rel=1e-13)
         assert np.allclose((f[sel][:, None] * xyz[sel]).sum(axis=0), t1[c] * m1 + t2[c] * m2, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_face_mass_closed_forms(oracle, order):
+    """The Robin term on two box faces with stiffnesses k1, k2: symmetric; each component block sums to
+    k1 A1 + k2 A2 (the face functions sum to one); for u = x e_c (held exactly) u.M u = k1 int x^2 + k2 int x^2 over
+    the faces (x = Lx: Lx^2 A1; z = 0: Lx^3/3 Ly); no coupling between components."""
+    from hfk.mesh import boundary_faces
+
+    sp = hex_space((3, 2, 2), order, domain=(0.0, 2.0, 0.0, 1.5, 0.0, 0.5), perturb=0.15, seed=1,
+                   dirichlet=dirichlet_closed_x0)
+    op = oracle.OracleOp(sp, "svk", 2.0, 1.0, dvals=np.zeros(sp.ndir))
+    fc1, fl1 = boundary_faces(sp, 0, 1)
+    fc2, fl2 = boundary_faces(sp, 2, 0)
+    k1, k2 = 1.3, 0.4
+    M = op.face_mass(np.concatenate([fc1, fc2]), np.concatenate([fl1, fl2]),
+                     np.concatenate([np.full(fc1.size, k1), np.full(fc2.size, k2)]))
+    assert np.abs(M - M.T).max() < 1e-15
+    comp = np.concatenate([np.arange(sp.nfree) % 3, np.arange(sp.ndir) % 3])
+    xyz = np.concatenate([sp.free_xyz(), np.repeat(sp.node_xyz[sp.dir_node_of], 3, axis=0)])
+    for c in range(3):
+        sel = comp == c
+        assert M[np.ix_(sel, sel)].sum() == pytest.approx(k1 * 1.5 * 0.5 + k2 * 2.0 * 1.5, rel=1e-13)
+        assert np.abs(M[np.ix_(sel, ~sel)]).max() == 0.0
+        u = np.where(sel, xyz[:, 0], 0.0)
+        assert u @ M @ u == pytest.approx(k1 * 4.0 * 0.75 + k2 * 8.0 / 3.0 * 1.5, rel=1e-13)
 
 
 @pytest.mark.parametrize("order,law", [(1, "neohookean"), (2, "svk"), (2, "linear")])
