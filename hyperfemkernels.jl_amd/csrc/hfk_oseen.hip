@@ -519,99 +519,120 @@ __device__ __forceinline__ double dof_value(int32_t id, const double* fr, const 
 }
 
 // The point data of a step from the FE functions of the two previous steps (the FE-function leaves of
-// test/common.jl:183-196, which src/fefunctions.jl materialises on the CPU): one thread per quadrature point of a
-// cell, then one per dGb point of an outlet face.  At a cell point: w = aw un + bw unm1, un, f = c1 un - c2 unm1
-// into fields[cell][q][9], and with lap != null the elementwise Laplacian of un into lap[cell][q][3]: constant on
-// an affine cell, sum_a un_a lap(N_a) with lap(N_a) = 4 |grad l_a|^2 at vertex a and 8 grad l_a . grad l_b at the
-// midpoint of edge (a, b), grad l the physical gradients of the barycentric coordinates (J^-T of their reference
-// gradients; J^-T = [e2 x e3, e3 x e1, e1 x e2] / det for the edge vectors e_d = x_d - x_0).  At a face point: w
-// into xb[face][q][4..6].
-__global__ __launch_bounds__(256) void k_oseen_fields(OseenDev op, StepFunctions sf, int64_t nfaces, int nqb,
-                                                      const int32_t* __restrict__ face_cell, double* __restrict__ fields,
-                                                      double* __restrict__ lap, double* __restrict__ xb) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t ncp = op.ncells * op.np;
-  int64_t cell;
-  double xi, et, ze;
-  double* xq = nullptr;
-  if (t < ncp) {
-    cell = t / op.np;
-    const double* p = op.qp + (t - cell * op.np) * 4;
-    xi = p[0];
-    et = p[1];
-    ze = p[2];
-  } else if (t - ncp < nfaces * nqb) {
-    const int64_t f = (t - ncp) / nqb;
-    cell = face_cell[f];
-    xq = xb + (t - ncp) * 7;
-    xi = xq[0];
-    et = xq[1];
-    ze = xq[2];
-  } else {
-    return;
+// test/common.jl:183-196, which src/fefunctions.jl materialises on the CPU).  A work-group takes cpw consecutive cells
+// (one thread per point of them): it stages their 30 nodal velocities of un and unm1, the P2 shape values at the
+// points and, with lap != null, each cell's Laplacian of un in LDS, computes w = aw un + bw unm1, un and
+// f = c1 un - c2 unm1 per point, and writes fields[cell][q][9] and lap[cell][q][3] (the points of its cells are
+// consecutive, so a wave's stores fill one contiguous block).
+// The Laplacian is constant on an affine cell: sum_a un_a lap(N_a) with lap(N_a) = 4 |grad l_a|^2 at vertex a and
+// 8 grad l_a . grad l_b at the midpoint of edge (a, b), grad l the physical gradients of the barycentric coordinates
+// (J^-T of their reference gradients; J^-T = [e2 x e3, e3 x e1, e1 x e2] / det for the edge vectors e_d = x_d - x_0).
+constexpr int kFieldCells = 32;  // cells per work-group of k_oseen_fields at most
+
+__global__ __launch_bounds__(256) void k_oseen_fields(OseenDev op, StepFunctions sf, int cpw, double* __restrict__ fields,
+                                                      double* __restrict__ lap) {
+  __shared__ double ue[kFieldCells][30], ve[kFieldCells][30], Lc[kFieldCells][10], lapc[kFieldCells][3];
+  __shared__ double Nq[kMaxPoints][10];
+  const int tid = threadIdx.x, np = op.np;
+  const int64_t c0 = (int64_t)blockIdx.x * cpw;
+  const int ncl = (int)(op.ncells - c0 < cpw ? op.ncells - c0 : cpw);
+  for (int it = tid; it < ncl * 30; it += 256) {
+    const int cell = it / 30, j = it - cell * 30;
+    const int32_t id = op.ids[(c0 + cell) * kND + j];
+    ue[cell][j] = dof_value(id, sf.un, sf.un_dir);
+    ve[cell][j] = dof_value(id, sf.unm1, sf.unm1_dir);
   }
+  for (int it = tid; it < np * 10; it += 256) {
+    const int q = it / 10, a = it - q * 10;
+    double n, g[3];
+    shape(a, op.qp[q * 4], op.qp[q * 4 + 1], op.qp[q * 4 + 2], n, g);
+    Nq[q][a] = n;
+  }
+  if (lap && tid < ncl) {
+    const double* X = op.coords + (c0 + tid) * 12;
+    double e[3][3];  // e[d] = x_{d+1} - x_0
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) e[d][i] = X[3 * (d + 1) + i] - X[i];
+    double G[4][3];  // physical gradients of the barycentric coordinates
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double* p = e[(d + 1) % 3];
+      const double* q = e[(d + 2) % 3];
+      G[d + 1][0] = p[1] * q[2] - p[2] * q[1];
+      G[d + 1][1] = p[2] * q[0] - p[0] * q[2];
+      G[d + 1][2] = p[0] * q[1] - p[1] * q[0];
+    }
+    const double det = e[0][0] * G[1][0] + e[0][1] * G[1][1] + e[0][2] * G[1][2];
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) G[k][i] /= det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) G[0][i] = -(G[1][i] + G[2][i] + G[3][i]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) Lc[tid][a] = 4.0 * (G[a][0] * G[a][0] + G[a][1] * G[a][1] + G[a][2] * G[a][2]);
+#pragma unroll
+    for (int ed = 0; ed < 6; ++ed) {
+      const int a = ed < 3 ? 0 : (ed < 5 ? 1 : 2);
+      const int b = ed < 3 ? ed + 1 : (ed < 5 ? ed - 1 : 3);
+      Lc[tid][4 + ed] = 8.0 * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
+    }
+  }
+  __syncthreads();
+  if (lap && tid < ncl * 3) {
+    const int cell = tid / 3, c = tid - cell * 3;
+    double s = 0.0;
+    for (int a = 0; a < 10; ++a) s += Lc[cell][a] * ue[cell][c * 10 + a];
+    lapc[cell][c] = s;
+  }
+  __syncthreads();
+  if (tid < ncl * np) {
+    const int cell = tid / np, q = tid - cell * np;
+    double u[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 10; ++a) {
+      const double n = Nq[q][a];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        u[c] += n * ue[cell][c * 10 + a];
+        v[c] += n * ve[cell][c * 10 + a];
+      }
+    }
+    double* fo = fields + (c0 * np + tid) * 9;  // the cells' points are consecutive: one contiguous block
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      fo[c] = sf.aw * u[c] + sf.bw * v[c];
+      fo[3 + c] = u[c];
+      fo[6 + c] = sf.c1 * u[c] - sf.c2 * v[c];
+    }
+    if (lap) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lap[(c0 * np + tid) * 3 + c] = lapc[cell][c];
+    }
+  }
+}
+
+// w = aw un + bw unm1 at the dGb points of the outlet faces, into xb[face][q][4..6]: one thread per point
+__global__ __launch_bounds__(256) void k_oseen_face_w(OseenDev op, StepFunctions sf, int64_t nfaces, int nqb,
+                                                      const int32_t* __restrict__ face_cell, double* __restrict__ xb) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nfaces * nqb) return;
+  const int64_t cell = face_cell[t / nqb];
+  double* xq = xb + t * 7;
   const int32_t* ids = op.ids + cell * kND;
-  double u[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+  double w[3] = {0.0, 0.0, 0.0};
   for (int a = 0; a < 10; ++a) {
     double n, g[3];
-    shape(a, xi, et, ze, n, g);
+    shape(a, xq[0], xq[1], xq[2], n, g);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int32_t id = ids[c * 10 + a];
-      u[c] += n * dof_value(id, sf.un, sf.un_dir);
-      v[c] += n * dof_value(id, sf.unm1, sf.unm1_dir);
+      w[c] += n * (sf.aw * dof_value(id, sf.un, sf.un_dir) + sf.bw * dof_value(id, sf.unm1, sf.unm1_dir));
     }
   }
-  if (xq) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xq[4 + c] = sf.aw * u[c] + sf.bw * v[c];
-    return;
-  }
-  double* fq = fields + t * 9;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    fq[c] = sf.aw * u[c] + sf.bw * v[c];
-    fq[3 + c] = u[c];
-    fq[6 + c] = sf.c1 * u[c] - sf.c2 * v[c];
-  }
-  if (!lap) return;
-  const double* X = op.coords + cell * 12;
-  double e[3][3];  // e[d] = x_{d+1} - x_0
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) e[d][i] = X[3 * (d + 1) + i] - X[i];
-  double G[4][3];  // physical gradients of the barycentric coordinates
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double* p = e[(d + 1) % 3];
-    const double* q = e[(d + 2) % 3];
-    G[d + 1][0] = p[1] * q[2] - p[2] * q[1];
-    G[d + 1][1] = p[2] * q[0] - p[0] * q[2];
-    G[d + 1][2] = p[0] * q[1] - p[1] * q[0];
-  }
-  const double det = e[0][0] * G[1][0] + e[0][1] * G[1][1] + e[0][2] * G[1][2];
-#pragma unroll
-  for (int k = 1; k < 4; ++k)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) G[k][i] /= det;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) G[0][i] = -(G[1][i] + G[2][i] + G[3][i]);
-  double L[10];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) L[a] = 4.0 * (G[a][0] * G[a][0] + G[a][1] * G[a][1] + G[a][2] * G[a][2]);
-#pragma unroll
-  for (int ed = 0; ed < 6; ++ed) {
-    const int a = ed < 3 ? 0 : (ed < 5 ? 1 : 2);
-    const int b = ed < 3 ? ed + 1 : (ed < 5 ? ed - 1 : 3);
-    L[4 + ed] = 8.0 * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double s = 0.0;
-    for (int a = 0; a < 10; ++a) s += L[a] * dof_value(ids[c * 10 + a], sf.un, sf.un_dir);
-    lap[t * 3 + c] = s;
-  }
+  for (int c = 0; c < 3; ++c) xq[4 + c] = w[c];
 }
 
 }  // namespace
@@ -748,11 +769,16 @@ int oseen_step_fields(hfk_oseen* op, const StepFunctions& sf, bool with_lap, hip
   const int64_t ncp = op->ncells * op->np;
   HIPCHK(op->ctx, op->fields.alloc((size_t)ncp * 9 * sizeof(double)));
   if (with_lap) HIPCHK(op->ctx, op->lap.alloc((size_t)ncp * 3 * sizeof(double)));
-  const int64_t total = ncp + op->nfaces * op->nqb;
-  hipLaunchKernelGGL(k_oseen_fields, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, op->dev(), sf, op->nfaces,
-                     op->nqb, op->face_cell.as<int32_t>(), op->fields.as<double>(),
-                     with_lap ? op->lap.as<double>() : nullptr, op->face_xb.as<double>());
+  const int cpw = std::min(256 / op->np, kFieldCells);
+  hipLaunchKernelGGL(k_oseen_fields, dim3((unsigned)((op->ncells + cpw - 1) / cpw)), dim3(256), 0, s, op->dev(), sf, cpw,
+                     op->fields.as<double>(), with_lap ? op->lap.as<double>() : nullptr);
   HIPCHK(op->ctx, hipGetLastError());
+  const int64_t nfp = op->nfaces * op->nqb;
+  if (nfp > 0) {
+    hipLaunchKernelGGL(k_oseen_face_w, dim3((unsigned)((nfp + 255) / 256)), dim3(256), 0, s, op->dev(), sf, op->nfaces,
+                       op->nqb, op->face_cell.as<int32_t>(), op->face_xb.as<double>());
+    HIPCHK(op->ctx, hipGetLastError());
+  }
   op->has_fields = true;
   op->has_lap = with_lap;
   return HFK_OK;

@@ -691,8 +691,18 @@ template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_reduce_partials(const double* __restrict__ part, int64_t n,
                                                               double* __restrict__ out) {
   __shared__ double red[kThreads];
+  constexpr int U = 8;  // loads issued together; each thread still combines part[t], part[t + 256], ... in order
   double v = identity_of<MODE>();
-  for (int64_t i = threadIdx.x; i < n; i += kThreads) v = combine<MODE>(v, part[i]);
+  for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)U * kThreads) {
+    double buf[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * kThreads;
+      buf[u] = i < n ? part[i] : identity_of<MODE>();
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) v = combine<MODE>(v, buf[u]);
+  }
   v = block_reduce<MODE>(v, red);
   if (threadIdx.x == 0) out[0] = v;
 }
