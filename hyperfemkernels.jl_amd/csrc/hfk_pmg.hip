@@ -487,6 +487,7 @@ struct hfk_pmg {
   DevBuf colptr, rowval, nzval;
   int64_t nnz = 0;
   DevBuf cr, cz, cp, cq, ct, cu, cv, cs;  // inner CG work vectors and device scalars
+  DevBuf pcg_part;                        // partial sums of the fused coarse PCG
   DevBuf cdscale;                         // D^-1/2 of the coarse matrix (network scaling)
   // the coarse matrix in node-block ELL form (3x3 blocks of the nodal dofs), filled from the CSC at every setup
   struct BlockEll {
@@ -740,6 +741,25 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
   const double bn = std::sqrt(h[1]);
   HIPCHK(g->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kRZ, s));
   if (bn == 0 || !(std::sqrt(h[2]) > rtol * bn)) return HFK_OK;
+  if (!flexible && g->bell.built) {  // block-Jacobi PCG on the FP64 block-ELL matrix: three launches per iteration
+    const int64_t G = (g->bell.nb + 255) / 256;
+    HIPCHK(g->ctx, g->pcg_part.alloc((size_t)(3 * G) * sizeof(double)));
+    HIPCHK(g->ctx, hipMemcpyAsync(sc + kRZ2 + 1, sc + kRZ, sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int it = 1; it <= maxiter; ++it) {
+      HIPCHK(g->ctx, pcg_fused_iteration(g->bell.nb, g->bell.W, g->bell.nbr.as<int32_t>(), g->bell.v64.as<double>(),
+                                         nb.dofs.as<int32_t>(), nb.Binv.as<double>(), x, r, z, p, q, sc,
+                                         kRZ2 + (it & 1), kRZ2 + ((it + 1) & 1), g->pcg_part.as<double>(), s));
+      g->last_inner = it;
+      if (it % 5 == 0 || it == maxiter) {
+        double rr = 0.0;
+        HIPCHK(g->ctx, hipMemcpyAsync(&rr, sc + kRR, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(g->ctx, hipStreamSynchronize(s));
+        if (!(std::sqrt(rr) > rtol * bn)) break;  // converged (or NaN: leave it to the outer check)
+      }
+    }
+    g->total_inner += g->last_inner;
+    return HFK_OK;
+  }
   for (int it = 1; it <= maxiter; ++it) {
     HIPCHK(g->ctx, coarse_spmv(g, p, q, 1.0, 0.0, s));
     HIPCHK(g->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, g->dots.part.as<double>(), sc + kPQ, s));

@@ -288,6 +288,117 @@ __global__ __launch_bounds__(256) void k_bell_spmv(int64_t nb, int W, const int3
   }
 }
 
+// ---- block-Jacobi PCG on the node-block ELL matrix in three launches per iteration ---------------------------
+// (1) q = A p per block row, with per-work-group partials of p.q;  (2) alpha = rz / sum(p.q partials), x += alpha p,
+// r -= alpha q, z = Binv r per node block, with per-work-group partials of r.z and r.r;  (3) beta = sum(r.z
+// partials) / rz, p = z + beta p, the new rz into the other of two device slots and r.r for the convergence test.
+// One thread per node block, G = ceil(nb / 256) work-groups; every work-group sums the partials of the previous
+// launch itself, in the same fixed order, so alpha and beta agree across work-groups and runs.  Replaces the nine
+// launches (product, two dot products with their final sums, update, block step, direction, scalar copy) of the
+// generic loop.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int G, double* sh) {
+  double v = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) v += part[b];
+  return block_sum(v, sh);
+}
+
+__global__ __launch_bounds__(256) void k_pcg_spmv(int64_t nb, int W, const int32_t* __restrict__ nbr,
+                                                  const double* __restrict__ val, const int32_t* __restrict__ dofs,
+                                                  const double* __restrict__ p, double* __restrict__ q,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double pq = 0.0;
+  if (i < nb) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = 0; k < W; ++k) {
+      const int64_t j = nbr[(size_t)k * nb + i];
+      const int d0 = dofs[3 * j], d1 = dofs[3 * j + 1], d2 = dofs[3 * j + 2];
+      const double x0 = d0 > 0 ? p[d0 - 1] : 0.0, x1 = d1 > 0 ? p[d1 - 1] : 0.0, x2 = d2 > 0 ? p[d2 - 1] : 0.0;
+      const double* v = val + (size_t)9 * k * nb + i;
+      s0 += v[0] * x0 + v[3 * nb] * x1 + v[6 * nb] * x2;
+      s1 += v[nb] * x0 + v[4 * nb] * x1 + v[7 * nb] * x2;
+      s2 += v[2 * nb] * x0 + v[5 * nb] * x1 + v[8 * nb] * x2;
+    }
+    const int o[3] = {dofs[3 * i], dofs[3 * i + 1], dofs[3 * i + 2]};
+    const double sv[3] = {s0, s1, s2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (o[c] > 0) {
+        q[o[c] - 1] = sv[c];
+        pq += p[o[c] - 1] * sv[c];
+      }
+  }
+  pq = block_sum(pq, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = pq;
+}
+
+__global__ __launch_bounds__(256) void k_pcg_update(int64_t nb, int G, const int32_t* __restrict__ dofs,
+                                                    const double* __restrict__ Binv, const double* __restrict__ p,
+                                                    const double* __restrict__ q, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ z,
+                                                    const double* __restrict__ sc, int slot, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const double alpha = sc[slot] / sum_partials(part, G, sh);
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (i < nb) {
+    int id[3];
+    double rv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      id[j] = dofs[3 * i + j];
+      rv[j] = 0.0;
+      if (id[j] > 0) {
+        x[id[j] - 1] += alpha * p[id[j] - 1];
+        rv[j] = r[id[j] - 1] - alpha * q[id[j] - 1];
+        r[id[j] - 1] = rv[j];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (id[a] <= 0) continue;
+      double zz = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (id[j] > 0) zz += Binv[(size_t)(a + 3 * j) * nb + i] * rv[j];
+      z[id[a] - 1] = zz;
+      rz += rv[a] * zz;
+      rr += rv[a] * rv[a];
+    }
+  }
+  rz = block_sum(rz, sh);
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    part[G + blockIdx.x] = rz;
+    part[2 * G + blockIdx.x] = rr;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pcg_direction(int64_t nb, int G, const int32_t* __restrict__ dofs,
+                                                       const double* __restrict__ z, double* __restrict__ p,
+                                                       double* __restrict__ sc, int slot, int next,
+                                                       const double* __restrict__ part) {
+  __shared__ double sh[256];
+  const double rzn = sum_partials(part + G, G, sh);
+  const double beta = rzn / sc[slot];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nb) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int id = dofs[3 * i + j];
+      if (id > 0) p[id - 1] = z[id - 1] + beta * p[id - 1];
+    }
+  }
+  if (blockIdx.x == 0) {  // uniform over the work-group
+    const double rr = sum_partials(part + 2 * G, G, sh);
+    if (threadIdx.x == 0) {
+      sc[next] = rzn;
+      sc[kRR] = rr;
+    }
+  }
+}
+
 // the block-ELL values from the CSC values: v64[pos[p]] = val[p] (and the FP32 copy when v32 is given)
 __global__ void k_bell_fill(int64_t nnz, const int32_t* __restrict__ pos, const double* __restrict__ val,
                             double* __restrict__ v64, float* __restrict__ v32) {
@@ -453,6 +564,15 @@ hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, boo
   else
     hipLaunchKernelGGL(k_bell_spmv<double>, grid_for(nb), dim3(256), 0, s, nb, W, nbr, static_cast<const double*>(val),
                        dofs, x, y, alpha, beta);
+  return hipGetLastError();
+}
+hipError_t pcg_fused_iteration(int64_t nb, int W, const int32_t* nbr, const double* val, const int32_t* dofs,
+                               const double* Binv, double* x, double* r, double* z, double* p, double* q, double* sc,
+                               int slot, int next, double* part, hipStream_t s) {
+  const int G = (int)((nb + 255) / 256);
+  hipLaunchKernelGGL(k_pcg_spmv, dim3((unsigned)G), dim3(256), 0, s, nb, W, nbr, val, dofs, p, q, part);
+  hipLaunchKernelGGL(k_pcg_update, dim3((unsigned)G), dim3(256), 0, s, nb, G, dofs, Binv, p, q, x, r, z, sc, slot, part);
+  hipLaunchKernelGGL(k_pcg_direction, dim3((unsigned)G), dim3(256), 0, s, nb, G, dofs, z, p, sc, slot, next, part);
   return hipGetLastError();
 }
 hipError_t bell_fill(int64_t nnz, const int32_t* pos, const double* val, double* v64, float* v32, hipStream_t s) {

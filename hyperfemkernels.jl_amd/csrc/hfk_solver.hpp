@@ -8,7 +8,7 @@
 namespace hfk {
 
 // device scalars of a CG/FCG iteration
-enum : int { kRZ = 0, kPQ = 1, kZQ = 2, kRZN = 3, kRR = 4, kBB = 5, kNS = 8 };
+enum : int { kRZ = 0, kPQ = 1, kZQ = 2, kRZN = 3, kRR = 4, kBB = 5, kRZ2 = 6, kNS = 8 };  // kRZ2, kRZ2 + 1: fused PCG
 
 // out[0..] = a0.b0 (, a1.b1 (, a2.b2)): deterministic two-pass reductions on a fixed grid; `part` holds
 // dot_partial_doubles() doubles
@@ -25,6 +25,11 @@ hipError_t multi_dot(int64_t n, const double* V, int64_t ldv, int K, const doubl
 hipError_t multi_axpy(int64_t n, const double* V, int64_t ldv, int K, const double* coef, double alpha, double* w,
                       hipStream_t s);
 // node-block ELL product y = alpha A x + beta y (val FP64, or FP32 when fp32) and its value fill from the CSC
+// one block-Jacobi PCG iteration on the FP64 node-block ELL matrix in three launches: rz in sc[slot] (the new rz goes
+// to sc[next], r.r to sc[kRR]); part holds 3 ceil(nb / 256) partial sums
+hipError_t pcg_fused_iteration(int64_t nb, int W, const int32_t* nbr, const double* val, const int32_t* dofs,
+                               const double* Binv, double* x, double* r, double* z, double* p, double* q, double* sc,
+                               int slot, int next, double* part, hipStream_t s);
 hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,
                      const double* x, double* y, double alpha, double beta, hipStream_t s);
 hipError_t bell_fill(int64_t nnz, const int32_t* pos, const double* val, double* v64, float* v32, hipStream_t s);
