@@ -196,3 +196,30 @@ def test_newton_quickstart_matches_oracle(oracle, hip_ctx):
         if np.linalg.norm(bo) <= 1e-10 * r0:
             break
     assert np.linalg.norm(b) <= 1e-10 * r0 and k <= 8
+
+
+@pytest.mark.parametrize("order,P,n", [(2, 3, (1, 1, 1)), (2, 3, (5, 1, 1)), (2, 3, (3, 3, 1)), (3, 4, (1, 1, 2)),
+                                       (1, 2, (1, 1, 1))])
+def test_small_and_ragged_meshes(oracle, hip_ctx, order, P, n):
+    """A single cell, ragged meshes (the last batch partly padding) and a mesh that fills its batch exactly: tangents,
+    Jv in every storage, residual, assembled values and the energy equal the oracle's."""
+    from hfk.functionals import KernelFunctional
+
+    print(f"small mesh order={order} n={n}", flush=True)
+    sp, dv, x = problem(order, P, "neohookean", n, perturb=0.0)
+    form = HyperelasticForm("neohookean", 2.0, 1.0, degree=2 * (P - 1))
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, "neohookean", 2.0, 1.0, points1d=P, dvals=dv)
+    prep = orc.prepare(x)
+    dx = np.random.default_rng(9).standard_normal(sp.nfree)
+    yref = orc.jv(prep, dx)
+    for st, tol in (("full64", TOL), ("sym64", TOL), ("full32", 1e-5), ("sym32", 1e-5)):
+        assert rel(MatrixFreeJacobian(op, st).prepare(x) @ dx, yref) < tol
+    assert rel(MatrixFreeJacobian(op, "full64").prepare(x).tangents(), oracle.reference_coordinate_tangents(*prep)) < TOL
+    assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < TOL
+    colptr, rowval, nzpos = op.pattern()
+    c2, r2, p2 = orc.pattern()
+    assert np.array_equal(colptr, c2) and np.array_equal(rowval, r2) and np.array_equal(nzpos, p2)
+    A = op.jacobian(op.allocate_jacobian(), x)
+    assert rel(A.data, orc.jacobian(x, p2, r2.size)) < TOL
+    assert KernelFunctional(op)(x) == pytest.approx(orc.energy(x), rel=1e-12)
