@@ -223,3 +223,19 @@ def test_small_and_ragged_meshes(oracle, hip_ctx, order, P, n):
     A = op.jacobian(op.allocate_jacobian(), x)
     assert rel(A.data, orc.jacobian(x, p2, r2.size)) < TOL
     assert KernelFunctional(op)(x) == pytest.approx(orc.energy(x), rel=1e-12)
+
+
+def test_check_against_reference(oracle, hip_ctx):
+    """check (src/verify.jl:9-17) with the oracle as the reference assembly: both differences below 1e-12, for the
+    assembled and the matrix-free kinds; a wrong reference (scaled residual, shifted matrix) is reported."""
+    from hfk.verify import check
+
+    sp, dv, x = problem(2, 3, "svk", (3, 2, 2), 0.1)
+    orc = oracle.OracleOp(sp, "svk", 2.0, 1.0, points1d=3, dvals=dv)
+    for kind in ("assembled", "matrixfree"):
+        op = KernelOperator(HyperelasticForm("svk", 2.0, 1.0, degree=4), sp, jacobian=kind, dirichlet_values=dv,
+                            ctx=hip_ctx)
+        r = check(op, orc.residual, orc.csc, x)
+        assert r.res < TOL and r.jac < TOL
+    bad = check(op, lambda v: 1.1 * orc.residual(v), lambda v: orc.csc(v) * 0.9, x)
+    assert bad.res == pytest.approx(0.1 / 1.1, rel=1e-6) and bad.jac == pytest.approx(0.1 / 0.9, rel=1e-6)
