@@ -65,6 +65,15 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
   const int batch = ATOMIC ? interleaved_batch(colfc, ncol, gx) : batch0 + blockIdx.x;
   if (ATOMIC && batch < 0) return;  // uniform over the work-group
   const int32_t* ids = op.ids + (size_t)batch * C * 3 * Dm::NN;
+  const TS* Dp = static_cast<const TS*>(tangents) + (size_t)batch * NE * NSLOT + tid;
+  // FP32 packed storage: the point's 45 entries are loaded before the gather and the interpolation sweeps (45
+  // registers), so that their latency overlaps those phases instead of following their barriers
+  constexpr bool PRE = SYM && std::is_same<TS, float>::value;
+  CT dpre[PRE ? 45 : 1];
+  if (PRE && tid < NSLOT) {
+#pragma unroll
+    for (int e = 0; e < (PRE ? 45 : 1); ++e) dpre[e] = static_cast<CT>(ld_stream(Dp + (size_t)e * NSLOT));
+  }
   for (int it = tid; it < C * 3 * Dm::NN; it += kThreads) {
     const int id = ids[it];
     const int cell = it / (3 * Dm::NN);
@@ -81,8 +90,16 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
       su[j] = s[j];
       sv[j] = 0.0;
     }
-    const TS* Dp = static_cast<const TS*>(tangents) + (size_t)batch * NE * NSLOT + tid;
-    if (!SYM) {
+    if (PRE) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j)
+#pragma unroll
+        for (int i = 0; i <= j; ++i) {
+          const CT d = dpre[PRE ? sym_index(i, j) : 0];
+          sv[i] += d * su[j];
+          if (i != j) sv[j] += d * su[i];
+        }
+    } else if (!SYM) {
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
