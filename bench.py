@@ -294,7 +294,7 @@ def newton_gmres_leg(rtol=1e-10, maxit=20, lin_rtol=1e-12):
 
 
 def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8,
-              learned=None, order=2, mu=None, state_amp=1e-2):
+              learned=None, order=2, mu=None, state_amp=1e-2, coarse_precision="fp64"):
     """CG-type solve J(x) du = -R(x) to ||b - J du|| <= rtol ||b|| with one p-multigrid V-cycle per iteration:
     Q2 -> assembled Q1 (fine quadrature), Chebyshev degree `degree` (range 20, inflate 1.1, 10 Arnoldi steps),
     coarse block-Jacobi PCG to coarse_rtol; flexible CG outside (the inexact coarse solve makes the cycle vary).
@@ -332,7 +332,7 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     b = -fop.residual(np.zeros(fs.nfree), x)
     mf = MatrixFreeJacobian(fop, outer_storage)
     pmg = PMultigrid([fop, cop], Chebyshev(degree=degree), coarse_rtol=coarse_rtol, coarse_maxiter=coarse_maxiter,
-                     smoother_storage=smoother_storage)
+                     smoother_storage=smoother_storage, coarse_precision=coarse_precision)
     dev = torch.device("cuda:0")
     xd = torch.from_numpy(x).to(dev)
     bd = torch.from_numpy(b).to(dev)
@@ -873,6 +873,8 @@ def main():
             res["solve_fp64_sym64_pmg"] = solve_leg(n, "sym64", None, coarse_rtol=0.3)
         if "mixed" in legs:
             res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3)
+            # the same with the coarse PCG's products on FP32 copies of the coarse values (FP64 vectors and sums)
+            res["solve_mixed_pmg_fp32_coarse"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3, coarse_precision="fp32")
         if "learned" in legs:
             res["solve_mixed_learned"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.1,
                                                    learned={"samples": 160, "steps": 500, "smoother_degree": 3,

@@ -502,6 +502,8 @@ struct hfk_pmg {
   int ccheb_degree = 0;  // its degree (0: the matrix-free levels' degree)
   Dots dots;
   bool ok = false, indefinite = false, built = false, coarse_ok = false;
+  // the coarse PCG's products on FP32 copies of the coarse values (FP64 vectors and accumulation)
+  bool coarse_fp32 = false;
   // coarse re-assembly every coarse_every updates (PLevels coarse_every, src/pmultigrid.jl:200-216)
   int coarse_every = 1, since = 0;
   bool coarse_built = false;
@@ -576,7 +578,7 @@ int bell_build(hfk_pmg* g) {
   if (!pos.empty()) HIPCHK(g->ctx, hipMemcpy(B.pos.p, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   const size_t nv = (size_t)9 * W * nb;
   HIPCHK(g->ctx, B.v64.alloc(nv * sizeof(double)));
-  B.has32 = g->smoother_storage == HFK_STORE_FULL32 || g->smoother_storage == HFK_STORE_SYM32;
+  B.has32 = g->smoother_storage == HFK_STORE_FULL32 || g->smoother_storage == HFK_STORE_SYM32 || g->coarse_fp32;
   if (B.has32) HIPCHK(g->ctx, B.v32.alloc(nv * sizeof(float)));
   B.nb = nb;
   B.W = W;
@@ -746,8 +748,9 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
     HIPCHK(g->ctx, g->pcg_part.alloc((size_t)(3 * G) * sizeof(double)));
     HIPCHK(g->ctx, hipMemcpyAsync(sc + kRZ2 + 1, sc + kRZ, sizeof(double), hipMemcpyDeviceToDevice, s));
     for (int it = 1; it <= maxiter; ++it) {
-      HIPCHK(g->ctx, pcg_fused_iteration(g->bell.nb, g->bell.W, g->bell.nbr.as<int32_t>(), g->bell.v64.as<double>(),
-                                         nb.dofs.as<int32_t>(), nb.Binv.as<double>(), x, r, z, p, q, sc,
+      const bool f32 = g->coarse_fp32 && g->bell.has32;
+      HIPCHK(g->ctx, pcg_fused_iteration(g->bell.nb, g->bell.W, g->bell.nbr.as<int32_t>(),
+                                         f32 ? g->bell.v32.p : g->bell.v64.p, f32, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), x, r, z, p, q, sc,
                                          kRZ2 + (it & 1), kRZ2 + ((it + 1) & 1), g->pcg_part.as<double>(), s));
       g->last_inner = it;
       if (it % 5 == 0 || it == maxiter) {
@@ -1341,6 +1344,15 @@ int hfk_pmg_set_coarse_every(hfk_pmg* g, int coarse_every) {
   if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
   if (coarse_every < 1) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: coarse_every must be >= 1");
   g->coarse_every = coarse_every;
+  return HFK_OK;
+}
+
+int hfk_pmg_set_coarse_precision(hfk_pmg* g, int bits) {
+  if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  if (bits != 32 && bits != 64) return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: coarse precision must be 32 or 64");
+  if (g->bell.built && bits == 32 && !g->bell.has32)
+    return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: FP32 coarse values must be chosen before the first update");
+  g->coarse_fp32 = bits == 32;
   return HFK_OK;
 }
 

@@ -302,8 +302,9 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
   return block_sum(v, sh);
 }
 
+template <class TV>
 __global__ __launch_bounds__(256) void k_pcg_spmv(int64_t nb, int W, const int32_t* __restrict__ nbr,
-                                                  const double* __restrict__ val, const int32_t* __restrict__ dofs,
+                                                  const TV* __restrict__ val, const int32_t* __restrict__ dofs,
                                                   const double* __restrict__ p, double* __restrict__ q,
                                                   double* __restrict__ part) {
   __shared__ double sh[256];
@@ -315,10 +316,10 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(int64_t nb, int W, const int32
       const int64_t j = nbr[(size_t)k * nb + i];
       const int d0 = dofs[3 * j], d1 = dofs[3 * j + 1], d2 = dofs[3 * j + 2];
       const double x0 = d0 > 0 ? p[d0 - 1] : 0.0, x1 = d1 > 0 ? p[d1 - 1] : 0.0, x2 = d2 > 0 ? p[d2 - 1] : 0.0;
-      const double* v = val + (size_t)9 * k * nb + i;
-      s0 += v[0] * x0 + v[3 * nb] * x1 + v[6 * nb] * x2;
-      s1 += v[nb] * x0 + v[4 * nb] * x1 + v[7 * nb] * x2;
-      s2 += v[2 * nb] * x0 + v[5 * nb] * x1 + v[8 * nb] * x2;
+      const TV* v = val + (size_t)9 * k * nb + i;
+      s0 += (double)v[0] * x0 + (double)v[3 * nb] * x1 + (double)v[6 * nb] * x2;
+      s1 += (double)v[nb] * x0 + (double)v[4 * nb] * x1 + (double)v[7 * nb] * x2;
+      s2 += (double)v[2 * nb] * x0 + (double)v[5 * nb] * x1 + (double)v[8 * nb] * x2;
     }
     const int o[3] = {dofs[3 * i], dofs[3 * i + 1], dofs[3 * i + 2]};
     const double sv[3] = {s0, s1, s2};
@@ -566,11 +567,16 @@ hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, boo
                        dofs, x, y, alpha, beta);
   return hipGetLastError();
 }
-hipError_t pcg_fused_iteration(int64_t nb, int W, const int32_t* nbr, const double* val, const int32_t* dofs,
+hipError_t pcg_fused_iteration(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,
                                const double* Binv, double* x, double* r, double* z, double* p, double* q, double* sc,
                                int slot, int next, double* part, hipStream_t s) {
   const int G = (int)((nb + 255) / 256);
-  hipLaunchKernelGGL(k_pcg_spmv, dim3((unsigned)G), dim3(256), 0, s, nb, W, nbr, val, dofs, p, q, part);
+  if (fp32)
+    hipLaunchKernelGGL(k_pcg_spmv<float>, dim3((unsigned)G), dim3(256), 0, s, nb, W, nbr, static_cast<const float*>(val),
+                       dofs, p, q, part);
+  else
+    hipLaunchKernelGGL(k_pcg_spmv<double>, dim3((unsigned)G), dim3(256), 0, s, nb, W, nbr, static_cast<const double*>(val),
+                       dofs, p, q, part);
   hipLaunchKernelGGL(k_pcg_update, dim3((unsigned)G), dim3(256), 0, s, nb, G, dofs, Binv, p, q, x, r, z, sc, slot, part);
   hipLaunchKernelGGL(k_pcg_direction, dim3((unsigned)G), dim3(256), 0, s, nb, G, dofs, z, p, sc, slot, next, part);
   return hipGetLastError();

@@ -25,9 +25,10 @@ hipError_t multi_dot(int64_t n, const double* V, int64_t ldv, int K, const doubl
 hipError_t multi_axpy(int64_t n, const double* V, int64_t ldv, int K, const double* coef, double alpha, double* w,
                       hipStream_t s);
 // node-block ELL product y = alpha A x + beta y (val FP64, or FP32 when fp32) and its value fill from the CSC
-// one block-Jacobi PCG iteration on the FP64 node-block ELL matrix in three launches: rz in sc[slot] (the new rz goes
-// to sc[next], r.r to sc[kRR]); part holds 3 ceil(nb / 256) partial sums
-hipError_t pcg_fused_iteration(int64_t nb, int W, const int32_t* nbr, const double* val, const int32_t* dofs,
+// one block-Jacobi PCG iteration on the node-block ELL matrix (FP64 values, or FP32 values with FP64 accumulation) in
+// three launches: rz in sc[slot] (the new rz goes to sc[next], r.r to sc[kRR]); part holds 3 ceil(nb / 256) partial
+// sums
+hipError_t pcg_fused_iteration(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,
                                const double* Binv, double* x, double* r, double* z, double* p, double* q, double* sc,
                                int slot, int next, double* part, hipStream_t s);
 hipError_t bell_spmv(int64_t nb, int W, const int32_t* nbr, const void* val, bool fp32, const int32_t* dofs,

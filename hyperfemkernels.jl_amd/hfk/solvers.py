@@ -166,7 +166,8 @@ class PMultigrid:
     finest first (the first is the Newton's, matrix-free; the last assembled)."""
 
     def __init__(self, levels, smoother: Chebyshev = Chebyshev(), coarse_rtol: float = 1e-2,
-                 coarse_maxiter: int = 500, smoother_storage: str | None = None, coarse_every: int = 1):
+                 coarse_maxiter: int = 500, smoother_storage: str | None = None, coarse_every: int = 1,
+                 coarse_precision: str = "fp64"):
         if len(levels) < 2:
             raise ValueError("HyperFEMKernels: PMultigrid needs at least two levels")
         if coarse_every < 1:
@@ -184,6 +185,10 @@ class PMultigrid:
         self.coarse_every = int(coarse_every)
         if coarse_every != 1:
             check(lib.hfk_pmg_set_coarse_every(h, int(coarse_every)), levels[0].ctx.handle)
+        if coarse_precision not in ("fp64", "fp32"):
+            raise ValueError("HyperFEMKernels: PMultigrid: coarse_precision must be fp64 or fp32")
+        if coarse_precision == "fp32":  # the coarse PCG's products on FP32 values, FP64 vectors and accumulation
+            check(lib.hfk_pmg_set_coarse_precision(h, 32), levels[0].ctx.handle)
 
     def coarse_builds(self) -> int:
         """Coarse set-ups so far (the reference's ``nbuild``)."""

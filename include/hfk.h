@@ -239,6 +239,10 @@ int hfk_pmg_update(hfk_pmg* g, int* status);
 int hfk_pmg_set_coarse_every(hfk_pmg* g, int coarse_every);
 /* number of coarse set-ups so far (the reference's nbuild) */
 int hfk_pmg_coarse_builds(hfk_pmg* g, int64_t* builds);
+/* The products of the coarse block-Jacobi PCG on FP32 copies of the coarse matrix values (bits = 32; vectors,
+ * dot products and accumulation stay FP64) or on the FP64 values (64, the default).  No counterpart in the reference
+ * (its coarse solver is LU or AMG); set before the first update.  The network cycle's coarse loop keeps FP64. */
+int hfk_pmg_set_coarse_precision(hfk_pmg* g, int bits);
 /* ldiv!: one V-cycle z = M^-1 r (NaN-filled when not ok) */
 int hfk_pmg_vcycle(hfk_pmg* g, const double* r, double* z);
 int hfk_pmg_vcycle_dev(hfk_pmg* g, const double* r_dev, double* z_dev, void* stream);

@@ -207,6 +207,25 @@ def test_pmultigrid_with_robin_faces(hip_ctx, oracle):
     assert rel(xs, np.linalg.solve(A, b)) < 1e-7
 
 
+def test_pmultigrid_fp32_coarse_values(hip_ctx, oracle):
+    """The coarse PCG's products on FP32 copies of the coarse values (FP64 vectors and sums): the flexible outer
+    solve still reaches the direct solution of the FP64 system (the coarse level only preconditions)."""
+    from hfk.synthetic import smooth_field
+
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-3, coarse_precision="fp32").update()
+    assert pmg.ok
+    b = np.random.default_rng(11).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=100, preconditioner=pmg).solve(xs, mf, b)
+    print("pmg fcg, fp32 coarse values:", res, pmg.info(), flush=True)
+    assert res.converged and res.iterations <= 30
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    assert rel(xs, spla.spsolve(orc.csc(x).tocsc(), b)) < 1e-7
+    with pytest.raises(ValueError):
+        PMultigrid([fop, cop], coarse_precision="bf16")
+
+
 def test_pmultigrid_q3_lognormal(hip_ctx, oracle):
     """Q3 working level over an assembled Q1 coarse level with the fine 4-point rule, heterogeneous (lognormal)
     shear modulus, FP32 smoothers: the flexible CG reaches the direct solution."""
