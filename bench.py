@@ -383,10 +383,13 @@ def lognormal_points(mu, xyz):
     wt = torch.from_numpy(w.T.copy()).to(dev)
     bt = torch.from_numpy(b).to(dev)
     out = np.empty(xyz.shape[0])
-    for s in range(0, xyz.shape[0], 1 << 18):
-        xt = torch.from_numpy(np.ascontiguousarray(xyz[s:s + (1 << 18)])).to(dev)
-        g = np.sqrt(2.0 / w.shape[0]) * torch.cos(xt @ wt + bt).sum(dim=1)
-        out[s:s + (1 << 18)] = torch.exp(mu["sigma"] * g).cpu().numpy()
+    step = 1 << 15
+    for s in range(0, xyz.shape[0], step):
+        xt = torch.from_numpy(np.ascontiguousarray(xyz[s:s + step])).to(dev)
+        # elementwise products rather than a GEMM, whose kernel choice (and rounding) can change between processes
+        ph = xt[:, 0:1] * wt[0:1, :] + xt[:, 1:2] * wt[1:2, :] + xt[:, 2:3] * wt[2:3, :] + bt
+        g = np.sqrt(2.0 / w.shape[0]) * torch.cos(ph).sum(dim=1)
+        out[s:s + step] = torch.exp(mu["sigma"] * g).cpu().numpy()
     return out
 
 

@@ -117,9 +117,11 @@ __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const uns
 
 // dW[tap][co][ci] += sum_v G[v][co] X[v + off(tap)][ci]: per (tap, co tile, ci tile) a GEMM over the voxels
 // (K = voxels).  A (M = co, K = v): lane m = l % 16 -> co, k = 4 (l / 16) + i -> voxel; B (K = v, N = ci).
-// Each wave reduces kVoxWave voxels and adds its 16 x 16 tile to dW with FP32 atomics.
+// Each wave reduces a chunk of kVoxWave voxels and writes its 16 x 16 tile to P[tile][chunk][256];
+// k_wgrad_reduce then adds the chunks of every weight in chunk order, so the gradient does not depend on the order
+// in which the waves finish (training is reproducible).
 constexpr int kVoxWave = 512;
-__global__ __launch_bounds__(256) void k_conv_wgrad(int N, View G, View X1, View X2, float* __restrict__ dW,
+__global__ __launch_bounds__(256) void k_conv_wgrad(int N, View G, View X1, View X2, float* __restrict__ P,
                                                     int ntco, int ntci) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nvox = N * N * N;
@@ -150,11 +152,24 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(int N, View G, View X1, View
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
   // D rows 4 (l / 16) + i -> co, column l % 16 -> ci
+  const int nchunks = (nvox + kVoxWave - 1) / kVoxWave;
+  float* pt = P + ((size_t)tile * nchunks + (blockIdx.x * 4 + wave)) * 256;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int cco = tco * 16 + 4 * kg + i;
-    atomicAdd(dW + ((size_t)tap * (ntco * 16) + cco) * Cin + tci * 16 + m, acc[i]);
-  }
+  for (int i = 0; i < 4; ++i) pt[(4 * kg + i) * 16 + m] = acc[i];
+}
+
+// dW[tap][co][ci] += sum over the chunks, in order, of P[tile][chunk][row * 16 + col]; one thread per weight
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ P, int nchunks, int ntiles, int ntco,
+                                                      int ntci, int Cin, float* __restrict__ dW) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= ntiles * 256) return;
+  const int tile = g >> 8, e = g & 255;
+  const int tap = tile / (ntco * ntci), r = tile % (ntco * ntci), tco = r / ntci, tci = r % ntci;
+  const float* pt = P + (size_t)tile * nchunks * 256 + e;
+  float sum = 0.f;
+  for (int c = 0; c < nchunks; ++c) sum += pt[(size_t)c * 256];
+  const int row = e >> 4, col = e & 15;
+  dW[((size_t)tap * (ntco * 16) + tco * 16 + row) * Cin + tci * 16 + col] += sum;
 }
 
 // Wt[tap][ci][co] = W[26 - tap][co][ci] (the adjoint of the zero-padded correlation)
@@ -438,6 +453,7 @@ struct hfk_net {
   hfk::DevBuf x0, a0, s0, r1, a2, s1, r2, s2, r3, a5, a6, u2, a7, u1, a8, u0, a9, out;
   hfk::DevBuf g_out, g_a9, g_cat0, g_a8tmp, g_cat1, g_a7tmp, g_cat2, g_a6tmp, g_a5, g_r3, g_r2, g_a2, g_r1, g_a0, gbf;
   hfk::DevBuf scal, part, dots;
+  hfk::DevBuf wpart;  // per-chunk weight-gradient tiles of one layer (k_conv_wgrad -> k_wgrad_reduce)
   hfk::DevBuf aux;  // auxiliary input channel per vertex (FP32)
   bool has_aux = false;
   int step = 0;
@@ -512,8 +528,12 @@ int conv_backward(hfk_net* nt, int li, View dY, View Yact, View X1, View X2, Vie
   View G{nt->gbf.p, L.cout, L.cout};
   hipLaunchKernelGGL(k_relu_back, grid1(nv * L.cout), dim3(256), 0, s, nv, dY, Yact, G, L.relu ? 1 : 0);
   const int ntco = L.cout / 16, ntci = cin / 16;
-  const dim3 gw((unsigned)((nv + 4 * kVoxWave - 1) / (4 * kVoxWave)), (unsigned)(27 * ntco * ntci));
-  hipLaunchKernelGGL(k_conv_wgrad, gw, dim3(256), 0, s, N, G, X1, X2, nt->grad.as<float>() + L.woff, ntco, ntci);
+  const int ntiles = 27 * ntco * ntci;
+  const int nchunks = (int)((nv + kVoxWave - 1) / kVoxWave);
+  const dim3 gw((unsigned)((nv + 4 * kVoxWave - 1) / (4 * kVoxWave)), (unsigned)ntiles);
+  hipLaunchKernelGGL(k_conv_wgrad, gw, dim3(256), 0, s, N, G, X1, X2, nt->wpart.as<float>(), ntco, ntci);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)ntiles), dim3(256), 0, s, nt->wpart.as<float>(), nchunks, ntiles,
+                     ntco, ntci, cin, nt->grad.as<float>() + L.woff);
   HIPCHK(ctx, hipGetLastError());
   if (dX.p) {
     const size_t nw = (size_t)27 * L.cout * cin;
@@ -719,9 +739,14 @@ int net_alloc(hfk_net* nt) {
     HIPCHK(ctx, nt->g_outf.alloc((size_t)nt->nvin * 16 * sizeof(float)));
     HIPCHK(ctx, hipMemset(nt->outf.p, 0, (size_t)nt->nvin * 16 * sizeof(float)));
   }
-  size_t maxw = 0;
-  for (const Layer& L : nt->layers) maxw = std::max(maxw, (size_t)27 * L.cout * (L.cin1 + L.cin2));
+  size_t maxw = 0, maxp = 0;
+  for (const Layer& L : nt->layers) {
+    maxw = std::max(maxw, (size_t)27 * L.cout * (L.cin1 + L.cin2));
+    const size_t nchunks = (size_t)((nt->nvox[L.level] + kVoxWave - 1) / kVoxWave);
+    maxp = std::max(maxp, (size_t)27 * (L.cout / 16) * ((L.cin1 + L.cin2) / 16) * nchunks * 256);
+  }
   HIPCHK(ctx, nt->wt.alloc(maxw * sizeof(unsigned short)));
+  HIPCHK(ctx, nt->wpart.alloc(maxp * sizeof(float)));
   HIPCHK(ctx, nt->scal.alloc(8 * sizeof(double)));
   HIPCHK(ctx, nt->part.alloc(512 * sizeof(double)));
   HIPCHK(ctx, nt->dots.alloc(dot_partial_doubles() * sizeof(double)));

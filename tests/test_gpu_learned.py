@@ -35,6 +35,9 @@ def test_network_gradient_matches_torch(hip_ctx, coarsen):
     r = rng.standard_normal(cs.nfree)
     e = rng.standard_normal(cs.nfree)
     loss, g = net.gradient(r, e)
+    # the weight gradients are summed in a fixed order: the same gradient bit for bit on a second evaluation
+    loss2, g2 = net.gradient(r, e)
+    assert loss2 == loss and np.array_equal(g2, g)
     Lr, gr, out = learned_ref.loss_and_grad(w, cs.lattice_dofs(), 17, 1.0 / 16, r, e, coarsen=coarsen)
     print(f"loss gpu {loss:.6e} torch {Lr:.6e}; grad rel {rel(g, gr):.3e}", flush=True)
     assert loss == pytest.approx(Lr, rel=2e-2)
