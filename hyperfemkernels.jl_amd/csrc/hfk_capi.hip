@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -26,6 +27,34 @@ using namespace hfk;
 namespace hfk {
 
 thread_local std::string t_err;
+
+// Scratch of the allocation-free energy (hfk_energy_noalloc_dev): its partial sums (one per batch, 2 x 256 for a
+// traction load, one per Robin face).  One copy per device, held until the process ends.  In host code `scratch`
+// names a host-side stand-in, not a device address: kernels only ever receive the address that
+// hipGetSymbolAddress returns for the current device (kept per device in g_scratch_ptr, copied into each context).
+constexpr int64_t kScratchDoubles = int64_t(1) << 17;  // 1 MiB: up to 131,072 partial sums
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_m;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// once per device, with that device current: the device address of `scratch`
+int scratch_setup(hfk_ctx* ctx) {
+  if (ctx->device >= kMaxDevices)
+    return set_err(nullptr, HFK_EARG, "HyperFEMKernels: device index beyond the library's device table");
+  std::lock_guard<std::mutex> lk(g_scratch_m);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(nullptr, HFK_EHIP, std::string("HyperFEMKernels: hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = static_cast<double*>(p);
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  return HFK_OK;
+}
+
 
 int set_err(hfk_ctx* ctx, int code, const std::string& msg) {
   t_err = msg;
@@ -359,10 +388,14 @@ int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) 
 // strain energy into out_dev[0]: per-batch sums, then their sum, both in a fixed order
 constexpr int kDotBlocks = 256;  // work-groups of the load's dot products (a fixed grid: a fixed order)
 
-int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) {
+// number of partial sums of one energy evaluation
+int64_t energy_partials(const hfk_op* op) {
+  return op->nbatches + (op->has_traction ? 2 * kDotBlocks : 0) + op->robin_n;
+}
+
+// the energy with its partial sums in `part` (energy_partials(op) doubles, every one overwritten)
+int run_energy_into(hfk_op* op, const double* x_dev, double* out_dev, double* part, hipStream_t s) {
   const int64_t nload = op->has_traction ? 2 * kDotBlocks : 0;
-  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(op->nbatches + nload + op->robin_n, 1) * sizeof(double)));
-  double* part = op->part.as<double>();
   if (op->rtc) {
     int rc = rtc_energy(op, x_dev, part, s);
     if (rc) return rc;
@@ -380,6 +413,11 @@ int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) 
                                        op->robin_M.as<double>(), op->robin_n, x_dev, part + op->nbatches + nload, s));
   HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches + nload + op->robin_n, 0, out_dev, s));
   return HFK_OK;
+}
+
+int run_energy(hfk_op* op, const double* x_dev, double* out_dev, hipStream_t s) {
+  HIPCHK(op->ctx, op->part.alloc((size_t)std::max<int64_t>(energy_partials(op), 1) * sizeof(double)));
+  return run_energy_into(op, x_dev, out_dev, op->part.as<double>(), s);
 }
 
 int stage_in(hfk_op* op, DevBuf& buf, const double* h, int64_t n) {
@@ -481,6 +519,12 @@ int hfk_open(int device, hfk_ctx** out) {
   if (e != hipSuccess) {
     delete ctx;
     return set_err(nullptr, HFK_EHIP, std::string("HyperFEMKernels: ") + hipGetErrorString(e));
+  }
+  const int rc = scratch_setup(ctx);
+  if (rc) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
   }
   *out = ctx;
   return HFK_OK;
@@ -817,6 +861,22 @@ int hfk_energy_dev(hfk_op* op, const double* x_dev, double* energy_dev, void* st
   if (rc) return rc;
   if (!x_dev || !energy_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
   return run_energy(op, x_dev, energy_dev, op->stream(stream));
+}
+
+int hfk_energy_noalloc_dev(hfk_op* op, const double* x_dev, double* energy_dev, void* stream) {
+  int rc = bind(op);
+  if (rc) return rc;
+  if (!x_dev || !energy_dev) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
+  if (!op->ctx->scratch) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: the context has no scratch");
+  const int64_t need = energy_partials(op);
+  if (need > kScratchDoubles)
+    return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: hfk_energy_noalloc_dev needs " + std::to_string(need) +
+                                          " partial sums; its scratch holds " + std::to_string(kScratchDoubles) +
+                                          " (batches + 512 for a traction load + Robin faces)");
+  if (op->rtc && !rtc_law_loaded(op))
+    return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: the compiled law's kernels are not loaded for this "
+                                        "operator yet (they are compiled by its first residual/prepare/energy call)");
+  return run_energy_into(op, x_dev, energy_dev, op->ctx->scratch, op->stream(stream));
 }
 
 int hfk_mf_prepare(hfk_op* op, const double* x, int storage) {

@@ -13,6 +13,7 @@ struct hfk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  double* scratch = nullptr;  // device address of the library's static scratch on this device (set by hfk_open)
 };
 
 namespace hfk {
@@ -161,4 +162,6 @@ int robin_blocks(hfk_op* op, const int32_t* block_of, double* B, hipStream_t s);
 int robin_assemble(hfk_op* op, double* nzval, hipStream_t s);
 // strain energy per batch with the operator's compiled law into part[nbatches] (HFK_EARG if it has no energy())
 int rtc_energy(hfk_op* op, const double* x_dev, double* part, hipStream_t s);
+// true when the operator's compiled law already has its kernels loaded for the operator's (Q, P)
+bool rtc_law_loaded(hfk_op* op);
 }  // namespace hfk

@@ -304,6 +304,12 @@ int rtc_residual(hfk_op* op, const double* x, double* b, hipStream_t s) {
   return HFK_OK;
 }
 
+bool rtc_law_loaded(hfk_op* op) {
+  hfk_law* law = op->rtc;
+  std::lock_guard<std::mutex> lk(law->m);
+  return law->inst.count(op->Q * 16 + op->P) > 0;
+}
+
 int rtc_energy(hfk_op* op, const double* x, double* part, hipStream_t s) {
   RtcInstance* ri = nullptr;
   int rc = law_instance(op, &ri);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_energy": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_energy_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_energy_noalloc_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_pointfn_create": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "hfk_pointfn_destroy": (C.c_int, [_vp]),
     "hfk_extremum": (C.c_int, [_vp, _vp, _f64p, _f64p, _f64p, _f64p, C.c_int64, C.c_int, _f64p]),

@@ -195,6 +195,20 @@ function (f::DeviceFunctional)(x::Vector{Float64})
     v[]
 end
 
+"""
+    energy_noalloc!(energy_dev, f::DeviceFunctional, x_dev; stream = C_NULL)
+
+The energy of `f` (reduce = :sum) from device pointers without any device allocation (no counterpart in the
+reference: for a line search captured in a HIP graph).  The partial sums use the library's static 1 MiB scratch per
+device: at most 131,072 of them (batches + 512 + Robin faces); calls that overlap on one device race.
+"""
+function energy_noalloc!(energy_dev::Ptr{Float64}, f::DeviceFunctional, x_dev::Ptr{Float64}; stream::Ptr{Cvoid}=C_NULL)
+    f.reduce === :sum || throw(ArgumentError("HyperFEMKernels: energy_noalloc! needs a reduce = :sum functional"))
+    check(ccall((:hfk_energy_noalloc_dev, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                f.op.ptr, x_dev, energy_dev, stream), f.op.ctx.ptr)
+    energy_dev
+end
+
 "Algebra.residual!(b, op, x) (src/assembly.jl:909-916)"
 function residual!(b::Vector{Float64}, op::DeviceOperator, x::Vector{Float64})
     length(x) == op.nfree == length(b) || throw(DimensionMismatch("HyperFEMKernels: vector lengths"))

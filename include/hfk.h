@@ -150,6 +150,14 @@ int hfk_residual_dev(hfk_op* op, const double* x_dev, double* b_dev, void* strea
  * on `stream`, the energy written to energy_dev[0]. */
 int hfk_energy(hfk_op* op, const double* x, double* energy);
 int hfk_energy_dev(hfk_op* op, const double* x_dev, double* energy_dev, void* stream);
+/* hfk_energy_dev without any allocation (no counterpart in the reference): for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight line-search loops, harnesses that forbid hipMalloc).  Same kernels and
+ * summation order as hfk_energy_dev, so the same value bit for bit; the partial sums go to a static 1 MiB scratch
+ * array per device (set up by hfk_open) instead of a buffer of the operator.  Limit: batches + 512 (traction load)
+ * + Robin faces <= 131,072 partial sums (Q2: about 1.17M cells), else HFK_EDIM.  A compiled law must have been
+ * used once by the operator (residual, prepare or energy) so that its kernels are loaded, else HFK_ESTATE.  Every
+ * call on one device shares that scratch: calls that overlap on one device (two streams, two threads) race. */
+int hfk_energy_noalloc_dev(hfk_op* op, const double* x_dev, double* energy_dev, void* stream);
 /* A pointwise function of the displacement gradient and the gradient of an increment, compiled with hiprtc:
  * `source` defines  __device__ double name(const double* Gu, const double* Gdu, const double* p);
  * G[c + 3J] = d u_c / d x_J (Gridap's ∇(u)'), p the parameters.  Cached per process by the source text. */
