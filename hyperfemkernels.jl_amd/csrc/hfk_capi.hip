@@ -55,7 +55,6 @@ int scratch_setup(hfk_ctx* ctx) {
   return HFK_OK;
 }
 
-
 int set_err(hfk_ctx* ctx, int code, const std::string& msg) {
   t_err = msg;
   if (ctx) ctx->err = msg;
@@ -71,7 +70,10 @@ int op_bind(hfk_op* op) {
 // prepare!: tangents at the state x_dev; a device copy of the state is kept (the reference's mf.x,
 // src/matrixfree.jl:140-143), read by the coarse levels of p-multigrid
 int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) {
-  if (storage < 0 || storage > 3) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
+  if (storage < 0 || storage > 5) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
+  if (storage >= 4 && !op->rtc && op->law == 0)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the factorised tangent storage needs the neo-Hookean or the "
+                                      "linear law (St Venant-Kirchhoff has no such form)");
   const size_t bytes = (size_t)op->nbatches * store_entries(storage) * op->C * op->np * store_bytes(storage);
   HIPCHK(op->ctx, op->tangents.alloc(bytes));
   HIPCHK(op->ctx, op->state.alloc((size_t)op->nfree * sizeof(double)));
@@ -494,6 +496,12 @@ int check_faces(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int3
     for (int c = 0; c < nv; ++c)
       if (!std::isfinite(v[nv * f + c])) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: face data not finite");
   }
+  // a face listed twice would land in one launch group, whose faces must not share dofs
+  std::vector<int64_t> key((size_t)nfaces);
+  for (int64_t f = 0; f < nfaces; ++f) key[f] = (int64_t)face_cell[f] * 6 + face_local[f];
+  std::sort(key.begin(), key.end());
+  if (std::adjacent_find(key.begin(), key.end()) != key.end())
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: a (cell, local face) pair is listed twice");
   return HFK_OK;
 }
 

@@ -67,8 +67,10 @@ hipError_t launch_fill(double* y, int64_t n, double v, hipStream_t s);
 hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* tangents, double* D, int64_t nbatches,
                                   hipStream_t s);
 
-// number of stored entries and bytes per entry of a storage mode
-inline int store_entries(int storage) { return (storage == 1 || storage == 3) ? 45 : 81; }
-inline int store_bytes(int storage) { return storage >= 2 ? 4 : 8; }
+// number of stored entries and bytes per entry of a storage mode (0 full64, 1 sym64, 2 full32, 3 sym32, 4 fac64,
+// 5 fac32)
+inline int store_entries(int storage) { return storage >= 4 ? 17 : ((storage == 1 || storage == 3) ? 45 : 81); }
+inline int store_bytes(int storage) { return (storage == 2 || storage == 3 || storage == 5) ? 4 : 8; }
+inline bool store_fp32(int storage) { return storage == 2 || storage == 3 || storage == 5; }
 
 }  // namespace hfk

@@ -48,7 +48,7 @@ __device__ __forceinline__ T ld_stream(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
-template <int Q, int P, class TS, bool SYM, bool ATOMIC>
+template <int Q, int P, class TS, int LAY, bool ATOMIC>
 __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __restrict__ tangents,
                                                     const double* __restrict__ dx, double* __restrict__ y,
                                                     int batch0, double alpha, const int* __restrict__ colfc,
@@ -57,7 +57,8 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
   // FP32 tangents (the smoothers' storages): the sweeps and the point product run in FP32 too, which halves the
   // LDS and register footprint of the work-group; y accumulates in FP64
   using CT = typename std::conditional<std::is_same<TS, float>::value, float, double>::type;
-  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
+  constexpr bool SYM = LAY == kLaySym, FAC = LAY == kLayFac;
+  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = lay_entries(LAY);
   __shared__ CT lds[Dm::LDS];
   CT* A = lds;
   CT* B = lds + C * SA;
@@ -68,11 +69,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
   const TS* Dp = static_cast<const TS*>(tangents) + (size_t)batch * NE * NSLOT + tid;
   // FP32 packed storage: the point's 45 entries are loaded before the gather and the interpolation sweeps (45
   // registers), so that their latency overlaps those phases instead of following their barriers
-  constexpr bool PRE = SYM && std::is_same<TS, float>::value;
-  CT dpre[PRE ? 45 : 1];
+  // (the 17 factors of the factorised layout likewise, in either precision)
+  constexpr bool PRE = (SYM && std::is_same<TS, float>::value) || FAC;
+  constexpr int NPRE = PRE ? NE : 1;
+  CT dpre[NPRE];
   if (PRE && tid < NSLOT) {
 #pragma unroll
-    for (int e = 0; e < (PRE ? 45 : 1); ++e) dpre[e] = static_cast<CT>(ld_stream(Dp + (size_t)e * NSLOT));
+    for (int e = 0; e < NPRE; ++e) dpre[e] = static_cast<CT>(ld_stream(Dp + (size_t)e * NSLOT));
   }
   for (int it = tid; it < C * 3 * Dm::NN; it += kThreads) {
     const int id = ids[it];
@@ -90,16 +93,43 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
       su[j] = s[j];
       sv[j] = 0.0;
     }
-    if (PRE) {
+    if constexpr (FAC) {
+      // sv[K+3c] = sum_M G_KM su[M+3c] + b sum_k H_Kk X_ck + l H_Kc tr(X),  X_ck = sum_M H_Mc su[M+3k]
+      const CT* G = dpre;
+      const CT* H = dpre + 6;
+      CT X[9];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          CT v = 0;
+#pragma unroll
+          for (int M = 0; M < 3; ++M) v += H[M + 3 * c] * su[M + 3 * k];
+          X[c + 3 * k] = v;
+        }
+      const CT tr = X[0] + X[4] + X[8];
+      const CT bm = dpre[15], bl = dpre[16];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int K = 0; K < 3; ++K) {
+          CT g = 0, h = 0;
+#pragma unroll
+          for (int M = 0; M < 3; ++M) g += G[K <= M ? sym_index(K, M) : sym_index(M, K)] * su[M + 3 * c];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) h += H[K + 3 * k] * X[c + 3 * k];
+          sv[K + 3 * c] = g + bm * h + bl * H[K + 3 * c] * tr;
+        }
+    } else if constexpr (PRE) {
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
         for (int i = 0; i <= j; ++i) {
-          const CT d = dpre[PRE ? sym_index(i, j) : 0];
+          const CT d = dpre[sym_index(i, j)];
           sv[i] += d * su[j];
           if (i != j) sv[j] += d * su[i];
         }
-    } else if (!SYM) {
+    } else if constexpr (!SYM) {
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
@@ -135,12 +165,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
 // BlockJacobi from the stored tangents (_blocks_from_tangents!, src/preconditioners.jl:234-263), the cells of
 // one colour: B_n[ci,cj] += sum_q sum_K sum_M dN_n/dxi_K D_q[(K,ci),(M,cj)] dN_n/dxi_M.  One thread per
 // (cell, node); the tangents of the current point of the batch's cells are staged in LDS.
-template <int Q, int P, class TS, bool SYM>
+template <int Q, int P, class TS, int LAY>
 __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __restrict__ tangents,
                                                      const int32_t* __restrict__ block_of, double* __restrict__ Bk,
                                                      int batch0) {
   using Dm = Dims<Q, P>;
-  constexpr int C = Dm::C, NN = Dm::NN, NP = Dm::NP, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
+  constexpr bool SYM = LAY == kLaySym;
+  constexpr int C = Dm::C, NN = Dm::NN, NP = Dm::NP, NSLOT = Dm::NSLOT, NE = lay_entries(LAY);
   __shared__ double Dl[C * 81];
   const int tid = threadIdx.x;
   const int batch = batch0 + blockIdx.x;
@@ -155,8 +186,12 @@ __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __res
     __syncthreads();
     for (int it = tid; it < C * 81; it += kThreads) {
       const int c2 = it / 81, e = it - c2 * 81, ii = e % 9, jj = e / 9;
-      const int se = SYM ? (ii <= jj ? sym_index(ii, jj) : sym_index(jj, ii)) : e;
-      Dl[it] = static_cast<double>(ld_stream(Dg + (size_t)se * NSLOT + c2 * NP + q));
+      if (LAY == kLayFac) {
+        Dl[it] = fac_entry(Dg + c2 * NP + q, NSLOT, ii, jj);
+      } else {
+        const int se = SYM ? (ii <= jj ? sym_index(ii, jj) : sym_index(jj, ii)) : e;
+        Dl[it] = static_cast<double>(ld_stream(Dg + (size_t)se * NSLOT + c2 * NP + q));
+      }
     }
     __syncthreads();
     if (active) {
@@ -459,9 +494,10 @@ __global__ __launch_bounds__(kThreads) void k_dot_partials(const double* __restr
 }
 
 // Expand the stored tangents of every batch to D[(batch*C + cell)*np + q][81] (FP64), for inspection.
-template <int P, class TS, bool SYM>
+template <int P, class TS, int LAY>
 __global__ void k_expand(const void* __restrict__ tangents, double* __restrict__ out, int64_t nbatches) {
-  constexpr int NP = P * P * P, C = kThreads / NP, NSLOT = C * NP, NE = SYM ? 45 : 81;
+  constexpr bool SYM = LAY == kLaySym;
+  constexpr int NP = P * P * P, C = kThreads / NP, NSLOT = C * NP, NE = lay_entries(LAY);
   const int64_t n = nbatches * NSLOT;
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
     const int64_t batch = g / NSLOT;
@@ -469,6 +505,10 @@ __global__ void k_expand(const void* __restrict__ tangents, double* __restrict__
     const TS* Dp = static_cast<const TS*>(tangents) + batch * NE * NSLOT + slot;
     for (int j = 0; j < 9; ++j)
       for (int i = 0; i < 9; ++i) {
+        if (LAY == kLayFac) {
+          out[g * 81 + i + 9 * j] = fac_entry(Dp, NSLOT, i, j);
+          continue;
+        }
         const int e = SYM ? (i <= j ? sym_index(i, j) : sym_index(j, i)) : i + 9 * j;
         out[g * 81 + i + 9 * j] = static_cast<double>(Dp[(size_t)e * NSLOT]);
       }
@@ -481,10 +521,18 @@ template <int Q, int P, class Law>
 hipError_t prepare_qp(const OpDev& op, int storage, void* T, const double* x, BatchRange r, hipStream_t s) {
   const dim3 g(r.count), b(kThreads);
   switch (storage) {
-    case 0: hipLaunchKernelGGL((k_prepare<Q, P, Law, double, false>), g, b, 0, s, op, T, x, r.first); break;
-    case 1: hipLaunchKernelGGL((k_prepare<Q, P, Law, double, true>), g, b, 0, s, op, T, x, r.first); break;
-    case 2: hipLaunchKernelGGL((k_prepare<Q, P, Law, float, false>), g, b, 0, s, op, T, x, r.first); break;
-    default: hipLaunchKernelGGL((k_prepare<Q, P, Law, float, true>), g, b, 0, s, op, T, x, r.first); break;
+    case 0: hipLaunchKernelGGL((k_prepare<Q, P, Law, double, kLayFull>), g, b, 0, s, op, T, x, r.first); break;
+    case 1: hipLaunchKernelGGL((k_prepare<Q, P, Law, double, kLaySym>), g, b, 0, s, op, T, x, r.first); break;
+    case 2: hipLaunchKernelGGL((k_prepare<Q, P, Law, float, kLayFull>), g, b, 0, s, op, T, x, r.first); break;
+    case 3: hipLaunchKernelGGL((k_prepare<Q, P, Law, float, kLaySym>), g, b, 0, s, op, T, x, r.first); break;
+    default:
+      if constexpr (std::is_same<Law, LawSVK>::value) {
+        return hipErrorInvalidValue;  // no factorised form (refused on the host before)
+      } else {
+        if (storage == 4) hipLaunchKernelGGL((k_prepare<Q, P, Law, double, kLayFac>), g, b, 0, s, op, T, x, r.first);
+        else hipLaunchKernelGGL((k_prepare<Q, P, Law, float, kLayFac>), g, b, 0, s, op, T, x, r.first);
+      }
+      break;
   }
   return hipGetLastError();
 }
@@ -505,10 +553,12 @@ hipError_t apply_qpa(const OpDev& op, int storage, const void* T, const double* 
   const int* fc = ATOMIC ? m->colfc : nullptr;
   const int nc = ATOMIC ? m->ncol : 0, gx = ATOMIC ? m->gx : 0;
   switch (storage) {
-    case 0: hipLaunchKernelGGL((k_apply<Q, P, double, false, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
-    case 1: hipLaunchKernelGGL((k_apply<Q, P, double, true, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
-    case 2: hipLaunchKernelGGL((k_apply<Q, P, float, false, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
-    default: hipLaunchKernelGGL((k_apply<Q, P, float, true, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 0: hipLaunchKernelGGL((k_apply<Q, P, double, kLayFull, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 1: hipLaunchKernelGGL((k_apply<Q, P, double, kLaySym, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 2: hipLaunchKernelGGL((k_apply<Q, P, float, kLayFull, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 3: hipLaunchKernelGGL((k_apply<Q, P, float, kLaySym, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 4: hipLaunchKernelGGL((k_apply<Q, P, double, kLayFac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    default: hipLaunchKernelGGL((k_apply<Q, P, float, kLayFac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
   }
   return hipGetLastError();
 }
@@ -525,10 +575,12 @@ hipError_t blocks_qp(const OpDev& op, int storage, const void* T, const int32_t*
                      hipStream_t s) {
   const dim3 g(r.count), b(kThreads);
   switch (storage) {
-    case 0: hipLaunchKernelGGL((k_blocks<Q, P, double, false>), g, b, 0, s, op, T, block_of, B, r.first); break;
-    case 1: hipLaunchKernelGGL((k_blocks<Q, P, double, true>), g, b, 0, s, op, T, block_of, B, r.first); break;
-    case 2: hipLaunchKernelGGL((k_blocks<Q, P, float, false>), g, b, 0, s, op, T, block_of, B, r.first); break;
-    default: hipLaunchKernelGGL((k_blocks<Q, P, float, true>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 0: hipLaunchKernelGGL((k_blocks<Q, P, double, kLayFull>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 1: hipLaunchKernelGGL((k_blocks<Q, P, double, kLaySym>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 2: hipLaunchKernelGGL((k_blocks<Q, P, float, kLayFull>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 3: hipLaunchKernelGGL((k_blocks<Q, P, float, kLaySym>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 4: hipLaunchKernelGGL((k_blocks<Q, P, double, kLayFac>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    default: hipLaunchKernelGGL((k_blocks<Q, P, float, kLayFac>), g, b, 0, s, op, T, block_of, B, r.first); break;
   }
   return hipGetLastError();
 }
@@ -707,10 +759,12 @@ hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* T, d
   const dim3 g(1024), b(256);
 #define EXP(P_)                                                                                           \
   switch (storage) {                                                                                      \
-    case 0: hipLaunchKernelGGL((k_expand<P_, double, false>), g, b, 0, s, T, D, nbatches); break;          \
-    case 1: hipLaunchKernelGGL((k_expand<P_, double, true>), g, b, 0, s, T, D, nbatches); break;           \
-    case 2: hipLaunchKernelGGL((k_expand<P_, float, false>), g, b, 0, s, T, D, nbatches); break;           \
-    default: hipLaunchKernelGGL((k_expand<P_, float, true>), g, b, 0, s, T, D, nbatches); break;           \
+    case 0: hipLaunchKernelGGL((k_expand<P_, double, kLayFull>), g, b, 0, s, T, D, nbatches); break;       \
+    case 1: hipLaunchKernelGGL((k_expand<P_, double, kLaySym>), g, b, 0, s, T, D, nbatches); break;        \
+    case 2: hipLaunchKernelGGL((k_expand<P_, float, kLayFull>), g, b, 0, s, T, D, nbatches); break;        \
+    case 3: hipLaunchKernelGGL((k_expand<P_, float, kLaySym>), g, b, 0, s, T, D, nbatches); break;         \
+    case 4: hipLaunchKernelGGL((k_expand<P_, double, kLayFac>), g, b, 0, s, T, D, nbatches); break;        \
+    default: hipLaunchKernelGGL((k_expand<P_, float, kLayFac>), g, b, 0, s, T, D, nbatches); break;        \
   }
   if (op.P == 2) {
     EXP(2)

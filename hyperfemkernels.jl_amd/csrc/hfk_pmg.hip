@@ -578,7 +578,7 @@ int bell_build(hfk_pmg* g) {
   if (!pos.empty()) HIPCHK(g->ctx, hipMemcpy(B.pos.p, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   const size_t nv = (size_t)9 * W * nb;
   HIPCHK(g->ctx, B.v64.alloc(nv * sizeof(double)));
-  B.has32 = g->smoother_storage == HFK_STORE_FULL32 || g->smoother_storage == HFK_STORE_SYM32 || g->coarse_fp32;
+  B.has32 = (g->smoother_storage >= 0 && store_fp32(g->smoother_storage)) || g->coarse_fp32;
   if (B.has32) HIPCHK(g->ctx, B.v32.alloc(nv * sizeof(float)));
   B.nb = nb;
   B.W = W;
@@ -649,13 +649,14 @@ int coarse_smoother(hfk_pmg* g, hipStream_t s, bool* usable) {
     ch->csr_colptr = g->colptr.as<int32_t>();
     ch->csr_rowval = g->rowval.as<int32_t>();
     ch->csr_val = g->nzval.as<double>();
-    if (g->bell.built) {  // FP32 values when the p-multigrid smooths in FP32
+    if (g->bell.built) {  // FP32 values when the p-multigrid smooths in FP32 (not for an FP32 coarse PCG alone)
+      const bool f32 = g->bell.has32 && g->smoother_storage >= 0 && store_fp32(g->smoother_storage);
       ch->bell_nb = g->bell.nb;
       ch->bell_W = g->bell.W;
       ch->bell_nbr = g->bell.nbr.as<int32_t>();
       ch->bell_dofs = c->blocks.dofs.as<int32_t>();
-      ch->bell_fp32 = g->bell.has32;
-      ch->bell_val = g->bell.has32 ? g->bell.v32.p : g->bell.v64.p;
+      ch->bell_fp32 = f32;
+      ch->bell_val = f32 ? g->bell.v32.p : g->bell.v64.p;
     }
     g->ccheb = std::move(ch);
   }
@@ -954,7 +955,8 @@ int krylov(hfk_op* op, hfk_pmg* g, int kind, int precond, const double* b, doubl
   double rt;
   if ((rc = w.dots.run(op->ctx, s, n, r, r, &rt))) return rc;
   rnorm[0] = std::sqrt(rt);
-  status[0] = (conv || rnorm[0] <= tol) && std::isfinite(rnorm[0]) ? 1 : 0;
+  // a stop on non-positive curvature is never converged (src/solvers.jl:141-146: converged = false, indefinite)
+  status[0] = !indef && (conv || rnorm[0] <= tol) && std::isfinite(rnorm[0]) ? 1 : 0;
   status[1] = it;
   status[2] = indef ? 1 : 0;
   return HFK_OK;
@@ -1008,6 +1010,7 @@ int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, d
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), hcol(m + 1);
   int it = 0;
   bool solved = false, stop = false;
+  const bool variable = !flexible && precond == 2;
   HIPCHK(op->ctx, vec_copy(n, b, rp, s));  // r = b - A x, x = 0
   double beta = bn;
   while (!stop && it < maxiter) {
@@ -1055,8 +1058,13 @@ int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, d
       for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i];
       const double est = std::fabs(gv[j + 1]);
       if (est <= tol) {
-        solved = true;
-        stop = true;
+        // GMRES with a p-multigrid preconditioner: the cycle (iterative coarse solve) differs from one
+        // application to the next, so x += M^-1 (V y) does not realise the estimate; the restart below decides
+        // on the true residual and continues when it is not met
+        if (!variable) {
+          solved = true;
+          stop = true;
+        }
         break;
       }
       if (!(hn > 1e-14 * den)) {  // breakdown: the Krylov space holds the solution
@@ -1101,6 +1109,7 @@ int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, d
   if ((rc = dots.run(op->ctx, s, n, rp, rp, &rt))) return rc;
   rnorm[0] = std::sqrt(rt);
   status[0] = (solved || rnorm[0] <= tol) && std::isfinite(rnorm[0]) ? 1 : 0;
+  if (variable) status[0] = rnorm[0] <= tol ? 1 : 0;  // only the true residual counts
   status[1] = it;
   status[2] = 0;
   return HFK_OK;
@@ -1300,7 +1309,7 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
   if ((rc = check_cheb_args(ctx, degree, range, inflate, eig_iters))) return rc;
   if (coarse_kind != 0 || !(coarse_rtol > 0) || coarse_maxiter < 1)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unsupported coarse solver settings");
-  if (smoother_storage < -1 || smoother_storage > 3)
+  if (smoother_storage < -1 || smoother_storage > 5)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unknown smoother storage");
   std::unique_ptr<hfk_pmg> g(new hfk_pmg());
   g->ctx = ctx;

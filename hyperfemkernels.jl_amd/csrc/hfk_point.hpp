@@ -125,6 +125,12 @@ struct LawNeoHooke {  // W = mu/2 (I1 - 3) - mu lnJ + lam/2 lnJ^2:  P = mu (F - 
   __device__ double tangent(int c, int J, int k, int L) const {
     return (c == k && J == L ? mu : 0.0) + (mu - lam * lnJ) * A[J + 3 * k] * A[L + 3 * c] + lam * A[J + 3 * c] * A[L + 3 * k];
   }
+  // the tangent in the isotropic factorised form mu d_ck d_JL + m Ao_Jk Ao_Lc + lam Ao_Jc Ao_Lk
+  __device__ void factor(double* Ao, double& m) const {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ao[i] = A[i];
+    m = mu - lam * lnJ;
+  }
 };
 
 struct LawLinear {  // sigma = lam tr(G) I + mu (G + G'), G = F - I
@@ -155,6 +161,12 @@ struct LawLinear {  // sigma = lam tr(G) I + mu (G + G'), G = F - I
   }
   __device__ double tangent(int c, int J, int k, int L) const {
     return (c == J && k == L ? lam : 0.0) + (c == k && J == L ? mu : 0.0) + (J == k && c == L ? mu : 0.0);
+  }
+  // the factorised form of the tangent (see LawNeoHooke::factor) with Ao = I and m = mu
+  __device__ void factor(double* Ao, double& m) const {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ao[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    m = mu;
   }
 };
 
@@ -390,6 +402,28 @@ __device__ __forceinline__ void point_coordinates(const Tab& tb, const double* _
 // Index of the packed upper triangle (i <= j) of a symmetric 9x9 matrix.
 __host__ __device__ constexpr int sym_index(int i, int j) { return j * (j + 1) / 2 + i; }
 
+// Layouts of the stored tangent D[(K,c),(M,k)] of one point (i = K + 3c, j = M + 3k):
+//   kLayFull  81 entries D[i + 9j];
+//   kLaySym   45 entries, the upper triangle of a symmetric D at sym_index(i, j);
+//   kLayFac   17 factors of an isotropic tangent of the form T = mu d_ck d_JL + m A_Jk A_Lc + lam A_Jc A_Lk (the
+//             built-in neo-Hookean law, A = F^-1, m = mu - lam lnJ; the linear law, A = I, m = mu).  With
+//             H = iJt' A (H[K + 3c] = sum_J iJt[J,K] A_Jc) and the metric g = iJt' iJt:
+//               D[(K,c),(M,k)] = G_KM d_ck + b H_Kk H_Mc + l H_Kc H_Mk,
+//             stored as G = dV mu g (entries 0..5, packed upper triangle sym_index(K, M)), H (6..14),
+//             b = dV m (15), l = dV lam (16).
+constexpr int kLayFull = 0, kLaySym = 1, kLayFac = 2;
+__host__ __device__ constexpr int lay_entries(int lay) { return lay == kLayFull ? 81 : (lay == kLaySym ? 45 : 17); }
+
+// entry D[i + 9j] of a point stored in the factorised layout; f[e * stride] is factor e
+template <class TS>
+__device__ __forceinline__ double fac_entry(const TS* f, size_t stride, int i, int j) {
+  const int K = i % 3, c = i / 3, M = j % 3, k = j / 3;
+  const double hKk = (double)f[(6 + K + 3 * k) * stride], hMc = (double)f[(6 + M + 3 * c) * stride];
+  const double hKc = (double)f[(6 + K + 3 * c) * stride], hMk = (double)f[(6 + M + 3 * k) * stride];
+  const double g = c == k ? (double)f[(K <= M ? sym_index(K, M) : sym_index(M, K)) * stride] : 0.0;
+  return g + (double)f[15 * stride] * hKk * hMc + (double)f[16 * stride] * hKc * hMk;
+}
+
 // the tangents are written once per prepare! and read by the next products, never from cache.  Non-temporal
 // stores measured faster for the packed symmetric storages (-12 % sym64, -20 % sym32 at 64^3 Q2) and slower for
 // the full ones (+15 % full64), so they are used for the former only
@@ -401,11 +435,12 @@ __device__ __forceinline__ void store_entry(void* base, size_t idx, double v) {
 
 // ---------------------------------------------------------------------------------------------------------
 // prepare!: per point, the reference-coordinate tangent D = dV iJt' T iJt (src/matrixfree.jl:199-228).
-template <int Q, int P, class Law, class TS, bool SYM>
+template <int Q, int P, class Law, class TS, int LAY>
 __global__ __launch_bounds__(kThreads) void k_prepare(OpDev op, void* __restrict__ tangents,
                                                       const double* __restrict__ x, int batch0) {
   using Dm = Dims<Q, P>;
-  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = SYM ? 45 : 81;
+  constexpr bool SYM = LAY == kLaySym;
+  constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = lay_entries(LAY);
   __shared__ double lds[Dm::LDS];
   double* A = lds;
   double* B = lds + C * SA;
@@ -439,6 +474,31 @@ __global__ __launch_bounds__(kThreads) void k_prepare(OpDev op, void* __restrict
     Law law;
     law.setup(F, op.lam, mu);
     const size_t base = (size_t)batch * NE * NSLOT + tid;
+    if constexpr (LAY == kLayFac) {
+      double Af[9], m;
+      law.factor(Af, m);
+#pragma unroll
+      for (int M = 0; M < 3; ++M)
+#pragma unroll
+        for (int K = 0; K <= M; ++K) {
+          double g = 0;
+#pragma unroll
+          for (int J = 0; J < 3; ++J) g += iJt[J + 3 * K] * iJt[J + 3 * M];
+          store_entry<TS, true>(tangents, base + (size_t)sym_index(K, M) * NSLOT, dV * mu * g);
+        }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int K = 0; K < 3; ++K) {
+          double h = 0;
+#pragma unroll
+          for (int J = 0; J < 3; ++J) h += iJt[J + 3 * K] * Af[J + 3 * c];
+          store_entry<TS, true>(tangents, base + (size_t)(6 + K + 3 * c) * NSLOT, h);
+        }
+      store_entry<TS, true>(tangents, base + (size_t)15 * NSLOT, dV * m);
+      store_entry<TS, true>(tangents, base + (size_t)16 * NSLOT, dV * op.lam);
+      return;
+    }
     // D[(K,c),(M,k)] = dV sum_J sum_L iJt[J,K] T[(J,c),(L,k)] iJt[L,M], one (c, k) block at a time
 #pragma unroll
     for (int k = 0; k < 3; ++k)

@@ -75,8 +75,8 @@ std::string coord_program(const char* source, const char* name) {
 
 std::vector<std::string> kernel_names(const std::string& name, int Q, int P) {
   const std::string qp = std::to_string(Q) + ", " + std::to_string(P) + ", " + name;
-  return {"hfk::k_prepare<" + qp + ", double, false>", "hfk::k_prepare<" + qp + ", double, true>",
-          "hfk::k_prepare<" + qp + ", float, false>", "hfk::k_prepare<" + qp + ", float, true>",
+  return {"hfk::k_prepare<" + qp + ", double, 0>", "hfk::k_prepare<" + qp + ", double, 1>",
+          "hfk::k_prepare<" + qp + ", float, 0>", "hfk::k_prepare<" + qp + ", float, 1>",
           "hfk::k_residual<" + qp + ">", "hfk::k_energy<" + qp + ">"};
 }
 
@@ -279,6 +279,9 @@ int fn_instance(hfk_op* op, hfk_pointfn* fn, FnInstance** out) {
 }  // namespace
 
 int rtc_prepare(hfk_op* op, int storage, void* T, const double* x, hipStream_t s) {
+  if (storage >= 4)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the factorised tangent storage needs a built-in neo-Hookean or "
+                                      "linear law; a compiled law stores full or symmetric tangents");
   RtcInstance* ri = nullptr;
   int rc = law_instance(op, &ri);
   if (rc) return rc;

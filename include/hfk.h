@@ -45,6 +45,11 @@ extern "C" {
 #define HFK_STORE_SYM64 1     /* 45 FP64 per point: the symmetric part of a tangent that is symmetric */
 #define HFK_STORE_FULL32 2    /* 81 FP32 per point (mixed precision) */
 #define HFK_STORE_SYM32 3     /* 45 FP32 per point */
+#define HFK_STORE_FAC64 4     /* 17 FP64 per point: the factors of an isotropic tangent mu d_ck d_JL + m A_Jk A_Lc +
+                                 lam A_Jc A_Lk (built-in neo-Hookean, A = F^-1; built-in linear, A = I), from which
+                                 the product rebuilds D su in registers (no counterpart in the reference, which stores
+                                 the 81 entries) */
+#define HFK_STORE_FAC32 5     /* the same 17 factors in FP32 (mixed precision) */
 
 typedef struct hfk_ctx hfk_ctx;
 typedef struct hfk_op hfk_op;

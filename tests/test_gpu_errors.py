@@ -81,3 +81,19 @@ def test_oseen_and_network_arguments(hip_ctx):
     cs = hex_space((4, 4, 4), 1, dirichlet=dirichlet_closed_x0)  # 5^3 lattice: too small for three halvings
     with pytest.raises(HFKError, match="ArgumentError"):
         CoarseNetwork(hip_ctx, cs)
+
+
+def test_duplicate_faces_rejected(hip_ctx):
+    """A (cell, local face) pair given twice would put both copies in one launch group, whose faces must share no
+    dof (two threads would add into the same entries): the traction and Robin set-ups refuse it."""
+    from hfk.mesh import boundary_faces
+
+    fs, cs, fop, cop = levels(hip_ctx)
+    fc, fl = boundary_faces(fs, 0, 1)
+    dup_c, dup_l = np.concatenate([fc, fc[:1]]), np.concatenate([fl, fl[:1]])
+    with pytest.raises(HFKError, match="ArgumentError"):
+        fop.set_traction(dup_c, dup_l, [1.0, 0.0, 0.0])
+    with pytest.raises(HFKError, match="ArgumentError"):
+        fop.set_robin(dup_c, dup_l, 1.0)
+    fop.set_traction(fc, fl, [1.0, 0.0, 0.0])  # the same faces once each are fine
+    fop.set_robin(fc, fl, 1.0)

@@ -66,7 +66,8 @@ def test_tangents_jv_residual(oracle, hip_ctx, order, P, law, n, perturb):
     assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < TOL
 
 
-@pytest.mark.parametrize("storage,tol", [("sym64", TOL), ("full32", 1e-5), ("sym32", 1e-5)])
+@pytest.mark.parametrize("storage,tol", [("sym64", TOL), ("full32", 1e-5), ("sym32", 1e-5), ("fac64", TOL),
+                                         ("fac32", 1e-5)])
 def test_storage_variants(oracle, hip_ctx, storage, tol):
     sp, dv, x = problem(2, 3, "neohookean", (4, 3, 3), 0.1)
     op = KernelOperator(HyperelasticForm("neohookean"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
@@ -78,8 +79,59 @@ def test_storage_variants(oracle, hip_ctx, storage, tol):
     print(f"storage {storage}: rel error {e:.3e}", flush=True)
     assert e < tol
     info = op.info()
-    nent = 45 if storage.startswith("sym") else 81
+    nent = {"sym": 45, "fac": 17, "ful": 81}[storage[:3]]
     assert info["tangent_bytes"] >= sp.ncells * 27 * nent * (4 if storage.endswith("32") else 8)
+
+
+@pytest.mark.parametrize("order,P,law,n,perturb", [(2, 3, "neohookean", (4, 3, 3), 0.15), (2, 3, "linear", (3, 3, 2), 0.1),
+                                                   (1, 2, "neohookean", (4, 3, 3), 0.1), (1, 3, "neohookean", (3, 3, 2), 0.1),
+                                                   (3, 4, "neohookean", (2, 2, 2), 0.1), (2, 4, "linear", (2, 2, 2), 0.0)])
+def test_factorised_storage(oracle, hip_ctx, order, P, law, n, perturb):
+    """The factorised storage (17 factors per point, G d_ck + b H_Kk H_Mc + l H_Kc H_Mk): the 81 entries it
+    represents equal the oracle's reference-coordinate tangents (1e-12), also with a shear-modulus field; Jv equals
+    the oracle's (fac64 1e-12, fac32 1e-5), in the colour-ordered and the atomic single-launch products; the node
+    blocks of BlockJacobi equal those built from the full storage.  St Venant-Kirchhoff and compiled laws, which
+    have no such form, are refused."""
+    from hfk.solvers import BlockJacobi
+
+    print(f"factorised order={order} P={P} law={law} n={n}", flush=True)
+    sp, dv, x = problem(order, P, law, n, perturb)
+    form = HyperelasticForm(law, 2.0, 1.0, degree=2 * (P - 1))
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, points1d=P, dvals=dv)
+    mu = np.exp(0.4 * np.random.default_rng(5).standard_normal(sp.ncells * P ** 3))
+    dx = np.random.default_rng(6).standard_normal(sp.nfree)
+    for field in (False, True):
+        if field:
+            op.set_mu_field(mu)
+            orc.set_mu_field(mu)
+        prep = orc.prepare(x)
+        yref = orc.jv(prep, dx)
+        mf = MatrixFreeJacobian(op, "fac64").prepare(x)
+        assert rel(mf.tangents(), oracle.reference_coordinate_tangents(*prep)) < TOL
+        assert rel(mf @ dx, yref) < TOL
+        op.set_deterministic(False)
+        assert rel(mf @ dx, yref) < TOL
+        op.set_deterministic(True)
+        b64 = BlockJacobi(mf).blocks()[1]
+        bfull = BlockJacobi(MatrixFreeJacobian(op, "full64").prepare(x)).blocks()[1]
+        assert rel(b64, bfull) < TOL
+        assert rel(MatrixFreeJacobian(op, "fac32").prepare(x) @ dx, yref) < 1e-5
+    assert op.info()["tangent_bytes"] >= sp.ncells * P ** 3 * 17 * 4
+
+
+def test_factorised_storage_refused(hip_ctx):
+    from hfk.laws import SOURCES, CompiledLaw
+
+    sp, dv, x = problem(2, 3, "svk", (2, 2, 2))
+    op = KernelOperator(HyperelasticForm("svk"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    with pytest.raises(hfk._lib.HFKError, match="ArgumentError"):
+        MatrixFreeJacobian(op, "fac64").prepare(x)
+    op2 = KernelOperator(HyperelasticForm("neohookean"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    src, name = SOURCES["neohookean"]
+    op2.set_law(CompiledLaw(src, name, ctx=hip_ctx))
+    with pytest.raises(hfk._lib.HFKError, match="ArgumentError"):
+        MatrixFreeJacobian(op2, "fac32").prepare(x)
 
 
 def test_mu_field(oracle, hip_ctx):
@@ -213,7 +265,7 @@ def test_small_and_ragged_meshes(oracle, hip_ctx, order, P, n):
     prep = orc.prepare(x)
     dx = np.random.default_rng(9).standard_normal(sp.nfree)
     yref = orc.jv(prep, dx)
-    for st, tol in (("full64", TOL), ("sym64", TOL), ("full32", 1e-5), ("sym32", 1e-5)):
+    for st, tol in (("full64", TOL), ("sym64", TOL), ("full32", 1e-5), ("sym32", 1e-5), ("fac64", TOL), ("fac32", 1e-5)):
         assert rel(MatrixFreeJacobian(op, st).prepare(x) @ dx, yref) < tol
     assert rel(MatrixFreeJacobian(op, "full64").prepare(x).tangents(), oracle.reference_coordinate_tangents(*prep)) < TOL
     assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < TOL

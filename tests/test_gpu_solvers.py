@@ -304,6 +304,23 @@ def test_gmres_pmultigrid(hip_ctx, oracle, krylov):
     assert rel(xs, spla.spsolve(orc.csc(x).tocsc(), b)) < 1e-7
 
 
+@pytest.mark.parametrize("coarse_rtol", [1e-2, 0.3])
+def test_gmres_default_pmultigrid_converged_means_true_residual(hip_ctx, coarse_rtol):
+    """Plain GMRES with the default p-multigrid (inexact coarse PCG: a cycle that varies between applications).
+    Its residual estimate is then not the true residual, so a converged status must rest on the true residual:
+    converged implies ||b - A x|| <= rtol ||b||, and the solve keeps restarting until that holds."""
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=coarse_rtol).update()
+    b = np.random.default_rng(12).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("gmres", rtol=1e-10, maxiter=200, preconditioner=pmg).solve(xs, mf, b)
+    print(f"gmres + default pmg (coarse_rtol {coarse_rtol}): {res}", flush=True)
+    true = np.linalg.norm(b - mf @ xs)
+    assert res.converged
+    assert res.residual == pytest.approx(true, rel=1e-6)
+    assert true <= 1e-10 * np.linalg.norm(b)
+
+
 def test_gmres_failed_preconditioner_returns_at_once(hip_ctx):
     """A p-multigrid that is not ok (NaN state) makes GMRES return at once, not converged (src/solvers.jl:114-121)."""
     fs, fop, cop, mf, x = make_levels(hip_ctx, (3, 3, 3))
