@@ -645,7 +645,8 @@ struct hfk_oseen {
   int64_t ncells = 0, nfree = 0, ndir = 0;
   int np = 0;
   double rho = 1, mu = 0.01, c0 = 0;
-  std::vector<int32_t> cell_dofs, color;
+  std::vector<int32_t> cell_dofs, color;  // cell_dofs in the library's local order
+  std::vector<int32_t> perm;              // perm[i] = the host's local dof of the library's local dof i
   std::vector<int64_t> color_start;  // cells of colour c: cells_by_color[color_start[c] .. color_start[c+1])
   bool pattern_built = false, has_fields = false;
   std::vector<int64_t> colptr, rowval;
@@ -788,14 +789,41 @@ int oseen_step_fields(hfk_oseen* op, const StepFunctions& sf, bool with_lap, hip
 
 extern "C" {
 
-int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, const double* coords, int np,
+int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* host_dofs, const int32_t* nodeperm,
+                     const int32_t* pnodeperm, const int32_t* ldof, const double* coords, int np,
                      const double* qpoints, const double* qweights, int64_t nfree, int64_t ndir, double rho, double mu,
                      double c0, hfk_oseen** out) {
-  if (!ctx || !out || !cell_dofs || !coords || !qpoints || !qweights)
+  if (!ctx || !out || !host_dofs || !coords || !qpoints || !qweights)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: null argument");
   if (ncells <= 0 || nfree <= 0 || ndir < 0 || np < 1 || np > kMaxPoints)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: invalid Oseen sizes (cells, dofs, 1..32 points)");
   if (!(rho > 0) || !(mu > 0) || !(c0 >= 0)) return set_err(ctx, HFK_EARG, "HyperFEMKernels: invalid rho, mu or c0");
+  // the host's local order (FieldTable ldof and the node permutations, src/tables.jl:171-237, 329-342): the library's
+  // local dof c*10 + f (P2 node f of its own order) is the host's ldof[c*10 + nodeperm[f]], its 30 + v (P1 node v)
+  // the host's ldof[30 + pnodeperm[v]]; NULL arrays mean the library's own order
+  auto is_perm = [](const int32_t* a, int n) {
+    std::vector<char> seen((size_t)n, 0);
+    for (int i = 0; i < n; ++i) {
+      if (a[i] < 0 || a[i] >= n || seen[a[i]]) return false;
+      seen[a[i]] = 1;
+    }
+    return true;
+  };
+  if ((nodeperm && !is_perm(nodeperm, 10)) || (pnodeperm && !is_perm(pnodeperm, 4)) || (ldof && !is_perm(ldof, kND)))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: nodeperm (10), pnodeperm (4) and ldof (34) must be permutations");
+  std::vector<int32_t> perm(kND);
+  for (int c = 0; c < 3; ++c)
+    for (int f = 0; f < 10; ++f) {
+      const int h = c * 10 + (nodeperm ? nodeperm[f] : f);
+      perm[c * 10 + f] = ldof ? ldof[h] : h;
+    }
+  for (int v = 0; v < 4; ++v) {
+    const int h = 30 + (pnodeperm ? pnodeperm[v] : v);
+    perm[30 + v] = ldof ? ldof[h] : h;
+  }
+  std::vector<int32_t> cell_dofs((size_t)ncells * kND);
+  for (int64_t cell = 0; cell < ncells; ++cell)
+    for (int i = 0; i < kND; ++i) cell_dofs[(size_t)cell * kND + i] = host_dofs[(size_t)cell * kND + perm[i]];
   for (int64_t i = 0; i < ncells * kND; ++i) {
     const int32_t id = cell_dofs[i];
     if (id == 0 || id > nfree || -(int64_t)id > ndir)
@@ -811,7 +839,8 @@ int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, con
   op->rho = rho;
   op->mu = mu;
   op->c0 = c0;
-  op->cell_dofs.assign(cell_dofs, cell_dofs + ncells * kND);
+  op->cell_dofs = std::move(cell_dofs);
+  op->perm = std::move(perm);
   if (!greedy_colors(op->cell_dofs.data(), kND, ncells, nfree, op->color))
     return set_err(ctx, HFK_ESTATE, "HyperFEMKernels: more than 128 colours needed");
   const int nc = *std::max_element(op->color.begin(), op->color.end());
@@ -827,7 +856,7 @@ int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, con
     q[4 * i + 3] = qweights[i];
   }
   HIPCHK(ctx, op->ids.alloc((size_t)ncells * kND * sizeof(int32_t)));
-  HIPCHK(ctx, hipMemcpy(op->ids.p, cell_dofs, (size_t)ncells * kND * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(op->ids.p, op->cell_dofs.data(), (size_t)ncells * kND * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(ctx, op->coords.alloc((size_t)ncells * 12 * sizeof(double)));
   HIPCHK(ctx, hipMemcpy(op->coords.p, coords, (size_t)ncells * 12 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(ctx, op->qp.alloc(q.size() * sizeof(double)));
@@ -1021,7 +1050,13 @@ int hfk_oseen_pattern(hfk_oseen* op, int64_t* colptr, int64_t* rowval, int32_t* 
   if ((rc = oseen_pattern(op))) return rc;
   if (colptr) std::copy(op->colptr.begin(), op->colptr.end(), colptr);
   if (rowval) std::copy(op->rowval.begin(), op->rowval.end(), rowval);
-  if (nzpos) std::copy(op->nzpos.begin(), op->nzpos.end(), nzpos);
+  if (nzpos) {  // in the host's local order: [cell][perm[ii] + 34 perm[jj]]
+    const size_t per = (size_t)kND * kND;
+    for (int64_t cell = 0; cell < op->ncells; ++cell)
+      for (int jj = 0; jj < kND; ++jj)
+        for (int ii = 0; ii < kND; ++ii)
+          nzpos[cell * per + op->perm[ii] + kND * op->perm[jj]] = op->nzpos[cell * per + ii + kND * jj];
+  }
   return HFK_OK;
 }
 

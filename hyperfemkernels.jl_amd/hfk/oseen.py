@@ -36,8 +36,14 @@ class OseenOperator:
         self.nfree, self.ndir = int(space.nfree), int(space.ndir)
         cd = np.ascontiguousarray(space.cell_dofs, dtype=np.int32)
         xy = np.ascontiguousarray(space.coords, dtype=np.float64)
+        # the space's local order, when it is not the library's own (tetmesh.reorder_local): the P2 and P1 table
+        # node of each of the library's nodes and the local dof of each (component, table node)
+        order = [getattr(space, k, None) for k in ("nodeperm", "pnodeperm", "ldof")]
+        order = [None if a is None else np.ascontiguousarray(a, dtype=np.int32) for a in order]
         h = C.c_void_p()
-        check(self.lib.hfk_oseen_create(self.ctx.handle, space.ncells, ptr(cd, C.c_int32), ptr(xy, C.c_double), self.np,
+        check(self.lib.hfk_oseen_create(self.ctx.handle, space.ncells, ptr(cd, C.c_int32),
+                                        *[None if a is None else ptr(a, C.c_int32) for a in order],
+                                        ptr(xy, C.c_double), self.np,
                                         ptr(self.qpoints, C.c_double), ptr(self.qweights, C.c_double), self.nfree,
                                         self.ndir, self.rho, self.mu, self.c0, C.byref(h)), self.ctx.handle)
         self.handle = h

@@ -57,6 +57,27 @@ class TetP2P1Space:
         return x0[:, None, :] + np.einsum("qd,cdi->cqi", np.asarray(qpts), J)
 
 
+def reorder_local(space: TetP2P1Space, nodeperm, pnodeperm, ldof) -> TetP2P1Space:
+    """The same space with another local order, as a host whose tables differ from the library's hands it over:
+    ``nodeperm[f]`` the host P2 table node of the library's node f, ``pnodeperm[v]`` the host P1 node of vertex v,
+    ``ldof[c*10 + n]`` / ``ldof[30 + n]`` the cell-local dof of (component c, host P2 node n) / (pressure, host P1
+    node n).  The cell dofs are rewritten in that order and the three tables attached (hfk_oseen_create reads
+    them)."""
+    from copy import copy
+
+    nodeperm, pnodeperm, ldof = (np.asarray(a, dtype=np.int32) for a in (nodeperm, pnodeperm, ldof))
+    perm = np.empty(34, dtype=np.int64)  # perm[i] = host local dof of the library's local dof i
+    for c in range(3):
+        perm[c * 10:(c + 1) * 10] = ldof[c * 10 + nodeperm]
+    perm[30:] = ldof[30 + pnodeperm]
+    out = copy(space)
+    cd = np.empty_like(space.cell_dofs)
+    cd[:, perm] = space.cell_dofs
+    out.cell_dofs = cd
+    out.nodeperm, out.pnodeperm, out.ldof, out.local_perm = nodeperm, pnodeperm, ldof, perm
+    return out
+
+
 def tet_p2p1_space(n, domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)) -> TetP2P1Space:
     nx, ny, nz = (int(v) for v in n)
     x0, x1, y0, y1, z0, z1 = domain

@@ -11,7 +11,8 @@ using SparseArrays
 
 const libhfk = get(ENV, "HFK_LIBHFK", joinpath(@__DIR__, "..", "hfk", "libhfk.so"))
 
-const HFK_STORE = Dict(:full64 => Cint(0), :sym64 => Cint(1), :full32 => Cint(2), :sym32 => Cint(3))
+const HFK_STORE = Dict(:full64 => Cint(0), :sym64 => Cint(1), :full32 => Cint(2), :sym32 => Cint(3), :fac64 => Cint(4),
+                       :fac32 => Cint(5))
 const HFK_LAW = Dict(:svk => Cint(0), :neohookean => Cint(1), :linear => Cint(2))
 
 struct HFKError <: Exception
@@ -343,8 +344,13 @@ set_coarse_network!(g::DevicePMultigrid, net::Union{Nothing,CoarseNetwork}, mode
           g.levels[1].ctx.ptr)
 
 # --- Oseen-BDF2 step on tetrahedra (P2-P1, SUPG/PSPG/LSIC; test/common.jl:165-228) ---------------------------------
-"KernelOperator of the Oseen step on the device: cell_dofs 34 x ncells (Gridap's MultiFieldFESpace ids, velocity
-components then pressure), coords 3 x 4 x ncells, quadrature points 3 x np and weights np of the unit tetrahedron"
+"""KernelOperator of the Oseen step on the device: cell_dofs 34 x ncells (Gridap's MultiFieldFESpace ids in Gridap's
+local order), coords 3 x 4 x ncells, quadrature points 3 x np and weights np of the unit tetrahedron.  The local
+order is Gridap's, passed as tables (0-based): `nodeperm` (10) the P2 table node of each of the library's nodes (the
+cell's vertices in the order of `coords`, then the midpoints of edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)),
+`pnodeperm` (4) the P1 table node of each vertex, `ldof` (34) the cell-local dof of (velocity component c, P2 node n)
+at c*10 + n and of the pressure at P1 node n at 30 + n (FieldTable ldof, src/tables.jl:171-237); `nothing` is the
+library's own order."""
 mutable struct DeviceOseen
     ptr::Ptr{Cvoid}
     ctx::Context
@@ -352,13 +358,16 @@ mutable struct DeviceOseen
     pattern::Union{Nothing,Tuple{Vector{Int},Vector{Int},Matrix{Int32}}}
 end
 function DeviceOseen(ctx::Context, cell_dofs::Matrix{Int32}, coords::Array{Float64,3}, qpoints::Matrix{Float64},
-                     qweights::Vector{Float64}, nfree::Integer, ndir::Integer; rho=1.0, mu=0.01, c0=150.0)
+                     qweights::Vector{Float64}, nfree::Integer, ndir::Integer; rho=1.0, mu=0.01, c0=150.0,
+                     nodeperm::Union{Nothing,Vector{Int32}}=nothing, pnodeperm::Union{Nothing,Vector{Int32}}=nothing,
+                     ldof::Union{Nothing,Vector{Int32}}=nothing)
     r = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:hfk_oseen_create, libhfk), Cint,
-                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Cdouble,
-                 Cdouble, Cdouble, Ptr{Ptr{Cvoid}}),
-                ctx.ptr, size(cell_dofs, 2), cell_dofs, coords, length(qweights), qpoints, qweights, nfree, ndir, rho,
-                mu, c0, r), ctx.ptr)
+    orptr(a) = a === nothing ? Ptr{Int32}(C_NULL) : pointer(a)
+    GC.@preserve nodeperm pnodeperm ldof check(ccall((:hfk_oseen_create, libhfk), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Cint, Ptr{Float64},
+                 Ptr{Float64}, Int64, Int64, Cdouble, Cdouble, Cdouble, Ptr{Ptr{Cvoid}}),
+                ctx.ptr, size(cell_dofs, 2), cell_dofs, orptr(nodeperm), orptr(pnodeperm), orptr(ldof), coords,
+                length(qweights), qpoints, qweights, nfree, ndir, rho, mu, c0, r), ctx.ptr)
     op = DeviceOseen(r[], ctx, nfree, nothing)
     finalizer(o -> ccall((:hfk_oseen_destroy, libhfk), Cint, (Ptr{Cvoid},), o.ptr), op)
     op

@@ -334,16 +334,23 @@ int hfk_krylov_solve_dev(hfk_op* op, hfk_pmg* pmg, int kind, int precond, const 
  * assembly of the residual and the CSC Jacobian, as residual!/jacobian!
  * (src/assembly.jl:909-927) do for that form, with the reference's sparsity (src/assembly.jl:773-899) and colouring
  * (src/coloring.jl:4-28) over the 34 dofs of each cell (velocity and pressure blocks all coupled).
- *   cell_dofs  ncells x 34 signed ids: [c*10 + a] velocity component c at P2 node a (vertices 0..3, then the
- *              midpoints of the edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)), [30 + a] pressure at vertex a;
- *              > 0 free (1-based, velocity and pressure in one numbering), < 0 Dirichlet (index into dvals)
+ *   cell_dofs  ncells x 34 signed ids in the host's local order (Gridap's get_cell_dof_ids of the multi-field
+ *              space, src/tables.jl:329-342); > 0 free (1-based, velocity and pressure in one numbering), < 0
+ *              Dirichlet (index into dvals)
+ *   nodeperm   10: the host's P2 table node of the library's node f, where f = 0..3 are the cell's vertices in the
+ *              order of `coords` and 4..9 the midpoints of their edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+ *   pnodeperm  4: the host's P1 (pressure) table node of vertex v
+ *   ldof       34: ldof[c*10 + n] = local dof of velocity component c at host P2 node n, ldof[30 + n] = local dof
+ *              of the pressure at host P1 node n (FieldTable ldof, src/tables.jl:171-237).  NULL for any of the three
+ *              arrays means the identity (velocity dofs [c*10 + f], pressure [30 + v])
  *   coords     ncells x 4 x 3 vertex coordinates (affine cells)
  *   qpoints    np x 3 reference coordinates of the quadrature points on the unit tetrahedron, qweights np weights
  *   rho, mu    density and dynamic viscosity; c0 the time-derivative coefficient (1/dt Euler, 1.5/dt BDF2)
  * Point data (hfk_oseen_set_fields): ncells x np x 9 = advecting velocity w, velocity un of the stabilisation
  * parameters, history term f. */
 typedef struct hfk_oseen hfk_oseen;
-int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, const double* coords, int np,
+int hfk_oseen_create(hfk_ctx* ctx, int64_t ncells, const int32_t* cell_dofs, const int32_t* nodeperm,
+                     const int32_t* pnodeperm, const int32_t* ldof, const double* coords, int np,
                      const double* qpoints, const double* qweights, int64_t nfree, int64_t ndir, double rho, double mu,
                      double c0, hfk_oseen** out);
 int hfk_oseen_destroy(hfk_oseen* op);
@@ -386,7 +393,8 @@ int hfk_oseen_set_deterministic(hfk_oseen* op, int deterministic);
 /* info[6]: nfree, ndir, ncells, np, ncolors, nnz (-1 before the pattern) */
 int hfk_oseen_info(hfk_oseen* op, int64_t* info);
 int hfk_oseen_colors(hfk_oseen* op, int32_t* colors);
-/* sparsity_pattern / nonzero_positions (src/assembly.jl:768-899): 1-based CSC and nzpos [cell][ii + 34 jj] */
+/* sparsity_pattern / nonzero_positions (src/assembly.jl:768-899): 1-based CSC and nzpos [cell][ii + 34 jj], ii and jj
+ * the host's local dofs (the order of the cell_dofs given to hfk_oseen_create) */
 int hfk_oseen_pattern_nnz(hfk_oseen* op, int64_t* nnz);
 int hfk_oseen_pattern(hfk_oseen* op, int64_t* colptr, int64_t* rowval, int32_t* nzpos);
 /* residual! (src/assembly.jl:909-916): b = a(x, .) - l(.) on the free test dofs, Dirichlet values lifted */

@@ -250,3 +250,71 @@ def test_oseen_two_faces_of_one_cell(hip_ctx):
     x = rng.standard_normal(sp.nfree)
     assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, Kt, lt)) < TOL
     assert rel(op.jacobian(op.allocate_jacobian()).data, O.jacobian_values(Kt, nzpos, rowval.size)) < TOL
+
+
+def test_oseen_host_local_order(hip_ctx):
+    """The host's local order (Gridap's tables, which the library cannot see): a 3 x 2 x 2 x 6 tet space whose cell
+    dofs come in a shuffled P2 node order, a shuffled P1 order and a layout with the pressure dofs first and the
+    velocity dofs node-major, handed over with its tables.  Pattern, nzpos (in the host's local order) and colours
+    equal the oracle's on those same host-ordered ids bit for bit, and every nzpos entry addresses the CSC entry
+    (row of test dof ii, column of trial dof jj); residual and CSC values equal the oracle's (1e-12), also with the
+    step's point data made from FE functions and the outlet terms."""
+    from oracle import oseen as O
+
+    from hfk.tetmesh import outlet_faces, reorder_local
+
+    sp = tet_p2p1_space((3, 2, 2), domain=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0))
+    rng = np.random.default_rng(23)
+    nodeperm, pnodeperm = rng.permutation(10), rng.permutation(4)
+    ldof = np.empty(34, dtype=np.int64)
+    for n in range(10):
+        for c in range(3):
+            ldof[c * 10 + n] = 4 + 3 * n + c
+    ldof[30:] = np.arange(4)
+    hs = reorder_local(sp, nodeperm, pnodeperm, ldof)
+    assert not np.array_equal(hs.cell_dofs, sp.cell_dofs)
+    qp, qw = tet_quadrature_deg5()
+    fields = rng.standard_normal((sp.ncells, len(qw), 9))
+    dv = rng.standard_normal(sp.ndir)
+    op = OseenOperator(hs, rho=1.0, mu=0.05, dt=0.01, bdf2=True, dirichlet_values=dv, ctx=hip_ctx)
+    op.set_fields(fields[..., 0:3], fields[..., 3:6], fields[..., 6:9])
+    colptr, rowval, nzpos = op.pattern()
+    c2, r2, p2 = O.pattern(hs.cell_dofs, hs.nfree)
+    assert np.array_equal(colptr, c2) and np.array_equal(rowval, r2) and np.array_equal(nzpos, p2)
+    assert np.array_equal(op.colors(), O.greedy_colors(hs.cell_dofs, hs.nfree))
+    nz = nzpos.reshape(sp.ncells, 34, 34, order="F")  # nz[cell, ii, jj] = nzpos[cell, ii + 34 jj]
+    col_of = np.repeat(np.arange(colptr.size - 1), np.diff(colptr))
+    for cell in range(0, sp.ncells, 7):
+        ids = hs.cell_dofs[cell]
+        for ii in range(34):
+            for jj in range(34):
+                p = nz[cell, ii, jj]
+                if ids[ii] > 0 and ids[jj] > 0:
+                    assert p > 0 and rowval[p - 1] == ids[ii] and col_of[p - 1] + 1 == ids[jj]
+                else:
+                    assert p == 0
+    # values: the oracle in the library's own order (global quantities do not depend on the local order)
+    _, _, p_std = O.pattern(sp.cell_dofs, sp.nfree)
+    K, load = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.05, op.c0)
+    A = op.jacobian(op.allocate_jacobian())
+    assert rel(A.data, O.jacobian_values(K, p_std, r2.size)) < TOL
+    x = rng.standard_normal(sp.nfree)
+    assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, K, load)) < TOL
+    # the step's point data from FE functions and the outlet terms, against the operator in the library's order
+    ref = OseenOperator(sp, rho=1.0, mu=0.05, dt=0.01, bdf2=True, dirichlet_values=dv, ctx=hip_ctx)
+    of = outlet_faces(sp)
+    un, unm1 = rng.standard_normal(sp.nfree), rng.standard_normal(sp.nfree)
+    und, unm1d = rng.standard_normal(sp.ndir), rng.standard_normal(sp.ndir)
+    for o in (op, ref):
+        o.set_outlet(of, None, Pc=0.7, ZA=2.3)
+        o.step_fields(un, und, unm1, unm1d)
+    assert rel(op.residual(np.zeros(sp.nfree), x), ref.residual(np.zeros(sp.nfree), x)) < 1e-13
+    assert rel(op.jacobian(op.allocate_jacobian()).data, ref.jacobian(ref.allocate_jacobian()).data) < 1e-13
+    from copy import copy
+
+    from hfk._lib import HFKError
+
+    bad = copy(hs)
+    bad.nodeperm = np.zeros(10, dtype=np.int32)  # not a permutation
+    with pytest.raises(HFKError, match="ArgumentError"):
+        OseenOperator(bad, ctx=hip_ctx)
