@@ -778,6 +778,17 @@ int net_backward_vec(hfk_net* nt, const double* gw, double sc, hipStream_t s) {
   return backward(nt, s);
 }
 
+int net_zero_grad(hfk_net* nt, hipStream_t s) {
+  HIPCHK(nt->ctx, hipMemsetAsync(nt->grad.p, 0, nt->nparam * sizeof(float), s));
+  return HFK_OK;
+}
+
+int net_copy_grad(hfk_net* nt, float* out, hipStream_t s) {
+  HIPCHK(nt->ctx, hipMemcpyAsync(out, nt->grad.p, nt->nparam * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(nt->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
 int net_adam(hfk_net* nt, double lr, int batch, hipStream_t s) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   nt->step++;

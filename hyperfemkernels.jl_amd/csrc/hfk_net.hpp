@@ -17,4 +17,7 @@ int net_forward_vec(hfk_net* nt, const double* r, double* w, hipStream_t s);
 int net_backward_vec(hfk_net* nt, const double* gw, double sc, hipStream_t s);
 // one Adam step on the gradient accumulated over `batch` samples (cleared afterwards)
 int net_adam(hfk_net* nt, double lr, int batch, hipStream_t s);
+// clear the accumulated weight gradient; copy it to host memory (FP32, num_params values)
+int net_zero_grad(hfk_net* nt, hipStream_t s);
+int net_copy_grad(hfk_net* nt, float* out, hipStream_t s);
 }  // namespace hfk

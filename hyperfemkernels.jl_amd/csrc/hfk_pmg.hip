@@ -1502,6 +1502,33 @@ int hfk_pmg_coarse_pairs_dev(hfk_pmg* g, const double* b_dev, int nsamples, doub
 // L(w) = w'(A/h) w - 2 w'x, x = r/s, whose minimiser is the coarse solve in the network's units (z = (s/h) w
 // = A^-1 r); in the scaled mode A/h -> D^-1/2 A D^-1/2 and r -> D^-1/2 r (pmg_net_apply).  Needs one product
 // with the assembled coarse matrix per sample and no exact solves.
+// one sample of the energy loss: w = N(r), q = (A/h) w - r/s, the weight gradient of L = (w'(A/h)w - 2 w'x)/n
+// accumulated into the network's gradient; *L receives the sample's loss
+int energy_sample(hfk_pmg* g, hfk_net* net, const double* r, DevBuf& w, DevBuf& q, double* L, hipStream_t s) {
+  const int64_t n = g->ops.back()->nfree;
+  const double invh = g->net_scaled ? 1.0 : 1.0 / net_h(net);  // the loss's operator: D^-1/2 A D^-1/2 or A/h
+  int rc;
+  if (g->net_scaled) {  // train on D^-1/2 A D^-1/2 and D^-1/2 r
+    HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), r, g->cv.as<double>(), s));
+    r = g->cv.as<double>();
+  }
+  if ((rc = net_forward_vec(net, r, w.as<double>(), s))) return rc;
+  const double* wa = w.as<double>();
+  if (g->net_scaled) {
+    HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), w.as<double>(), g->cu.as<double>(), s));
+    wa = g->cu.as<double>();
+  }
+  HIPCHK(g->ctx, coarse_spmv(g, wa, q.as<double>(), invh, 0.0, s));
+  if (g->net_scaled) HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), q.as<double>(), q.as<double>(), s));
+  double h3[3];
+  if ((rc = g->dots.run(g->ctx, s, n, w.as<double>(), q.as<double>(), h3, w.as<double>(), r, r, r))) return rc;
+  const double sc = std::sqrt(h3[2] / (double)n);
+  const double sinv = sc > 0 ? 1.0 / sc : 1.0;
+  *L = (h3[0] - 2.0 * h3[1] * sinv) / (double)n;
+  HIPCHK(g->ctx, vec_axpby(n, -sinv, r, 1.0, q.as<double>(), s));  // q = (A/h) w - x
+  return net_backward_vec(net, q.as<double>(), 2.0 / (double)n, s);
+}
+
 int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, int steps, int batch,
                                  double lr, uint64_t seed, double* history, void* stream) {
   if (!g || !net || !bc_dev || nsamples < 1 || steps < 0 || batch < 1 || !(lr > 0))
@@ -1514,38 +1541,47 @@ int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev,
   hipStream_t s = g->ops[0]->stream(stream);
   DevBuf w, q;
   if ((rc = vec_alloc(g->ctx, w, n)) || (rc = vec_alloc(g->ctx, q, n))) return rc;
-  const double invh = g->net_scaled ? 1.0 : 1.0 / net_h(net);  // the loss's operator: D^-1/2 A D^-1/2 or A/h
   uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
   for (int step = 0; step < steps; ++step) {
     double Lsum = 0;
     for (int b = 0; b < batch; ++b) {
       st = st * 6364136223846793005ull + 1442695040888963407ull;
       const int k = (int)((st >> 33) % (uint64_t)nsamples);
-      const double* r = bc_dev + (size_t)k * n;
-      if (g->net_scaled) {  // train on D^-1/2 A D^-1/2 and D^-1/2 r
-        HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), r, g->cv.as<double>(), s));
-        r = g->cv.as<double>();
-      }
-      if ((rc = net_forward_vec(net, r, w.as<double>(), s))) return rc;
-      const double* wa = w.as<double>();
-      if (g->net_scaled) {
-        HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), w.as<double>(), g->cu.as<double>(), s));
-        wa = g->cu.as<double>();
-      }
-      HIPCHK(g->ctx, coarse_spmv(g, wa, q.as<double>(), invh, 0.0, s));
-      if (g->net_scaled) HIPCHK(g->ctx, vec_mul(n, g->cdscale.as<double>(), q.as<double>(), q.as<double>(), s));
-      double h3[3];
-      if ((rc = g->dots.run(g->ctx, s, n, w.as<double>(), q.as<double>(), h3, w.as<double>(), r, r, r))) return rc;
-      const double sc = std::sqrt(h3[2] / (double)n);
-      const double sinv = sc > 0 ? 1.0 / sc : 1.0;
-      Lsum += (h3[0] - 2.0 * h3[1] * sinv) / (double)n;
-      HIPCHK(g->ctx, vec_axpby(n, -sinv, r, 1.0, q.as<double>(), s));  // q = (A/h) w - x
-      if ((rc = net_backward_vec(net, q.as<double>(), 2.0 / (double)n, s))) return rc;
+      double L;
+      if ((rc = energy_sample(g, net, bc_dev + (size_t)k * n, w, q, &L, s))) return rc;
+      Lsum += L;
     }
     if ((rc = net_adam(net, lr, batch, s))) return rc;
     if (history) history[step] = Lsum / batch;
   }
   HIPCHK(g->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+// The energy loss summed over the samples bc_k (device) and its weight gradient (host, FP32), without a training
+// step: the quantity hfk_pmg_train_coarse_network descends, for checking it against another implementation.
+int hfk_pmg_coarse_network_energy_gradient(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, float* grad,
+                                           double* loss, void* stream) {
+  if (!g || !net || !bc_dev || nsamples < 1)
+    return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: invalid arguments");
+  if (!g->built || !g->colptr.p) return set_err(g->ctx, HFK_ESTATE, "HyperFEMKernels: p-multigrid not updated");
+  const int64_t n = g->ops.back()->nfree;
+  if (net_ncoarse(net) != n) return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: the network is not built for this coarse level");
+  int rc = op_bind(g->ops[0]);
+  if (rc) return rc;
+  hipStream_t s = g->ops[0]->stream(stream);
+  DevBuf w, q;
+  if ((rc = vec_alloc(g->ctx, w, n)) || (rc = vec_alloc(g->ctx, q, n))) return rc;
+  if ((rc = net_zero_grad(net, s))) return rc;
+  double total = 0;
+  for (int k = 0; k < nsamples; ++k) {
+    double L;
+    if ((rc = energy_sample(g, net, bc_dev + (size_t)k * n, w, q, &L, s))) return rc;
+    total += L;
+  }
+  if (grad && (rc = net_copy_grad(net, grad, s))) return rc;
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
+  if (loss) *loss = total;
   return HFK_OK;
 }
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "hfk_pmg_train_coarse_network": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p,
                                                _vp]),
     "hfk_pmg_coarse_network_error": (C.c_int, [_vp, _vp, _vp, C.c_int, _f64p, _vp]),
+    "hfk_pmg_coarse_network_energy_gradient": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _f64p, _vp]),
     "hfk_oseen_create": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _i32p, _i32p, _f64p, C.c_int, _f64p, _f64p, C.c_int64,
                                    C.c_int64, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]),
     "hfk_oseen_destroy": (C.c_int, [_vp]),

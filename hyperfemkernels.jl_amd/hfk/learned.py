@@ -144,6 +144,16 @@ def train_energy_problems(problems, net: "CoarseNetwork", steps: int, chunk: int
     return np.concatenate(hist) if hist else np.zeros(0)
 
 
+def energy_gradient_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, stream: int = 0):
+    """The energy loss the training descends, summed over the device samples, and its weight gradient (FP32)."""
+    g = np.zeros(net.nparams, dtype=np.float32)
+    loss = C.c_double()
+    check(pmg.lib.hfk_pmg_coarse_network_energy_gradient(pmg.handle, net.handle, C.c_void_p(bc_ptr), nsamples,
+                                                         g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss),
+                                                         C.c_void_p(stream)), pmg.levels[0].ctx.handle)
+    return loss.value, g
+
+
 def coarse_error_dev(pmg, net: "CoarseNetwork", bc_ptr: int, nsamples: int, stream: int = 0) -> float:
     """Mean relative energy-norm error of the network's coarse correction (against the coarse solve)."""
     err = C.c_double()

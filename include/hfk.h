@@ -311,6 +311,10 @@ int hfk_net_set_aux(hfk_net* net, const float* aux);
  * minimiser is the coarse solve; one product with the assembled coarse matrix per sample, no exact solves. */
 int hfk_pmg_train_coarse_network(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, int steps, int batch,
                                  double lr, uint64_t seed, double* history, void* stream);
+/* The energy loss of hfk_pmg_train_coarse_network summed over the samples bc_k (device) and its weight gradient
+ * (host FP32, hfk_net_num_params values; may be NULL), without a training step (no counterpart in the reference). */
+int hfk_pmg_coarse_network_energy_gradient(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, float* grad,
+                                           double* loss, void* stream);
 /* Mean relative energy-norm error of the network's coarse correction against the coarse solve to 1e-10. */
 int hfk_pmg_coarse_network_error(hfk_pmg* g, hfk_net* net, const double* bc_dev, int nsamples, double* err,
                                  void* stream);

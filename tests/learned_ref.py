@@ -93,3 +93,41 @@ def loss_and_grad(w, node_dofs, N, h, r, e, coarsen=False):
     L.backward()
     g = torch.cat([W.grad.permute(2, 3, 4, 0, 1).reshape(-1) for W in Ws]).numpy()
     return float(L.detach()), g, out.detach().numpy()
+
+
+def energy_loss_and_grad(w, node_dofs, N, h, A, r, coarsen=False):
+    """The energy loss of the device training, L = (w'(A/h)w - 2 w'x) / n with x = r/s (s = RMS(r)) and w the
+    network's output on the free dofs (n of them), and its gradient with respect to the weights.  A is the
+    assembled coarse matrix (scipy sparse, free dofs)."""
+    Ws = [t.clone().requires_grad_(True) for t in unflatten(w)]
+    n = r.size
+    s = np.sqrt(np.dot(r, r) / n)
+    x0 = np.zeros((16, N ** 3), dtype=np.float32)
+    for c in range(3):
+        ids = node_dofs[:, c]
+        f = ids > 0
+        x0[c, f] = r[ids[f] - 1] / s
+    x0t = bf(torch.tensor(x0).reshape(1, 16, N, N, N))
+    if coarsen:
+        out = prolong(forward(Ws, bf(restrict(x0t)))[:, :3])[0].reshape(3, -1)
+    else:
+        out = forward(Ws, x0t)[0, :3].reshape(3, -1)
+    # gather the output at the free dofs: wv[id - 1] = out[c, vertex]
+    idx_c, idx_v, idx_d = [], [], []
+    for c in range(3):
+        ids = node_dofs[:, c]
+        f = np.nonzero(ids > 0)[0]
+        idx_c.append(np.full(f.size, c))
+        idx_v.append(f)
+        idx_d.append(ids[f] - 1)
+    idx_c, idx_v, idx_d = (np.concatenate(a) for a in (idx_c, idx_v, idx_d))
+    order = np.argsort(idx_d)
+    wv = out[torch.tensor(idx_c[order]), torch.tensor(idx_v[order])].to(torch.float64)
+    Ac = A.tocoo()
+    At = torch.sparse_coo_tensor(np.vstack([Ac.row, Ac.col]), Ac.data / h, (n, n)).coalesce()
+    Aw = torch.sparse.mm(At, wv[:, None])[:, 0]
+    x = torch.tensor(r / s)
+    L = (wv @ Aw - 2.0 * (wv @ x)) / n
+    L.backward()
+    g = torch.cat([W.grad.permute(2, 3, 4, 0, 1).reshape(-1) for W in Ws]).numpy()
+    return float(L.detach()), g

@@ -237,10 +237,10 @@ def test_mu_function_of_x(oracle, hip_ctx):
 
 # ---- update with the same structure: numbers rebound, nothing rebuilt (src/assembly.jl:693-744) ----------------
 
-def test_update_fast_path(hip_ctx):
+def test_update_fast_path(hip_ctx, oracle):
     """update(op, res, jac) with other lam, mu: the same operator object with its pattern, the compiled law's module
-    kept, and residual, tangents (Jv) and assembled values bitwise equal to a fresh operator of the new numbers;
-    another law builds a new operator."""
+    kept, and residual, tangents (Jv) and assembled values bitwise equal to a fresh operator of the new numbers and
+    equal to the oracle of lam = 3, mu = 0.5 (1e-12); another law builds a new operator."""
     sp, dv, x, rng = problem(2, (3, 2, 2), 0.1, seed=21)
     f1, f2 = HyperelasticForm("neohookean", 2.0, 1.0, degree=4), HyperelasticForm("neohookean", 3.0, 0.5, degree=4)
     law = CompiledLaw.builtin("neohookean", ctx=hip_ctx)
@@ -260,5 +260,11 @@ def test_update_fast_path(hip_ctx):
         assert not np.array_equal(A2.data, A1.data)
         y = MatrixFreeJacobian(op, "full64").prepare(x) @ dx
         assert np.array_equal(y, MatrixFreeJacobian(ref, "full64").prepare(x) @ dx)
+        orc = oracle.OracleOp(sp, "neohookean", 3.0, 0.5, dvals=dv)
+        rl = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rl(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < 1e-12
+        assert rl(y, orc.jv(orc.prepare(x), dx)) < 1e-12
+        _, rowval, nzpos = orc.pattern()
+        assert rl(A2.data, orc.jacobian(x, nzpos, rowval.size)) < 1e-12
     op2 = op.update(HyperelasticForm("svk", 3.0, 0.5, degree=4))
     assert op2 is not op and op2.form.law == "svk"

@@ -40,9 +40,11 @@ def test_network_gradient_matches_torch(hip_ctx, coarsen):
     assert loss2 == loss and np.array_equal(g2, g)
     Lr, gr, out = learned_ref.loss_and_grad(w, cs.lattice_dofs(), 17, 1.0 / 16, r, e, coarsen=coarsen)
     print(f"loss gpu {loss:.6e} torch {Lr:.6e}; grad rel {rel(g, gr):.3e}", flush=True)
-    assert loss == pytest.approx(Lr, rel=2e-2)
+    # BF16 activations and weights with FP32 accumulation on both sides; the two differ only by summation order
+    # (measured: loss 2e-5, weight gradient 1.7e-3 to 1.8e-3 relative)
+    assert loss == pytest.approx(Lr, rel=1e-3)
     cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
-    assert cos > 0.995 and rel(g, gr) < 0.08
+    assert cos > 0.99998 and rel(g, gr) < 5e-3
     # the forward output through apply: z = (s/h) out on the free dofs
     z = net.apply(r)
     s = np.sqrt(r @ r / r.size)
@@ -52,6 +54,46 @@ def test_network_gradient_matches_torch(hip_ctx, coarsen):
         f = ld[:, c] > 0
         zr[ld[f, c] - 1] = out[c, f] * s * 16
     assert rel(z, zr) < 2e-2
+
+
+@pytest.mark.parametrize("coarsen", [False, True])
+def test_energy_loss_gradient_matches_torch(hip_ctx, oracle, coarsen):
+    """The training objective itself: the energy loss L = (w'(A/h)w - 2 w'x)/n (x = r/s) of the device training,
+    summed over three coarse samples, and its weight gradient, against the torch restatement with the coarse matrix
+    of the oracle (the hierarchy at the zero state: the coarse matrix is the oracle's Jacobian at zero with the
+    fine rule)."""
+    import torch
+
+    import learned_ref
+    from hfk.learned import energy_gradient_dev
+
+    fs = hex_space((16, 16, 16), 2, dirichlet=dirichlet_closed_x0)
+    cs = hex_space((16, 16, 16), 1, dirichlet=dirichlet_closed_x0)
+    form = HyperelasticForm("neohookean", degree=4)
+    fop = KernelOperator(form, fs, jacobian="matrixfree", ctx=hip_ctx)
+    cop = KernelOperator(form, cs, jacobian="assembled", ctx=hip_ctx)
+    MatrixFreeJacobian(fop).prepare(np.zeros(fs.nfree))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=3)).update()
+    net = CoarseNetwork(hip_ctx, cs, seed=5, coarsen=coarsen)
+    set_coarse_network(pmg, net, "cycle", scaled=False)
+    rng = np.random.default_rng(2)
+    bc = rng.standard_normal((3, cs.nfree))
+    bcd = torch.tensor(bc, device="cuda")
+    torch.cuda.synchronize()
+    loss, g = energy_gradient_dev(pmg, net, bcd.data_ptr(), 3)
+    loss2, g2 = energy_gradient_dev(pmg, net, bcd.data_ptr(), 3)
+    assert loss2 == loss and np.array_equal(g2, g)  # fixed-order reductions
+    A = oracle.OracleOp(cs, "neohookean", 2.0, 1.0, points1d=3).csc(np.zeros(cs.nfree))
+    w = net.weights()
+    Lr, gr = 0.0, np.zeros_like(g, dtype=np.float64)
+    for k in range(3):
+        lk, gk = learned_ref.energy_loss_and_grad(w, cs.lattice_dofs(), 17, 1.0 / 16, A, bc[k], coarsen=coarsen)
+        Lr += lk
+        gr += gk
+    print(f"energy loss gpu {loss:.8e} torch {Lr:.8e}; grad rel {rel(g, gr):.3e}", flush=True)
+    assert loss == pytest.approx(Lr, rel=1e-3)
+    cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
+    assert cos > 0.99998 and rel(g, gr) < 5e-3
 
 
 def make_problem(hip_ctx, n, seed=0, mu_sigma=0.5):

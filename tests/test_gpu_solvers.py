@@ -387,3 +387,77 @@ def test_indefinite_tangent_reported(hip_ctx):
     assert not pmg.ok and pmg.indefinite
     res = MatrixFreeSolver("fcg", rtol=1e-8, maxiter=50, preconditioner=pmg).solve(np.zeros(fs.nfree), mf, np.ones(fs.nfree))
     assert not res.converged and res.indefinite and res.iterations == 0
+
+
+def test_vcycle_symmetric_positive_definite(hip_ctx):
+    """The V-cycle is a symmetric positive definite operator on an SPD tangent (test/solvers/test_pmultigrid.jl:74-84):
+    Q2 -> Q1 on 2^3 cells at the zero state, coarse level solved to 1e-14 (a fixed operator up to that tolerance);
+    the matrix B of the cycle, column by column: ||B - B'|| <= 1e-10 ||B|| and the smallest eigenvalue of its
+    symmetric part is positive."""
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (2, 2, 2), amp=0.0)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=2), coarse_rtol=1e-14, coarse_maxiter=5000).update()
+    assert pmg.ok
+    n = fs.nfree
+    B = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        B[:, j] = pmg.ldiv(np.zeros(n), e)
+    asym = np.linalg.norm(B - B.T) / np.linalg.norm(B)
+    lmin = np.linalg.eigvalsh(0.5 * (B + B.T)).min()
+    print(f"V-cycle: ||B - B'||/||B|| = {asym:.2e}, min eig {lmin:.3e}", flush=True)
+    assert asym <= 1e-10 and lmin > 0
+
+
+def q3_levels(hip_ctx, n, orders):
+    """Levels of orders (3, 1) or (3, 2, 1) on n^3 cells clamped on x = 0, zero state; every coarser level keeps the
+    fine 4-point rule (src/pmultigrid.jl:248-251); the last one assembled."""
+    ops = []
+    for i, o in enumerate(orders):
+        sp = hex_space((n, n, n), o, dirichlet=dirichlet_closed_x0)
+        kind = "assembled" if i == len(orders) - 1 else "matrixfree"
+        ops.append((sp, KernelOperator(HyperelasticForm("neohookean", degree=6), sp, jacobian=kind, ctx=hip_ctx)))
+    return ops
+
+
+@pytest.mark.parametrize("orders", [(3, 1), (3, 2, 1)])
+def test_q3_hierarchies(hip_ctx, oracle, orders):
+    """Q3 -> Q1 and Q3 -> Q2 -> Q1 (test/solvers/test_pmultigrid.jl:53-66): GMRES(rtol 1e-10) with one V-cycle per
+    iteration converges on n = 2 and 4 to the direct solution (1e-7), within 40 iterations, nearly h-independent
+    (its(4) <= 1.5 its(2) + 3); the coarse level solved to 1e-13 so that the cycle is a fixed operator."""
+    import scipy.sparse.linalg as spla
+
+    its = []
+    for n in (2, 4):
+        lv = q3_levels(hip_ctx, n, orders)
+        fs, fop = lv[0]
+        mf = MatrixFreeJacobian(fop).prepare(np.zeros(fs.nfree))
+        pmg = PMultigrid([op for _, op in lv], Chebyshev(degree=4), coarse_rtol=1e-13, coarse_maxiter=5000).update()
+        assert pmg.ok
+        b = np.random.default_rng(n + 30).standard_normal(fs.nfree)
+        xs = np.zeros(fs.nfree)
+        res = MatrixFreeSolver("gmres", rtol=1e-10, maxiter=200, preconditioner=pmg).solve(xs, mf, b)
+        A = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, points1d=4).csc(np.zeros(fs.nfree))
+        xd = spla.spsolve(A.tocsc(), b)
+        print(f"{orders} n={n}: {res}", flush=True)
+        assert res.converged and rel(xs, xd) < 1e-7
+        its.append(res.iterations)
+    assert max(its) <= 40 and its[1] <= 1.5 * its[0] + 3
+
+
+def test_plain_cg_with_symmetric_cycle(hip_ctx, oracle):
+    """CG (not flexible) with the symmetric cycle (test/solvers/test_pmultigrid.jl:67-71): Q2 -> Q1 on 4^3 cells,
+    the coarse level solved to 1e-13: converged, the direct solution to 1e-8."""
+    import scipy.sparse.linalg as spla
+
+    fs, fop, cop, mf, x = make_levels(hip_ctx, (4, 4, 4))
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-13, coarse_maxiter=5000).update()
+    b = np.random.default_rng(41).standard_normal(fs.nfree)
+    xs = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("cg", rtol=1e-10, maxiter=200, preconditioner=pmg).solve(xs, mf, b)
+    from hfk.synthetic import smooth_field
+
+    orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+    print(f"cg + symmetric cycle: {res}", flush=True)
+    assert res.converged and not res.indefinite
+    assert rel(xs, spla.spsolve(orc.csc(x).tocsc(), b)) < 1e-8

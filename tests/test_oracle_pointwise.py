@@ -149,3 +149,77 @@ def test_neohookean_closed_form(oracle):
         Fit = np.linalg.inv(F).T
         Pm = 1.0 * (F - Fit) + 2.0 * np.log(np.linalg.det(F)) * Fit
         assert np.allclose(oracle.law_P("neohookean", 2.0, 1.0, g).reshape(3, 3), Pm, atol=1e-13)
+
+
+@pytest.mark.parametrize("law", ["svk", "neohookean", "linear"])
+def test_energy_vanishes_at_identity(oracle, law):
+    """The absolute energy constant: W(F = I) = 0 for every law (the stress-free reference state), and the
+    hyperelastic laws are objective: W(R) = 0 for a rigid rotation R."""
+    assert oracle.law_W(law, 2.0, 1.0, np.zeros(9)) == 0.0
+    if law != "linear":
+        th = 0.7
+        R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+        g = (R - np.eye(3)).reshape(-1)  # g[J + 3c] = F[c, J] - delta
+        assert abs(oracle.law_W(law, 2.0, 1.0, g)) < 1e-14
+
+
+def test_energy_closed_forms(oracle):
+    """W of St Venant-Kirchhoff, lam/2 tr(E)^2 + mu E:E (test/common.jl:2-8 integrated), of the neo-Hookean law
+    mu/2 (I1 - 3) - mu ln J + lam/2 (ln J)^2 and of the linear law lam/2 tr(eps)^2 + mu eps:eps, in numpy."""
+    rng = np.random.default_rng(5)
+    lam, mu = 2.0, 1.3
+    for _ in range(5):
+        g = _rand_g(rng, 0.2)
+        F = _F(g)
+        E = 0.5 * (F.T @ F - np.eye(3))
+        assert oracle.law_W("svk", lam, mu, g) == pytest.approx(0.5 * lam * np.trace(E) ** 2 + mu * np.sum(E * E),
+                                                                rel=1e-13)
+        J = np.linalg.det(F)
+        wn = 0.5 * mu * (np.sum(F * F) - 3) - mu * np.log(J) + 0.5 * lam * np.log(J) ** 2
+        assert oracle.law_W("neohookean", lam, mu, g) == pytest.approx(wn, rel=1e-12)
+        eps = 0.5 * (F - np.eye(3) + (F - np.eye(3)).T)
+        assert oracle.law_W("linear", lam, mu, g) == pytest.approx(0.5 * lam * np.trace(eps) ** 2 + mu * np.sum(eps * eps),
+                                                                   rel=1e-13)
+
+
+@pytest.mark.parametrize("law", ["neohookean", "linear"])
+def test_factorised_tangent_form(oracle, law):
+    """The factorised tangent storage rests on T[(J,c),(L,k)] = mu d_ck d_JL + m A_Jk A_Lc + lam A_Jc A_Lk with
+    A = F^-1 and m = mu - lam ln J (neo-Hookean) or A = I and m = mu (linear); and, in reference coordinates with
+    H = iJt' A and g = iJt' iJt, D = dV iJt' T iJt = dV (mu g_KM d_ck + m H_Kk H_Mc + lam H_Kc H_Mk)."""
+    rng = np.random.default_rng(6)
+    lam, mu = 2.0, 1.3
+    for _ in range(4):
+        g = _rand_g(rng, 0.2)
+        F = _F(g)
+        A = np.linalg.inv(F) if law == "neohookean" else np.eye(3)
+        m = mu - lam * np.log(np.linalg.det(F)) if law == "neohookean" else mu
+        d = np.eye(3)
+        T = oracle.law_T(law, lam, mu, g).reshape(9, 9, order="F")  # T[J + 3c, L + 3k]
+        Tf = np.zeros((9, 9))
+        for c in range(3):
+            for J in range(3):
+                for k in range(3):
+                    for L in range(3):
+                        Tf[J + 3 * c, L + 3 * k] = (mu * d[c, k] * d[J, L] + m * A[J, k] * A[L, c]
+                                                    + lam * A[J, c] * A[L, k])
+        assert np.allclose(Tf, T, atol=1e-13, rtol=1e-13)
+        iJt = np.eye(3) + 0.3 * rng.standard_normal((3, 3))  # iJt[J, K]
+        dV = 0.37
+        D = np.zeros((9, 9))
+        for c in range(3):
+            for K in range(3):
+                for k in range(3):
+                    for M in range(3):
+                        D[K + 3 * c, M + 3 * k] = dV * sum(iJt[J, K] * T[J + 3 * c, L + 3 * k] * iJt[L, M]
+                                                           for J in range(3) for L in range(3))
+        H = iJt.T @ A
+        G = iJt.T @ iJt
+        Df = np.zeros((9, 9))
+        for c in range(3):
+            for K in range(3):
+                for k in range(3):
+                    for M in range(3):
+                        Df[K + 3 * c, M + 3 * k] = dV * (mu * G[K, M] * d[c, k] + m * H[K, k] * H[M, c]
+                                                         + lam * H[K, c] * H[M, k])
+        assert np.allclose(Df, D, atol=1e-13, rtol=1e-12)
