@@ -164,4 +164,11 @@ int robin_assemble(hfk_op* op, double* nzval, hipStream_t s);
 int rtc_energy(hfk_op* op, const double* x_dev, double* part, hipStream_t s);
 // true when the operator's compiled law already has its kernels loaded for the operator's (Q, P)
 bool rtc_law_loaded(hfk_op* op);
+// The kernels of an Oseen point form compiled with hiprtc (k_oseen_form_jac / k_oseen_form_res for the form struct
+// `name` of `source`, colour-ordered [0] and atomic [1]), loaded on the current device; cached by source text.
+struct OseenFormKernels {
+  hipModule_t mod = nullptr;
+  hipFunction_t jac[2] = {}, res[2] = {};
+};
+int rtc_oseen_form(hfk_ctx* ctx, const char* source, const char* name, OseenFormKernels* out);
 }  // namespace hfk

@@ -29,63 +29,11 @@
 #include <vector>
 
 #include "hfk_op.hpp"
+#include "hfk_oseen_form.hpp"
 #include "hfk_solver.hpp"
 
 namespace hfk {
 namespace {
-
-constexpr int kND = 34;             // dofs per cell
-constexpr int kMaxPoints = 32;
-
-struct OseenDev {
-  int64_t ncells;
-  int np;
-  double rho, mu, c0;
-  const int32_t* ids;     // [cell][34]
-  const double* coords;   // [cell][4][3]
-  const double* qp;       // [q][4]: xi, eta, zeta, weight
-  const double* fields;   // [cell][q][9]: w, un, f
-  const double* lap;      // [cell][q][3] or null
-  const double* dvals;
-  const int32_t* nzpos;   // [cell][ii + 34 jj], 1-based, 0: not stored
-};
-
-// barycentric coordinate k (0..3) of a reference point and its (constant) reference gradient
-__device__ __forceinline__ double bary(int k, double xi, double et, double ze) {
-  return k == 0 ? 1.0 - xi - et - ze : (k == 1 ? xi : (k == 2 ? et : ze));
-}
-__device__ __forceinline__ void bary_grad(int k, double g[3]) {
-  g[0] = k == 0 ? -1.0 : (k == 1 ? 1.0 : 0.0);
-  g[1] = k == 0 ? -1.0 : (k == 2 ? 1.0 : 0.0);
-  g[2] = k == 0 ? -1.0 : (k == 3 ? 1.0 : 0.0);
-}
-
-// shape function f of the cell at a reference point: f = 0..3 P2 vertex functions l(2l - 1), 4..9 P2 edge
-// functions 4 l_a l_b (edges (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)), 10..13 the P1 functions l; value and
-// reference gradient
-__device__ __forceinline__ void shape(int f, double xi, double et, double ze, double& n, double g[3]) {
-  if (f < 4 || f >= 10) {
-    const int k = f < 4 ? f : f - 10;
-    const double l = bary(k, xi, et, ze);
-    double gl[3];
-    bary_grad(k, gl);
-    const double s = f < 4 ? 4.0 * l - 1.0 : 1.0;
-    n = f < 4 ? l * (2.0 * l - 1.0) : l;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) g[d] = s * gl[d];
-  } else {
-    const int e = f - 4;
-    const int a = e < 3 ? 0 : (e < 5 ? 1 : 2);
-    const int b = e < 3 ? e + 1 : (e < 5 ? e - 1 : 3);
-    const double la = bary(a, xi, et, ze), lb = bary(b, xi, et, ze);
-    double ga[3], gb[3];
-    bary_grad(a, ga);
-    bary_grad(b, gb);
-    n = 4.0 * la * lb;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) g[d] = 4.0 * (lb * ga[d] + la * gb[d]);
-  }
-}
 
 // One work-group per cell, the element matrix in factorised form.  The velocity-velocity block of test (i, a)
 // and trial (j, b) is  delta_ij S_ab + P_ab[j][i] + Q_ab[i][j]  with the point sums
@@ -96,15 +44,6 @@ __device__ __forceinline__ void shape(int f, double xi, double et, double ze, do
 // and 1 numbers).  All points of the cell are staged in LDS first (one barrier), and the entries go to the CSC
 // values through nzpos.  The residual does not form the element matrix (k_oseen_res).
 constexpr int kVV = 100, kVP = 40, kPV = 40, kPP = 16, kPairs = kVV + kVP + kPV + kPP;  // 196 pair threads
-
-// Cell of a work-group: the cells of one colour (cells != null), or, in the single-launch mode (cells == null),
-// every cell with XCD x (work-groups b = x mod 8) taking the contiguous block [x chunk, (x + 1) chunk) in mesh
-// order, so that neighbouring cells, which add into the same CSC lines, meet in one L2.  -1: idle.
-__device__ __forceinline__ int64_t work_item(const int32_t* cells, int64_t n, int64_t chunk) {
-  if (cells) return cells[blockIdx.x];
-  const int64_t c = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
-  return c < n ? c : -1;
-}
 
 template <bool ATOMIC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_oseen(OseenDev op, const int32_t* __restrict__ cells, int64_t chunk,
@@ -662,7 +601,26 @@ struct hfk_oseen {
   std::vector<int64_t> face_start;  // faces of launch group g: faces_d[face_start[g] .. face_start[g+1])
   DevBuf face_cell, face_normal, face_xo, face_xb, faces_d;
   DevBuf step_stage;  // host FE-function values of hfk_oseen_step_fields
+  // a point form compiled at run time (hfk_oseen_set_form) in place of the built-in integrand, with its own
+  // parameters after rho, mu, c0
+  std::unique_ptr<OseenFormKernels> form;
+  std::vector<double> form_params;
   hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
+  FormParams form_args() const {
+    FormParams p{};
+    p.v[0] = rho;
+    p.v[1] = mu;
+    p.v[2] = c0;
+    for (size_t i = 0; i < form_params.size(); ++i) p.v[3 + i] = form_params[i];
+    return p;
+  }
+  void drop_form() {
+    if (!form) return;
+    if (hipSetDevice(ctx->device) == hipSuccess) (void)hipDeviceSynchronize();  // launches may still be queued
+    (void)hipModuleUnload(form->mod);
+    form.reset();
+  }
+  ~hfk_oseen() { drop_form(); }
   OseenDev dev() const {
     OseenDev d;
     d.ncells = ncells;
@@ -711,6 +669,29 @@ int oseen_pattern(hfk_oseen* op) {
 }
 
 // colour by colour (or in one launch with atomic adds); x may be null (zero free values) when only K is wanted
+// the volume kernels of a compiled point form over `n` cells (cells: the list of one colour, or null for every
+// cell in the single-launch mode)
+int form_launch(hfk_oseen* op, bool atomic, const int32_t* cells, int64_t n, const double* x, double* b,
+                double* nzval, hipStream_t s) {
+  OseenDev d = op->dev();
+  FormParams prm = op->form_args();
+  const int a = atomic ? 1 : 0;
+  if (nzval) {
+    const int64_t chunk = atomic ? (op->ncells + 7) / 8 : 0;
+    const unsigned grid = (unsigned)(atomic ? 8 * chunk : n);
+    void* args[] = {&d, &cells, (void*)&chunk, &nzval, &prm};
+    HIPCHK(op->ctx, hipModuleLaunchKernel(op->form->jac[a], grid, 1, 1, 256, 1, 1,
+                                          (unsigned)(form_jac_shared(op->np) * sizeof(double)), s, args, nullptr));
+  }
+  if (b) {
+    const int cpw = std::min(256 / op->np, 32);
+    const unsigned grid = (unsigned)((n + cpw - 1) / cpw);
+    void* args[] = {&d, &cells, &n, &x, &b, &prm};
+    HIPCHK(op->ctx, hipModuleLaunchKernel(op->form->res[a], grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+  }
+  return HFK_OK;
+}
+
 template <bool A>
 int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipStream_t s) {
   const OseenDev d = op->dev();
@@ -726,10 +707,15 @@ int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipSt
   };
   if (A) {  // one launch each, cells in mesh order
     const int64_t chunk = (op->ncells + 7) / 8;
-    if (nzval) hipLaunchKernelGGL((k_oseen<true>), dim3((unsigned)(8 * chunk)), t, 0, s, d, nullptr, chunk, nzval);
-    if (b)
-      hipLaunchKernelGGL((k_oseen_res<true>), dim3((unsigned)((op->ncells + cpw - 1) / cpw)), t, 0, s, d, nullptr,
-                         op->ncells, x, b);
+    if (op->form) {
+      const int rc = form_launch(op, true, nullptr, op->ncells, x, b, nzval, s);
+      if (rc) return rc;
+    } else {
+      if (nzval) hipLaunchKernelGGL((k_oseen<true>), dim3((unsigned)(8 * chunk)), t, 0, s, d, nullptr, chunk, nzval);
+      if (b)
+        hipLaunchKernelGGL((k_oseen_res<true>), dim3((unsigned)((op->ncells + cpw - 1) / cpw)), t, 0, s, d, nullptr,
+                           op->ncells, x, b);
+    }
     HIPCHK(op->ctx, hipGetLastError());
     if (op->nfaces > 0) {
       const int64_t fchunk = (op->nfaces + 7) / 8;
@@ -742,6 +728,11 @@ int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipSt
     const int64_t n = op->color_start[c + 1] - op->color_start[c];
     if (n <= 0) continue;
     const int32_t* cells = op->cells_d.as<int32_t>() + op->color_start[c];
+    if (op->form) {
+      const int rc = form_launch(op, false, cells, n, x, b, nzval, s);
+      if (rc) return rc;
+      continue;
+    }
     if (nzval) hipLaunchKernelGGL((k_oseen<false>), dim3((unsigned)n), t, 0, s, d, cells, 0, nzval);  // a cell each
     if (b) hipLaunchKernelGGL((k_oseen_res<false>), dim3((unsigned)((n + cpw - 1) / cpw)), t, 0, s, d, cells, n, x, b);
     HIPCHK(op->ctx, hipGetLastError());
@@ -1000,6 +991,24 @@ int hfk_oseen_set_parameters(hfk_oseen* op, double rho, double mu, double c0) {
   op->rho = rho;
   op->mu = mu;
   op->c0 = c0;
+  return HFK_OK;
+}
+
+int hfk_oseen_set_form(hfk_oseen* op, const char* source, const char* name, const double* params, int nparams) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (nparams < 0 || nparams > kFormParams - 3 || (nparams > 0 && !params))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: a point form takes 0..5 parameters after rho, mu, c0");
+  if (!source) {  // back to the built-in integrand
+    op->drop_form();
+    op->form_params.clear();
+    return HFK_OK;
+  }
+  std::unique_ptr<OseenFormKernels> k(new OseenFormKernels());
+  if ((rc = rtc_oseen_form(op->ctx, source, name, k.get()))) return rc;
+  op->drop_form();
+  op->form = std::move(k);
+  op->form_params.assign(params, params + nparams);
   return HFK_OK;
 }
 

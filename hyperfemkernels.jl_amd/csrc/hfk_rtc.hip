@@ -23,7 +23,8 @@
 
 #include "hfk_op.hpp"
 
-// kTypesSrc, kPointSrc: the text of hfk_types.hpp and hfk_point.hpp (generated at build time)
+// kTypesSrc, kPointSrc, kOseenFormSrc: the text of hfk_types.hpp, hfk_point.hpp and hfk_oseen_form.hpp (generated at
+// build time)
 #include "hfk_embedded.inc"
 
 namespace hfk {
@@ -103,10 +104,10 @@ std::vector<std::string> extremum_names(int Q, int P) {
 // instantiations to emit, their lowered names returned in `lowered`.
 bool compile(const std::string& program, const std::vector<std::string>& names, std::vector<char>& code,
              std::vector<std::string>& lowered, std::string& log) {
-  const char* hdr_src[] = {kTypesSrc, kPointSrc};
-  const char* hdr_name[] = {"hfk_types.hpp", "hfk_point.hpp"};
+  const char* hdr_src[] = {kTypesSrc, kPointSrc, kOseenFormSrc};
+  const char* hdr_name[] = {"hfk_types.hpp", "hfk_point.hpp", "hfk_oseen_form.hpp"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, program.c_str(), "hfk_law.hip", 2, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
+  if (hiprtcCreateProgram(&prog, program.c_str(), "hfk_law.hip", 3, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
     log = "hiprtcCreateProgram failed";
     return false;
   }
@@ -313,6 +314,33 @@ bool rtc_law_loaded(hfk_op* op) {
   return law->inst.count(op->Q * 16 + op->P) > 0;
 }
 
+std::string oseen_form_program(const char* source) {
+  return std::string("#include \"hfk_oseen_form.hpp\"\n") + source + "\n";
+}
+
+std::vector<std::string> oseen_form_names(const std::string& n) {
+  return {"hfk::k_oseen_form_jac<" + n + ", false>", "hfk::k_oseen_form_jac<" + n + ", true>",
+          "hfk::k_oseen_form_res<" + n + ", false>", "hfk::k_oseen_form_res<" + n + ", true>"};
+}
+
+int rtc_oseen_form(hfk_ctx* ctx, const char* source, const char* name, OseenFormKernels* out) {
+  if (!source || !valid_identifier(name))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: a point form needs its source and a C identifier as its name");
+  const std::string program = oseen_form_program(source);
+  const std::vector<std::string> names = oseen_form_names(name);
+  CacheEntry e;
+  bool cached = false;
+  std::string log;
+  if (!compile_cached(program, names, e, cached, log))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: the point form does not compile: " + log);
+  HIPCHK(ctx, hipModuleLoadData(&out->mod, e.code.data()));
+  for (int a = 0; a < 2; ++a) {
+    HIPCHK(ctx, hipModuleGetFunction(&out->jac[a], out->mod, e.lowered[a].c_str()));
+    HIPCHK(ctx, hipModuleGetFunction(&out->res[a], out->mod, e.lowered[2 + a].c_str()));
+  }
+  return HFK_OK;
+}
+
 int rtc_energy(hfk_op* op, const double* x, double* part, hipStream_t s) {
   RtcInstance* ri = nullptr;
   int rc = law_instance(op, &ri);
@@ -425,6 +453,20 @@ int hfk_law_check(const char* source, const char* name, char* log, int64_t log_s
   }
   if (!ok) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: the law does not compile: " + msg);
   return HFK_OK;
+}
+
+int hfk_oseen_form_check(const char* source, const char* name, char* log, int64_t log_size) {
+  if (!source || !valid_identifier(name)) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: invalid form source or name");
+  CacheEntry e;
+  bool cached = false;
+  std::string msg;
+  const bool ok = compile_cached(oseen_form_program(source), oseen_form_names(name), e, cached, msg);
+  if (log && log_size > 0) {
+    const size_t n = std::min<size_t>(msg.size(), (size_t)log_size - 1);
+    std::memcpy(log, msg.data(), n);
+    log[n] = '\0';
+  }
+  return ok ? HFK_OK : set_err(nullptr, HFK_EARG, "HyperFEMKernels: the point form does not compile: " + msg);
 }
 
 int hfk_law_destroy(hfk_law* law) {

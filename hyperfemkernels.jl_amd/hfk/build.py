@@ -29,7 +29,7 @@ def _run(cmd):
 
 
 # device headers compiled again at run time by hiprtc (laws given as source): embedded in the library as text
-EMBEDDED = (("kTypesSrc", "hfk_types.hpp"), ("kPointSrc", "hfk_point.hpp"))
+EMBEDDED = (("kTypesSrc", "hfk_types.hpp"), ("kPointSrc", "hfk_point.hpp"), ("kOseenFormSrc", "hfk_oseen_form.hpp"))
 
 
 def write_embedded(objdir: str) -> None:

@@ -48,8 +48,26 @@ class OseenOperator:
                                         self.ndir, self.rho, self.mu, self.c0, C.byref(h)), self.ctx.handle)
         self.handle = h
         self._pattern = None
+        self.form = None
         if dirichlet_values is not None:
             self.set_dirichlet(dirichlet_values)
+
+    def set_form(self, form=None, params=None):
+        """Evaluate the volume integrand with a point form compiled at run time (``formgen.OseenPointForm``, e.g.
+        ``formgen.oseen_supg()``), its own parameters by name in ``params``; ``None`` restores the built-in
+        integrand.  rho, mu and c0 stay the operator's (``update``)."""
+        if form is None:
+            check(self.lib.hfk_oseen_set_form(self.handle, None, None, None, 0), self.ctx.handle)
+            self.form = None
+            return self
+        params = params or {}
+        extra = np.ascontiguousarray([float(params[n]) for n in form.params], dtype=np.float64)
+        src = form.source().encode()
+        check(self.lib.hfk_oseen_set_form(self.handle, src, form.name.encode(),
+                                          ptr(extra, C.c_double) if extra.size else None, int(extra.size)),
+              self.ctx.handle)
+        self.form = form
+        return self
 
     def set_dirichlet(self, dvals):
         d = np.ascontiguousarray(dvals, dtype=np.float64)

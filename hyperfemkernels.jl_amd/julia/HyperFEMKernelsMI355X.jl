@@ -372,6 +372,19 @@ function DeviceOseen(ctx::Context, cell_dofs::Matrix{Int32}, coords::Array{Float
     finalizer(o -> ccall((:hfk_oseen_destroy, libhfk), Cint, (Ptr{Cvoid},), o.ptr), op)
     op
 end
+"""
+    set_form!(op::DeviceOseen, source, name, params = Float64[])
+
+The volume integrand as a point form compiled with hiprtc (the PointForm of the captured integrand,
+src/codegen.jl:122-198): HIP source of a struct `name` with `residual` and `tangent` (see include/hfk.h); the Julia
+host emits it from the captured tree as hfk/formgen.py does from a Python one.  `nothing` restores the built-in
+SUPG/PSPG/LSIC integrand.
+"""
+set_form!(op::DeviceOseen, source::Union{Nothing,AbstractString}, name::AbstractString="", params::Vector{Float64}=Float64[]) =
+    check(ccall((:hfk_oseen_set_form, libhfk), Cint, (Ptr{Cvoid}, Cstring, Cstring, Ptr{Float64}, Cint), op.ptr,
+                source === nothing ? C_NULL : source, source === nothing ? C_NULL : name,
+                isempty(params) ? C_NULL : params, length(params)), op.ctx.ptr)
+
 "point data 9 x np x ncells: advecting velocity w, stabilisation velocity un, history f"
 set_fields!(op::DeviceOseen, fields::Array{Float64,3}) =
     check(ccall((:hfk_oseen_set_fields, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, fields), op.ctx.ptr)

@@ -390,6 +390,21 @@ int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell
 int hfk_oseen_set_parameters(hfk_oseen* op, double rho, double mu, double c0);
 /* The outlet pressure Pc alone (a number of the integrand, updated every step by a Windkessel model). */
 int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc);
+/* The volume integrand of the step as a point form compiled at run time with hiprtc (the generated pointwise code of
+ * the captured integrand, PointForm, src/codegen.jl:122-198), in place of the built-in SUPG/PSPG/LSIC integrand:
+ * `source` (HIP) defines a struct `name` with
+ *   static __device__ void residual(const double* s, const double* d, double h, const double* prm, double* co);
+ *   static __device__ void tangent(const double* d, double h, const double* prm, double* k);
+ * s: the 16 state slots at a point (u (3), du_i/dx_J at 3 + 3i + J, p, dp/dx_J at 13 + J); d: w, un, f, lap (12);
+ * h the cell size (6 sqrt(2) |K|)^(1/3); prm: rho, mu, c0 (hfk_oseen_set_parameters), then `params` (at most 5).
+ * residual: the coefficient of each of the 16 test slots; tangent: the 50 factors of the factorised tangent (see
+ * hfk_oseen_form.hpp) and `static constexpr unsigned long long kNonzero`, bit i set unless factor i is zero.  Code objects
+ * are cached by source text; a form that does not compile gives HFK_EARG with the
+ * compiler log.  source = NULL restores the built-in integrand.  hfk/formgen.py emits such sources from an
+ * integrand written in Python. */
+int hfk_oseen_set_form(hfk_oseen* op, const char* source, const char* name, const double* params, int nparams);
+/* Compile a point form without a device (hiprtc only): HFK_OK, or HFK_EARG with the compiler log in `log`. */
+int hfk_oseen_form_check(const char* source, const char* name, char* log, int64_t log_size);
 /* Assembly colour by colour with plain adds (1, the default: the reference's colour order, bitwise reproducible,
  * src/assembly.jl:1413-1432) or in one launch over the cells in mesh order with FP64 atomic adds (0: the CSC values
  * that neighbouring cells share stay in L2; the order of the additions into one entry varies between runs). */

@@ -35,10 +35,12 @@ def basis(qp):
     return N, G
 
 
-def element_terms(coords, qp, qw, fields, rho, mu, c0, lap=None):
+def element_terms(coords, qp, qw, fields, rho, mu, c0, lap=None, tau=None):
     """Per cell, the matrices of every term of a((u,p),(v,q)) (dict name -> (ncells, 34 test, 34 trial)) and the
     load vector (ncells, 34).  fields (ncells, np, 9): w, un, f; lap (ncells, np, 3): the Laplacian of un in the
-    stabilisation load rk = rho f + mu lap (test/common.jl:216-218), or None."""
+    stabilisation load rk = rho f + mu lap (test/common.jl:216-218), or None.  tau: the stabilisation parameters
+    (ts, tp, tl) as a function of (|un|, h, rho, mu, c0) (arrays over cells and points); None for the reference's
+    (test/common.jl:201-205)."""
     coords = np.asarray(coords)
     nc = coords.shape[0]
     Nr, Gr = basis(qp)
@@ -51,9 +53,12 @@ def element_terms(coords, qp, qw, fields, rho, mu, c0, lap=None):
     h = (6 * np.sqrt(2) * vol) ** (1.0 / 3.0)
     w, un, f = fields[..., 0:3], fields[..., 3:6], fields[..., 6:9]
     unn = np.linalg.norm(un, axis=2)
-    ts = ((2 * unn / h[:, None]) ** 2 + 9 * (4 * mu / (rho * h[:, None] ** 2)) ** 2) ** -0.5
-    tp = ts / rho
-    tl = rho * unn * h[:, None] * 0.5
+    if tau is None:
+        ts = ((2 * unn / h[:, None]) ** 2 + 9 * (4 * mu / (rho * h[:, None] ** 2)) ** 2) ** -0.5
+        tp = ts / rho
+        tl = rho * unn * h[:, None] * 0.5
+    else:
+        ts, tp, tl = tau(unn, np.broadcast_to(h[:, None], unn.shape), rho, mu, c0)
     npt = len(qw)
     # the 34 functions: (u (3), grad u (3x3: [J, c]), p, grad p) at every point
     U = np.zeros((nc, npt, 34, 3))
@@ -87,8 +92,8 @@ def element_terms(coords, qp, qw, fields, rho, mu, c0, lap=None):
     return T, load
 
 
-def element_system(coords, qp, qw, fields, rho, mu, c0, lap=None):
-    T, load = element_terms(coords, qp, qw, fields, rho, mu, c0, lap)
+def element_system(coords, qp, qw, fields, rho, mu, c0, lap=None, tau=None):
+    T, load = element_terms(coords, qp, qw, fields, rho, mu, c0, lap, tau)
     return sum(T.values()), load
 
 

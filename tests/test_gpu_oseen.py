@@ -318,3 +318,66 @@ def test_oseen_host_local_order(hip_ctx):
     bad.nodeperm = np.zeros(10, dtype=np.int32)  # not a permutation
     with pytest.raises(HFKError, match="ArgumentError"):
         OseenOperator(bad, ctx=hip_ctx)
+
+
+def test_generated_point_form(hip_ctx):
+    """The volume integrand compiled at run time from the captured form (hfk.formgen, hiprtc), in place of the
+    built-in kernels: the reference's SUPG/PSPG/LSIC integrand gives the built-in operator's residual and CSC
+    values and the oracle's (1e-12) with the Laplacian and the outlet terms, in the colour-ordered and the atomic
+    launches, after update(rho, mu, dt) too; a stabilisation variant the library does not build in (the transient
+    SUPG parameter) equals the oracle with that tau in numpy; removing the form restores the built-in kernels."""
+    from oracle import oseen as O
+
+    from hfk import formgen as fg
+    from hfk.tetmesh import outlet_faces
+
+    print("oseen generated point form", flush=True)
+    sp, qp, qw, fields, dv, op, rng = setup(hip_ctx, (3, 2, 2), True, seed=29)
+    ref = OseenOperator(sp, rho=1.0, mu=0.05, dt=0.01, bdf2=True, dirichlet_values=dv, ctx=hip_ctx)
+    ref.set_fields(fields[..., 0:3], fields[..., 3:6], fields[..., 6:9])
+    lap = rng.standard_normal((sp.ncells, len(qw), 3))
+    of = outlet_faces(sp)
+    wb = rng.standard_normal((len(of.cell), of.xb.shape[1], 3))
+    for o in (op, ref):
+        o.set_laplacian(lap)
+        o.set_outlet(of, wb, Pc=0.7, ZA=2.3)
+    op.set_form(fg.oseen_supg())
+    colptr, rowval, nzpos = op.pattern()
+    x = rng.standard_normal(sp.nfree)
+    xb = np.concatenate([of.xb, wb], axis=2)
+    Kf, Lf = O.outlet_terms(of.xo, xb, of.normal, 1.0, 0.7, 2.3)
+
+    def oracle_check(rho, mu, c0, tau=None):
+        K, load = O.element_system(sp.coords, qp, qw, fields, rho, mu, c0, lap=lap, tau=tau)
+        Kf, Lf = O.outlet_terms(of.xo, xb, of.normal, rho, 0.7, 2.3)
+        Kt, lt = O.add_faces(K, load, of.cell, Kf, Lf)
+        A = op.jacobian(op.allocate_jacobian())
+        assert rel(A.data, O.jacobian_values(Kt, nzpos, rowval.size)) < TOL
+        assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, Kt, lt)) < TOL
+        return A
+
+    for det in (True, False):
+        op.set_deterministic(det)
+        ref.set_deterministic(det)
+        A = oracle_check(1.0, 0.05, op.c0)
+        assert rel(A.data, ref.jacobian(ref.allocate_jacobian()).data) < TOL
+        assert rel(op.residual(np.zeros(sp.nfree), x), ref.residual(np.zeros(sp.nfree), x)) < TOL
+    op.update(rho=1.3, mu=0.02, dt=0.02)
+    oracle_check(1.3, 0.02, op.c0)
+    # a stabilisation variant: the time scale under the root of tau_s
+    ct = 0.5
+
+    def tau_transient(D):
+        unn = fg.norm(D.un)
+        r0, r1, r2 = D.param("ct") * D.c0, 2.0 * unn / D.h, 3.0 * (4.0 * D.mu / (D.rho * D.h * D.h))
+        ts = 1.0 / fg.sqrt(r0 * r0 + r1 * r1 + r2 * r2)
+        return ts, ts / D.rho, D.rho * unn * D.h * 0.5
+
+    def tau_np(unn, h, rho, mu, c0):
+        ts = ((ct * c0) ** 2 + (2 * unn / h) ** 2 + 9 * (4 * mu / (rho * h ** 2)) ** 2) ** -0.5
+        return ts, ts / rho, rho * unn * h * 0.5
+
+    op.set_form(fg.oseen_supg(tau=tau_transient, params=("ct",), name="OseenTransientSupg"), {"ct": ct})
+    oracle_check(1.3, 0.02, op.c0, tau=tau_np)
+    op.set_form(None)
+    oracle_check(1.3, 0.02, op.c0)
