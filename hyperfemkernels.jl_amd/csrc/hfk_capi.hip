@@ -149,7 +149,12 @@ int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s,
 // the cells that contain the column's dof (same CSC, bit for bit), in parallel over columns.  Generic over the
 // cell dof table (any element, any number of fields: every (test, trial) pair of a cell is coupled).
 void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
-                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos) {
+                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks) {
+  const bool masked = blocks.mask != nullptr;
+  const int nf = blocks.nf;
+  auto block_set = [&](int64_t cell, int ii, int jj) {
+    return !masked || blocks.mask[(size_t)cell * nf * nf + blocks.field_of[ii] + nf * blocks.field_of[jj]] != 0;
+  };
   // cells of every free dof
   std::vector<int64_t> cstart((size_t)ncols + 1, 0);
   for (int64_t cell = 0; cell < ncells; ++cell)
@@ -169,9 +174,13 @@ void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols
   auto rows_of = [&](int64_t c, std::vector<int32_t>& buf) {
     buf.clear();
     for (int64_t k = cstart[c]; k < cstart[c + 1]; ++k) {
-      const int32_t* ids = &cell_dofs[(size_t)cells_of[k] * nd];
+      const int64_t cell = cells_of[k];
+      const int32_t* ids = &cell_dofs[(size_t)cell * nd];
+      int jj = 0;  // the local dof of column c in this cell
+      if (masked)
+        while (ids[jj] != c + 1) ++jj;
       for (int ii = 0; ii < nd; ++ii)
-        if (ids[ii] > 0) buf.push_back(ids[ii]);
+        if (ids[ii] > 0 && block_set(cell, ii, jj)) buf.push_back(ids[ii]);
     }
     std::sort(buf.begin(), buf.end());
     buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
@@ -221,7 +230,8 @@ void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols
               const int32_t r = ids[ii];
               if (r <= 0) continue;
               const int64_t* it = std::lower_bound(rowval.data() + lo, rowval.data() + hi, (int64_t)r);
-              pos[ii + nd * jj] = (int32_t)(it - rowval.data() + 1);
+              const bool stored = it != rowval.data() + hi && *it == r;
+              pos[ii + nd * jj] = stored ? (int32_t)(it - rowval.data() + 1) : 0;
             }
           }
         }

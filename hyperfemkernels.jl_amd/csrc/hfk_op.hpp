@@ -144,9 +144,18 @@ int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev,
                    double alpha, bool accumulate);
 int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s);
 int op_build_pattern(hfk_op* op);
-// the reference's CSC sparsity and cell positions (1-based) of a cell dof table (> 0 free ids)
+// Field blocks of a multi-field cell (block_spec / cell_block_mask, src/assembly.jl:276-323, 825-832): the field of
+// each local dof and, per cell, which (test field, trial field) blocks carry a term (mask[cell * nf^2 + fi + nf fj]).
+struct BlockInfo {
+  const int8_t* field_of = nullptr;  // nd entries
+  int nf = 1;
+  const uint8_t* mask = nullptr;     // ncells x nf^2; null: every block of every cell
+};
+// the reference's CSC sparsity and cell positions (1-based) of a cell dof table (> 0 free ids): the keys of the
+// free pairs of every cell (full_keys!), or of the blocks set on each cell (block_keys!); nzpos of a pair outside the
+// cell's blocks is its stored position when another cell stores it, 0 otherwise (cell_positions!, :864-899)
 void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
-                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos);
+                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks = BlockInfo());
 // the reference's greedy colouring (1-based colours); false when more than 128 colours are needed
 bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color);
 int op_build_blocks(hfk_op* op);

@@ -586,6 +586,7 @@ struct hfk_oseen {
   double rho = 1, mu = 0.01, c0 = 0;
   std::vector<int32_t> cell_dofs, color;  // cell_dofs in the library's local order
   std::vector<int32_t> perm;              // perm[i] = the host's local dof of the library's local dof i
+  std::vector<uint8_t> block_mask;        // ncells x 4 (velocity/pressure test x trial blocks) or empty: all
   std::vector<int64_t> color_start;  // cells of colour c: cells_by_color[color_start[c] .. color_start[c+1])
   bool pattern_built = false, has_fields = false;
   std::vector<int64_t> colptr, rowval;
@@ -661,7 +662,15 @@ int oseen_bind(hfk_oseen* op) {
 
 int oseen_pattern(hfk_oseen* op) {
   if (op->pattern_built) return HFK_OK;
-  csc_pattern(op->cell_dofs.data(), kND, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos);
+  BlockInfo bi;
+  std::vector<int8_t> field_of(kND);
+  for (int i = 0; i < kND; ++i) field_of[i] = i < 30 ? 0 : 1;  // velocity, then pressure (the library's local order)
+  if (!op->block_mask.empty()) {
+    bi.field_of = field_of.data();
+    bi.nf = 2;
+    bi.mask = op->block_mask.data();
+  }
+  csc_pattern(op->cell_dofs.data(), kND, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos, bi);
   HIPCHK(op->ctx, op->nzpos_d.alloc(op->nzpos.size() * sizeof(int32_t)));
   HIPCHK(op->ctx, hipMemcpy(op->nzpos_d.p, op->nzpos.data(), op->nzpos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   op->pattern_built = true;
@@ -1009,6 +1018,27 @@ int hfk_oseen_set_form(hfk_oseen* op, const char* source, const char* name, cons
   op->drop_form();
   op->form = std::move(k);
   op->form_params.assign(params, params + nparams);
+  return HFK_OK;
+}
+
+int hfk_oseen_set_blocks(hfk_oseen* op, const int32_t* mask, const int32_t* cell_mask) {
+  int rc = oseen_bind(op);
+  if (rc) return rc;
+  if (!mask && !cell_mask) {
+    op->block_mask.clear();
+  } else {
+    std::vector<uint8_t> m((size_t)op->ncells * 4);
+    for (int64_t cell = 0; cell < op->ncells; ++cell)
+      for (int k = 0; k < 4; ++k) m[(size_t)cell * 4 + k] = (cell_mask ? cell_mask[cell * 4 + k] : mask[k]) != 0;
+    bool all = true;
+    for (uint8_t v : m) all = all && v;
+    if (all) m.clear();  // every block of every cell: the full pattern (cell_block_mask returns nothing)
+    op->block_mask = std::move(m);
+  }
+  op->pattern_built = false;  // the pattern, its positions and the device copy are rebuilt at the next use
+  op->colptr.clear();
+  op->rowval.clear();
+  op->nzpos.clear();
   return HFK_OK;
 }
 

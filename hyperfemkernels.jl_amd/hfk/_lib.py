@@ -113,6 +113,7 @@ SIGNATURES = {
                                    C.c_int64, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]),
     "hfk_oseen_destroy": (C.c_int, [_vp]),
     "hfk_oseen_set_form": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int]),
+    "hfk_oseen_set_blocks": (C.c_int, [_vp, _i32p, _i32p]),
     "hfk_oseen_form_check": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64]),
     "hfk_oseen_set_fields": (C.c_int, [_vp, _f64p]),
     "hfk_oseen_set_dirichlet": (C.c_int, [_vp, _f64p]),

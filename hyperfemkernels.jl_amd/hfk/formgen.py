@@ -294,6 +294,19 @@ class OseenPointForm:
                 raise ValueError("HyperFEMKernels: the form's tangent is not of the factorised velocity-pressure class "
                                  "(isotropic in the components, one advecting direction per term)")
 
+    def blocks(self) -> np.ndarray:
+        """The (test field, trial field) blocks that carry a term (2 x 2, velocity 0, pressure 1): a block whose
+        tangent entries are all the constant 0 carries none (form_blocks, src/assembly.jl:288-303: by structure, so
+        a term multiplied by a parameter that happens to be 0 keeps its block)."""
+        fld = lambda s: 0 if s < 12 else 1
+        m = np.zeros((2, 2), dtype=bool)
+        for s in range(NSLOT):
+            for r in range(NSLOT):
+                t = self.T[s][r]
+                if not (t.op == "const" and t.value == 0.0):
+                    m[fld(s), fld(r)] = True
+        return m
+
     # -- evaluation and code --------------------------------------------------------------------------------
     def evaluate(self, state, data, h, prm):
         """(co (16), T (16 x 16), k (50)) at one point with numpy: state slots, data (12), h, prm (rho, mu, c0,

@@ -69,6 +69,19 @@ class OseenOperator:
         self.form = form
         return self
 
+    def set_blocks(self, mask=None, cell_mask=None):
+        """Field blocks that carry a term (src/assembly.jl:276-323, 825-893): ``mask`` 2 x 2 booleans [test field,
+        trial field] (velocity 0, pressure 1) for every cell, or ``cell_mask`` (ncells, 2, 2); both None: every
+        block.  The sparsity pattern is rebuilt at its next use."""
+        m = None if mask is None else np.ascontiguousarray(np.asarray(mask, dtype=np.int32).T.reshape(-1))
+        cm = None
+        if cell_mask is not None:
+            cm = np.ascontiguousarray(np.transpose(np.asarray(cell_mask, dtype=np.int32), (0, 2, 1)).reshape(-1))
+        check(self.lib.hfk_oseen_set_blocks(self.handle, None if m is None else ptr(m, C.c_int32),
+                                            None if cm is None else ptr(cm, C.c_int32)), self.ctx.handle)
+        self._pattern = None
+        return self
+
     def set_dirichlet(self, dvals):
         d = np.ascontiguousarray(dvals, dtype=np.float64)
         if d.size != self.ndir:

@@ -385,6 +385,23 @@ set_form!(op::DeviceOseen, source::Union{Nothing,AbstractString}, name::Abstract
                 source === nothing ? C_NULL : source, source === nothing ? C_NULL : name,
                 isempty(params) ? C_NULL : params, length(params)), op.ctx.ptr)
 
+"""
+    set_blocks!(op::DeviceOseen, mask)            # mask::BitMatrix 2 x 2, or a 2 x 2 x ncells array per cell
+
+The (test field, trial field) blocks that carry a term (`form_blocks`/`cell_block_mask`, src/assembly.jl:288-303,
+825-832), field 1 the velocity and 2 the pressure; `nothing`: every block.  The pattern is rebuilt at its next use.
+"""
+function set_blocks!(op::DeviceOseen, mask)
+    op.pattern = nothing
+    mask === nothing && return check(ccall((:hfk_oseen_set_blocks, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}),
+                                           op.ptr, C_NULL, C_NULL), op.ctx.ptr)
+    m = Int32.(mask)  # column-major: [fi + 2 fj] (0-based fields) per cell
+    ndims(m) == 2 ? check(ccall((:hfk_oseen_set_blocks, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}), op.ptr, m,
+                                C_NULL), op.ctx.ptr) :
+                    check(ccall((:hfk_oseen_set_blocks, libhfk), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}), op.ptr,
+                                C_NULL, m), op.ctx.ptr)
+end
+
 "point data 9 x np x ncells: advecting velocity w, stabilisation velocity un, history f"
 set_fields!(op::DeviceOseen, fields::Array{Float64,3}) =
     check(ccall((:hfk_oseen_set_fields, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}), op.ptr, fields), op.ctx.ptr)

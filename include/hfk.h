@@ -414,6 +414,12 @@ int hfk_oseen_info(hfk_oseen* op, int64_t* info);
 int hfk_oseen_colors(hfk_oseen* op, int32_t* colors);
 /* sparsity_pattern / nonzero_positions (src/assembly.jl:768-899): 1-based CSC and nzpos [cell][ii + 34 jj], ii and jj
  * the host's local dofs (the order of the cell_dofs given to hfk_oseen_create) */
+/* Field blocks (block_spec, cell_block_mask, block_keys!: src/assembly.jl:276-323, 825-893): which (test field,
+ * trial field) blocks carry a term, mask[fi + 2 fj] with field 0 the velocity and 1 the pressure, the same on every
+ * cell, or cell_mask (ncells x 4) per cell; both NULL: every block (the default).  The pattern then holds the free
+ * pairs of the set blocks only, and nzpos of a pair outside its cell's blocks is its stored position when another
+ * cell stores it, 0 otherwise.  Rebuilds the pattern at its next use (allocate the Jacobian again). */
+int hfk_oseen_set_blocks(hfk_oseen* op, const int32_t* mask, const int32_t* cell_mask);
 int hfk_oseen_pattern_nnz(hfk_oseen* op, int64_t* nnz);
 int hfk_oseen_pattern(hfk_oseen* op, int64_t* colptr, int64_t* rowval, int32_t* nzpos);
 /* residual! (src/assembly.jl:909-916): b = a(x, .) - l(.) on the free test dofs, Dirichlet values lifted */

@@ -205,6 +205,38 @@ def pattern(cell_dofs, nfree):
     return colptr, rowval, nzpos
 
 
+def block_pattern(cell_dofs, nfree, field_of, cell_mask):
+    """The reference's pattern with field blocks (block_keys!, cell_positions! with FULL = false,
+    src/assembly.jl:825-899): keys (col << 32) | row of the free pairs (ii, jj) whose block
+    cell_mask[cell, field_of[ii], field_of[jj]] is set, sorted and unique; nzpos of every free pair is the position
+    of its key when stored (by any cell), 0 otherwise."""
+    ids = np.asarray(cell_dofs, dtype=np.int64)
+    nc, nd = ids.shape
+    fo = np.asarray(field_of)
+    cm = np.asarray(cell_mask, dtype=bool)
+    rows = np.repeat(ids[:, :, None], nd, axis=2)  # [cell][ii][jj]
+    cols = np.repeat(ids[:, None, :], nd, axis=1)
+    blk = cm[:, fo[:, None], fo[None, :]]  # [cell][ii][jj]
+    m = (rows > 0) & (cols > 0) & blk
+    keys = np.unique((cols[m] << 32) | rows[m])
+    col = keys >> 32
+    colptr = np.ones(nfree + 1, dtype=np.int64)
+    np.add.at(colptr, col, 1)
+    colptr = np.cumsum(colptr) - np.arange(nfree + 1)
+    colptr[0] = 1
+    rowval = (keys & 0xFFFFFFFF).astype(np.int64)
+    nzpos = np.zeros((nc, nd * nd), dtype=np.int32)
+    cc = np.transpose(cols, (0, 2, 1)).reshape(nc, -1)
+    rr = np.transpose(rows, (0, 2, 1)).reshape(nc, -1)
+    mm = (cc > 0) & (rr > 0)
+    q = (cc[mm] << 32) | rr[mm]
+    k = np.searchsorted(keys, q)
+    found = (k < keys.size) & (keys[np.minimum(k, keys.size - 1)] == q)
+    vals = np.where(found, k + 1, 0).astype(np.int32)
+    nzpos[mm] = vals
+    return colptr, rowval, nzpos
+
+
 def jacobian_values(K, nzpos, nnz):
     nz = np.zeros(nnz)
     Kf = np.transpose(K, (0, 2, 1)).reshape(K.shape[0], -1)  # [cell][jj][ii] -> index ii + nd jj

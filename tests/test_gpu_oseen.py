@@ -381,3 +381,55 @@ def test_generated_point_form(hip_ctx):
     oracle_check(1.3, 0.02, op.c0, tau=tau_np)
     op.set_form(None)
     oracle_check(1.3, 0.02, op.c0)
+
+
+def test_field_block_sparsity(hip_ctx):
+    """Field blocks (test/test_block_pattern.jl): the Oseen integrand without PSPG (generated form with tp = 0) has
+    no pressure-pressure term, so its blocks leave that block out; the device pattern, nzpos and colours equal the
+    oracle's block pattern bit for bit, the pressure-pressure part of the matrix is empty, and the values equal the
+    oracle's (1e-12).  Then the block on half of the cells only (per-cell masks, as a coupling on part of the
+    domain), and back to every block (the full pattern)."""
+    from oracle import oseen as O
+
+    from hfk import formgen as fg
+
+    print("oseen field blocks", flush=True)
+    sp, qp, qw, fields, dv, op, rng = setup(hip_ctx, (3, 2, 2), True, seed=31)
+
+    def tau_no_pspg(D):
+        ts, _, tl = fg.tau_reference(D)
+        return ts, fg.const(0.0), tl
+
+    form = fg.oseen_supg(tau=tau_no_pspg, name="OseenNoPspg")
+    blocks = form.blocks()
+    assert blocks.tolist() == [[True, True], [True, False]]
+    op.set_form(form)
+    op.set_blocks(blocks)
+    fo = np.array([0] * 30 + [1] * 4)
+    colptr, rowval, nzpos = op.pattern()
+    cm = np.broadcast_to(blocks, (sp.ncells, 2, 2))
+    c2, r2, p2 = O.block_pattern(sp.cell_dofs, sp.nfree, fo, cm)
+    assert np.array_equal(colptr, c2) and np.array_equal(rowval, r2) and np.array_equal(nzpos, p2)
+    assert rowval.size < O.pattern(sp.cell_dofs, sp.nfree)[1].size
+    A = op.jacobian(op.allocate_jacobian())
+    nu = sp.nfree_u
+    assert A[nu:, nu:].nnz == 0
+
+    def tau_np(unn, h, rho, mu, c0):
+        ts = ((2 * unn / h) ** 2 + 9 * (4 * mu / (rho * h ** 2)) ** 2) ** -0.5
+        return ts, 0.0 * ts, rho * unn * h * 0.5
+
+    K, load = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.05, op.c0, tau=tau_np)
+    assert rel(A.data, O.jacobian_values(K, p2, r2.size)) < TOL
+    x = rng.standard_normal(sp.nfree)
+    assert rel(op.residual(np.zeros(sp.nfree), x), O.residual(sp, x, dv, K, load)) < TOL
+    part = np.array(cm)
+    part[rng.random(sp.ncells) < 0.5, 1, 1] = True
+    op.set_blocks(cell_mask=part)
+    colptr, rowval, nzpos = op.pattern()
+    c3, r3, p3 = O.block_pattern(sp.cell_dofs, sp.nfree, fo, part)
+    assert np.array_equal(colptr, c3) and np.array_equal(rowval, r3) and np.array_equal(nzpos, p3)
+    A = op.jacobian(op.allocate_jacobian())
+    assert rel(A.data, O.jacobian_values(K, p3, r3.size)) < TOL
+    op.set_blocks(None)
+    assert np.array_equal(op.pattern()[1], O.pattern(sp.cell_dofs, sp.nfree)[1])

@@ -373,3 +373,41 @@ def test_step_fields_closed_forms(channel):
                          sp.dirichlet_values(q), 2.0, -1.0)
     Xb = of.xb_phys.reshape(-1, 3)
     assert np.abs(wb - (2 * p(Xb) - q(Xb)).reshape(wb.shape)).max() < 1e-13
+
+
+def test_block_pattern_against_sets():
+    """The oracle's field-block pattern (block_keys!/cell_positions!, src/assembly.jl:825-899) against a Python set of
+    (row, col) pairs: no pressure-pressure block anywhere (Taylor-Hood Oseen without PSPG), then the pressure-pressure
+    block on a random half of the cells only; every free pair's nzpos addresses its CSC entry when some cell's blocks
+    store it and is 0 otherwise; with every block set it is the full pattern."""
+    sp = tet_p2p1_space((3, 2, 2))
+    ids = sp.cell_dofs
+    fo = np.array([0] * 30 + [1] * 4)
+    rng = np.random.default_rng(3)
+    no_pp = np.ones((sp.ncells, 2, 2), dtype=bool)
+    no_pp[:, 1, 1] = False
+    part = no_pp.copy()
+    part[rng.random(sp.ncells) < 0.5, 1, 1] = True
+    full = O.pattern(ids, sp.nfree)
+    assert all(np.array_equal(a, b) for a, b in zip(O.block_pattern(ids, sp.nfree, fo, np.ones((sp.ncells, 2, 2))), full))
+    for cm in (no_pp, part):
+        colptr, rowval, nzpos = O.block_pattern(ids, sp.nfree, fo, cm)
+        pairs = set()
+        for c in range(sp.ncells):
+            for ii in range(34):
+                for jj in range(34):
+                    if ids[c, ii] > 0 and ids[c, jj] > 0 and cm[c, fo[ii], fo[jj]]:
+                        pairs.add((int(ids[c, ii]), int(ids[c, jj])))
+        got = {(int(rowval[k]), j + 1) for j in range(sp.nfree) for k in range(colptr[j] - 1, colptr[j + 1] - 1)}
+        assert got == pairs and rowval.size == len(pairs)
+        assert rowval.size < full[1].size
+        col_of = np.repeat(np.arange(sp.nfree), np.diff(colptr))
+        for c in range(sp.ncells):
+            for ii in range(34):
+                for jj in range(34):
+                    p = nzpos[c, ii + 34 * jj]
+                    r, q = ids[c, ii], ids[c, jj]
+                    if r > 0 and q > 0 and (int(r), int(q)) in pairs:
+                        assert p > 0 and rowval[p - 1] == r and col_of[p - 1] + 1 == q
+                    else:
+                        assert p == 0
