@@ -697,6 +697,31 @@ int hfk_op_destroy(hfk_op* op) {
   return HFK_OK;
 }
 
+int hfk_op_set_refinement(hfk_op* fine, hfk_op* coarse, const int32_t* parent, const int32_t* child) {
+  if (!fine || !fine->ctx) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  if (!coarse) {
+    fine->refines = nullptr;
+    fine->parent.clear();
+    fine->child.clear();
+    return HFK_OK;
+  }
+  if (!parent || !child || coarse->ctx != fine->ctx)
+    return set_err(fine->ctx, HFK_EARG, "HyperFEMKernels: invalid refinement arguments");
+  std::vector<int64_t> key((size_t)fine->ncells);
+  for (int64_t f = 0; f < fine->ncells; ++f) {
+    if (parent[f] < 0 || parent[f] >= coarse->ncells || child[f] < 0 || child[f] > 7)
+      return set_err(fine->ctx, HFK_EDIM, "HyperFEMKernels: refinement out of range (coarse cell, child 0..7)");
+    key[f] = (int64_t)parent[f] * 8 + child[f];
+  }
+  std::sort(key.begin(), key.end());
+  if (std::adjacent_find(key.begin(), key.end()) != key.end())
+    return set_err(fine->ctx, HFK_EARG, "HyperFEMKernels: two cells claim the same child of a coarse cell");
+  fine->refines = coarse;
+  fine->parent.assign(parent, parent + fine->ncells);
+  fine->child.assign(child, child + fine->ncells);
+  return HFK_OK;
+}
+
 int hfk_op_set_dirichlet(hfk_op* op, const double* dvals) {
   int rc = bind(op);
   if (rc) return rc;

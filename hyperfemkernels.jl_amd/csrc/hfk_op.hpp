@@ -76,6 +76,10 @@ struct hfk_op {
   hfk::Tab tab{};
   std::vector<int32_t> cell_dofs;               // Gridap order, ncells*nd
   std::vector<int32_t> ldof, nodeperm;          // as passed
+  // a nested refinement of another operator's mesh (h-multigrid levels): the coarse cell of every cell and the
+  // position of the cell in it (cx + 2 cy + 4 cz, each 0 or 1)
+  const hfk_op* refines = nullptr;
+  std::vector<int32_t> parent, child;
   std::vector<int32_t> color;                   // 1-based colour of each cell
   std::vector<hfk::BatchRange> color_batches;   // batches of each colour
   std::vector<int64_t> batch_cell;              // nbatches*C, -1 padding

@@ -360,13 +360,135 @@ struct hfk_transfer {
   hfk_op* fine = nullptr;
   int NF = 0, NC = 0;
   DevBuf E, Einj, cid, invmult;
+  // between nested meshes (the fine operator refines the coarse one): P (fine free x coarse free) and R = P' as CSR,
+  // and the fine source of every coarse free and Dirichlet value for the injection
+  bool nested = false;
+  DevBuf prp, pcol, pval, rrp, rcol, rval, inj, injd;
 };
 
 namespace {
 
+// A transfer between nested meshes (h-multigrid): the fine operator's cells each lie in one coarse cell (parent) as
+// one of its 8 children.  P[f, c] = coarse shape function of dof c at the node of fine dof f (trilinear for Q1,
+// snapped to 0/1 like the p-transfers), R = P', injection: each coarse node is a node of the fine mesh.  Built on
+// the host from the two operators' cell dof tables.
+int transfer_build_nested(hfk_op* coarse, hfk_op* fine, hfk_transfer* T) {
+  hfk_ctx* ctx = fine->ctx;
+  if (fine->refines != coarse)
+    return set_err(ctx, HFK_EDIM, "HyperFEMKernels: the two levels live on different meshes and the fine one is not "
+                                  "declared a refinement of the coarse one (hfk_op_set_refinement)");
+  const int Qf = fine->Q, Qc = coarse->Q, NF = fine->nn, NC = coarse->nn;
+  T->coarse = coarse;
+  T->fine = fine;
+  T->NF = NF;
+  T->NC = NC;
+  T->nested = true;
+  auto lag = [](int Q, double xi, int i) {
+    const double h = 1.0 / (Q - 1);
+    double v = 1;
+    for (int j = 0; j < Q; ++j)
+      if (j != i) v *= (xi - j * h) / (i * h - j * h);
+    return v;
+  };
+  auto snap = [](double e) { return std::fabs(e) < 1e-12 ? 0.0 : (std::fabs(e - 1) < 1e-12 ? 1.0 : e); };
+  const int64_t nf = fine->nfree, nc = coarse->nfree;
+  std::vector<std::vector<std::pair<int32_t, double>>> rows((size_t)nf);
+  std::vector<int32_t> inj((size_t)nc, 0), injd((size_t)coarse->ndir, 0);
+  for (int64_t f = 0; f < fine->ncells; ++f) {
+    const int64_t pc = fine->parent[f];
+    const int ch = fine->child[f];
+    const int cpos[3] = {ch & 1, (ch >> 1) & 1, (ch >> 2) & 1};
+    const int32_t* fd = &fine->cell_dofs[(size_t)f * fine->nd];
+    const int32_t* cd = &coarse->cell_dofs[(size_t)pc * coarse->nd];
+    for (int tf = 0; tf < NF; ++tf) {
+      const int fi[3] = {tf % Qf, (tf / Qf) % Qf, tf / (Qf * Qf)};
+      double xi[3];
+      for (int d = 0; d < 3; ++d) xi[d] = 0.5 * (cpos[d] + fi[d] / double(Qf - 1));
+      for (int c = 0; c < 3; ++c) {
+        const int32_t fid = fd[fine->ldof[c * NF + fine->nodeperm[tf]]];
+        for (int tc = 0; tc < NC; ++tc) {
+          const int ci[3] = {tc % Qc, (tc / Qc) % Qc, tc / (Qc * Qc)};
+          double e = 1;
+          for (int d = 0; d < 3; ++d) e *= lag(Qc, xi[d], ci[d]);
+          e = snap(e);
+          if (e == 0.0) continue;
+          const int32_t cid = cd[coarse->ldof[c * NC + coarse->nodeperm[tc]]];
+          if (cid > 0 && fid <= 0)
+            return set_err(ctx, HFK_ESTATE, "HyperFEMKernels: the two levels have different Dirichlet boundaries (a "
+                                            "fine Dirichlet dof is interpolated from a free coarse dof)");
+          if (cid <= 0 || fid <= 0) continue;
+          auto& row = rows[fid - 1];
+          bool seen = false;
+          for (auto& pr : row) seen = seen || pr.first == cid - 1;
+          if (!seen) row.emplace_back(cid - 1, e);
+        }
+      }
+    }
+    // injection: the coarse nodes of the parent that are nodes of this child
+    for (int tc = 0; tc < NC; ++tc) {
+      const int ci[3] = {tc % Qc, (tc / Qc) % Qc, tc / (Qc * Qc)};
+      int fi[3];
+      bool inside = true;
+      for (int d = 0; d < 3; ++d) {
+        const double t = (2.0 * ci[d] / double(Qc - 1) - cpos[d]) * (Qf - 1);
+        fi[d] = (int)std::lround(t);
+        inside = inside && t > -1e-9 && t < Qf - 1 + 1e-9 && std::fabs(t - fi[d]) < 1e-9;
+      }
+      if (!inside) continue;
+      const int tf = fi[0] + Qf * (fi[1] + Qf * fi[2]);
+      for (int c = 0; c < 3; ++c) {
+        const int32_t cid = cd[coarse->ldof[c * NC + coarse->nodeperm[tc]]];
+        const int32_t fid = fd[fine->ldof[c * NF + fine->nodeperm[tf]]];
+        if ((cid > 0) != (fid > 0))
+          return set_err(ctx, HFK_ESTATE, "HyperFEMKernels: the two levels have different Dirichlet boundaries (a node "
+                                          "is free on one level and Dirichlet on the other)");
+        if (cid > 0) inj[cid - 1] = fid;
+        else injd[-cid - 1] = fid;
+      }
+    }
+  }
+  for (int32_t v : inj)
+    if (v == 0) return set_err(ctx, HFK_ESTATE, "HyperFEMKernels: a coarse node is not a node of the fine mesh");
+  for (int32_t v : injd)
+    if (v == 0) return set_err(ctx, HFK_ESTATE, "HyperFEMKernels: a coarse Dirichlet node is not a node of the fine mesh");
+  // P (CSR over the fine free dofs, columns sorted) and R = P'
+  std::vector<int64_t> prp((size_t)nf + 1, 0), rrp((size_t)nc + 1, 0);
+  for (int64_t i = 0; i < nf; ++i) {
+    std::sort(rows[i].begin(), rows[i].end());
+    prp[i + 1] = prp[i] + (int64_t)rows[i].size();
+    for (auto& pr : rows[i]) rrp[pr.first + 1]++;
+  }
+  for (int64_t j = 0; j < nc; ++j) rrp[j + 1] += rrp[j];
+  std::vector<int32_t> pcol((size_t)prp[nf]), rcol((size_t)prp[nf]);
+  std::vector<double> pval((size_t)prp[nf]), rval((size_t)prp[nf]);
+  std::vector<int64_t> fill(rrp.begin(), rrp.end() - 1);
+  for (int64_t i = 0; i < nf; ++i)
+    for (size_t k = 0; k < rows[i].size(); ++k) {
+      pcol[prp[i] + k] = rows[i][k].first;
+      pval[prp[i] + k] = rows[i][k].second;
+      const int64_t q = fill[rows[i][k].first]++;
+      rcol[q] = (int32_t)i;
+      rval[q] = rows[i][k].second;
+    }
+  auto up = [&](DevBuf& b, const void* h, size_t bytes) -> hipError_t {
+    hipError_t e = b.alloc(std::max<size_t>(bytes, 8));
+    if (e == hipSuccess && bytes) e = hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  HIPCHK(ctx, up(T->prp, prp.data(), prp.size() * sizeof(int64_t)));
+  HIPCHK(ctx, up(T->pcol, pcol.data(), pcol.size() * sizeof(int32_t)));
+  HIPCHK(ctx, up(T->pval, pval.data(), pval.size() * sizeof(double)));
+  HIPCHK(ctx, up(T->rrp, rrp.data(), rrp.size() * sizeof(int64_t)));
+  HIPCHK(ctx, up(T->rcol, rcol.data(), rcol.size() * sizeof(int32_t)));
+  HIPCHK(ctx, up(T->rval, rval.data(), rval.size() * sizeof(double)));
+  HIPCHK(ctx, up(T->inj, inj.data(), inj.size() * sizeof(int32_t)));
+  HIPCHK(ctx, up(T->injd, injd.data(), injd.size() * sizeof(int32_t)));
+  return HFK_OK;
+}
+
 int transfer_build(hfk_op* coarse, hfk_op* fine, hfk_transfer* T) {
   hfk_ctx* ctx = fine->ctx;
-  if (coarse->ncells != fine->ncells) return set_err(ctx, HFK_EDIM, "HyperFEMKernels: the two levels live on different meshes");
+  if (coarse->ncells != fine->ncells || fine->refines == coarse) return transfer_build_nested(coarse, fine, T);
   if (coarse->Q >= fine->Q) return set_err(ctx, HFK_EARG, "HyperFEMKernels: the coarse space has more nodes per cell than the fine one");
   const int Qf = fine->Q, Qc = coarse->Q, NF = fine->nn, NC = coarse->nn;
   T->coarse = coarse;
@@ -445,6 +567,10 @@ int transfer_build(hfk_op* coarse, hfk_op* fine, hfk_transfer* T) {
 // prolongate! (src/transfers.jl:238-245): yf = P xc, Dirichlet dofs read as zero and never written
 int do_prolongate(hfk_transfer* T, const double* xc, double* yf, hipStream_t s) {
   hfk_op* f = T->fine;
+  if (T->nested) {
+    HIPCHK(f->ctx, csr_spmv(f->nfree, T->prp.as<int64_t>(), T->pcol.as<int32_t>(), T->pval.as<double>(), xc, yf, s));
+    return HFK_OK;
+  }
   HIPCHK(f->ctx, launch_fill(yf, f->nfree, 0.0, s));
   HIPCHK(f->ctx, transfer_prolongate(f->nslots(), T->NF, T->NC, f->ids.as<int32_t>(), T->cid.as<int32_t>(),
                                      T->E.as<double>(), xc, yf, s));
@@ -454,6 +580,11 @@ int do_prolongate(hfk_transfer* T, const double* xc, double* yf, hipStream_t s) 
 // restrict! (src/transfers.jl:252-259): yc = P' yf, colour by colour of the fine cells
 int do_restrict(hfk_transfer* T, const double* yf, double* yc, hipStream_t s) {
   hfk_op* f = T->fine;
+  if (T->nested) {
+    HIPCHK(f->ctx, csr_spmv(T->coarse->nfree, T->rrp.as<int64_t>(), T->rcol.as<int32_t>(), T->rval.as<double>(), yf, yc,
+                            s));
+    return HFK_OK;
+  }
   HIPCHK(f->ctx, launch_fill(yc, T->coarse->nfree, 0.0, s));
   for (const BatchRange& r : f->color_batches)
     HIPCHK(f->ctx, transfer_restrict((int64_t)r.first * f->C, (int64_t)r.count * f->C, T->NF, T->NC, f->ids.as<int32_t>(),
@@ -464,6 +595,11 @@ int do_restrict(hfk_transfer* T, const double* yf, double* yc, hipStream_t s) {
 // inject! (src/transfers.jl:285-313)
 int do_inject(hfk_transfer* T, const double* xf, const double* df, double* xc, double* dc, hipStream_t s) {
   hfk_op* f = T->fine;
+  if (T->nested) {
+    HIPCHK(f->ctx, inject_map(T->coarse->nfree, T->inj.as<int32_t>(), xf, df, xc, s));
+    if (dc) HIPCHK(f->ctx, inject_map(T->coarse->ndir, T->injd.as<int32_t>(), xf, df, dc, s));
+    return HFK_OK;
+  }
   HIPCHK(f->ctx, transfer_inject(f->nslots(), T->NF, T->NC, f->ids.as<int32_t>(), T->cid.as<int32_t>(),
                                  T->Einj.as<double>(), xf, df, xc, dc, s));
   return HFK_OK;
@@ -479,7 +615,8 @@ struct hfk_pmg {
   std::vector<std::unique_ptr<hfk_transfer>> P;  // P[l]: level l+1 -> level l
   std::vector<std::unique_ptr<hfk_cheb>> cheb;   // smoothed levels 0..L-2
   std::vector<std::unique_ptr<DevBuf>> r, e, bc, xc;
-  int coarse_kind = 0, coarse_maxiter = 500;
+  int coarse_kind = 0, coarse_maxiter = 500;  // 0: block-Jacobi PCG to coarse_rtol; 1: direct (dense Cholesky)
+  DevBuf cinv;                                 // kind 1: the inverse of the coarse matrix, n x n
   hfk_net* net = nullptr;  // learned coarse correction (not owned)
   int net_mode = 0;
   double coarse_rtol = 1e-2;
@@ -595,6 +732,56 @@ hipError_t coarse_spmv(hfk_pmg* g, const double* x, double* y, double alpha, dou
                     alpha, beta, s);
 }
 
+// The direct coarse solve (the role of the reference's LUSolver on the coarsest level, src/pmultigrid.jl:229-239):
+// the assembled matrix of a small coarsest level (at most kMaxDirect free dofs, the bottom of an h-hierarchy) is
+// factorised by Cholesky on the host once per set-up and its inverse kept on the device, so that every cycle applies
+// it with one dense product.  *ok = false when the matrix is not symmetric positive definite (or not finite).
+constexpr int64_t kMaxDirect = 4000;
+
+int coarse_direct_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
+  hfk_op* c = g->ops.back();
+  const int64_t n = c->nfree;
+  std::vector<double> val((size_t)g->nnz);
+  HIPCHK(g->ctx, hipMemcpyAsync(val.data(), g->nzval.p, val.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(g->ctx, hipStreamSynchronize(s));
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (int64_t col = 0; col < n; ++col)
+    for (int64_t p = c->colptr[col] - 1; p < c->colptr[col + 1] - 1; ++p) A[(size_t)(c->rowval[p] - 1) * n + col] = val[p];
+  *ok = false;
+  // Cholesky A = L L' in place (lower triangle), on the symmetric part
+  for (int64_t j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int64_t k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0) || !std::isfinite(d)) return HFK_OK;
+    const double ljj = std::sqrt(d);
+    A[(size_t)j * n + j] = ljj;
+    for (int64_t i = j + 1; i < n; ++i) {
+      double v = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+      for (int64_t k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = v / ljj;
+    }
+  }
+  // inverse, column by column: L y = e_k, L' x = y
+  std::vector<double> X((size_t)n * n), y((size_t)n);
+  for (int64_t k = 0; k < n; ++k) {
+    for (int64_t i = 0; i < n; ++i) {
+      double v = i == k ? 1.0 : 0.0;
+      for (int64_t m = 0; m < i; ++m) v -= A[(size_t)i * n + m] * y[m];
+      y[i] = v / A[(size_t)i * n + i];
+    }
+    for (int64_t i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      for (int64_t m = i + 1; m < n; ++m) v -= A[(size_t)m * n + i] * X[(size_t)m * n + k];
+      X[(size_t)i * n + k] = v / A[(size_t)i * n + i];
+    }
+  }
+  HIPCHK(g->ctx, g->cinv.alloc(X.size() * sizeof(double)));
+  HIPCHK(g->ctx, hipMemcpyAsync(g->cinv.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(g->ctx, hipStreamSynchronize(s));  // X is read from host memory by the copy above
+  *ok = true;
+  return HFK_OK;
+}
+
 int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
   hfk_op* c = g->ops.back();
   int rc;
@@ -622,6 +809,14 @@ int coarse_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
     if (g->bell.has32) HIPCHK(g->ctx, hipMemsetAsync(g->bell.v32.p, 0, nv * sizeof(float), s));
     HIPCHK(g->ctx, bell_fill(g->nnz, g->bell.pos.as<int32_t>(), g->nzval.as<double>(), g->bell.v64.as<double>(),
                              g->bell.has32 ? g->bell.v32.as<float>() : nullptr, s));
+  }
+  if (g->coarse_kind == 1) {
+    bool dok = false;
+    if ((rc = coarse_direct_setup(g, s, &dok))) return rc;
+    if (!dok) {
+      *ok = false;
+      return HFK_OK;
+    }
   }
   NodeBlocks& nb = c->blocks;
   HIPCHK(g->ctx, csr_blocks(nb.nblocks, nb.dofs.as<int32_t>(), g->colptr.as<int32_t>(), g->rowval.as<int32_t>(),
@@ -722,6 +917,10 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
     if (!usable) mode = 0;
   }
   if (mode == 4) return net_cycle(g, b, x, s);
+  if (mode == 0 && g->coarse_kind == 1) {  // direct: x = A^-1 b
+    HIPCHK(g->ctx, dense_matvec(n, g->cinv.as<double>(), b, x, s));
+    return HFK_OK;
+  }
   const bool flexible = mode == 3;
   auto precondition = [&](const double* rin, double* zout) -> int {
     if (flexible) return net_cycle(g, rin, zout, s);
@@ -1307,8 +1506,11 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
   int rc = op_bind(levels[0]);
   if (rc) return rc;
   if ((rc = check_cheb_args(ctx, degree, range, inflate, eig_iters))) return rc;
-  if (coarse_kind != 0 || !(coarse_rtol > 0) || coarse_maxiter < 1)
+  if ((coarse_kind != 0 && coarse_kind != 1) || !(coarse_rtol > 0) || coarse_maxiter < 1)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unsupported coarse solver settings");
+  if (coarse_kind == 1 && levels[nlevels - 1] && levels[nlevels - 1]->nfree > kMaxDirect)
+    return set_err(ctx, HFK_EDIM, "HyperFEMKernels: PMultigrid: the direct coarse solve takes at most " +
+                                      std::to_string(kMaxDirect) + " coarse dofs (add coarser levels)");
   if (smoother_storage < -1 || smoother_storage > 5)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unknown smoother storage");
   std::unique_ptr<hfk_pmg> g(new hfk_pmg());

@@ -3,6 +3,8 @@
 // level.  Host orchestration is in hfk_pmg.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 
 #include "hfk_solver.hpp"
@@ -603,6 +605,65 @@ hipError_t transfer_inject(int64_t nslot, int NF, int NC, const int32_t* fid, co
 }
 hipError_t count_nonfinite(int64_t n, const double* x, int* bad, hipStream_t s) {
   hipLaunchKernelGGL(k_count_nonfinite, grid_for(n), dim3(256), 0, s, n, x, bad);
+  return hipGetLastError();
+}
+
+
+namespace {
+
+__global__ void k_csr_spmv(int64_t nrows, const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                           const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) v += val[k] * x[col[k]];
+    y[i] = v;
+  }
+}
+
+__global__ void k_inject_map(int64_t n, const int32_t* __restrict__ src, const double* __restrict__ xf,
+                             const double* __restrict__ df, double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t k = src[i];
+    out[i] = k > 0 ? xf[k - 1] : df[-k - 1];
+  }
+}
+
+// one work-group per row: a fixed-order tree over the row (deterministic)
+__global__ void k_dense_matvec(int64_t n, const double* __restrict__ A, const double* __restrict__ x,
+                               double* __restrict__ y) {
+  __shared__ double red[256];
+  const int64_t i = blockIdx.x;
+  double v = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) v += A[i * n + j] * x[j];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[i] = red[0];
+}
+
+int grid_of(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
+
+}  // namespace
+
+hipError_t csr_spmv(int64_t nrows, const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                    hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_spmv, dim3(grid_of(nrows)), dim3(256), 0, s, nrows, rp, col, val, x, y);
+  return hipGetLastError();
+}
+
+hipError_t inject_map(int64_t n, const int32_t* src, const double* xf, const double* df, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inject_map, dim3(grid_of(n)), dim3(256), 0, s, n, src, xf, df, out);
+  return hipGetLastError();
+}
+
+hipError_t dense_matvec(int64_t n, const double* A, const double* x, double* y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dense_matvec, dim3((unsigned)n), dim3(256), 0, s, n, A, x, y);
   return hipGetLastError();
 }
 

@@ -54,5 +54,12 @@ hipError_t transfer_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const
 hipError_t transfer_inject(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* Einj,
                            const double* xf, const double* df, double* xc, double* dc, hipStream_t s);
 hipError_t count_nonfinite(int64_t n, const double* x, int* bad, hipStream_t s);
+// y = A x for a CSR matrix with 0-based columns (rp: nrows + 1 offsets)
+hipError_t csr_spmv(int64_t nrows, const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                    hipStream_t s);
+// out[i] = src[i] > 0 ? xf[src[i] - 1] : df[-src[i] - 1] (nodal injection between nested meshes)
+hipError_t inject_map(int64_t n, const int32_t* src, const double* xf, const double* df, double* out, hipStream_t s);
+// y = A x with a dense n x n matrix (row-major)
+hipError_t dense_matvec(int64_t n, const double* A, const double* x, double* y, hipStream_t s);
 
 }  // namespace hfk

@@ -34,6 +34,7 @@ SIGNATURES = {
                                 C.c_int, C.c_double, C.c_double, C.POINTER(_vp)]),
     "hfk_op_destroy": (C.c_int, [_vp]),
     "hfk_op_set_dirichlet": (C.c_int, [_vp, _f64p]),
+    "hfk_op_set_refinement": (C.c_int, [_vp, _vp, _i32p, _i32p]),
     "hfk_op_set_parameters": (C.c_int, [_vp, C.c_double, C.c_double]),
     "hfk_op_set_mu_field": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_traction": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),

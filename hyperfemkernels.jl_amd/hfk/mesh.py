@@ -232,3 +232,30 @@ def boundary_faces(space: HexSpace, axis: int, side: int):
     on = idx == (0 if side == 0 else space.n[axis] - 1)
     face_cell = cell[on].astype(np.int32)
     return face_cell, np.full(face_cell.size, 2 * axis + side, dtype=np.int32)
+
+
+def refinement(fine: HexSpace, coarse: HexSpace):
+    """(parent, child) of every cell of ``fine``, whose Cartesian mesh halves every cell of ``coarse`` (n_fine =
+    2 n_coarse on the same box): fine cell (i, j, k) lies in coarse cell (i//2, j//2, k//2) as child
+    (i%2) + 2 (j%2) + 4 (k%2) -- the arrays hfk_op_set_refinement takes (Gridap's refinement glue on a Julia host)."""
+    nf = tuple(int(v) for v in fine.n)
+    nc = tuple(int(v) for v in coarse.n)
+    if any(a != 2 * b for a, b in zip(nf, nc)) or tuple(fine.domain) != tuple(coarse.domain):
+        raise ValueError("HyperFEMKernels: the fine mesh must halve every cell of the coarse one on the same box")
+    i, j, k = np.meshgrid(np.arange(nf[0]), np.arange(nf[1]), np.arange(nf[2]), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).reshape(-1) for a in (i, j, k))  # cells lexicographic, x fastest
+    parent = (i // 2) + nc[0] * ((j // 2) + nc[1] * (k // 2))
+    child = (i % 2) + 2 * (j % 2) + 4 * (k % 2)
+    return parent.astype(np.int32), child.astype(np.int32)
+
+
+def h_hierarchy(space: HexSpace, levels: int, dirichlet=None):
+    """Q1 spaces on the meshes of ``space`` coarsened ``levels`` times by 2 in every direction (finest first, the
+    first on space's own mesh), with the same Dirichlet rule: the h-levels below a p-hierarchy's Q1 level."""
+    out = [hex_space(space.n, 1, domain=space.domain, dirichlet=dirichlet)]
+    for _ in range(levels):
+        n = out[-1].n
+        if any(int(v) % 2 for v in n):
+            raise ValueError(f"HyperFEMKernels: the mesh {tuple(n)} cannot be halved")
+        out.append(hex_space(tuple(int(v) // 2 for v in n), 1, domain=space.domain, dirichlet=dirichlet))
+    return out

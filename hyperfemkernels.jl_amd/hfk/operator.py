@@ -107,6 +107,23 @@ class KernelOperator:
             self.set_dirichlet(dirichlet_values)
 
     # --- data --------------------------------------------------------------------------------------------
+    def set_refinement(self, coarse, parent=None, child=None):
+        """Declare this operator's mesh a nested refinement of ``coarse``'s (an h-level pair of a p-multigrid):
+        ``parent``/``child`` per cell (default ``mesh.refinement`` of the two spaces); ``coarse=None`` removes it."""
+        if coarse is None:
+            check(self.lib.hfk_op_set_refinement(self.handle, None, None, None), self.ctx.handle)
+            return self
+        from .mesh import refinement
+
+        if parent is None:
+            parent, child = refinement(self.space, coarse.space)
+        pa = np.ascontiguousarray(parent, dtype=np.int32)
+        ch = np.ascontiguousarray(child, dtype=np.int32)
+        check(self.lib.hfk_op_set_refinement(self.handle, coarse.handle, ptr(pa, C.c_int32), ptr(ch, C.c_int32)),
+              self.ctx.handle)
+        self._refines = coarse  # kept alive while declared
+        return self
+
     def set_dirichlet(self, dvals):
         d = np.ascontiguousarray(dvals, dtype=np.float64)
         if d.size != self.ndir:

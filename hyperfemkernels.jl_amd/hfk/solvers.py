@@ -167,7 +167,7 @@ class PMultigrid:
 
     def __init__(self, levels, smoother: Chebyshev = Chebyshev(), coarse_rtol: float = 1e-2,
                  coarse_maxiter: int = 500, smoother_storage: str | None = None, coarse_every: int = 1,
-                 coarse_precision: str = "fp64"):
+                 coarse_precision: str = "fp64", coarse: str = "pcg"):
         if len(levels) < 2:
             raise ValueError("HyperFEMKernels: PMultigrid needs at least two levels")
         if coarse_every < 1:
@@ -177,8 +177,13 @@ class PMultigrid:
         h = C.c_void_p()
         lib = levels[0].lib
         sst = -1 if smoother_storage is None else STORE[smoother_storage]
+        # coarse: "pcg" (block-Jacobi PCG to coarse_rtol) or "direct" (Cholesky of a small coarsest level, the
+        # bottom of an h-hierarchy: a fixed cycle)
+        if coarse not in ("pcg", "direct"):
+            raise ValueError("HyperFEMKernels: PMultigrid: coarse must be 'pcg' or 'direct'")
         check(lib.hfk_pmg_create(arr, len(levels), smoother.degree, smoother.range, smoother.inflate,
-                                 smoother.eig_iters, sst, 0, float(coarse_rtol), int(coarse_maxiter), C.byref(h)),
+                                 smoother.eig_iters, sst, 0 if coarse == "pcg" else 1, float(coarse_rtol),
+                                 int(coarse_maxiter), C.byref(h)),
               levels[0].ctx.handle)
         self.handle, self.lib, self.smoother = h, lib, smoother
         self.ok = self.indefinite = False

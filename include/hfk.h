@@ -74,6 +74,12 @@ int hfk_op_create(hfk_ctx* ctx, int Q, int P, int64_t ncells, const int32_t* cel
                   const int32_t* nodeperm, const double* coords, int64_t nfree, int64_t ndir, int law,
                   double lam, double mu, hfk_op** op);
 int hfk_op_destroy(hfk_op* op);
+/* Declare `fine` a nested refinement of `coarse`'s mesh (no counterpart in the reference, whose p-multigrid keeps
+ * one mesh): cell f of `fine` lies in cell parent[f] of `coarse` as its child child[f] = cx + 2 cy + 4 cz (cx, cy,
+ * cz in {0, 1}: the half of the coarse reference cell along each direction).  A p-multigrid whose consecutive levels
+ * are such a pair transfers between them geometrically (trilinear prolongation, its transpose, nodal injection of
+ * the state): an h-hierarchy below the p-levels.  coarse = NULL removes the declaration. */
+int hfk_op_set_refinement(hfk_op* fine, hfk_op* coarse, const int32_t* parent, const int32_t* child);
 /* Dirichlet values, re-read by every assembly (refresh_dirichlet_values!, src/assembly.jl:393-410). */
 int hfk_op_set_dirichlet(hfk_op* op, const double* dvals);
 /* The numbers of the form, rebound without rebuilding anything (the fast path of update(op, res, jac) for a capture
@@ -236,7 +242,10 @@ typedef struct hfk_pmg hfk_pmg;
 /* PMultigrid(spaces; smoother = Chebyshev(...), coarse): levels[0] is the Newton's operator (prepared with
  * hfk_mf_prepare*), levels[1..n-2] matrix-free, levels[n-1] assembled.  coarse_kind 0: block-Jacobi PCG on the
  * assembled coarse matrix to coarse_rtol (at most coarse_maxiter iterations; a variable cycle, so the outer
- * Krylov method should be the flexible CG).  The coarse matrix must be symmetric.  smoother_storage (an
+ * Krylov method should be the flexible CG); coarse_kind 1: a direct solve (the reference's LUSolver): Cholesky of
+ * the assembled coarse matrix at every coarse set-up, at most 4000 coarse dofs (the bottom of an h-hierarchy, see
+ * hfk_op_set_refinement), a fixed cycle (CG and GMRES apply).  The coarse matrix must be symmetric.  Consecutive
+ * levels on one mesh are p-levels; a level declared a refinement of the next one is an h-level.  smoother_storage (an
  * HFK_STORE_* value, or -1 for the working operator's own tangents) selects the tangents every smoother and
  * matrix-free coarse level applies: HFK_STORE_SYM32 gives FP32 level applies and smoothers while the outer
  * Krylov product keeps the working operator's FP64 tangents (mixed precision). */

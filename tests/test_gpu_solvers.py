@@ -461,3 +461,98 @@ def test_plain_cg_with_symmetric_cycle(hip_ctx, oracle):
     print(f"cg + symmetric cycle: {res}", flush=True)
     assert res.converged and not res.indefinite
     assert rel(xs, spla.spsolve(orc.csc(x).tocsc(), b)) < 1e-8
+
+
+def h_levels(hip_ctx, n, amp=0.01, depth=2, coarse_storage=None):
+    """Q2 on n^3 -> Q1 on n^3 (fine 27-point rule) -> Q1 on n^3/8 -> ... (depth halvings, each rediscretised with its
+    own 8-point rule and declared a refinement of the next), the last assembled for the direct solve."""
+    from hfk.mesh import h_hierarchy
+    from hfk.synthetic import smooth_field
+
+    fs = hex_space((n, n, n), 2, dirichlet=dirichlet_closed_x0)
+    u = smooth_field(2, amp=amp)
+    x = fs.free_values(u)
+    fop = KernelOperator(HyperelasticForm("neohookean", degree=4), fs, jacobian="matrixfree",
+                         dirichlet_values=fs.dirichlet_values(u), ctx=hip_ctx)
+    hs = h_hierarchy(fs, depth, dirichlet=dirichlet_closed_x0)
+    ops = [fop]
+    for i, sp in enumerate(hs):
+        deg = 4 if i == 0 else 2
+        kind = "assembled" if i == len(hs) - 1 else "matrixfree"
+        ops.append(KernelOperator(HyperelasticForm("neohookean", degree=deg), sp, jacobian=kind, ctx=hip_ctx))
+    for a, b in zip(ops[1:-1], ops[2:]):
+        a.set_refinement(b)
+    mf = MatrixFreeJacobian(fop).prepare(x)
+    return fs, ops, mf, x
+
+
+def test_nested_transfers(hip_ctx):
+    """Q1 on 4 x 4 x 4 cells refining Q1 on 2 x 2 x 2 (closed x = 0 Dirichlet face on both): prolongation reproduces
+    a trilinear field exactly at the fine free nodes, restriction is its transpose (adjointness), injection of a
+    prolongated field gives it back, and levels with other Dirichlet sets or without the declaration are refused."""
+    from hfk._lib import HFKError
+    from hfk.mesh import dirichlet_closed_x_faces
+
+    fs = hex_space((4, 4, 4), 1, dirichlet=dirichlet_closed_x0)
+    cs = hex_space((2, 2, 2), 1, dirichlet=dirichlet_closed_x0)
+    form = HyperelasticForm("svk", degree=2)
+    fop = KernelOperator(form, fs, jacobian="matrixfree", ctx=hip_ctx)
+    cop = KernelOperator(form, cs, jacobian="matrixfree", ctx=hip_ctx)
+    with pytest.raises(HFKError):
+        Prolongation(cop, fop)  # not declared a refinement
+    fop.set_refinement(cop)
+    P = Prolongation(cop, fop)
+    # trilinear fields that vanish on the Dirichlet face x = 0 (corrections carry no Dirichlet values)
+    f = lambda X: X[:, 0:1] * np.stack([1 + 2 * X[:, 1] - 0.5 * X[:, 1] * X[:, 2], X[:, 2] - X[:, 1], 0.3 + X[:, 1] * X[:, 2]],
+                                       axis=1)
+    assert rel(P @ cs.free_values(f), fs.free_values(f)) < 1e-14
+    rng = np.random.default_rng(4)
+    xc, yf = rng.standard_normal(cs.nfree), rng.standard_normal(fs.nfree)
+    yc = P.restrict(np.zeros(cs.nfree), yf)
+    assert abs(yc @ xc - yf @ (P @ xc)) < 1e-13 * np.linalg.norm(yc) * np.linalg.norm(xc)
+    xi, di = P.inject(P @ xc, np.zeros(fs.ndir))
+    assert rel(xi, xc) < 1e-15
+    fs2 = hex_space((4, 4, 4), 1, dirichlet=dirichlet_closed_x_faces)
+    f2 = KernelOperator(form, fs2, jacobian="matrixfree", ctx=hip_ctx)
+    f2.set_refinement(cop)
+    with pytest.raises(HFKError, match="Dirichlet"):
+        Prolongation(cop, f2)
+
+
+@pytest.mark.parametrize("krylov", ["cg", "gmres", "fcg"])
+def test_h_hierarchy_solution_and_h_independence(hip_ctx, oracle, krylov):
+    """p-multigrid with an h-hierarchy below the Q1 level and a direct solve at the bottom (a fixed, symmetric
+    cycle): CG, GMRES and flexible CG reach the direct solution of the Q2 system (1e-8); iterations at most 30 and
+    nearly independent of h over n = 4, 8, 16."""
+    its = []
+    for n in (4, 8, 16):
+        fs, ops, mf, x = h_levels(hip_ctx, n, depth={4: 1, 8: 2, 16: 3}[n])
+        pmg = PMultigrid(ops, Chebyshev(degree=4), coarse="direct").update()
+        assert pmg.ok
+        b = np.random.default_rng(n + 50).standard_normal(fs.nfree)
+        xs = np.zeros(fs.nfree)
+        res = MatrixFreeSolver(krylov, rtol=1e-10, maxiter=200, preconditioner=pmg).solve(xs, mf, b)
+        print(f"h-hierarchy {krylov} n={n} levels={len(ops)}: {res}", flush=True)
+        assert res.converged and not res.indefinite
+        its.append(res.iterations)
+        if n == 8:
+            from hfk.synthetic import smooth_field
+
+            orc = oracle.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(smooth_field(2, amp=0.01)))
+            assert rel(xs, spla.spsolve(orc.csc(x).tocsc(), b)) < 1e-8
+    assert max(its) <= 30 and its[2] <= its[1] + 6
+
+
+def test_h_hierarchy_cycle_is_spd(hip_ctx):
+    """With the direct coarse solve the V-cycle through the h-levels is a fixed symmetric positive definite
+    operator (test/solvers/test_pmultigrid.jl:74-84): Q2 on 4^3, Q1 on 4^3 and 2^3."""
+    fs, ops, mf, x = h_levels(hip_ctx, 4, amp=0.0, depth=1)
+    pmg = PMultigrid(ops, Chebyshev(degree=2), coarse="direct").update()
+    n = fs.nfree
+    B = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        B[:, j] = pmg.ldiv(np.zeros(n), e)
+    assert np.linalg.norm(B - B.T) <= 1e-12 * np.linalg.norm(B)
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > 0
