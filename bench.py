@@ -316,8 +316,31 @@ def newton_gmres_leg(rtol=1e-10, maxit=20, lin_rtol=1e-12):
             "gmres_iterations": lin, "newton_total_s": t, "s_per_newton_step": t / max(it, 1)}
 
 
+def h_levels(fs, fine_degree, dirichlet, mu=None, bottom=4000):
+    """The levels below a p-hierarchy's working level: Q1 on the working mesh (matrix-free, the working level's
+    quadrature, src/pmultigrid.jl:248-251), then Q1 on the mesh halved in every direction (rediscretised with its own
+    8-point rule, declared a refinement of the next) until the coarsest has at most `bottom` free dofs or cannot be
+    halved; the coarsest assembled for the direct solve.  The shear modulus function is set on every level."""
+    from hfk import HyperelasticForm, KernelOperator
+    from hfk.mesh import hex_space
+
+    spaces = [hex_space(fs.n, 1, domain=fs.domain, dirichlet=dirichlet)]
+    while spaces[-1].nfree > bottom and all(int(v) % 2 == 0 for v in spaces[-1].n):
+        spaces.append(hex_space(tuple(int(v) // 2 for v in spaces[-1].n), 1, domain=fs.domain, dirichlet=dirichlet))
+    ops = []
+    for i, sp in enumerate(spaces):
+        form = HyperelasticForm("neohookean", 2.0, 1.0, degree=fine_degree if i == 0 else 2)
+        ops.append(KernelOperator(form, sp, jacobian="assembled" if i == len(spaces) - 1 else "matrixfree"))
+    for a, b in zip(ops[:-1], ops[1:]):
+        if a.space.n != b.space.n:
+            a.set_refinement(b)
+    if mu is not None:
+        set_lognormal_mu(mu, *ops)
+    return ops
+
+
 def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, coarse_maxiter=400, rtol=1e-8,
-              learned=None, order=2, mu=None, state_amp=1e-2, coarse_precision="fp64"):
+              learned=None, order=2, mu=None, state_amp=1e-2, coarse_precision="fp64", coarse="pcg"):
     """CG-type solve J(x) du = -R(x) to ||b - J du|| <= rtol ||b|| with one p-multigrid V-cycle per iteration:
     Q2 -> assembled Q1 (fine quadrature), Chebyshev degree `degree` (range 20, inflate 1.1, 10 Arnoldi steps),
     coarse block-Jacobi PCG to coarse_rtol; flexible CG outside (the inexact coarse solve makes the cycle vary).
@@ -346,22 +369,30 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     x = fs.free_values(u)
     form = HyperelasticForm("neohookean", 2.0, 1.0, degree=2 * order)
     fop = KernelOperator(form, fs, jacobian="matrixfree", dirichlet_values=fs.dirichlet_values(u))
-    cop = KernelOperator(form, cs, jacobian="assembled")
     t_mu = None
-    if mu is not None:  # lognormal shear modulus at the quadrature points (the coarse level keeps the fine rule)
-        t0 = time.perf_counter()
-        set_lognormal_mu(mu, fop, cop)
+    t0 = time.perf_counter()
+    if coarse == "hmg":  # Q1 h-levels below the working level, a direct solve at the bottom: a fixed cycle
+        lower = h_levels(fs, 2 * order, dirichlet_closed_x_faces, mu=mu)
+        if mu is not None:
+            set_lognormal_mu(mu, fop)
+    else:
+        lower = [KernelOperator(form, cs, jacobian="assembled")]
+        if mu is not None:  # lognormal shear modulus at the quadrature points (the coarse level keeps the fine rule)
+            set_lognormal_mu(mu, fop, *lower)
+    if mu is not None:
         t_mu = time.perf_counter() - t0
     b = -fop.residual(np.zeros(fs.nfree), x)
     mf = MatrixFreeJacobian(fop, outer_storage)
-    pmg = PMultigrid([fop, cop], Chebyshev(degree=degree), coarse_rtol=coarse_rtol, coarse_maxiter=coarse_maxiter,
-                     smoother_storage=smoother_storage, coarse_precision=coarse_precision)
+    pmg = PMultigrid([fop, *lower], Chebyshev(degree=degree), coarse_rtol=coarse_rtol, coarse_maxiter=coarse_maxiter,
+                     smoother_storage=smoother_storage, coarse_precision=coarse_precision,
+                     coarse="direct" if coarse == "hmg" else "pcg")
     dev = torch.device("cuda:0")
     xd = torch.from_numpy(x).to(dev)
     bd = torch.from_numpy(b).to(dev)
     sol_d = torch.zeros_like(bd)
     sptr = torch.cuda.current_stream(dev).cuda_stream
-    solver = MatrixFreeSolver("fcg", rtol=rtol, maxiter=200, preconditioner=pmg)
+    # a fixed symmetric cycle (h-levels, direct bottom): CG; an iterative coarse solve: flexible CG
+    solver = MatrixFreeSolver("cg" if coarse == "hmg" else "fcg", rtol=rtol, maxiter=200, preconditioner=pmg)
     out = {}
     train = None
     for rep in range(2):  # the first pass also builds the coarse pattern and the work buffers
@@ -381,6 +412,8 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
         t2 = time.perf_counter()
         info = pmg.info()
         out = {"update_s": t1 - t0, "solve_s": t2 - t1, "iterations": res.iterations, "converged": res.converged,
+               "coarse": "h-levels + direct" if coarse == "hmg" else f"block-Jacobi PCG to {coarse_rtol}",
+               "levels": len(pmg.levels), "outer_storage": outer_storage, "smoother_storage": smoother_storage,
                "coarse_rtol": coarse_rtol, "indefinite": res.indefinite, "pmg_ok": info["ok"],
                "state_amp": state_amp,
                "rel_residual": res.residual / res.rhs_norm, "coarse_iters": info["coarse_iters"] - inner0,
@@ -390,6 +423,28 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
         out.update(train)
     if t_mu is not None:  # both levels, first call: includes the hiprtc compile of the function (cached after)
         out["mu_function_s"] = t_mu
+    return out
+
+
+def solve_summary(res):
+    """The solve half of the metric in one place: seconds to 1e-8 of the 64^3 Q2 legs (FP64 p-MG, mixed precision,
+    learned) and the held-out lognormal fields' speed-ups of the learned solve over the best FP64 p-MG."""
+    def pick(key):
+        leg = res.get(key)
+        return None if leg is None else {"solve_s": leg["solve_s"], "iterations": leg["iterations"],
+                                         "converged": leg["converged"]}
+
+    out = {k: pick(k) for k in ("solve_fp64_pmg", "solve_fp64_hmg", "solve_fp64_fac64_hmg", "solve_mixed_pmg",
+                                "solve_mixed_fac_hmg", "solve_mixed_learned") if k in res}
+    if "solve_learned_speedup_vs_fp64" in res:
+        out["learned_speedup_vs_best_fp64"] = res["solve_learned_speedup_vs_fp64"]
+        out["best_fp64"] = res["solve_learned_speedup_vs_fp64_against"]
+    f = res.get("learned_fields_32")
+    if f:
+        out["fields_32"] = {k: f[k] for k in ("heldout_fields", "fp64_solve_s_mean", "mixed_solve_s_mean",
+                                              "learned_solve_s_mean", "speedup_mean", "speedup_median",
+                                              "speedup_min", "converged", "learned_fallbacks",
+                                              "learned_converged_where_fp64_did")}
     return out
 
 
@@ -459,9 +514,10 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
     as auxiliary input, acting on the diagonally scaled coarse system) trained on the plain p-MG's coarse-smoothed
     residuals of `samples` random right-hand sides on each training field (Adam, step size 1e-3 decayed along a
     half cosine), then on every held-out field (seeds never trained on) FCG + p-MG with the network's two-grid
-    cycle as the coarse correction (mode "cycle"; FP32 smoothers, FP64 outer FCG and coarse matrix), against the
-    plain FP64 p-MG (coarse PCG to 0.3, its best in the 64^3 sweep) and against the same mixed-precision p-MG
-    without the network, all to rtol.  Solve times are the second of two identical solves."""
+    cycle as the coarse correction (mode "cycle"; FP32 smoothers, FP64 outer FCG and coarse matrix, the plain cycle
+    as the fallback on stagnation), against the best plain FP64 p-MG (Q1 h-levels down to a direct solve, plain CG)
+    and against the same hierarchy with FP32 smoothers, all to rtol on the 17-factor FP64 tangent storage.  Solve
+    times are the second of two identical solves."""
     import numpy as np
     import torch
 
@@ -474,9 +530,9 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
     train, keep = [], []
     for sd in train_seeds:
         p = lognormal_problem(n, sd)
-        mf = MatrixFreeJacobian(p["fop"], "sym64").prepare(p["x"])
+        mf = MatrixFreeJacobian(p["fop"], "fac64").prepare(p["x"])
         pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=0.1, coarse_maxiter=400,
-                         smoother_storage="sym32").update()
+                         smoother_storage="fac32").update()
         if not pmg.ok:
             continue
         gen = torch.Generator(device=dev).manual_seed(sd)
@@ -510,28 +566,37 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
     t_train = time.perf_counter() - t0
     del train, keep
     rows = []
-    legs = (("fp64", "full64", None, 0.3, False), ("mixed", "sym64", "sym32", 0.3, False),
-            ("learned", "sym64", "sym32", 0.1, True))
+    # (leg, outer storage, smoother storage, coarse, network)
+    legs = (("fp64", "fac64", None, "hmg", False), ("mixed", "fac64", "fac32", "hmg", False),
+            ("learned", "fac64", "fac32", 0.1, True))
+    from hfk.mesh import dirichlet_closed_x_faces
+
     for sd in test_seeds:
         p = lognormal_problem(n, sd)
+        lower = h_levels(p["fs"], 4, dirichlet_closed_x_faces, mu=p["mu"])
         row = {"seed": sd, "sigma": p["mu"]["sigma"], "ell": p["mu"]["ell"]}
-        for leg, st, sst, crt, use_net in legs:
+        for leg, st, sst, coarse, use_net in legs:
             mf = MatrixFreeJacobian(p["fop"], st).prepare(p["x"])
-            pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=crt, coarse_maxiter=400,
-                             smoother_storage=sst).update()
+            if coarse == "hmg":
+                pmg = PMultigrid([p["fop"], *lower], Chebyshev(degree=4), smoother_storage=sst,
+                                 coarse="direct").update()
+            else:
+                pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=coarse, coarse_maxiter=400,
+                                 smoother_storage=sst).update()
             if use_net:
                 net.set_aux(p["aux"])
                 set_coarse_network(pmg, net, mode, 3, scaled)
-            solver = MatrixFreeSolver("fcg", rtol=rtol, maxiter=300, preconditioner=pmg)
+            solver = MatrixFreeSolver("cg" if coarse == "hmg" else "fcg", rtol=rtol, maxiter=300, preconditioner=pmg)
             for _ in range(2):
                 xs = np.zeros(p["fs"].nfree)
                 t1 = time.perf_counter()
                 res = solver.solve(xs, mf, p["b"])
                 ts = time.perf_counter() - t1
             row[leg] = {"solve_s": ts, "iterations": res.iterations, "converged": res.converged,
-                        "rel_residual": res.residual / res.rhs_norm}
+                        "rel_residual": res.residual / res.rhs_norm, "fell_back": res.fell_back}
             set_coarse_network(pmg, None, "off")
             del pmg, mf
+        del lower
         rows.append(row)
         if progress and len(rows) % 10 == 0:
             print(f"bench: fields: {len(rows)}/{len(test_seeds)} held-out fields solved", file=sys.stderr, flush=True)
@@ -554,6 +619,8 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
             "all_converged": all(r[k]["converged"] for r in rows for k in ("fp64", "mixed", "learned")),
             "converged": {k: int(sum(r[k]["converged"] for r in rows)) for k in ("fp64", "mixed", "learned")},
             "speedup_mean_all_converged": _mean_ratio(rows, ("fp64", "mixed", "learned"), "fp64", "learned"),
+            "learned_fallbacks": int(sum(r["learned"]["fell_back"] for r in rows)),
+            "learned_converged_where_fp64_did": all(r["learned"]["converged"] for r in rows if r["fp64"]["converged"]),
             "fields": rows}
 
 
@@ -895,23 +962,25 @@ def main():
         print("bench: solve legs (FP64 p-MG; mixed precision FP32 smoothers; mixed + learned coarse level)",
               file=sys.stderr, flush=True)
         legs = [s for s in args.solve_legs.split(",") if s]
-        # each leg at the coarse tolerance that was fastest for it over {1e-2, 3e-2, 1e-1, 3e-1} at 64^3
+        # the plain p-multigrid two ways: the reference's two-level structure (Q2 -> assembled Q1, block-Jacobi PCG
+        # coarse solve to the tolerance that was fastest for it over {1e-2, 3e-2, 1e-1, 3e-1} at 64^3), and the best
+        # FP64 p-multigrid here (Q1 h-levels below the working level down to a direct solve, a fixed symmetric
+        # cycle under plain CG), both on the reference's 81-entry layout and on the 17-factor FP64 storage
         if "fp64" in legs:
-            # the FP64 baseline keeps the reference's 81-entry tangent layout; the same FP64 p-MG on the symmetric
-            # 45-entry storage (FP64 arithmetic throughout, fewer bytes per product) is reported beside it
             res["solve_fp64_pmg"] = solve_leg(n, "full64", None, coarse_rtol=0.3)
-            res["solve_fp64_sym64_pmg"] = solve_leg(n, "sym64", None, coarse_rtol=0.3)
+            res["solve_fp64_hmg"] = solve_leg(n, "full64", None, coarse="hmg")
+            res["solve_fp64_fac64_hmg"] = solve_leg(n, "fac64", None, coarse="hmg")
         if "mixed" in legs:
             res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3)
-            # the same with the coarse PCG's products on FP32 copies of the coarse values (FP64 vectors and sums)
-            res["solve_mixed_pmg_fp32_coarse"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3, coarse_precision="fp32")
+            res["solve_mixed_fac_hmg"] = solve_leg(n, "fac64", "fac32", coarse="hmg")
         if "learned" in legs:
-            res["solve_mixed_learned"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.1,
+            res["solve_mixed_learned"] = solve_leg(n, "fac64", "fac32", coarse_rtol=0.1,
                                                    learned={"samples": 160, "steps": 500, "smoother_degree": 3,
                                                             "scaled": False})
         if "fp64" in legs and "learned" in legs:
-            res["solve_learned_speedup_vs_fp64"] = (res["solve_fp64_pmg"]["solve_s"] /
-                                                    res["solve_mixed_learned"]["solve_s"])
+            best = min(("solve_fp64_pmg", "solve_fp64_hmg", "solve_fp64_fac64_hmg"), key=lambda k: res[k]["solve_s"])
+            res["solve_learned_speedup_vs_fp64"] = res[best]["solve_s"] / res["solve_mixed_learned"]["solve_s"]
+            res["solve_learned_speedup_vs_fp64_against"] = best
         if "fields" in legs:
             print("bench: learned coarse correction across lognormal modulus fields (32^3 Q2)", file=sys.stderr,
                   flush=True)
@@ -929,6 +998,8 @@ def main():
             res["q3_48_jv"] = precision_leg(48, 3, stream)
             res["q3_48_solve_fp64_pmg"] = solve_leg(48, "full64", None, coarse_rtol=0.3, order=3)
             res["q3_48_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3)
+            res["q3_48_solve_fp64_fac64_hmg"] = solve_leg(48, "fac64", None, order=3, coarse="hmg")
+            res["q3_48_solve_mixed_fac_hmg"] = solve_leg(48, "fac64", "fac32", order=3, coarse="hmg")
             # with the lognormal mu the state of amplitude 1e-2 makes the tangent indefinite where mu is small
             # (ln J > 2 mu / lam): the CG stops on negative curvature, as the reference's cg with linesearch.  The
             # lognormal legs linearise at the same smooth state scaled to amplitude 3e-3 (positive definite).
@@ -937,6 +1008,10 @@ def main():
                                                               state_amp=3e-3)
             res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
                                                                state_amp=3e-3)
+            res["q3_48_lognormal_solve_fp64_fac64_hmg"] = solve_leg(48, "fac64", None, order=3, mu=lmu,
+                                                                    state_amp=3e-3, coarse="hmg")
+            res["q3_48_lognormal_solve_mixed_fac_hmg"] = solve_leg(48, "fac64", "fac32", order=3, mu=lmu,
+                                                                   state_amp=3e-3, coarse="hmg")
     if not args.no_law:
         print("bench: 64^3 Q2 neo-Hookean with the law compiled at run time (hiprtc) beside the built-in kernels",
               file=sys.stderr, flush=True)
@@ -952,6 +1027,7 @@ def main():
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 64^3 Jv, min of 5 / median of 20)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()
+    res["solve_summary"] = solve_summary(res)
     print(json.dumps(res))
 
 

@@ -619,6 +619,12 @@ struct hfk_pmg {
   DevBuf cinv;                                 // kind 1: the inverse of the coarse matrix, n x n
   hfk_net* net = nullptr;  // learned coarse correction (not owned)
   int net_mode = 0;
+  // the plain cycle as the fallback of a solve with the network: set aside for the rest of a Krylov solve whose
+  // residual has not fallen by fb_factor over the last fb_window iterations (fb_window 0: never)
+  int fb_window = 16;
+  double fb_factor = 0.5;
+  bool net_suspended = false;
+  int64_t fallbacks = 0;
   double coarse_rtol = 1e-2;
   // coarse CSR (CSC of A, 0-based int32)
   DevBuf colptr, rowval, nzval;
@@ -733,52 +739,26 @@ hipError_t coarse_spmv(hfk_pmg* g, const double* x, double* y, double alpha, dou
 }
 
 // The direct coarse solve (the role of the reference's LUSolver on the coarsest level, src/pmultigrid.jl:229-239):
-// the assembled matrix of a small coarsest level (at most kMaxDirect free dofs, the bottom of an h-hierarchy) is
-// factorised by Cholesky on the host once per set-up and its inverse kept on the device, so that every cycle applies
-// it with one dense product.  *ok = false when the matrix is not symmetric positive definite (or not finite).
+// the symmetric part of the assembled matrix of a small coarsest level (at most kMaxDirect free dofs, the bottom of
+// an h-hierarchy) is inverted on the device once per set-up (Gauss-Jordan, symmetric positive definite: no pivoting),
+// so that every cycle applies it with one dense product.  *ok = false when a pivot is not positive or not finite.
 constexpr int64_t kMaxDirect = 4000;
 
 int coarse_direct_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
-  hfk_op* c = g->ops.back();
-  const int64_t n = c->nfree;
-  std::vector<double> val((size_t)g->nnz);
-  HIPCHK(g->ctx, hipMemcpyAsync(val.data(), g->nzval.p, val.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(g->ctx, hipStreamSynchronize(s));
-  std::vector<double> A((size_t)n * n, 0.0);
-  for (int64_t col = 0; col < n; ++col)
-    for (int64_t p = c->colptr[col] - 1; p < c->colptr[col + 1] - 1; ++p) A[(size_t)(c->rowval[p] - 1) * n + col] = val[p];
-  *ok = false;
-  // Cholesky A = L L' in place (lower triangle), on the symmetric part
-  for (int64_t j = 0; j < n; ++j) {
-    double d = A[(size_t)j * n + j];
-    for (int64_t k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-    if (!(d > 0) || !std::isfinite(d)) return HFK_OK;
-    const double ljj = std::sqrt(d);
-    A[(size_t)j * n + j] = ljj;
-    for (int64_t i = j + 1; i < n; ++i) {
-      double v = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
-      for (int64_t k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-      A[(size_t)i * n + j] = v / ljj;
-    }
-  }
-  // inverse, column by column: L y = e_k, L' x = y
-  std::vector<double> X((size_t)n * n), y((size_t)n);
-  for (int64_t k = 0; k < n; ++k) {
-    for (int64_t i = 0; i < n; ++i) {
-      double v = i == k ? 1.0 : 0.0;
-      for (int64_t m = 0; m < i; ++m) v -= A[(size_t)i * n + m] * y[m];
-      y[i] = v / A[(size_t)i * n + i];
-    }
-    for (int64_t i = n - 1; i >= 0; --i) {
-      double v = y[i];
-      for (int64_t m = i + 1; m < n; ++m) v -= A[(size_t)m * n + i] * X[(size_t)m * n + k];
-      X[(size_t)i * n + k] = v / A[(size_t)i * n + i];
-    }
-  }
-  HIPCHK(g->ctx, g->cinv.alloc(X.size() * sizeof(double)));
-  HIPCHK(g->ctx, hipMemcpyAsync(g->cinv.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(g->ctx, hipStreamSynchronize(s));  // X is read from host memory by the copy above
-  *ok = true;
+  const int64_t n = g->ops.back()->nfree;
+  DevBuf W, rc, bad;
+  HIPCHK(g->ctx, W.alloc((size_t)n * n * sizeof(double)));
+  HIPCHK(g->ctx, rc.alloc((size_t)2 * n * sizeof(double)));
+  HIPCHK(g->ctx, bad.alloc(sizeof(int)));
+  HIPCHK(g->ctx, g->cinv.alloc((size_t)n * n * sizeof(double)));
+  HIPCHK(g->ctx, hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  HIPCHK(g->ctx, dense_spd_inverse(n, g->colptr.as<int32_t>(), g->rowval.as<int32_t>(), g->nzval.as<double>(),
+                                   W.as<double>(), rc.as<double>(), rc.as<double>() + n, g->cinv.as<double>(),
+                                   bad.as<int>(), s));
+  int nbad = 0;
+  HIPCHK(g->ctx, hipMemcpyAsync(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(g->ctx, hipStreamSynchronize(s));  // also: W and rc are freed on return
+  *ok = nbad == 0;
   return HFK_OK;
 }
 
@@ -908,7 +888,7 @@ int coarse_solve(hfk_pmg* g, const double* b, double* x, hipStream_t s, bool all
   double* p = g->cp.as<double>();
   double* q = g->cq.as<double>();
   double* sc = g->cs.as<double>();
-  int mode = allow_net && g->net ? g->net_mode : 0;
+  int mode = allow_net && g->net && !g->net_suspended ? g->net_mode : 0;
   int rc;
   g->last_inner = 0;
   if (mode >= 3) {
@@ -1060,6 +1040,41 @@ int pmg_update(hfk_pmg* g, hipStream_t s) {
   return HFK_OK;
 }
 
+// Stagnation watch of a Krylov solve preconditioned by a p-multigrid that carries a coarse network: when the
+// residual norm has not fallen by fb_factor over the last fb_window iterations (or the preconditioned direction is
+// not finite), the network is set aside for the rest of the solve and the plain cycle preconditions from there on,
+// so the solve converges wherever the plain p-multigrid does.  The next solve starts with the network again.
+struct StagnationWatch {
+  hfk_pmg* g = nullptr;
+  bool armed = false;
+  std::vector<double> hist;
+  StagnationWatch(hfk_pmg* g_, int precond) {
+    if (precond == 2 && g_) {
+      g = g_;
+      g->net_suspended = false;
+      armed = g->net && g->net_mode > 0 && g->fb_window > 0;
+    }
+  }
+  ~StagnationWatch() {
+    if (g) g->net_suspended = false;
+  }
+  bool fall_back() {
+    if (!armed) return false;
+    armed = false;
+    g->net_suspended = true;
+    g->fallbacks++;
+    return true;
+  }
+  // true when the network was set aside at this residual norm
+  bool step(double rn) {
+    if (!armed) return false;
+    hist.push_back(rn);
+    const size_t w = (size_t)g->fb_window;
+    if (hist.size() > w && !(rn <= g->fb_factor * hist[hist.size() - 1 - w])) return fall_back();
+    return false;
+  }
+};
+
 int pmg_apply(hfk_pmg* g, const double* r, double* z, hipStream_t s) {
   if (!g->ok) {  // ldiv! returns NaN: the Krylov solve reports a failure (src/pmultigrid.jl:331-335)
     HIPCHK(g->ctx, launch_fill(z, g->ops[0]->nfree, NAN, s));
@@ -1118,9 +1133,16 @@ int krylov(hfk_op* op, hfk_pmg* g, int kind, int precond, const double* b, doubl
     return HFK_OK;
   }
   const double tol = atol + rtol * bn;
-  if ((rc = M(r, z))) return rc;
-  HIPCHK(op->ctx, vec_copy(n, z, p, s));
-  HIPCHK(op->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, w.dots.part.as<double>(), sc + kRZ, s));
+  StagnationWatch watch(g, precond);
+  // p = z = M r and sc[kRZ] = r.z: the first direction, and a restart after the fallback to the plain cycle
+  auto restart = [&]() -> int {
+    int rc2;
+    if ((rc2 = M(r, z))) return rc2;
+    HIPCHK(op->ctx, vec_copy(n, z, p, s));
+    HIPCHK(op->ctx, vec_dots(n, r, z, nullptr, nullptr, nullptr, nullptr, w.dots.part.as<double>(), sc + kRZ, s));
+    return HFK_OK;
+  };
+  if ((rc = restart())) return rc;
   int it = 0;
   bool conv = false, indef = false;
   while (it < maxiter) {
@@ -1129,6 +1151,10 @@ int krylov(hfk_op* op, hfk_pmg* g, int kind, int precond, const double* b, doubl
     HIPCHK(op->ctx, vec_dots(n, p, q, nullptr, nullptr, nullptr, nullptr, w.dots.part.as<double>(), sc + kPQ, s));
     HIPCHK(op->ctx, hipMemcpyAsync(&pq, sc + kPQ, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(op->ctx, hipStreamSynchronize(s));
+    if (!std::isfinite(pq) && watch.fall_back()) {  // the network's direction is not finite: x and r still are
+      if ((rc = restart())) return rc;
+      continue;
+    }
     if (!(pq > 0)) {  // negative curvature (or NaN): stop, as cg with linesearch
       indef = std::isfinite(pq);
       break;
@@ -1141,6 +1167,10 @@ int krylov(hfk_op* op, hfk_pmg* g, int kind, int precond, const double* b, doubl
     if (std::sqrt(rr) <= tol) {
       conv = true;
       break;
+    }
+    if (watch.step(std::sqrt(rr))) {  // stagnation with the network: the plain cycle from here on
+      if ((rc = restart())) return rc;
+      continue;
     }
     if ((rc = M(r, z))) return rc;
     HIPCHK(op->ctx, vec_dots(n, r, z, z, q, nullptr, nullptr, w.dots.part.as<double>(), sc + kRZN, s));
@@ -1210,6 +1240,7 @@ int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, d
   int it = 0;
   bool solved = false, stop = false;
   const bool variable = !flexible && precond == 2;
+  StagnationWatch watch(g, precond);
   HIPCHK(op->ctx, vec_copy(n, b, rp, s));  // r = b - A x, x = 0
   double beta = bn;
   while (!stop && it < maxiter) {
@@ -1236,6 +1267,10 @@ int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, d
       double ww;
       if ((rc = dots.run(op->ctx, s, n, w, w, &ww))) return rc;
       const double hn = std::sqrt(ww);
+      if (!std::isfinite(hn) && watch.fall_back()) {  // the network's column is not finite: redo it plainly
+        --j;
+        continue;
+      }
       ++it;
       if (!std::isfinite(hn)) {
         stop = true;  // NaN: keep the columns before it, leave the decision to the true residual
@@ -1271,6 +1306,9 @@ int gmres(hfk_op* op, hfk_pmg* g, bool flexible, int precond, const double* b, d
         break;
       }
       HIPCHK(op->ctx, vec_axpby(n, 0.0, w, 1.0 / hn, w, s));  // v_{j+1} = w / hn
+      // stagnation with the network: the plain cycle for the rest of the solve (FGMRES keeps Z_j of every
+      // preconditioner; GMRES decides on the true residual at the restart)
+      (void)watch.step(est);
     }
     if (k == 0) break;
     // y = R^-1 g (upper triangular k x k), then x += Z y (FGMRES) or x += M^-1 (V y) (GMRES)
@@ -1647,6 +1685,21 @@ int hfk_pmg_coarse_matvec(hfk_pmg* g, const double* x, double* y, int scalar_csc
     HIPCHK(g->ctx, coarse_spmv(g, xd.as<double>(), yd.as<double>(), 1.0, 0.0, s));
   HIPCHK(g->ctx, hipMemcpyAsync(y, yd.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(g->ctx, hipStreamSynchronize(s));
+  return HFK_OK;
+}
+
+int hfk_pmg_set_fallback(hfk_pmg* g, int window, double factor) {
+  if (!g) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null p-multigrid");
+  if (window < 0 || !(factor > 0 && factor < 1))
+    return set_err(g->ctx, HFK_EARG, "HyperFEMKernels: fallback window must be >= 0 and factor in (0, 1)");
+  g->fb_window = window;
+  g->fb_factor = factor;
+  return HFK_OK;
+}
+
+int hfk_pmg_fallbacks(hfk_pmg* g, int64_t* count) {
+  if (!g || !count) return set_err(g ? g->ctx : nullptr, HFK_EARG, "HyperFEMKernels: null argument");
+  *count = g->fallbacks;
   return HFK_OK;
 }
 

@@ -661,6 +661,71 @@ hipError_t inject_map(int64_t n, const int32_t* src, const double* xf, const dou
   return hipGetLastError();
 }
 
+// the dense symmetric part of a CSC matrix (0-based): each entry adds half its value at (r, c) and at (c, r); every
+// position receives at most two additions (a CSC pattern holds no duplicates), so the sum is order independent
+__global__ void k_csc_dense_sym(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                                const double* __restrict__ val, double* __restrict__ A) {
+  const int64_t c = blockIdx.x;
+  for (int64_t p = colptr[c] + threadIdx.x; p < colptr[c + 1]; p += blockDim.x) {
+    const int64_t r = rowval[p];
+    const double v = 0.5 * val[p];
+    atomicAdd(&A[r * n + c], v);
+    atomicAdd(&A[c * n + r], v);
+  }
+}
+
+// Gauss-Jordan inversion in place, one pivot per pair of launches: save row and column k, then
+//   a_kk <- 1/p, a_kj <- a_kj/p, a_ik <- -a_ik/p, a_ij <- a_ij - a_ik (a_kj/p)   (p = a_kk, old values)
+// No pivoting: the matrices are symmetric positive definite; a pivot that is not positive (or not finite) sets *bad.
+__global__ void k_gj_save(int64_t n, int64_t k, const double* __restrict__ A, double* __restrict__ rk,
+                          double* __restrict__ ck) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) {
+    rk[i] = A[k * n + i];
+    ck[i] = A[i * n + k];
+  }
+}
+
+__global__ void k_gj_step(int64_t n, int64_t k, double* __restrict__ A, const double* __restrict__ rk,
+                          const double* __restrict__ ck, int* __restrict__ bad) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = blockIdx.y;
+  if (j >= n) return;
+  const double p = rk[k];
+  double* a = A + i * n + j;
+  if (i == k && j == k) {
+    if (!(p > 0) || !isfinite(p)) *bad = 1;
+    *a = 1.0 / p;
+  } else if (i == k) {
+    *a = rk[j] / p;
+  } else if (j == k) {
+    *a = -ck[i] / p;
+  } else {
+    *a -= ck[i] * (rk[j] / p);
+  }
+}
+
+__global__ void k_sym_part(int64_t n, const double* __restrict__ X, double* __restrict__ Y) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = blockIdx.y;
+  if (j < n) Y[i * n + j] = 0.5 * (X[i * n + j] + X[j * n + i]);
+}
+
+hipError_t dense_spd_inverse(int64_t n, const int32_t* colptr, const int32_t* rowval, const double* val, double* W,
+                             double* rk, double* ck, double* Ainv, int* bad, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipError_t e;
+  if ((e = hipMemsetAsync(W, 0, (size_t)n * n * sizeof(double), s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_csc_dense_sym, dim3((unsigned)n), dim3(64), 0, s, n, colptr, rowval, val, W);
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+  for (int64_t k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_save, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, k, W, rk, ck);
+    hipLaunchKernelGGL(k_gj_step, grid, dim3(256), 0, s, n, k, W, rk, ck, bad);
+  }
+  hipLaunchKernelGGL(k_sym_part, grid, dim3(256), 0, s, n, W, Ainv);
+  return hipGetLastError();
+}
+
 hipError_t dense_matvec(int64_t n, const double* A, const double* x, double* y, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_dense_matvec, dim3((unsigned)n), dim3(256), 0, s, n, A, x, y);

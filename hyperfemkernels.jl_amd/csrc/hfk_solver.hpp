@@ -61,5 +61,9 @@ hipError_t csr_spmv(int64_t nrows, const int64_t* rp, const int32_t* col, const 
 hipError_t inject_map(int64_t n, const int32_t* src, const double* xf, const double* df, double* out, hipStream_t s);
 // y = A x with a dense n x n matrix (row-major)
 hipError_t dense_matvec(int64_t n, const double* A, const double* x, double* y, hipStream_t s);
+// Ainv = inverse of the symmetric part of an n x n CSC matrix (0-based), by Gauss-Jordan without pivoting on the
+// device (W: n*n work, rk, ck: n each; *bad set when a pivot is not positive or not finite; *bad must start at 0)
+hipError_t dense_spd_inverse(int64_t n, const int32_t* colptr, const int32_t* rowval, const double* val, double* W,
+                             double* rk, double* ck, double* Ainv, int* bad, hipStream_t s);
 
 }  // namespace hfk

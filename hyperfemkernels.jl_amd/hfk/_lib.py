@@ -90,6 +90,8 @@ SIGNATURES = {
     "hfk_pmg_set_coarse_every": (C.c_int, [_vp, C.c_int]),
     "hfk_pmg_coarse_builds": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "hfk_pmg_set_coarse_precision": (C.c_int, [_vp, C.c_int]),
+    "hfk_pmg_set_fallback": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "hfk_pmg_fallbacks": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "hfk_pmg_vcycle": (C.c_int, [_vp, _f64p, _f64p]),
     "hfk_pmg_vcycle_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hfk_pmg_info": (C.c_int, [_vp, _f64p]),

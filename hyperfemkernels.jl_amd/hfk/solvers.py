@@ -201,6 +201,18 @@ class PMultigrid:
         check(self.lib.hfk_pmg_coarse_builds(self.handle, C.byref(n)), self.levels[0].ctx.handle)
         return int(n.value)
 
+    def set_fallback(self, window: int = 16, factor: float = 0.5):
+        """With a coarse network attached: set the network aside for the rest of a solve whose residual has not
+        fallen by ``factor`` over the last ``window`` iterations (the plain cycle is the fallback); window 0: never."""
+        check(self.lib.hfk_pmg_set_fallback(self.handle, int(window), float(factor)), self.levels[0].ctx.handle)
+        return self
+
+    def fallbacks(self) -> int:
+        """Solves so far that fell back to the plain cycle."""
+        n = C.c_int64()
+        check(self.lib.hfk_pmg_fallbacks(self.handle, C.byref(n)), self.levels[0].ctx.handle)
+        return int(n.value)
+
     def update(self):
         """preconditioner_update!: follow the state of the working operator's last prepare."""
         st = np.zeros(2, dtype=np.int32)
@@ -245,6 +257,7 @@ class SolveResult:
     indefinite: bool
     residual: float
     rhs_norm: float
+    fell_back: bool = False  # the coarse network was set aside for the plain cycle during the solve
 
 
 KRYLOV = {"cg": 0, "fcg": 1, "gmres": 2, "fgmres": 3}
@@ -277,10 +290,12 @@ class MatrixFreeSolver:
         prec, g = (0, None) if pc is None else ((1, None) if pc == "blockjacobi" else (2, pc.handle))
         st = np.zeros(3, dtype=np.int32)
         rn = np.zeros(2)
+        n0 = pc.fallbacks() if prec == 2 else 0
         check(op.lib.hfk_krylov_solve(op.handle, g, KRYLOV[self.krylov], prec, ptr(b, C.c_double), ptr(x, C.c_double),
                                       self.rtol, self.atol, self.maxiter, self.restart, ptr(st, C.c_int32),
                                       ptr(rn, C.c_double)), op.ctx.handle)
-        self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
+        self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]),
+                                prec == 2 and pc.fallbacks() > n0)
         return self.last
 
     def solve_dev(self, x_ptr: int, mf: MatrixFreeJacobian, b_ptr: int, stream: int = 0) -> SolveResult:
@@ -290,8 +305,10 @@ class MatrixFreeSolver:
         prec, g = (0, None) if pc is None else ((1, None) if pc == "blockjacobi" else (2, pc.handle))
         st = np.zeros(3, dtype=np.int32)
         rn = np.zeros(2)
+        n0 = pc.fallbacks() if prec == 2 else 0
         check(op.lib.hfk_krylov_solve_dev(op.handle, g, KRYLOV[self.krylov], prec, C.c_void_p(b_ptr), C.c_void_p(x_ptr),
                                           self.rtol, self.atol, self.maxiter, self.restart, ptr(st, C.c_int32),
                                           ptr(rn, C.c_double), C.c_void_p(stream)), op.ctx.handle)
-        self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]))
+        self.last = SolveResult(bool(st[0]), int(st[1]), bool(st[2]), float(rn[0]), float(rn[1]),
+                                prec == 2 and pc.fallbacks() > n0)
         return self.last

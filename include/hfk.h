@@ -308,6 +308,13 @@ int hfk_net_train_dev(hfk_net* net, const double* r_dev, const double* e_dev, in
  * network acts on the diagonally scaled coarse system, z = D^-1/2 N(D^-1/2 r) with D = diag(A) (also in training
  * and in hfk_pmg_coarse_network_error); 0: z = N(r).  The setting stays with the p-multigrid. */
 int hfk_pmg_set_coarse_network(hfk_pmg* g, hfk_net* net, int mode, int smoother_degree, int scaled);
+/* The plain p-multigrid as the fallback of a solve with a coarse network (hfk_krylov_solve with precond 2): when the
+ * residual norm has not fallen by `factor` (0 < factor < 1) over the last `window` iterations, or the preconditioned
+ * direction is not finite, the network is set aside for the rest of that solve and the plain cycle (the coarse PCG
+ * or direct solve) preconditions from there on; CG/FCG restart their direction there.  window 0 turns the watch
+ * off.  Default window 16, factor 0.5.  *count receives the number of solves that fell back so far. */
+int hfk_pmg_set_fallback(hfk_pmg* g, int window, double factor);
+int hfk_pmg_fallbacks(hfk_pmg* g, int64_t* count);
 /* Training data from the plain p-multigrid: for each fine right-hand side b_k ([k][nfine], device) the
  * pre-smoothed residual restricted to the coarse level (bc_k) -- with coarse_smoothed != 0, further the residual of
  * the coarse Chebyshev pre-smoothing of it (the input of the network cycle) -- and, when ec_dev is not NULL, the

@@ -176,3 +176,49 @@ def test_training_and_learned_cycle(hip_ctx, scaled):
     print(f"fgmres + network cycle: {res}", flush=True)
     assert res.converged and rel(x, sols["off"][0]) < 1e-7
     set_coarse_network(pmg, None, "off")
+
+
+def test_fallback_to_plain_cycle(hip_ctx):
+    """Plain p-MG is the fallback of the learned solve: a network whose output is not finite sets itself aside at
+    its first direction (FCG and FGMRES), and a stagnation watch that fires at once (forced from the host) hands the
+    solve to the plain cycle; both still reach the plain cycle's answer.  With the watch off the same network fails
+    the solve."""
+    fs, cs, fop, cop, mf = make_problem(hip_ctx, 16)
+    pmg = PMultigrid([fop, cop], Chebyshev(degree=4), coarse_rtol=1e-2, coarse_maxiter=400).update()
+    assert pmg.ok
+    b = np.random.default_rng(3).standard_normal(fs.nfree)
+    x_ref = np.zeros(fs.nfree)
+    ref = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=400, preconditioner=pmg).solve(x_ref, mf, b)
+    assert ref.converged and not ref.fell_back
+    net = CoarseNetwork(hip_ctx, cs, seed=3, coarsen=True)
+    good = net.weights()
+    net.set_weights(np.full_like(good, np.nan))
+    set_coarse_network(pmg, net, "cycle", 0, True)
+    count = pmg.fallbacks()
+    for krylov in ("fcg", "fgmres"):
+        x = np.zeros(fs.nfree)
+        res = MatrixFreeSolver(krylov, rtol=1e-10, maxiter=400, preconditioner=pmg, restart=30).solve(x, mf, b)
+        print(f"non-finite network, {krylov}: {res}", flush=True)
+        assert res.converged and res.fell_back
+        assert rel(x, x_ref) < 1e-7
+        count += 1
+        assert pmg.fallbacks() == count
+    pmg.set_fallback(window=0)
+    x = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=400, preconditioner=pmg).solve(x, mf, b)
+    assert not res.converged and not res.fell_back
+    # an untrained network with a watch that fires at the second iteration
+    net.set_weights(good)
+    pmg.set_fallback(window=1, factor=1e-300)
+    x = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=400, preconditioner=pmg).solve(x, mf, b)
+    print(f"forced stagnation: {res}", flush=True)
+    assert res.converged and res.fell_back and rel(x, x_ref) < 1e-7
+    # the default watch: an untrained network that still converges keeps it or falls back, the answer is the same
+    pmg.set_fallback()
+    x = np.zeros(fs.nfree)
+    res = MatrixFreeSolver("fcg", rtol=1e-10, maxiter=400, preconditioner=pmg).solve(x, mf, b)
+    assert res.converged and rel(x, x_ref) < 1e-7
+    with pytest.raises(RuntimeError):
+        pmg.set_fallback(window=4, factor=1.5)
+    set_coarse_network(pmg, None, "off")
