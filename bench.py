@@ -33,8 +33,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per deterministic Jv at 64^3 Q2 (8 colour launches), from the PMC counters FETCH_SIZE x 2.0 and
 # WRITE_SIZE x 1.0 (the factors a 1 GiB copy of known size reads), per dispatch: 324,203 / 27,479 KiB (full64),
 # 196,835 / 27,476 KiB (sym64), 96,111 / 27,451 KiB (fac64)
-JV_TRAFFIC_64 = {"full64": 8 * (324203 * 2 + 27479) * 1024, "sym64": 8 * (196835 * 2 + 27476) * 1024,
-                 "fac64": 8 * (96111 * 2 + 27451) * 1024}
+# HBM bytes per deterministic Jv at 64^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (KiB per
+# colour dispatch x 8 colours, FETCH calibrated x2 on the 1 GiB copy) plus the fill of y (49,557 KiB written)
+JV_TRAFFIC_64 = {"full64": (8 * (324203 * 2 + 27479) + 49557) * 1024,
+                 "sym64": (8 * (196835 * 2 + 27476) + 49557) * 1024,
+                 "fac64": (8 * (96113 * 2 + 27448) + 49557) * 1024}
 NENT = {"ful": 81, "sym": 45, "fac": 17}  # stored entries per point of each tangent storage
 
 
@@ -85,7 +88,10 @@ def cpu_baseline(n=64, reps=20):
     qop.jacobian(qx, nzpos, rowval.size)
     t_jac = time.perf_counter() - t0
     del prep
+    pmg = cpu_pmg_sample(cores=cores)
+    oseen = cpu_oseen_sample()
     return {"value": sp.nfree / t / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "port",
+            "pmg_solve_16_host": pmg, "oseen_jacobian_host": oseen,
             "median_of_20_GDoFs": sp.nfree / t_med / 1e9,
             "sample": f"oracle stored-T (81 FP64 per point) sum-factorised Jv (OpenMP, colour chunks), the same "
                       f"Q2 neo-Hookean {n}^3 workload ({sp.nfree} free dofs), after 1 warm-up: min of the first 5 "
@@ -94,6 +100,100 @@ def cpu_baseline(n=64, reps=20):
                                     "jacobian_ms": t_jac * 1e3, "cores": 1,
                                     "sample": "oracle residual, pattern and Jacobian of the 20x10x10 Q2 SVK "
                                               "quick start at x = 0.01 N(0,1) (seed 1), one call each"}}
+
+
+def cpu_pmg_sample(n=16, rtol=1e-8, cores=1):
+    """The plain p-multigrid solve on the host (oracle pieces): Q2 neo-Hookean on n^3 (both x faces clamped),
+    linearised at the smooth state of amplitude 1e-2 as the device solve legs; CG with one V-cycle of two levels
+    (Chebyshev degree 4 with nodal block Jacobi on the matrix-free Q2 level -- the oracle's stored-T Jv on `cores`
+    threads -- and the reference's default LU of the assembled Q1 level, pmultigrid.jl:229-239).  The set-up
+    (tangents, blocks, interval, coarse LU) is timed apart from the solve."""
+    import numpy as np
+    import scipy.sparse.linalg as spla
+
+    from hfk.mesh import dirichlet_closed_x_faces, hex_space
+    from hfk.synthetic import smooth_field
+    from oracle import oracle as orc
+
+    fs = hex_space((n, n, n), 2, dirichlet=dirichlet_closed_x_faces)
+    cs = hex_space((n, n, n), 1, dirichlet=dirichlet_closed_x_faces)
+    u = smooth_field(2, amp=1e-2)
+    t0 = time.perf_counter()
+    fop = orc.OracleOp(fs, "neohookean", 2.0, 1.0, dvals=fs.dirichlet_values(u))
+    cop = orc.OracleOp(cs, "neohookean", 2.0, 1.0, points1d=3, dvals=cs.dirichlet_values(u))
+    x = fs.free_values(u)
+    prep = fop.prepare(x)
+
+    class Fine:
+        def __matmul__(self, v):
+            return fop.jv(prep, v, nthreads=cores)
+
+    A = Fine()
+    dofs, Binv = fop.blocks_from_tangents(prep)
+    Minv = orc.block_inverse_matrix(dofs, Binv, fs.nfree)
+    lmin, lmax = orc.arnoldi_bounds(lambda v: Minv @ (A @ v), orc.chebyshev_start(fs.nfree), 10)
+    bb = 1.1 * lmax
+    aa = bb / 20.0
+    P = orc.prolongation_matrix(cs, fs).tocsr()
+    R = P.T.tocsr()
+    lu = spla.splu(cop.csc(cs.free_values(u)).tocsc())
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(fs.nfree)
+
+    def vcycle(r):
+        z = orc.chebyshev_smooth(A, Minv, None, r, aa, bb, 4, zero_guess=True)
+        rc = R @ (r - A @ z)
+        z = z + P @ lu.solve(rc)
+        return orc.chebyshev_smooth(A, Minv, z, r, aa, bb, 4)
+
+    t0 = time.perf_counter()
+    xs = np.zeros(fs.nfree)
+    r = b.copy()
+    z = vcycle(r)
+    p = z.copy()
+    rz = r @ z
+    bn = np.linalg.norm(b)
+    it = 0
+    while it < 200:
+        q = A @ p
+        al = rz / (p @ q)
+        xs += al * p
+        r -= al * q
+        it += 1
+        if np.linalg.norm(r) <= rtol * bn:
+            break
+        z = vcycle(r)
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    t_solve = time.perf_counter() - t0
+    return {"free_dofs": fs.nfree, "coarse_free_dofs": cs.nfree, "iterations": it, "setup_s": t_setup,
+            "solve_s": t_solve, "cores": cores, "rel_residual": float(np.linalg.norm(b - A @ xs) / bn),
+            "sample": f"{n}^3 Q2 neo-Hookean, CG + two-level V-cycle (Chebyshev 4, block Jacobi, LU coarse), "
+                      f"rtol {rtol}, N(0,1) right-hand side"}
+
+
+def cpu_oseen_sample(n=6):
+    """The Oseen element systems and CSC values on the host (oracle, numpy over all cells at once, one core): the
+    P2-P1 SUPG/PSPG/LSIC Jacobian of n^3 x 6 tetrahedra at random fields, and the per-tetrahedron rate scaled to
+    the 40^3 x 6 step (384,000 tetrahedra)."""
+    import numpy as np
+
+    from hfk.tetmesh import tet_p2p1_space, tet_quadrature_deg5
+    from oracle import oseen as O
+
+    sp = tet_p2p1_space((n, n, n))
+    qp, qw = tet_quadrature_deg5()
+    fields = np.random.default_rng(3).standard_normal((sp.ncells, len(qw), 9))
+    colptr, rowval, nzpos = O.pattern(sp.cell_dofs, sp.nfree)
+    t0 = time.perf_counter()
+    K, _ = O.element_system(sp.coords, qp, qw, fields, 1.0, 0.01, 150.0)
+    O.jacobian_values(K, nzpos, rowval.size)
+    t = time.perf_counter() - t0
+    return {"tets": sp.ncells, "jacobian_s": t, "per_tet_us": t / sp.ncells * 1e6,
+            "scaled_to_384000_tets_s": t / sp.ncells * 384000, "cores": 1,
+            "sample": f"{n}^3 x 6 P2-P1 tetrahedra, element systems + CSC scatter (numpy), one call"}
 
 
 def svk_quickstart_space():
@@ -154,6 +254,56 @@ def newton_leg(rtol=1e-10, maxit=20):
             "pattern_s": t_pattern, "residual_ms_mean": 1e3 * float(np.mean(tr)),
             "jacobian_ms_mean": 1e3 * float(np.mean(tj)), "lu_ms_mean": 1e3 * float(np.mean(tl)),
             "newton_total_s": t_all, "note": "residual!/jacobian! timed through host arrays (PCIe included)"}
+
+
+def assembly_leg(reps=20):
+    """The quick start's residual! and jacobian! with device-resident state and CSC values, each call bracketed by
+    HIP events on the launch stream (no PCIe), against the FP64 peak: the reference's count of the element-matrix
+    work (1.417 MFLOP per Q2 SVK cell, benchmark/roofline.jl:65-74) and this kernel's own count (per point the
+    81 x 9 product D G and the 81 x 81 x 3 contraction: 43,740 flop, x 27 points)."""
+    import numpy as np
+    import torch
+
+    from hfk import HyperelasticForm, KernelOperator
+    from hfk.synthetic import normal
+
+    sp, dv = svk_quickstart_space()
+    op = KernelOperator(HyperelasticForm("svk", 2.0, 1.0, degree=4), sp, jacobian="assembled", dirichlet_values=dv)
+    _, rowval, _ = op.pattern()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(dev)
+    sptr = stream.cuda_stream
+    xd = torch.from_numpy(0.01 * normal(1, sp.nfree)).to(dev)
+    nz = torch.zeros(rowval.size, dtype=torch.float64, device=dev)
+    bd = torch.zeros(sp.nfree, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        ts = np.array([a.elapsed_time(b) * 1e-3 for a, b in evs])
+        return float(ts.min()), float(np.median(ts))
+
+    j_min, j_med = timed(lambda: op.jacobian_dev(nz.data_ptr(), xd.data_ptr(), sptr))
+    r_min, r_med = timed(lambda: op.residual_dev(bd.data_ptr(), xd.data_ptr(), sptr))
+    ref_flop = 1.417e6 * sp.ncells
+    own_flop = 43740.0 * 27 * sp.ncells
+    return {"workload": "20x10x10 Q2 SVK quick start", "cells": sp.ncells, "nnz": int(rowval.size),
+            "jacobian_ms_min": j_min * 1e3, "jacobian_ms_median": j_med * 1e3,
+            "residual_ms_min": r_min * 1e3, "residual_ms_median": r_med * 1e3,
+            "jacobian_reference_flop": ref_flop, "jacobian_own_flop": own_flop,
+            "jacobian_TFLOPs_reference_count": ref_flop / j_med / 1e12,
+            "jacobian_TFLOPs_own_count": own_flop / j_med / 1e12,
+            "fp64_peak_TFLOPs": 78.6, "frac_of_fp64_peak": own_flop / j_med / 78.6e12,
+            "csc_write_GBs": rowval.size * 8 * 2 / j_med / 1e9,
+            "note": "device arrays, HIP events on the launch stream; the CSC values are zeroed and accumulated "
+                    "colour by colour (8 launches)"}
 
 
 def oseen_leg(n=40, reps=10):
@@ -821,11 +971,16 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--order", type=int, default=2, choices=[1, 2, 3],
+                    help="element order of the Jv workload (2: the headline; 3 on 48^3: the Q3 parity workload)")
+    ap.add_argument("--no-assembly", action="store_true", help="skip the device-timed quick-start assembly leg")
     ap.add_argument("--storage", default="fac64", choices=["full64", "sym64", "full32", "sym32", "fac64", "fac32"])
     ap.add_argument("--variants", default="sym64,full64,fac32,sym32", help="extra storages timed beside the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--newton-lu", action="store_true",
+                    help="also the quick-start Newton with host sparse LU (minutes of single-core SuperLU)")
     ap.add_argument("--no-oseen", action="store_true")
     ap.add_argument("--no-law", action="store_true", help="skip the run-time compiled law leg")
     ap.add_argument("--fields-full", action="store_true",
@@ -846,13 +1001,14 @@ def main():
     from hfk.synthetic import normal, smooth_field
 
     t0 = time.perf_counter()
-    n = args.n
-    sp = hex_space((n, n, n), 2)
+    n, order = args.n, args.order
+    sp = hex_space((n, n, n), order)
     u = smooth_field(2)
     x, dvals = sp.free_values(u), sp.dirichlet_values(u)
     v = normal(3, sp.nfree)
-    print(f"bench: mesh {n}^3 Q2, {sp.nfree} free dofs ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    op = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=4), sp, jacobian="matrixfree",
+    print(f"bench: mesh {n}^3 Q{order}, {sp.nfree} free dofs ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+          flush=True)
+    op = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=2 * order), sp, jacobian="matrixfree",
                         dirichlet_values=dvals)
     dev = torch.device("cuda:0")
     # a stream of our own (default priority): the library launches on the stream it is given, and the events
@@ -886,7 +1042,7 @@ def main():
     mf, t_prep, per, wall = time_storage(args.storage, args.steps, args.warmup)
     y = yd.cpu().numpy()
     t_jv = float(per.mean())
-    ncells, nq, ndl = sp.ncells, 27, sp.nd
+    ncells, nq, ndl = sp.ncells, (order + 1) ** 3, sp.nd
     B = jv_bytes(ncells, nq, ndl, sp.nfree)
     own_B = jv_bytes(ncells, nq, ndl, sp.nfree, s_t=4 if args.storage.endswith("32") else 8,
                      nent=NENT[args.storage[:3]]) - ncells * 10 * 8
@@ -905,21 +1061,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (SplitMix64 smooth state seed 2, v ~ N(0,1) seed 3)",
-        "config": {"workload": f"Jv, compressible neo-Hookean lam=2 mu=1, unit cube {n}^3 Q2 hex, 27 points/cell, "
-                               f"FP64, tangent storage {args.storage}"
+        "config": {"workload": f"Jv, compressible neo-Hookean lam=2 mu=1, unit cube {n}^3 Q{order} hex, {nq} "
+                               f"points/cell, FP64, tangent storage {args.storage}"
                                + {"sym64": " (the 45 independent entries of the symmetric tangent)",
                                   "fac64": " (the 17 factors of the isotropic neo-Hookean tangent)"}.get(args.storage, ""),
                    "free_dofs": sp.nfree, "cells": ncells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": JV_TRAFFIC_64.get(args.storage) if n == 64 else None,
+                     "traffic": JV_TRAFFIC_64.get(args.storage) if (n, order) == (64, 2) else None,
                      "traffic_source": f"rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this command (64^3, "
                                        f"{args.storage}), calibrated x2.0 / x1.0 on the 1 GiB copy; HBM bytes per Jv",
-                     "kernel": "one Jv = 1 fill + 8 colour launches of k_apply<3,3>",
+                     "kernel": f"one Jv = 1 fill + 8 colour launches of k_apply<{order + 1},{order + 1}>",
                      "algorithmic_bytes": own_B,
                      "algorithmic_model": f"the {args.storage} layout's own bytes: stored tangents, cell ids, dx and y once",
                      "traffic_frac_of_peak": (JV_TRAFFIC_64[args.storage] / t_jv / 1e9 / HBM_PEAK_GBS
-                                              if n == 64 and args.storage in JV_TRAFFIC_64 else None),
+                                              if (n, order) == (64, 2) and args.storage in JV_TRAFFIC_64 else None),
                      "reference_model_bytes": B,
                      "reference_model_effective_GBs": B / t_jv / 1e9,
                      "stored_tangent_bytes": info["tangent_bytes"],
@@ -957,7 +1113,9 @@ def main():
         vv.pop("yv")
     op.set_deterministic(True)
     res["variants"] = variants
-    if not args.no_solve:
+    if not args.no_solve and order != 2:
+        print("bench: the solve legs run on the Q2 headline workload only (--order 2)", file=sys.stderr, flush=True)
+    if not args.no_solve and order == 2:
         del mf
         print("bench: solve legs (FP64 p-MG; mixed precision FP32 smoothers; mixed + learned coarse level)",
               file=sys.stderr, flush=True)
@@ -1021,9 +1179,14 @@ def main():
               file=sys.stderr, flush=True)
         res["oseen_tet_40"] = oseen_leg()
     if not args.no_newton:
-        print("bench: SVK quick start 20x10x10 Q2, Newton with host LU", file=sys.stderr, flush=True)
-        res["newton_svk_20x10x10"] = newton_leg()
+        print("bench: SVK quick start 20x10x10 Q2, Newton (device GMRES; host LU with --newton-lu)", file=sys.stderr,
+              flush=True)
+        if args.newton_lu:
+            res["newton_svk_20x10x10"] = newton_leg()
         res["newton_svk_20x10x10_device_gmres"] = newton_gmres_leg()
+    if not args.no_assembly:
+        print("bench: SVK quick start 20x10x10 Q2, device-timed residual! and jacobian!", file=sys.stderr, flush=True)
+        res["assembly_svk_20x10x10_device"] = assembly_leg()
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 64^3 Jv, min of 5 / median of 20)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()
