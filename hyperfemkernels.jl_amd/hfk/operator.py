@@ -300,6 +300,17 @@ class KernelOperator:
         check(self.lib.hfk_jacobian(self.handle, ptr(x, C.c_double), ptr(nz, C.c_double)), self.ctx.handle)
         return nz
 
+    def residual_dev(self, b_ptr: int, x_ptr: int, stream: int = 0):
+        """``residual!`` on device vectors (raw addresses of nfree doubles), enqueued on ``stream``."""
+        check(self.lib.hfk_residual_dev(self.handle, C.c_void_p(x_ptr), C.c_void_p(b_ptr), C.c_void_p(stream)),
+              self.ctx.handle)
+
+    def jacobian_dev(self, nz_ptr: int, x_ptr: int, stream: int = 0):
+        """``jacobian!`` into device CSC values (raw address of nnz doubles, the pattern of ``pattern()``)."""
+        self.pattern()
+        check(self.lib.hfk_jacobian_dev(self.handle, C.c_void_p(x_ptr), C.c_void_p(nz_ptr), C.c_void_p(stream)),
+              self.ctx.handle)
+
     def residual_and_jacobian(self, b, A, x):
         self.residual(b, x)
         self.jacobian(A, x)
