@@ -691,7 +691,9 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
         for k0 in range(0, samples, 32):  # fine right-hand sides in flight
             k1 = min(k0 + 32, samples)
             bf = torch.randn((k1 - k0, p["fs"].nfree), dtype=torch.float64, device=dev, generator=gen)
-            coarse_pairs_dev(pmg, bf.data_ptr(), k1 - k0, bc[k0:].data_ptr(), None, coarse_smoothed=True)
+            # on torch's stream, which made bf and bc: the library orders its work after theirs
+            coarse_pairs_dev(pmg, bf.data_ptr(), k1 - k0, bc[k0:].data_ptr(), None, coarse_smoothed=True,
+                             stream=torch.cuda.current_stream(dev).cuda_stream)
             del bf
         train.append((pmg, bc.data_ptr(), samples, p["aux"]))
         keep.append((p, mf, pmg, bc))
@@ -796,7 +798,8 @@ def train_coarse_network(pmg, fs, cs, dev, samples=160, steps=500, seed=11, smoo
     for k0 in range(0, samples + held, chunk):
         k1 = min(k0 + chunk, samples + held)
         bf = torch.randn((k1 - k0, fs.nfree), dtype=torch.float64, device=dev, generator=gen)
-        coarse_pairs_dev(pmg, bf.data_ptr(), k1 - k0, bc[k0:].data_ptr(), None, coarse_smoothed=True)
+        coarse_pairs_dev(pmg, bf.data_ptr(), k1 - k0, bc[k0:].data_ptr(), None, coarse_smoothed=True,
+                         stream=torch.cuda.current_stream(dev).cuda_stream)  # torch's stream made bf and bc
         del bf
     torch.cuda.synchronize()
     t1 = time.perf_counter()

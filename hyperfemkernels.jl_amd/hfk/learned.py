@@ -99,7 +99,9 @@ def coarse_pairs_dev(pmg, b_ptr: int, nsamples: int, bc_ptr: int, ec_ptr: int | 
                      rtol: float = 1e-10, maxiter: int = 5000, stream: int = 0):
     """Training data from the plain p-multigrid (device arrays [k][n]): restricted pre-smoothed residuals (with
     ``coarse_smoothed``, the residuals after the coarse Chebyshev pre-smoothing: the network cycle's input), and
-    their coarse solutions when ec_ptr is given."""
+    their coarse solutions when ec_ptr is given.  The work runs on ``stream`` (0: the context's own stream, which
+    does not wait for other streams): pass the stream that wrote the inputs and the output buffers, or synchronise
+    first."""
     check(pmg.lib.hfk_pmg_coarse_pairs_dev(pmg.handle, C.c_void_p(b_ptr), nsamples, C.c_void_p(bc_ptr),
                                            C.c_void_p(ec_ptr) if ec_ptr else None, int(bool(coarse_smoothed)), rtol,
                                            maxiter, C.c_void_p(stream)),
