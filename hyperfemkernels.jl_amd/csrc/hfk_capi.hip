@@ -105,8 +105,10 @@ int op_run_apply_t(hfk_op* op, const void* T, int storage, const double* dx_dev,
       HIPCHK(op->ctx, launch_apply(d, storage, T, dx_dev, y_dev, r, s, accumulate ? alpha : 1.0));
   }
   // the boundary group of the Robin faces (its Jacobian is the face mass matrix; increments vanish on the
-  // Dirichlet dofs)
-  return robin_apply(op, dx_dev, nullptr, y_dev, accumulate ? alpha : 1.0, s);
+  // Dirichlet dofs), then the skeleton group at the prepared state
+  int rc = robin_apply(op, dx_dev, nullptr, y_dev, accumulate ? alpha : 1.0, s);
+  if (rc) return rc;
+  return skel_apply(op, dx_dev, y_dev, accumulate ? alpha : 1.0, s);
 }
 
 int robin_apply(hfk_op* op, const double* uf, const double* ud, double* y, double alpha, hipStream_t s) {
@@ -149,7 +151,8 @@ int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s,
 // the cells that contain the column's dof (same CSC, bit for bit), in parallel over columns.  Generic over the
 // cell dof table (any element, any number of fields: every (test, trial) pair of a cell is coupled).
 void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
-                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks) {
+                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks,
+                 const CellNeighbours* nbr) {
   const bool masked = blocks.mask != nullptr;
   const int nf = blocks.nf;
   auto block_set = [&](int64_t cell, int ii, int jj) {
@@ -181,6 +184,12 @@ void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols
         while (ids[jj] != c + 1) ++jj;
       for (int ii = 0; ii < nd; ++ii)
         if (ids[ii] > 0 && block_set(cell, ii, jj)) buf.push_back(ids[ii]);
+      if (nbr)  // the rows of the cells across the facets of this cell
+        for (int64_t k2 = nbr->start[cell]; k2 < nbr->start[cell + 1]; ++k2) {
+          const int32_t* nid = &cell_dofs[(size_t)nbr->cells[k2] * nd];
+          for (int ii = 0; ii < nd; ++ii)
+            if (nid[ii] > 0) buf.push_back(nid[ii]);
+        }
     }
     std::sort(buf.begin(), buf.end());
     buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
@@ -242,7 +251,13 @@ void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols
 
 int op_build_pattern(hfk_op* op) {
   if (op->pattern_built) return HFK_OK;
-  csc_pattern(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos);
+  if (skel_facets(op) > 0) {
+    CellNeighbours nb;
+    skel_neighbours(op, nb);
+    csc_pattern(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos, BlockInfo(), &nb);
+  } else {
+    csc_pattern(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos);
+  }
   op->pattern_built = true;
   return HFK_OK;
 }
@@ -394,7 +409,9 @@ int run_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) 
   // the boundary groups after the volume group: minus the traction load on the free rows, plus the springs
   if (op->has_traction)
     HIPCHK(op->ctx, launch_sub_scaled(b_dev, op->face_load.as<double>(), op->load_factor, op->nfree, s));
-  return robin_apply(op, x_dev, op->dvals.as<double>(), b_dev, 1.0, s);
+  int rc = robin_apply(op, x_dev, op->dvals.as<double>(), b_dev, 1.0, s);
+  if (rc) return rc;
+  return skel_residual(op, x_dev, b_dev, s);  // the skeleton group last
 }
 
 // strain energy into out_dev[0]: per-batch sums, then their sum, both in a fixed order
@@ -402,7 +419,7 @@ constexpr int kDotBlocks = 256;  // work-groups of the load's dot products (a fi
 
 // number of partial sums of one energy evaluation
 int64_t energy_partials(const hfk_op* op) {
-  return op->nbatches + (op->has_traction ? 2 * kDotBlocks : 0) + op->robin_n;
+  return op->nbatches + (op->has_traction ? 2 * kDotBlocks : 0) + op->robin_n + skel_facets(op);
 }
 
 // the energy with its partial sums in `part` (energy_partials(op) doubles, every one overwritten)
@@ -423,7 +440,11 @@ int run_energy_into(hfk_op* op, const double* x_dev, double* out_dev, double* pa
   if (op->robin_n > 0)  // plus the energy of the springs, one partial per face
     HIPCHK(op->ctx, launch_face_energy(op->dev(), op->robin_slot.as<int64_t>(), op->robin_local.as<int32_t>(),
                                        op->robin_M.as<double>(), op->robin_n, x_dev, part + op->nbatches + nload, s));
-  HIPCHK(op->ctx, launch_reduce_partials(part, op->nbatches + nload + op->robin_n, 0, out_dev, s));
+  if (skel_facets(op) > 0) {  // plus the facet energies, one partial per facet
+    int rc = skel_energy(op, x_dev, part + op->nbatches + nload + op->robin_n, s);
+    if (rc) return rc;
+  }
+  HIPCHK(op->ctx, launch_reduce_partials(part, energy_partials(op), 0, out_dev, s));
   return HFK_OK;
 }
 
@@ -465,7 +486,8 @@ int op_run_jacobian(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t
   if ((rc = op_prepare_into(op, 0, op->D64.p, x_dev, s))) return rc;
   HIPCHK(op->ctx, launch_fill(nz_dev, (int64_t)op->rowval.size(), 0.0, s));
   for (const BatchRange& r : op->color_batches) HIPCHK(op->ctx, launch_assemble(d, op->D64.as<double>(), nz_dev, r, s));
-  return robin_assemble(op, nz_dev, s);
+  if ((rc = robin_assemble(op, nz_dev, s))) return rc;
+  return skel_assemble(op, x_dev, nz_dev, s);
 }
 
 }  // namespace hfk

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,11 @@ struct DevBuf {
   T* as() const {
     return static_cast<T*>(p);
   }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
 };
 
 // Nodal blocks of one operator (FieldBlocks, src/preconditioners.jl:120-171): one block per node with a free
@@ -63,6 +69,27 @@ struct NodeBlocks {
   DevBuf block_of;              // [slot][t] tensor order, -1: every component fixed
   DevBuf dofs, B, Binv;
   DevBuf bad;                   // int counter of non-finite inverted blocks
+};
+
+// Interior facets (skeleton integrals, src/skeleton.jl) of a hexahedral operator: the plus and minus cell of every
+// facet, the facets by colour (two facets of one colour share no free dof of their cells), and the facet kernels of
+// the form, compiled with hiprtc for the operator's (Q, P).
+struct SkelData {
+  int64_t nf = 0;
+  std::vector<int32_t> cells;   // [f][2]
+  std::vector<int64_t> start;   // colour c: order[start[c] .. start[c+1])
+  DevBuf slot, face, order;     // [f][2] int64 batch slots, [f][3] int32 faces and orientation, int32 facet order
+  DevBuf colptr, rowval;        // the CSC pattern (0-based) for the Jacobian's positions
+  double p[8] = {};
+  hipModule_t mod = nullptr;
+  hipFunction_t res = nullptr, jv = nullptr, energy = nullptr, jac = nullptr;
+  int split = 1;                // work-groups per facet and side block of the Jacobian kernel
+  SkelData() = default;
+  SkelData(const SkelData&) = delete;
+  SkelData& operator=(const SkelData&) = delete;
+  ~SkelData() {
+    if (mod) (void)hipModuleUnload(mod);
+  }
 };
 
 }  // namespace hfk
@@ -113,6 +140,8 @@ struct hfk_op {
   hfk::NodeBlocks blocks;
   // a pointwise law compiled at run time (nullptr: the built-in law)
   hfk_law* rtc = nullptr;
+  // skeleton integrals (nullptr: none)
+  std::unique_ptr<hfk::SkelData> skel;
 
   hfk::OpDev dev() const {
     hfk::OpDev d;
@@ -158,8 +187,12 @@ struct BlockInfo {
 // the reference's CSC sparsity and cell positions (1-based) of a cell dof table (> 0 free ids): the keys of the
 // free pairs of every cell (full_keys!), or of the blocks set on each cell (block_keys!); nzpos of a pair outside the
 // cell's blocks is its stored position when another cell stores it, 0 otherwise (cell_positions!, :864-899)
+// With `nbr`, the rows of every column also hold the free dofs of the facet neighbours of the column's cells (the
+// cross blocks of skeleton_keys!, src/skeleton.jl:355-390); nzpos stays the cells' own.
+struct CellNeighbours;
 void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
-                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks = BlockInfo());
+                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks = BlockInfo(),
+                 const CellNeighbours* nbr = nullptr);
 // the reference's greedy colouring (1-based colours); false when more than 128 colours are needed
 bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color);
 int op_build_blocks(hfk_op* op);
@@ -184,4 +217,19 @@ struct OseenFormKernels {
   hipFunction_t jac[2] = {}, res[2] = {};
 };
 int rtc_oseen_form(hfk_ctx* ctx, const char* source, const char* name, OseenFormKernels* out);
+// The facet kernels of a skeleton point form (source null: the built-in gradient-jump penalty) for (Q, P), loaded on
+// the current device into sk (mod, res, jv, energy, jac, split).
+int rtc_skel(hfk_ctx* ctx, int Q, int P, const char* source, const char* name, SkelData* sk);
+// the skeleton's launches (hfk_skel.hip): b += r; y += alpha J dx at the prepared state; the CSC values; one energy
+// partial per facet into part; the facet neighbours of every cell (for the pattern)
+int skel_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s);
+int skel_apply(hfk_op* op, const double* dx_dev, double* y_dev, double alpha, hipStream_t s);
+int skel_assemble(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s);
+int skel_energy(hfk_op* op, const double* x_dev, double* part, hipStream_t s);
+int64_t skel_facets(const hfk_op* op);
+struct CellNeighbours {
+  std::vector<int64_t> start;  // cell k: cells[start[k] .. start[k+1])
+  std::vector<int32_t> cells;
+};
+void skel_neighbours(const hfk_op* op, CellNeighbours& nb);
 }  // namespace hfk

@@ -104,10 +104,10 @@ std::vector<std::string> extremum_names(int Q, int P) {
 // instantiations to emit, their lowered names returned in `lowered`.
 bool compile(const std::string& program, const std::vector<std::string>& names, std::vector<char>& code,
              std::vector<std::string>& lowered, std::string& log) {
-  const char* hdr_src[] = {kTypesSrc, kPointSrc, kOseenFormSrc};
-  const char* hdr_name[] = {"hfk_types.hpp", "hfk_point.hpp", "hfk_oseen_form.hpp"};
+  const char* hdr_src[] = {kTypesSrc, kPointSrc, kOseenFormSrc, kSkelSrc};
+  const char* hdr_name[] = {"hfk_types.hpp", "hfk_point.hpp", "hfk_oseen_form.hpp", "hfk_skel.hpp"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, program.c_str(), "hfk_law.hip", 3, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
+  if (hiprtcCreateProgram(&prog, program.c_str(), "hfk_law.hip", 4, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
     log = "hiprtcCreateProgram failed";
     return false;
   }
@@ -341,6 +341,39 @@ int rtc_oseen_form(hfk_ctx* ctx, const char* source, const char* name, OseenForm
   return HFK_OK;
 }
 
+// the program and kernel instantiations of a facet form for (Q, P); false when the name is not an identifier
+bool skel_program(int Q, int P, const char* source, const char* name, std::string& program,
+                  std::vector<std::string>& names) {
+  std::string form = "hfk::HfkGradJump";  // the built-in gradient-jump penalty
+  if (source) {
+    if (!valid_identifier(name)) return false;
+    form = name;
+  }
+  program = std::string("#include \"hfk_skel.hpp\"\n") + (source ? source : "") + "\n";
+  const std::string qp = std::to_string(Q) + ", " + std::to_string(P) + ", " + form;
+  names = {"hfk::k_skel<" + qp + ", 0>", "hfk::k_skel<" + qp + ", 1>", "hfk::k_skel<" + qp + ", 2>",
+           "hfk::k_skel_jac<" + qp + ">"};
+  return true;
+}
+
+int rtc_skel(hfk_ctx* ctx, int Q, int P, const char* source, const char* name, SkelData* sk) {
+  std::string program;
+  std::vector<std::string> names;
+  if (!skel_program(Q, P, source, name, program, names))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: a facet form needs its source and a C identifier as its name");
+  CacheEntry e;
+  bool cached = false;
+  std::string log;
+  if (!compile_cached(program, names, e, cached, log))
+    return set_err(ctx, HFK_EARG, "HyperFEMKernels: the facet form does not compile: " + log);
+  HIPCHK(ctx, hipModuleLoadData(&sk->mod, e.code.data()));
+  HIPCHK(ctx, hipModuleGetFunction(&sk->res, sk->mod, e.lowered[0].c_str()));
+  HIPCHK(ctx, hipModuleGetFunction(&sk->jv, sk->mod, e.lowered[1].c_str()));
+  HIPCHK(ctx, hipModuleGetFunction(&sk->energy, sk->mod, e.lowered[2].c_str()));
+  HIPCHK(ctx, hipModuleGetFunction(&sk->jac, sk->mod, e.lowered[3].c_str()));
+  return HFK_OK;
+}
+
 int rtc_energy(hfk_op* op, const double* x, double* part, hipStream_t s) {
   RtcInstance* ri = nullptr;
   int rc = law_instance(op, &ri);
@@ -467,6 +500,22 @@ int hfk_oseen_form_check(const char* source, const char* name, char* log, int64_
     log[n] = '\0';
   }
   return ok ? HFK_OK : set_err(nullptr, HFK_EARG, "HyperFEMKernels: the point form does not compile: " + msg);
+}
+
+int hfk_skel_form_check(int Q, int P, const char* source, const char* name, char* log, int64_t log_size) {
+  std::string program, msg;
+  std::vector<std::string> names;
+  if (Q < 2 || Q > 4 || P < 2 || P > 4 || !skel_program(Q, P, source, name, program, names))
+    return set_err(nullptr, HFK_EARG, "HyperFEMKernels: invalid (Q, P) or facet form name");
+  CacheEntry e;
+  bool cached = false;
+  const bool ok = compile_cached(program, names, e, cached, msg);
+  if (log && log_size > 0) {
+    const size_t n = std::min<size_t>(msg.size(), (size_t)log_size - 1);
+    std::memcpy(log, msg.data(), n);
+    log[n] = '\0';
+  }
+  return ok ? HFK_OK : set_err(nullptr, HFK_EARG, "HyperFEMKernels: the facet form does not compile: " + msg);
 }
 
 int hfk_law_destroy(hfk_law* law) {

@@ -13,7 +13,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")
 LIB = os.path.join(HERE, "libhfk.so")
 SOURCES = ["hfk_kernels.hip", "hfk_capi.hip", "hfk_solver.hip", "hfk_pmg.hip", "hfk_net.hip", "hfk_oseen.hip",
-           "hfk_rtc.hip"]
+           "hfk_rtc.hip", "hfk_skel.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # packed FP32 instructions stay off (-packed-fp32-ops): the compiler may not form v_pk_* FP32 arithmetic.
@@ -29,7 +29,8 @@ def _run(cmd):
 
 
 # device headers compiled again at run time by hiprtc (laws given as source): embedded in the library as text
-EMBEDDED = (("kTypesSrc", "hfk_types.hpp"), ("kPointSrc", "hfk_point.hpp"), ("kOseenFormSrc", "hfk_oseen_form.hpp"))
+EMBEDDED = (("kTypesSrc", "hfk_types.hpp"), ("kPointSrc", "hfk_point.hpp"), ("kOseenFormSrc", "hfk_oseen_form.hpp"),
+            ("kSkelSrc", "hfk_skel.hpp"))
 
 
 def write_embedded(objdir: str) -> None:

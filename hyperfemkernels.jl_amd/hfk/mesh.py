@@ -234,6 +234,61 @@ def boundary_faces(space: HexSpace, axis: int, side: int):
     return face_cell, np.full(face_cell.size, 2 * axis + side, dtype=np.int32)
 
 
+def interior_facets(space: HexSpace):
+    """The interior facets of the Cartesian mesh (a SkeletonTriangulation's, the cells numbered x fastest): for
+    every pair of cells sharing a face, (plus cell, plus local face, minus cell, minus local face, orientation) as
+    int32 arrays -- the plus cell below along the facet's normal axis with its face 2 axis + 1, the minus cell above
+    with its face 2 axis; both cells have the same local axes, so every orientation is 0."""
+    nx, ny, nz = space.n
+    cell = lambda i, j, k: i + nx * (j + ny * k)
+    out = [[], [], [], [], []]
+    for axis in range(3):
+        for k in range(nz):
+            for j in range(ny):
+                for i in range(nx):
+                    ijk = [i, j, k]
+                    if ijk[axis] + 1 >= space.n[axis]:
+                        continue
+                    nb = list(ijk)
+                    nb[axis] += 1
+                    for lst, v in zip(out, (cell(*ijk), 2 * axis + 1, cell(*nb), 2 * axis, 0)):
+                        lst.append(v)
+    return tuple(np.asarray(a, dtype=np.int32) for a in out)
+
+
+def _trilinear(X, xi):
+    w = [(xi[0] if v & 1 else 1 - xi[0]) * (xi[1] if v & 2 else 1 - xi[1]) * (xi[2] if v & 4 else 1 - xi[2])
+         for v in range(8)]
+    return np.einsum("v,vi->i", np.asarray(w), np.asarray(X))
+
+
+def _face_xi(lf, s, t):
+    d, side = lf >> 1, lf & 1
+    e1, e2 = (1, 2) if d == 0 else ((0, 2) if d == 1 else (0, 1))
+    xi = [0.0, 0.0, 0.0]
+    xi[d], xi[e1], xi[e2] = float(side), s, t
+    return xi
+
+
+def facet_orientation(coords_plus, face_plus, coords_minus, face_minus) -> int:
+    """The orientation code (0..7) under which the minus face's coordinates follow the plus face's:
+    (s', t') = (t, s) if o & 1 else (s, t), then s' -> 1 - s' if o & 2, t' -> 1 - t' if o & 4 (the corners of the
+    two faces of the cells' trilinear maps are matched); raises when the faces do not coincide."""
+    for o in range(8):
+        ok = True
+        for s, t in ((0, 0), (1, 0), (0, 1), (1, 1)):
+            s2, t2 = (t, s) if o & 1 else (s, t)
+            s2, t2 = (1 - s2 if o & 2 else s2), (1 - t2 if o & 4 else t2)
+            xp = _trilinear(coords_plus, _face_xi(face_plus, s, t))
+            xm = _trilinear(coords_minus, _face_xi(face_minus, s2, t2))
+            if not np.allclose(xp, xm, atol=1e-12):
+                ok = False
+                break
+        if ok:
+            return o
+    raise ValueError("HyperFEMKernels: the two faces do not coincide")
+
+
 def refinement(fine: HexSpace, coarse: HexSpace):
     """(parent, child) of every cell of ``fine``, whose Cartesian mesh halves every cell of ``coarse`` (n_fine =
     2 n_coarse on the same box): fine cell (i, j, k) lies in coarse cell (i//2, j//2, k//2) as child

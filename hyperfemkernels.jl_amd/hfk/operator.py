@@ -174,6 +174,35 @@ class KernelOperator:
         check(self.lib.hfk_op_set_robin(self.handle, fc.size, ptr(fc, C.c_int32), ptr(fl, C.c_int32),
                                         ptr(k, C.c_double)), self.ctx.handle)
 
+    def set_skeleton(self, facets, source: str | None = None, name: str | None = None, params=(1.0,)):
+        """Skeleton integrals on interior facets (``mesh.interior_facets``: plus cell, plus face, minus cell, minus
+        face, orientation, each an array of the facets): the built-in penalty on the jump of the displacement
+        gradient (gamma = params[0]) or a facet point form given as HIP source (``source``, struct ``name``).
+        residual!, Jv, jacobian!, the energy and the sparsity pattern include them; ``facets=None`` removes them.
+        Prepared tangents are invalidated (prepare again)."""
+        self._pattern = None
+        if facets is None:
+            check(self.lib.hfk_op_set_skeleton(self.handle, 0, None, None, None, None, None, None, None, None, 0),
+                  self.ctx.handle)
+            self.nfacets = 0
+            return
+        pc, pf, mc, mf, ori = (np.ascontiguousarray(a, dtype=np.int32) for a in facets)
+        if not (pc.size == pf.size == mc.size == mf.size == ori.size):
+            raise ValueError("HyperFEMKernels: DimensionMismatch: skeleton facets")
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+        check(self.lib.hfk_op_set_skeleton(self.handle, pc.size, ptr(pc, C.c_int32), ptr(pf, C.c_int32),
+                                           ptr(mc, C.c_int32), ptr(mf, C.c_int32), ptr(ori, C.c_int32),
+                                           source.encode() if source is not None else None,
+                                           name.encode() if name is not None else None,
+                                           ptr(p, C.c_double) if p.size else None, p.size), self.ctx.handle)
+        self.nfacets = pc.size
+
+    def skeleton_colors(self) -> np.ndarray:
+        """The colour (1-based) of every facet of the skeleton."""
+        c = np.zeros(self.nfacets, dtype=np.int32)
+        check(self.lib.hfk_op_skeleton_colors(self.handle, ptr(c, C.c_int32)), self.ctx.handle)
+        return c
+
     def set_load_factor(self, load_factor: float):
         """The load factor Λ of the traction (a number of the form: nothing is recomputed)."""
         check(self.lib.hfk_op_set_load_factor(self.handle, float(load_factor)), self.ctx.handle)

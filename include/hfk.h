@@ -102,6 +102,25 @@ int hfk_op_set_traction(hfk_op* op, int64_t nfaces, const int32_t* face_cell, co
  * removes the term. */
 int hfk_op_set_robin(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
                      const double* stiffness);
+/* Skeleton integrals (SkeletonTriangulation terms, src/skeleton.jl; the facet kernels of cells!, :291-330, the cross
+ * blocks of skeleton_keys!, :355-390): interior facets, each with a plus cell and a minus cell (0-based), the local
+ * face of each (0..5: face xi_d = side is 2 d + side) and the orientation o (0..7) that maps the plus face's
+ * coordinates (s, t) along its tangential directions (in increasing order) to the minus face's:
+ * (s', t') = o & 1 ? (t, s) : (s, t), then s' -> 1 - s' if o & 2, t' -> 1 - t' if o & 4.  The faces must coincide
+ * (corners, and the dof ids of every face node) and each (cell, face) may appear once.  source/name: a facet
+ * point form (a struct with static residual, jvp and energy of the 24 facet slots, see hfk_skel.hpp); source NULL:
+ * the built-in penalty on the jump of the displacement gradient, energy (gamma h / 2) |grad u+ - grad u-|^2 with
+ * gamma = params[0] and h = sqrt(facet area).  params: up to 8 numbers.  residual!, mul!, jacobian!, the energy and
+ * the sparsity pattern (which gains the cross blocks of every facet) then include the facets; BlockJacobi and the
+ * p-multigrid refuse such an operator, as the reference's do.  nfacets = 0 removes the skeleton. */
+int hfk_op_set_skeleton(hfk_op* op, int64_t nfacets, const int32_t* plus_cell, const int32_t* plus_face,
+                        const int32_t* minus_cell, const int32_t* minus_face, const int32_t* orient,
+                        const char* source, const char* name, const double* params, int nparams);
+/* the colour (1-based) of every facet of the skeleton (color_facets, src/skeleton.jl:39-69) */
+int hfk_op_skeleton_colors(hfk_op* op, int32_t* color);
+/* Compile the facet kernels of a form (source NULL: the built-in one) for (Q, P) without a device: HFK_OK, or
+ * HFK_EARG with the compiler's log in `log` (emit!'s checks at capture time, src/codegen.jl:166-198). */
+int hfk_skel_form_check(int Q, int P, const char* source, const char* name, char* log, int64_t log_size);
 /* The load factor Λ of the traction (1 by default; a number of the form, rebound without recomputing f). */
 int hfk_op_set_load_factor(hfk_op* op, double load_factor);
 /* The traction load vector f (unscaled): nfree values on the free ids, then ndir on the Dirichlet ids. */
