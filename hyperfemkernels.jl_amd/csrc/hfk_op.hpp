@@ -71,10 +71,12 @@ struct NodeBlocks {
   DevBuf bad;                   // int counter of non-finite inverted blocks
 };
 
-// Interior facets (skeleton integrals, src/skeleton.jl) of a hexahedral operator: the plus and minus cell of every
-// facet, the facets by colour (two facets of one colour share no free dof of their cells), and the facet kernels of
-// the form, compiled with hiprtc for the operator's (Q, P).
+// Face integrals of a hexahedral operator with a compiled point form: interior facets (ns = 2, skeleton integrals,
+// src/skeleton.jl: a plus and a minus cell per facet) or boundary faces (ns = 1, one cell per face); the faces by
+// colour (two faces of one colour share no free dof of their cells), and the face kernels of the form, compiled with
+// hiprtc for the operator's (Q, P).
 struct SkelData {
+  int ns = 2;
   int64_t nf = 0;
   std::vector<int32_t> cells;   // [f][2]
   std::vector<int64_t> start;   // colour c: order[start[c] .. start[c+1])
@@ -140,8 +142,8 @@ struct hfk_op {
   hfk::NodeBlocks blocks;
   // a pointwise law compiled at run time (nullptr: the built-in law)
   hfk_law* rtc = nullptr;
-  // skeleton integrals (nullptr: none)
-  std::unique_ptr<hfk::SkelData> skel;
+  // skeleton integrals and a boundary form (nullptr: none)
+  std::unique_ptr<hfk::SkelData> skel, bnd;
 
   hfk::OpDev dev() const {
     hfk::OpDev d;
@@ -217,11 +219,12 @@ struct OseenFormKernels {
   hipFunction_t jac[2] = {}, res[2] = {};
 };
 int rtc_oseen_form(hfk_ctx* ctx, const char* source, const char* name, OseenFormKernels* out);
-// The facet kernels of a skeleton point form (source null: the built-in gradient-jump penalty) for (Q, P), loaded on
-// the current device into sk (mod, res, jv, energy, jac, split).
-int rtc_skel(hfk_ctx* ctx, int Q, int P, const char* source, const char* name, SkelData* sk);
-// the skeleton's launches (hfk_skel.hip): b += r; y += alpha J dx at the prepared state; the CSC values; one energy
-// partial per facet into part; the facet neighbours of every cell (for the pattern)
+// The face kernels of a point form with ns sides (source null: the built-in gradient-jump penalty for ns = 2, the
+// follower pressure for ns = 1) for (Q, P), loaded on the current device into sk (mod, res, jv, energy, jac).
+int rtc_skel(hfk_ctx* ctx, int Q, int P, int ns, const char* source, const char* name, SkelData* sk);
+// the face terms' launches (hfk_skel.hip), skeleton then boundary form: b += r; y += alpha J dx at the prepared
+// state; the CSC values; one energy partial per face into part; skel_facets counts the faces of both (the energy's
+// partials); the facet neighbours of every cell (for the pattern)
 int skel_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s);
 int skel_apply(hfk_op* op, const double* dx_dev, double* y_dev, double alpha, hipStream_t s);
 int skel_assemble(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s);

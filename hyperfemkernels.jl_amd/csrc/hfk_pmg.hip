@@ -138,7 +138,8 @@ struct Dots {
 // BlockJacobi on the stored tangents of op (block_diagonal!, src/preconditioners.jl:186-218)
 int blocks_update(hfk_op* op, const void* T, int storage, hipStream_t s, bool* ok) {
   if (skel_facets(op) > 0)  // src/preconditioners.jl:141 (and the p-multigrid's smoother built on it)
-    return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: BlockJacobi is not available with skeleton integrals");
+    return set_err(op->ctx, HFK_ESTATE,
+                   "HyperFEMKernels: BlockJacobi is not available with skeleton integrals or a compiled boundary form");
   int rc = op_build_blocks(op);
   if (rc) return rc;
   if (storage < 0 || !T) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: BlockJacobi before prepare!");

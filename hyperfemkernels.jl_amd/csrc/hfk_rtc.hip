@@ -341,25 +341,28 @@ int rtc_oseen_form(hfk_ctx* ctx, const char* source, const char* name, OseenForm
   return HFK_OK;
 }
 
-// the program and kernel instantiations of a facet form for (Q, P); false when the name is not an identifier
-bool skel_program(int Q, int P, const char* source, const char* name, std::string& program,
+// the program and kernel instantiations of a face form with ns sides (2: skeleton, 1: boundary) for (Q, P); false
+// when the name is not an identifier
+bool skel_program(int Q, int P, int ns, const char* source, const char* name, std::string& program,
                   std::vector<std::string>& names) {
-  std::string form = "hfk::HfkGradJump";  // the built-in gradient-jump penalty
+  // the built-in forms: the gradient-jump penalty (skeleton), the follower pressure (boundary)
+  std::string form = ns == 2 ? "hfk::HfkGradJump" : "hfk::HfkFollowerPressure";
   if (source) {
     if (!valid_identifier(name)) return false;
     form = name;
   }
   program = std::string("#include \"hfk_skel.hpp\"\n") + (source ? source : "") + "\n";
   const std::string qp = std::to_string(Q) + ", " + std::to_string(P) + ", " + form;
-  names = {"hfk::k_skel<" + qp + ", 0>", "hfk::k_skel<" + qp + ", 1>", "hfk::k_skel<" + qp + ", 2>",
-           "hfk::k_skel_jac<" + qp + ">"};
+  const std::string n = std::to_string(ns);
+  names = {"hfk::k_skel<" + qp + ", 0, " + n + ">", "hfk::k_skel<" + qp + ", 1, " + n + ">",
+           "hfk::k_skel<" + qp + ", 2, " + n + ">", "hfk::k_skel_jac<" + qp + ", " + n + ">"};
   return true;
 }
 
-int rtc_skel(hfk_ctx* ctx, int Q, int P, const char* source, const char* name, SkelData* sk) {
+int rtc_skel(hfk_ctx* ctx, int Q, int P, int ns, const char* source, const char* name, SkelData* sk) {
   std::string program;
   std::vector<std::string> names;
-  if (!skel_program(Q, P, source, name, program, names))
+  if (!skel_program(Q, P, ns, source, name, program, names))
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: a facet form needs its source and a C identifier as its name");
   CacheEntry e;
   bool cached = false;
@@ -502,10 +505,11 @@ int hfk_oseen_form_check(const char* source, const char* name, char* log, int64_
   return ok ? HFK_OK : set_err(nullptr, HFK_EARG, "HyperFEMKernels: the point form does not compile: " + msg);
 }
 
-int hfk_skel_form_check(int Q, int P, const char* source, const char* name, char* log, int64_t log_size) {
+int hfk_skel_form_check(int Q, int P, int sides, const char* source, const char* name, char* log, int64_t log_size) {
   std::string program, msg;
   std::vector<std::string> names;
-  if (Q < 2 || Q > 4 || P < 2 || P > 4 || !skel_program(Q, P, source, name, program, names))
+  if (Q < 2 || Q > 4 || P < 2 || P > 4 || (sides != 1 && sides != 2) ||
+      !skel_program(Q, P, sides, source, name, program, names))
     return set_err(nullptr, HFK_EARG, "HyperFEMKernels: invalid (Q, P) or facet form name");
   CacheEntry e;
   bool cached = false;

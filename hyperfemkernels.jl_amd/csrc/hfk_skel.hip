@@ -1,6 +1,7 @@
-// Skeleton integrals of the hexahedral operator (src/skeleton.jl): the facet list and its validation, the facet
-// colouring, the facet neighbours for the sparsity pattern, and the launches of the hiprtc-compiled facet kernels
-// (hfk_skel.hpp) in residual!, the matrix-free product, jacobian! and the energy.
+// Face integrals of the hexahedral operator with compiled point forms: skeleton integrals (src/skeleton.jl) on
+// interior facets and a boundary form on faces of the cells (the boundary groups of src/assembly.jl:539-578).  The
+// face lists and their validation, the colouring, the facet neighbours for the sparsity pattern, and the launches of
+// the hiprtc-compiled face kernels (hfk_skel.hpp) in residual!, the matrix-free product, jacobian! and the energy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,11 +15,13 @@
 
 namespace hfk {
 
-int64_t skel_facets(const hfk_op* op) { return op->skel ? op->skel->nf : 0; }
+int64_t skel_facets(const hfk_op* op) { return (op->skel ? op->skel->nf : 0) + (op->bnd ? op->bnd->nf : 0); }
 
 void skel_neighbours(const hfk_op* op, CellNeighbours& nb) {
-  const SkelData& sk = *op->skel;
   nb.start.assign((size_t)op->ncells + 1, 0);
+  nb.cells.clear();
+  if (!op->skel) return;
+  const SkelData& sk = *op->skel;
   for (int64_t f = 0; f < sk.nf; ++f) {
     nb.start[sk.cells[2 * f] + 1]++;
     nb.start[sk.cells[2 * f + 1] + 1]++;
@@ -75,9 +78,8 @@ int face_tnode_host(int Q, int lf, int m1, int m2) {
   return idx[0] + Q * (idx[1] + Q * idx[2]);
 }
 
-int launch_skel(hfk_op* op, hipFunction_t fn, int per_facet, const double* xf, const double* dx, double* out,
-                double alpha, bool jac, hipStream_t s, bool energy = false) {
-  SkelData& sk = *op->skel;
+int launch_skel(hfk_op* op, SkelData& sk, hipFunction_t fn, int per_facet, const double* xf, const double* dx,
+                double* out, double alpha, bool jac, hipStream_t s, bool energy = false) {
   OpDev d = op->dev();
   SkelDev sd{};
   sd.slot = sk.slot.as<int64_t>();
@@ -93,7 +95,7 @@ int launch_skel(hfk_op* op, hipFunction_t fn, int per_facet, const double* xf, c
     // one energy partial per facet, at the facet's position in the colour order
     double* o = energy ? out + sk.start[c] : out;
     if (jac) {
-      for (int sides = 0; sides < 4; ++sides) {  // one launch per (row side, column side) block
+      for (int sides = 0; sides < sk.ns * sk.ns; ++sides) {  // one launch per (row side, column side) block
         void* args[] = {&d, &sd, &xf, &o, &sides};
         HIPCHK(op->ctx, hipModuleLaunchKernel(fn, (unsigned)(n * per_facet), 1, 1, 256, 1, 1, 0, s, args, nullptr));
       }
@@ -108,23 +110,37 @@ int launch_skel(hfk_op* op, hipFunction_t fn, int per_facet, const double* xf, c
 }  // namespace
 
 int skel_residual(hfk_op* op, const double* x_dev, double* b_dev, hipStream_t s) {
-  if (skel_facets(op) == 0) return HFK_OK;
-  return launch_skel(op, op->skel->res, 1, x_dev, nullptr, b_dev, 1.0, false, s);
+  for (SkelData* sk : {op->skel.get(), op->bnd.get()})
+    if (sk) {
+      int rc = launch_skel(op, *sk, sk->res, 1, x_dev, nullptr, b_dev, 1.0, false, s);
+      if (rc) return rc;
+    }
+  return HFK_OK;
 }
 
 int skel_apply(hfk_op* op, const double* dx_dev, double* y_dev, double alpha, hipStream_t s) {
   if (skel_facets(op) == 0) return HFK_OK;
   if (!op->state.p) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: mul! before prepare! (no state)");
-  return launch_skel(op, op->skel->jv, 1, op->state.as<double>(), dx_dev, y_dev, alpha, false, s);
+  for (SkelData* sk : {op->skel.get(), op->bnd.get()})
+    if (sk) {
+      int rc = launch_skel(op, *sk, sk->jv, 1, op->state.as<double>(), dx_dev, y_dev, alpha, false, s);
+      if (rc) return rc;
+    }
+  return HFK_OK;
 }
 
 int skel_energy(hfk_op* op, const double* x_dev, double* part, hipStream_t s) {
-  return launch_skel(op, op->skel->energy, 1, x_dev, nullptr, part, 1.0, false, s, true);
+  for (SkelData* sk : {op->skel.get(), op->bnd.get()})
+    if (sk) {
+      int rc = launch_skel(op, *sk, sk->energy, 1, x_dev, nullptr, part, 1.0, false, s, true);
+      if (rc) return rc;
+      part += sk->nf;
+    }
+  return HFK_OK;
 }
 
-int skel_assemble(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s) {
-  if (skel_facets(op) == 0) return HFK_OK;
-  SkelData& sk = *op->skel;
+namespace {
+int face_assemble(hfk_op* op, SkelData& sk, const double* x_dev, double* nz_dev, hipStream_t s) {
   if (!sk.colptr.p) {  // the pattern (built by the caller) in 0-based form on the device
     std::vector<int64_t> cp(op->colptr.size());
     std::vector<int32_t> rv(op->rowval.size());
@@ -135,7 +151,83 @@ int skel_assemble(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s
     HIPCHK(op->ctx, hipMemcpy(sk.colptr.p, cp.data(), cp.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(op->ctx, hipMemcpy(sk.rowval.p, rv.data(), rv.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   }
-  return launch_skel(op, sk.jac, sk.split, x_dev, nullptr, nz_dev, 1.0, true, s);
+  return launch_skel(op, sk, sk.jac, sk.split, x_dev, nullptr, nz_dev, 1.0, true, s);
+}
+}  // namespace
+
+int skel_assemble(hfk_op* op, const double* x_dev, double* nz_dev, hipStream_t s) {
+  for (SkelData* sk : {op->skel.get(), op->bnd.get()})
+    if (sk) {
+      int rc = face_assemble(op, *sk, x_dev, nz_dev, s);
+      if (rc) return rc;
+    }
+  return HFK_OK;
+}
+
+// greedy first fit over the free dofs of the faces' cells (color_facets, src/skeleton.jl:39-69), 128-bit masks; the
+// faces in colour order (order, start); false when more than 128 colours are needed
+bool color_faces(const hfk_op* op, const std::vector<int32_t>& cells, int ns, std::vector<int32_t>& order,
+                 std::vector<int64_t>& start) {
+  const int64_t nf = (int64_t)cells.size() / ns;
+  std::vector<unsigned __int128> masks((size_t)op->nfree, 0);
+  std::vector<int> color((size_t)nf, 0);
+  int ncol = 0;
+  const int nd = op->nd;
+  for (int64_t f = 0; f < nf; ++f) {
+    unsigned __int128 busy = 0;
+    for (int side = 0; side < ns; ++side)
+      for (int j = 0; j < nd; ++j) {
+        const int32_t id = op->cell_dofs[(size_t)cells[ns * f + side] * nd + j];
+        if (id > 0) busy |= masks[id - 1];
+      }
+    int c = 0;
+    while (c < 128 && ((busy >> c) & 1)) ++c;
+    if (c >= 128) return false;
+    color[f] = c;
+    ncol = std::max(ncol, c + 1);
+    const unsigned __int128 bit = (unsigned __int128)1 << c;
+    for (int side = 0; side < ns; ++side)
+      for (int j = 0; j < nd; ++j) {
+        const int32_t id = op->cell_dofs[(size_t)cells[ns * f + side] * nd + j];
+        if (id > 0) masks[id - 1] |= bit;
+      }
+  }
+  start.assign((size_t)ncol + 1, 0);
+  for (int64_t f = 0; f < nf; ++f) start[color[f] + 1]++;
+  for (int c = 0; c < ncol; ++c) start[c + 1] += start[c];
+  std::vector<int64_t> fillp(start.begin(), start.end() - 1);
+  order.assign((size_t)nf, 0);
+  for (int64_t f = 0; f < nf; ++f) order[fillp[color[f]]++] = (int32_t)f;
+  return true;
+}
+
+// upload the face arrays and compile the form: sk->cells and sk->nf set by the caller
+int face_setup(hfk_op* op, SkelData* sk, const int32_t* plus_face, const int32_t* minus_face, const int32_t* orient,
+               const char* source, const char* name, const double* params, int nparams, const char* what) {
+  const int64_t nf = sk->nf;
+  if (params) std::memcpy(sk->p, params, (size_t)nparams * sizeof(double));
+  std::vector<int32_t> order;
+  if (!color_faces(op, sk->cells, sk->ns, order, sk->start))
+    return set_err(op->ctx, HFK_EDIM, std::string("HyperFEMKernels: more than 128 colours needed for the ") + what);
+  std::vector<int64_t> slot((size_t)2 * nf);
+  std::vector<int32_t> face((size_t)3 * nf);
+  for (int64_t f = 0; f < nf; ++f) {
+    slot[2 * f] = op->slot_of_cell[sk->cells[sk->ns * f]];
+    slot[2 * f + 1] = sk->ns == 2 ? op->slot_of_cell[sk->cells[2 * f + 1]] : slot[2 * f];
+    face[3 * f] = plus_face[f];
+    face[3 * f + 1] = minus_face ? minus_face[f] : plus_face[f];
+    face[3 * f + 2] = orient ? orient[f] : 0;
+  }
+  HIPCHK(op->ctx, sk->slot.alloc(slot.size() * sizeof(int64_t)));
+  HIPCHK(op->ctx, sk->face.alloc(face.size() * sizeof(int32_t)));
+  HIPCHK(op->ctx, sk->order.alloc(order.size() * sizeof(int32_t)));
+  HIPCHK(op->ctx, hipMemcpy(sk->slot.p, slot.data(), slot.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, hipMemcpy(sk->face.p, face.data(), face.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, hipMemcpy(sk->order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  int rc = rtc_skel(op->ctx, op->Q, op->P, sk->ns, source, name, sk);
+  if (rc) return rc;
+  sk->split = skel_jac_split(op->Q);
+  return HFK_OK;
 }
 
 }  // namespace hfk
@@ -209,62 +301,47 @@ int hfk_op_set_skeleton(hfk_op* op, int64_t nfacets, const int32_t* plus_cell, c
   if (std::adjacent_find(used.begin(), used.end()) != used.end())
     return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: skeleton: a (cell, local face) is used by two facets");
   std::unique_ptr<SkelData> sk(new SkelData());
+  sk->ns = 2;
   sk->nf = nfacets;
   sk->cells.resize((size_t)2 * nfacets);
   for (int64_t f = 0; f < nfacets; ++f) {
     sk->cells[2 * f] = plus_cell[f];
     sk->cells[2 * f + 1] = minus_cell[f];
   }
-  if (params) std::memcpy(sk->p, params, (size_t)nparams * sizeof(double));
-  // color_facets (src/skeleton.jl:39-69): greedy first fit over the free dofs of both cells, 128-bit masks
-  std::vector<unsigned __int128> masks((size_t)op->nfree, 0);
-  std::vector<int> color((size_t)nfacets, 0);
-  int ncol = 0;
-  const int nd = op->nd;
-  for (int64_t f = 0; f < nfacets; ++f) {
-    unsigned __int128 busy = 0;
-    for (int side = 0; side < 2; ++side)
-      for (int j = 0; j < nd; ++j) {
-        const int32_t id = op->cell_dofs[(size_t)sk->cells[2 * f + side] * nd + j];
-        if (id > 0) busy |= masks[id - 1];
-      }
-    int c = 0;
-    while (c < 128 && ((busy >> c) & 1)) ++c;
-    if (c >= 128) return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: more than 128 colours needed for the skeleton facets");
-    color[f] = c;
-    ncol = std::max(ncol, c + 1);
-    const unsigned __int128 bit = (unsigned __int128)1 << c;
-    for (int side = 0; side < 2; ++side)
-      for (int j = 0; j < nd; ++j) {
-        const int32_t id = op->cell_dofs[(size_t)sk->cells[2 * f + side] * nd + j];
-        if (id > 0) masks[id - 1] |= bit;
-      }
-  }
-  sk->start.assign((size_t)ncol + 1, 0);
-  for (int64_t f = 0; f < nfacets; ++f) sk->start[color[f] + 1]++;
-  for (int c = 0; c < ncol; ++c) sk->start[c + 1] += sk->start[c];
-  std::vector<int64_t> fillp(sk->start.begin(), sk->start.end() - 1);
-  std::vector<int32_t> order((size_t)nfacets);
-  for (int64_t f = 0; f < nfacets; ++f) order[fillp[color[f]]++] = (int32_t)f;
-  std::vector<int64_t> slot((size_t)2 * nfacets);
-  std::vector<int32_t> face((size_t)3 * nfacets);
-  for (int64_t f = 0; f < nfacets; ++f) {
-    slot[2 * f] = op->slot_of_cell[plus_cell[f]];
-    slot[2 * f + 1] = op->slot_of_cell[minus_cell[f]];
-    face[3 * f] = plus_face[f];
-    face[3 * f + 1] = minus_face[f];
-    face[3 * f + 2] = orient[f];
-  }
-  HIPCHK(op->ctx, sk->slot.alloc(slot.size() * sizeof(int64_t)));
-  HIPCHK(op->ctx, sk->face.alloc(face.size() * sizeof(int32_t)));
-  HIPCHK(op->ctx, sk->order.alloc(order.size() * sizeof(int32_t)));
-  HIPCHK(op->ctx, hipMemcpy(sk->slot.p, slot.data(), slot.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  HIPCHK(op->ctx, hipMemcpy(sk->face.p, face.data(), face.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIPCHK(op->ctx, hipMemcpy(sk->order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  int rc = rtc_skel(op->ctx, op->Q, op->P, source, name, sk.get());
+  int rc = face_setup(op, sk.get(), plus_face, minus_face, orient, source, name, params, nparams, "skeleton facets");
   if (rc) return rc;
-  sk->split = skel_jac_split(op->Q);
   op->skel = std::move(sk);
+  return HFK_OK;
+}
+
+int hfk_op_set_boundary_form(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                             const char* source, const char* name, const double* params, int nparams) {
+  if (!op) return set_err(nullptr, HFK_EARG, "HyperFEMKernels: null operator");
+  HIPCHK(op->ctx, hipSetDevice(op->ctx->device));
+  if (nfaces < 0 || nparams < 0 || nparams > 8 || (nparams > 0 && !params))
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: boundary form: invalid face count or parameters (at most 8)");
+  op->storage = -1;  // the tangents of a matrix-free Jacobian belong to the operator without the boundary form
+  if (nfaces == 0) {
+    op->bnd.reset();
+    return HFK_OK;
+  }
+  if (!face_cell || !face_local) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: boundary form: null face array");
+  std::vector<int64_t> used((size_t)nfaces);
+  for (int64_t f = 0; f < nfaces; ++f) {
+    if (face_cell[f] < 0 || face_cell[f] >= op->ncells || face_local[f] < 0 || face_local[f] > 5)
+      return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: boundary form: face out of range (cell, local face 0..5)");
+    used[f] = (int64_t)face_cell[f] * 6 + face_local[f];
+  }
+  std::sort(used.begin(), used.end());
+  if (std::adjacent_find(used.begin(), used.end()) != used.end())
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: boundary form: a (cell, local face) pair is listed twice");
+  std::unique_ptr<SkelData> sk(new SkelData());
+  sk->ns = 1;
+  sk->nf = nfaces;
+  sk->cells.assign(face_cell, face_cell + nfaces);
+  int rc = face_setup(op, sk.get(), face_local, nullptr, nullptr, source, name, params, nparams, "boundary faces");
+  if (rc) return rc;
+  op->bnd = std::move(sk);
   return HFK_OK;
 }
 

@@ -43,7 +43,8 @@ SIGNATURES = {
     "hfk_op_set_skeleton": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _i32p, _i32p, _i32p, C.c_char_p, C.c_char_p,
                                       _f64p, C.c_int]),
     "hfk_op_skeleton_colors": (C.c_int, [_vp, _i32p]),
-    "hfk_skel_form_check": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64]),
+    "hfk_skel_form_check": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64]),
+    "hfk_op_set_boundary_form": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, C.c_char_p, C.c_char_p, _f64p, C.c_int]),
     "hfk_op_traction_load": (C.c_int, [_vp, _f64p]),
     "hfk_op_set_mu_function": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _f64p, C.c_int64]),
     "hfk_op_mu_field": (C.c_int, [_vp, _f64p]),

@@ -197,6 +197,24 @@ class KernelOperator:
                                            ptr(p, C.c_double) if p.size else None, p.size), self.ctx.handle)
         self.nfacets = pc.size
 
+    def set_boundary_form(self, face_cell, face_local, source: str | None = None, name: str | None = None,
+                          params=(1.0,)):
+        """A boundary group with a compiled point form on faces of the cells (``mesh.boundary_faces``): the built-in
+        follower pressure (p = params[0]) or a form given as HIP source (``source``, struct ``name``); in residual!,
+        Jv, jacobian! and the energy.  ``face_cell=None`` removes it.  Prepared tangents are invalidated."""
+        if face_cell is None:
+            check(self.lib.hfk_op_set_boundary_form(self.handle, 0, None, None, None, None, None, 0), self.ctx.handle)
+            return
+        fc = np.ascontiguousarray(face_cell, dtype=np.int32)
+        fl = np.ascontiguousarray(face_local, dtype=np.int32)
+        if fl.size != fc.size:
+            raise ValueError("HyperFEMKernels: DimensionMismatch: boundary faces")
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+        check(self.lib.hfk_op_set_boundary_form(self.handle, fc.size, ptr(fc, C.c_int32), ptr(fl, C.c_int32),
+                                                source.encode() if source is not None else None,
+                                                name.encode() if name is not None else None,
+                                                ptr(p, C.c_double) if p.size else None, p.size), self.ctx.handle)
+
     def skeleton_colors(self) -> np.ndarray:
         """The colour (1-based) of every facet of the skeleton."""
         c = np.zeros(self.nfacets, dtype=np.int32)

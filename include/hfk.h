@@ -108,7 +108,8 @@ int hfk_op_set_robin(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const
  * coordinates (s, t) along its tangential directions (in increasing order) to the minus face's:
  * (s', t') = o & 1 ? (t, s) : (s, t), then s' -> 1 - s' if o & 2, t' -> 1 - t' if o & 4.  The faces must coincide
  * (corners, and the dof ids of every face node) and each (cell, face) may appear once.  source/name: a facet
- * point form (a struct with static residual, jvp and energy of the 24 facet slots, see hfk_skel.hpp); source NULL:
+ * point form (a struct with static residual(s, x, n, h, p, R), jvp(s, ds, x, n, h, p, dR) and energy(s, x, n, h, p)
+ * of the 24 facet slots s, the point x, the normal n, h and p; see hfk_skel.hpp); source NULL:
  * the built-in penalty on the jump of the displacement gradient, energy (gamma h / 2) |grad u+ - grad u-|^2 with
  * gamma = params[0] and h = sqrt(facet area).  params: up to 8 numbers.  residual!, mul!, jacobian!, the energy and
  * the sparsity pattern (which gains the cross blocks of every facet) then include the facets; BlockJacobi and the
@@ -118,9 +119,19 @@ int hfk_op_set_skeleton(hfk_op* op, int64_t nfacets, const int32_t* plus_cell, c
                         const char* source, const char* name, const double* params, int nparams);
 /* the colour (1-based) of every facet of the skeleton (color_facets, src/skeleton.jl:39-69) */
 int hfk_op_skeleton_colors(hfk_op* op, int32_t* color);
-/* Compile the facet kernels of a form (source NULL: the built-in one) for (Q, P) without a device: HFK_OK, or
- * HFK_EARG with the compiler's log in `log` (emit!'s checks at capture time, src/codegen.jl:166-198). */
-int hfk_skel_form_check(int Q, int P, const char* source, const char* name, char* log, int64_t log_size);
+/* A boundary group with a captured integrand (src/assembly.jl:539-578: the boundary classes of a
+ * BoundaryTriangulation): faces of the cells (cell 0-based, local face 0..5, each pair once) and a point form of the
+ * 12 slots of the cell at the face point (u, then du_c/dx_K at 3 + 3c + K), the point x, the outward unit normal n
+ * of the reference configuration, h = sqrt(face area) and params (static residual, jvp and energy as for the
+ * skeleton).  source NULL: the built-in follower pressure p = params[0] on the deformed face, nominal traction
+ * -p cof(F) N (not conservative: no energy).  residual!, mul!, jacobian! and the energy include it; BlockJacobi and
+ * the p-multigrid refuse such an operator.  nfaces = 0 removes it. */
+int hfk_op_set_boundary_form(hfk_op* op, int64_t nfaces, const int32_t* face_cell, const int32_t* face_local,
+                             const char* source, const char* name, const double* params, int nparams);
+/* Compile the face kernels of a form with `sides` sides (2: skeleton, 1: boundary; source NULL: the built-in one)
+ * for (Q, P) without a device: HFK_OK, or HFK_EARG with the compiler's log in `log` (emit!'s checks at capture time,
+ * src/codegen.jl:166-198). */
+int hfk_skel_form_check(int Q, int P, int sides, const char* source, const char* name, char* log, int64_t log_size);
 /* The load factor Λ of the traction (1 by default; a number of the form, rebound without recomputing f). */
 int hfk_op_set_load_factor(hfk_op* op, double load_factor);
 /* The traction load vector f (unscaled): nfree values on the free ids, then ndir on the Dirichlet ids. */

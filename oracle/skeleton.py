@@ -1,4 +1,5 @@
-"""Skeleton (interior facet) integrals of the hexahedral operator, restated in numpy for small meshes.
+"""Face integrals of the hexahedral operator -- skeleton (interior facet) integrals and boundary groups with a
+captured integrand -- restated in numpy for small meshes.
 
 Test infrastructure: only tests/ and the checker import this module; the product never does.
 
@@ -8,11 +9,14 @@ It follows the reference's skeleton path:
   (``w meas(Jf+ . Jt+)``), the point form sees the unknown of both sides;
 * src/skeleton.jl:291-330 (``cells!``) -- per facet: gather both cells, element residual / Jacobian over the points,
   scatter both halves;
-* src/skeleton.jl:39-69 (``color_facets``) and :355-390 (``skeleton_keys!``: the cross blocks of every facet).
+* src/skeleton.jl:39-69 (``color_facets``) and :355-390 (``skeleton_keys!``: the cross blocks of every facet);
+* src/assembly.jl:539-578 -- a boundary group: the same with one cell per face.
 
-A facet point form here is an object with ``residual(s, n, h, p)``, ``jvp(s, ds, n, h, p)`` and
-``energy(s, n, h, p)`` of the 24 slots s (plus: u 0..2, du_c/dx_K at 3 + 3c + K; minus: the same at 12..23), the unit
-normal n outward from the plus cell, h = sqrt(facet area) and the parameters p -- the interface of the device forms.
+A point form here is an object with ``residual(s, x, n, h, p)``, ``jvp(s, ds, x, n, h, p)`` and
+``energy(s, x, n, h, p)`` of the 12 slots per side s (plus: u 0..2, du_c/dx_K at 3 + 3c + K; minus: the same at
+12..23), the physical point x, the unit normal n outward from the plus cell, h = sqrt(face area) and the parameters
+p -- the interface of the device forms.  Faces: a skeleton's (plus cell, plus face, minus cell, minus face,
+orientation) arrays, or a boundary group's (cell, face) arrays.
 """
 from __future__ import annotations
 
@@ -70,28 +74,36 @@ def shape(Q, xi):
     return N, G
 
 
-def facet_tables(space, P, plus_cell, plus_face, minus_cell, minus_face, orient):
-    """Per point q = a + P b of the facet: dS, n, and per side the shape values (NN,) and physical gradients (NN, 3)."""
+def trilinear_point(X, xi):
+    return sum(((xi[0] if v & 1 else 1 - xi[0]) * (xi[1] if v & 2 else 1 - xi[1]) * (xi[2] if v & 4 else 1 - xi[2])) * X[v]
+               for v in range(8))
+
+
+def facet_tables(space, P, plus_cell, plus_face, minus_cell=None, minus_face=None, orient=0):
+    """Per point q = a + P b of the face: (dS, n, x, [per side (shape values (NN,), physical gradients (NN, 3))]);
+    one side without a minus cell."""
     Q = space.Q
     g, w = gauss01(P)
-    Xp, Xm = space.coords[plus_cell], space.coords[minus_cell]
+    cells = [(space.coords[plus_cell], plus_face, False)]
+    if minus_cell is not None:
+        cells.append((space.coords[minus_cell], minus_face, True))
     pts = []
     for b in range(P):
         for a in range(P):
             s, t = g[a], g[b]
-            s2, t2 = orient_st(orient, s, t)
             sides = []
-            for X, lf, (u, v) in ((Xp, plus_face, (s, t)), (Xm, minus_face, (s2, t2))):
-                xi = face_xi(lf, u, v)
+            for X, lf, minus in cells:
+                xi = face_xi(lf, *(orient_st(orient, s, t) if minus else (s, t)))
                 J = trilinear_jacobian(X, xi)
                 N, Gr = shape(Q, xi)
-                sides.append((N, Gr @ np.linalg.inv(J), J))
+                sides.append((N, Gr @ np.linalg.inv(J), J, xi))
             d, side, e1, e2 = face_dirs(plus_face)
             J = sides[0][2]
             n = np.cross(J[:, e1], J[:, e2])
             nn = np.linalg.norm(n)
             sg = 1.0 if ((n @ J[:, d] > 0) == (side == 1)) else -1.0
-            pts.append((w[a] * w[b] * nn, sg * n / nn, sides[0][:2], sides[1][:2]))
+            x = trilinear_point(cells[0][0], sides[0][3])
+            pts.append((w[a] * w[b] * nn, sg * n / nn, x, [sd[:2] for sd in sides]))
     return pts
 
 
@@ -109,9 +121,9 @@ def _values(ids, xf, dvals):
     return out
 
 
-def _slots(pt, Up, Um):
-    s = np.zeros(24)
-    for side, (N, Gx), U in ((0, pt[2], Up), (1, pt[3], Um)):
+def _slots(pt, Us):
+    s = np.zeros(12 * len(Us))
+    for side, ((N, Gx), U) in enumerate(zip(pt[3], Us)):
         for c in range(3):
             s[12 * side + c] = N @ U[c]
             s[12 * side + 3 + 3 * c: 12 * side + 6 + 3 * c] = Gx.T @ U[c]
@@ -119,66 +131,79 @@ def _slots(pt, Up, Um):
 
 
 def _contract(pt, R):
-    """Element vectors of both sides ([c][t] each) of the slot coefficients R (24,) at one point."""
+    """Element vectors of every side ([c][t] each) of the slot coefficients R (12 per side) at one point."""
     out = []
-    for side, (N, Gx) in ((0, pt[2]), (1, pt[3])):
+    for side, (N, Gx) in enumerate(pt[3]):
         r = R[12 * side: 12 * side + 12]
         out.append(np.array([r[c] * N + Gx @ r[3 + 3 * c: 6 + 3 * c] for c in range(3)]))
     return out
 
 
-def skeleton_terms(space, facets, form, params, xf, dvals, P=None, dx=None):
-    """Residual r (nfree,), energy, and (if dx is given) the product J dx (nfree,) of the skeleton at state xf."""
+def _faces(faces):
+    """(plus cell, plus face, minus cell, minus face, orientation) per face; None for the minus side of a boundary
+    group's faces."""
+    if len(faces) == 5:
+        return [tuple(int(a[f]) for a in faces) for f in range(len(faces[0]))]
+    return [(int(faces[0][f]), int(faces[1][f]), None, None, 0) for f in range(len(faces[0]))]
+
+
+def face_terms(space, faces, form, params, xf, dvals, P=None, dx=None):
+    """Residual r (nfree,), energy, and (if dx is given) the product J dx (nfree,) of the face integrals at state xf."""
     P = P or space.Q
-    pc, pf, mc, mf, ori = facets
     r = np.zeros(space.nfree)
     y = np.zeros(space.nfree)
     energy = 0.0
-    for f in range(len(pc)):
-        pts = facet_tables(space, P, pc[f], pf[f], mc[f], mf[f], ori[f])
+    for pc, pf, mc, mf, ori in _faces(faces):
+        pts = facet_tables(space, P, pc, pf, mc, mf, ori)
         h = np.sqrt(sum(pt[0] for pt in pts))
-        idp, idm = tensor_ids(space, pc[f]), tensor_ids(space, mc[f])
-        Up, Um = _values(idp, xf, dvals), _values(idm, xf, dvals)
-        if dx is not None:
-            dUp, dUm = _values(idp, dx, None), _values(idm, dx, None)
+        ids = [tensor_ids(space, c) for c in ((pc,) if mc is None else (pc, mc))]
+        Us = [_values(i, xf, dvals) for i in ids]
+        dUs = [_values(i, dx, None) for i in ids] if dx is not None else None
         for pt in pts:
-            s = _slots(pt, Up, Um)
-            energy += pt[0] * form.energy(s, pt[1], h, params)
-            for vec, R in ((r, form.residual(s, pt[1], h, params)),) + (
-                    ((y, form.jvp(s, _slots(pt, dUp, dUm), pt[1], h, params)),) if dx is not None else ()):
-                for ids, e in zip((idp, idm), _contract(pt, pt[0] * R)):
-                    m = ids > 0
-                    np.add.at(vec, ids[m] - 1, e[m])
+            s = _slots(pt, Us)
+            energy += pt[0] * form.energy(s, pt[2], pt[1], h, params)
+            pairs = [(r, form.residual(s, pt[2], pt[1], h, params))]
+            if dx is not None:
+                pairs.append((y, form.jvp(s, _slots(pt, dUs), pt[2], pt[1], h, params)))
+            for vec, R in pairs:
+                for i, e in zip(ids, _contract(pt, pt[0] * R)):
+                    m = i > 0
+                    np.add.at(vec, i[m] - 1, e[m])
     return r, energy, (y if dx is not None else None)
 
 
-def skeleton_matrix(space, facets, form, params, xf, dvals, P=None):
-    """The dense skeleton Jacobian (nfree, nfree) at state xf: columns of the form's jvp in the unit directions."""
+def face_matrix(space, faces, form, params, xf, dvals, P=None):
+    """The dense Jacobian (nfree, nfree) of the face integrals at state xf: columns of the form's jvp in the unit
+    directions."""
     P = P or space.Q
-    pc, pf, mc, mf, ori = facets
     A = np.zeros((space.nfree, space.nfree))
     nd = space.nd
-    for f in range(len(pc)):
-        pts = facet_tables(space, P, pc[f], pf[f], mc[f], mf[f], ori[f])
+    for pc, pf, mc, mf, ori in _faces(faces):
+        pts = facet_tables(space, P, pc, pf, mc, mf, ori)
         h = np.sqrt(sum(pt[0] for pt in pts))
-        idp, idm = tensor_ids(space, pc[f]), tensor_ids(space, mc[f])
-        Up, Um = _values(idp, xf, dvals), _values(idm, xf, dvals)
-        ids = np.concatenate([idp.reshape(-1), idm.reshape(-1)])
-        K = np.zeros((2 * nd, 2 * nd))
+        cells = (pc,) if mc is None else (pc, mc)
+        ns = len(cells)
+        tids = [tensor_ids(space, c) for c in cells]
+        Us = [_values(i, xf, dvals) for i in tids]
+        ids = np.concatenate([i.reshape(-1) for i in tids])
+        K = np.zeros((ns * nd, ns * nd))
         for pt in pts:
-            s = _slots(pt, Up, Um)
-            for j in range(2 * nd):
+            s = _slots(pt, Us)
+            for j in range(ns * nd):
                 if ids[j] <= 0:
                     continue
-                e = np.zeros(2 * nd)
+                e = np.zeros(ns * nd)
                 e[j] = 1.0
-                ds = _slots(pt, e[:nd].reshape(3, -1), e[nd:].reshape(3, -1))
-                a, b = _contract(pt, pt[0] * form.jvp(s, ds, pt[1], h, params))
-                K[:, j] += np.concatenate([a.reshape(-1), b.reshape(-1)])
+                ds = _slots(pt, [e[k * nd:(k + 1) * nd].reshape(3, -1) for k in range(ns)])
+                K[:, j] += np.concatenate([v.reshape(-1) for v in _contract(pt, pt[0] * form.jvp(s, ds, pt[2], pt[1], h, params))])
         m = ids > 0
         g = ids[m] - 1  # the face nodes appear on both sides: accumulate, do not overwrite
         np.add.at(A, (g[:, None], g[None, :]), K[np.ix_(m, m)])
     return A
+
+
+skeleton_terms = face_terms
+skeleton_matrix = face_matrix
 
 
 def skeleton_pairs(space, facets):
@@ -197,7 +222,7 @@ class GradJump:
     """The built-in facet form: energy (gamma h / 2) |grad u+ - grad u-|^2, gamma = p[0]."""
 
     @staticmethod
-    def residual(s, n, h, p):
+    def residual(s, x, n, h, p):
         R = np.zeros(24)
         j = s[3:12] - s[15:24]
         R[3:12] = p[0] * h * j
@@ -205,11 +230,11 @@ class GradJump:
         return R
 
     @staticmethod
-    def jvp(s, ds, n, h, p):
-        return GradJump.residual(ds, n, h, p)
+    def jvp(s, ds, x, n, h, p):
+        return GradJump.residual(ds, x, n, h, p)
 
     @staticmethod
-    def energy(s, n, h, p):
+    def energy(s, x, n, h, p):
         j = s[3:12] - s[15:24]
         return 0.5 * p[0] * h * (j @ j)
 
@@ -226,7 +251,7 @@ class NonlinearFacet:
         return J, um
 
     @staticmethod
-    def energy(s, n, h, p):
+    def energy(s, x, n, h, p):
         g, k, b, a = p[:4]
         J, um = NonlinearFacet._parts(s, n)
         jj = np.sum(J * J)
@@ -234,7 +259,7 @@ class NonlinearFacet:
         return g * h * (0.5 * jj + 0.25 * k * jj * jj) + 0.5 * b * (jn @ jn) + 0.5 * a * (um @ um) * jj
 
     @staticmethod
-    def residual(s, n, h, p):
+    def residual(s, x, n, h, p):
         g, k, b, a = p[:4]
         J, um = NonlinearFacet._parts(s, n)
         jj = np.sum(J * J)
@@ -249,7 +274,7 @@ class NonlinearFacet:
         return R
 
     @staticmethod
-    def jvp(s, ds, n, h, p):
+    def jvp(s, ds, x, n, h, p):
         g, k, b, a = p[:4]
         J, um = NonlinearFacet._parts(s, n)
         dJ, dum = NonlinearFacet._parts(ds, n)
@@ -265,4 +290,73 @@ class NonlinearFacet:
         R[15:24] = -ddJ.reshape(-1)
         R[0:3] = 0.5 * ddum
         R[12:15] = 0.5 * ddum
+        return R
+
+
+class FollowerPressure:
+    """The built-in boundary form: follower pressure p[0], R_c = p (cof F N)_c with F = I + grad u (the virtual work
+    of the nominal traction -p cof(F) N); no energy."""
+
+    @staticmethod
+    def _F(s):
+        return np.eye(3) + s[3:12].reshape(3, 3)
+
+    @staticmethod
+    def residual(s, x, n, h, p):
+        F = FollowerPressure._F(s)
+        cof = np.linalg.det(F) * np.linalg.inv(F).T
+        R = np.zeros(12)
+        R[0:3] = p[0] * cof @ n
+        return R
+
+    @staticmethod
+    def jvp(s, ds, x, n, h, p):
+        F = FollowerPressure._F(s)
+        dF = ds[3:12].reshape(3, 3)
+        # d cof(F) = cof(F) tr(F^-1 dF) - cof(F) dF^T F^-T  (from cof F = det(F) F^-T)
+        Fi = np.linalg.inv(F)
+        cof = np.linalg.det(F) * Fi.T
+        dcof = cof * np.trace(Fi @ dF) - cof @ dF.T @ Fi.T
+        R = np.zeros(12)
+        R[0:3] = p[0] * dcof @ n
+        return R
+
+    @staticmethod
+    def energy(s, x, n, h, p):
+        return 0.0
+
+
+class NonlinearBoundary:
+    """A nonlinear boundary form with every slot and the point: e = k |u|^2 / 2 + kappa |u|^4 / 4 + beta |grad u n|^2 / 2
+    + gamma x_0 |u|^2 / 2, p = (k, kappa, beta, gamma)."""
+
+    @staticmethod
+    def energy(s, x, n, h, p):
+        k, kap, b, g = p[:4]
+        u = s[0:3]
+        G = s[3:12].reshape(3, 3)
+        uu = u @ u
+        gn = G @ n
+        return 0.5 * k * uu + 0.25 * kap * uu * uu + 0.5 * b * (gn @ gn) + 0.5 * g * x[0] * uu
+
+    @staticmethod
+    def residual(s, x, n, h, p):
+        k, kap, b, g = p[:4]
+        u = s[0:3]
+        G = s[3:12].reshape(3, 3)
+        uu = u @ u
+        R = np.zeros(12)
+        R[0:3] = (k + kap * uu + g * x[0]) * u
+        R[3:12] = b * np.outer(G @ n, n).reshape(-1)
+        return R
+
+    @staticmethod
+    def jvp(s, ds, x, n, h, p):
+        k, kap, b, g = p[:4]
+        u, du = s[0:3], ds[0:3]
+        dG = ds[3:12].reshape(3, 3)
+        uu = u @ u
+        R = np.zeros(12)
+        R[0:3] = (k + kap * uu + g * x[0]) * du + 2.0 * kap * (u @ du) * u
+        R[3:12] = b * np.outer(dG @ n, n).reshape(-1)
         return R

@@ -1,6 +1,6 @@
-"""Skeleton (interior facet) integrals of the hexahedral operator on the device against the oracle
-(oracle/skeleton.py, pinned in tests/test_oracle_skeleton.py), for Q1-Q3 on perturbed (non-affine) meshes with
-Dirichlet faces:
+"""Face integrals of the hexahedral operator on the device against the oracle (oracle/skeleton.py, pinned in
+tests/test_oracle_skeleton.py), for Q1-Q3 on perturbed (non-affine) meshes with Dirichlet faces.  Skeleton
+(interior facet) integrals:
 
 * the sparsity pattern is the volume pattern plus the cross blocks of every facet (skeleton_keys!), bit for bit
   against a Python set of pairs; facets of one colour share no free dof (color_facets);
@@ -9,6 +9,9 @@ Dirichlet faces:
 * the closed form of the penalty of a kinked field; a cell with rotated local axes (orientation 5) gives the
   operator of the unrotated mesh;
 * facets that do not coincide, a (cell, face) used twice, and BlockJacobi on such an operator are refused.
+Boundary groups with a captured integrand: the built-in follower pressure and a nonlinear form compiled from HIP
+source, alone and together with a skeleton: residual!, Jv, the CSC values and the energy equal the oracle's (1e-12),
+the pattern stays the volume pattern.
 """
 import numpy as np
 import pytest
@@ -18,7 +21,7 @@ from hfk._lib import HFKError
 from hfk.functionals import KernelFunctional
 from hfk.mesh import dirichlet_closed_x0, dirichlet_none, hex_space, interior_facets
 from oracle import skeleton as S
-from skeleton_forms import NONLINEAR_FACET_SOURCE
+from skeleton_forms import NONLINEAR_BOUNDARY_SOURCE, NONLINEAR_FACET_SOURCE
 
 pytestmark = pytest.mark.gpu
 
@@ -141,3 +144,54 @@ def test_refused(hip_ctx, oracle):
 
     with pytest.raises(HFKError):  # as the reference, src/preconditioners.jl:141
         BlockJacobi(mfj)
+
+
+def some_boundary_faces(sp):
+    from hfk.mesh import boundary_faces
+
+    return tuple(np.concatenate(a) for a in zip(boundary_faces(sp, 0, 1), boundary_faces(sp, 1, 0),
+                                                boundary_faces(sp, 2, 1)))
+
+
+@pytest.mark.parametrize("order,form,skeleton", [(1, "follower", False), (2, "follower", False),
+                                                 (2, "compiled", False), (3, "compiled", False),
+                                                 (2, "compiled", True)])
+def test_boundary_form_against_oracle(hip_ctx, oracle, order, form, skeleton):
+    print(f"boundary form Q{order} {form} skeleton={skeleton}", flush=True)
+    sp, dv, op, orc = make(hip_ctx, oracle, order)
+    fc, fl = some_boundary_faces(sp)
+    if form == "follower":
+        params, oform = [2.0], S.FollowerPressure
+        op.set_boundary_form(fc, fl, params=params)
+    else:
+        params, oform = [0.7, 3.0, 0.4, 1.3], S.NonlinearBoundary
+        op.set_boundary_form(fc, fl, source=NONLINEAR_BOUNDARY_SOURCE, name="NonlinearBoundary", params=params)
+    fac = interior_facets(sp)
+    if skeleton:
+        op.set_skeleton(fac, params=[0.9])
+    colptr, rowval, _ = op.pattern()
+    c0, r0, _ = orc.pattern()
+    if not skeleton:  # a boundary group couples the dofs of one cell: the volume pattern
+        assert np.array_equal(colptr, c0) and np.array_equal(rowval, r0)
+    rng = np.random.default_rng(order)
+    x = 0.01 / order * rng.standard_normal(sp.nfree)
+    dx = rng.standard_normal(sp.nfree)
+    rb, eb, yb = S.face_terms(sp, (fc, fl), oform, params, x, dv, dx=dx)
+    Ab = S.face_matrix(sp, (fc, fl), oform, params, x, dv)
+    if skeleton:
+        rs, es, ys = S.face_terms(sp, fac, S.GradJump, [0.9], x, dv, dx=dx)
+        rb, eb, yb = rb + rs, eb + es, yb + ys
+        Ab = Ab + S.face_matrix(sp, fac, S.GradJump, [0.9], x, dv)
+    assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x) + rb) < 1e-12
+    mf = MatrixFreeJacobian(op, "full64").prepare(x)
+    assert rel(mf @ dx, orc.jv(orc.prepare(x), dx) + yb) < 1e-12
+    A = op.jacobian(op.allocate_jacobian(), x)
+    Ad = dense_csc(colptr, rowval, A.data, sp.nfree)
+    Ao = orc.csc(x).toarray() + Ab
+    assert np.abs(Ad - Ao).max() < 1e-12 * np.abs(Ao).max()
+    assert KernelFunctional(op)(x) == pytest.approx(orc.energy(x) + eb, rel=1e-12)
+    with pytest.raises(HFKError):  # a face listed twice
+        op.set_boundary_form(np.r_[fc, fc[:1]], np.r_[fl, fl[:1]])
+    op.set_boundary_form(None, None)
+    op.set_skeleton(None)
+    assert rel(op.residual(np.zeros(sp.nfree), x), orc.residual(x)) < 1e-12

@@ -134,3 +134,50 @@ def test_pairs_are_the_facet_cells():
         nbr[int(fac[2][f])].add(int(fac[0][f]))
     for c, r in pairs:  # every pair comes from two cells across a facet
         assert any(cr in nbr[cc] for cc in cells_of[c] for cr in cells_of[r])
+
+
+# ---- boundary groups with a captured integrand (one cell per face) -------------------------------------------
+
+def x1_faces(sp):
+    from hfk.mesh import boundary_faces
+
+    return boundary_faces(sp, 0, 1)
+
+
+@pytest.mark.parametrize("u,force", [(lambda X: np.stack([0.2 * X[:, 0], 0 * X[:, 0], 0 * X[:, 0]], 1), (1.0, 0, 0)),
+                                     (lambda X: np.stack([0 * X[:, 0], 0.3 * X[:, 1], 0 * X[:, 0]], 1), (1.3, 0, 0)),
+                                     (lambda X: np.stack([0 * X[:, 0], 0 * X[:, 0], -0.25 * X[:, 2]], 1), (0.75, 0, 0))])
+def test_follower_pressure_closed_form(u, force):
+    """On the face x = 1 of the unit cube under a homogeneous deformation the total follower force is
+    p cof(F) e_x |face|: stretching along x leaves it p e_x, stretching y or z by (1 + b) scales it by (1 + b)."""
+    sp = hex_space((2, 2, 2), 2, dirichlet=dirichlet_none, perturb=0.2, seed=1)  # boundary vertices stay put
+    r, e, _ = S.face_terms(sp, x1_faces(sp), S.FollowerPressure, [2.0], sp.free_values(u), None)
+    tot = np.array([r[c::3].sum() for c in range(3)])  # free ids interleave the components
+    assert np.allclose(tot, 2.0 * np.array(force), atol=1e-12) and e == 0.0
+
+
+@pytest.mark.parametrize("form,params", [(S.FollowerPressure, [2.0]), (S.NonlinearBoundary, [0.7, 3.0, 0.4, 1.3])])
+def test_boundary_form_derivatives(form, params):
+    from hfk.mesh import boundary_faces
+
+    sp = hex_space((2, 2, 1), 2, dirichlet=dirichlet_none, perturb=0.2, seed=2)
+    faces = tuple(np.concatenate(a) for a in zip(boundary_faces(sp, 0, 1), boundary_faces(sp, 1, 0)))
+    rng = np.random.default_rng(6)
+    x = 0.05 * rng.standard_normal(sp.nfree)
+    dx = rng.standard_normal(sp.nfree)
+    r, e, y = S.face_terms(sp, faces, form, params, x, None, dx=dx)
+    A = S.face_matrix(sp, faces, form, params, x, None)
+    h = 1e-5
+    rp = S.face_terms(sp, faces, form, params, x + h * dx, None)[0]
+    rm = S.face_terms(sp, faces, form, params, x - h * dx, None)[0]
+    fd = (rp - rm) / (2 * h)
+    assert np.linalg.norm(fd - A @ dx) < 1e-7 * np.linalg.norm(fd)
+    assert np.linalg.norm(y - A @ dx) < 1e-13 * np.linalg.norm(y)
+    if form is S.NonlinearBoundary:  # conservative: r = dE/dx, a symmetric Jacobian
+        for k in rng.choice(sp.nfree, 8, replace=False):
+            xp, xm = x.copy(), x.copy()
+            xp[k] += h
+            xm[k] -= h
+            d = (S.face_terms(sp, faces, form, params, xp, None)[1] - S.face_terms(sp, faces, form, params, xm, None)[1])
+            assert d / (2 * h) == pytest.approx(r[k], rel=1e-6, abs=1e-8 * np.abs(r).max())
+        assert np.abs(A - A.T).max() < 1e-12 * np.abs(A).max()
