@@ -97,6 +97,39 @@ in residual!, Jv, the assembled Jacobian, the node blocks of BlockJacobi and the
 set_robin!(op::DeviceOperator, face_cell::Vector{Int32}, face_local::Vector{Int32}, k::Vector{Float64}) =
     check(ccall((:hfk_op_set_robin, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}), op.ptr,
                 length(face_cell), face_cell, face_local, k), op.ctx.ptr)
+"""
+    set_skeleton!(op, plus_cell, plus_face, minus_cell, minus_face, orient; source=nothing, name="", params=[1.0])
+
+Skeleton integrals on the interior facets of a `SkeletonTriangulation` (src/skeleton.jl): per facet the 0-based plus
+and minus cells (the `SkeletonPair` glue), their local faces 0..5 and the orientation code 0..7 of the minus face's
+coordinates relative to the plus face's (the permutation the glue's minus map carries).  `source === nothing`: the
+built-in gradient-jump penalty `(γ h/2)|∇u⁺ − ∇u⁻|²`, γ = params[1]; otherwise the HIP source of a facet point form
+(struct `name` with static `residual`, `jvp`, `energy` of the 24 facet slots; the captured integrand emitted as HIP).
+The sparsity pattern gains the cross blocks; rebuild the matrix after this call.
+"""
+set_skeleton!(op::DeviceOperator, plus_cell::Vector{Int32}, plus_face::Vector{Int32}, minus_cell::Vector{Int32},
+              minus_face::Vector{Int32}, orient::Vector{Int32}; source::Union{Nothing,AbstractString}=nothing,
+              name::AbstractString="", params::Vector{Float64}=[1.0]) =
+    check(ccall((:hfk_op_set_skeleton, libhfk), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Cstring, Cstring,
+                 Ptr{Float64}, Cint), op.ptr, length(plus_cell), plus_cell, plus_face, minus_cell, minus_face, orient,
+                source === nothing ? C_NULL : source, source === nothing ? C_NULL : name, params, length(params)),
+          op.ctx.ptr)
+
+"""
+    set_boundary_form!(op, face_cell, face_local; source=nothing, name="", params=[p])
+
+A boundary group with a captured integrand (src/assembly.jl:539-578) on faces of the cells (as in `set_traction!`):
+`source === nothing` is the built-in follower pressure `p = params[1]` (nominal traction `-p cof(F) N`); otherwise the
+HIP source of a point form of the cell's 12 slots at the face point (struct `name`).
+"""
+set_boundary_form!(op::DeviceOperator, face_cell::Vector{Int32}, face_local::Vector{Int32};
+                   source::Union{Nothing,AbstractString}=nothing, name::AbstractString="", params::Vector{Float64}=[1.0]) =
+    check(ccall((:hfk_op_set_boundary_form, libhfk), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Cstring, Cstring, Ptr{Float64}, Cint), op.ptr,
+                length(face_cell), face_cell, face_local, source === nothing ? C_NULL : source,
+                source === nothing ? C_NULL : name, params, length(params)), op.ctx.ptr)
+
 "The load factor Λ of the traction (a number of the form, as in `en(Λ)` of test/solvers/test_functionals.jl:40-45)."
 set_load_factor!(op::DeviceOperator, Λ::Real) =
     check(ccall((:hfk_op_set_load_factor, libhfk), Cint, (Ptr{Cvoid}, Cdouble), op.ptr, Λ), op.ctx.ptr)
