@@ -4,13 +4,16 @@ Workload (the plain matrix-free configuration): compressible neo-Hookean (lam = 
 64^3 Q2 hexahedra (6,343,293 free dofs), FP64, 27 Gauss points per cell.  State: the smooth random displacement
 (8 Fourier modes per component, amplitude 1e-2, SplitMix64 seed 2); v ~ N(0, 1) (seed 3).  A step is one
 y = J(x) v over all free dofs with the prepared tangents, inputs resident in HBM.  The headline stores each point's
-tangent in the factorised form of the isotropic neo-Hookean tangent (``--storage fac64``: 17 FP64 factors per
-point, G d_ck + b H_Kk H_Mc + l H_Kc H_Mk rebuilt in registers, FP64 arithmetic, equal to the 81-entry product within
-1e-12); the symmetric 45-entry and the reference's 81-entry layouts are timed as variants.
+tangent in the factorised form of the isotropic neo-Hookean tangent on affine cells (``--storage afac64``: 11 FP64
+per point, the metric iJt'iJt and det J per cell -- the per-cell geometry of the reference's byte model --,
+G d_ck + b H_Kk H_Mc + l H_Kc H_Mk rebuilt in registers, FP64 arithmetic, equal to the 81-entry product within
+1e-12); the general factorised (17 per point), symmetric 45-entry and the reference's 81-entry layouts are timed as
+variants.
 
 Prints one JSON line.  ``value`` is Jv GDoF/s (free dofs / Jv time).  ``roofline`` prices the Jv against the
 HBM roofline with the bytes the timed layout must move at least once per product (its own bytes):
-    B_own = Ncell*Nq*NE*s_T + Ncell*ndl*4 + 2*Nfree*8      (NE = 17, 45 or 81 stored entries, s_T = 8 or 4)
+    B_own = Ncell*Nq*NE*s_T + Ncell*G*8 + Ncell*ndl*4 + 2*Nfree*8
+            (NE = 11, 17, 45 or 81 stored entries, s_T = 8 or 4; G = 8 per-cell numbers for the affine storage)
 and carries the reference's stored-T byte model
     B_ref = Ncell*Nq*81*8 + Ncell*10*8 + Ncell*ndl*4 + 2*Nfree*8
 only as a separately labelled effective bandwidth.  ``roofline.traffic`` is the HBM traffic per product that the
@@ -38,7 +41,8 @@ HBM_PEAK_GBS = 8000.0
 JV_TRAFFIC_64 = {"full64": (8 * (324203 * 2 + 27479) + 49557) * 1024,
                  "sym64": (8 * (196835 * 2 + 27476) + 49557) * 1024,
                  "fac64": (8 * (96113 * 2 + 27448) + 49557) * 1024}
-NENT = {"ful": 81, "sym": 45, "fac": 17}  # stored entries per point of each tangent storage
+NENT = {"ful": 81, "sym": 45, "fac": 17, "afa": 11}  # stored entries per point of each tangent storage
+CELL_GEO = {"afa": 8}  # per-cell geometry numbers (FP64) the affine factorised storage reads
 
 
 def jv_bytes(ncells, nq, ndl, nfree, s_t=8, s_v=8, nent=81):
@@ -584,7 +588,7 @@ def solve_summary(res):
         return None if leg is None else {"solve_s": leg["solve_s"], "iterations": leg["iterations"],
                                          "converged": leg["converged"]}
 
-    out = {k: pick(k) for k in ("solve_fp64_pmg", "solve_fp64_hmg", "solve_fp64_fac64_hmg", "solve_mixed_pmg",
+    out = {k: pick(k) for k in ("solve_fp64_pmg", "solve_fp64_hmg", "solve_fp64_fac_hmg", "solve_mixed_pmg",
                                 "solve_mixed_fac_hmg", "solve_mixed_learned") if k in res}
     if "solve_learned_speedup_vs_fp64" in res:
         out["learned_speedup_vs_best_fp64"] = res["solve_learned_speedup_vs_fp64"]
@@ -680,9 +684,9 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
     train, keep = [], []
     for sd in train_seeds:
         p = lognormal_problem(n, sd)
-        mf = MatrixFreeJacobian(p["fop"], "fac64").prepare(p["x"])
+        mf = MatrixFreeJacobian(p["fop"], "afac64").prepare(p["x"])
         pmg = PMultigrid([p["fop"], p["cop"]], Chebyshev(degree=4), coarse_rtol=0.1, coarse_maxiter=400,
-                         smoother_storage="fac32").update()
+                         smoother_storage="afac32").update()
         if not pmg.ok:
             continue
         gen = torch.Generator(device=dev).manual_seed(sd)
@@ -719,8 +723,8 @@ def learned_fields_leg(n=32, train_seeds=range(1000, 1032), test_seeds=range(100
     del train, keep
     rows = []
     # (leg, outer storage, smoother storage, coarse, network)
-    legs = (("fp64", "fac64", None, "hmg", False), ("mixed", "fac64", "fac32", "hmg", False),
-            ("learned", "fac64", "fac32", 0.1, True))
+    legs = (("fp64", "afac64", None, "hmg", False), ("mixed", "afac64", "afac32", "hmg", False),
+            ("learned", "afac64", "afac32", 0.1, True))
     from hfk.mesh import dirichlet_closed_x_faces
 
     for sd in test_seeds:
@@ -977,8 +981,10 @@ def main():
     ap.add_argument("--order", type=int, default=2, choices=[1, 2, 3],
                     help="element order of the Jv workload (2: the headline; 3 on 48^3: the Q3 parity workload)")
     ap.add_argument("--no-assembly", action="store_true", help="skip the device-timed quick-start assembly leg")
-    ap.add_argument("--storage", default="fac64", choices=["full64", "sym64", "full32", "sym32", "fac64", "fac32"])
-    ap.add_argument("--variants", default="sym64,full64,fac32,sym32", help="extra storages timed beside the headline")
+    ap.add_argument("--storage", default="afac64",
+                    choices=["full64", "sym64", "full32", "sym32", "fac64", "fac32", "afac64", "afac32"])
+    ap.add_argument("--variants", default="fac64,sym64,full64,afac32,sym32",
+                    help="extra storages timed beside the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
@@ -1048,7 +1054,7 @@ def main():
     ncells, nq, ndl = sp.ncells, (order + 1) ** 3, sp.nd
     B = jv_bytes(ncells, nq, ndl, sp.nfree)
     own_B = jv_bytes(ncells, nq, ndl, sp.nfree, s_t=4 if args.storage.endswith("32") else 8,
-                     nent=NENT[args.storage[:3]]) - ncells * 10 * 8
+                     nent=NENT[args.storage[:3]]) - ncells * 10 * 8 + ncells * 8 * CELL_GEO.get(args.storage[:3], 0)
     info = op.info()
     achieved = own_B / t_jv / 1e9
     res = {
@@ -1067,7 +1073,9 @@ def main():
         "config": {"workload": f"Jv, compressible neo-Hookean lam=2 mu=1, unit cube {n}^3 Q{order} hex, {nq} "
                                f"points/cell, FP64, tangent storage {args.storage}"
                                + {"sym64": " (the 45 independent entries of the symmetric tangent)",
-                                  "fac64": " (the 17 factors of the isotropic neo-Hookean tangent)"}.get(args.storage, ""),
+                                  "fac64": " (the 17 factors of the isotropic neo-Hookean tangent)",
+                                  "afac64": " (the isotropic neo-Hookean tangent factorised on affine cells: 11 FP64 per "
+                                            "point, the metric and det J per cell)"}.get(args.storage, ""),
                    "free_dofs": sp.nfree, "cells": ncells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -1099,7 +1107,7 @@ def main():
         tv = float(perv.mean())
         nent = NENT[st[:3]]
         s_t = 4 if st.endswith("32") else 8
-        own = jv_bytes(ncells, nq, ndl, sp.nfree, s_t=s_t, nent=nent) - ncells * 10 * 8
+        own = jv_bytes(ncells, nq, ndl, sp.nfree, s_t=s_t, nent=nent) - ncells * 10 * 8 + ncells * 8 * CELL_GEO.get(st[:3], 0)
         variants[st + ("" if det else "_atomic")] = {
             "gdofs": sp.nfree / tv / 1e9, "ms": tv * 1e3, "prepare_ms": tpv * 1e3, "own_bytes": own,
             "own_GBs": own / tv / 1e9, "own_frac": own / tv / 1e9 / HBM_PEAK_GBS,
@@ -1130,16 +1138,16 @@ def main():
         if "fp64" in legs:
             res["solve_fp64_pmg"] = solve_leg(n, "full64", None, coarse_rtol=0.3)
             res["solve_fp64_hmg"] = solve_leg(n, "full64", None, coarse="hmg")
-            res["solve_fp64_fac64_hmg"] = solve_leg(n, "fac64", None, coarse="hmg")
+            res["solve_fp64_fac_hmg"] = solve_leg(n, "afac64", None, coarse="hmg")
         if "mixed" in legs:
             res["solve_mixed_pmg"] = solve_leg(n, "sym64", "sym32", coarse_rtol=0.3)
-            res["solve_mixed_fac_hmg"] = solve_leg(n, "fac64", "fac32", coarse="hmg")
+            res["solve_mixed_fac_hmg"] = solve_leg(n, "afac64", "afac32", coarse="hmg")
         if "learned" in legs:
-            res["solve_mixed_learned"] = solve_leg(n, "fac64", "fac32", coarse_rtol=0.1,
+            res["solve_mixed_learned"] = solve_leg(n, "afac64", "afac32", coarse_rtol=0.1,
                                                    learned={"samples": 160, "steps": 500, "smoother_degree": 3,
                                                             "scaled": False})
         if "fp64" in legs and "learned" in legs:
-            best = min(("solve_fp64_pmg", "solve_fp64_hmg", "solve_fp64_fac64_hmg"), key=lambda k: res[k]["solve_s"])
+            best = min(("solve_fp64_pmg", "solve_fp64_hmg", "solve_fp64_fac_hmg"), key=lambda k: res[k]["solve_s"])
             res["solve_learned_speedup_vs_fp64"] = res[best]["solve_s"] / res["solve_mixed_learned"]["solve_s"]
             res["solve_learned_speedup_vs_fp64_against"] = best
         if "fields" in legs:
@@ -1159,8 +1167,8 @@ def main():
             res["q3_48_jv"] = precision_leg(48, 3, stream)
             res["q3_48_solve_fp64_pmg"] = solve_leg(48, "full64", None, coarse_rtol=0.3, order=3)
             res["q3_48_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3)
-            res["q3_48_solve_fp64_fac64_hmg"] = solve_leg(48, "fac64", None, order=3, coarse="hmg")
-            res["q3_48_solve_mixed_fac_hmg"] = solve_leg(48, "fac64", "fac32", order=3, coarse="hmg")
+            res["q3_48_solve_fp64_fac_hmg"] = solve_leg(48, "afac64", None, order=3, coarse="hmg")
+            res["q3_48_solve_mixed_fac_hmg"] = solve_leg(48, "afac64", "afac32", order=3, coarse="hmg")
             # with the lognormal mu the state of amplitude 1e-2 makes the tangent indefinite where mu is small
             # (ln J > 2 mu / lam): the CG stops on negative curvature, as the reference's cg with linesearch.  The
             # lognormal legs linearise at the same smooth state scaled to amplitude 3e-3 (positive definite).
@@ -1169,9 +1177,9 @@ def main():
                                                               state_amp=3e-3)
             res["q3_48_lognormal_solve_mixed_pmg"] = solve_leg(48, "sym64", "sym32", coarse_rtol=0.3, order=3, mu=lmu,
                                                                state_amp=3e-3)
-            res["q3_48_lognormal_solve_fp64_fac64_hmg"] = solve_leg(48, "fac64", None, order=3, mu=lmu,
+            res["q3_48_lognormal_solve_fp64_fac_hmg"] = solve_leg(48, "afac64", None, order=3, mu=lmu,
                                                                     state_amp=3e-3, coarse="hmg")
-            res["q3_48_lognormal_solve_mixed_fac_hmg"] = solve_leg(48, "fac64", "fac32", order=3, mu=lmu,
+            res["q3_48_lognormal_solve_mixed_fac_hmg"] = solve_leg(48, "afac64", "afac32", order=3, mu=lmu,
                                                                    state_amp=3e-3, coarse="hmg")
     if not args.no_law:
         print("bench: 64^3 Q2 neo-Hookean with the law compiled at run time (hiprtc) beside the built-in kernels",

@@ -70,10 +70,7 @@ int op_bind(hfk_op* op) {
 // prepare!: tangents at the state x_dev; a device copy of the state is kept (the reference's mf.x,
 // src/matrixfree.jl:140-143), read by the coarse levels of p-multigrid
 int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) {
-  if (storage < 0 || storage > 5) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
-  if (storage >= 4 && !op->rtc && op->law == 0)
-    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the factorised tangent storage needs the neo-Hookean or the "
-                                      "linear law (St Venant-Kirchhoff has no such form)");
+  if (storage < 0 || storage > 7) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
   const size_t bytes = (size_t)op->nbatches * store_entries(storage) * op->C * op->np * store_bytes(storage);
   HIPCHK(op->ctx, op->tangents.alloc(bytes));
   HIPCHK(op->ctx, op->state.alloc((size_t)op->nfree * sizeof(double)));
@@ -86,6 +83,16 @@ int op_run_prepare(hfk_op* op, const double* x_dev, int storage, hipStream_t s) 
 }
 
 int op_prepare_into(hfk_op* op, int storage, void* T, const double* x_dev, hipStream_t s) {
+  if (storage < 0 || storage > 7) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: unknown tangent storage");
+  if (store_factorised(storage) && !op->rtc && op->law == 0)
+    return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the factorised tangent storage needs the neo-Hookean or the "
+                                      "linear law (St Venant-Kirchhoff has no such form)");
+  if (store_affine(storage)) {
+    if (!op->affine)
+      return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: the affine factorised storage needs parallelepiped cells "
+                                        "(an affine map per cell); use the factorised storage");
+    HIPCHK(op->ctx, op->cellgeo.alloc((size_t)op->nslots() * 8 * sizeof(double)));
+  }
   if (op->rtc) return rtc_prepare(op, storage, T, x_dev, s);
   HIPCHK(op->ctx, launch_prepare(op->dev(), storage, T, x_dev, BatchRange{0, (int)op->nbatches}, s));
   return HFK_OK;
@@ -678,6 +685,22 @@ int hfk_op_create(hfk_ctx* ctx, int Q, int P, int64_t ncells, const int32_t* cel
   op->loc_of.resize(nd);
   for (int c = 0; c < 3; ++c)
     for (int t = 0; t < nn; ++t) op->loc_of[c * nn + t] = ldof[c * nn + nodeperm[t]];
+  // affine cells: the bilinear and trilinear terms of every cell map vanish (x = X0 + A xi)
+  op->affine = true;
+  for (int64_t cell = 0; cell < ncells && op->affine; ++cell) {
+    const double* X = &coords[(size_t)cell * 24];
+    double scale = 0.0;
+    for (int v = 1; v < 8; ++v)
+      for (int i = 0; i < 3; ++i) scale = std::max(scale, std::fabs(X[v * 3 + i] - X[i]));
+    for (int i = 0; i < 3 && op->affine; ++i) {
+      const double x0 = X[i], x1 = X[3 + i], x2 = X[6 + i], x3 = X[9 + i], x4 = X[12 + i], x5 = X[15 + i],
+                   x6 = X[18 + i], x7 = X[21 + i];
+      const double t[4] = {x3 - x2 - x1 + x0, x5 - x4 - x1 + x0, x6 - x4 - x2 + x0,
+                           x7 - x6 - x5 - x3 + x4 + x2 + x1 - x0};
+      for (double v : t)
+        if (std::fabs(v) > 1e-12 * scale) op->affine = false;
+    }
+  }
   HIPCHK(ctx, op->ids.alloc(hids.size() * sizeof(int32_t)));
   HIPCHK(ctx, hipMemcpy(op->ids.p, hids.data(), hids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(ctx, op->coords.alloc(hx.size() * sizeof(double)));

@@ -69,8 +69,13 @@ hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* tang
 
 // number of stored entries and bytes per entry of a storage mode (0 full64, 1 sym64, 2 full32, 3 sym32, 4 fac64,
 // 5 fac32)
-inline int store_entries(int storage) { return storage >= 4 ? 17 : ((storage == 1 || storage == 3) ? 45 : 81); }
-inline int store_bytes(int storage) { return (storage == 2 || storage == 3 || storage == 5) ? 4 : 8; }
-inline bool store_fp32(int storage) { return storage == 2 || storage == 3 || storage == 5; }
+inline int store_entries(int storage) {
+  return storage >= 6 ? 11 : (storage >= 4 ? 17 : ((storage == 1 || storage == 3) ? 45 : 81));
+}
+inline int store_bytes(int storage) { return (storage == 2 || storage == 3 || storage == 5 || storage == 7) ? 4 : 8; }
+inline bool store_fp32(int storage) { return storage == 2 || storage == 3 || storage == 5 || storage == 7; }
+// the factorised storages (an isotropic law with a factorised tangent); the affine ones need parallelepiped cells
+inline bool store_factorised(int storage) { return storage >= 4 && storage <= 7; }
+inline bool store_affine(int storage) { return storage == 6 || storage == 7; }
 
 }  // namespace hfk

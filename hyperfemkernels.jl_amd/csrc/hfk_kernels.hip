@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
   // FP32 tangents (the smoothers' storages): the sweeps and the point product run in FP32 too, which halves the
   // LDS and register footprint of the work-group; y accumulates in FP64
   using CT = typename std::conditional<std::is_same<TS, float>::value, float, double>::type;
-  constexpr bool SYM = LAY == kLaySym, FAC = LAY == kLayFac;
+  constexpr bool SYM = LAY == kLaySym, FAC = LAY == kLayFac || LAY == kLayAfac, AFAC = LAY == kLayAfac;
   constexpr int C = Dm::C, SA = Dm::SA, SB = Dm::SB, NSLOT = Dm::NSLOT, NE = lay_entries(LAY);
   __shared__ CT lds[Dm::LDS];
   CT* A = lds;
@@ -95,8 +95,22 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
     }
     if constexpr (FAC) {
       // sv[K+3c] = sum_M G_KM su[M+3c] + b sum_k H_Kk X_ck + l H_Kc tr(X),  X_ck = sum_M H_Mc su[M+3k]
-      const CT* G = dpre;
-      const CT* H = dpre + 6;
+      CT G[6], bm, bl;
+      const CT* H;
+      if constexpr (AFAC) {  // G = g iJt'iJt, l = lam w_q det J from the cell's geometry
+        const double* cg = op.cellgeo + ((size_t)batch * C + cell) * 8;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) G[e] = dpre[10] * static_cast<CT>(cg[e]);
+        H = dpre;
+        bm = dpre[9];
+        bl = static_cast<CT>(op.lam * point_weight<P>(op.tab, q) * cg[6]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) G[e] = dpre[e];
+        H = dpre + 6;
+        bm = dpre[15];
+        bl = dpre[16];
+      }
       CT X[9];
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -108,7 +122,6 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
           X[c + 3 * k] = v;
         }
       const CT tr = X[0] + X[4] + X[8];
-      const CT bm = dpre[15], bl = dpre[16];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -188,6 +201,9 @@ __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __res
       const int c2 = it / 81, e = it - c2 * 81, ii = e % 9, jj = e / 9;
       if (LAY == kLayFac) {
         Dl[it] = fac_entry(Dg + c2 * NP + q, NSLOT, ii, jj);
+      } else if (LAY == kLayAfac) {
+        Dl[it] = afac_entry(Dg + c2 * NP + q, NSLOT, op.cellgeo + ((size_t)batch * C + c2) * 8,
+                            point_weight<P>(op.tab, q), op.lam, ii, jj);
       } else {
         const int se = SYM ? (ii <= jj ? sym_index(ii, jj) : sym_index(jj, ii)) : e;
         Dl[it] = static_cast<double>(ld_stream(Dg + (size_t)se * NSLOT + c2 * NP + q));
@@ -495,7 +511,7 @@ __global__ __launch_bounds__(kThreads) void k_dot_partials(const double* __restr
 
 // Expand the stored tangents of every batch to D[(batch*C + cell)*np + q][81] (FP64), for inspection.
 template <int P, class TS, int LAY>
-__global__ void k_expand(const void* __restrict__ tangents, double* __restrict__ out, int64_t nbatches) {
+__global__ void k_expand(OpDev op, const void* __restrict__ tangents, double* __restrict__ out, int64_t nbatches) {
   constexpr bool SYM = LAY == kLaySym;
   constexpr int NP = P * P * P, C = kThreads / NP, NSLOT = C * NP, NE = lay_entries(LAY);
   const int64_t n = nbatches * NSLOT;
@@ -507,6 +523,11 @@ __global__ void k_expand(const void* __restrict__ tangents, double* __restrict__
       for (int i = 0; i < 9; ++i) {
         if (LAY == kLayFac) {
           out[g * 81 + i + 9 * j] = fac_entry(Dp, NSLOT, i, j);
+          continue;
+        }
+        if (LAY == kLayAfac) {
+          out[g * 81 + i + 9 * j] = afac_entry(Dp, NSLOT, op.cellgeo + (batch * C + slot / NP) * 8,
+                                               point_weight<P>(op.tab, slot % NP), op.lam, i, j);
           continue;
         }
         const int e = SYM ? (i <= j ? sym_index(i, j) : sym_index(j, i)) : i + 9 * j;
@@ -530,7 +551,9 @@ hipError_t prepare_qp(const OpDev& op, int storage, void* T, const double* x, Ba
         return hipErrorInvalidValue;  // no factorised form (refused on the host before)
       } else {
         if (storage == 4) hipLaunchKernelGGL((k_prepare<Q, P, Law, double, kLayFac>), g, b, 0, s, op, T, x, r.first);
-        else hipLaunchKernelGGL((k_prepare<Q, P, Law, float, kLayFac>), g, b, 0, s, op, T, x, r.first);
+        else if (storage == 5) hipLaunchKernelGGL((k_prepare<Q, P, Law, float, kLayFac>), g, b, 0, s, op, T, x, r.first);
+        else if (storage == 6) hipLaunchKernelGGL((k_prepare<Q, P, Law, double, kLayAfac>), g, b, 0, s, op, T, x, r.first);
+        else hipLaunchKernelGGL((k_prepare<Q, P, Law, float, kLayAfac>), g, b, 0, s, op, T, x, r.first);
       }
       break;
   }
@@ -558,7 +581,9 @@ hipError_t apply_qpa(const OpDev& op, int storage, const void* T, const double* 
     case 2: hipLaunchKernelGGL((k_apply<Q, P, float, kLayFull, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
     case 3: hipLaunchKernelGGL((k_apply<Q, P, float, kLaySym, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
     case 4: hipLaunchKernelGGL((k_apply<Q, P, double, kLayFac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
-    default: hipLaunchKernelGGL((k_apply<Q, P, float, kLayFac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 5: hipLaunchKernelGGL((k_apply<Q, P, float, kLayFac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    case 6: hipLaunchKernelGGL((k_apply<Q, P, double, kLayAfac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
+    default: hipLaunchKernelGGL((k_apply<Q, P, float, kLayAfac, ATOMIC>), g, b, 0, s, op, T, dx, y, r.first, alpha, fc, nc, gx); break;
   }
   return hipGetLastError();
 }
@@ -580,7 +605,9 @@ hipError_t blocks_qp(const OpDev& op, int storage, const void* T, const int32_t*
     case 2: hipLaunchKernelGGL((k_blocks<Q, P, float, kLayFull>), g, b, 0, s, op, T, block_of, B, r.first); break;
     case 3: hipLaunchKernelGGL((k_blocks<Q, P, float, kLaySym>), g, b, 0, s, op, T, block_of, B, r.first); break;
     case 4: hipLaunchKernelGGL((k_blocks<Q, P, double, kLayFac>), g, b, 0, s, op, T, block_of, B, r.first); break;
-    default: hipLaunchKernelGGL((k_blocks<Q, P, float, kLayFac>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 5: hipLaunchKernelGGL((k_blocks<Q, P, float, kLayFac>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    case 6: hipLaunchKernelGGL((k_blocks<Q, P, double, kLayAfac>), g, b, 0, s, op, T, block_of, B, r.first); break;
+    default: hipLaunchKernelGGL((k_blocks<Q, P, float, kLayAfac>), g, b, 0, s, op, T, block_of, B, r.first); break;
   }
   return hipGetLastError();
 }
@@ -759,12 +786,14 @@ hipError_t launch_expand_tangents(const OpDev& op, int storage, const void* T, d
   const dim3 g(1024), b(256);
 #define EXP(P_)                                                                                           \
   switch (storage) {                                                                                      \
-    case 0: hipLaunchKernelGGL((k_expand<P_, double, kLayFull>), g, b, 0, s, T, D, nbatches); break;       \
-    case 1: hipLaunchKernelGGL((k_expand<P_, double, kLaySym>), g, b, 0, s, T, D, nbatches); break;        \
-    case 2: hipLaunchKernelGGL((k_expand<P_, float, kLayFull>), g, b, 0, s, T, D, nbatches); break;        \
-    case 3: hipLaunchKernelGGL((k_expand<P_, float, kLaySym>), g, b, 0, s, T, D, nbatches); break;         \
-    case 4: hipLaunchKernelGGL((k_expand<P_, double, kLayFac>), g, b, 0, s, T, D, nbatches); break;        \
-    default: hipLaunchKernelGGL((k_expand<P_, float, kLayFac>), g, b, 0, s, T, D, nbatches); break;        \
+    case 0: hipLaunchKernelGGL((k_expand<P_, double, kLayFull>), g, b, 0, s, op, T, D, nbatches); break;   \
+    case 1: hipLaunchKernelGGL((k_expand<P_, double, kLaySym>), g, b, 0, s, op, T, D, nbatches); break;    \
+    case 2: hipLaunchKernelGGL((k_expand<P_, float, kLayFull>), g, b, 0, s, op, T, D, nbatches); break;    \
+    case 3: hipLaunchKernelGGL((k_expand<P_, float, kLaySym>), g, b, 0, s, op, T, D, nbatches); break;     \
+    case 4: hipLaunchKernelGGL((k_expand<P_, double, kLayFac>), g, b, 0, s, op, T, D, nbatches); break;    \
+    case 5: hipLaunchKernelGGL((k_expand<P_, float, kLayFac>), g, b, 0, s, op, T, D, nbatches); break;     \
+    case 6: hipLaunchKernelGGL((k_expand<P_, double, kLayAfac>), g, b, 0, s, op, T, D, nbatches); break;   \
+    default: hipLaunchKernelGGL((k_expand<P_, float, kLayAfac>), g, b, 0, s, op, T, D, nbatches); break;   \
   }
   if (op.P == 2) {
     EXP(2)

@@ -115,6 +115,9 @@ struct hfk_op {
   std::vector<int64_t> slot_of_cell;            // cell -> batch*C + position
   std::vector<int32_t> loc_of;                  // tensor index c*nn + t -> table-local dof
   hfk::DevBuf ids, coords, mu_pts, dvals, tangents, loc_of_d, nzpos_d, x_stage, y_stage, D64, state;
+  // every cell map affine (parallelepipeds), and the per-cell geometry of the affine factorised storage
+  bool affine = false;
+  hfk::DevBuf cellgeo;
   hfk::DevBuf part;  // per-batch partial results of the functionals
   // boundary traction (Neumann load): its load vector over the free then the Dirichlet ids, scaled by load_factor in
   // residual! and the energy
@@ -163,6 +166,7 @@ struct hfk_op {
     d.dvals = dvals.as<double>();
     d.loc_of = loc_of_d.as<int32_t>();
     d.nzpos = nzpos_d.as<int32_t>();
+    d.cellgeo = cellgeo.as<double>();
     return d;
   }
   hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : ctx->stream; }

@@ -1552,7 +1552,7 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
   if (coarse_kind == 1 && levels[nlevels - 1] && levels[nlevels - 1]->nfree > kMaxDirect)
     return set_err(ctx, HFK_EDIM, "HyperFEMKernels: PMultigrid: the direct coarse solve takes at most " +
                                       std::to_string(kMaxDirect) + " coarse dofs (add coarser levels)");
-  if (smoother_storage < -1 || smoother_storage > 5)
+  if (smoother_storage < -1 || smoother_storage > 7)
     return set_err(ctx, HFK_EARG, "HyperFEMKernels: PMultigrid: unknown smoother storage");
   std::unique_ptr<hfk_pmg> g(new hfk_pmg());
   g->ctx = ctx;

@@ -411,8 +411,31 @@ __host__ __device__ constexpr int sym_index(int i, int j) { return j * (j + 1) /
 //               D[(K,c),(M,k)] = G_KM d_ck + b H_Kk H_Mc + l H_Kc H_Mk,
 //             stored as G = dV mu g (entries 0..5, packed upper triangle sym_index(K, M)), H (6..14),
 //             b = dV m (15), l = dV lam (16).
-constexpr int kLayFull = 0, kLaySym = 1, kLayFac = 2;
-__host__ __device__ constexpr int lay_entries(int lay) { return lay == kLayFull ? 81 : (lay == kLaySym ? 45 : 17); }
+//   kLayAfac  the same tangent on affine cells (parallelepipeds: iJt and det J constant per cell, dV = w_q det J),
+//             11 entries per point: H (0..8), b = dV m (9), g = dV mu (10); per cell (OpDev::cellgeo) the metric
+//             iJt' iJt (packed, 0..5) and det J (6), so that G = g iJt' iJt and l = lam w_q det J.
+constexpr int kLayFull = 0, kLaySym = 1, kLayFac = 2, kLayAfac = 3;
+__host__ __device__ constexpr int lay_entries(int lay) {
+  return lay == kLayFull ? 81 : (lay == kLaySym ? 45 : (lay == kLayFac ? 17 : 11));
+}
+
+// the quadrature weight of tensor point q = a + P b + P^2 c
+template <int P>
+__device__ __forceinline__ double point_weight(const Tab& tb, int q) {
+  return tb.wq[q % P] * tb.wq[(q / P) % P] * tb.wq[q / (P * P)];
+}
+
+// entry D[i + 9j] of a point stored in the affine factorised layout (f[e * stride] is entry e, cg the cell's
+// geometry, w the point's weight)
+template <class TS>
+__device__ __forceinline__ double afac_entry(const TS* f, size_t stride, const double* cg, double w, double lam, int i,
+                                             int j) {
+  const int K = i % 3, c = i / 3, M = j % 3, k = j / 3;
+  const double hKk = (double)f[(K + 3 * k) * stride], hMc = (double)f[(M + 3 * c) * stride];
+  const double hKc = (double)f[(K + 3 * c) * stride], hMk = (double)f[(M + 3 * k) * stride];
+  const double g = c == k ? (double)f[10 * stride] * cg[K <= M ? sym_index(K, M) : sym_index(M, K)] : 0.0;
+  return g + (double)f[9 * stride] * hKk * hMc + lam * w * cg[6] * hKc * hMk;
+}
 
 // entry D[i + 9j] of a point stored in the factorised layout; f[e * stride] is factor e
 template <class TS>
@@ -474,6 +497,35 @@ __global__ __launch_bounds__(kThreads) void k_prepare(OpDev op, void* __restrict
     Law law;
     law.setup(F, op.lam, mu);
     const size_t base = (size_t)batch * NE * NSLOT + tid;
+    if constexpr (LAY == kLayAfac) {  // per point H, b, g; per cell the metric and det J (from its first point)
+      double Af[9], m;
+      law.factor(Af, m);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int K = 0; K < 3; ++K) {
+          double h = 0;
+#pragma unroll
+          for (int J = 0; J < 3; ++J) h += iJt[J + 3 * K] * Af[J + 3 * c];
+          store_entry<TS, true>(tangents, base + (size_t)(K + 3 * c) * NSLOT, h);
+        }
+      store_entry<TS, true>(tangents, base + (size_t)9 * NSLOT, dV * m);
+      store_entry<TS, true>(tangents, base + (size_t)10 * NSLOT, dV * mu);
+      if (q == 0) {
+        double* cg = op.cellgeo + ((size_t)batch * C + cell) * 8;
+#pragma unroll
+        for (int M = 0; M < 3; ++M)
+#pragma unroll
+          for (int K = 0; K <= M; ++K) {
+            double g = 0;
+#pragma unroll
+            for (int J = 0; J < 3; ++J) g += iJt[J + 3 * K] * iJt[J + 3 * M];
+            cg[sym_index(K, M)] = g;
+          }
+        cg[6] = dV / point_weight<P>(op.tab, 0);
+      }
+      return;
+    }
     if constexpr (LAY == kLayFac) {
       double Af[9], m;
       law.factor(Af, m);

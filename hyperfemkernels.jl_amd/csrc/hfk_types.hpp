@@ -29,6 +29,7 @@ struct OpDev {
   const double* dvals;    // Dirichlet values
   const int32_t* loc_of;  // tensor index c*nn + t -> table-local dof (assembly)
   const int32_t* nzpos;   // [cell (batch order)][ii + nd*jj] table-local, 1-based, or null
+  double* cellgeo;        // [slot][8] per-cell geometry of the affine factorised storage (iJt'iJt packed, det J)
 };
 
 }  // namespace hfk

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libhfk.so")
 HFK_OK = 0
 STATUS = {-1: "ArgumentError", -2: "DimensionMismatch", -3: "ErrorException", -4: "HIP error", -5: "allocation failure"}
 
-STORE = {"full64": 0, "sym64": 1, "full32": 2, "sym32": 3, "fac64": 4, "fac32": 5}
+STORE = {"full64": 0, "sym64": 1, "full32": 2, "sym32": 3, "fac64": 4, "fac32": 5, "afac64": 6, "afac32": 7}
 LAWS = {"svk": 0, "neohookean": 1, "linear": 2}
 
 _i32p = C.POINTER(C.c_int32)

@@ -50,6 +50,10 @@ extern "C" {
                                  the product rebuilds D su in registers (no counterpart in the reference, which stores
                                  the 81 entries) */
 #define HFK_STORE_FAC32 5     /* the same 17 factors in FP32 (mixed precision) */
+#define HFK_STORE_AFAC64 6    /* the factorised tangent on affine cells (parallelepipeds; refused otherwise): 11 FP64
+                                 per point (iJt' F^-1, dV m, dV mu) and 8 per cell (the metric iJt' iJt and det J, the
+                                 per-cell geometry of the reference's Jv byte model) */
+#define HFK_STORE_AFAC32 7    /* the same 11 per point in FP32 */
 
 typedef struct hfk_ctx hfk_ctx;
 typedef struct hfk_op hfk_op;

@@ -120,6 +120,48 @@ def test_factorised_storage(oracle, hip_ctx, order, P, law, n, perturb):
     assert op.info()["tangent_bytes"] >= sp.ncells * P ** 3 * 17 * 4
 
 
+@pytest.mark.parametrize("order,P,law,n", [(2, 3, "neohookean", (4, 3, 3)), (2, 3, "linear", (3, 3, 2)),
+                                           (1, 2, "neohookean", (4, 3, 3)), (3, 4, "neohookean", (2, 2, 2)),
+                                           (1, 3, "neohookean", (3, 2, 2))])
+def test_affine_factorised_storage(oracle, hip_ctx, order, P, law, n):
+    """The affine factorised storage (11 numbers per point, the metric and det J per cell) on parallelepiped cells --
+    the box mesh under a global affine map, so that iJt is a full matrix: the 81 entries it represents equal the
+    oracle's tangents (1e-12), also with a shear-modulus field; Jv equals the oracle's (afac64 1e-12, afac32 1e-5) in
+    both product modes; BlockJacobi's blocks equal those of the full storage.  Non-affine cells are refused."""
+    from hfk.solvers import BlockJacobi
+
+    print(f"affine factorised order={order} P={P} law={law} n={n}", flush=True)
+    sp, dv, x = problem(order, P, law, n, 0.0)
+    A = np.array([[1.1, 0.2, -0.1], [0.05, 0.9, 0.3], [-0.2, 0.1, 1.2]])
+    sp.coords[:] = sp.coords @ A.T + np.array([0.3, -0.1, 0.2])
+    form = HyperelasticForm(law, 2.0, 1.0, degree=2 * (P - 1))
+    op = KernelOperator(form, sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp, law, 2.0, 1.0, points1d=P, dvals=dv)
+    mu = np.exp(0.4 * np.random.default_rng(5).standard_normal(sp.ncells * P ** 3))
+    dx = np.random.default_rng(6).standard_normal(sp.nfree)
+    for field in (False, True):
+        if field:
+            op.set_mu_field(mu)
+            orc.set_mu_field(mu)
+        prep = orc.prepare(x)
+        yref = orc.jv(prep, dx)
+        mf = MatrixFreeJacobian(op, "afac64").prepare(x)
+        assert rel(mf.tangents(), oracle.reference_coordinate_tangents(*prep)) < TOL
+        assert rel(mf @ dx, yref) < TOL
+        op.set_deterministic(False)
+        assert rel(mf @ dx, yref) < TOL
+        op.set_deterministic(True)
+        bfac = BlockJacobi(mf).blocks()[1]
+        bfull = BlockJacobi(MatrixFreeJacobian(op, "full64").prepare(x)).blocks()[1]
+        assert rel(bfac, bfull) < TOL
+        assert rel(MatrixFreeJacobian(op, "afac32").prepare(x) @ dx, yref) < 1e-5
+    assert op.info()["tangent_bytes"] >= sp.ncells * P ** 3 * 11 * 4
+    spp, dvp, xp = problem(order, P, law, n, 0.1)  # trilinear (non-affine) cells
+    opp = KernelOperator(form, spp, jacobian="matrixfree", dirichlet_values=dvp, ctx=hip_ctx)
+    with pytest.raises(hfk._lib.HFKError, match="ArgumentError"):
+        MatrixFreeJacobian(opp, "afac64").prepare(xp)
+
+
 def test_factorised_storage_refused(hip_ctx):
     from hfk.laws import SOURCES, CompiledLaw
 
@@ -127,6 +169,8 @@ def test_factorised_storage_refused(hip_ctx):
     op = KernelOperator(HyperelasticForm("svk"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
     with pytest.raises(hfk._lib.HFKError, match="ArgumentError"):
         MatrixFreeJacobian(op, "fac64").prepare(x)
+    with pytest.raises(hfk._lib.HFKError, match="ArgumentError"):
+        MatrixFreeJacobian(op, "afac64").prepare(x)
     op2 = KernelOperator(HyperelasticForm("neohookean"), sp, jacobian="matrixfree", dirichlet_values=dv, ctx=hip_ctx)
     src, name = SOURCES["neohookean"]
     op2.set_law(CompiledLaw(src, name, ctx=hip_ctx))
