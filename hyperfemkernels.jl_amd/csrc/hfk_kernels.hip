@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void k_dot_partials(const double* __restr
 template <int P, class TS, int LAY>
 __global__ void k_expand(OpDev op, const void* __restrict__ tangents, double* __restrict__ out, int64_t nbatches) {
   constexpr bool SYM = LAY == kLaySym;
-  constexpr int NP = P * P * P, C = kThreads / NP, NSLOT = C * NP, NE = lay_entries(LAY);
+  constexpr int NP = P * P * P, C = cells_per_batch(P), NSLOT = C * NP, NE = lay_entries(LAY);
   const int64_t n = nbatches * NSLOT;
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
     const int64_t batch = g / NSLOT;

@@ -12,7 +12,7 @@ template <int Q, int P>
 struct Dims {
   static constexpr int NN = Q * Q * Q;
   static constexpr int NP = P * P * P;
-  static constexpr int C = kThreads / NP;
+  static constexpr int C = cells_per_batch(P);
   static constexpr int NSLOT = C * NP;
   static constexpr int SA = (3 * NN > 9 * P * P * Q) ? 3 * NN : 9 * P * P * Q;
   static constexpr int SB = (6 * P * Q * Q > 9 * NP) ? 6 * P * Q * Q : 9 * NP;

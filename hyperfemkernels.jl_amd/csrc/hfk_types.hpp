@@ -16,7 +16,7 @@ struct Tab {
 
 // Cells per work-group batch: the point phase maps one thread to one (cell, point) pair of the batch.
 constexpr int kThreads = 256;
-constexpr int cells_per_batch(int P) { return kThreads / (P * P * P); }
+constexpr int cells_per_batch(int P) { return P == 3 ? 8 : kThreads / (P * P * P); }
 
 struct OpDev {
   int Q, P, C, nn, np, nd;
