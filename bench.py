@@ -33,14 +33,14 @@ for p in (ROOT, os.path.join(ROOT, "hyperfemkernels.jl_amd")):
 
 METRIC = "Q2 hyperelastic Jv GDoF/s + solve time to 1e-8: learned/FP32-MG vs FP64 p-MG"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per deterministic Jv at 64^3 Q2 (8 colour launches), from the PMC counters FETCH_SIZE x 2.0 and
-# WRITE_SIZE x 1.0 (the factors a 1 GiB copy of known size reads), per dispatch: 324,203 / 27,479 KiB (full64),
-# 196,835 / 27,476 KiB (sym64), 96,111 / 27,451 KiB (fac64)
-# HBM bytes per deterministic Jv at 64^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (KiB per
-# colour dispatch x 8 colours, FETCH calibrated x2 on the 1 GiB copy) plus the fill of y (49,557 KiB written)
+# HBM bytes per deterministic Jv at 64^3 Q2 (8 colour launches) from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+# of this script: KiB per colour dispatch x 8, FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0 (the factors a 1 GiB copy of
+# known size reads), plus the fill of y (49,557 KiB written); per dispatch 324,203 / 27,479 KiB (full64),
+# 196,835 / 27,476 KiB (sym64), 96,113 / 27,448 KiB (fac64), 75,715 / 27,451 KiB (afac64)
 JV_TRAFFIC_64 = {"full64": (8 * (324203 * 2 + 27479) + 49557) * 1024,
                  "sym64": (8 * (196835 * 2 + 27476) + 49557) * 1024,
-                 "fac64": (8 * (96113 * 2 + 27448) + 49557) * 1024}
+                 "fac64": (8 * (96113 * 2 + 27448) + 49557) * 1024,
+                 "afac64": (8 * (75715 * 2 + 27451) + 49557) * 1024}
 NENT = {"ful": 81, "sym": 45, "fac": 17, "afa": 11}  # stored entries per point of each tangent storage
 CELL_GEO = {"afa": 8}  # per-cell geometry numbers (FP64) the affine factorised storage reads
 
@@ -320,6 +320,8 @@ def oseen_leg(n=40, reps=10):
     the faces).  Times the device residual!, jacobian! and the fused pass (state and outputs on the device, HIP events
     on the stream), the step's point data made on the device from the FE functions of the previous steps, and the
     host-side pattern build; the 14-point degree-5 rule in the cells."""
+    import hashlib
+
     import numpy as np
     import torch
 
@@ -422,6 +424,7 @@ def oseen_leg(n=40, reps=10):
             "jacobian_ms": t_jac * 1e3, "residual_and_jacobian_ms": t_both * 1e3,
             "jacobian_Mcells_per_s": ncells / t_jac / 1e6, "jacobian_min_bytes": jac_bytes,
             "jacobian_GBs_of_min_bytes": jac_bytes / t_jac / 1e9,
+            "jacobian_sha256": hashlib.sha256(nz_det.tobytes()).hexdigest()[:16],
             "atomic_single_launch": {"residual_ms": t_res_a * 1e3, "jacobian_ms": t_jac_a * 1e3,
                                      "residual_and_jacobian_ms": t_both_a * 1e3,
                                      "jacobian_GBs_of_min_bytes": jac_bytes / t_jac_a / 1e9,
