@@ -148,6 +148,19 @@ SIGNATURES = {
                                    _i32p, _f64p]),
     "hfk_krylov_solve_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                        _i32p, _f64p, _vp]),
+    "hfk_sys_create": (C.c_int, [_vp, C.c_int64, _i64p, _i64p, C.POINTER(_vp)]),
+    "hfk_sys_destroy": (C.c_int, [_vp]),
+    "hfk_sys_set_blocks": (C.c_int, [_vp, C.c_int64, _i32p]),
+    "hfk_sys_update_dev": (C.c_int, [_vp, _vp, _i64p, _vp]),
+    "hfk_sys_matvec_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_sys_blocks_apply_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_sys_smooth_dev": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp, _vp]),
+    "hfk_sys_blocks": (C.c_int, [_vp, _f64p, _f64p]),
+    "hfk_sys_set_patches": (C.c_int, [_vp, C.c_int64, _i64p, _i32p, C.POINTER(C.c_int8)]),
+    "hfk_sys_patches_apply_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hfk_sys_patch_inverses": (C.c_int, [_vp, _f64p]),
+    "hfk_sys_solve_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp, C.c_double, C.c_double,
+                                    C.c_int, C.c_int, _i32p, _f64p, _vp]),
 }
 
 _lib = None

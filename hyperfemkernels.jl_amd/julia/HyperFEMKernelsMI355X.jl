@@ -506,4 +506,54 @@ function jacobian!(A::SparseMatrixCSC{Float64,Int}, op::DeviceOseen, x::Vector{F
     A
 end
 
+"""
+    DeviceSystem(ctx, A::SparseMatrixCSC)        # the pattern of an assembled matrix (e.g. `sparsity_pattern(op)`)
+
+An assembled square matrix on the device for Krylov solves of a multi-field system: `set_blocks!` with nodal blocks
+(the reference's `field_blocks` per field, src/preconditioners.jl:151-171, or blocks coupling the fields at a node;
+6 × nblocks free ids, 0 = empty slot), `set_patches!` with Vanka patches (restricted additive Schwarz), `update!`
+with new values (extraction and inversion on the device), `solve!` as `MatrixFreeSolver` (src/solvers.jl:111-158).
+"""
+mutable struct DeviceSystem
+    ptr::Ptr{Cvoid}
+    ctx::Context
+    n::Int
+end
+function DeviceSystem(ctx::Context, A::SparseMatrixCSC)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    colptr = Int64.(A.colptr); rowval = Int64.(A.rowval)
+    check(ccall((:hfk_sys_create, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Ptr{Cvoid}}),
+                ctx.ptr, size(A, 2), colptr, rowval, r), ctx.ptr)
+    S = DeviceSystem(r[], ctx, size(A, 2))
+    finalizer(o -> ccall((:hfk_sys_destroy, libhfk), Cint, (Ptr{Cvoid},), o.ptr), S)
+    S
+end
+set_blocks!(S::DeviceSystem, blocks::Matrix{Int32}) =
+    check(ccall((:hfk_sys_set_blocks, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int32}), S.ptr, size(blocks, 2), blocks),
+          S.ctx.ptr)
+"Vanka patches: the dofs of patch p are dofs[ptr[p]+1 : ptr[p+1]] (ptr 0-based offsets), owned[k] ≠ 0 where patch p writes dofs[k]."
+set_patches!(S::DeviceSystem, ptr::Vector{Int64}, dofs::Vector{Int32}, owned::Vector{Int8}) =
+    check(ccall((:hfk_sys_set_patches, libhfk), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int32}, Ptr{Int8}),
+                S.ptr, length(ptr) - 1, ptr, dofs, owned), S.ctx.ptr)
+"New values on the device (`nonzeros` of the CSC matrix, device pointer); returns the number of singular blocks/patches."
+function update!(S::DeviceSystem, nzval_dev::Ptr{Float64}; stream::Ptr{Cvoid}=C_NULL)
+    bad = Ref{Int64}(0)
+    check(ccall((:hfk_sys_update_dev, libhfk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Int64}, Ptr{Cvoid}),
+                S.ptr, nzval_dev, bad, stream), S.ctx.ptr)
+    bad[]
+end
+"""solve!(x_dev, S, b_dev; krylov = :gmres, preconditioner = :blocks)   # or :sweeps, :patches, :none"""
+function solve!(x_dev::Ptr{Float64}, S::DeviceSystem, b_dev::Ptr{Float64}; krylov::Symbol=:gmres,
+                preconditioner::Symbol=:blocks, sweeps::Integer=2, omega::Real=0.8, rtol::Real=1e-8, atol::Real=0.0,
+                maxiter::Integer=1000, restart::Integer=50, stream::Ptr{Cvoid}=C_NULL)
+    kind = krylov === :gmres ? 2 : krylov === :fgmres ? 3 : throw(ArgumentError("krylov must be :gmres or :fgmres"))
+    pc = Dict(:none => 0, :blocks => 1, :sweeps => 2, :patches => 3)[preconditioner]
+    st = zeros(Int32, 3); rn = zeros(2)
+    check(ccall((:hfk_sys_solve_dev, libhfk), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cint, Cint,
+                 Ptr{Int32}, Ptr{Float64}, Ptr{Cvoid}),
+                S.ptr, kind, pc, sweeps, omega, b_dev, x_dev, rtol, atol, maxiter, restart, st, rn, stream), S.ctx.ptr)
+    (converged = st[1] == 1, iterations = Int(st[2]), residual = rn[1], rhs_norm = rn[2])
+end
+
 end # module

@@ -482,6 +482,52 @@ int hfk_oseen_jacobian_dev(hfk_oseen* op, const double* x_dev, double* nzval_dev
 int hfk_oseen_residual_jacobian_dev(hfk_oseen* op, const double* x_dev, double* b_dev, double* nzval_dev,
                                     void* stream);
 
+/* ------------------------------------------------------------------------------------------------------------
+ * Assembled multi-field systems: a square CSC matrix on the device (the values of jacobian!, e.g.
+ * hfk_oseen_jacobian_dev) with block Jacobi over nodal blocks that couple every field's dofs at a mesh node, damped
+ * block-Jacobi sweeps as a smoother, and GMRES / FGMRES.  The reference's BlockJacobi (src/preconditioners.jl:
+ * 136-171 field_blocks, 284-340 invert_blocks!/ldiv!) keeps one set of nodal blocks per field, which leaves the
+ * pressure blocks of a mixed formulation without a pressure diagonal singular; its p-multigrid is single-field
+ * for that reason (src/pmultigrid.jl:252-254).  A block here is any group of at most 6 free dofs given by the
+ * host: the reference's per-field blocks and blocks coupling velocity and pressure at a node run on the same
+ * kernels.  Empty slots act as the identity (as the reference's fixed components).
+ */
+typedef struct hfk_sys hfk_sys;
+/* colptr (n + 1) and rowval (colptr[n] - 1): the CSC pattern, 1-based, rows ascending within a column */
+int hfk_sys_create(hfk_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval, hfk_sys** out);
+int hfk_sys_destroy(hfk_sys* sys);
+/* nblocks x 6 free dofs (1-based, 0 = an empty slot); every dof in exactly one slot (preconditioners.jl:151-171:
+ * the blocks of field_blocks, here over any grouping) */
+int hfk_sys_set_blocks(hfk_sys* sys, int64_t nblocks, const int32_t* block_dofs);
+/* new values (CSC order, device): the CSR copy for products, then the blocks extracted and inverted
+ * (block_diagonal! + invert_blocks!, preconditioners.jl:186-218, 284-304); nsingular = blocks whose inverse is not
+ * finite (the reference's `ok` cleared, or its strict error) */
+int hfk_sys_update_dev(hfk_sys* sys, const double* nzval_dev, int64_t* nsingular, void* stream);
+/* y = A x */
+int hfk_sys_matvec_dev(hfk_sys* sys, const double* x_dev, double* y_dev, void* stream);
+/* z = Binv r block by block (ldiv!, preconditioners.jl:310-340) */
+int hfk_sys_blocks_apply_dev(hfk_sys* sys, const double* r_dev, double* z_dev, void* stream);
+/* `sweeps` damped block-Jacobi steps z <- z + omega Binv (r - A z) from z = 0 (0 < omega < 2) */
+int hfk_sys_smooth_dev(hfk_sys* sys, int sweeps, double omega, const double* r_dev, double* z_dev, void* stream);
+/* host copies of the extracted blocks and their inverses, nblocks x 6 x 6 row-major each (either may be NULL) */
+int hfk_sys_blocks(hfk_sys* sys, double* B, double* Binv);
+/* Overlapping patches (no counterpart in the reference: the Vanka blocks of a saddle point, a pressure dof with the
+ * velocity dofs around it, whose nodal blocks alone are too weakly coupled): the dofs of patch p are
+ * dofs[ptr[p] .. ptr[p+1]) (1-based, ptr[0] = 0, 1 to 80 per patch), owned[k] != 0 when patch p writes dof dofs[k];
+ * every dof owned by exactly one patch.  The next hfk_sys_update_dev extracts the dense patch matrices and inverts
+ * them (singular ones counted in nsingular); the application is restricted additive Schwarz:
+ * z[d] = (A_p^-1 r_p)[d] from the patch p that owns d. */
+int hfk_sys_set_patches(hfk_sys* sys, int64_t npatch, const int64_t* ptr, const int32_t* dofs, const int8_t* owned);
+int hfk_sys_patches_apply_dev(hfk_sys* sys, const double* r_dev, double* z_dev, void* stream);
+/* host copy of the patch inverses, patch after patch, each m x m row-major (sum of m^2 doubles) */
+int hfk_sys_patch_inverses(hfk_sys* sys, double* Pinv);
+/* solve! as MatrixFreeSolver (src/solvers.jl:111-158) on the assembled matrix: kind 2 = GMRES, 3 = FGMRES
+ * (right preconditioned, restarted every `restart`); precond 0 = none, 1 = the blocks, 2 = `sweeps` damped sweeps
+ * with `omega`, 3 = the patches; from x = 0 to ||b - A x|| <= atol + rtol ||b||; status = (converged, iterations,
+ * 0), rnorm = (true residual norm, ||b||) */
+int hfk_sys_solve_dev(hfk_sys* sys, int kind, int precond, int sweeps, double omega, const double* b_dev, double* x_dev,
+                      double rtol, double atol, int maxiter, int restart, int* status, double* rnorm, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
