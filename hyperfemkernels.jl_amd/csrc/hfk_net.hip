@@ -24,7 +24,8 @@ using namespace hfk;
 namespace hfk {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest even
@@ -50,16 +51,21 @@ __device__ __forceinline__ bool neighbour(int v, int tap, int N, int& nb) {
   return true;
 }
 
-// 4 consecutive BF16 channels (8 bytes; channel offset and stride are multiples of 4)
-__device__ __forceinline__ bf16x4 load4(const unsigned short* p) { return *reinterpret_cast<const bf16x4*>(p); }
+// 8 consecutive BF16 channels (16 bytes; channel offsets and strides are multiples of 8)
+__device__ __forceinline__ bf16x8 load8(const unsigned short* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const s16x8*>(p));
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, s16x8{0, 0, 0, 0, 0, 0, 0, 0}); }
 
 // Y[v][co] = act( sum_tap sum_ci X[v + off(tap)][ci] W[tap][co][ci] ), X = [X1 | X2] (zero outside the lattice).
-// Implicit GEMM on v_mfma_f32_16x16x16_bf16: M = voxels, N = output channels, K = 27 x input channels.  One
-// work-group computes 16 voxels x all output channels (NT tiles of 16); its 4 waves split the 27 taps (wave w takes
-// taps w, w + 4, ...) and their partial sums are added through LDS in wave order.  On the small lattices of the
-// coarse levels (5^3 .. 33^3 voxels) this gives 4x the waves of a one-wave-per-tile mapping and a quarter of the
-// dependent MFMA chain.  Lane l: A rows m = l % 16 (a voxel), k = 4 (l / 16) + i; B columns n = l % 16 (an output
-// channel); D rows 4 (l / 16) + i, column l % 16.
+// Implicit GEMM on gfx950's v_mfma_f32_16x16x32_bf16: M = voxels, N = output channels, K = 27 x input channels
+// in steps of 32 (a 16-channel view fills the lower half of K, the upper half is zero).  One work-group computes 16
+// voxels x all output channels (NT tiles of 16); its 4 waves split the 27 taps (wave w takes taps w, w + 4, ...)
+// and their partial sums are added through LDS in wave order.  On the small lattices of the coarse levels
+// (5^3 .. 33^3 voxels) this gives 4x the waves of a one-wave-per-tile mapping and a quarter of the dependent MFMA
+// chain.  Lane l: A row m = l % 16 (a voxel) with k = 8 (l / 16) + i, B column n = l % 16 (an output channel) with
+// the same k (A and B share the k map, so the product does not depend on the hardware's order of k); D rows
+// 4 (l / 16) + i, column l % 16.
 // OUT: 0 = BF16 with ReLU, 1 = FP32 (overwrite), 2 = FP32 accumulate.
 template <int NT, int OUT>
 __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const unsigned short* __restrict__ W, View Y) {
@@ -82,13 +88,14 @@ __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const uns
       const View& X = part_ == 0 ? X1 : X2;
       if (!X.p) continue;
       const int cbase = part_ == 0 ? 0 : X1.C;
-      const unsigned short* xp = static_cast<const unsigned short*>(X.p) + (size_t)nb * X.stride + 4 * kg;
-      for (int c0 = 0; c0 < X.C; c0 += 16) {
-        bf16x4 a = in ? load4(xp + c0) : bf16x4{0, 0, 0, 0};
+      const unsigned short* xp = static_cast<const unsigned short*>(X.p) + (size_t)nb * X.stride + 8 * kg;
+      for (int c0 = 0; c0 < X.C; c0 += 32) {
+        const bool kin = c0 + 8 * kg < X.C;  // false for the upper half of K past a 16-channel remainder
+        const bf16x8 a = in && kin ? load8(xp + c0) : zero8();
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const bf16x4 b = load4(wt + (size_t)(t * 16 + m) * Cin + cbase + c0 + 4 * kg);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+          const bf16x8 b = kin ? load8(wt + (size_t)(t * 16 + m) * Cin + cbase + c0 + 8 * kg) : zero8();
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
         }
       }
     }
@@ -116,7 +123,8 @@ __global__ __launch_bounds__(256) void k_conv(int N, View X1, View X2, const uns
 }
 
 // dW[tap][co][ci] += sum_v G[v][co] X[v + off(tap)][ci]: per (tap, co tile, ci tile) a GEMM over the voxels
-// (K = voxels).  A (M = co, K = v): lane m = l % 16 -> co, k = 4 (l / 16) + i -> voxel; B (K = v, N = ci).
+// (K = voxels, 32 per v_mfma_f32_16x16x32_bf16).  A (M = co, K = v): lane m = l % 16 -> co, k = 8 (l / 16) + i ->
+// voxel; B (K = v, N = ci) with the same k map.
 // Each wave reduces a chunk of kVoxWave voxels and writes its 16 x 16 tile to P[tile][chunk][256];
 // k_wgrad_reduce then adds the chunks of every weight in chunk order, so the gradient does not depend on the order
 // in which the waves finish (training is reproducible).
@@ -139,17 +147,18 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(int N, View G, View X1, View
   const unsigned short* x = static_cast<const unsigned short*>(X.p);
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   const int vend = min(vstart + kVoxWave, nvox);
-  for (int vb = vstart; vb < vend; vb += 16) {
-    bf16x4 a, b;
+  for (int vb = vstart; vb < vend; vb += 32) {
+    s16x8 a, b;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int v = vb + 4 * kg + i;
+    for (int i = 0; i < 8; ++i) {
+      const int v = vb + 8 * kg + i;
       int nb = 0;
       const bool ok = v < vend;
       a[i] = ok ? (short)g[(size_t)v * G.stride + co] : (short)0;
       b[i] = (ok && neighbour(v, tap, N, nb)) ? (short)x[(size_t)nb * X.stride + cx] : (short)0;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                   0, 0, 0);
   }
   // D rows 4 (l / 16) + i -> co, column l % 16 -> ci
   const int nchunks = (nvox + kVoxWave - 1) / kVoxWave;
