@@ -152,119 +152,25 @@ int op_run_apply(hfk_op* op, const double* dx_dev, double* y_dev, hipStream_t s,
   return op_run_apply_t(op, op->tangents.p, op->storage, dx_dev, y_dev, s, alpha, accumulate);
 }
 
-// sparsity (src/assembly.jl:773-818, full_keys!/cell_keys! :835-893) and cell_positions! (:864-880).
-// The reference sorts the keys (col << 32) | row of every free pair of every cell and removes duplicates; the
-// result is, per column, the sorted set of rows coupled to it.  Here each column's set is formed directly from
-// the cells that contain the column's dof (same CSC, bit for bit), in parallel over columns.  Generic over the
-// cell dof table (any element, any number of fields: every (test, trial) pair of a cell is coupled).
-void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
-                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks,
-                 const CellNeighbours* nbr) {
-  const bool masked = blocks.mask != nullptr;
-  const int nf = blocks.nf;
-  auto block_set = [&](int64_t cell, int ii, int jj) {
-    return !masked || blocks.mask[(size_t)cell * nf * nf + blocks.field_of[ii] + nf * blocks.field_of[jj]] != 0;
-  };
-  // cells of every free dof
-  std::vector<int64_t> cstart((size_t)ncols + 1, 0);
-  for (int64_t cell = 0; cell < ncells; ++cell)
-    for (int j = 0; j < nd; ++j) {
-      const int32_t id = cell_dofs[(size_t)cell * nd + j];
-      if (id > 0) cstart[id]++;
-    }
-  for (int64_t c = 0; c < ncols; ++c) cstart[c + 1] += cstart[c];
-  std::vector<int64_t> fillp(cstart.begin(), cstart.end() - 1);
-  std::vector<int32_t> cells_of((size_t)cstart[ncols]);
-  for (int64_t cell = 0; cell < ncells; ++cell)
-    for (int j = 0; j < nd; ++j) {
-      const int32_t id = cell_dofs[(size_t)cell * nd + j];
-      if (id > 0) cells_of[fillp[id - 1]++] = (int32_t)cell;
-    }
-  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  auto rows_of = [&](int64_t c, std::vector<int32_t>& buf) {
-    buf.clear();
-    for (int64_t k = cstart[c]; k < cstart[c + 1]; ++k) {
-      const int64_t cell = cells_of[k];
-      const int32_t* ids = &cell_dofs[(size_t)cell * nd];
-      int jj = 0;  // the local dof of column c in this cell
-      if (masked)
-        while (ids[jj] != c + 1) ++jj;
-      for (int ii = 0; ii < nd; ++ii)
-        if (ids[ii] > 0 && block_set(cell, ii, jj)) buf.push_back(ids[ii]);
-      if (nbr)  // the rows of the cells across the facets of this cell
-        for (int64_t k2 = nbr->start[cell]; k2 < nbr->start[cell + 1]; ++k2) {
-          const int32_t* nid = &cell_dofs[(size_t)nbr->cells[k2] * nd];
-          for (int ii = 0; ii < nd; ++ii)
-            if (nid[ii] > 0) buf.push_back(nid[ii]);
-        }
-    }
-    std::sort(buf.begin(), buf.end());
-    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
-  };
-  std::vector<int64_t> cnt((size_t)ncols, 0);
-  {
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < nth; ++t)
-      pool.emplace_back([&, t]() {
-        std::vector<int32_t> buf;
-        for (int64_t c = t; c < ncols; c += nth) {
-          rows_of(c, buf);
-          cnt[c] = (int64_t)buf.size();
-        }
-      });
-    for (auto& th : pool) th.join();
-  }
-  colptr.assign((size_t)ncols + 1, 1);
-  for (int64_t c = 0; c < ncols; ++c) colptr[c + 1] = colptr[c] + cnt[c];
-  rowval.assign((size_t)(colptr[ncols] - 1), 0);
-  {
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < nth; ++t)
-      pool.emplace_back([&, t]() {
-        std::vector<int32_t> buf;
-        for (int64_t c = t; c < ncols; c += nth) {
-          rows_of(c, buf);
-          int64_t* out = &rowval[(size_t)(colptr[c] - 1)];
-          for (size_t k = 0; k < buf.size(); ++k) out[k] = buf[k];
-        }
-      });
-    for (auto& th : pool) th.join();
-  }
-  nzpos.assign((size_t)ncells * nd * nd, 0);
-  {
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < nth; ++t)
-      pool.emplace_back([&, t]() {
-        for (int64_t cell = t; cell < ncells; cell += nth) {
-          const int32_t* ids = &cell_dofs[(size_t)cell * nd];
-          int32_t* pos = &nzpos[(size_t)cell * nd * nd];
-          for (int jj = 0; jj < nd; ++jj) {
-            const int32_t c = ids[jj];
-            if (c <= 0) continue;
-            const int64_t lo = colptr[c - 1] - 1, hi = colptr[c] - 1;
-            for (int ii = 0; ii < nd; ++ii) {
-              const int32_t r = ids[ii];
-              if (r <= 0) continue;
-              const int64_t* it = std::lower_bound(rowval.data() + lo, rowval.data() + hi, (int64_t)r);
-              const bool stored = it != rowval.data() + hi && *it == r;
-              pos[ii + nd * jj] = stored ? (int32_t)(it - rowval.data() + 1) : 0;
-            }
-          }
-        }
-      });
-    for (auto& th : pool) th.join();
-  }
-}
-
+// sparsity (src/assembly.jl:773-818, full_keys!/cell_keys! :835-893) and cell_positions! (:864-880), with the
+// cross blocks of the skeleton (src/skeleton.jl:355-390): built on the device (hfk_pattern.hip), then copied into
+// the operator's host arrays in the reference's form (1-based Int64 colptr and rowval, nzpos per cell), which the
+// coarse-level set-up, the skeleton and the host API read.
 int op_build_pattern(hfk_op* op) {
   if (op->pattern_built) return HFK_OK;
-  if (skel_facets(op) > 0) {
-    CellNeighbours nb;
-    skel_neighbours(op, nb);
-    csc_pattern(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos, BlockInfo(), &nb);
-  } else {
-    csc_pattern(op->cell_dofs.data(), op->nd, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos);
-  }
+  CellNeighbours nb;
+  const bool skel = skel_facets(op) > 0;
+  if (skel) skel_neighbours(op, nb);
+  DevPattern pt;
+  int rc = device_pattern(op->ctx, op->ctx->stream, op->cell_dofs.data(), op->nd, op->ncells, op->nfree, BlockInfo(),
+                          skel ? &nb : nullptr, pt);
+  if (rc) return rc;
+  op->colptr.resize((size_t)op->nfree + 1);
+  op->rowval.resize((size_t)pt.nnz);
+  op->nzpos.resize((size_t)op->ncells * op->nd * op->nd);
+  if ((rc = device_pattern_to_host(op->ctx, op->ctx->stream, pt, op->nfree, op->ncells, op->nd, op->colptr.data(),
+                                   op->rowval.data(), op->nzpos.data(), nullptr)))
+    return rc;
   op->pattern_built = true;
   return HFK_OK;
 }

@@ -190,15 +190,25 @@ struct BlockInfo {
   int nf = 1;
   const uint8_t* mask = nullptr;     // ncells x nf^2; null: every block of every cell
 };
-// the reference's CSC sparsity and cell positions (1-based) of a cell dof table (> 0 free ids): the keys of the
-// free pairs of every cell (full_keys!), or of the blocks set on each cell (block_keys!); nzpos of a pair outside the
-// cell's blocks is its stored position when another cell stores it, 0 otherwise (cell_positions!, :864-899)
-// With `nbr`, the rows of every column also hold the free dofs of the facet neighbours of the column's cells (the
-// cross blocks of skeleton_keys!, src/skeleton.jl:355-390); nzpos stays the cells' own.
+// The reference's CSC sparsity and cell positions of a cell dof table (> 0 free ids), built on the device
+// (hfk_pattern.hip): the keys of the free pairs of every cell (full_keys!), or of the blocks set on each cell
+// (block_keys!); nzpos of a pair outside the cell's blocks is its stored position when another cell stores it, 0
+// otherwise (cell_positions!, src/assembly.jl:864-899).  With `nbr`, the rows of every column also hold the free
+// dofs of the facet neighbours of the column's cells (the cross blocks of skeleton_keys!, src/skeleton.jl:355-390);
+// nzpos stays the cells' own.  cell_dofs, blocks and nbr are host arrays.
 struct CellNeighbours;
-void csc_pattern(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols, std::vector<int64_t>& colptr,
-                 std::vector<int64_t>& rowval, std::vector<int32_t>& nzpos, const BlockInfo& blocks = BlockInfo(),
-                 const CellNeighbours* nbr = nullptr);
+struct DevPattern {
+  int64_t nnz = 0;
+  DevBuf colptr;  // int64, 0-based, ncols + 1
+  DevBuf rowval;  // int32, 0-based, nnz
+  DevBuf nzpos;   // int32 [cell][ii + nd jj]: 1 + stored position, 0 = not stored
+};
+int device_pattern(hfk_ctx* ctx, hipStream_t s, const int32_t* cell_dofs, int nd, int64_t ncells, int64_t ncols,
+                   const BlockInfo& blocks, const CellNeighbours* nbr, DevPattern& out);
+// copies in the reference's form: 1-based Int64 colptr and rowval, nzpos (in the local order perm[ii] when perm is
+// given: out[cell][perm[ii] + nd perm[jj]]); null outputs are skipped
+int device_pattern_to_host(hfk_ctx* ctx, hipStream_t s, const DevPattern& pt, int64_t ncols, int64_t ncells, int nd,
+                           int64_t* colptr, int64_t* rowval, int32_t* nzpos, const int32_t* perm);
 // the reference's greedy colouring (1-based colours); false when more than 128 colours are needed
 bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color);
 int op_build_blocks(hfk_op* op);

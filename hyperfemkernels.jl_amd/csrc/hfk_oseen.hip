@@ -589,9 +589,8 @@ struct hfk_oseen {
   std::vector<uint8_t> block_mask;        // ncells x 4 (velocity/pressure test x trial blocks) or empty: all
   std::vector<int64_t> color_start;  // cells of colour c: cells_by_color[color_start[c] .. color_start[c+1])
   bool pattern_built = false, has_fields = false;
-  std::vector<int64_t> colptr, rowval;
-  std::vector<int32_t> nzpos;
-  DevBuf ids, coords, qp, fields, lap, dvals, cells_d, nzpos_d, x_stage, y_stage, nz_stage;
+  DevPattern pat;  // the CSC pattern and the cells' positions, on the device only
+  DevBuf ids, coords, qp, fields, lap, dvals, cells_d, x_stage, y_stage, nz_stage;
   bool has_lap = false;
   // colour by colour with plain adds (bitwise reproducible), or one launch with FP64 atomic adds
   bool deterministic = true;
@@ -635,7 +634,7 @@ struct hfk_oseen {
     d.fields = fields.as<double>();
     d.lap = has_lap ? lap.as<double>() : nullptr;
     d.dvals = dvals.as<double>();
-    d.nzpos = nzpos_d.as<int32_t>();
+    d.nzpos = pat.nzpos.as<int32_t>();
     return d;
   }
   OseenFaceDev face_dev() const {
@@ -670,9 +669,9 @@ int oseen_pattern(hfk_oseen* op) {
     bi.nf = 2;
     bi.mask = op->block_mask.data();
   }
-  csc_pattern(op->cell_dofs.data(), kND, op->ncells, op->nfree, op->colptr, op->rowval, op->nzpos, bi);
-  HIPCHK(op->ctx, op->nzpos_d.alloc(op->nzpos.size() * sizeof(int32_t)));
-  HIPCHK(op->ctx, hipMemcpy(op->nzpos_d.p, op->nzpos.data(), op->nzpos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  const int rc = device_pattern(op->ctx, op->ctx->stream, op->cell_dofs.data(), kND, op->ncells, op->nfree, bi, nullptr,
+                                op->pat);
+  if (rc) return rc;
   op->pattern_built = true;
   return HFK_OK;
 }
@@ -760,7 +759,7 @@ int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipSt
 int oseen_run(hfk_oseen* op, const double* x, double* b, double* nzval, hipStream_t s) {
   if (!op->has_fields) return set_err(op->ctx, HFK_ESTATE, "HyperFEMKernels: Oseen fields (w, un, f) not set");
   if (b) HIPCHK(op->ctx, launch_fill(b, op->nfree, 0.0, s));
-  if (nzval) HIPCHK(op->ctx, launch_fill(nzval, (int64_t)op->rowval.size(), 0.0, s));
+  if (nzval) HIPCHK(op->ctx, launch_fill(nzval, op->pat.nnz, 0.0, s));
   return op->deterministic ? oseen_launch<false>(op, x, b, nzval, s) : oseen_launch<true>(op, x, b, nzval, s);
 }
 
@@ -1036,9 +1035,10 @@ int hfk_oseen_set_blocks(hfk_oseen* op, const int32_t* mask, const int32_t* cell
     op->block_mask = std::move(m);
   }
   op->pattern_built = false;  // the pattern, its positions and the device copy are rebuilt at the next use
-  op->colptr.clear();
-  op->rowval.clear();
-  op->nzpos.clear();
+  op->pat.colptr.reset();
+  op->pat.rowval.reset();
+  op->pat.nzpos.reset();
+  op->pat.nnz = 0;
   return HFK_OK;
 }
 
@@ -1064,7 +1064,7 @@ int hfk_oseen_info(hfk_oseen* op, int64_t* info) {
   info[2] = op->ncells;
   info[3] = op->np;
   info[4] = (int64_t)op->color_start.size() - 1;
-  info[5] = op->pattern_built ? (int64_t)op->rowval.size() : -1;
+  info[5] = op->pattern_built ? op->pat.nnz : -1;
   return HFK_OK;
 }
 
@@ -1079,7 +1079,7 @@ int hfk_oseen_pattern_nnz(hfk_oseen* op, int64_t* nnz) {
   if (rc) return rc;
   if (!nnz) return set_err(op->ctx, HFK_EARG, "HyperFEMKernels: null argument");
   if ((rc = oseen_pattern(op))) return rc;
-  *nnz = (int64_t)op->rowval.size();
+  *nnz = op->pat.nnz;
   return HFK_OK;
 }
 
@@ -1087,16 +1087,9 @@ int hfk_oseen_pattern(hfk_oseen* op, int64_t* colptr, int64_t* rowval, int32_t* 
   int rc = oseen_bind(op);
   if (rc) return rc;
   if ((rc = oseen_pattern(op))) return rc;
-  if (colptr) std::copy(op->colptr.begin(), op->colptr.end(), colptr);
-  if (rowval) std::copy(op->rowval.begin(), op->rowval.end(), rowval);
-  if (nzpos) {  // in the host's local order: [cell][perm[ii] + 34 perm[jj]]
-    const size_t per = (size_t)kND * kND;
-    for (int64_t cell = 0; cell < op->ncells; ++cell)
-      for (int jj = 0; jj < kND; ++jj)
-        for (int ii = 0; ii < kND; ++ii)
-          nzpos[cell * per + op->perm[ii] + kND * op->perm[jj]] = op->nzpos[cell * per + ii + kND * jj];
-  }
-  return HFK_OK;
+  // nzpos in the host's local order: [cell][perm[ii] + 34 perm[jj]]
+  return device_pattern_to_host(op->ctx, op->ctx->stream, op->pat, op->nfree, op->ncells, kND, colptr, rowval, nzpos,
+                                op->perm.data());
 }
 
 int hfk_oseen_residual_dev(hfk_oseen* op, const double* x_dev, double* b_dev, void* stream) {
@@ -1145,7 +1138,7 @@ int hfk_oseen_jacobian(hfk_oseen* op, const double* x, double* nzval) {
   (void)x;
   if ((rc = oseen_pattern(op))) return rc;
   hipStream_t s = op->ctx->stream;
-  const size_t nnz = op->rowval.size();
+  const size_t nnz = (size_t)op->pat.nnz;
   HIPCHK(op->ctx, op->nz_stage.alloc(std::max<size_t>(nnz, 1) * sizeof(double)));
   if ((rc = oseen_run(op, nullptr, nullptr, op->nz_stage.as<double>(), s))) return rc;
   HIPCHK(op->ctx, hipMemcpyAsync(nzval, op->nz_stage.p, nnz * sizeof(double), hipMemcpyDeviceToHost, s));

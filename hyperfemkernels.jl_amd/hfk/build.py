@@ -13,7 +13,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")
 LIB = os.path.join(HERE, "libhfk.so")
 SOURCES = ["hfk_kernels.hip", "hfk_capi.hip", "hfk_solver.hip", "hfk_pmg.hip", "hfk_net.hip", "hfk_oseen.hip",
-           "hfk_rtc.hip", "hfk_skel.hip", "hfk_sys.hip"]
+           "hfk_rtc.hip", "hfk_skel.hip", "hfk_sys.hip", "hfk_pattern.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # packed FP32 instructions stay off (-packed-fp32-ops): the compiler may not form v_pk_* FP32 arithmetic.

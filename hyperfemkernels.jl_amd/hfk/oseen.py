@@ -216,6 +216,12 @@ class OseenOperator:
         check(self.lib.hfk_oseen_residual(self.handle, ptr(x, C.c_double), ptr(b, C.c_double)), self.ctx.handle)
         return b
 
+    def pattern_nnz(self) -> int:
+        """Number of stored entries of the CSC pattern (built on the device at the first call)."""
+        nnz = C.c_int64()
+        check(self.lib.hfk_oseen_pattern_nnz(self.handle, C.byref(nnz)), self.ctx.handle)
+        return int(nnz.value)
+
     def pattern(self):
         if self._pattern is None:
             nnz = C.c_int64()
