@@ -260,6 +260,31 @@ def newton_leg(rtol=1e-10, maxit=20):
             "newton_total_s": t_all, "note": "residual!/jacobian! timed through host arrays (PCIe included)"}
 
 
+def pattern_leg(n=40):
+    """The sparsity pattern of Q1 vector elasticity on n^3 hexahedra, the workload for which the reference quotes its
+    pattern build (CHANGELOG.md:25-26: 0.511-0.520 s on 4 Apple M5 Max cores at 40^3): built on the device, then
+    copied into the operator's host arrays (1-based Int64 colptr and rowval, nzpos per cell); best of 3 operators."""
+    import ctypes as C
+
+    from hfk import HyperelasticForm, KernelOperator
+    from hfk._lib import check
+    from hfk.mesh import hex_space
+
+    sp = hex_space((n, n, n), 1)
+    best = None
+    for _ in range(3):
+        op = KernelOperator(HyperelasticForm("svk", 2.0, 1.0, degree=2), sp, jacobian="assembled")
+        nnz = C.c_int64()
+        t0 = time.perf_counter()
+        check(op.lib.hfk_pattern_nnz(op.handle, C.byref(nnz)), op.ctx.handle)
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+        op.close()
+    return {"workload": f"Q1 vector elasticity, {n}^3 hexahedra", "free_dofs": sp.nfree, "cells": n ** 3,
+            "nnz": int(nnz.value), "pattern_s_min_of_3": best, "reference_quoted_s": [0.511, 0.520],
+            "reference_hardware": "Mac M5 Max, 4 threads (CHANGELOG.md:25-26)"}
+
+
 def assembly_leg(reps=20):
     """The quick start's residual! and jacobian! with device-resident state and CSC values, each call bracketed by
     HIP events on the launch stream (no PCIe), against the FP64 peak: the reference's count of the element-matrix
@@ -1260,6 +1285,7 @@ def main():
     if not args.no_assembly:
         print("bench: SVK quick start 20x10x10 Q2, device-timed residual! and jacobian!", file=sys.stderr, flush=True)
         res["assembly_svk_20x10x10_device"] = assembly_leg()
+        res["pattern_q1_elasticity_40"] = pattern_leg()
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 64^3 Jv, min of 5 / median of 20)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline()
