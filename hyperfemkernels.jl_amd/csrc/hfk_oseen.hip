@@ -51,14 +51,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ double Sh[kMaxPoints][14][5];  // per point and shape function: N, grad N (3), w . grad N
   __shared__ double Pq[kMaxPoints][4];      // per point: dV, ts, tp, tl
   __shared__ double Ke[kND][kND + 1];       // element matrix [test][trial]
-  __shared__ double Jit[9], Cm[2];
   const int tid = threadIdx.x;
   const int64_t cell = work_item(cells, op.ncells, chunk);
   if (cell < 0) return;  // uniform over the work-group, before any barrier
   const int np = op.np;
-  if (tid == 0) {
+  // Everything this thread reads from memory before the pair phase is requested first, so that the latencies
+  // overlap: the CSC positions of its element-matrix entries e = ii + 34 jj (entry Ke[ii][jj]), the point data of
+  // its staging items and the cell's vertices.  The values the entries add to are loaded after the staging (loads
+  // complete in issue order, so that their wait does not hold the staging) and arrive while the pair phase, which
+  // reads only LDS, runs.  Within one colour launch no other work-group touches these entries.
+  constexpr int kPerThread = (kND * kND + 255) / 256;  // the launch has 256 threads per work-group
+  constexpr int kStage = (kMaxPoints * 15 + 255) / 256;  // staging items per thread
+  int32_t pk[kPerThread];
+  double old[kPerThread];
+  {
+    const int32_t* pos = op.nzpos + cell * kND * kND;
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int e = tid + 256 * k;
+      pk[k] = e < kND * kND ? pos[e] : 0;
+    }
+  }
+  double fv[kStage][3];  // w at the point (shape-function items) or un (the point-scalar item)
+#pragma unroll
+  for (int k = 0; k < kStage; ++k) {
+    const int it = tid + 256 * k;
+    if (it < np * 15) {
+      const int q = it / 15, f = it - q * 15;
+      const double* fq = op.fields + ((size_t)cell * np + q) * 9 + (f < 14 ? 0 : 3);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) fv[k][i] = fq[i];
+    }
+  }
+  // the affine map of the cell, in every thread (no barrier): J (column d: x_{d+1} - x_0), det J, the transposed
+  // inverse Jit, |det J| and h
+  double Jit[9], adet;
+  {
     const double* X = op.coords + cell * 12;
-    double J[9];  // column d: x_{d+1} - x_0
+    double J[9];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -66,8 +96,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[3] * (J[1] * J[8] - J[7] * J[2]) +
                        J[6] * (J[1] * J[5] - J[4] * J[2]);
     const double id = 1.0 / det;
-    // inverse (column-major), stored transposed
-    double I[9];
+    double I[9];  // inverse (column-major)
     I[0] = (J[4] * J[8] - J[7] * J[5]) * id;
     I[3] = -(J[3] * J[8] - J[6] * J[5]) * id;
     I[6] = (J[3] * J[7] - J[6] * J[4]) * id;
@@ -81,29 +110,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) Jit[i + 3 * j] = I[j + 3 * i];
-    Cm[0] = fabs(det);
-    Cm[1] = cbrt(6.0 * sqrt(2.0) * fabs(det) / 6.0);
-  }
-  __syncthreads();
-  // CSC positions of this thread's element-matrix entries e = ii + 34 jj (entry Ke[ii][jj]); the values they add to
-  // are loaded after the staging below (loads complete in issue order, so that their wait does not hold the
-  // staging) and arrive while the pair phase, which reads only LDS, runs.  Within one colour launch no other
-  // work-group touches these entries.
-  constexpr int kPerThread = (kND * kND + 255) / 256;  // the launch has 256 threads per work-group
-  int32_t pk[kPerThread];
-  double old[kPerThread];
-  {
-    const int32_t* pos = op.nzpos + cell * kND * kND;
-#pragma unroll
-    for (int k = 0; k < kPerThread; ++k) {
-      const int e = tid + 256 * k;
-      pk[k] = e < kND * kND ? pos[e] : 0;
-    }
+    adet = fabs(det);
   }
   // stage every (point, shape function) and the point scalars
-  for (int it = tid; it < np * 15; it += blockDim.x) {
+#pragma unroll
+  for (int k = 0; k < kStage; ++k) {
+    const int it = tid + 256 * k;
+    if (it >= np * 15) break;
     const int q = it / 15, f = it - q * 15;
-    const double* fq = op.fields + ((size_t)cell * np + q) * 9;
     if (f < 14) {
       double n, g[3];
       shape(f, op.qp[4 * q], op.qp[4 * q + 1], op.qp[4 * q + 2], n, g);
@@ -111,7 +125,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         gx[i] = Jit[i] * g[0] + Jit[i + 3] * g[1] + Jit[i + 6] * g[2];
-        wg += fq[i] * gx[i];
+        wg += fv[k][i] * gx[i];
       }
       Sh[q][f][0] = n;
       Sh[q][f][1] = gx[0];
@@ -119,11 +133,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       Sh[q][f][3] = gx[2];
       Sh[q][f][4] = wg;
     } else {
-      const double un = sqrt(fq[3] * fq[3] + fq[4] * fq[4] + fq[5] * fq[5]);
-      const double h = Cm[1];
+      const double un = sqrt(fv[k][0] * fv[k][0] + fv[k][1] * fv[k][1] + fv[k][2] * fv[k][2]);
+      const double h = cbrt(6.0 * sqrt(2.0) * adet / 6.0);
       const double r1 = 2.0 * un / h, r2 = 3.0 * (4.0 * op.mu / (op.rho * h * h));
       const double ts = 1.0 / sqrt(r1 * r1 + r2 * r2);
-      Pq[q][0] = Cm[0] * op.qp[4 * q + 3];
+      Pq[q][0] = adet * op.qp[4 * q + 3];
       Pq[q][1] = ts;
       Pq[q][2] = ts / op.rho;
       Pq[q][3] = op.rho * un * h * 0.5;
