@@ -94,6 +94,42 @@ def test_transfers(hip_ctx, oracle, qc, qf):
         assert rel(xi, xc) < 1e-14
 
 
+@pytest.mark.parametrize("law", ["linear", "neohookean"])
+def test_galerkin_coarse_operator(hip_ctx, law):
+    """The assembled coarse level keeps the working quadrature (src/pmultigrid.jl:248-251), so for a linear law it
+    is the Galerkin product P' A P of the working operator to round-off; at a state of the neo-Hookean law (the
+    coarse tangent taken at the injected state) the gap is O(h) (test/solvers/test_pmultigrid_levels.jl:40-63).
+    The state is a quadratic field, which Q2 holds exactly and whose nodal Q1 interpolant has an O(h) gradient
+    error."""
+    def u(X):
+        return 0.05 * np.stack([X[:, 1] * X[:, 2], X[:, 0] ** 2, X[:, 0] * X[:, 2]], axis=1)
+
+    gaps = []
+    for n in ((4, 4, 4), (8, 8, 8)):
+        print(f"galerkin law={law} n={n}", flush=True)
+        fs = hex_space(n, 2, dirichlet=dirichlet_closed_x0)
+        cs = hex_space(n, 1, dirichlet=dirichlet_closed_x0)
+        fop = KernelOperator(HyperelasticForm(law, degree=4), fs, jacobian="matrixfree",
+                             dirichlet_values=fs.dirichlet_values(u), ctx=hip_ctx)
+        cop = KernelOperator(HyperelasticForm(law, degree=4), cs, jacobian="assembled", ctx=hip_ctx)
+        mf = MatrixFreeJacobian(fop).prepare(fs.free_values(u))
+        pmg = PMultigrid([fop, cop], Chebyshev(degree=2), coarse_rtol=1e-3).update()
+        assert pmg.ok
+        P = Prolongation(cop, fop)
+        rng = np.random.default_rng(21)
+        gap = 0.0
+        for _ in range(3):
+            xc = rng.standard_normal(cop.nfree)
+            ptap = P.restrict(np.zeros(cop.nfree), mf @ (P @ xc))
+            gap = max(gap, rel(pmg.coarse_matvec(xc), ptap))
+        gaps.append(gap)
+    print("galerkin gaps", law, gaps, flush=True)
+    if law == "linear":
+        assert max(gaps) < 1e-12
+    else:
+        assert 1e-8 < gaps[1] < 0.6 * gaps[0]  # halving h about halves it
+
+
 def make_levels(hip_ctx, n, law="neohookean", seed=0, amp=0.01):
     """Q2 working level and assembled Q1 coarse level on n^3 cells clamped on the closed face x = 0, at the
     smooth random state of amplitude amp (1e-2 keeps |grad u| <= 0.19: a positive definite tangent)."""
