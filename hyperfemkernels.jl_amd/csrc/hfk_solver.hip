@@ -433,6 +433,36 @@ __global__ void k_prolongate(int64_t nslot, int NF, int NC, const int32_t* __res
   }
 }
 
+// The same with one thread per (slot, component): the NC coarse values are loaded once and every fine node of the
+// cell is formed from them in the order above (bitwise the same values), instead of NF threads each loading them.
+template <int NC>
+__global__ void k_prolongate_cell(int64_t nslot, int NF, const int32_t* __restrict__ fid,
+                                  const int32_t* __restrict__ cid, const double* __restrict__ E,
+                                  const double* __restrict__ xc, double* __restrict__ yf) {
+  const int64_t n = nslot * 3;
+  for (int64_t sc = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; sc < n; sc += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* ci = cid + sc * NC;
+    int jc[NC];
+    double xv[NC];
+#pragma unroll
+    for (int tc = 0; tc < NC; ++tc) jc[tc] = ci[tc];
+#pragma unroll
+    for (int tc = 0; tc < NC; ++tc) xv[tc] = jc[tc] > 0 ? xc[jc[tc] - 1] : 0.0;
+    const int32_t* fi = fid + sc * NF;
+    for (int tf = 0; tf < NF; ++tf) {
+      const int id = fi[tf];
+      if (id <= 0) continue;
+      double v = 0;
+#pragma unroll
+      for (int tc = 0; tc < NC; ++tc) {
+        const double e = E[tf * NC + tc];
+        if (e != 0.0 && jc[tc] > 0) v += e * xv[tc];
+      }
+      yf[id - 1] = v;  // insertion: the cells sharing a fine dof write the same value
+    }
+  }
+}
+
 // restriction over the batches of one colour of the fine operator (cells of a colour share no fine free dof,
 // hence no coarse free dof): yc[j] += sum_tf E[tf, tc] yf[id] / mult[id]
 __global__ void k_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* __restrict__ fid,
@@ -452,6 +482,39 @@ __global__ void k_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const i
       if (e != 0.0 && id > 0) v += e * yf[id - 1] * invmult[id - 1];
     }
     yc[j - 1] += v;
+  }
+}
+
+// The same with one thread per (slot, component): every fine value of the cell is loaded once and added to the NC
+// coarse sums in the order above (bitwise the same values), instead of NC threads each loading all of them.
+template <int NC>
+__global__ void k_restrict_cell(int64_t slot0, int64_t nslot, int NF, const int32_t* __restrict__ fid,
+                                const int32_t* __restrict__ cid, const double* __restrict__ E,
+                                const double* __restrict__ invmult, const double* __restrict__ yf,
+                                double* __restrict__ yc) {
+  const int64_t n = nslot * 3;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t sc = slot0 * 3 + g;
+    const int32_t* ci = cid + sc * NC;
+    const int32_t* fi = fid + sc * NF;
+    double v[NC];
+#pragma unroll
+    for (int tc = 0; tc < NC; ++tc) v[tc] = 0;
+    for (int tf = 0; tf < NF; ++tf) {
+      const int id = fi[tf];
+      if (id <= 0) continue;
+      const double yv = yf[id - 1], im = invmult[id - 1];
+#pragma unroll
+      for (int tc = 0; tc < NC; ++tc) {
+        const double e = E[tf * NC + tc];
+        if (e != 0.0) v[tc] += e * yv * im;
+      }
+    }
+#pragma unroll
+    for (int tc = 0; tc < NC; ++tc) {
+      const int j = ci[tc];
+      if (j > 0) yc[j - 1] += v[tc];
+    }
   }
 }
 
@@ -589,13 +652,17 @@ hipError_t bell_fill(int64_t nnz, const int32_t* pos, const double* val, double*
 }
 hipError_t transfer_prolongate(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* E,
                                const double* xc, double* yf, hipStream_t s) {
-  hipLaunchKernelGGL(k_prolongate, grid_for(nslot * 3 * NF), dim3(256), 0, s, nslot, NF, NC, fid, cid, E, xc, yf);
+  if (NC == 8) hipLaunchKernelGGL(k_prolongate_cell<8>, grid_for(nslot * 3), dim3(256), 0, s, nslot, NF, fid, cid, E, xc, yf);
+  else if (NC == 27) hipLaunchKernelGGL(k_prolongate_cell<27>, grid_for(nslot * 3), dim3(256), 0, s, nslot, NF, fid, cid, E, xc, yf);
+  else hipLaunchKernelGGL(k_prolongate, grid_for(nslot * 3 * NF), dim3(256), 0, s, nslot, NF, NC, fid, cid, E, xc, yf);
   return hipGetLastError();
 }
 hipError_t transfer_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid,
                              const double* E, const double* invmult, const double* yf, double* yc, hipStream_t s) {
   if (nslot <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_restrict, grid_for(nslot * 3 * NC), dim3(256), 0, s, slot0, nslot, NF, NC, fid, cid, E, invmult, yf, yc);
+  if (NC == 8) hipLaunchKernelGGL(k_restrict_cell<8>, grid_for(nslot * 3), dim3(256), 0, s, slot0, nslot, NF, fid, cid, E, invmult, yf, yc);
+  else if (NC == 27) hipLaunchKernelGGL(k_restrict_cell<27>, grid_for(nslot * 3), dim3(256), 0, s, slot0, nslot, NF, fid, cid, E, invmult, yf, yc);
+  else hipLaunchKernelGGL(k_restrict, grid_for(nslot * 3 * NC), dim3(256), 0, s, slot0, nslot, NF, NC, fid, cid, E, invmult, yf, yc);
   return hipGetLastError();
 }
 hipError_t transfer_inject(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* Einj,
