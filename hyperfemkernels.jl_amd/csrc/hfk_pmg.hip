@@ -164,6 +164,9 @@ struct hfk_cheb {
   const int32_t* bell_dofs = nullptr;
   const void* bell_val = nullptr;
   bool bell_fp32 = false;
+  // with an assembled operator: the nodal blocks still come from the level's stored tangents (else from the
+  // assembled matrix, set up by its owner)
+  bool csr_tangent_blocks = false;
   int nbuild = 0;
   DevBuf r, d, tmp, start, V;
   Dots dots;
@@ -254,7 +257,7 @@ int cheb_update(hfk_cheb* ch, hipStream_t s) {
   bool bok = ch->blocks_ok;
   int rc;
   // an assembled operator brings its nodal blocks (and their check) from its own set-up
-  if (!ch->csr_val && (rc = blocks_update(op, ch->T(), ch->storage(), s, &bok))) return rc;
+  if ((!ch->csr_val || ch->csr_tangent_blocks) && (rc = blocks_update(op, ch->T(), ch->storage(), s, &bok))) return rc;
   ch->blocks_ok = bok;
   if (!bok) {
     ch->ok = false;
@@ -635,6 +638,14 @@ struct hfk_pmg {
   int nupdate = 0, ncycle = 0, last_inner = 0, total_inner = 0;
   int storage = 0;           // storage of the matrix-free coarse levels' tangents
   int smoother_storage = -1;  // -1: the working operator's own tangents; else a copy in this storage
+  // Intermediate levels with at most kAssembledLevel free dofs: their eight colour launches of a few dozen
+  // work-groups are bound by launch latency, so each update also assembles the level (CSC, FP64) and its smoother
+  // applies it with one CSC product per Jv; the nodal blocks still come from the stored tangents.
+  struct LevelCsc {
+    bool built = false;
+    DevBuf colptr, rowval, nzval;  // 0-based int32 pattern, FP64 values
+  };
+  std::vector<std::unique_ptr<LevelCsc>> lcsc;  // per level; null: matrix-free
 };
 
 namespace {
@@ -740,6 +751,35 @@ int coarse_direct_setup(hfk_pmg* g, hipStream_t s, bool* ok) {
   HIPCHK(g->ctx, hipMemcpyAsync(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(g->ctx, hipStreamSynchronize(s));  // also: W and rc are freed on return
   *ok = nbad == 0;
+  return HFK_OK;
+}
+
+constexpr int64_t kAssembledLevel = 200000;
+
+// the assembled matrix of intermediate level l at its injected state, handed to the level's smoother
+int level_csc_update(hfk_pmg* g, size_t l, hipStream_t s) {
+  hfk_op* op = g->ops[l];
+  auto& L = *g->lcsc[l];
+  int rc;
+  if (!L.built) {
+    if ((rc = op_build_pattern(op))) return rc;
+    if ((int64_t)op->rowval.size() >= ((int64_t)1 << 31))
+      return set_err(g->ctx, HFK_EDIM, "HyperFEMKernels: level matrix too large");
+    std::vector<int32_t> cp(op->colptr.size()), rv(op->rowval.size());
+    for (size_t i = 0; i < cp.size(); ++i) cp[i] = (int32_t)(op->colptr[i] - 1);
+    for (size_t i = 0; i < rv.size(); ++i) rv[i] = (int32_t)(op->rowval[i] - 1);
+    HIPCHK(g->ctx, L.colptr.alloc(cp.size() * sizeof(int32_t)));
+    HIPCHK(g->ctx, hipMemcpy(L.colptr.p, cp.data(), cp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(g->ctx, L.rowval.alloc(std::max<size_t>(rv.size(), 1) * sizeof(int32_t)));
+    HIPCHK(g->ctx, hipMemcpy(L.rowval.p, rv.data(), rv.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(g->ctx, L.nzval.alloc(std::max<size_t>(rv.size(), 1) * sizeof(double)));
+    L.built = true;
+  }
+  if ((rc = op_run_jacobian(op, op->state.as<double>(), L.nzval.as<double>(), s))) return rc;
+  hfk_cheb* ch = g->cheb[l].get();
+  ch->csr_colptr = L.colptr.as<int32_t>();
+  ch->csr_rowval = L.rowval.as<int32_t>();
+  ch->csr_val = L.nzval.as<double>();
   return HFK_OK;
 }
 
@@ -989,6 +1029,7 @@ int pmg_update(hfk_pmg* g, hipStream_t s) {
       return rc;
     if (l + 1 < g->ops.size()) {
       if ((rc = op_run_prepare(pc, pc->state.as<double>(), g->storage, s))) return rc;
+      if (g->lcsc[l] && (rc = level_csc_update(g, l, s))) return rc;
     } else {
       // the coarse state is injected at every update; the matrix is re-assembled and set up on the first update,
       // every coarse_every updates, and after one that was not finite (src/pmultigrid.jl:200-216)
@@ -1413,6 +1454,12 @@ int hfk_pmg_create(hfk_op** levels, int nlevels, int degree, double range, doubl
       return rc;
   }
   for (int l = 1; l < nlevels; ++l) HIPCHK(ctx, g->ops[l]->state.alloc((size_t)g->ops[l]->nfree * sizeof(double)));
+  g->lcsc.resize(nlevels);
+  for (int l = 1; l + 1 < nlevels; ++l)
+    if (g->ops[l]->nfree <= kAssembledLevel) {
+      g->lcsc[l].reset(new hfk_pmg::LevelCsc());
+      g->cheb[l]->csr_tangent_blocks = true;
+    }
   const int64_t nc = g->ops.back()->nfree;
   if ((rc = vec_alloc(ctx, g->cr, nc)) || (rc = vec_alloc(ctx, g->cz, nc)) || (rc = vec_alloc(ctx, g->cp, nc)) ||
       (rc = vec_alloc(ctx, g->cq, nc)) || (rc = vec_alloc(ctx, g->ct, nc)) || (rc = vec_alloc(ctx, g->cu, nc)) ||
