@@ -214,6 +214,35 @@ def test_pattern_and_assembly(oracle, hip_ctx, order, P, law, n):
     assert rel(mf @ dx, A @ dx) < 1e-11
 
 
+@pytest.mark.parametrize("nfree_pad", [0, 1])
+def test_pattern_random_numbering(oracle, hip_ctx, nfree_pad):
+    """The sparsity pattern and positions (built on the device by a radix sort of (column, row) keys) equal the
+    oracle's for a randomly renumbered dof table: no locality in the keys.  With nfree_pad, the free ids are spread
+    over the next power of two (columns that no cell holds stay empty; the column bits of the keys are then full)."""
+    import dataclasses
+
+    sp, dv, x = problem(2, 3, "svk", (5, 4, 3), 0.1)
+    rng = np.random.default_rng(7 + nfree_pad)
+    n = sp.nfree if not nfree_pad else 1 << int(np.ceil(np.log2(sp.nfree + 1)))
+    newid = rng.permutation(n)[: sp.nfree] + 1  # new id of old free id k: newid[k - 1]
+    cd = sp.cell_dofs.copy()
+    m = cd > 0
+    cd[m] = newid[cd[m] - 1]
+    sp2 = dataclasses.replace(sp, cell_dofs=np.ascontiguousarray(cd, dtype=np.int32), nfree=n)
+    print(f"pattern numbering nfree={n} cells={cd.shape[0]}", flush=True)
+    op = KernelOperator(HyperelasticForm("svk"), sp2, dirichlet_values=dv, ctx=hip_ctx)
+    orc = oracle.OracleOp(sp2, "svk", 2.0, 1.0, dvals=dv)
+    colptr, rowval, nzpos = op.pattern()
+    c2, r2, p2 = orc.pattern()
+    assert np.array_equal(colptr, c2) and np.array_equal(rowval, r2) and np.array_equal(nzpos, p2)
+    if nfree_pad:
+        assert (np.diff(colptr) == 0).sum() == n - sp.nfree  # the empty columns
+    xp = np.zeros(n)
+    xp[newid - 1] = x
+    A = op.jacobian(op.allocate_jacobian(), xp)
+    assert rel(A.data, orc.jacobian(xp, p2, r2.size)) < TOL
+
+
 @pytest.mark.parametrize("order,P,storage", [(2, 3, "full64"), (2, 3, "sym32"), (1, 2, "full64"), (3, 4, "sym64")])
 def test_atomic_single_launch(oracle, hip_ctx, order, P, storage):
     """The single-launch product with atomic adds equals the oracle to the same bar (its summation order differs
