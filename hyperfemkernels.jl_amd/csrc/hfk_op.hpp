@@ -209,6 +209,11 @@ int device_pattern(hfk_ctx* ctx, hipStream_t s, const int32_t* cell_dofs, int nd
 // given: out[cell][perm[ii] + nd perm[jj]]); null outputs are skipped
 int device_pattern_to_host(hfk_ctx* ctx, hipStream_t s, const DevPattern& pt, int64_t ncols, int64_t ncells, int nd,
                            int64_t* colptr, int64_t* rowval, int32_t* nzpos, const int32_t* perm);
+// CSR of a square CSC pattern given in the reference's 1-based form (host arrays): rp (int64, 0-based, n + 1), the
+// columns of every row ascending (int32, 0-based) and perm[k] = the CSC position of CSR entry k; built on the device
+// by a radix sort of (row, column) keys.  *bad: 0, or 1 (a row out of range) | 2 (rows not ascending in a column).
+int device_csc_to_csr(hfk_ctx* ctx, hipStream_t s, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                      DevBuf& rp, DevBuf& col, DevBuf& perm, int* bad);
 // the reference's greedy colouring (1-based colours); false when more than 128 colours are needed
 bool greedy_colors(const int32_t* cell_dofs, int nd, int64_t ncells, int64_t nfree, std::vector<int32_t>& color);
 int op_build_blocks(hfk_op* op);

@@ -134,6 +134,31 @@ __global__ __launch_bounds__(kPT) void k_nzpos_perm(const int32_t* __restrict__ 
   out[cell * per + perm[ii] + nd * perm[jj]] = in[p];
 }
 
+// (row, column) keys of a CSC pattern given 1-based (colptr, rowval): one thread per column; the value of key k is
+// its CSC position.  bad |= 1 for a row out of range, 2 for rows that do not ascend within a column.
+__global__ __launch_bounds__(kPT) void k_csc_keys(int64_t n, const int64_t* __restrict__ colptr,
+                                                  const int64_t* __restrict__ rowval, int cb,
+                                                  uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                                  int* __restrict__ bad) {
+  const int64_t j = (int64_t)blockIdx.x * kPT + threadIdx.x;
+  if (j >= n) return;
+  int flags = 0;
+  for (int64_t k = colptr[j] - 1; k < colptr[j + 1] - 1; ++k) {
+    const int64_t r = rowval[k];
+    if (r < 1 || r > n) flags |= 1;
+    if (k > colptr[j] - 1 && r <= rowval[k - 1]) flags |= 2;
+    keys[k] = ((uint64_t)(r - 1) << cb) | (uint64_t)j;
+    vals[k] = (int32_t)k;
+  }
+  if (flags) atomicOr(bad, flags);
+}
+
+__global__ __launch_bounds__(kPT) void k_csr_cols(const uint64_t* __restrict__ u, int64_t nnz, uint64_t cmask,
+                                                  int32_t* __restrict__ col) {
+  const int64_t k = (int64_t)blockIdx.x * kPT + threadIdx.x;
+  if (k < nnz) col[k] = (int32_t)(u[k] & cmask);
+}
+
 template <class T>
 hipError_t upload(DevBuf& b, const T* h, size_t n, hipStream_t s) {
   hipError_t e = b.alloc(std::max<size_t>(n, 1) * sizeof(T));
@@ -216,6 +241,51 @@ int device_pattern(hfk_ctx* ctx, hipStream_t s, const int32_t* cell_dofs, int nd
   if (ncell_keys > 0)
     hipLaunchKernelGGL(k_nzpos, dim3(grid_of(ncell_keys)), dim3(kPT), 0, s, dofs_d.as<int32_t>(), nd, ncells,
                        out.colptr.as<int64_t>(), out.rowval.as<int32_t>(), out.nzpos.as<int32_t>());
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(s));  // the local buffers are freed on return
+  return HFK_OK;
+}
+
+int device_csc_to_csr(hfk_ctx* ctx, hipStream_t s, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                      DevBuf& rp, DevBuf& col, DevBuf& perm, int* bad) {
+  const int64_t nnz = colptr[n] - 1;
+  const int cb = bit_width((uint64_t)std::max<int64_t>(n - 1, 1));  // columns 0..n-1
+  const int rb = bit_width((uint64_t)std::max<int64_t>(n - 1, 1));  // rows 0..n-1
+  DevBuf cp_d, rv_d, keys, sorted, vals, vals_s, tmp, bad_d;
+  HIPCHK(ctx, upload(cp_d, colptr, (size_t)n + 1, s));
+  HIPCHK(ctx, upload(rv_d, rowval, (size_t)nnz, s));
+  HIPCHK(ctx, keys.alloc(std::max<int64_t>(nnz, 1) * sizeof(uint64_t)));
+  HIPCHK(ctx, sorted.alloc(std::max<int64_t>(nnz, 1) * sizeof(uint64_t)));
+  HIPCHK(ctx, vals.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  HIPCHK(ctx, vals_s.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  HIPCHK(ctx, bad_d.alloc(sizeof(int)));
+  HIPCHK(ctx, hipMemsetAsync(bad_d.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_csc_keys, dim3(grid_of(n)), dim3(kPT), 0, s, n, cp_d.as<int64_t>(), rv_d.as<int64_t>(), cb,
+                     keys.as<uint64_t>(), vals.as<int32_t>(), bad_d.as<int>());
+  HIPCHK(ctx, hipGetLastError());
+  int flags = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&flags, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  *bad = flags;
+  if (flags) return HFK_OK;
+  rv_d.reset();
+  HIPCHK(ctx, rp.alloc((size_t)(n + 1) * sizeof(int64_t)));
+  HIPCHK(ctx, col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  HIPCHK(ctx, perm.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  if (nnz > 0) {
+    // rows in the high bits: the sort gives the CSR order, columns ascending within a row (the keys are distinct)
+    size_t tb = 0;
+    HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, tb, keys.as<uint64_t>(), sorted.as<uint64_t>(), vals.as<int32_t>(),
+                                          vals_s.as<int32_t>(), (size_t)nnz, 0, rb + cb, s));
+    HIPCHK(ctx, tmp.alloc(std::max<size_t>(tb, 1)));
+    HIPCHK(ctx, rocprim::radix_sort_pairs(tmp.p, tb, keys.as<uint64_t>(), sorted.as<uint64_t>(), vals.as<int32_t>(),
+                                          vals_s.as<int32_t>(), (size_t)nnz, 0, rb + cb, s));
+    hipLaunchKernelGGL(k_csr_cols, dim3(grid_of(nnz)), dim3(kPT), 0, s, sorted.as<uint64_t>(), nnz,
+                       ((uint64_t)1 << cb) - 1, col.as<int32_t>());
+    HIPCHK(ctx, hipMemcpyAsync(perm.p, vals_s.p, (size_t)nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(k_colptr, dim3(grid_of(n + 1)), dim3(kPT), 0, s, sorted.as<uint64_t>(), nnz, n, cb,
+                     rp.as<int64_t>());  // the first position of every row
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(s));  // the local buffers are freed on return
   return HFK_OK;

@@ -393,25 +393,6 @@ int hfk_sys_create(hfk_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t
     if (colptr[j + 1] < colptr[j]) return set_err(ctx, HFK_EARG, "HyperFEMKernels: hfk_sys_create: colptr decreases");
   const int64_t nnz = colptr[n] - 1;
   if (nnz >= (int64_t)1 << 31) return set_err(ctx, HFK_EARG, "HyperFEMKernels: hfk_sys_create: more than 2^31 - 1 entries");
-  // CSR of the same matrix: row counts, then the entries column by column (rows ascend within a CSR row because the
-  // columns are visited in order); perm[k] = the CSC position of CSR entry k
-  std::vector<int64_t> rp(n + 1, 0);
-  for (int64_t k = 0; k < nnz; ++k) {
-    const int64_t r = rowval[k];
-    if (r < 1 || r > n) return set_err(ctx, HFK_EARG, "HyperFEMKernels: hfk_sys_create: row index out of range");
-    rp[r]++;
-  }
-  for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
-  std::vector<int32_t> col(nnz), perm(nnz);
-  std::vector<int64_t> next(rp.begin(), rp.end() - 1);
-  for (int64_t j = 0; j < n; ++j)
-    for (int64_t k = colptr[j] - 1; k < colptr[j + 1] - 1; ++k) {
-      if (k > colptr[j] - 1 && rowval[k] <= rowval[k - 1])
-        return set_err(ctx, HFK_EARG, "HyperFEMKernels: hfk_sys_create: rows of a column must ascend");
-      const int64_t dst = next[rowval[k] - 1]++;
-      col[dst] = (int32_t)j;
-      perm[dst] = (int32_t)k;
-    }
   auto* A = new hfk_sys();
   A->ctx = ctx;
   A->n = n;
@@ -421,15 +402,16 @@ int hfk_sys_create(hfk_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t
     return rc;
   };
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(set_err(ctx, HFK_EHIP, "HyperFEMKernels: hipSetDevice failed"));
-  if (A->rp.alloc((n + 1) * sizeof(int64_t)) != hipSuccess || A->col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)) != hipSuccess ||
-      A->perm.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)) != hipSuccess ||
-      A->val.alloc(std::max<int64_t>(nnz, 1) * sizeof(double)) != hipSuccess ||
-      A->t.alloc(n * sizeof(double)) != hipSuccess || A->bad.alloc(sizeof(int)) != hipSuccess)
+  // CSR of the same matrix, built on the device: rp, the columns of every row (ascending) and perm[k] = the CSC
+  // position of CSR entry k
+  int bad = 0;
+  int rc = device_csc_to_csr(ctx, ctx->stream, n, colptr, rowval, A->rp, A->col, A->perm, &bad);
+  if (rc) return fail(rc);
+  if (bad & 1) return fail(set_err(ctx, HFK_EARG, "HyperFEMKernels: hfk_sys_create: row index out of range"));
+  if (bad & 2) return fail(set_err(ctx, HFK_EARG, "HyperFEMKernels: hfk_sys_create: rows of a column must ascend"));
+  if (A->val.alloc(std::max<int64_t>(nnz, 1) * sizeof(double)) != hipSuccess || A->t.alloc(n * sizeof(double)) != hipSuccess ||
+      A->bad.alloc(sizeof(int)) != hipSuccess)
     return fail(set_err(ctx, HFK_EHIP, "HyperFEMKernels: hfk_sys_create: device allocation failed"));
-  if (hipMemcpy(A->rp.p, rp.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
-      (nnz > 0 && hipMemcpy(A->col.p, col.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) ||
-      (nnz > 0 && hipMemcpy(A->perm.p, perm.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess))
-    return fail(set_err(ctx, HFK_EHIP, "HyperFEMKernels: hfk_sys_create: copy to the device failed"));
   *out = A;
   return HFK_OK;
 }

@@ -59,7 +59,11 @@ def node_blocks(cell_dofs, node_table, slot_field, mode: str = "coupled") -> np.
     flat = keys.reshape(-1, keys.shape[2])
     has_prim = (flat[:, prim] > 0).any(axis=1)
     has_any = (flat > 0).any(axis=1)
-    blocks_k = _first_seen(flat[has_prim])
+    pflat = flat[has_prim]
+    first, inv = _row_groups(pflat)
+    rank = np.empty(first.size, dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(first.size)
+    blocks_k = pflat[first[np.argsort(first, kind="stable")]]
     blocks = np.zeros((blocks_k.shape[0], BLOCK), dtype=np.int32)
     blocks[:, :blocks_k.shape[1]] = blocks_k
     orphan = has_any & ~has_prim  # only non-primary dofs free at this node
@@ -67,7 +71,7 @@ def node_blocks(cell_dofs, node_table, slot_field, mode: str = "coupled") -> np.
         return _packed(blocks)
     # block of each (cell, node) with a primary dof, and the first (cell, node) of each orphan key
     block_of = np.full(flat.shape[0], -1, dtype=np.int64)
-    order = _first_seen_index(flat[has_prim])
+    order = rank[inv]
     block_of[np.nonzero(has_prim)[0]] = order
     okeys, ofirst = _unique_first(flat[orphan])
     oidx = np.nonzero(orphan)[0]
@@ -107,10 +111,40 @@ def _packed(blocks):
     return np.ascontiguousarray(blocks, dtype=np.int32)
 
 
+def _row_groups(k):
+    """Distinct rows of k: (first, inv) with first[g] the index of the first occurrence of distinct row g and
+    inv[i] the group of row i.  A stable lexicographic sort of the rows (np.unique over whole rows sorts them as
+    byte strings, seconds for the 5M rows at 40^3 x 6 tetrahedra); equal rows keep their order within a group, so the
+    first of every group is its first occurrence."""
+    k = np.asarray(k, dtype=np.int64)
+    if k.shape[0] == 0:
+        return np.zeros(0, dtype=np.int64), np.zeros(0, dtype=np.int64)
+    bits = max(int(k.max()).bit_length(), 1)
+    if bits * k.shape[1] <= 63:  # the whole row in one non-negative int64 key
+        pk = np.zeros(k.shape[0], dtype=np.int64)
+        for c in range(k.shape[1]):
+            pk = (pk << bits) | k[:, c]
+        o = np.argsort(pk, kind="stable")
+        ks = pk[o]
+        new = np.ones(ks.shape[0], dtype=bool)
+        new[1:] = ks[1:] != ks[:-1]
+    else:  # two 31-bit ids per key (dof ids are int32)
+        if k.shape[1] % 2:
+            k = np.concatenate([k, np.zeros((k.shape[0], 1), dtype=np.int64)], axis=1)
+        pk = (k[:, 0::2] << 31) | k[:, 1::2]
+        o = np.lexsort(pk.T[::-1])
+        ks = pk[o]
+        new = np.ones(ks.shape[0], dtype=bool)
+        new[1:] = (ks[1:] != ks[:-1]).any(axis=1)
+    inv = np.empty(k.shape[0], dtype=np.int64)
+    inv[o] = np.cumsum(new) - 1
+    return o[new], inv
+
+
 def _unique_first(k):
     """Distinct rows of k and the index of their first occurrence."""
-    u, first = np.unique(k, axis=0, return_index=True)
-    return u, first
+    first, _ = _row_groups(k)
+    return np.asarray(k)[first], first
 
 
 def _first_seen(k):
@@ -121,10 +155,10 @@ def _first_seen(k):
 
 def _first_seen_index(k):
     """For every row of k, the first-seen rank of its distinct value."""
-    u, first, inv = np.unique(k, axis=0, return_index=True, return_inverse=True)
+    first, inv = _row_groups(k)
     rank = np.empty(first.size, dtype=np.int64)
     rank[np.argsort(first, kind="stable")] = np.arange(first.size)
-    return rank[inv.reshape(-1)]
+    return rank[inv]
 
 
 def taylor_hood_table():
@@ -191,12 +225,15 @@ def taylor_hood_vanka_patches(space):
     members = [vd[bounds[k]:bounds[k + 1]] for k in range(npatch)]
     empty = [k for k in range(npatch) if members[k].size == 0]
     if empty:  # the velocity of every node of the cells holding the vertex
-        cell_rows = {}
-        for r, k in zip(rows // 4, pid_rows):
-            cell_rows.setdefault(int(k), []).append(int(r))
-        for k in empty:
-            cells = np.array(cell_rows[k])
-            v = np.stack([cd[cells, c * 10:(c + 1) * 10] for c in range(3)], axis=-1).reshape(-1)  # (cell, node, c)
+        emp = np.asarray(empty, dtype=np.int64)
+        sel = np.isin(pid_rows, emp)
+        kk, cc = pid_rows[sel], (rows // 4)[sel]
+        srt = np.argsort(kk, kind="stable")  # the cells of each patch stay in cell order
+        kk, cc = kk[srt], cc[srt]
+        lo, hi = np.searchsorted(kk, emp, side="left"), np.searchsorted(kk, emp, side="right")
+        vel_of = np.stack([cd[:, c * 10:(c + 1) * 10] for c in range(3)], axis=-1)  # (cell, node, c)
+        for i, k in enumerate(empty):
+            v = vel_of[cc[lo[i]:hi[i]]].reshape(-1)
             v = v[v > 0]
             _, f = np.unique(v, return_index=True)
             members[k] = v[np.sort(f)]
