@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include <cmath>
+#include <cstdlib>
 
 #include "hfk_internal.hpp"
 #include "hfk_point.hpp"
@@ -260,29 +261,54 @@ __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __res
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// jacobian!: element matrices from the prepared FP64 tangents, one work-group per cell of a colour, summed
-// into the nonzeros through nzpos (element_jacobian_mm!/scatter_jacobian!, src/assembly.jl:1316-1408).
+// jacobian!: element matrices from the prepared FP64 tangents, per cell of a colour, summed into the nonzeros
+// through nzpos (element_jacobian_mm!/scatter_jacobian!, src/assembly.jl:1316-1408).
 // Ke[(n,c),(m,k)] = sum_q sum_K sum_M dN_n/dxi_K(q) D_q[(K,c),(M,k)] dN_m/dxi_M(q), in two steps per point:
 //   H[K][c][(m,k)] = sum_M D_q[(K,c),(M,k)] dN_m/dxi_M(q);   Ke[(n,c),(m,k)] += sum_K dN_n/dxi_K(q) H[K][c][(m,k)]
-template <int Q, int P, int NT>
+// The cell's tangents at all its points are loaded into LDS first (one round of loads instead of one per point).
+// SPLIT > 1: the cell's columns (m,k) are shared out over SPLIT work-groups (each forms H for its columns only),
+// for colours with too few cells to occupy the GPU with one work-group per cell.  The sums of every entry are
+// formed in the same order whatever the split.
+template <int Q, int P, int NT, int SPLIT>
 __global__ __launch_bounds__(NT) void k_assemble(OpDev op, const double* __restrict__ D, double* __restrict__ nzval,
                                                  int batch0) {
   using Dm = Dims<Q, P>;
-  constexpr int C = Dm::C, NN = Dm::NN, ND = 3 * NN, NSLOT = Dm::NSLOT;
-  constexpr int NE = (ND * ND + NT - 1) / NT;
-  __shared__ double H[9 * ND];
+  constexpr int C = Dm::C, NN = Dm::NN, ND = 3 * NN, NSLOT = Dm::NSLOT, NP = Dm::NP;
+  constexpr int NCOL = (ND + SPLIT - 1) / SPLIT;  // columns of one work-group, at most
+  // Each thread accumulates 3 x 3 blocks of entries: 3 consecutive rows (n, c) of one component c by 3
+  // consecutive columns, so that the 9 gradient values and 9 H values it reads per point serve 27 products.
+  constexpr int RB = (NN + 2) / 3;        // row blocks per component
+  constexpr int NRB = 3 * RB;             // row blocks
+  constexpr int NCB = (NCOL + 2) / 3;     // column blocks, at most
+  constexpr int NBLK = (NRB * NCB + NT - 1) / NT;  // blocks per thread
+  __shared__ double Dl[NP * 81];  // Dl[q * 81 + e]: entry e of the cell's tangent at point q
+  __shared__ double H[9 * NCOL];
   __shared__ double G[3 * NN];  // dN_t/dxi_K at the current point
-  __shared__ double Dq[81];
   const int tid = threadIdx.x;
-  const int batch = batch0 + blockIdx.x / C, cell = blockIdx.x % C;
+  const int part = SPLIT > 1 ? (int)(blockIdx.x % SPLIT) : 0;
+  const int cb = (int)(blockIdx.x / SPLIT);
+  const int batch = batch0 + cb / C, cell = cb % C;
+  const int col0 = part * NCOL, ncol = ND - col0 < NCOL ? ND - col0 : NCOL;
   const int32_t* ids = op.ids + ((size_t)batch * C + cell) * ND;
   bool pad = true;
   for (int j = 0; j < ND; ++j) pad = pad && ids[j] == 0;
   if (pad) return;  // padding cell of the last batch of a colour (uniform over the work-group)
-  double acc[NE];
+  {
+    const double* Dc = D + (size_t)batch * 81 * NSLOT + cell * NP;
+    for (int it = tid; it < 81 * NP; it += NT) {
+      const int e = it / NP, q = it - e * NP;  // consecutive threads read consecutive points of one entry
+      Dl[q * 81 + e] = Dc[(size_t)e * NSLOT + q];
+    }
+  }
+  const int ncb = (ncol + 2) / 3;
+  double acc[NBLK][3][3];
 #pragma unroll
-  for (int e = 0; e < NE; ++e) acc[e] = 0.0;
-  for (int q = 0; q < Dm::NP; ++q) {
+  for (int k = 0; k < NBLK; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc[k][i][j] = 0.0;
+  for (int q = 0; q < NP; ++q) {
     const int a = q % P, b = (q / P) % P, cc = q / (P * P);
     for (int it = tid; it < 3 * NN; it += NT) {
       const int t = it / 3, K = it - 3 * t;
@@ -290,39 +316,61 @@ __global__ __launch_bounds__(NT) void k_assemble(OpDev op, const double* __restr
       const double na = op.tab.N[a * Q + i], nb = op.tab.N[b * Q + j], nc = op.tab.N[cc * Q + k];
       G[it] = K == 0 ? op.tab.dN[a * Q + i] * nb * nc : K == 1 ? na * op.tab.dN[b * Q + j] * nc : na * nb * op.tab.dN[cc * Q + k];
     }
-    for (int it = tid; it < 81; it += NT) Dq[it] = D[(size_t)batch * 81 * NSLOT + (size_t)it * NSLOT + cell * Dm::NP + q];
     __syncthreads();
-    for (int it = tid; it < 9 * ND; it += NT) {
-      const int Kc = it % 9, mk = it / 9;  // H[Kc + 9 mk], Kc = K + 3c, mk = m + NN k (tensor order)
+    const double* Dq = Dl + q * 81;
+    for (int it = tid; it < 9 * ncol; it += NT) {
+      const int Kc = it % 9, mk = col0 + it / 9;  // H[Kc + 9 (mk - col0)], Kc = K + 3c, mk = m + NN k (tensor order)
       const int k = mk / NN, m = mk - k * NN;
-      double s = 0;
+      double sum = 0;
 #pragma unroll
-      for (int M = 0; M < 3; ++M) s += Dq[Kc + 9 * (M + 3 * k)] * G[3 * m + M];
-      H[it] = s;
+      for (int M = 0; M < 3; ++M) sum += Dq[Kc + 9 * (M + 3 * k)] * G[3 * m + M];
+      H[it] = sum;
     }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
-      const int idx = tid + e * NT;
-      if (idx < ND * ND) {
-        const int nc_ = idx % ND, mk = idx / ND;  // row (n, c) and column (m, k), tensor order c*NN + n
-        const int c = nc_ / NN, n = nc_ - c * NN;
-        double s = 0;
+    for (int kb = 0; kb < NBLK; ++kb) {
+      const int blk = tid + kb * NT;
+      if (blk < NRB * ncb) {
+        const int rb = blk % NRB, cbk = blk / NRB;
+        const int c = rb / RB, n0 = 3 * (rb - c * RB), l0 = 3 * cbk;
+        double g[3][3], h[3][3];  // g[i][K] = G of row n0 + i, h[K][j] = H of column l0 + j
 #pragma unroll
-        for (int K = 0; K < 3; ++K) s += G[3 * n + K] * H[(K + 3 * c) + 9 * mk];
-        acc[e] += s;
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int K = 0; K < 3; ++K) g[i][K] = n0 + i < NN ? G[3 * (n0 + i) + K] : 0.0;
+#pragma unroll
+        for (int K = 0; K < 3; ++K)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) h[K][j] = l0 + j < ncol ? H[(K + 3 * c) + 9 * (l0 + j)] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            double sum = 0;
+#pragma unroll
+            for (int K = 0; K < 3; ++K) sum += g[i][K] * h[K][j];
+            acc[kb][i][j] += sum;
+          }
       }
     }
     __syncthreads();
   }
   const int32_t* pos = op.nzpos + ((size_t)batch * C + cell) * ND * ND;
 #pragma unroll
-  for (int e = 0; e < NE; ++e) {
-    const int idx = tid + e * NT;
-    if (idx < ND * ND) {
-      const int ii = op.loc_of[idx % ND], jj = op.loc_of[idx / ND];
-      const int p = pos[ii + ND * jj];
-      if (p > 0) nzval[p - 1] += acc[e];  // race-free: the cells of one colour share no free test row
+  for (int kb = 0; kb < NBLK; ++kb) {
+    const int blk = tid + kb * NT;
+    if (blk < NRB * ncb) {
+      const int rb = blk % NRB, cbk = blk / NRB;
+      const int c = rb / RB, n0 = 3 * (rb - c * RB), l0 = 3 * cbk;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (n0 + i >= NN || l0 + j >= ncol) continue;
+          const int ii = op.loc_of[c * NN + n0 + i], jj = op.loc_of[col0 + l0 + j];
+          const int p = pos[ii + ND * jj];
+          if (p > 0) nzval[p - 1] += acc[kb][i][j];  // race-free: the cells of one colour share no free test row
+        }
     }
   }
 }
@@ -657,7 +705,20 @@ template <int Q, int P>
 hipError_t assemble_qp(const OpDev& op, const double* D, double* nzval, BatchRange r, hipStream_t s) {
   constexpr int C = Dims<Q, P>::C;
   constexpr int NT = Q >= 4 ? 1024 : 256;
-  hipLaunchKernelGGL((k_assemble<Q, P, NT>), dim3(r.count * C), dim3(NT), 0, s, op, D, nzval, r.first);
+  // Q2 and Q3 cells: the columns of a cell always in 4 work-groups (with all 81 x 81 or 192 x 192 entries in one,
+  // the accumulators take every register: one wave per SIMD, and spills at Q3); Q1 cells: split only when a colour
+  // has too few cells to occupy the GPU with one work-group per cell
+  const int64_t cells = (int64_t)r.count * C;
+  if constexpr (Q >= 3) {
+    hipLaunchKernelGGL((k_assemble<Q, P, NT, 4>), dim3(r.count * C * 4), dim3(NT), 0, s, op, D, nzval, r.first);
+  } else {
+    if (cells < 1024)
+      hipLaunchKernelGGL((k_assemble<Q, P, NT, 4>), dim3(r.count * C * 4), dim3(NT), 0, s, op, D, nzval, r.first);
+    else if (cells < 2048)
+      hipLaunchKernelGGL((k_assemble<Q, P, NT, 2>), dim3(r.count * C * 2), dim3(NT), 0, s, op, D, nzval, r.first);
+    else
+      hipLaunchKernelGGL((k_assemble<Q, P, NT, 1>), dim3(r.count * C), dim3(NT), 0, s, op, D, nzval, r.first);
+  }
   return hipGetLastError();
 }
 
