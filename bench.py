@@ -667,6 +667,56 @@ def solve_leg(n, outer_storage, smoother_storage, degree=4, coarse_rtol=1e-2, co
     return out
 
 
+def pmg_gmres_leg(sizes=(10, 20, 32, 64), degree=2, rtol=1e-10):
+    """GMRES(50) iterations to rtol 1e-10 with one p-multigrid V-cycle per iteration at the reference's defaults,
+    across sizes: the reference quotes 25-28 GMRES iterations per Newton iteration from 25k to 957k dofs
+    (docs/guide/README.md:262-269: Chebyshev degree 2, range 20; the coarse level one GAMG V-cycle; Q2 nonlinear
+    elasticity on a unit cube, Krylov rtol 1e-10).  Here: Q2 neo-Hookean on n^3 (closed x faces clamped) at the
+    smooth state of amplitude 1e-2, b = -R(x), the FP64 factorised outer product, Chebyshev degree 2 on every level
+    and a fixed coarse V-cycle (Q1 on the working mesh with its quadrature, Q1 h-levels, a direct bottom)."""
+    import numpy as np
+    import torch
+
+    from hfk import HyperelasticForm, KernelOperator, MatrixFreeJacobian
+    from hfk.mesh import dirichlet_closed_x_faces, hex_space
+    from hfk.solvers import Chebyshev, MatrixFreeSolver, PMultigrid
+    from hfk.synthetic import smooth_field
+
+    out = {"krylov": "gmres", "restart": 50, "rtol": rtol, "chebyshev_degree": degree,
+           "reference_quoted_iterations": [25, 28], "reference_dofs": [25000, 957000], "sizes": []}
+    dev = torch.device("cuda:0")
+    for n in sizes:
+        fs = hex_space((n, n, n), 2, dirichlet=dirichlet_closed_x_faces)
+        u = smooth_field(2, amp=1e-2)
+        x = fs.free_values(u)
+        fop = KernelOperator(HyperelasticForm("neohookean", 2.0, 1.0, degree=4), fs, jacobian="matrixfree",
+                             dirichlet_values=fs.dirichlet_values(u))
+        lower = h_levels(fs, 4, dirichlet_closed_x_faces)
+        b = -fop.residual(np.zeros(fs.nfree), x)
+        mf = MatrixFreeJacobian(fop, "afac64")
+        pmg = PMultigrid([fop, *lower], Chebyshev(degree=degree), coarse="direct")
+        solver = MatrixFreeSolver("gmres", rtol=rtol, maxiter=500, restart=50, preconditioner=pmg)
+        xd = torch.from_numpy(x).to(dev)
+        bd = torch.from_numpy(b).to(dev)
+        sptr = torch.cuda.current_stream(dev).cuda_stream
+        for rep in range(2):  # the first pass builds the patterns and work buffers
+            sol = torch.zeros_like(bd)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mf.prepare_dev(xd.data_ptr(), sptr)
+            pmg.update()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            res = solver.solve_dev(sol.data_ptr(), mf, bd.data_ptr(), sptr)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        out["sizes"].append({"n": n, "free_dofs": fs.nfree, "levels": len(pmg.levels), "iterations": res.iterations,
+                             "converged": res.converged, "rel_residual": res.residual / res.rhs_norm,
+                             "update_s": t1 - t0, "solve_s": t2 - t1})
+        print(f"bench: p-MG GMRES n={n}: {res.iterations} iterations, {t2 - t1:.3f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def solve_summary(res):
     """The solve half of the metric in one place: seconds to 1e-8 of the 64^3 Q2 legs (FP64 p-MG, mixed precision,
     learned) and the held-out lognormal fields' speed-ups of the learned solve over the best FP64 p-MG."""
@@ -1081,8 +1131,8 @@ def main():
     ap.add_argument("--no-law", action="store_true", help="skip the run-time compiled law leg")
     ap.add_argument("--fields-full", action="store_true",
                     help="learned fields leg at the configured scale (20k training pairs, 100 held-out fields)")
-    ap.add_argument("--solve-legs", default="fp64,mixed,learned,fields,q3",
-                    help="solve legs: fp64, mixed, learned, fields, q3")
+    ap.add_argument("--solve-legs", default="fp64,mixed,learned,fields,gmres,q3",
+                    help="solve legs: fp64, mixed, learned, fields, gmres, q3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the Jv would shard by cell partition across GPUs); "
@@ -1246,6 +1296,10 @@ def main():
                                                               test_seeds=range(10000, 10100), progress=True)
             else:
                 res["learned_fields_32"] = learned_fields_leg(progress=True)
+        if "gmres" in legs:
+            print("bench: p-multigrid GMRES iterations to 1e-10 across sizes (Chebyshev degree 2)", file=sys.stderr,
+                  flush=True)
+            res["pmg_gmres_iterations"] = pmg_gmres_leg()
         if "q3" in legs:
             # Q3 on 48^3 (64 points per cell): FP32 Jv against FP64; mixed-precision solves (FP32 smoothers, FP64
             # outer FCG and coarse level) with mu = 1 and with a lognormal mu (sigma 1, correlation length 0.1)
