@@ -168,7 +168,7 @@ struct hfk_cheb {
   // assembled matrix, set up by its owner)
   bool csr_tangent_blocks = false;
   int nbuild = 0;
-  DevBuf r, d, tmp, start, V;
+  DevBuf r, d, tmp, start, V, hess, hpart;
   Dots dots;
   const void* T() const { return secondary ? op->tangents_s.p : op->tangents.p; }
   int storage() const { return secondary ? op->storage_s : op->storage; }
@@ -199,29 +199,42 @@ int arnoldi_bounds(hfk_op* op, hfk_cheb* ch, int k, hipStream_t s, double* lmin,
   HIPCHK(op->ctx, ch->V.alloc((size_t)(k + 1) * n * sizeof(double)));
   double* V = ch->V.as<double>();
   HIPCHK(op->ctx, vec_copy(n, ch->start.as<double>(), V, s));
-  std::vector<double> Hm((size_t)(k + 1) * k, 0.0);  // (k+1) x k column-major
+  // The Hessenberg matrix stays on the device until the last step: each step orthogonalises against the basis with
+  // classical Gram-Schmidt applied twice (two batched projections, as GMRES does), so no step waits for the host;
+  // the matrix is read back once.  A zero or non-finite norm leaves the later columns non-finite: the host stops
+  // at the first such step, as a step-by-step process would.
+  const int ldh = k + 1;
+  HIPCHK(op->ctx, ch->hess.alloc((size_t)(ldh * k + 2 * ldh + 4) * sizeof(double)));
+  HIPCHK(op->ctx, ch->hpart.alloc(std::max(multi_dot_partial_doubles(ldh), dot_partial_doubles()) * sizeof(double)));
+  double* Hd = ch->hess.as<double>();
+  double* c1 = Hd + (size_t)ldh * k;
+  double* c2 = c1 + ldh;
+  double* ww = c2 + ldh;
+  double* part = ch->hpart.as<double>();
+  HIPCHK(op->ctx, launch_fill(Hd, (int64_t)ldh * k, 0.0, s));
   int rc;
-  int steps = k;
   for (int j = 0; j < k; ++j) {
     double* vj = V + (size_t)j * n;
     double* w = V + (size_t)(j + 1) * n;
     if ((rc = blocks_apply(op, vj, ch->tmp.as<double>(), s))) return rc;
     if ((rc = cheb_op_apply(ch, ch->tmp.as<double>(), w, 1.0, false, s))) return rc;
-    for (int i = 0; i <= j; ++i) {
-      double hij;
-      if ((rc = ch->dots.run(op->ctx, s, n, V + (size_t)i * n, w, &hij))) return rc;
-      Hm[i + (k + 1) * j] = hij;
-      HIPCHK(op->ctx, vec_axpby(n, -hij, V + (size_t)i * n, 1.0, w, s));
-    }
-    double ww;
-    if ((rc = ch->dots.run(op->ctx, s, n, w, w, &ww))) return rc;
-    const double hn = std::sqrt(ww);
-    Hm[j + 1 + (k + 1) * j] = hn;
+    HIPCHK(op->ctx, multi_dot(n, V, n, j + 1, w, part, c1, s));
+    HIPCHK(op->ctx, multi_axpy(n, V, n, j + 1, c1, -1.0, w, s));
+    HIPCHK(op->ctx, multi_dot(n, V, n, j + 1, w, part, c2, s));
+    HIPCHK(op->ctx, multi_axpy(n, V, n, j + 1, c2, -1.0, w, s));
+    HIPCHK(op->ctx, vec_dots(n, w, w, nullptr, nullptr, nullptr, nullptr, part, ww, s));
+    HIPCHK(op->ctx, arnoldi_column(n, j, ldh, c1, c2, ww, Hd, w, s));
+  }
+  std::vector<double> Hm((size_t)ldh * k, 0.0);  // (k+1) x k column-major
+  HIPCHK(op->ctx, hipMemcpyAsync(Hm.data(), Hd, Hm.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(op->ctx, hipStreamSynchronize(s));
+  int steps = k;
+  for (int j = 0; j < k; ++j) {
+    const double hn = Hm[j + 1 + ldh * j];
     if (!(hn > 0) || !std::isfinite(hn)) {
       steps = j + 1;
       break;
     }
-    HIPCHK(op->ctx, vec_axpby(n, 0.0, w, 1.0 / hn, w, s));
   }
   // truncate at the first (near) breakdown, as the reference
   int jj = steps;

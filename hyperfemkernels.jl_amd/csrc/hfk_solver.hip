@@ -104,6 +104,22 @@ __global__ void k_multi_axpy(int64_t n, const double* __restrict__ V, int64_t ld
   }
 }
 
+// Arnoldi step j (classical Gram-Schmidt applied twice): column j of the Hessenberg matrix, H[i + ldh j] =
+// c1[i] + c2[i] for i <= j and H[j + 1 + ldh j] = sqrt(ww), kept on the device
+__global__ void k_arnoldi_column(int j, int ldh, const double* __restrict__ c1, const double* __restrict__ c2,
+                                 const double* __restrict__ ww, double* __restrict__ H) {
+  const int i = threadIdx.x;
+  if (i <= j) H[i + (size_t)ldh * j] = c1[i] + c2[i];
+  if (i == 0) H[j + 1 + (size_t)ldh * j] = sqrt(ww[0]);
+}
+
+// w /= sqrt(ww) (a zero or non-finite norm leaves w non-finite: the host truncates the Arnoldi process there)
+__global__ void k_scale_inv_sqrt(int64_t n, const double* __restrict__ ww, double* __restrict__ w) {
+  const double sc = 1.0 / sqrt(ww[0]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = sc * w[i];
+}
+
 __global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = a * x[i] + b * y[i];
@@ -574,6 +590,13 @@ hipError_t multi_dot(int64_t n, const double* V, int64_t ldv, int K, const doubl
 hipError_t multi_axpy(int64_t n, const double* V, int64_t ldv, int K, const double* coef, double alpha, double* w,
                       hipStream_t s) {
   hipLaunchKernelGGL(k_multi_axpy, grid_for(n), dim3(256), 0, s, n, V, ldv, K, coef, alpha, w);
+  return hipGetLastError();
+}
+
+hipError_t arnoldi_column(int64_t n, int j, int ldh, const double* c1, const double* c2, const double* ww, double* H,
+                          double* w, hipStream_t s) {
+  hipLaunchKernelGGL(k_arnoldi_column, dim3(1), dim3((unsigned)(((j + 1 + 63) / 64) * 64)), 0, s, j, ldh, c1, c2, ww, H);
+  hipLaunchKernelGGL(k_scale_inv_sqrt, grid_for(n), dim3(256), 0, s, n, ww, w);
   return hipGetLastError();
 }
 

@@ -24,6 +24,9 @@ hipError_t multi_dot(int64_t n, const double* V, int64_t ldv, int K, const doubl
                      hipStream_t s);
 hipError_t multi_axpy(int64_t n, const double* V, int64_t ldv, int K, const double* coef, double alpha, double* w,
                       hipStream_t s);
+// Arnoldi step j on the device: H[i + ldh j] = c1[i] + c2[i] (i <= j), H[j + 1 + ldh j] = sqrt(ww[0]), w /= sqrt(ww[0])
+hipError_t arnoldi_column(int64_t n, int j, int ldh, const double* c1, const double* c2, const double* ww, double* H,
+                          double* w, hipStream_t s);
 // node-block ELL product y = alpha A x + beta y (val FP64, or FP32 when fp32) and its value fill from the CSC
 // one block-Jacobi PCG iteration on the node-block ELL matrix (FP64 values, or FP32 values with FP64 accumulation) in
 // three launches: rz in sc[slot] (the new rz goes to sc[next], r.r to sc[kRR]); part holds 3 ceil(nb / 256) partial
