@@ -197,15 +197,15 @@ __global__ __launch_bounds__(kThreads) void k_apply(OpDev op, const void* __rest
 // ---------------------------------------------------------------------------------------------------------
 // BlockJacobi from the stored tangents (_blocks_from_tangents!, src/preconditioners.jl:234-263), the cells of
 // one colour: B_n[ci,cj] += sum_q sum_K sum_M dN_n/dxi_K D_q[(K,ci),(M,cj)] dN_n/dxi_M.  One thread per
-// (cell, node); the tangents of the current point of the batch's cells are staged in LDS.
+// (cell, node); the factorised storages form B_n from their factors directly, the others stage the tangents of the
+// current point of the batch's cells in LDS.
 template <int Q, int P, class TS, int LAY>
 __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __restrict__ tangents,
                                                      const int32_t* __restrict__ block_of, double* __restrict__ Bk,
                                                      int batch0) {
   using Dm = Dims<Q, P>;
-  constexpr bool SYM = LAY == kLaySym;
+  constexpr bool SYM = LAY == kLaySym, FAC = LAY == kLayFac || LAY == kLayAfac, AFAC = LAY == kLayAfac;
   constexpr int C = Dm::C, NN = Dm::NN, NP = Dm::NP, NSLOT = Dm::NSLOT, NE = lay_entries(LAY);
-  __shared__ double Dl[C * 81];
   const int tid = threadIdx.x;
   const int batch = batch0 + blockIdx.x;
   const bool active = tid < C * NN;
@@ -215,40 +215,86 @@ __global__ __launch_bounds__(kThreads) void k_blocks(OpDev op, const void* __res
   double acc[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) acc[e] = 0.0;
-  for (int q = 0; q < NP; ++q) {
+  if constexpr (FAC) {
+    // The factorised tangent D[(K,c),(M,k)] = G_KM d_ck + b H_Kk H_Mc + l H_Kc H_Mk contracted with the node's
+    // gradient g on both sides: B_n = sum_q (g'G g) I + (b + l) u u', u_c = sum_K g_K H_Kc.  The batch's factors
+    // are staged in LDS once (entry-major, as stored: coalesced), then every (cell, node) thread walks its points.
+    __shared__ double Fl[NE * NSLOT];
+    for (int it = tid; it < NE * NSLOT; it += kThreads) Fl[it] = static_cast<double>(ld_stream(Dg + it));
     __syncthreads();
-    for (int it = tid; it < C * 81; it += kThreads) {
-      const int c2 = it / 81, e = it - c2 * 81, ii = e % 9, jj = e / 9;
-      if (LAY == kLayFac) {
-        Dl[it] = fac_entry(Dg + c2 * NP + q, NSLOT, ii, jj);
-      } else if (LAY == kLayAfac) {
-        Dl[it] = afac_entry(Dg + c2 * NP + q, NSLOT, op.cellgeo + ((size_t)batch * C + c2) * 8,
-                            point_weight<P>(op.tab, q), op.lam, ii, jj);
-      } else {
+    if (active) {
+      double S[6] = {0, 0, 0, 0, 0, 0}, dj = 0;
+      if constexpr (AFAC) {  // the cell's metric iJt'iJt and |det J|
+        const double* cg = op.cellgeo + ((size_t)batch * C + cell) * 8;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) S[e] = cg[e];
+        dj = cg[6];
+      }
+      for (int q = 0; q < NP; ++q) {
+        const int a = q % P, b = (q / P) % P, cc = q / (P * P);
+        const double na = op.tab.N[a * Q + i], nb = op.tab.N[b * Q + j], nc = op.tab.N[cc * Q + k];
+        const double gr[3] = {op.tab.dN[a * Q + i] * nb * nc, na * op.tab.dN[b * Q + j] * nc, na * nb * op.tab.dN[cc * Q + k]};
+        const double* f = Fl + cell * NP + q;  // factor e of this point: f[e * NSLOT]
+        double Gm[6], bb;
+        constexpr int HO = AFAC ? 0 : 6;  // first factor of H
+        if constexpr (AFAC) {
+          const double g = f[10 * NSLOT];
+#pragma unroll
+          for (int e = 0; e < 6; ++e) Gm[e] = g * S[e];
+          bb = f[9 * NSLOT] + op.lam * point_weight<P>(op.tab, q) * dj;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 6; ++e) Gm[e] = f[e * NSLOT];
+          bb = f[15 * NSLOT] + f[16 * NSLOT];
+        }
+        double u[3], gg = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double v = 0;
+#pragma unroll
+          for (int K = 0; K < 3; ++K) v += gr[K] * f[(HO + K + 3 * c) * NSLOT];
+          u[c] = v;
+        }
+#pragma unroll
+        for (int K = 0; K < 3; ++K)
+#pragma unroll
+          for (int M = 0; M < 3; ++M) gg += gr[K] * Gm[K <= M ? sym_index(K, M) : sym_index(M, K)] * gr[M];
+#pragma unroll
+        for (int cj = 0; cj < 3; ++cj)
+#pragma unroll
+          for (int ci = 0; ci < 3; ++ci) acc[ci + 3 * cj] += (ci == cj ? gg : 0.0) + bb * u[ci] * u[cj];
+      }
+    }
+  } else {  // full or symmetric storage: the batch's tangents at one point staged in LDS
+    __shared__ double Dl[C * 81];
+    for (int q = 0; q < NP; ++q) {
+      __syncthreads();
+      for (int it = tid; it < C * 81; it += kThreads) {
+        const int c2 = it / 81, e = it - c2 * 81, ii = e % 9, jj = e / 9;
         const int se = SYM ? (ii <= jj ? sym_index(ii, jj) : sym_index(jj, ii)) : e;
         Dl[it] = static_cast<double>(ld_stream(Dg + (size_t)se * NSLOT + c2 * NP + q));
       }
-    }
-    __syncthreads();
-    if (active) {
-      const int a = q % P, b = (q / P) % P, cc = q / (P * P);
-      const double na = op.tab.N[a * Q + i], nb = op.tab.N[b * Q + j], nc = op.tab.N[cc * Q + k];
-      const double G[3] = {op.tab.dN[a * Q + i] * nb * nc, na * op.tab.dN[b * Q + j] * nc, na * nb * op.tab.dN[cc * Q + k]};
-      const double* D = Dl + cell * 81;
+      __syncthreads();
+      if (active) {
+        const int a = q % P, b = (q / P) % P, cc = q / (P * P);
+        const double na = op.tab.N[a * Q + i], nb = op.tab.N[b * Q + j], nc = op.tab.N[cc * Q + k];
+        const double G[3] = {op.tab.dN[a * Q + i] * nb * nc, na * op.tab.dN[b * Q + j] * nc, na * nb * op.tab.dN[cc * Q + k]};
+        const double* D = Dl + cell * 81;
 #pragma unroll
-      for (int cj = 0; cj < 3; ++cj)
+        for (int cj = 0; cj < 3; ++cj)
 #pragma unroll
-        for (int ci = 0; ci < 3; ++ci) {
-          double s = 0;
+          for (int ci = 0; ci < 3; ++ci) {
+            double s = 0;
 #pragma unroll
-          for (int M = 0; M < 3; ++M) {
-            double u = 0;
+            for (int M = 0; M < 3; ++M) {
+              double u = 0;
 #pragma unroll
-            for (int K = 0; K < 3; ++K) u += G[K] * D[(K + 3 * ci) + 9 * (M + 3 * cj)];
-            s += u * G[M];
+              for (int K = 0; K < 3; ++K) u += G[K] * D[(K + 3 * ci) + 9 * (M + 3 * cj)];
+              s += u * G[M];
+            }
+            acc[ci + 3 * cj] += s;
           }
-          acc[ci + 3 * cj] += s;
-        }
+      }
     }
   }
   if (active) {
