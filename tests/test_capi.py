@@ -27,6 +27,15 @@ def test_exports_every_declared_symbol():
     assert lib.hfk_version() >= 1
 
 
+def test_julia_binding_calls_declared_symbols():
+    """Every entry point the Julia ccall layer binds is declared in include/hfk.h (and so exported, above)."""
+    src = open(os.path.join(ROOT, "hyperfemkernels.jl_amd", "julia", "HyperFEMKernelsMI355X.jl")).read()
+    called = set(re.findall(r":(hfk_[a-z0-9_]+)\b", src))
+    assert len(called) >= 20
+    missing = sorted(called - set(declared()))
+    assert not missing, missing
+
+
 def test_library_is_gfx950_only():
     from hfk import _lib
 
