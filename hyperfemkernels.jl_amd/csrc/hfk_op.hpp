@@ -68,6 +68,9 @@ struct NodeBlocks {
   std::vector<int32_t> dofs_h;  // 3*nblocks, free ids (1-based), 0 = fixed component
   DevBuf block_of;              // [slot][t] tensor order, -1: every component fixed
   DevBuf dofs, B, Binv;
+  // the inverses rounded to FP32 when the blocks come from FP32 tangents: the smoother's block steps read them
+  DevBuf Binv32;
+  bool fp32 = false;
   DevBuf bad;                   // int counter of non-finite inverted blocks
 };
 

@@ -129,11 +129,23 @@ int blocks_update(hfk_op* op, const void* T, int storage, hipStream_t s, bool* o
   if ((rc = robin_blocks(op, nb.block_of.as<int32_t>(), nb.B.as<double>(), s))) return rc;  // the Robin faces
   HIPCHK(op->ctx, hipMemsetAsync(nb.bad.p, 0, sizeof(int), s));
   HIPCHK(op->ctx, blocks_invert(nb.nblocks, nb.dofs.as<int32_t>(), nb.B.as<double>(), nb.Binv.as<double>(), nb.bad.as<int>(), s));
+  nb.fp32 = store_fp32(storage);  // a smoother on FP32 tangents reads the inverses in FP32
+  if (nb.fp32) {
+    HIPCHK(op->ctx, nb.Binv32.alloc((size_t)std::max<int64_t>(nb.nblocks, 1) * 9 * sizeof(float)));
+    HIPCHK(op->ctx, vec_to_fp32(nb.nblocks * 9, nb.Binv.as<double>(), nb.Binv32.as<float>(), s));
+  }
   int bad = 0;
   HIPCHK(op->ctx, hipMemcpyAsync(&bad, nb.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(op->ctx, hipStreamSynchronize(s));
   *ok = bad == 0;
   return HFK_OK;
+}
+
+// one block step of a smoother: z = M^-1 r; d = cd d + cz z; x += d (the FP32 inverses for FP32 tangents)
+hipError_t smoother_block_step(const NodeBlocks& nb, const double* r, double* d, double* x, double cd, double cz,
+                               hipStream_t s) {
+  if (nb.fp32) return blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv32.as<float>(), r, d, x, cd, cz, s);
+  return blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, cd, cz, s);
 }
 
 int blocks_apply(hfk_op* op, const double* r, double* y, hipStream_t s) {
@@ -308,7 +320,7 @@ int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, dou
     if ((rc = cheb_op_apply(ch, x, r, -1.0, true, s))) return rc;  // r = rhs - A x
   }
   // z = M^-1 r; d = z / theta; x += d
-  HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, 0.0, 1.0 / th, s));
+  HIPCHK(op->ctx, smoother_block_step(nb, r, d, x, 0.0, 1.0 / th, s));
   double rho = 1.0 / sg;
   const int k = ch->degree;
   for (int i = 1; i <= k; ++i) {
@@ -316,7 +328,7 @@ int cheb_smooth(hfk_cheb* ch, double* x, const double* rhs, bool zero_guess, dou
     if ((rc = cheb_op_apply(ch, d, r, -1.0, true, s))) return rc;  // r -= A d
     if (i == k) break;
     const double rn = 1.0 / (2.0 * sg - rho);
-    HIPCHK(op->ctx, blocks_step(nb.nblocks, nb.dofs.as<int32_t>(), nb.Binv.as<double>(), r, d, x, rn * rho, 2.0 * rn / de, s));
+    HIPCHK(op->ctx, smoother_block_step(nb, r, d, x, rn * rho, 2.0 * rn / de, s));
     rho = rn;
   }
   if (residual) HIPCHK(op->ctx, vec_copy(n, r, residual, s));

@@ -219,7 +219,8 @@ __global__ void k_invert_blocks(int64_t nb, const int32_t* __restrict__ dofs, co
 // _apply_blocks! (src/preconditioners.jl:327-342) fused with the Chebyshev direction update
 // (src/smoothers.jl:130-141):  z = Binv r on the block;  d = cd d + cz z;  x += d (if x)
 // With cd = 0, cz = 1 and x = null this is ldiv!(y = d, blocks, r).
-__global__ void k_block_step(int64_t nb, const int32_t* __restrict__ dofs, const double* __restrict__ Binv,
+template <class BT>
+__global__ void k_block_step(int64_t nb, const int32_t* __restrict__ dofs, const BT* __restrict__ Binv,
                              const double* __restrict__ r, double* __restrict__ d, double* __restrict__ x, double cd,
                              double cz) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
@@ -233,7 +234,7 @@ __global__ void k_block_step(int64_t nb, const int32_t* __restrict__ dofs, const
       if (id[i] <= 0) continue;
       double z = 0;
       for (int j = 0; j < 3; ++j)
-        if (id[j] > 0) z += Binv[(size_t)(i + 3 * j) * nb + k] * rv[j];
+        if (id[j] > 0) z += static_cast<double>(Binv[(size_t)(i + 3 * j) * nb + k]) * rv[j];
       const double dn = (cd == 0.0 ? 0.0 : cd * d[id[i] - 1]) + cz * z;
       d[id[i] - 1] = dn;
       if (x) x[id[i] - 1] += dn;
@@ -632,7 +633,20 @@ hipError_t blocks_invert(int64_t nb, const int32_t* dofs, const double* B, doubl
 }
 hipError_t blocks_step(int64_t nb, const int32_t* dofs, const double* Binv, const double* r, double* d, double* x,
                        double cd, double cz, hipStream_t s) {
-  hipLaunchKernelGGL(k_block_step, grid_for(nb), dim3(256), 0, s, nb, dofs, Binv, r, d, x, cd, cz);
+  hipLaunchKernelGGL(k_block_step<double>, grid_for(nb), dim3(256), 0, s, nb, dofs, Binv, r, d, x, cd, cz);
+  return hipGetLastError();
+}
+hipError_t blocks_step(int64_t nb, const int32_t* dofs, const float* Binv, const double* r, double* d, double* x,
+                       double cd, double cz, hipStream_t s) {
+  hipLaunchKernelGGL(k_block_step<float>, grid_for(nb), dim3(256), 0, s, nb, dofs, Binv, r, d, x, cd, cz);
+  return hipGetLastError();
+}
+__global__ void k_to_fp32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = static_cast<float>(a[i]);
+}
+hipError_t vec_to_fp32(int64_t n, const double* a, float* b, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_fp32, grid_for(n), dim3(256), 0, s, n, a, b);
   return hipGetLastError();
 }
 hipError_t csr_blocks(int64_t nb, const int32_t* dofs, const int32_t* colptr, const int32_t* rowval, const double* val,

@@ -46,6 +46,10 @@ hipError_t cg_direction(int64_t n, double* sc, const double* z, double* p, int f
 hipError_t blocks_invert(int64_t nb, const int32_t* dofs, const double* B, double* Binv, int* bad, hipStream_t s);
 hipError_t blocks_step(int64_t nb, const int32_t* dofs, const double* Binv, const double* r, double* d, double* x,
                        double cd, double cz, hipStream_t s);
+// the same with FP32 inverses (a smoother on FP32 tangents); b[i] = (float) a[i]
+hipError_t blocks_step(int64_t nb, const int32_t* dofs, const float* Binv, const double* r, double* d, double* x,
+                       double cd, double cz, hipStream_t s);
+hipError_t vec_to_fp32(int64_t n, const double* a, float* b, hipStream_t s);
 hipError_t csr_blocks(int64_t nb, const int32_t* dofs, const int32_t* colptr, const int32_t* rowval, const double* val,
                       double* B, hipStream_t s);
 hipError_t csc_t_spmv(int64_t ncols, const int32_t* colptr, const int32_t* rowval, const double* val, const double* x,
