@@ -379,7 +379,7 @@ struct OseenFaceDev {
   const double* xb;       // [face][nqb][7]
 };
 
-// One work-group per face of one colour (the colour of its cell, so faces of a launch share no free test dof):
+// One work-group per face of one face colour (the faces of a launch share no free dof of their cells):
 // the velocity block Kf[(i,a)][(j,b)] = delta_ij Mb_ab + ZA n_i n_j Mo_ab with
 //   Mb_ab = sum_dGb dS (-rho negpart(w.n)) Na Nb,   Mo_ab = sum_dGo dS Na Nb,
 // and the load -Pc n_i sum_dGo dS Na, added to the CSC values through the cell's nzpos and to the residual rows.
@@ -608,7 +608,7 @@ struct hfk_oseen {
   bool has_lap = false;
   // colour by colour with plain adds (bitwise reproducible), or one launch with FP64 atomic adds
   bool deterministic = true;
-  // outlet faces, ordered by the colour of their cell
+  // outlet faces, ordered by their colour (faces coloured among themselves by shared free dofs)
   int64_t nfaces = 0;
   int nqo = 0, nqb = 0;
   double Pc = 0, ZA = 0;
@@ -959,21 +959,38 @@ int hfk_oseen_set_outlet(hfk_oseen* op, int64_t nfaces, const int32_t* face_cell
   for (int64_t f = 0; f < nfaces; ++f)
     if (face_cell[f] < 0 || face_cell[f] >= op->ncells)
       return set_err(op->ctx, HFK_EDIM, "HyperFEMKernels: outlet face cell out of range");
-  // faces grouped by the colour of their cell and, within it, by their rank among the faces of that cell (0 for
-  // the first face of a cell, 1 for a second one, ...): the faces of one launch belong to different cells of one
-  // colour, so no two of them add into the same CSC entries
-  const size_t nc = op->color_start.size() - 1;
-  std::vector<int> rank((size_t)nfaces);
-  int maxrank = 0;
+  // faces coloured among themselves, greedily in face order: a face takes the lowest colour that no earlier face
+  // sharing a free dof of its cell holds (two faces of one cell share every dof), so the faces of one launch add
+  // into disjoint CSC entries and residual rows; the launches run in colour order after every cell colour
+  std::vector<int> fcol((size_t)nfaces, 0);
+  size_t ng = 0;
   {
-    std::vector<int> seen((size_t)op->ncells, 0);
+    std::vector<int64_t> dstart((size_t)op->nfree + 1, 0);  // free dof -> the faces touching it
+    auto cell_ids = [&](int64_t f) { return op->cell_dofs.data() + (size_t)face_cell[f] * kND; };
+    for (int64_t f = 0; f < nfaces; ++f)
+      for (int k = 0; k < kND; ++k)
+        if (cell_ids(f)[k] > 0) dstart[cell_ids(f)[k]]++;
+    for (int64_t v = 0; v < op->nfree; ++v) dstart[v + 1] += dstart[v];
+    std::vector<int64_t> dfill(dstart.begin(), dstart.end() - 1);
+    std::vector<int32_t> dface((size_t)dstart[op->nfree]);
+    for (int64_t f = 0; f < nfaces; ++f)
+      for (int k = 0; k < kND; ++k)
+        if (cell_ids(f)[k] > 0) dface[dfill[cell_ids(f)[k] - 1]++] = (int32_t)f;
+    std::vector<int64_t> stamp((size_t)nfaces + 1, -1);  // stamp[c] == f: colour c taken by a neighbour of f
     for (int64_t f = 0; f < nfaces; ++f) {
-      rank[f] = seen[face_cell[f]]++;
-      maxrank = std::max(maxrank, rank[f]);
+      for (int k = 0; k < kND; ++k) {
+        const int32_t id = cell_ids(f)[k];
+        if (id <= 0) continue;
+        for (int64_t e = dstart[id - 1]; e < dstart[id]; ++e)
+          if (dface[e] < f) stamp[fcol[dface[e]]] = f;
+      }
+      int c = 0;
+      while (stamp[c] == f) ++c;
+      fcol[f] = c;
+      ng = std::max(ng, (size_t)c + 1);
     }
   }
-  const size_t nr = (size_t)maxrank + 1, ng = nc * nr;
-  auto group = [&](int64_t f) { return (size_t)(op->color[face_cell[f]] - 1) * nr + (size_t)rank[f]; };
+  auto group = [&](int64_t f) { return (size_t)fcol[f]; };
   std::vector<int64_t> start(ng + 1, 0);
   for (int64_t f = 0; f < nfaces; ++f) start[group(f) + 1]++;
   for (size_t g = 0; g < ng; ++g) start[g + 1] += start[g];
