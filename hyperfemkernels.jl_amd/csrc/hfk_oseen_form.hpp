@@ -148,15 +148,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   constexpr bool X0 = any_nz<Form>(34, 34), XG = any_nz<Form>(35, 37), XH = any_nz<Form>(38, 40), XX = any_nz<Form>(41, 49);
   extern __shared__ double dyn[];
   __shared__ double Ke[kND][kND + 1];
-  __shared__ double Jit[9], Cm[2];
   const int tid = threadIdx.x;
   const int64_t cell = work_item(cells, op.ncells, chunk);
   if (cell < 0) return;  // uniform over the work-group, before any barrier
   const int np = op.np;
   double* Sh = dyn;                   // [q][f][kShV]
   double* Pk = dyn + np * 14 * kShV;  // [q][kFormCoefs + 1]: the form's factors, then dV
-  if (tid == 0) tet_geometry(op.coords + cell * 12, Jit, Cm[0], Cm[1]);
-  __syncthreads();
+  // the CSC positions of this thread's entries and the point data of its point are requested first, so that
+  // their latencies overlap; the affine map is then computed by every thread (no serial step, no barrier)
   constexpr int kPerThread = (kND * kND + 255) / 256;
   int32_t pk[kPerThread];
   double old[kPerThread];
@@ -168,16 +167,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       pk[k] = e < kND * kND ? pos[e] : 0;
     }
   }
+  double d[12];
   if (tid < np) {
     const double* fq = op.fields + ((size_t)cell * np + tid) * 9;
-    double d[12];
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] = fq[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) d[9 + i] = op.lap ? op.lap[((size_t)cell * np + tid) * 3 + i] : 0.0;
+  }
+  double Jit[9], adet, hc;
+  tet_geometry(op.coords + cell * 12, Jit, adet, hc);
+  if (tid < np) {
     double* k = Pk + tid * (kFormCoefs + 1);
-    Form::tangent(d, Cm[1], prm.v, k);
-    k[kFormCoefs] = Cm[0] * op.qp[4 * tid + 3];
+    Form::tangent(d, hc, prm.v, k);
+    k[kFormCoefs] = adet * op.qp[4 * tid + 3];
   }
   __syncthreads();
   for (int it = tid; it < np * 14; it += blockDim.x) {
