@@ -373,9 +373,48 @@ struct hfk_transfer {
   // and the fine source of every coarse free and Dirichlet value for the injection
   bool nested = false;
   DevBuf prp, pcol, pval, rrp, rcol, rval, inj, injd;
+  // a transfer on one mesh to Q1 (at most 8 coarse dofs per fine row) keeps R as CSR and prolongates from the cell
+  // owning each fine dof: each fine value written once and each coarse sum formed by one group of lanes, instead
+  // of per cell with a colour per launch
+  bool csr = false;
+  int p_lanes = 1, r_lanes = 1;  // threads per row of the two CSR products
+  DevBuf owner;                  // same mesh, coarse Q1: the cell slot (slot * 3 + c) * NF + tf of each fine free dof
 };
 
 namespace {
+
+// P (CSR over the fine free dofs, its rows given with sorted columns; kept on the device when with_p) and R = P'
+int upload_transfer_csr(hfk_ctx* ctx, hfk_transfer* T, int64_t nf, int64_t nc, const std::vector<int64_t>& prp,
+                        const std::vector<int32_t>& pcol, const std::vector<double>& pval, bool with_p) {
+  std::vector<int64_t> rrp((size_t)nc + 1, 0);
+  for (int64_t k = 0; k < prp[nf]; ++k) rrp[pcol[k] + 1]++;
+  for (int64_t j = 0; j < nc; ++j) rrp[j + 1] += rrp[j];
+  std::vector<int32_t> rcol((size_t)prp[nf]);
+  std::vector<double> rval((size_t)prp[nf]);
+  std::vector<int64_t> fill(rrp.begin(), rrp.end() - 1);
+  for (int64_t i = 0; i < nf; ++i)
+    for (int64_t k = prp[i]; k < prp[i + 1]; ++k) {
+      const int64_t q = fill[pcol[k]]++;
+      rcol[q] = (int32_t)i;
+      rval[q] = pval[k];
+    }
+  auto up = [&](DevBuf& b, const void* h, size_t bytes) -> hipError_t {
+    hipError_t e = b.alloc(std::max<size_t>(bytes, 8));
+    if (e == hipSuccess && bytes) e = hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  if (with_p) {
+    HIPCHK(ctx, up(T->prp, prp.data(), prp.size() * sizeof(int64_t)));
+    HIPCHK(ctx, up(T->pcol, pcol.data(), pcol.size() * sizeof(int32_t)));
+    HIPCHK(ctx, up(T->pval, pval.data(), pval.size() * sizeof(double)));
+  }
+  HIPCHK(ctx, up(T->rrp, rrp.data(), rrp.size() * sizeof(int64_t)));
+  HIPCHK(ctx, up(T->rcol, rcol.data(), rcol.size() * sizeof(int32_t)));
+  HIPCHK(ctx, up(T->rval, rval.data(), rval.size() * sizeof(double)));
+  T->p_lanes = csr_lanes(nf, prp[nf]);
+  T->r_lanes = csr_lanes(nc, prp[nf]);
+  return HFK_OK;
+}
 
 // A transfer between nested meshes (h-multigrid): the fine operator's cells each lie in one coarse cell (parent) as
 // one of its 8 children.  P[f, c] = coarse shape function of dof c at the node of fine dof f (trilinear for Q1,
@@ -461,35 +500,24 @@ int transfer_build_nested(hfk_op* coarse, hfk_op* fine, hfk_transfer* T) {
   for (int32_t v : injd)
     if (v == 0) return set_err(ctx, HFK_ESTATE, "HyperFEMKernels: a coarse Dirichlet node is not a node of the fine mesh");
   // P (CSR over the fine free dofs, columns sorted) and R = P'
-  std::vector<int64_t> prp((size_t)nf + 1, 0), rrp((size_t)nc + 1, 0);
+  std::vector<int64_t> prp((size_t)nf + 1, 0);
   for (int64_t i = 0; i < nf; ++i) {
     std::sort(rows[i].begin(), rows[i].end());
     prp[i + 1] = prp[i] + (int64_t)rows[i].size();
-    for (auto& pr : rows[i]) rrp[pr.first + 1]++;
   }
-  for (int64_t j = 0; j < nc; ++j) rrp[j + 1] += rrp[j];
-  std::vector<int32_t> pcol((size_t)prp[nf]), rcol((size_t)prp[nf]);
-  std::vector<double> pval((size_t)prp[nf]), rval((size_t)prp[nf]);
-  std::vector<int64_t> fill(rrp.begin(), rrp.end() - 1);
+  std::vector<int32_t> pcol((size_t)prp[nf]);
+  std::vector<double> pval((size_t)prp[nf]);
   for (int64_t i = 0; i < nf; ++i)
     for (size_t k = 0; k < rows[i].size(); ++k) {
       pcol[prp[i] + k] = rows[i][k].first;
       pval[prp[i] + k] = rows[i][k].second;
-      const int64_t q = fill[rows[i][k].first]++;
-      rcol[q] = (int32_t)i;
-      rval[q] = rows[i][k].second;
     }
+  if (int rc = upload_transfer_csr(ctx, T, nf, nc, prp, pcol, pval, true)) return rc;
   auto up = [&](DevBuf& b, const void* h, size_t bytes) -> hipError_t {
     hipError_t e = b.alloc(std::max<size_t>(bytes, 8));
     if (e == hipSuccess && bytes) e = hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice);
     return e;
   };
-  HIPCHK(ctx, up(T->prp, prp.data(), prp.size() * sizeof(int64_t)));
-  HIPCHK(ctx, up(T->pcol, pcol.data(), pcol.size() * sizeof(int32_t)));
-  HIPCHK(ctx, up(T->pval, pval.data(), pval.size() * sizeof(double)));
-  HIPCHK(ctx, up(T->rrp, rrp.data(), rrp.size() * sizeof(int64_t)));
-  HIPCHK(ctx, up(T->rcol, rcol.data(), rcol.size() * sizeof(int32_t)));
-  HIPCHK(ctx, up(T->rval, rval.data(), rval.size() * sizeof(double)));
   HIPCHK(ctx, up(T->inj, inj.data(), inj.size() * sizeof(int32_t)));
   HIPCHK(ctx, up(T->injd, injd.data(), injd.size() * sizeof(int32_t)));
   return HFK_OK;
@@ -570,14 +598,66 @@ int transfer_build(hfk_op* coarse, hfk_op* fine, hfk_transfer* T) {
   HIPCHK(ctx, hipMemcpy(T->cid.p, cid.data(), cid.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(ctx, T->invmult.alloc(inv.size() * sizeof(double)));
   HIPCHK(ctx, hipMemcpy(T->invmult.p, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (NC == 8) {
+    // the row of a fine free dof from the first cell holding it: every cell holding the fine node holds the coarse
+    // nodes whose shape functions are nonzero there (they lie on the smallest face or edge through the node), with
+    // the same E, so the rows agree across cells; each fine dof summed once makes R = P' the sum over cells of
+    // E yf / mult
+    const int64_t nf = fine->nfree, nc = coarse->nfree;
+    std::vector<int64_t> owner((size_t)nf, -1);  // (slot * 3 + c) * NF + tf
+    for (int64_t sl = 0; sl < nslot; ++sl) {
+      const int64_t cell = fine->batch_cell[sl];
+      if (cell < 0) continue;
+      const int32_t* fd = &fine->cell_dofs[(size_t)cell * fine->nd];
+      for (int c = 0; c < 3; ++c)
+        for (int tf = 0; tf < NF; ++tf) {
+          const int32_t id = fd[fine->ldof[c * NF + fine->nodeperm[tf]]];
+          if (id > 0 && owner[id - 1] < 0) owner[id - 1] = ((int64_t)sl * 3 + c) * NF + tf;
+        }
+    }
+    std::vector<int64_t> prp((size_t)nf + 1, 0);
+    std::vector<int32_t> pcol;
+    std::vector<double> pval;
+    pcol.reserve((size_t)nf * 4);
+    pval.reserve((size_t)nf * 4);
+    for (int64_t i = 0; i < nf; ++i) {
+      std::pair<int32_t, double> row[8];
+      int m = 0;
+      if (owner[i] >= 0) {
+        const int64_t sc = owner[i] / NF;
+        const int tf = (int)(owner[i] - sc * NF);
+        for (int tc = 0; tc < NC; ++tc) {
+          const double e = E[(size_t)tf * NC + tc];
+          const int32_t j = cid[(size_t)sc * NC + tc];
+          if (e != 0.0 && j > 0) row[m++] = {j - 1, e};
+        }
+      }
+      std::sort(row, row + m);
+      for (int k = 0; k < m; ++k) {
+        pcol.push_back(row[k].first);
+        pval.push_back(row[k].second);
+      }
+      prp[i + 1] = (int64_t)pcol.size();
+    }
+    if (int rc = upload_transfer_csr(ctx, T, nf, nc, prp, pcol, pval, false)) return rc;
+    HIPCHK(ctx, T->owner.alloc(std::max<size_t>(owner.size() * sizeof(int64_t), 8)));
+    if (nf) HIPCHK(ctx, hipMemcpy(T->owner.p, owner.data(), owner.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    T->csr = true;
+  }
   return HFK_OK;
 }
 
 // prolongate! (src/transfers.jl:238-245): yf = P xc, Dirichlet dofs read as zero and never written
 int do_prolongate(hfk_transfer* T, const double* xc, double* yf, hipStream_t s) {
   hfk_op* f = T->fine;
+  if (T->csr) {  // one thread per fine dof from its owning cell: each value written once, no matrix read
+    HIPCHK(f->ctx, transfer_prolongate_owner(f->nfree, T->NF, T->owner.as<int64_t>(), T->cid.as<int32_t>(),
+                                             T->E.as<double>(), xc, yf, s));
+    return HFK_OK;
+  }
   if (T->nested) {
-    HIPCHK(f->ctx, csr_spmv(f->nfree, T->prp.as<int64_t>(), T->pcol.as<int32_t>(), T->pval.as<double>(), xc, yf, s));
+    HIPCHK(f->ctx, csr_spmv_lanes(f->nfree, T->prp.as<int64_t>(), T->pcol.as<int32_t>(), T->pval.as<double>(), xc, yf,
+                                  T->p_lanes, s));
     return HFK_OK;
   }
   HIPCHK(f->ctx, launch_fill(yf, f->nfree, 0.0, s));
@@ -589,9 +669,9 @@ int do_prolongate(hfk_transfer* T, const double* xc, double* yf, hipStream_t s) 
 // restrict! (src/transfers.jl:252-259): yc = P' yf, colour by colour of the fine cells
 int do_restrict(hfk_transfer* T, const double* yf, double* yc, hipStream_t s) {
   hfk_op* f = T->fine;
-  if (T->nested) {
-    HIPCHK(f->ctx, csr_spmv(T->coarse->nfree, T->rrp.as<int64_t>(), T->rcol.as<int32_t>(), T->rval.as<double>(), yf, yc,
-                            s));
+  if (T->nested || T->csr) {
+    HIPCHK(f->ctx, csr_spmv_lanes(T->coarse->nfree, T->rrp.as<int64_t>(), T->rcol.as<int32_t>(), T->rval.as<double>(),
+                                  yf, yc, T->r_lanes, s));
     return HFK_OK;
   }
   HIPCHK(f->ctx, launch_fill(yc, T->coarse->nfree, 0.0, s));

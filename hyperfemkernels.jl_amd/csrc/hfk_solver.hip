@@ -480,6 +480,30 @@ __global__ void k_prolongate_cell(int64_t nslot, int NF, const int32_t* __restri
   }
 }
 
+// The same with one thread per fine free dof, from the cell slot that owns it (owner = (slot * 3 + c) * NF + tf,
+// -1 for a dof no cell holds): each fine value is written once, the sum in the order above (bitwise the same)
+template <int NC>
+__global__ void k_prolongate_owner(int64_t nf, int NF, const int64_t* __restrict__ owner,
+                                   const int32_t* __restrict__ cid, const double* __restrict__ E,
+                                   const double* __restrict__ xc, double* __restrict__ yf) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nf; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = owner[i];
+    double v = 0;
+    if (o >= 0) {
+      const int64_t sc = o / NF;
+      const int tf = (int)(o - sc * NF);
+      const int32_t* ci = cid + sc * NC;
+#pragma unroll
+      for (int tc = 0; tc < NC; ++tc) {
+        const double e = E[tf * NC + tc];
+        const int j = ci[tc];
+        if (e != 0.0 && j > 0) v += e * xc[j - 1];
+      }
+    }
+    yf[i] = v;
+  }
+}
+
 // restriction over the batches of one colour of the fine operator (cells of a colour share no fine free dof,
 // hence no coarse free dof): yc[j] += sum_tf E[tf, tc] yf[id] / mult[id]
 __global__ void k_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* __restrict__ fid,
@@ -694,6 +718,12 @@ hipError_t transfer_prolongate(int64_t nslot, int NF, int NC, const int32_t* fid
   else hipLaunchKernelGGL(k_prolongate, grid_for(nslot * 3 * NF), dim3(256), 0, s, nslot, NF, NC, fid, cid, E, xc, yf);
   return hipGetLastError();
 }
+hipError_t transfer_prolongate_owner(int64_t nf, int NF, const int64_t* owner, const int32_t* cid, const double* E,
+                                     const double* xc, double* yf, hipStream_t s) {
+  if (nf <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prolongate_owner<8>, grid_for(nf), dim3(256), 0, s, nf, NF, owner, cid, E, xc, yf);
+  return hipGetLastError();
+}
 hipError_t transfer_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid,
                              const double* E, const double* invmult, const double* yf, double* yc, hipStream_t s) {
   if (nslot <= 0) return hipSuccess;
@@ -721,6 +751,24 @@ __global__ void k_csr_spmv(int64_t nrows, const int64_t* __restrict__ rp, const 
     double v = 0.0;
     for (int64_t k = rp[i]; k < rp[i + 1]; ++k) v += val[k] * x[col[k]];
     y[i] = v;
+  }
+}
+
+// The same with L lanes per row (rows with many entries): lane l sums the entries rp[i] + l, rp[i] + l + L, ... in
+// order, then the L partial sums are added by a fixed butterfly (deterministic); the L lanes read L consecutive
+// entries per step
+template <int L>
+__global__ void k_csr_spmv_vec(int64_t nrows, const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                               const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x & (L - 1);
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x / L);
+  for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L; i < nrows; i += stride) {
+    double v = 0.0;
+    const int64_t e = rp[i + 1];
+    for (int64_t k = rp[i] + lane; k < e; k += L) v += val[k] * x[col[k]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+    if (lane == 0) y[i] = v;
   }
 }
 
@@ -756,6 +804,23 @@ hipError_t csr_spmv(int64_t nrows, const int64_t* rp, const int32_t* col, const 
                     hipStream_t s) {
   if (nrows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_csr_spmv, dim3(grid_of(nrows)), dim3(256), 0, s, nrows, rp, col, val, x, y);
+  return hipGetLastError();
+}
+int csr_lanes(int64_t nrows, int64_t nnz) {
+  const double avg = nrows > 0 ? (double)nnz / (double)nrows : 0.0;
+  return avg <= 6 ? 1 : avg <= 12 ? 4 : avg <= 24 ? 8 : 16;
+}
+hipError_t csr_spmv_lanes(int64_t nrows, const int64_t* rp, const int32_t* col, const double* val, const double* x,
+                          double* y, int lanes, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  const dim3 g(grid_of(nrows * lanes)), t(256);
+  switch (lanes) {
+    case 1: hipLaunchKernelGGL(k_csr_spmv, g, t, 0, s, nrows, rp, col, val, x, y); break;
+    case 4: hipLaunchKernelGGL(k_csr_spmv_vec<4>, g, t, 0, s, nrows, rp, col, val, x, y); break;
+    case 8: hipLaunchKernelGGL(k_csr_spmv_vec<8>, g, t, 0, s, nrows, rp, col, val, x, y); break;
+    case 16: hipLaunchKernelGGL(k_csr_spmv_vec<16>, g, t, 0, s, nrows, rp, col, val, x, y); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

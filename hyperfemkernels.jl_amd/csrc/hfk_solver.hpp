@@ -58,12 +58,19 @@ hipError_t transfer_prolongate(int64_t nslot, int NF, int NC, const int32_t* fid
                                const double* xc, double* yf, hipStream_t s);
 hipError_t transfer_restrict(int64_t slot0, int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid,
                              const double* E, const double* invmult, const double* yf, double* yc, hipStream_t s);
+// yf = P xc with one thread per fine free dof from its owning cell slot (coarse Q1: NC = 8)
+hipError_t transfer_prolongate_owner(int64_t nf, int NF, const int64_t* owner, const int32_t* cid, const double* E,
+                                     const double* xc, double* yf, hipStream_t s);
 hipError_t transfer_inject(int64_t nslot, int NF, int NC, const int32_t* fid, const int32_t* cid, const double* Einj,
                            const double* xf, const double* df, double* xc, double* dc, hipStream_t s);
 hipError_t count_nonfinite(int64_t n, const double* x, int* bad, hipStream_t s);
 // y = A x for a CSR matrix with 0-based columns (rp: nrows + 1 offsets)
 hipError_t csr_spmv(int64_t nrows, const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                     hipStream_t s);
+// the same with `lanes` (1, 4, 8 or 16) threads per row; csr_lanes picks them from the mean row length
+int csr_lanes(int64_t nrows, int64_t nnz);
+hipError_t csr_spmv_lanes(int64_t nrows, const int64_t* rp, const int32_t* col, const double* val, const double* x,
+                          double* y, int lanes, hipStream_t s);
 // out[i] = src[i] > 0 ? xf[src[i] - 1] : df[-src[i] - 1] (nodal injection between nested meshes)
 hipError_t inject_map(int64_t n, const int32_t* src, const double* xf, const double* df, double* out, hipStream_t s);
 // y = A x with a dense n x n matrix (row-major)
