@@ -223,21 +223,34 @@ template <class BT>
 __global__ void k_block_step(int64_t nb, const int32_t* __restrict__ dofs, const BT* __restrict__ Binv,
                              const double* __restrict__ r, double* __restrict__ d, double* __restrict__ x, double cd,
                              double cz) {
+  // every load of a block is issued before the arithmetic (ids, then r, d, x and the inverse), so that their
+  // latencies overlap
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
     int id[3];
-    double rv[3];
+    double rv[3], dv[3], xv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) id[j] = dofs[3 * k + j];
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
-      id[j] = dofs[3 * k + j];
-      rv[j] = id[j] > 0 ? r[id[j] - 1] : 0.0;
+      const bool f = id[j] > 0;
+      const int64_t o = f ? id[j] - 1 : 0;
+      rv[j] = f ? r[o] : 0.0;
+      dv[j] = f && cd != 0.0 ? d[o] : 0.0;
+      xv[j] = f && x ? x[o] : 0.0;
     }
+    double b[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) b[e] = static_cast<double>(Binv[(size_t)e * nb + k]);
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
       if (id[i] <= 0) continue;
       double z = 0;
+#pragma unroll
       for (int j = 0; j < 3; ++j)
-        if (id[j] > 0) z += static_cast<double>(Binv[(size_t)(i + 3 * j) * nb + k]) * rv[j];
-      const double dn = (cd == 0.0 ? 0.0 : cd * d[id[i] - 1]) + cz * z;
+        if (id[j] > 0) z += b[i + 3 * j] * rv[j];
+      const double dn = (cd == 0.0 ? 0.0 : cd * dv[i]) + cz * z;
       d[id[i] - 1] = dn;
-      if (x) x[id[i] - 1] += dn;
+      if (x) x[id[i] - 1] = xv[i] + dn;
     }
   }
 }
