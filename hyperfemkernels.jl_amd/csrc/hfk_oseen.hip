@@ -361,7 +361,8 @@ __global__ __launch_bounds__(256) void k_oseen_res(OseenDev op, const int32_t* _
       if (r < 30) acc += co[i] * Nr[q][f][0] + co[3 + 3 * i] * g[0] + co[4 + 3 * i] * g[1] + co[5 + 3 * i] * g[2];
       else acc += co[12] * Nr[q][f][0] + co[13] * g[0] + co[14] * g[1] + co[15] * g[2];
     }
-    if (ATOMIC) unsafeAtomicAdd(b + id - 1, acc);
+    if (ATOMIC && op.eres) op.eres[(c0 + lc) * kND + r] = acc;  // summed per dof afterwards, in a fixed order
+    else if (ATOMIC) unsafeAtomicAdd(b + id - 1, acc);
     else b[id - 1] += acc;
   }
 }
@@ -588,6 +589,19 @@ __global__ __launch_bounds__(256) void k_oseen_face_w(OseenDev op, StepFunctions
   for (int c = 0; c < 3; ++c) xq[4 + c] = w[c];
 }
 
+// b[i] = the element residuals of free dof i summed in ascending (cell, row) order: rows of a single residual launch
+// over every cell (each stored once, no atomics), gathered so that the sum is deterministic without colour launches
+__global__ __launch_bounds__(256) void k_oseen_gather(int64_t nfree, const int64_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ idx,
+                                                      const double* __restrict__ eres, double* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nfree; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    const int64_t e = ptr[i + 1];
+    for (int64_t k = ptr[i]; k < e; ++k) v += eres[idx[k]];
+    b[i] = v;
+  }
+}
+
 }  // namespace
 }  // namespace hfk
 
@@ -608,6 +622,10 @@ struct hfk_oseen {
   bool has_lap = false;
   // colour by colour with plain adds (bitwise reproducible), or one launch with FP64 atomic adds
   bool deterministic = true;
+  // the deterministic residual: element residuals of one launch ([cell][34]) and, per free dof, the positions of
+  // its rows in them (ascending), summed by k_oseen_gather
+  DevBuf eres, gat_ptr, gat_idx;
+  bool gather_built = false;
   // outlet faces, ordered by their colour (faces coloured among themselves by shared free dofs)
   int64_t nfaces = 0;
   int nqo = 0, nqb = 0;
@@ -649,6 +667,7 @@ struct hfk_oseen {
     d.lap = has_lap ? lap.as<double>() : nullptr;
     d.dvals = dvals.as<double>();
     d.nzpos = pat.nzpos.as<int32_t>();
+    d.eres = nullptr;
     return d;
   }
   OseenFaceDev face_dev() const {
@@ -694,8 +713,9 @@ int oseen_pattern(hfk_oseen* op) {
 // the volume kernels of a compiled point form over `n` cells (cells: the list of one colour, or null for every
 // cell in the single-launch mode)
 int form_launch(hfk_oseen* op, bool atomic, const int32_t* cells, int64_t n, const double* x, double* b,
-                double* nzval, hipStream_t s) {
+                double* nzval, hipStream_t s, double* eres = nullptr) {
   OseenDev d = op->dev();
+  d.eres = eres;
   FormParams prm = op->form_args();
   const int a = atomic ? 1 : 0;
   if (nzval) {
@@ -711,6 +731,54 @@ int form_launch(hfk_oseen* op, bool atomic, const int32_t* cells, int64_t n, con
     void* args[] = {&d, &cells, &n, &x, &b, &prm};
     HIPCHK(op->ctx, hipModuleLaunchKernel(op->form->res[a], grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
   }
+  return HFK_OK;
+}
+
+// the per-dof lists of k_oseen_gather, built once from the cell dof table
+int oseen_build_gather(hfk_oseen* op) {
+  const int64_t nf = op->nfree, nc = op->ncells;
+  std::vector<int64_t> ptr((size_t)nf + 1, 0);
+  for (int64_t k = 0; k < nc * kND; ++k)
+    if (op->cell_dofs[k] > 0) ptr[op->cell_dofs[k]]++;
+  for (int64_t i = 0; i < nf; ++i) ptr[i + 1] += ptr[i];
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  std::vector<int32_t> idx((size_t)ptr[nf]);
+  for (int64_t k = 0; k < nc * kND; ++k)
+    if (op->cell_dofs[k] > 0) idx[fill[op->cell_dofs[k] - 1]++] = (int32_t)k;
+  HIPCHK(op->ctx, op->eres.alloc((size_t)std::max<int64_t>(nc * kND, 1) * sizeof(double)));
+  HIPCHK(op->ctx, op->gat_ptr.alloc(ptr.size() * sizeof(int64_t)));
+  HIPCHK(op->ctx, hipMemcpy(op->gat_ptr.p, ptr.data(), ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(op->ctx, op->gat_idx.alloc(std::max<size_t>(idx.size(), 1) * sizeof(int32_t)));
+  if (!idx.empty())
+    HIPCHK(op->ctx, hipMemcpy(op->gat_idx.p, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  op->gather_built = true;
+  return HFK_OK;
+}
+
+// the deterministic residual without colour launches: one launch storing every cell's element residual, then one
+// thread per free dof summing its rows in ascending (cell, row) order (b overwritten); used when the cell table's
+// positions fit 32 bits
+int oseen_residual_gather(hfk_oseen* op, const double* x, double* b, hipStream_t s) {
+  if (!op->gather_built) {
+    const int rc = oseen_build_gather(op);
+    if (rc) return rc;
+  }
+  double* er = op->eres.as<double>();
+  if (op->form) {
+    const int rc = form_launch(op, true, nullptr, op->ncells, x, b, nullptr, s, er);
+    if (rc) return rc;
+  } else {
+    OseenDev d = op->dev();
+    d.eres = er;
+    const int cpw = std::min(256 / op->np, 32);
+    hipLaunchKernelGGL((k_oseen_res<true>), dim3((unsigned)((op->ncells + cpw - 1) / cpw)), dim3(256), 0, s, d, nullptr,
+                       op->ncells, x, b);
+  }
+  const int64_t nf = op->nfree;
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nf + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_oseen_gather, dim3(g), dim3(256), 0, s, nf, op->gat_ptr.as<int64_t>(),
+                     op->gat_idx.as<int32_t>(), er, b);
+  HIPCHK(op->ctx, hipGetLastError());
   return HFK_OK;
 }
 
@@ -746,17 +814,23 @@ int oseen_launch(hfk_oseen* op, const double* x, double* b, double* nzval, hipSt
     }
     return HFK_OK;
   }
+  const bool gather = b && op->ncells * kND < (int64_t)INT32_MAX;  // the residual by element rows and a gather
+  if (gather) {
+    const int rc = oseen_residual_gather(op, x, b, s);
+    if (rc) return rc;
+  }
+  double* bc = gather ? nullptr : b;  // what the colour launches still add into
   for (size_t c = 0; c + 1 < op->color_start.size(); ++c) {
     const int64_t n = op->color_start[c + 1] - op->color_start[c];
-    if (n <= 0) continue;
+    if (n <= 0 || (!nzval && !bc)) continue;
     const int32_t* cells = op->cells_d.as<int32_t>() + op->color_start[c];
     if (op->form) {
-      const int rc = form_launch(op, false, cells, n, x, b, nzval, s);
+      const int rc = form_launch(op, false, cells, n, x, bc, nzval, s);
       if (rc) return rc;
       continue;
     }
     if (nzval) hipLaunchKernelGGL((k_oseen<false>), dim3((unsigned)n), t, 0, s, d, cells, 0, nzval);  // a cell each
-    if (b) hipLaunchKernelGGL((k_oseen_res<false>), dim3((unsigned)((n + cpw - 1) / cpw)), t, 0, s, d, cells, n, x, b);
+    if (bc) hipLaunchKernelGGL((k_oseen_res<false>), dim3((unsigned)((n + cpw - 1) / cpw)), t, 0, s, d, cells, n, x, bc);
     HIPCHK(op->ctx, hipGetLastError());
   }
   if (op->nfaces > 0) {

@@ -24,6 +24,7 @@ struct OseenDev {
   const double* lap;      // [cell][q][3] or null
   const double* dvals;
   const int32_t* nzpos;   // [cell][ii + 34 jj], 1-based, 0: not stored
+  double* eres;           // [cell][34]: element residuals stored by a single launch over every cell, or null
 };
 
 // barycentric coordinate k (0..3) of a reference point and its (constant) reference gradient
@@ -411,7 +412,8 @@ __global__ __launch_bounds__(256) void k_oseen_form_res(OseenDev op, const int32
       if (r < 30) acc += co[i] * Nr[q][f][0] + co[3 + 3 * i] * g[0] + co[4 + 3 * i] * g[1] + co[5 + 3 * i] * g[2];
       else acc += co[12] * Nr[q][f][0] + co[13] * g[0] + co[14] * g[1] + co[15] * g[2];
     }
-    if (ATOMIC) unsafeAtomicAdd(b + id - 1, acc);
+    if (ATOMIC && op.eres) op.eres[(c0 + lc) * kND + r] = acc;  // summed per dof afterwards, in a fixed order
+    else if (ATOMIC) unsafeAtomicAdd(b + id - 1, acc);
     else b[id - 1] += acc;
   }
 }
