@@ -455,9 +455,11 @@ int hfk_oseen_set_outlet_pressure(hfk_oseen* op, double Pc);
 int hfk_oseen_set_form(hfk_oseen* op, const char* source, const char* name, const double* params, int nparams);
 /* Compile a point form without a device (hiprtc only): HFK_OK, or HFK_EARG with the compiler log in `log`. */
 int hfk_oseen_form_check(const char* source, const char* name, char* log, int64_t log_size);
-/* Assembly colour by colour with plain adds (1, the default: the reference's colour order, bitwise reproducible,
- * src/assembly.jl:1413-1432) or in one launch over the cells in mesh order with FP64 atomic adds (0: the CSC values
- * that neighbouring cells share stay in L2; the order of the additions into one entry varies between runs). */
+/* Deterministic assembly (1, the default: the Jacobian colour by colour with plain adds in the reference's colour
+ * order, src/assembly.jl:1413-1432; the residual as element rows of one launch summed per dof in ascending (cell,
+ * row) order; both bitwise reproducible) or one launch over the cells in mesh order with FP64 atomic adds (0: the
+ * CSC values that neighbouring cells share stay in L2; the order of the additions into one entry varies between
+ * runs). */
 int hfk_oseen_set_deterministic(hfk_oseen* op, int deterministic);
 /* info[6]: nfree, ndir, ncells, np, ncolors, nnz (-1 before the pattern) */
 int hfk_oseen_info(hfk_oseen* op, int64_t* info);
