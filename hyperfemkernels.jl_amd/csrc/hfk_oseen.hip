@@ -597,7 +597,16 @@ __global__ __launch_bounds__(256) void k_oseen_gather(int64_t nfree, const int64
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nfree; i += (int64_t)gridDim.x * blockDim.x) {
     double v = 0.0;
     const int64_t e = ptr[i + 1];
-    for (int64_t k = ptr[i]; k < e; ++k) v += eres[idx[k]];
+    int64_t k = ptr[i];
+    for (; k + 4 <= e; k += 4) {  // four rows in flight at a time, added in order
+      const int32_t j0 = idx[k], j1 = idx[k + 1], j2 = idx[k + 2], j3 = idx[k + 3];
+      const double e0 = eres[j0], e1 = eres[j1], e2 = eres[j2], e3 = eres[j3];
+      v += e0;
+      v += e1;
+      v += e2;
+      v += e3;
+    }
+    for (; k < e; ++k) v += eres[idx[k]];
     b[i] = v;
   }
 }
